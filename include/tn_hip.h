@@ -1,0 +1,170 @@
+/*
+ * tn_hip.h - C ABI of the MI355X (gfx950) tensor-network kernel library.
+ *
+ * This is the backend seam of Uni10: the free functions that the tensor
+ * layer calls with raw element pointers (reference:
+ * uni10/src/uni10/tools/uni10_tools.h:56-135 and
+ * uni10/src/uni10/numeric/lapack/uni10_lapack.h:54-83), restated as a plain C
+ * ABI over device memory.  Every entry point:
+ *   - takes plain pointers and sizes (no C++ or torch types),
+ *   - returns 0 on success or a negative TN_E* code; the message of the last
+ *     failure on the calling thread is available from tn_last_error(),
+ *   - never throws and never calls exit()/abort() (the reference's CUDA
+ *     helpers exit() on error, helper_uni10.h:86-105),
+ *   - enqueues on the given stream (NULL = the library's default stream) and
+ *     returns without synchronising unless it says otherwise.
+ * All element buffers are FP64, row-major, resident in device memory.
+ */
+#ifndef TN_HIP_H
+#define TN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define TN_OK 0
+#define TN_EINVAL -1     /* bad argument (shape, permutation, mode, NULL)   */
+#define TN_ENOMEM -2     /* device allocation failed or budget exceeded      */
+#define TN_EHIP -3       /* a HIP runtime call failed                        */
+#define TN_ENODEV -4     /* no usable gfx950 device                          */
+#define TN_ENOCONV -5    /* iterative solver did not converge                */
+#define TN_EUNSUPP -6    /* shape outside what the kernel supports           */
+
+/* Message of the last failure on this thread ("" if none). */
+const char* tn_last_error(void);
+
+/* ---- device, streams, memory --------------------------------------------
+ * Replaces guni10_create/guni10_destroy and the elem* family
+ * (uni10_tools_gpu.cu:7-163: cudaMallocManaged, cudaMemcpy, cudaMemset).
+ * Device memory comes from a caching pool with an optional byte budget. */
+int tn_init(int device);                 /* select the device; idempotent     */
+int tn_device_info(int* cu_count, size_t* hbm_bytes, char* arch, int arch_len);
+int tn_set_budget(size_t bytes);         /* 0 = unlimited                      */
+int tn_mem_stats(size_t* in_use, size_t* cached, size_t* peak, size_t* budget);
+int tn_malloc(void** dptr, size_t bytes);            /* elemAlloc             */
+int tn_free(void* dptr);                             /* elemFree              */
+int tn_pool_trim(void);                              /* release cached blocks */
+int tn_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int tn_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int tn_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+int tn_zero(void* dst, size_t bytes, void* stream);  /* elemBzero (fill kernel) */
+int tn_stream_create(void** stream);
+int tn_stream_destroy(void* stream);
+int tn_stream_sync(void* stream);
+int tn_device_sync(void);
+/* Events for timing on a given stream (milliseconds between two records). */
+int tn_event_create(void** ev);
+int tn_event_destroy(void* ev);
+int tn_event_record(void* ev, void* stream);
+int tn_event_elapsed_ms(void* ev_start, void* ev_stop, float* ms);
+
+/* ---- permute / reshapeElem ----------------------------------------------
+ * Dense (non-symmetric) permute.  Output axis j takes source axis perm[j]
+ * (rsp_outin of UniTensorReal.cpp:545-558); dst has dims[perm[j]].
+ * Bit-exact copy.  Replaces reshapeElem/_reshapeElem
+ * (uni10_tools_gpu.cu:319-339) and the host odometer of
+ * UniTensorReal.cpp:601-646. */
+int tn_permute_dense_f64(const double* src, double* dst, int rank,
+                         const int64_t* dims, const int* perm, void* stream);
+
+/* reshapeElem with the reference's own argument convention
+ * (uni10_tools_gpu.cu:332-339, table built at UniTensorReal.cpp:586-595):
+ * offset[0..r) are the source row-major strides, offset[r+i] is the
+ * destination stride of source axis i. */
+int tn_reshape_elem_f64(const double* old_elem, int bond_num, size_t elem_num,
+                        const size_t* offset, double* new_elem, void* stream);
+
+/* Symmetric (block-sparse) permute: one move per source sub-block (Qidx),
+ * the per-Qidx copy of UniTensorReal.cpp:683-756, in one launch.
+ * Host arrays, nmoves entries:
+ *   meta[6*m+0..5] = {src corner offset, src block cols (Bin_cDim),
+ *                     src sub-block cols (sBin_cDim), dst corner offset,
+ *                     dst block cols (Bot_cDim), dst sub-block cols (sBot_cDim)}
+ *   sub_dims[rank*m+b] = source sub-block extent of bond b (sBin_sBdims)
+ *   bot_acc[rank*m+b]  = destination linear stride of source bond b (sBot_acc)
+ *   sign[m]            = +1/-1 (fermionic swap sign; NULL = all +1)
+ * A plan uploads these once; executing it is one kernel launch. */
+typedef struct tn_blockperm_plan_s* tn_blockperm_plan;
+int tn_blockperm_plan_create(tn_blockperm_plan* plan, int rank, int64_t nmoves,
+                             const int64_t* meta, const int64_t* sub_dims,
+                             const int64_t* bot_acc, const double* sign,
+                             int64_t dst_elems);
+int tn_blockperm_exec(tn_blockperm_plan plan, const double* src, double* dst,
+                      void* stream);
+int tn_blockperm_plan_destroy(tn_blockperm_plan plan);
+
+/* Row-major transpose AT(N x M) = A(M x N)^T; replaces setTranspose
+ * (uni10_lapack_cpu.cpp:350-355; GPU uni10_lapack_gpu.cu:568-598). */
+int tn_transpose_f64(const double* A, int64_t M, int64_t N, double* AT, void* stream);
+
+/* ---- grouped sector GEMM -------------------------------------------------
+ * The per-Qnum loop of contract (UniTensorTools.cpp:191-203), each step a
+ * matrixMul (uni10_lapack_cpu.cpp:41-44; GPU uni10_lapack_gpu.cu:42-152):
+ * row-major C_g(M x N) = A_g(M x K) B_g(K x N) for every group member g, in
+ * one launch.  Offsets are in elements from the A/B/C base pointers.
+ * Modes: arithmetic of the products; storage stays FP64. */
+#define TN_GEMM_F64 0      /* FP64 MFMA, FP64 accumulate (plain drop-in)        */
+#define TN_GEMM_F32 1      /* FP32 MFMA on FP64->FP32 rounded inputs, FP32 acc. */
+#define TN_GEMM_BF16X3 2   /* split-BF16 (hi+lo), 3 products, FP32 accumulate    */
+#define TN_GEMM_BF16X6 3   /* split-BF16 (hi+mid+lo), 6 products, FP32 accumulate*/
+typedef struct {
+  int64_t M, N, K;
+  int64_t a_off, b_off, c_off;
+  int64_t lda, ldb, ldc;
+} tn_gemm_desc;
+typedef struct tn_gemm_plan_s* tn_gemm_plan;
+int tn_gemm_plan_create(tn_gemm_plan* plan, int mode, int64_t ngroups,
+                        const tn_gemm_desc* descs);
+int tn_gemm_plan_set_tile(tn_gemm_plan plan, int tile_id); /* autotuner hook  */
+int tn_gemm_plan_info(tn_gemm_plan plan, int64_t* tiles, double* flops, int* tile_id);
+int tn_gemm_exec(tn_gemm_plan plan, const double* A, const double* B, double* C,
+                 void* stream);
+int tn_gemm_plan_destroy(tn_gemm_plan plan);
+/* One matrixMul (a group of one), same argument order as the reference. */
+int tn_matmul_f64(const double* A, const double* B, int64_t M, int64_t N, int64_t K,
+                  double* C, int mode, void* stream);
+
+/* ---- vectors / diagonal scaling -----------------------------------------
+ * vectorNorm (uni10_lapack_cpu.cpp:397-414), vectorScal (:75-89),
+ * diagRowMul / diagColMul (:46-57).  tn_norm2_f64 writes ||x||_2 to a device
+ * double with a fixed reduction order (deterministic, run to run). */
+int tn_norm2_f64(const double* x, int64_t n, double* out_dev, void* stream);
+int tn_scale_f64(double* x, int64_t n, double a, void* stream);
+int tn_scale_inv_dev_f64(double* x, int64_t n, const double* denom_dev, void* stream);
+int tn_diag_row_mul_f64(double* mat, const double* diag, int64_t M, int64_t N, void* stream);
+int tn_diag_col_mul_f64(double* mat, const double* diag, int64_t M, int64_t N, void* stream);
+
+/* ---- dense FP64 decompositions -------------------------------------------
+ * matrixSVD (uni10_lapack_cpu.cpp:289-313): thin SVD of the row-major
+ * M x N matrix A: U (M x k), S (k, descending), VT (k x N), k = min(M, N).
+ * One-sided Jacobi; singular vectors are unique up to a sign per pair. */
+int tn_svd_f64(const double* A, int64_t M, int64_t N, double* U, double* S,
+               double* VT, void* stream);
+/* Batched: problem i reads A + a_off[i] (m[i] x n[i]) and writes
+ * U + u_off[i], S + s_off[i], VT + vt_off[i]. */
+int tn_svd_f64_batched(int64_t batch, const int64_t* m, const int64_t* n,
+                       const double* A, const int64_t* a_off,
+                       double* U, const int64_t* u_off,
+                       double* S, const int64_t* s_off,
+                       double* VT, const int64_t* vt_off, void* stream);
+/* eigSyDecompose (uni10_lapack_cpu.cpp:131-152): symmetric N x N;
+ * eigenvalues ascending; eigenvectors returned as ROWS of EigVec. */
+int tn_eigh_f64(const double* A, int64_t N, double* eig, double* eigvec_rows,
+                void* stream);
+
+/* ---- TEBD truncation ------------------------------------------------------
+ * The truncate-and-renormalise of iTEBD.cpp:157-161 plus the discarded
+ * weight: given S (k, descending) on the device, keep dim = min(chi, k)
+ * values, write lambda = S[:dim] / ||S[:dim]||, norm_out = ||S[:dim]|| and
+ * trunc_err_out = sum_{i>=dim} S_i^2 (all device doubles). */
+int tn_truncate_f64(const double* S, int64_t k, int64_t chi, double* lambda_out,
+                    double* norm_out, double* trunc_err_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TN_HIP_H */

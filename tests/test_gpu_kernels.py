@@ -1,0 +1,358 @@
+"""GPU parity of the C-ABI kernels against the oracle (run with -m gpu).
+
+Bars: permutes are bit-exact against the oracle's restatement of the
+reference loops; the FP64 GEMM is within 1e-12 relative Frobenius error of
+numpy's dgemm; FP32-MFMA and split-BF16 modes carry their own stated
+tolerances; the SVD/eigh match SciPy's dgesvd/dsyev (the routines the
+reference calls) to 1e-10 and up to the sign of each singular pair.
+"""
+import ctypes
+import itertools
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _rand(seed, shape):
+    return np.random.default_rng(seed).uniform(-1, 1, size=shape)
+
+
+# ---------------------------------------------------------------------------
+# dense permute
+DENSE_CASES = [
+    ((256, 2, 2, 256), (3, 1, 0, 2)),   # contraction-and-permute workload, output permute
+    ((256, 2, 2, 256), (1, 0, 2, 3)),   # its internal B permute (row copy)
+    ((7,), (0,)),
+    ((5, 3), (1, 0)),
+    ((64, 64), (1, 0)),
+    ((65, 130), (1, 0)),
+    ((1, 1, 1), (2, 0, 1)),
+    ((3, 1, 4, 1, 5), (4, 2, 0, 3, 1)),
+    ((2, 2, 2, 2, 2, 2), (5, 4, 3, 2, 1, 0)),
+    ((33, 17, 9), (2, 0, 1)),
+    ((33, 17, 9), (1, 2, 0)),
+    ((100, 3, 70), (2, 1, 0)),
+    ((4, 300, 5), (0, 2, 1)),
+    ((8, 8, 8, 8), (0, 1, 3, 2)),
+    ((2, 1000, 3), (1, 0, 2)),
+    ((129, 2, 3, 2, 67), (4, 1, 3, 0, 2)),
+]
+
+
+@pytest.mark.parametrize("dims,perm", DENSE_CASES)
+def test_permute_dense_bitexact(tn, ora, dims, perm):
+    _say("permute_dense dims=%s perm=%s" % (dims, perm))
+    n = int(np.prod(dims))
+    x = _rand(1, n)
+    ref = np.empty(n)
+    d64 = np.asarray(dims, np.int64)
+    p32 = np.asarray(perm, np.int32)
+    ora.lib().ora_permute_dense(ora._ptr(x), ora._ptr(ref), len(dims), ora._ptr(d64), ora._ptr(p32))
+    assert np.array_equal(ref, np.transpose(x.reshape(dims), perm).reshape(-1))
+    src = tn.DeviceArray.from_host(x)
+    dst = tn.DeviceArray((n,))
+    tn.permute_dense(src, dst, dims, perm)
+    out = dst.to_host()
+    assert out.tobytes() == ref.tobytes()
+
+
+def test_permute_dense_random_sweep(tn, ora):
+    rng = np.random.default_rng(7)
+    for case in range(60):
+        r = int(rng.integers(1, 7))
+        dims = [int(v) for v in rng.integers(1, 9, size=r)]
+        if case % 5 == 0:
+            dims[int(rng.integers(0, r))] = int(rng.integers(60, 200))
+        perm = [int(v) for v in rng.permutation(r)]
+        n = int(np.prod(dims))
+        _say("permute sweep %d dims=%s perm=%s" % (case, dims, perm))
+        x = rng.standard_normal(n)
+        src = tn.DeviceArray.from_host(x)
+        dst = tn.DeviceArray((n,))
+        tn.permute_dense(src, dst, dims, perm)
+        ref = np.transpose(x.reshape(dims), perm).reshape(-1)
+        assert dst.to_host().tobytes() == ref.tobytes()
+
+
+def test_permute_dense_hbm_size_roundtrip(tn):
+    """Full-size (4096,2,2,4096) FP64 permute (512 MiB each way): checked by
+    applying the inverse permutation on the device and by a host transpose."""
+    dims = (4096, 2, 2, 4096)
+    perm = (3, 1, 0, 2)
+    inv = tuple(int(v) for v in np.argsort(perm))
+    n = int(np.prod(dims))
+    _say("hbm-size permute n=%d" % n)
+    x = np.random.default_rng(3).uniform(-1, 1, n)
+    src = tn.DeviceArray.from_host(x)
+    mid = tn.DeviceArray((n,))
+    back = tn.DeviceArray((n,))
+    tn.permute_dense(src, mid, dims, perm)
+    out_dims = tuple(dims[p] for p in perm)
+    tn.permute_dense(mid, back, out_dims, inv)
+    assert back.to_host().tobytes() == x.tobytes()
+    ref = np.ascontiguousarray(np.transpose(x.reshape(dims), perm)).reshape(-1)
+    assert mid.to_host().tobytes() == ref.tobytes()
+
+
+def test_reshape_elem_reference_table(tn, ora):
+    """tn_reshape_elem_f64 with the offset table the reference builds
+    (UniTensorReal.cpp:586-595), against the oracle's _reshapeElem formula."""
+    for dims, perm in DENSE_CASES[:8]:
+        r = len(dims)
+        n = int(np.prod(dims))
+        new_dims = [dims[p] for p in perm]
+        new_acc = [1] * r
+        src_acc = [1] * r
+        for b in range(r - 1, 0, -1):
+            new_acc[b - 1] = new_acc[b] * new_dims[b]
+            src_acc[b - 1] = src_acc[b] * dims[b]
+        off = src_acc + [0] * r
+        for b in range(r):
+            off[r + perm[b]] = new_acc[b]
+        x = _rand(11, n)
+        ref = np.empty(n)
+        ora.lib().ora_reshape_elem(ora._ptr(x), ora._ptr(ref), r, ora._ptr(np.asarray(dims, np.int64)),
+                                   ora._ptr(np.asarray(perm, np.int32)))
+        src = tn.DeviceArray.from_host(x)
+        dst = tn.DeviceArray((n,))
+        tn.reshape_elem(src, dst, off)
+        assert dst.to_host().tobytes() == ref.tobytes()
+
+
+def test_transpose(tn):
+    for M, N in [(1, 1), (3, 7), (64, 65), (300, 17)]:
+        a = _rand(M * 31 + N, (M, N))
+        A = tn.DeviceArray.from_host(a)
+        AT = tn.DeviceArray((N, M))
+        tn.call("tn_transpose_f64", tn._p(A), M, N, tn._p(AT), None)
+        assert AT.to_host().tobytes() == np.ascontiguousarray(a.T).tobytes()
+
+
+# ---------------------------------------------------------------------------
+# symmetric block permute
+def _u1_bond(ora, tp, charges_degs):
+    qs = []
+    for c, d in charges_degs:
+        qs += [ora.Qnum(c)] * d
+    return ora.Bond(tp, qs)
+
+
+def _u1_tensor(ora, seed):
+    rng = np.random.default_rng(seed)
+    phys = _u1_bond(ora, ora.BD_IN, [(1, 1), (-1, 1)])
+    chi_l = _u1_bond(ora, ora.BD_IN, [(c, int(rng.integers(1, 5))) for c in (-3, -1, 1, 3)])
+    chi_r = _u1_bond(ora, ora.BD_OUT, [(c, int(rng.integers(1, 5))) for c in (-3, -1, 1, 3)])
+    phys_o = _u1_bond(ora, ora.BD_OUT, [(1, 1), (-1, 1)])
+    T = ora.UniTensor([chi_l, phys, chi_r, phys_o])
+    T.set_raw_elem(rng.uniform(-1, 1, size=[b.dim for b in T.bonds]))
+    return T
+
+
+@pytest.mark.parametrize("perm,rb", [((2, 0, 3, 1), 1), ((3, 2, 1, 0), 2), ((0, 1, 2, 3), 3),
+                                     ((1, 3, 0, 2), 2), ((0, 2, 1, 3), 0), ((2, 3, 0, 1), 4)])
+def test_block_permute_bitexact(tn, ora, perm, rb):
+    T = _u1_tensor(ora, 5)
+    new_labels = [T.labels[p] for p in perm]
+    _say("block permute perm=%s rb=%d blocks=%d" % (perm, rb, len(T.layout.blocks)))
+    out = T.copy().permute(new_labels, rb)   # oracle result
+    src_T = T.copy()
+    # moves as the oracle computes them (the product derives the same from its own layout)
+    rsp = list(perm)
+    tgt = ora.UniTensor([src_T.bonds[p].copy().change(ora.BD_IN if i < rb else ora.BD_OUT)
+                         for i, p in enumerate(perm)])
+    moves = ora.permute_moves(src_T, tgt, rsp, new_labels)
+    plan = tn.BlockPermPlan(4, moves[0], moves[1], moves[2], moves[3], tgt.elem_num())
+    src = tn.DeviceArray.from_host(src_T.elem)
+    dst = tn.DeviceArray((tgt.elem_num(),))
+    plan(src, dst)
+    got = dst.to_host()
+    assert got.tobytes() == out.elem.tobytes()
+    # densify invariant
+    dense = np.transpose(T.get_raw_elem().reshape([b.dim for b in T.bonds]), perm)
+    tgt.set_elem(got)
+    assert np.array_equal(tgt.get_raw_elem().reshape(dense.shape), dense)
+
+
+# ---------------------------------------------------------------------------
+# grouped GEMM
+GEMM_TOL = {0: 1e-12, 1: 2e-6, 2: 5e-5, 3: 2e-6}
+
+
+def _group_case(seed, shapes):
+    rng = np.random.default_rng(seed)
+    descs = []
+    a_parts, b_parts = [], []
+    ao = bo = co = 0
+    for (M, N, K) in shapes:
+        a_parts.append(rng.uniform(-1, 1, M * K))
+        b_parts.append(rng.uniform(-1, 1, K * N))
+        descs.append((M, N, K, ao, bo, co, max(K, 1), max(N, 1), max(N, 1)))
+        ao += M * K
+        bo += K * N
+        co += M * N
+    A = np.concatenate(a_parts) if a_parts else np.zeros(0)
+    B = np.concatenate(b_parts) if b_parts else np.zeros(0)
+    return descs, A, B, co
+
+
+def _check_gemm(tn, mode, shapes, seed=0, tile=None):
+    descs, A, B, nc = _group_case(seed, shapes)
+    plan = tn.GemmPlan(mode, np.array(descs, dtype=tn.GEMM_DESC))
+    if tile is not None:
+        plan.set_tile(tile)
+    dA = tn.DeviceArray.from_host(A if A.size else np.zeros(1))
+    dB = tn.DeviceArray.from_host(B if B.size else np.zeros(1))
+    dC = tn.DeviceArray((max(nc, 1),))
+    plan(dA, dB, dC)
+    C = dC.to_host()
+    for (M, N, K, ao, bo, co, *_r) in descs:
+        ref = A[ao:ao + M * K].reshape(M, K) @ B[bo:bo + K * N].reshape(K, N)
+        got = C[co:co + M * N].reshape(M, N)
+        den = max(np.linalg.norm(ref), 1e-300)
+        err = np.linalg.norm(got - ref) / den if M * N else 0.0
+        assert err <= GEMM_TOL[mode], (mode, M, N, K, err)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_gemm_single_512(tn, mode):
+    _say("gemm 512^3 mode=%d" % mode)
+    _check_gemm(tn, mode, [(512, 512, 512)])
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
+def test_gemm_grouped_uneven(tn, mode, tile):
+    shapes = [(1, 1, 1), (17, 33, 5), (64, 64, 64), (130, 70, 200), (3, 200, 1), (200, 3, 300),
+              (0, 5, 5), (5, 0, 5), (5, 5, 0), (96, 128, 17)]
+    _say("gemm grouped mode=%d tile=%d" % (mode, tile))
+    _check_gemm(tn, mode, shapes, seed=mode * 10 + tile, tile=tile)
+
+
+# ---------------------------------------------------------------------------
+# SVD / eigh
+def _svd_check(tn, ora, a, tol=1e-10):
+    M, N = a.shape
+    A = tn.DeviceArray.from_host(a)
+    U, S, VT = tn.svd(A, M, N)
+    u, s, vt = U.to_host(), S.to_host(), VT.to_host()
+    ur, sr, vtr = ora.svd(a)
+    smax = max(sr.max(), 1e-300)
+    assert np.max(np.abs(s - sr)) <= tol * smax
+    assert np.all(np.diff(s) <= 0)
+    assert np.linalg.norm(u @ np.diag(s) @ vt - a) <= 1e-12 * max(np.linalg.norm(a), 1e-300) * 10
+    k = min(M, N)
+    gaps = np.abs(np.diff(sr)) if k > 1 else np.array([1.0])
+    for j in range(k):
+        sep = min(gaps[j - 1] if j > 0 else np.inf, gaps[j] if j < k - 1 else np.inf)
+        if sr[j] < 1e-8 * smax or sep < 1e-6 * smax:
+            continue
+        sg = np.sign(np.dot(u[:, j], ur[:, j])) or 1.0
+        assert np.max(np.abs(sg * u[:, j] - ur[:, j])) <= 1e-8
+        assert np.max(np.abs(sg * vt[j] - vtr[j])) <= 1e-8
+
+
+@pytest.mark.parametrize("M,N", [(64, 64), (4, 5), (5, 4), (64, 32), (32, 64), (100, 80), (1, 7),
+                                 (7, 1), (1, 1), (128, 128), (200, 90), (300, 300)])
+def test_svd_random(tn, ora, M, N):
+    _say("svd %dx%d" % (M, N))
+    _svd_check(tn, ora, _rand(M * 1000 + N, (M, N)))
+
+
+def test_svd_golden_matrix(tn, ora, golden):
+    """Rebuild the 4x5 matrix from the reference's saved U, Sigma, VT dump and
+    check the device SVD returns the dumped Sigma (descending) and factors."""
+    U = np.fromfile(golden + "/mat_U", dtype=np.float64).reshape(4, 4)
+    S = np.fromfile(golden + "/mat_Sigma", dtype=np.float64)
+    VT = np.fromfile(golden + "/mat_VT", dtype=np.float64).reshape(4, 5)
+    a = U @ np.diag(S) @ VT
+    A = tn.DeviceArray.from_host(a)
+    u, s, vt = (x.to_host() for x in tn.svd(A, 4, 5))
+    assert np.max(np.abs(s - S)) <= 1e-13
+    for j in range(4):
+        sg = np.sign(np.dot(u[:, j], U[:, j]))
+        assert np.max(np.abs(sg * u[:, j] - U[:, j])) <= 1e-12
+        assert np.max(np.abs(sg * vt[j] - VT[j])) <= 1e-12
+
+
+def test_svd_batched_uneven(tn, ora):
+    shapes = [(64, 64), (10, 3), (3, 10), (50, 50), (1, 1), (120, 60)]
+    mats = [_rand(i + 40, s) for i, s in enumerate(shapes)]
+    A = np.concatenate([m.reshape(-1) for m in mats])
+    offs = np.cumsum([0] + [m.size for m in mats])[:-1].astype(np.int64)
+    ks = [min(s) for s in shapes]
+    u_off = np.cumsum([0] + [s[0] * k for s, k in zip(shapes, ks)])[:-1].astype(np.int64)
+    s_off = np.cumsum([0] + ks)[:-1].astype(np.int64)
+    vt_off = np.cumsum([0] + [k * s[1] for s, k in zip(shapes, ks)])[:-1].astype(np.int64)
+    dA = tn.DeviceArray.from_host(A)
+    dU = tn.DeviceArray((int(sum(s[0] * k for s, k in zip(shapes, ks))),))
+    dS = tn.DeviceArray((int(sum(ks)),))
+    dV = tn.DeviceArray((int(sum(k * s[1] for s, k in zip(shapes, ks))),))
+    m = np.array([s[0] for s in shapes], np.int64)
+    n = np.array([s[1] for s in shapes], np.int64)
+    tn.call("tn_svd_f64_batched", len(shapes), tn._p(m), tn._p(n), tn._p(dA), tn._p(offs),
+            tn._p(dU), tn._p(u_off), tn._p(dS), tn._p(s_off), tn._p(dV), tn._p(vt_off), None)
+    Sall = dS.to_host()
+    for i, mat in enumerate(mats):
+        sr = ora.svd(mat)[1]
+        assert np.max(np.abs(Sall[s_off[i]:s_off[i] + ks[i]] - sr)) <= 1e-12 * sr.max()
+
+
+def test_eigh_heisenberg_gate(tn, ora):
+    H = ora.heisenberg_two_site(1.0)
+    dH = tn.DeviceArray.from_host(H)
+    w, v = (x.to_host() for x in tn.eigh(dH, 4))
+    assert np.allclose(w, [-0.75, 0.25, 0.25, 0.25], atol=1e-15)
+    # rows are eigenvectors
+    assert np.allclose(v @ H @ v.T, np.diag(w), atol=1e-14)
+    # exph from these factors equals the closed form
+    tau = 0.01
+    G = v.T @ np.diag(np.exp(-tau * w)) @ v
+    Gref = ora.exph(-tau, H)
+    assert np.max(np.abs(G - Gref)) <= 1e-15
+
+
+@pytest.mark.parametrize("n", [2, 9, 20, 64])
+def test_eigh_random(tn, ora, n):
+    a = _rand(n, (n, n))
+    a = a + a.T
+    dA = tn.DeviceArray.from_host(a)
+    w, v = (x.to_host() for x in tn.eigh(dA, n))
+    wr, vr = ora.eigh(a)
+    assert np.max(np.abs(w - wr)) <= 1e-12 * np.abs(wr).max()
+    assert np.max(np.abs(v @ a @ v.T - np.diag(w))) <= 1e-12 * np.abs(wr).max()
+
+
+# ---------------------------------------------------------------------------
+def test_vector_ops(tn):
+    x = _rand(9, 12345)
+    dx = tn.DeviceArray.from_host(x)
+    out = tn.DeviceArray((1,))
+    tn.call("tn_norm2_f64", tn._p(dx), x.size, tn._p(out), None)
+    assert abs(out.to_host()[0] - np.linalg.norm(x)) <= 1e-13 * np.linalg.norm(x)
+    tn.call("tn_scale_f64", tn._p(dx), x.size, ctypes.c_double(0.5), None)
+    assert np.array_equal(dx.to_host(), x * 0.5)
+    M, N = 37, 53
+    m = _rand(1, (M, N))
+    d = _rand(2, M)
+    dm = tn.DeviceArray.from_host(m)
+    dd = tn.DeviceArray.from_host(d)
+    tn.call("tn_diag_row_mul_f64", tn._p(dm), tn._p(dd), M, N, None)
+    assert np.array_equal(dm.to_host(), m * d[:, None])
+    s = np.sort(np.abs(_rand(3, 64)))[::-1].copy()
+    ds = tn.DeviceArray.from_host(s)
+    lam = tn.DeviceArray((32,))
+    nrm = tn.DeviceArray((1,))
+    err = tn.DeviceArray((1,))
+    tn.call("tn_truncate_f64", tn._p(ds), 64, 32, tn._p(lam), tn._p(nrm), tn._p(err), None)
+    ref_n = np.sqrt(np.sum(s[:32] ** 2))
+    assert abs(nrm.to_host()[0] - ref_n) <= 1e-14 * ref_n
+    assert np.allclose(lam.to_host(), s[:32] / ref_n, rtol=1e-14, atol=0)
+    assert abs(err.to_host()[0] - np.sum(s[32:] ** 2)) <= 1e-14

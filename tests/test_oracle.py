@@ -1,0 +1,405 @@
+"""Pins the CPU oracle (oracle/uni10_oracle.py, oracle/uni10_cpu.c) to the
+reference's golden data, its unit-test known answers and closed forms.
+
+Nothing here re-types an oracle formula as its own check: every expected
+value comes from a reference fixture/test, a textbook routine
+(numpy.transpose, dense tensordot, LAPACK), or mathematics (closed forms,
+invariants, brute force).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+
+# ---------------------------------------------------------------------------
+# Qnum known answers (uni10/test/testQnum.cpp)
+def test_qnum_known_answers(ora):
+    q = ora.Qnum(100, ora.PRT_EVEN, ora.PRTF_ODD)
+    assert q.hash() == 401                                   # testQnum.cpp:92-95
+    q1 = -ora.Qnum(-1)
+    assert (q1.U1, q1.prt) == (1, ora.PRT_EVEN)              # :97-108
+    q2 = -ora.Qnum(3, ora.PRT_ODD, ora.PRTF_EVEN)
+    assert (q2.U1, q2.prt, q2.prtF) == (-3, ora.PRT_ODD, ora.PRTF_EVEN)
+    assert ora.Qnum(-1) < ora.Qnum(3)                        # :110-122
+    assert ora.Qnum(0, 0, 0) < ora.Qnum(0, 0, 1)
+    assert ora.Qnum(0, 0, 0) < ora.Qnum(0, 1, 0)
+    a = ora.Qnum(3, ora.PRT_ODD, ora.PRTF_EVEN) * ora.Qnum(2, ora.PRT_ODD, ora.PRTF_ODD)
+    assert (a.U1, a.prt, a.prtF) == (5, ora.PRT_EVEN, ora.PRTF_ODD)   # :148-160
+    b = ora.Qnum(-1, ora.PRT_ODD) * ora.Qnum(2, ora.PRT_ODD)
+    assert (b.U1, b.prt, b.prtF) == (1, ora.PRT_EVEN, ora.PRTF_EVEN)
+    assert not (ora.Qnum(0, 0, 0) == ora.Qnum(0, 1, 0))
+    with pytest.raises(ValueError):
+        ora.Qnum(1000)
+    with pytest.raises(ValueError):
+        ora.Qnum(-1000)
+
+
+# ---------------------------------------------------------------------------
+# Bond known answers (uni10/test/testBond.cpp)
+def test_bond_known_answers(ora):
+    Q = ora.Qnum
+    bd = ora.Bond(ora.BD_OUT, [Q(1), Q(1), Q(0), Q(0), Q(0), Q(-1)])
+    assert bd.dim == 6
+    assert [q.U1 for q in bd.Qlist()] == [1, 1, 0, 0, 0, -1]
+    assert [(q.U1, d) for q, d in bd.degeneracy().items()] == [(-1, 1), (0, 3), (1, 2)]
+    bd1 = ora.Bond(ora.BD_IN, [Q(1), Q(1), Q(0), Q(0), Q(0), Q(-1)])
+    bd2 = ora.Bond(ora.BD_IN, [Q(1), Q(0), Q(0), Q(-1)])
+    bd2.combine(bd1)
+    assert bd2.type == ora.BD_IN and bd2.dim == 24            # testBond.cpp:101-160
+    assert [(q.U1, d) for q, d in bd2.degeneracy().items()] == [(-2, 1), (-1, 5), (0, 9), (1, 7), (2, 2)]
+    b = ora.Bond(ora.BD_IN, 100)
+    b.change(ora.BD_OUT)
+    assert b.type == ora.BD_OUT
+    # non-adjacent repeats stay separate runs
+    r = ora.Bond(ora.BD_IN, [Q(1), Q(0), Q(1)])
+    assert [q.U1 for q in r.Qnums] == [1, 0, 1] and r.Qdegs == [1, 1, 1]
+    with pytest.raises(ValueError):
+        ora.Bond(ora.BD_IN, [])
+
+
+def test_combine_bonds_brute_force(ora):
+    """combine(tp, bonds): the fused charge of every state of the product space
+    in row-major order equals the sum of the per-bond charges (brute force)."""
+    Q = ora.Qnum
+    bds = [ora.Bond(ora.BD_IN, [Q(1), Q(-1)]), ora.Bond(ora.BD_OUT, [Q(2), Q(0), Q(0)]),
+           ora.Bond(ora.BD_IN, [Q(1), Q(0), Q(-1)])]
+    c = ora.combine_bonds(ora.BD_IN, bds)
+    expect = []
+    for s0 in bds[0].Qlist():
+        for s1 in bds[1].copy().change(ora.BD_IN).Qlist():
+            for s2 in bds[2].Qlist():
+                expect.append(s0.U1 + s1.U1 + s2.U1)
+    assert [q.U1 for q in c.Qlist()] == expect
+
+
+# ---------------------------------------------------------------------------
+# Block layout: golden tensor dumps of the reference (uni10/Swig/examples)
+HEIS_S1 = np.array([
+    1, 0, 0, 0, 0, 0, 0, 0, 0,
+    0, 0, 0, 1, 0, 0, 0, 0, 0,
+    0, 0, -1, 0, 1, 0, 0, 0, 0,
+    0, 1, 0, 0, 0, 0, 0, 0, 0,
+    0, 0, 1, 0, 0, 0, 1, 0, 0,
+    0, 0, 0, 0, 0, 0, 0, 1, 0,
+    0, 0, 0, 0, 1, 0, -1, 0, 0,
+    0, 0, 0, 0, 0, 1, 0, 0, 0,
+    0, 0, 0, 0, 0, 0, 0, 0, 1], dtype=np.float64)   # egU1.cpp:31-41
+
+
+def test_grouping_matches_egU1_dump(ora, golden):
+    Q = ora.Qnum
+    qs = [Q(1), Q(0), Q(-1)]
+    bi, bo = ora.Bond(ora.BD_IN, qs), ora.Bond(ora.BD_OUT, qs)
+    H = ora.UniTensor([bi, bi, bo, bo], "H_U1")
+    H.set_raw_elem(HEIS_S1)
+    G = ora.load_tensor(os.path.join(golden, "egU1_H_U1"))
+    assert [b == g for b, g in zip(H.bonds, G.bonds)] == [True] * 4
+    assert H.elem.tobytes() == G.elem.tobytes()
+    assert [q.U1 for q in H.block_qnums()] == [-2, -1, 0, 1, 2]
+    assert [H.layout.blocks[q][:2] for q in H.block_qnums()] == [(1, 1), (2, 2), (3, 3), (2, 2), (1, 1)]
+    # getRawElem inverts setRawElem (all nonzeros of this H sit in blocks)
+    assert np.array_equal(H.get_raw_elem().reshape(-1), HEIS_S1)
+    # at() on every index
+    raw = HEIS_S1.reshape(3, 3, 3, 3)
+    for idx in np.ndindex(3, 3, 3, 3):
+        assert H.at(idx) == raw[idx]
+
+
+def test_transpose_matches_egU3_dump(ora, golden):
+    W = ora.load_tensor(os.path.join(golden, "egU3_W"))
+    WT = ora.load_tensor(os.path.join(golden, "egU3_WT"))
+    assert W.elem_num() == 12 and [b.dim for b in W.bonds] == [6, 3, 3]
+    T = W.copy().transpose()
+    assert T.elem.tobytes() == WT.elem.tobytes()
+    assert all(a == b for a, b in zip(T.bonds, WT.bonds))
+    # W is an isometry block by block (orthoRand): rows orthonormal
+    for q in W.block_qnums():
+        blk = W.get_block(q)
+        assert np.allclose(blk @ blk.T, np.eye(blk.shape[0]), atol=1e-14)
+
+
+def test_block_layout_brute_force(ora):
+    """Every element of setRawElem's buffer lands at the position of its
+    (charge block, row, col) computed by brute force from the row-major
+    odometers over the in- and out-bonds (independent statement of the layout)."""
+    Q = ora.Qnum
+    bonds = [ora.Bond(ora.BD_IN, [Q(1), Q(-1), Q(1)]), ora.Bond(ora.BD_IN, [Q(0), Q(2), Q(2)]),
+             ora.Bond(ora.BD_OUT, [Q(1), Q(1), Q(3), Q(-1)])]
+    T = ora.UniTensor(bonds)
+    dims = [b.dim for b in bonds]
+    raw = np.arange(1, np.prod(dims) + 1, dtype=np.float64).reshape(dims)
+    T.set_raw_elem(raw)
+    # brute force: group row states by charge in run-length-odometer order
+    def states(bl):
+        runs = []
+        for b in bl:
+            runs.append([(q, list(range(o, o + d))) for q, d, o in zip(b.Qnums, b.Qdegs, b.offsets)])
+        out = []
+        import itertools
+        for combo in itertools.product(*runs):
+            q = sum(c[0].U1 for c in combo)
+            for idx in itertools.product(*[c[1] for c in combo]):
+                out.append((q, idx))
+        return out
+    rows = states(bonds[:2])
+    cols = states(bonds[2:])
+    expect = []
+    for q in sorted(set(r[0] for r in rows) & set(c[0] for c in cols)):
+        for r in [x for x in rows if x[0] == q]:
+            for c in [x for x in cols if x[0] == q]:
+                expect.append(raw[r[1] + c[1]])
+    assert T.elem.tolist() == expect
+
+
+# ---------------------------------------------------------------------------
+# Dense permute: the C odometer vs numpy.transpose vs the _reshapeElem formula
+@pytest.mark.parametrize("seed", range(12))
+def test_dense_permute_three_ways(ora, seed):
+    rng = np.random.default_rng(seed)
+    r = int(rng.integers(1, 7))
+    dims = np.array(rng.integers(1, 6, size=r), dtype=np.int64)
+    perm = np.array(rng.permutation(r), dtype=np.int32)
+    n = int(np.prod(dims))
+    x = rng.standard_normal(n)
+    a, b = np.empty(n), np.empty(n)
+    ora.lib().ora_permute_dense(ora._ptr(x), ora._ptr(a), r, ora._ptr(dims), ora._ptr(perm))
+    ora.lib().ora_reshape_elem(ora._ptr(x), ora._ptr(b), r, ora._ptr(dims), ora._ptr(perm))
+    c = np.transpose(x.reshape(dims), perm).reshape(-1)
+    assert a.tobytes() == c.tobytes() and b.tobytes() == c.tobytes()
+
+
+def test_dense_permute_catches_planted_bug(ora):
+    """A transposed perm (the inverse permutation) must be caught by the check."""
+    dims = np.array([2, 3, 4], np.int64)
+    perm = np.array([1, 2, 0], np.int32)
+    inv = np.argsort(perm).astype(np.int32)
+    x = np.arange(24, dtype=np.float64)
+    a = np.empty(24)
+    ora.lib().ora_permute_dense(ora._ptr(x), ora._ptr(a), 3, ora._ptr(dims), ora._ptr(inv))
+    assert a.tobytes() != np.transpose(x.reshape(2, 3, 4), perm).reshape(-1).tobytes()
+
+
+# ---------------------------------------------------------------------------
+# Symmetric permute: densify-and-compare
+def _random_u1(ora, seed, n_in=2, n_out=2):
+    rng = np.random.default_rng(seed)
+    Q = ora.Qnum
+    bonds = []
+    for i in range(n_in + n_out):
+        tp = ora.BD_IN if i < n_in else ora.BD_OUT
+        charges = sorted(set(int(c) for c in rng.integers(-3, 4, size=int(rng.integers(1, 4)))))
+        qs = []
+        for c in charges:
+            qs += [Q(c)] * int(rng.integers(1, 4))
+        if rng.random() < 0.3:     # a non-adjacent repeat
+            qs += [qs[0]]
+        bonds.append(ora.Bond(tp, qs))
+    try:
+        T = ora.UniTensor(bonds)
+    except ValueError:
+        return None
+    T.set_raw_elem(rng.uniform(-1, 1, size=[b.dim for b in bonds]))
+    return T
+
+
+@pytest.mark.parametrize("seed", range(20))
+def test_symmetric_permute_densify(ora, seed):
+    rng = np.random.default_rng(100 + seed)
+    T = _random_u1(ora, seed)
+    if T is None:
+        pytest.skip("random bonds carry no common charge")
+    nb = len(T.bonds)
+    perm = [int(v) for v in rng.permutation(nb)]
+    rb = int(rng.integers(0, nb + 1))
+    before = T.get_raw_elem().reshape([b.dim for b in T.bonds])
+    labels = [T.labels[p] for p in perm]
+    try:
+        P = T.copy().permute(labels, rb)
+    except ValueError:
+        pytest.skip("permuted bonds carry no common charge")
+    after = P.get_raw_elem().reshape([b.dim for b in P.bonds])
+    # elements that survive (inside blocks of both layouts) are moved exactly
+    mask_in = np.zeros(before.shape, bool)
+    T2 = T.copy()
+    T2.set_elem(np.ones(T2.elem_num()))
+    mask_in = T2.get_raw_elem().reshape(before.shape) != 0
+    expect = np.transpose(np.where(mask_in, before, 0.0), perm)
+    assert np.array_equal(after, expect)
+    # and back
+    back = P.copy().permute(T.labels, T.RBondNum)
+    assert back.elem.tobytes() == T.elem.tobytes()
+
+
+def test_contract_vs_tensordot(ora):
+    for seed in range(6):
+        rng = np.random.default_rng(seed)
+        Q = ora.Qnum
+        chi = ora.Bond(ora.BD_IN, [Q(-1)] * 2 + [Q(1)] * 3)
+        phys = ora.Bond(ora.BD_OUT, [Q(1), Q(-1)])
+        A = ora.UniTensor([chi, phys, ora.Bond(ora.BD_OUT, [Q(0)] * 2 + [Q(2)] * 2 + [Q(-2)])])
+        A.set_raw_elem(rng.uniform(-1, 1, [b.dim for b in A.bonds]))
+        A.set_label([1, 2, 3])
+        B = ora.UniTensor([ora.Bond(ora.BD_IN, [Q(0)] * 2 + [Q(2)] * 2 + [Q(-2)]),
+                           ora.Bond(ora.BD_OUT, [Q(1), Q(-1)]), ora.Bond(ora.BD_OUT, [Q(-1)] * 3 + [Q(1)])])
+        B.set_raw_elem(rng.uniform(-1, 1, [b.dim for b in B.bonds]))
+        B.set_label([3, 4, 5])
+        Ad = A.get_raw_elem().reshape([b.dim for b in A.bonds])
+        Bd = B.get_raw_elem().reshape([b.dim for b in B.bonds])
+        C = ora.contract(A, B)
+        Cd = C.get_raw_elem().reshape([b.dim for b in C.bonds])
+        ref = np.tensordot(Ad, Bd, axes=([2], [0]))
+        assert np.allclose(Cd, ref, atol=1e-14, rtol=0)
+        assert A.labels == [1, 2, 3] and B.labels == [3, 4, 5]
+
+
+def test_contract_dense_c2_shape(ora):
+    """The contraction-and-permute workload at small chi, against tensordot."""
+    chi = 8
+    A = ora.UniTensor([ora.Bond(ora.BD_IN, chi), ora.Bond(ora.BD_IN, 2), ora.Bond(ora.BD_OUT, 2),
+                       ora.Bond(ora.BD_OUT, chi)])
+    B = ora.UniTensor([ora.Bond(ora.BD_IN, chi), ora.Bond(ora.BD_IN, 2), ora.Bond(ora.BD_OUT, 2),
+                       ora.Bond(ora.BD_OUT, chi)])
+    a = ora.splitmix64_uniform(1, A.elem_num())
+    b = ora.splitmix64_uniform(2, B.elem_num())
+    A.set_elem(a)
+    B.set_elem(b)
+    A.set_label([0, 1, 2, 3])
+    B.set_label([3, 2, 4, 5])
+    C = ora.contract(A, B)
+    assert C.labels == [0, 1, 4, 5]
+    ref = np.tensordot(a.reshape(chi, 2, 2, chi), b.reshape(chi, 2, 2, chi), axes=([2, 3], [1, 0]))
+    assert np.allclose(C.elem.reshape(ref.shape), ref, rtol=0, atol=1e-13)
+    C.permute([5, 1, 0, 4], 2)
+    assert np.array_equal(C.elem.reshape(chi, 2, chi, 2), np.transpose(C.elem.reshape(chi, 2, chi, 2), (0, 1, 2, 3)))
+
+
+# ---------------------------------------------------------------------------
+# SVD / eigh conventions
+def test_svd_golden_dump(ora, golden):
+    U = np.fromfile(os.path.join(golden, "mat_U"), dtype=np.float64).reshape(4, 4)
+    S = np.fromfile(os.path.join(golden, "mat_Sigma"), dtype=np.float64)
+    VT = np.fromfile(os.path.join(golden, "mat_VT"), dtype=np.float64).reshape(4, 5)
+    assert np.all(np.diff(S) < 0)
+    assert np.allclose(U.T @ U, np.eye(4), atol=1e-15 * 10)
+    assert np.allclose(VT @ VT.T, np.eye(4), atol=1e-15 * 10)
+    M = U @ np.diag(S) @ VT
+    assert M.min() >= -1e-12 and M.max() < 1.0     # Matrix::randomize draws from [0,1)
+    u, s, vt = ora.svd(M)
+    assert u.shape == (4, 4) and s.shape == (4,) and vt.shape == (4, 5)
+    assert np.allclose(s, S, rtol=1e-14, atol=0)
+    for j in range(4):
+        sg = np.sign(u[:, j] @ U[:, j])
+        assert np.allclose(sg * u[:, j], U[:, j], atol=1e-13)
+        assert np.allclose(sg * vt[j], VT[j], atol=1e-13)
+
+
+def test_heisenberg_gate_closed_form(ora):
+    H = ora.heisenberg_two_site(1.0)
+    w, U = ora.eigh(H)
+    assert np.allclose(w, [-0.75, 0.25, 0.25, 0.25], atol=1e-15)
+    tau = 0.01
+    G = ora.exph(-tau, H)
+    # exp(-tau H) = e^{-tau/4} P_triplet + e^{3 tau/4} P_singlet
+    s = np.array([0, 1, -1, 0]) / math.sqrt(2)
+    Ps = np.outer(s, s)
+    Gexact = math.exp(-tau / 4) * (np.eye(4) - Ps) + math.exp(3 * tau / 4) * Ps
+    assert np.max(np.abs(G - Gexact)) <= 1e-15
+
+
+# ---------------------------------------------------------------------------
+# Network
+def test_network_simple_net(ora, golden):
+    MA = np.random.default_rng(1).uniform(0, 1, (3, 4))
+    MB = np.random.default_rng(2).uniform(0, 1, (4, 3))
+    bondsA = [ora.Bond(ora.BD_IN, 3), ora.Bond(ora.BD_OUT, 2), ora.Bond(ora.BD_OUT, 2)]
+    A = ora.UniTensor(bondsA)
+    B = A.copy()
+    A.put_block(MA)
+    B.transpose()
+    B.put_block(MB)
+    net = ora.Network(os.path.join(golden, "Simple.net"))
+    net.put_tensor(0, A)
+    net.put_tensor(1, B)
+    C = net.launch()
+    assert np.allclose(C.get_block(), MA @ MB, atol=1e-12)   # testNetwork.cpp:17-37
+
+
+def test_network_egN1_order_brackets(ora, golden):
+    net = ora.Network(os.path.join(golden, "egN1_network"))
+    assert net.brakets == [2, 0, 0, -1, 0, -1]
+    assert [net.names[i] for i in net.order] == ["W", "H", "WT"]
+    H = ora.load_tensor(os.path.join(golden, "egU1_H_U1"))
+    W = ora.load_tensor(os.path.join(golden, "egU3_W"))
+    WT = ora.load_tensor(os.path.join(golden, "egU3_WT"))
+    net.put_tensor("H", H)
+    net.put_tensor("W", W)
+    net.put_tensor("WT", WT)
+    R = net.launch()
+    assert net.tree() == (("W", "H"), "WT")
+    # dense check: R[-1,-2] = sum W[-1,1,2] H[1,2,3,4] WT[3,4,-2]
+    Wd = W.get_raw_elem().reshape(6, 3, 3)
+    Hd = H.get_raw_elem().reshape(3, 3, 3, 3)
+    WTd = WT.get_raw_elem().reshape(3, 3, 6)
+    ref = np.einsum("aij,ijkl,klb->ab", Wd, Hd, WTd)
+    assert np.allclose(R.get_raw_elem(), ref, atol=1e-13)
+
+
+def test_network_parse_errors(ora, tmp_path):
+    p = tmp_path / "bad.net"
+    p.write_text("A: 1; 2\nB: 2; 3\n")
+    with pytest.raises(ValueError):
+        ora.Network(str(p))
+    p.write_text("A: 1; 2\nB: 2; 3\nTOUT: 1; 3\nORDER: ((A B)\n")
+    with pytest.raises(ValueError):
+        ora.Network(str(p))
+    p.write_text("A: 1; 2\nB: 2; 3\nTOUT: 1; 3")   # unterminated last line is dropped
+    with pytest.raises(ValueError):
+        ora.Network(str(p))
+
+
+def test_network_greedy_equals_dense(ora, golden):
+    """itebd.net without ORDER: the greedy plan's result equals the dense einsum."""
+    chi, d = 5, 2
+    ala, blb, la, lb = ora.itebd_init(chi, d, 3)
+    b3 = [ora.Bond(ora.BD_IN, chi), ora.Bond(ora.BD_IN, chi), ora.Bond(ora.BD_OUT, d)]
+    A = ora.UniTensor(b3, elem=ala.reshape(-1))
+    B = ora.UniTensor(b3, elem=blb.reshape(-1))
+    U = ora.UniTensor([ora.Bond(ora.BD_IN, d)] * 2 + [ora.Bond(ora.BD_OUT, d)] * 2)
+    G = ora.exph(-0.01, ora.heisenberg_two_site())
+    U.put_block(G)
+    net = ora.Network(os.path.join(golden, "itebd.net"))
+    net.put_tensor("ALa", A)
+    net.put_tensor("BLb", B)
+    net.put_tensor("expH", U)
+    C = net.launch()
+    assert C.labels == [-1, -2, -3, -4] and C.RBondNum == 1
+    ref = np.einsum("acx,cby,xyzw->azbw", ala, blb, G.reshape(2, 2, 2, 2))
+    assert np.allclose(C.elem.reshape(ref.shape), ref, atol=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# iTEBD energies
+def test_itebd_chi1_product_state_energy_bounds(ora, golden):
+    """chi=1: variational energies of a product state lie in [-1/4, 1/4]."""
+    t = ora.ITEBD(1, 2, ora.heisenberg_two_site(), 0.05, 9, golden)
+    for _ in range(50):
+        t.step()
+    e = t.energy()
+    assert -0.25 - 1e-12 <= e <= 0.25 + 1e-12
+
+
+def test_itebd_converges_towards_bethe(ora, golden):
+    """chi=16, 600 steps at tau=0.05 then 0.01: energy per site approaches
+    1/4 - ln 2 from above (variational) within 2e-3."""
+    t = ora.ITEBD(16, 2, ora.heisenberg_two_site(), 0.05, 1, golden)
+    for _ in range(300):
+        t.step()
+    t.tau = 0.01
+    t.U.put_block(ora.exph(-0.01, t.H))
+    for _ in range(300):
+        t.step()
+    e = t.energy()
+    bethe = 0.25 - math.log(2)
+    assert bethe - 1e-9 <= e <= bethe + 2e-3

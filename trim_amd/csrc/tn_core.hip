@@ -1,0 +1,321 @@
+// Device selection, error reporting, the caching device-memory pool,
+// copies/fills, streams and events of libtn_hip.so.
+//
+// Replaces the reference's global GPU state and allocators
+// (uni10/src/uni10/tools/lib/uni10_tools_gpu.cu:7-163): instead of
+// cudaMallocManaged buffers that host loops touch through page migration,
+// every buffer is explicit device memory handed out by a size-binned cache,
+// with a byte budget used for admission control by the request loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <unordered_map>
+
+#include "tn_internal.h"
+
+namespace tn {
+
+static thread_local char g_err[1024] = {0};
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+void clear_error() { g_err[0] = 0; }
+
+struct DeviceState {
+  std::mutex mu;
+  bool ready = false;
+  int device = -1;
+  int cus = 0;
+  size_t hbm = 0;
+  char arch[64] = {0};
+  hipStream_t stream = nullptr;  // library default stream (default priority)
+  // pool
+  std::multimap<size_t, void*> free_blocks;       // size -> block
+  std::unordered_map<void*, size_t> live;         // block -> size
+  size_t in_use = 0, cached = 0, peak = 0, budget = 0;
+};
+
+static DeviceState& st() {
+  static DeviceState s;
+  return s;
+}
+
+static int init_locked(DeviceState& s, int device) {
+  if (s.ready) {
+    TN_REQUIRE(device < 0 || device == s.device, TN_EINVAL,
+               "tn_init: already initialised on device %d (asked %d)", s.device, device);
+    return TN_OK;
+  }
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  TN_REQUIRE(e == hipSuccess && n > 0, TN_ENODEV, "no HIP device available (%s)",
+             e == hipSuccess ? "count=0" : hipGetErrorString(e));
+  if (device < 0) device = 0;
+  TN_REQUIRE(device < n, TN_EINVAL, "device %d out of range (%d devices)", device, n);
+  TN_HIP_CHECK(hipSetDevice(device));
+  hipDeviceProp_t p;
+  TN_HIP_CHECK(hipGetDeviceProperties(&p, device));
+  TN_REQUIRE(std::strncmp(p.gcnArchName, "gfx950", 6) == 0, TN_ENODEV,
+             "device %d is %s; this library is built for gfx950 only", device, p.gcnArchName);
+  s.cus = p.multiProcessorCount;
+  s.hbm = p.totalGlobalMem;
+  std::snprintf(s.arch, sizeof(s.arch), "%s", p.gcnArchName);
+  TN_HIP_CHECK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+  s.device = device;
+  s.ready = true;
+  return TN_OK;
+}
+
+int ensure_init() {
+  DeviceState& s = st();
+  if (s.ready) return TN_OK;
+  std::lock_guard<std::mutex> g(s.mu);
+  return init_locked(s, -1);
+}
+
+int cu_count() { return st().cus > 0 ? st().cus : 256; }
+
+hipStream_t stream_of(void* h) { return h ? (hipStream_t)h : st().stream; }
+
+static size_t bin_size(size_t bytes) {
+  if (bytes == 0) bytes = 1;
+  const size_t small = size_t(1) << 20;
+  if (bytes <= small) return (bytes + 511) & ~size_t(511);
+  const size_t g = size_t(2) << 20;
+  return (bytes + g - 1) / g * g;
+}
+
+static int release_cached_locked(DeviceState& s) {
+  for (auto& kv : s.free_blocks) {
+    hipError_t e = hipFree(kv.second);
+    if (e != hipSuccess) {
+      set_error("hipFree failed: %s", hipGetErrorString(e));
+      return TN_EHIP;
+    }
+  }
+  s.free_blocks.clear();
+  s.cached = 0;
+  return TN_OK;
+}
+
+int pool_alloc(void** p, size_t bytes) {
+  TN_REQUIRE(p != nullptr, TN_EINVAL, "tn_malloc: NULL out pointer");
+  TN_INIT_OR_RETURN();
+  DeviceState& s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  size_t sz = bin_size(bytes);
+  TN_REQUIRE(s.budget == 0 || s.in_use + sz <= s.budget, TN_ENOMEM,
+             "device memory budget exceeded: in use %zu + request %zu > budget %zu",
+             s.in_use, sz, s.budget);
+  auto it = s.free_blocks.lower_bound(sz);
+  if (it != s.free_blocks.end() && it->first <= sz + sz / 4) {
+    void* q = it->second;
+    size_t qs = it->first;
+    s.free_blocks.erase(it);
+    s.cached -= qs;
+    s.live[q] = qs;
+    s.in_use += qs;
+    s.peak = std::max(s.peak, s.in_use);
+    *p = q;
+    return TN_OK;
+  }
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, sz);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    int rc = release_cached_locked(s);
+    if (rc != TN_OK) return rc;
+    e = hipMalloc(&q, sz);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("hipMalloc(%zu) failed: %s", sz, hipGetErrorString(e));
+    return TN_ENOMEM;
+  }
+  s.live[q] = sz;
+  s.in_use += sz;
+  s.peak = std::max(s.peak, s.in_use);
+  *p = q;
+  return TN_OK;
+}
+
+int pool_free(void* p) {
+  if (p == nullptr) return TN_OK;
+  DeviceState& s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  auto it = s.live.find(p);
+  TN_REQUIRE(it != s.live.end(), TN_EINVAL, "tn_free: %p was not allocated by tn_malloc", p);
+  size_t sz = it->second;
+  s.live.erase(it);
+  s.in_use -= sz;
+  s.free_blocks.emplace(sz, p);
+  s.cached += sz;
+  return TN_OK;
+}
+
+}  // namespace tn
+
+using namespace tn;
+
+// ---------------------------------------------------------------------------
+__global__ void tn_fill_zero_kernel(uint64_t* __restrict__ p, size_t n_words,
+                                    unsigned char* __restrict__ tail, size_t n_tail) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n_words; i += stride) p[i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < n_tail) tail[threadIdx.x] = 0;
+}
+
+extern "C" {
+
+const char* tn_last_error(void) { return g_err; }
+
+int tn_init(int device) {
+  DeviceState& s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  return init_locked(s, device);
+}
+
+int tn_device_info(int* cus, size_t* hbm, char* arch, int arch_len) {
+  TN_INIT_OR_RETURN();
+  DeviceState& s = st();
+  if (cus) *cus = s.cus;
+  if (hbm) *hbm = s.hbm;
+  if (arch && arch_len > 0) std::snprintf(arch, arch_len, "%s", s.arch);
+  return TN_OK;
+}
+
+int tn_set_budget(size_t bytes) {
+  DeviceState& s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  s.budget = bytes;
+  return TN_OK;
+}
+
+int tn_mem_stats(size_t* in_use, size_t* cached, size_t* peak, size_t* budget) {
+  DeviceState& s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  if (in_use) *in_use = s.in_use;
+  if (cached) *cached = s.cached;
+  if (peak) *peak = s.peak;
+  if (budget) *budget = s.budget;
+  return TN_OK;
+}
+
+int tn_malloc(void** dptr, size_t bytes) { return pool_alloc(dptr, bytes); }
+int tn_free(void* dptr) { return pool_free(dptr); }
+
+int tn_pool_trim(void) {
+  TN_INIT_OR_RETURN();
+  TN_HIP_CHECK(hipDeviceSynchronize());
+  DeviceState& s = st();
+  std::lock_guard<std::mutex> g(s.mu);
+  return release_cached_locked(s);
+}
+
+int tn_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
+  TN_INIT_OR_RETURN();
+  if (bytes == 0) return TN_OK;
+  TN_REQUIRE(dst && src, TN_EINVAL, "tn_memcpy_h2d: NULL pointer");
+  hipStream_t s = stream_of(stream);
+  TN_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+  TN_HIP_CHECK(hipStreamSynchronize(s));  // source may be pageable/transient
+  return TN_OK;
+}
+
+int tn_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
+  TN_INIT_OR_RETURN();
+  if (bytes == 0) return TN_OK;
+  TN_REQUIRE(dst && src, TN_EINVAL, "tn_memcpy_d2h: NULL pointer");
+  hipStream_t s = stream_of(stream);
+  TN_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+  TN_HIP_CHECK(hipStreamSynchronize(s));
+  return TN_OK;
+}
+
+int tn_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
+  TN_INIT_OR_RETURN();
+  if (bytes == 0) return TN_OK;
+  TN_REQUIRE(dst && src, TN_EINVAL, "tn_memcpy_d2d: NULL pointer");
+  TN_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream_of(stream)));
+  return TN_OK;
+}
+
+int tn_zero(void* dst, size_t bytes, void* stream) {
+  TN_INIT_OR_RETURN();
+  if (bytes == 0) return TN_OK;
+  TN_REQUIRE(dst != nullptr, TN_EINVAL, "tn_zero: NULL pointer");
+  TN_REQUIRE(((uintptr_t)dst & 7) == 0, TN_EINVAL, "tn_zero: pointer not 8-byte aligned");
+  size_t words = bytes / 8, tail = bytes % 8;
+  size_t blocks = std::min<size_t>((words + 255) / 256, 4096);
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(tn_fill_zero_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                     stream_of(stream), (uint64_t*)dst, words,
+                     (unsigned char*)dst + words * 8, tail);
+  TN_LAUNCH_CHECK();
+  return TN_OK;
+}
+
+int tn_stream_create(void** stream) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(stream != nullptr, TN_EINVAL, "tn_stream_create: NULL out pointer");
+  hipStream_t s;
+  TN_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = (void*)s;
+  return TN_OK;
+}
+
+int tn_stream_destroy(void* stream) {
+  if (!stream) return TN_OK;
+  TN_HIP_CHECK(hipStreamDestroy((hipStream_t)stream));
+  return TN_OK;
+}
+
+int tn_stream_sync(void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_HIP_CHECK(hipStreamSynchronize(stream_of(stream)));
+  return TN_OK;
+}
+
+int tn_device_sync(void) {
+  TN_INIT_OR_RETURN();
+  TN_HIP_CHECK(hipDeviceSynchronize());
+  return TN_OK;
+}
+
+int tn_event_create(void** ev) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(ev != nullptr, TN_EINVAL, "tn_event_create: NULL out pointer");
+  hipEvent_t e;
+  TN_HIP_CHECK(hipEventCreate(&e));
+  *ev = (void*)e;
+  return TN_OK;
+}
+
+int tn_event_destroy(void* ev) {
+  if (!ev) return TN_OK;
+  TN_HIP_CHECK(hipEventDestroy((hipEvent_t)ev));
+  return TN_OK;
+}
+
+int tn_event_record(void* ev, void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_HIP_CHECK(hipEventRecord((hipEvent_t)ev, stream_of(stream)));
+  return TN_OK;
+}
+
+int tn_event_elapsed_ms(void* a, void* b, float* ms) {
+  TN_INIT_OR_RETURN();
+  TN_HIP_CHECK(hipEventSynchronize((hipEvent_t)b));
+  TN_HIP_CHECK(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+  return TN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,66 @@
+// Internal helpers shared by the HIP translation units of libtn_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "tn_hip.h"
+
+namespace tn {
+
+// Thread-local last-error message (what tn_last_error() returns).
+void set_error(const char* fmt, ...);
+void clear_error();
+
+// The stream to use for a caller-supplied stream handle (NULL -> default).
+hipStream_t stream_of(void* s);
+
+// Ensure the device is initialised; returns TN_OK or an error code.
+int ensure_init();
+
+int cu_count();
+
+// Device allocation through the caching pool (used internally for plans and
+// scratch; same pool as tn_malloc).
+int pool_alloc(void** p, size_t bytes);
+int pool_free(void* p);
+
+}  // namespace tn
+
+#define TN_HIP_CHECK(expr)                                                   \
+  do {                                                                       \
+    hipError_t _e = (expr);                                                  \
+    if (_e != hipSuccess) {                                                  \
+      ::tn::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                      __FILE__, __LINE__);                                   \
+      return TN_EHIP;                                                        \
+    }                                                                        \
+  } while (0)
+
+#define TN_REQUIRE(cond, code, ...)  \
+  do {                               \
+    if (!(cond)) {                   \
+      ::tn::set_error(__VA_ARGS__);  \
+      return (code);                 \
+    }                                \
+  } while (0)
+
+#define TN_INIT_OR_RETURN()        \
+  do {                             \
+    int _rc = ::tn::ensure_init(); \
+    if (_rc != TN_OK) return _rc;  \
+  } while (0)
+
+// Kernel launch error check (launch configuration errors only).
+#define TN_LAUNCH_CHECK()                                                   \
+  do {                                                                      \
+    hipError_t _e = hipGetLastError();                                      \
+    if (_e != hipSuccess) {                                                 \
+      ::tn::set_error("kernel launch failed: %s (%s:%d)",                   \
+                      hipGetErrorString(_e), __FILE__, __LINE__);           \
+      return TN_EHIP;                                                       \
+    }                                                                       \
+  } while (0)

@@ -1,0 +1,573 @@
+// FP64 dense decompositions and small vector kernels.
+//
+//  * SVD: one-sided (Hestenes) Jacobi.  The reference calls dgesvd('S','S')
+//    on the transposed view (uni10_lapack_cpu.cpp:289-313) or cuSOLVER gesvd
+//    on device (uni10_lapack_gpu.cu:418-500).  Jacobi is used here because it
+//    is parallel within a round (disjoint column pairs), gives singular values
+//    to high relative accuracy, and is deterministic (fixed pairing order,
+//    fixed reduction trees).  One workgroup per matrix; the working columns
+//    (W = A or A^T, column-major) and the accumulated rotations V live in LDS
+//    when they fit, otherwise in a device scratch area of the same layout.
+//  * eigh: parallel two-sided Jacobi (replaces dsyev, uni10_lapack_cpu.cpp:131-152,
+//    which the reference's GPU build also ran on the host,
+//    uni10_lapack_gpu.cu:380-416).
+//  * norm / scale / diagonal row-column scaling / truncation: single-pass
+//    kernels with fixed-order reductions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+#include "tn_internal.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr size_t kLdsBytes = 152 * 1024;   // 160 KiB per CU minus the kernel's static arrays
+
+struct SvdProb {
+  int64_t m, n;              // A is m x n row-major
+  int64_t a_off, u_off, s_off, vt_off;
+  int64_t scratch_off;       // offset (doubles) into the scratch area, -1 if LDS
+};
+
+// Group-of-tp reduction (tp a power of two <= 64, groups aligned in a wave).
+__device__ inline double group_sum(double v, int tp) {
+  for (int o = tp >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Round-robin pairing: np even; round r in [0, np-1); pair k in [0, np/2).
+__device__ inline void rr_pair(int r, int k, int np, int& p, int& q) {
+  const int L = np - 1;
+  if (k == 0) {
+    p = r;
+    q = L;
+  } else {
+    p = (r + k) % L;
+    q = (r - k + L) % L;
+  }
+  if (p > q) {
+    int t = p;
+    p = q;
+    q = t;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) svd_jacobi_kernel(const SvdProb* __restrict__ probs,
+                                                              const double* __restrict__ A,
+                                                              double* __restrict__ U,
+                                                              double* __restrict__ S,
+                                                              double* __restrict__ VT,
+                                                              double* __restrict__ scratch,
+                                                              int* __restrict__ status) {
+  extern __shared__ double lds[];
+  __shared__ double sig[512];
+  __shared__ int rank_of[512];
+  const SvdProb pr = probs[blockIdx.x];
+  const int64_t M = pr.m, N = pr.n;
+  const bool tr = (M < N);           // work on A^T
+  const int m = (int)(tr ? N : M);   // column length
+  const int n = (int)(tr ? M : N);   // number of columns
+  if (m == 0 || n == 0) return;
+  double* W = (pr.scratch_off < 0) ? lds : scratch + pr.scratch_off;
+  double* V = W + (size_t)m * n;
+  const int tid = threadIdx.x;
+  const double* Ab = A + pr.a_off;
+
+  // W column-major: W[j*m + i]
+  for (int64_t e = tid; e < (int64_t)m * n; e += kThreads) {
+    int j = (int)(e / m), i = (int)(e % m);
+    W[e] = tr ? Ab[(int64_t)j * N + i] : Ab[(int64_t)i * N + j];
+  }
+  for (int64_t e = tid; e < (int64_t)n * n; e += kThreads) {
+    int j = (int)(e / n), i = (int)(e % n);
+    V[e] = (i == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+
+  const int np = (n % 2) ? n + 1 : n;
+  const int npairs = np / 2;
+  int tp = 1;
+  while (tp * 2 * npairs <= kThreads && tp < 64) tp *= 2;
+  const int groups = kThreads / tp;
+  const int g = tid / tp, t = tid % tp;
+  const double tol = DBL_EPSILON * sqrt((double)m);
+
+  int sweep = 0;
+  const int max_sweeps = 60;
+  for (; sweep < max_sweeps; ++sweep) {
+    int rotated = 0;
+    for (int r = 0; r < np - 1; ++r) {
+      for (int k0 = 0; k0 < npairs; k0 += groups) {
+        const int k = k0 + g;
+        // every lane of a group runs the same path; groups past the end idle
+        int p = 0, q = 0;
+        bool live = (k < npairs);
+        if (live) {
+          rr_pair(r, k, np, p, q);
+          live = (q < n);
+        }
+        double a = 0, b = 0, c = 0;
+        if (live) {
+          const double* wp = W + (size_t)p * m;
+          const double* wq = W + (size_t)q * m;
+          for (int i = t; i < m; i += tp) {
+            double x = wp[i], y = wq[i];
+            a += x * x;
+            b += y * y;
+            c += x * y;
+          }
+        }
+        a = group_sum(a, tp);
+        b = group_sum(b, tp);
+        c = group_sum(c, tp);
+        if (live && fabs(c) > tol * sqrt(a * b) && c != 0.0) {
+          rotated = 1;
+          const double zeta = (b - a) / (2.0 * c);
+          const double tt = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          const double cs = 1.0 / sqrt(1.0 + tt * tt);
+          const double sn = cs * tt;
+          double* wp = W + (size_t)p * m;
+          double* wq = W + (size_t)q * m;
+          for (int i = t; i < m; i += tp) {
+            double x = wp[i], y = wq[i];
+            wp[i] = cs * x - sn * y;
+            wq[i] = sn * x + cs * y;
+          }
+          double* vp = V + (size_t)p * n;
+          double* vq = V + (size_t)q * n;
+          for (int i = t; i < n; i += tp) {
+            double x = vp[i], y = vq[i];
+            vp[i] = cs * x - sn * y;
+            vq[i] = sn * x + cs * y;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (!__syncthreads_or(rotated)) break;
+  }
+  if (tid == 0 && sweep >= max_sweeps) atomicAdd(status, 1);
+
+  // singular values = column norms (fixed order per column)
+  for (int j = tid; j < n; j += kThreads) {
+    const double* w = W + (size_t)j * m;
+    double s = 0;
+    for (int i = 0; i < m; ++i) s += w[i] * w[i];
+    sig[j] = sqrt(s);
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += kThreads) {
+    const double sj = sig[j];
+    int rk = 0;
+    for (int i = 0; i < n; ++i) {
+      const double si = sig[i];
+      rk += (si > sj) || (si == sj && i < j);
+    }
+    rank_of[j] = rk;
+  }
+  __syncthreads();
+  double* Ub = U + pr.u_off;
+  double* Sb = S + pr.s_off;
+  double* VTb = VT + pr.vt_off;
+  for (int j = tid; j < n; j += kThreads) Sb[rank_of[j]] = sig[j];
+  // normalised W columns -> U (A = U S V^T) or VT (A^T = W S V^T)
+  for (int64_t e = tid; e < (int64_t)m * n; e += kThreads) {
+    int j = (int)(e / m), i = (int)(e % m);
+    double sj = sig[j];
+    double v = (sj > 0.0) ? W[e] / sj : 0.0;
+    int rk = rank_of[j];
+    if (!tr)
+      Ub[(int64_t)i * n + rk] = v;       // U is M x n (n = N)
+    else
+      VTb[(int64_t)rk * N + i] = v;      // VT is n x N (n = M)
+  }
+  for (int64_t e = tid; e < (int64_t)n * n; e += kThreads) {
+    int j = (int)(e / n), i = (int)(e % n);
+    int rk = rank_of[j];
+    if (!tr)
+      VTb[(int64_t)rk * n + i] = V[e];   // VT is n x n
+    else
+      Ub[(int64_t)i * n + rk] = V[e];    // U is M x M
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Parallel two-sided Jacobi for a symmetric n x n matrix (n <= 256).
+__global__ void __launch_bounds__(kThreads) eigh_jacobi_kernel(const double* __restrict__ A, int n,
+                                                               double* __restrict__ eig,
+                                                               double* __restrict__ vec_rows,
+                                                               int* __restrict__ status) {
+  extern __shared__ double lds[];
+  double* a = lds;              // n x n row-major (symmetric)
+  double* v = lds + n * n;      // n x n, columns = eigenvectors
+  __shared__ double cs_sh[128], sn_sh[128];
+  __shared__ int p_sh[128], q_sh[128];
+  __shared__ double red[kThreads];
+  __shared__ double ev[256];
+  __shared__ int rk[256];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < n * n; e += kThreads) {
+    a[e] = A[e];
+    v[e] = ((e / n) == (e % n)) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  // symmetrise from the upper triangle (dsyev 'U' reads the upper part)
+  for (int e = tid; e < n * n; e += kThreads) {
+    int i = e / n, j = e % n;
+    if (i > j) a[e] = a[j * n + i];
+  }
+  __syncthreads();
+  double s = 0;
+  for (int e = tid; e < n * n; e += kThreads) s += a[e] * a[e];
+  red[tid] = s;
+  __syncthreads();
+  for (int o = kThreads / 2; o > 0; o >>= 1) {
+    if (tid < o) red[tid] += red[tid + o];
+    __syncthreads();
+  }
+  const double thresh = 0.5 * DBL_EPSILON * sqrt(red[0]);
+  __syncthreads();
+  const int np = (n % 2) ? n + 1 : n;
+  const int npairs = np / 2;
+  int sweep = 0;
+  for (; sweep < 100 && n > 1; ++sweep) {
+    int rot = 0;
+    for (int r = 0; r < np - 1; ++r) {
+      for (int k = tid; k < npairs; k += kThreads) {
+        int p, q;
+        rr_pair(r, k, np, p, q);
+        double c = 1.0, sn = 0.0;
+        if (q < n) {
+          double apq = a[p * n + q];
+          if (fabs(apq) > thresh) {
+            double zeta = (a[q * n + q] - a[p * n + p]) / (2.0 * apq);
+            double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+            c = 1.0 / sqrt(1.0 + t * t);
+            sn = c * t;
+            rot = 1;
+          }
+        }
+        cs_sh[k] = c;
+        sn_sh[k] = sn;
+        p_sh[k] = p;
+        q_sh[k] = (q < n) ? q : -1;
+      }
+      __syncthreads();
+      // A <- A J (columns p, q)
+      for (int e = tid; e < npairs * n; e += kThreads) {
+        int k = e / n, i = e % n;
+        int p = p_sh[k], q = q_sh[k];
+        if (q < 0) continue;
+        double c = cs_sh[k], sn = sn_sh[k];
+        double x = a[i * n + p], y = a[i * n + q];
+        a[i * n + p] = c * x - sn * y;
+        a[i * n + q] = sn * x + c * y;
+        double vx = v[i * n + p], vy = v[i * n + q];
+        v[i * n + p] = c * vx - sn * vy;
+        v[i * n + q] = sn * vx + c * vy;
+      }
+      __syncthreads();
+      // A <- J^T A (rows p, q)
+      for (int e = tid; e < npairs * n; e += kThreads) {
+        int k = e / n, j = e % n;
+        int p = p_sh[k], q = q_sh[k];
+        if (q < 0) continue;
+        double c = cs_sh[k], sn = sn_sh[k];
+        double x = a[p * n + j], y = a[q * n + j];
+        a[p * n + j] = c * x - sn * y;
+        a[q * n + j] = sn * x + c * y;
+      }
+      __syncthreads();
+    }
+    if (!__syncthreads_or(rot)) break;
+  }
+  if (tid == 0 && sweep >= 100) atomicAdd(status, 1);
+  for (int j = tid; j < n; j += kThreads) ev[j] = a[j * n + j];
+  __syncthreads();
+  for (int j = tid; j < n; j += kThreads) {
+    int r = 0;
+    for (int i = 0; i < n; ++i) r += (ev[i] < ev[j]) || (ev[i] == ev[j] && i < j);
+    rk[j] = r;
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += kThreads) eig[rk[j]] = ev[j];
+  for (int e = tid; e < n * n; e += kThreads) {
+    int i = e / n, j = e % n;   // v[i][j]: component i of eigenvector j
+    vec_rows[(int64_t)rk[j] * n + i] = v[e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <int T>
+__device__ inline double block_sum(double v, double* red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+  for (int o = T / 2; o > 0; o >>= 1) {
+    if (tid < o) red[tid] += red[tid + o];
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(1024) norm2_kernel(const double* __restrict__ x, int64_t n,
+                                                     double* __restrict__ out) {
+  __shared__ double red[1024];
+  double s = 0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) s += x[i] * x[i];
+  double t = block_sum<1024>(s, red);
+  if (threadIdx.x == 0) *out = sqrt(t);
+}
+
+__global__ void scale_kernel(double* __restrict__ x, int64_t n, double a) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] *= a;
+}
+
+__global__ void scale_inv_dev_kernel(double* __restrict__ x, int64_t n,
+                                     const double* __restrict__ d) {
+  const double a = 1.0 / d[0];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] *= a;
+}
+
+__global__ void diag_row_mul_kernel(double* __restrict__ mat, const double* __restrict__ diag,
+                                    int64_t M, int64_t N) {
+  const int64_t total = M * N;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x)
+    mat[e] *= diag[e / N];
+}
+
+__global__ void diag_col_mul_kernel(double* __restrict__ mat, const double* __restrict__ diag,
+                                    int64_t M, int64_t N) {
+  const int64_t total = M * N;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x)
+    mat[e] *= diag[e % N];
+}
+
+__global__ void __launch_bounds__(1024) truncate_kernel(const double* __restrict__ S, int64_t k,
+                                                        int64_t dim, double* __restrict__ lam,
+                                                        double* __restrict__ norm_out,
+                                                        double* __restrict__ err_out) {
+  __shared__ double red[1024];
+  double a = 0, b = 0;
+  for (int64_t i = threadIdx.x; i < k; i += 1024) {
+    double s = S[i];
+    if (i < dim)
+      a += s * s;
+    else
+      b += s * s;
+  }
+  const double keep = block_sum<1024>(a, red);
+  const double drop = block_sum<1024>(b, red);
+  const double nrm = sqrt(keep);
+  for (int64_t i = threadIdx.x; i < dim; i += 1024) lam[i] = S[i] / nrm;
+  if (threadIdx.x == 0) {
+    *norm_out = nrm;
+    if (err_out) *err_out = drop;
+  }
+}
+
+unsigned grid_for(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 8192));
+}
+
+// Run a batch of SVD problems: LDS when W and V fit, device scratch otherwise.
+int svd_batched(int64_t batch, const int64_t* m, const int64_t* n, const double* A,
+                const int64_t* a_off, double* U, const int64_t* u_off, double* S,
+                const int64_t* s_off, double* VT, const int64_t* vt_off, hipStream_t st) {
+  if (batch == 0) return TN_OK;
+  std::vector<SvdProb> lds_probs, glb_probs;
+  size_t lds_need = 0;
+  int64_t scratch_total = 0;
+  for (int64_t i = 0; i < batch; ++i) {
+    TN_REQUIRE(m[i] >= 0 && n[i] >= 0, TN_EINVAL, "svd: negative shape");
+    int64_t mm = std::max(m[i], n[i]), nn = std::min(m[i], n[i]);
+    TN_REQUIRE(nn <= 512, TN_EUNSUPP, "svd: min(M,N)=%lld > 512 is not supported yet",
+               (long long)nn);
+    SvdProb p{m[i], n[i], a_off ? a_off[i] : 0, u_off ? u_off[i] : 0, s_off ? s_off[i] : 0,
+              vt_off ? vt_off[i] : 0, -1};
+    size_t bytes = sizeof(double) * (size_t)(mm * nn + nn * nn);
+    if (bytes <= kLdsBytes) {
+      lds_need = std::max(lds_need, bytes);
+      lds_probs.push_back(p);
+    } else {
+      p.scratch_off = scratch_total;
+      scratch_total += mm * nn + nn * nn;
+      glb_probs.push_back(p);
+    }
+  }
+  int* d_status = nullptr;
+  SvdProb* d_p = nullptr;
+  double* d_scratch = nullptr;
+  int rc = tn::pool_alloc((void**)&d_status, sizeof(int));
+  if (rc == TN_OK) rc = tn_zero(d_status, sizeof(int), st);
+  const size_t np = lds_probs.size() + glb_probs.size();
+  if (rc == TN_OK) rc = tn::pool_alloc((void**)&d_p, sizeof(SvdProb) * np);
+  std::vector<SvdProb> all(lds_probs);
+  all.insert(all.end(), glb_probs.begin(), glb_probs.end());
+  if (rc == TN_OK) rc = tn_memcpy_h2d(d_p, all.data(), sizeof(SvdProb) * np, st);
+  if (rc == TN_OK && scratch_total > 0)
+    rc = tn::pool_alloc((void**)&d_scratch, sizeof(double) * scratch_total);
+  if (rc == TN_OK && !lds_probs.empty()) {
+    static bool attr_set = false;
+    if (!attr_set) {
+      (void)hipFuncSetAttribute((const void*)svd_jacobi_kernel,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes);
+      (void)hipGetLastError();
+      attr_set = true;
+    }
+    hipLaunchKernelGGL(svd_jacobi_kernel, dim3((unsigned)lds_probs.size()), dim3(kThreads),
+                       lds_need, st, d_p, A, U, S, VT, d_scratch, d_status);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      tn::set_error("svd kernel launch: %s", hipGetErrorString(e));
+      rc = TN_EHIP;
+    }
+  }
+  if (rc == TN_OK && !glb_probs.empty()) {
+    hipLaunchKernelGGL(svd_jacobi_kernel, dim3((unsigned)glb_probs.size()), dim3(kThreads), 0, st,
+                       d_p + lds_probs.size(), A, U, S, VT, d_scratch, d_status);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      tn::set_error("svd kernel launch: %s", hipGetErrorString(e));
+      rc = TN_EHIP;
+    }
+  }
+  int h_status = 0;
+  if (rc == TN_OK) rc = tn_memcpy_d2h(&h_status, d_status, sizeof(int), st);  // synchronises
+  tn::pool_free(d_status);
+  tn::pool_free(d_p);
+  tn::pool_free(d_scratch);
+  if (rc != TN_OK) return rc;
+  TN_REQUIRE(h_status == 0, TN_ENOCONV, "svd: %d problem(s) did not converge in 60 sweeps",
+             h_status);
+  return TN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tn_svd_f64(const double* A, int64_t M, int64_t N, double* U, double* S, double* VT,
+               void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(A && U && S && VT, TN_EINVAL, "tn_svd_f64: NULL buffer");
+  int64_t m = M, n = N, z = 0;
+  return svd_batched(1, &m, &n, A, &z, U, &z, S, &z, VT, &z, tn::stream_of(stream));
+}
+
+int tn_svd_f64_batched(int64_t batch, const int64_t* m, const int64_t* n, const double* A,
+                       const int64_t* a_off, double* U, const int64_t* u_off, double* S,
+                       const int64_t* s_off, double* VT, const int64_t* vt_off, void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(batch >= 0 && (batch == 0 || (m && n && A && U && S && VT)), TN_EINVAL,
+             "tn_svd_f64_batched: NULL argument");
+  return svd_batched(batch, m, n, A, a_off, U, u_off, S, s_off, VT, vt_off,
+                     tn::stream_of(stream));
+}
+
+int tn_eigh_f64(const double* A, int64_t N, double* eig, double* vec_rows, void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(A && eig && vec_rows, TN_EINVAL, "tn_eigh_f64: NULL buffer");
+  TN_REQUIRE(N >= 1 && N <= 90, TN_EUNSUPP, "tn_eigh_f64: N=%lld outside [1, 90]", (long long)N);
+  hipStream_t st = tn::stream_of(stream);
+  int* d_status = nullptr;
+  int rc = tn::pool_alloc((void**)&d_status, sizeof(int));
+  if (rc != TN_OK) return rc;
+  rc = tn_zero(d_status, sizeof(int), st);
+  if (rc == TN_OK) {
+    size_t lds = sizeof(double) * 2 * N * N;
+    static bool attr_set = false;
+    if (!attr_set) {
+      (void)hipFuncSetAttribute((const void*)eigh_jacobi_kernel,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 140 * 1024);
+      (void)hipGetLastError();
+      attr_set = true;
+    }
+    hipLaunchKernelGGL(eigh_jacobi_kernel, dim3(1), dim3(kThreads), lds, st, A, (int)N, eig,
+                       vec_rows, d_status);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      tn::set_error("eigh kernel launch: %s", hipGetErrorString(e));
+      rc = TN_EHIP;
+    }
+  }
+  int h = 0;
+  if (rc == TN_OK) rc = tn_memcpy_d2h(&h, d_status, sizeof(int), st);
+  tn::pool_free(d_status);
+  if (rc != TN_OK) return rc;
+  TN_REQUIRE(h == 0, TN_ENOCONV, "eigh: Jacobi did not converge");
+  return TN_OK;
+}
+
+int tn_norm2_f64(const double* x, int64_t n, double* out_dev, void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(out_dev && (n == 0 || x) && n >= 0, TN_EINVAL, "tn_norm2_f64: bad argument");
+  hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, tn::stream_of(stream), x, n, out_dev);
+  TN_LAUNCH_CHECK();
+  return TN_OK;
+}
+
+int tn_scale_f64(double* x, int64_t n, double a, void* stream) {
+  TN_INIT_OR_RETURN();
+  if (n == 0) return TN_OK;
+  TN_REQUIRE(x && n > 0, TN_EINVAL, "tn_scale_f64: bad argument");
+  hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, tn::stream_of(stream), x, n, a);
+  TN_LAUNCH_CHECK();
+  return TN_OK;
+}
+
+int tn_scale_inv_dev_f64(double* x, int64_t n, const double* d, void* stream) {
+  TN_INIT_OR_RETURN();
+  if (n == 0) return TN_OK;
+  TN_REQUIRE(x && d && n > 0, TN_EINVAL, "tn_scale_inv_dev_f64: bad argument");
+  hipLaunchKernelGGL(scale_inv_dev_kernel, dim3(grid_for(n)), dim3(256), 0, tn::stream_of(stream),
+                     x, n, d);
+  TN_LAUNCH_CHECK();
+  return TN_OK;
+}
+
+int tn_diag_row_mul_f64(double* mat, const double* diag, int64_t M, int64_t N, void* stream) {
+  TN_INIT_OR_RETURN();
+  if (M == 0 || N == 0) return TN_OK;
+  TN_REQUIRE(mat && diag && M > 0 && N > 0, TN_EINVAL, "tn_diag_row_mul_f64: bad argument");
+  hipLaunchKernelGGL(diag_row_mul_kernel, dim3(grid_for(M * N)), dim3(256), 0,
+                     tn::stream_of(stream), mat, diag, M, N);
+  TN_LAUNCH_CHECK();
+  return TN_OK;
+}
+
+int tn_diag_col_mul_f64(double* mat, const double* diag, int64_t M, int64_t N, void* stream) {
+  TN_INIT_OR_RETURN();
+  if (M == 0 || N == 0) return TN_OK;
+  TN_REQUIRE(mat && diag && M > 0 && N > 0, TN_EINVAL, "tn_diag_col_mul_f64: bad argument");
+  hipLaunchKernelGGL(diag_col_mul_kernel, dim3(grid_for(M * N)), dim3(256), 0,
+                     tn::stream_of(stream), mat, diag, M, N);
+  TN_LAUNCH_CHECK();
+  return TN_OK;
+}
+
+int tn_truncate_f64(const double* S, int64_t k, int64_t chi, double* lam, double* norm_out,
+                    double* err_out, void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(S && lam && norm_out && k >= 1 && chi >= 1, TN_EINVAL, "tn_truncate_f64: bad argument");
+  int64_t dim = std::min(chi, k);
+  hipLaunchKernelGGL(truncate_kernel, dim3(1), dim3(1024), 0, tn::stream_of(stream), S, k, dim, lam,
+                     norm_out, err_out);
+  TN_LAUNCH_CHECK();
+  return TN_OK;
+}
+
+}  // extern "C"
