@@ -119,6 +119,21 @@ typedef struct {
 typedef struct tn_gemm_plan_s* tn_gemm_plan;
 int tn_gemm_plan_create(tn_gemm_plan* plan, int mode, int64_t ngroups,
                         const tn_gemm_desc* descs);
+/* Gather-addressed operands: a group member may address an operand through
+ * offset tables instead of a leading dimension, so a contraction reads its
+ * inputs (and writes its output) in any bond order without permuting them:
+ *   A(m,k) = A[a_off + tab[a_row+m] + tab[a_col+k]]   (a_row >= 0)
+ *          = A[a_off + m*lda + k]                       (a_row <  0)
+ * and likewise B(k,n) with b_row/b_col and C(m,n) with c_row/c_col.
+ * tab holds ntab element offsets (int64), uploaded with the plan. */
+typedef struct {
+  int64_t M, N, K;
+  int64_t a_off, b_off, c_off;
+  int64_t lda, ldb, ldc;
+  int64_t a_row, a_col, b_row, b_col, c_row, c_col;
+} tn_gemm_desc_ex;
+int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups,
+                           const tn_gemm_desc_ex* descs, int64_t ntab, const int64_t* tab);
 int tn_gemm_plan_set_tile(tn_gemm_plan plan, int tile_id); /* autotuner hook  */
 int tn_gemm_plan_info(tn_gemm_plan plan, int64_t* tiles, double* flops, int* tile_id);
 int tn_gemm_exec(tn_gemm_plan plan, const double* A, const double* B, double* C,

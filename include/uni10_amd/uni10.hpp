@@ -1,0 +1,332 @@
+/*
+ * uni10.hpp - Uni10 C++ API (Qnum, Bond, Matrix, UniTensor, Network) over the
+ * MI355X kernel library (tn_hip.h).
+ *
+ * Same class names, signatures and argument meanings as Uni10 v1.0
+ * (uni10/src/uni10/datatype/Qnum.h, data-structure/Bond.h, Block.h,
+ * tensor-network/Matrix.h, UniTensor.h, Network.h) for the real (FP64) path,
+ * so caller code such as the iTEBD example compiles unchanged.  Differences:
+ *   - element buffers always live in device memory (HBM); isOngpu() is true;
+ *     host access (operator[], at, getRawElem, getHostElem) copies back;
+ *   - errors throw std::runtime_error with the reference's message text;
+ *   - a thread-local "current stream" (setStream) selects where work is queued.
+ */
+#ifndef UNI10_AMD_UNI10_HPP
+#define UNI10_AMD_UNI10_HPP
+
+#include <cstddef>
+#include <cstdint>
+#include <iosfwd>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace uni10 {
+
+typedef double Real;
+
+enum bondType { BD_IN = 1, BD_OUT = -1 };
+enum parityType { PRT_EVEN = 0, PRT_ODD = 1 };
+enum parityFType { PRTF_EVEN = 0, PRTF_ODD = 1 };
+
+/* GEMM arithmetic for contractions (storage stays FP64). */
+enum GemmMode { GEMM_F64 = 0, GEMM_F32 = 1, GEMM_BF16X3 = 2, GEMM_BF16X6 = 3 };
+void setGemmMode(GemmMode m);          // thread-local; default GEMM_F64
+GemmMode gemmMode();
+void setStream(void* stream);          // thread-local; NULL = library default
+void* currentStream();
+void synchronize();                    // wait for the current stream
+
+// ---------------------------------------------------------------------------
+// Qnum (Qnum.h:52-170)
+class Qnum {
+ public:
+  static const int U1_UPB = 1000;
+  static const int U1_LOB = -1000;
+  Qnum(int U1 = 0, parityType prt = PRT_EVEN);
+  Qnum(parityFType prtF, int U1 = 0, parityType prt = PRT_EVEN);
+  int U1() const { return m_U1; }
+  parityType prt() const { return m_prt; }
+  parityFType prtF() const { return m_prtF; }
+  void assign(int U1 = 0, parityType prt = PRT_EVEN);
+  void assign(parityFType prtF, int U1 = 0, parityType prt = PRT_EVEN);
+  static bool isFermionic();
+  long int hash() const { return (long int)m_U1 * 4 + m_prt * 2 + m_prtF; }
+  friend bool operator<(const Qnum& a, const Qnum& b) { return a.hash() < b.hash(); }
+  friend bool operator<=(const Qnum& a, const Qnum& b) { return a.hash() <= b.hash(); }
+  friend bool operator==(const Qnum& a, const Qnum& b) {
+    return a.m_U1 == b.m_U1 && a.m_prt == b.m_prt && a.m_prtF == b.m_prtF;
+  }
+  friend bool operator!=(const Qnum& a, const Qnum& b) { return !(a == b); }
+  friend Qnum operator-(const Qnum& q);
+  friend Qnum operator*(const Qnum& a, const Qnum& b);
+  friend std::ostream& operator<<(std::ostream& os, const Qnum& q);
+
+ private:
+  int m_U1;
+  parityType m_prt;
+  parityFType m_prtF;
+};
+
+// ---------------------------------------------------------------------------
+// Bond (Bond.h:51-180)
+class Bond {
+ public:
+  Bond() : m_type(BD_IN), m_dim(0) {}
+  Bond(bondType tp, size_t dim);
+  Bond(bondType tp, const std::vector<Qnum>& qnums);
+  void assign(bondType tp, size_t dim);
+  void assign(bondType tp, const std::vector<Qnum>& qnums);
+  bondType type() const { return m_type; }
+  int dim() const { return m_dim; }
+  std::map<Qnum, int> degeneracy() const;
+  std::vector<Qnum> Qlist() const;
+  Bond& change(bondType tp);
+  Bond& dummy_change(bondType tp);
+  Bond& combine(Bond bd);
+  friend bool operator==(const Bond& a, const Bond& b);
+  friend std::ostream& operator<<(std::ostream& os, const Bond& b);
+  // run-length representation (public read access for planners)
+  const std::vector<Qnum>& qnums() const { return Qnums; }
+  const std::vector<int>& qdegs() const { return Qdegs; }
+  const std::vector<int>& qoffsets() const { return offsets; }
+
+ private:
+  void setting(const std::vector<Qnum>& qnums);
+  bondType m_type;
+  int m_dim;
+  std::vector<Qnum> Qnums;
+  std::vector<int> Qdegs;
+  std::vector<int> offsets;
+  friend class UniTensor;
+};
+Bond combine(bondType tp, const std::vector<Bond>& bds);
+Bond combine(const std::vector<Bond>& bds);
+
+// ---------------------------------------------------------------------------
+// Device buffer (owning, pool-backed)
+class DevBuf {
+ public:
+  DevBuf() = default;
+  explicit DevBuf(size_t n_doubles, bool zero = false);
+  ~DevBuf();
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept;
+  DevBuf& operator=(DevBuf&& o) noexcept;
+  double* get() const { return p_; }
+  size_t size() const { return n_; }
+  void reset(size_t n_doubles, bool zero = false);
+
+ private:
+  double* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Matrix (Matrix.h:47-391): owning dense or diagonal FP64 matrix in HBM.
+class Matrix {
+ public:
+  Matrix();
+  Matrix(size_t Rnum, size_t Cnum, bool diag = false, bool ongpu = true);
+  Matrix(size_t Rnum, size_t Cnum, const double* elem, bool diag = false, bool src_ongpu = false);
+  Matrix(const Matrix& m);
+  Matrix(Matrix&& m) noexcept;
+  Matrix& operator=(const Matrix& m);
+  Matrix& operator=(Matrix&& m) noexcept;
+  ~Matrix() = default;
+
+  size_t row() const { return Rnum; }
+  size_t col() const { return Cnum; }
+  bool isDiag() const { return diag; }
+  bool isOngpu() const { return true; }
+  size_t elemNum() const { return diag ? (Rnum < Cnum ? Rnum : Cnum) : Rnum * Cnum; }
+  double* getElem() const { return buf.get(); }     // device pointer
+  std::vector<double> getHostElem() const;
+  void setElem(const double* elem, bool src_ongpu = false);
+  void setElem(const std::vector<double>& elem, bool src_ongpu = false);
+  double operator[](size_t idx) const;
+  double at(size_t r, size_t c) const;
+  void set_zero();
+  void identity();
+  void randomize(uint64_t seed = 0);                 // SplitMix64 uniform[-1,1)
+  Matrix& resize(size_t row, size_t col);
+  Matrix& transpose();
+  Matrix& cTranspose() { return transpose(); }
+  double norm() const;
+  double sum() const;
+  double trace() const;
+  Matrix& operator*=(double a);
+  Matrix& operator+=(const Matrix& m);
+  std::vector<Matrix> svd() const;
+  std::vector<Matrix> eigh() const;
+  Matrix getDiag() const;
+  friend Matrix operator*(const Matrix& a, const Matrix& b);
+  friend Matrix operator*(const Matrix& a, double s);
+  friend Matrix operator*(double s, const Matrix& a);
+  friend Matrix operator+(const Matrix& a, const Matrix& b);
+  friend bool operator==(const Matrix& a, const Matrix& b);
+  friend std::ostream& operator<<(std::ostream& os, const Matrix& m);
+
+ private:
+  size_t Rnum = 0, Cnum = 0;
+  bool diag = false;
+  DevBuf buf;
+  friend class UniTensor;
+};
+typedef Matrix Block;  // read-only views are returned as copies
+Matrix takeExp(double a, const Matrix& mat);
+Matrix exph(double a, const Matrix& mat);
+Matrix RDotR(const Matrix& a, const Matrix& b);
+
+// ---------------------------------------------------------------------------
+// UniTensor (UniTensor.h:58-971), real FP64.
+struct BlockInfo {
+  size_t Rnum = 0, Cnum = 0, offset = 0;
+};
+struct Layout;                                   // block plan (grouping result)
+typedef std::shared_ptr<const Layout> LayoutPtr;
+
+class UniTensor {
+ public:
+  UniTensor();
+  explicit UniTensor(double val);
+  UniTensor(const std::vector<Bond>& bonds, const std::string& name = "");
+  UniTensor(const std::vector<Bond>& bonds, std::vector<int>& labels, const std::string& name = "");
+  explicit UniTensor(const std::string& fname);   // binary dump (current or older format)
+  UniTensor(const UniTensor& T);
+  UniTensor(UniTensor&& T) noexcept;
+  UniTensor& operator=(const UniTensor& T);
+  UniTensor& operator=(UniTensor&& T) noexcept;
+  ~UniTensor();
+
+  UniTensor& assign(const std::vector<Bond>& bonds);
+  void setName(const std::string& name) { m_name = name; }
+  std::string getName() const { return m_name; }
+  void setLabel(const std::vector<int>& newLabels);
+  void setLabel(int* newLabels);
+  std::vector<int> label() const { return labels; }
+  int label(size_t idx) const { return labels.at(idx); }
+  std::vector<Bond> bond() const { return bonds; }
+  Bond bond(size_t idx) const { return bonds.at(idx); }
+  size_t bondNum() const { return bonds.size(); }
+  size_t inBondNum() const { return RBondNum; }
+  size_t elemNum() const { return m_elemNum; }
+  size_t blockNum() const;
+  std::vector<Qnum> blockQnum() const;
+  Qnum blockQnum(size_t idx) const;
+  int typeID() const { return 1; }
+
+  void setRawElem(const std::vector<double>& raw);
+  void setRawElem(const double* raw);
+  Matrix getRawElem() const;
+  void setElem(const double* elem, bool src_ongpu = false);
+  void setElem(const std::vector<double>& elem, bool src_ongpu = false);
+  double* getElem() const { return buf.get(); }     // device pointer, block layout
+  std::vector<double> getHostElem() const;
+  double at(const std::vector<size_t>& idxs) const;
+  double at(size_t idx) const;
+  double operator[](size_t idx) const { return at(idx); }
+
+  void putBlock(const Matrix& mat);
+  void putBlock(const Qnum& qnum, const Matrix& mat);
+  Matrix getBlock(bool diag = false) const;
+  Matrix getBlock(const Qnum& qnum, bool diag = false) const;
+  std::map<Qnum, Matrix> getBlocks() const;
+
+  UniTensor& permute(const std::vector<int>& newLabels, int inBondNum);
+  UniTensor& permute(int* newLabels, int inBondNum);
+  UniTensor& permute(int inBondNum);
+  UniTensor& transpose();
+  UniTensor& operator*=(double a);
+  double norm() const;
+  UniTensor& normalize();
+  void randomize(uint64_t seed = 0);
+  void set_zero();
+  void save(const std::string& fname) const;
+  std::string printDiagram() const;
+
+  friend UniTensor contract(UniTensor& Ta, UniTensor& Tb, bool fast);
+  friend UniTensor contract_impl(UniTensor& Ta, UniTensor& Tb, bool fast, bool allow_gather);
+  friend UniTensor operator*(const UniTensor& Ta, const UniTensor& Tb);
+  friend UniTensor operator*(const UniTensor& T, double a);
+  friend UniTensor operator*(double a, const UniTensor& T);
+  friend UniTensor otimes(const UniTensor& Ta, const UniTensor& Tb);
+  friend std::ostream& operator<<(std::ostream& os, const UniTensor& T);
+  friend class Network;
+
+  // Block plan access (for planners / bindings)
+  const std::map<Qnum, BlockInfo>& blockInfo() const;
+  LayoutPtr layout() const { return m_layout; }
+  bool hasElem() const { return (status & HAVEELEM) != 0; }
+  bool hasBond() const { return (status & HAVEBOND) != 0; }
+
+  // profiling counters (UniTensor.h:945-948)
+  static std::string profile(bool print = true);
+
+  // Construction without the zero fill, for outputs that are fully written.
+  enum InitFill { Zero, NoZero };
+  UniTensor(const std::vector<Bond>& bonds, const std::string& name, InitFill fill);
+
+ private:
+  void initUniT(bool zero = true);
+  std::string m_name;
+  std::vector<Bond> bonds;
+  std::vector<int> labels;
+  LayoutPtr m_layout;
+  DevBuf buf;
+  size_t m_elemNum = 0;
+  int RBondNum = 0;
+  int status = 0;
+  static const int HAVEBOND = 1;
+  static const int HAVEELEM = 2;
+  static int64_t COUNTER, ELEMNUM, MAXELEMNUM, MAXELEMTEN;
+};
+UniTensor contract(UniTensor& Ta, UniTensor& Tb, bool fast = false);
+UniTensor otimes(const UniTensor& Ta, const UniTensor& Tb);
+
+// ---------------------------------------------------------------------------
+// Network (Network.h:52-230)
+struct NetNode;
+class Network {
+ public:
+  explicit Network(const std::string& fname);
+  Network(const std::string& fname, const std::vector<UniTensor*>& tens);
+  ~Network();
+  Network(const Network&) = delete;
+  Network& operator=(const Network&) = delete;
+  void putTensor(size_t idx, const UniTensor* T, bool force = true);
+  void putTensor(size_t idx, const UniTensor& T, bool force = true);
+  void putTensor(const std::string& name, const UniTensor* T, bool force = true);
+  void putTensor(const std::string& name, const UniTensor& T, bool force = true);
+  void putTensorT(const std::string& nameT, const UniTensor* T, bool force = true);
+  void putTensorT(const std::string& nameT, const UniTensor& T, bool force = true);
+  UniTensor launch(const std::string& name = "");
+  std::string profile(bool print = true);
+  std::string tree() const;     // contraction tree as nested (a b) of tensor names
+  friend std::ostream& operator<<(std::ostream& os, Network& net);
+
+ private:
+  void fromfile(const std::string& fname);
+  void construct();
+  void destruct();
+  void matching(NetNode* sbj, NetNode* tar);
+  void branch(NetNode* sbj, NetNode* tar);
+  UniTensor merge(NetNode* nd);
+  std::vector<std::string> names;
+  std::map<std::string, size_t> name2pos;
+  std::vector<std::vector<int>> label_arr;
+  std::vector<int> Rnums;
+  std::vector<int> order;
+  std::vector<int> brakets;
+  std::vector<UniTensor*> tensors;
+  std::vector<NetNode*> leafs;
+  std::vector<NetNode*> owned;   // internal nodes
+  NetNode* root = nullptr;
+  bool load = false;
+};
+
+}  // namespace uni10
+#endif  // UNI10_AMD_UNI10_HPP

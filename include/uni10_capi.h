@@ -1,0 +1,85 @@
+/*
+ * uni10_capi.h - C facade of the Uni10 C++ API (libuni10_amd.so).
+ *
+ * The binding surface a foreign-language wrapper uses, in the role of the
+ * reference's SWIG module (uni10/Swig/pyUni10.i): opaque handles for
+ * UniTensor and Network, host arrays in and out, status codes instead of
+ * exceptions (u10_last_error() holds the message).  Every call runs the C++
+ * API, which runs the HIP kernels behind tn_hip.h.
+ *
+ * Qnums are passed as int triples {U1, prt, prtF}.
+ */
+#ifndef UNI10_CAPI_H
+#define UNI10_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct u10_tensor_s* u10_tensor;
+typedef struct u10_network_s* u10_network;
+typedef struct u10_itebd_s* u10_itebd;
+
+const char* u10_last_error(void);
+int u10_set_gemm_mode(int mode);          /* GemmMode of uni10.hpp (thread-local) */
+int u10_set_stream(void* stream);         /* thread-local current stream          */
+
+/* UniTensor(bonds, name): bond b has type types[b] (1 = BD_IN, -1 = BD_OUT)
+ * and nruns[b] runs; run r of all bonds, in order, has charge
+ * run_qnums[3*r..3*r+2] and degeneracy run_degs[r]. */
+int u10_tensor_create(int nbonds, const int* types, const int* nruns, const int* run_qnums,
+                      const int* run_degs, const char* name, u10_tensor* out);
+int u10_tensor_scalar(double value, u10_tensor* out);
+int u10_tensor_load(const char* fname, u10_tensor* out);        /* UniTensor(fname) */
+int u10_tensor_save(u10_tensor t, const char* fname);
+int u10_tensor_copy(u10_tensor t, u10_tensor* out);
+int u10_tensor_free(u10_tensor t);
+int u10_tensor_info(u10_tensor t, int* nbonds, int* in_bonds, int64_t* elem_num, int* nblocks);
+int u10_tensor_labels(u10_tensor t, int* labels);
+int u10_tensor_set_label(u10_tensor t, int n, const int* labels);
+int u10_tensor_bond(u10_tensor t, int b, int* type, int* dim, int* nruns);
+int u10_tensor_bond_runs(u10_tensor t, int b, int* run_qnums, int* run_degs);
+int u10_tensor_block(u10_tensor t, int i, int* qnum, int64_t* rows, int64_t* cols, int64_t* offset);
+int u10_tensor_set_elem(u10_tensor t, const double* host);     /* block layout */
+int u10_tensor_get_elem(u10_tensor t, double* host);
+int u10_tensor_set_raw(u10_tensor t, const double* raw);       /* setRawElem */
+int u10_tensor_raw_shape(u10_tensor t, int64_t* rows, int64_t* cols);
+int u10_tensor_get_raw(u10_tensor t, double* raw);             /* getRawElem */
+int u10_tensor_at(u10_tensor t, const int64_t* idxs, double* out);
+int u10_tensor_put_block(u10_tensor t, const int* qnum, int64_t rows, int64_t cols, const double* host);
+int u10_tensor_get_block(u10_tensor t, const int* qnum, double* host);
+int u10_tensor_permute(u10_tensor t, int n, const int* labels, int in_bond_num);
+int u10_tensor_transpose(u10_tensor t);
+int u10_tensor_scale(u10_tensor t, double a);
+int u10_tensor_norm(u10_tensor t, double* out);
+int u10_tensor_device_ptr(u10_tensor t, void** dptr);          /* elements in HBM */
+int u10_contract(u10_tensor a, u10_tensor b, int fast, u10_tensor* out);
+int u10_otimes(u10_tensor a, u10_tensor b, u10_tensor* out);
+
+int u10_network_create(const char* fname, u10_network* out);
+int u10_network_put(u10_network net, const char* name, u10_tensor t, int transposed);
+int u10_network_launch(u10_network net, const char* name, u10_tensor* out);
+int u10_network_tree(u10_network net, char* buf, int len);
+int u10_network_free(u10_network net);
+
+/* Matrix::svd / eigh / exph on host arrays (computed on the device). */
+int u10_svd(const double* A, int64_t M, int64_t N, double* U, double* S, double* VT);
+int u10_eigh(const double* A, int64_t N, double* eig, double* vec_rows);
+int u10_exph(double a, const double* H, int64_t N, double* out);
+
+/* iTEBD of the spin-1/2 XXZ chain through Networks (the example driver). */
+int u10_itebd_create(int chi, int d, double delta, double tau, uint64_t seed, const char* netdir,
+                     u10_itebd* out);
+int u10_itebd_step(u10_itebd t, int nsteps);
+int u10_itebd_energy(u10_itebd t, double* out);
+int u10_itebd_lambda(u10_itebd t, int which, double* out, int* n);
+int u10_itebd_trunc_err(u10_itebd t, double* out);
+int u10_itebd_free(u10_itebd t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UNI10_CAPI_H */

@@ -1,0 +1,87 @@
+"""CPU-side checks of the native boundary (no GPU calls).
+
+* both libraries build and load;
+* every function declared in include/tn_hip.h and include/uni10_capi.h is
+  exported by the matching library and has a ctypes signature in the binding;
+* the host C++ layer links only against the C ABI (libtn_hip.so), not HIP
+  headers of the kernel library internals;
+* calls that need the device fail with an error code (never a CPU fallback)
+  when no GPU is present.
+"""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _decls(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b((?:tn|u10)_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def built():
+    from trim_amd import _build
+    if not (os.path.exists(_build.TN_LIB) and os.path.exists(_build.U10_LIB)):
+        _build.build(verbose=False)
+    return _build
+
+
+def _exports(path):
+    out = subprocess.run(["nm", "-D", "--defined-only", path], stdout=subprocess.PIPE, text=True).stdout
+    return set(l.split()[-1] for l in out.splitlines() if " T " in l)
+
+
+def test_tn_hip_exports_every_declared_symbol(built):
+    decl = _decls("tn_hip.h")
+    assert len(decl) >= 40
+    exp = _exports(built.TN_LIB)
+    missing = [d for d in decl if d not in exp]
+    assert not missing, missing
+    from trim_amd import tn
+    assert sorted(tn.SIGNATURES) == decl
+
+
+def test_uni10_capi_exports_every_declared_symbol(built):
+    decl = _decls("uni10_capi.h")
+    exp = _exports(built.U10_LIB)
+    missing = [d for d in decl if d not in exp]
+    assert not missing, missing
+    from trim_amd import uni10
+    assert sorted(uni10.SIGNATURES) == decl
+
+
+def test_host_layer_depends_only_on_the_c_abi(built):
+    deps = subprocess.run(["ldd", built.U10_LIB], stdout=subprocess.PIPE, text=True).stdout
+    assert "libtn_hip.so" in deps
+    src_dir = os.path.join(ROOT, "trim_amd", "csrc", "uni10")
+    for f in os.listdir(src_dir):
+        text = open(os.path.join(src_dir, f)).read()
+        assert "hip_runtime" not in text and "tn_internal.h" not in text, f
+
+
+def test_libraries_load_and_fail_loudly_without_device(built):
+    from trim_amd import tn
+    L = tn.load()
+    import torch  # noqa: F401  (the image's torch sees no device here)
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    rc = L.tn_init(0)
+    assert rc != 0
+    assert L.tn_last_error().decode()   # a message, not a silent fallback
+    p = ctypes.c_void_p()
+    assert L.tn_malloc(ctypes.byref(p), 64) != 0
+
+
+def test_no_oracle_in_product():
+    """The product never imports or links the oracle."""
+    for dirpath, _, files in os.walk(os.path.join(ROOT, "trim_amd")):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hip", ".h")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "oracle" not in text.lower().replace("oracle-free", ""), os.path.join(dirpath, f)

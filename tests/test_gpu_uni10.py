@@ -1,0 +1,295 @@
+"""GPU parity of the drop-in Uni10 API (C++ over HIP) against the oracle.
+
+Bars: block layout, setRawElem/getRawElem, permute, transpose and file I/O
+are bit-exact; contractions are within 1e-12 relative Frobenius error of the
+oracle's dgemm; network plans (contraction trees) are identical; iTEBD
+energies and Schmidt spectra agree with the oracle's to the stated bounds.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def u10(tn):
+    from trim_amd import uni10
+    uni10.load()
+    uni10.set_gemm_mode(uni10.GEMM_F64)
+    return uni10
+
+
+def to_product(u10, T):
+    """Product UniTensor with the oracle tensor's bonds, labels and elements."""
+    bonds = [u10.Bond(b.type, [q.triple() for q in b.Qlist()]) for b in T.bonds]
+    P = u10.UniTensor(bonds, T.name)
+    P.setLabel(T.labels)
+    P.setElem(T.elem)
+    return P
+
+
+def same_layout(P, T):
+    pb = [(q.triple(), r, c, o) for q, r, c, o in P.blocks()]
+    tb = [(q.triple(), r, c, o) for q, (r, c, o) in T.layout.blocks.items()]
+    return pb == tb
+
+
+HEIS_S1 = np.array([
+    1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, -1, 0, 1, 0, 0, 0, 0,
+    0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0,
+    0, 0, 0, 0, 1, 0, -1, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1], float)
+
+
+def test_grouping_setraw_matches_golden(u10, ora, golden):
+    qs = [(1, 0, 0), (0, 0, 0), (-1, 0, 0)]
+    H = u10.UniTensor([u10.Bond(1, qs), u10.Bond(1, qs), u10.Bond(-1, qs), u10.Bond(-1, qs)], "H_U1")
+    H.setRawElem(HEIS_S1)
+    G = ora.load_tensor(os.path.join(golden, "egU1_H_U1"))
+    assert H.getElem().tobytes() == G.elem.tobytes()
+    assert same_layout(H, G)
+    assert np.array_equal(H.getRawElem().reshape(-1), HEIS_S1)
+    assert H.at([0, 2, 1, 1]) == HEIS_S1.reshape(3, 3, 3, 3)[0, 2, 1, 1]
+
+
+def test_file_loader_and_transpose_golden(u10, ora, golden, tmp_path):
+    W = u10.UniTensor.load(os.path.join(golden, "egU3_W"))
+    WT = ora.load_tensor(os.path.join(golden, "egU3_WT"))
+    Wo = ora.load_tensor(os.path.join(golden, "egU3_W"))
+    assert W.getElem().tobytes() == Wo.elem.tobytes()
+    assert W.label() == [-1, 1, 2]
+    W.transpose()
+    assert W.getElem().tobytes() == WT.elem.tobytes()
+    assert same_layout(W, WT)
+    # save (current format) and read back
+    f = str(tmp_path / "w.bin")
+    W.save(f)
+    R = u10.UniTensor.load(f)
+    assert R.getElem().tobytes() == W.getElem().tobytes() and R.label() == W.label()
+    assert ora.load_tensor(f).elem.tobytes() == W.getElem().tobytes()
+
+
+def _random_u1(ora, seed, n_in=2, n_out=2, maxdeg=4):
+    rng = np.random.default_rng(seed)
+    Q = ora.Qnum
+    for attempt in range(50):
+        bonds = []
+        for i in range(n_in + n_out):
+            tp = ora.BD_IN if i < n_in else ora.BD_OUT
+            charges = sorted(set(int(c) for c in rng.integers(-3, 4, size=int(rng.integers(1, 4)))))
+            qs = []
+            for c in charges:
+                qs += [Q(c)] * int(rng.integers(1, maxdeg + 1))
+            if rng.random() < 0.3:
+                qs += [qs[0]]
+            bonds.append(ora.Bond(tp, qs))
+        try:
+            T = ora.UniTensor(bonds)
+        except ValueError:
+            continue
+        T.set_raw_elem(rng.uniform(-1, 1, size=[b.dim for b in bonds]))
+        return T
+    raise RuntimeError("no valid tensor")
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_symmetric_permute_bitexact(u10, ora, seed):
+    rng = np.random.default_rng(500 + seed)
+    T = _random_u1(ora, seed, n_in=int(rng.integers(1, 3)), n_out=int(rng.integers(1, 3)))
+    nb = len(T.bonds)
+    perm = [int(v) for v in rng.permutation(nb)]
+    rb = int(rng.integers(0, nb + 1))
+    labels = [T.labels[p] for p in perm]
+    _say("sym permute seed=%d nb=%d perm=%s rb=%d" % (seed, nb, perm, rb))
+    P = to_product(u10, T)
+    assert same_layout(P, T)
+    try:
+        To = T.copy().permute(labels, rb)
+    except ValueError:
+        with pytest.raises(u10.Uni10Error):
+            P.permute(labels, rb)
+        return
+    P.permute(labels, rb)
+    assert P.label() == labels and P.inBondNum() == rb
+    assert same_layout(P, To)
+    assert P.getElem().tobytes() == To.elem.tobytes()
+
+
+def test_dense_permute_workload_shape(u10, ora):
+    dims = (256, 2, 2, 256)
+    bonds = [ora.Bond(ora.BD_IN, 256), ora.Bond(ora.BD_IN, 2), ora.Bond(ora.BD_OUT, 2), ora.Bond(ora.BD_OUT, 256)]
+    T = ora.UniTensor(bonds, elem=ora.splitmix64_uniform(9, int(np.prod(dims))))
+    T.set_label([0, 1, 4, 5])
+    P = to_product(u10, T)
+    To = T.copy().permute([5, 1, 0, 4], 2)
+    P.permute([5, 1, 0, 4], 2)
+    assert P.getElem().tobytes() == To.elem.tobytes()
+
+
+def _pair_u1(ora, seed):
+    rng = np.random.default_rng(seed)
+    Q = ora.Qnum
+    def bond(tp, charges):
+        qs = []
+        for c in charges:
+            qs += [Q(c)] * int(rng.integers(1, 6))
+        return ora.Bond(tp, qs)
+    chi_c = [-3, -1, 1, 3]
+    mid = [-4, -2, 0, 2, 4]
+    a_l = bond(ora.BD_IN, chi_c)
+    p = ora.Bond(ora.BD_OUT, [Q(1), Q(-1)])
+    m_out = bond(ora.BD_OUT, mid)
+    A = ora.UniTensor([a_l, p, m_out])
+    A.set_raw_elem(rng.uniform(-1, 1, [b.dim for b in A.bonds]))
+    A.set_label([1, 2, 3])
+    m_in = m_out.copy()
+    m_in.type = ora.BD_IN
+    B = ora.UniTensor([m_in, ora.Bond(ora.BD_OUT, [Q(1), Q(-1)]), bond(ora.BD_OUT, chi_c)])
+    B.set_raw_elem(rng.uniform(-1, 1, [b.dim for b in B.bonds]))
+    B.set_label([3, 4, 5])
+    return A, B
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("fast", [False, True])
+def test_symmetric_contract(u10, ora, seed, fast):
+    A, B = _pair_u1(ora, seed)
+    PA, PB = to_product(u10, A), to_product(u10, B)
+    C = ora.contract(A, B, fast)
+    PC = u10.contract(PA, PB, fast)
+    assert PC.label() == C.labels and same_layout(PC, C)
+    ref = C.elem
+    err = np.linalg.norm(PC.getElem() - ref) / np.linalg.norm(ref)
+    assert err <= 1e-12
+    # inputs: restored when fast is False, left permuted when True (like the reference)
+    assert PA.label() == A.labels and PB.label() == B.labels
+    assert PA.getElem().tobytes() == A.elem.tobytes()
+
+
+def test_contract_dense_workload_full_size(u10, ora):
+    """A(256,2,2,256) x B(256,2,2,256) over two bonds, then permute (3,1,0,2)."""
+    chi = 256
+    mk = lambda: [ora.Bond(ora.BD_IN, chi), ora.Bond(ora.BD_IN, 2), ora.Bond(ora.BD_OUT, 2), ora.Bond(ora.BD_OUT, chi)]
+    n = chi * 4 * chi
+    A = ora.UniTensor(mk(), elem=ora.splitmix64_uniform(1, n))
+    B = ora.UniTensor(mk(), elem=ora.splitmix64_uniform(2, n))
+    A.set_label([0, 1, 2, 3])
+    B.set_label([3, 2, 4, 5])
+    PA, PB = to_product(u10, A), to_product(u10, B)
+    C = ora.contract(A, B)
+    PC = u10.contract(PA, PB)
+    assert PC.label() == [0, 1, 4, 5]
+    err = np.linalg.norm(PC.getElem() - C.elem) / np.linalg.norm(C.elem)
+    assert err <= 1e-12, err
+    # the permute is bit-exact: the oracle's permute of the product's own contraction output
+    Cc = C.copy()
+    Cc.set_elem(PC.getElem())
+    Cc.permute([5, 1, 0, 4], 2)
+    PC.permute([5, 1, 0, 4], 2)
+    assert PC.label() == [5, 1, 0, 4]
+    assert PC.getElem().tobytes() == Cc.elem.tobytes()
+
+
+def test_scalar_contraction_and_otimes(u10, ora):
+    A, B = _pair_u1(ora, 3)
+    PA, PB = to_product(u10, A), to_product(u10, B)
+    # full contraction to a scalar: A with itself
+    A2 = A.copy()
+    P2 = to_product(u10, A)
+    s = ora.contract(A, A2)
+    ps = u10.contract(PA, P2)
+    assert ps.bondNum() == 0
+    assert abs(ps.getElem()[0] - s.elem[0]) <= 1e-12 * abs(s.elem[0])
+    o = u10.otimes(PA, PB)
+    assert o.bondNum() == 6
+
+
+def test_network_simple_and_order(u10, ora, golden):
+    # Simple.net (testNetwork.cpp:17-37)
+    MA = np.random.default_rng(1).uniform(0, 1, (3, 4))
+    MB = np.random.default_rng(2).uniform(0, 1, (4, 3))
+    A = u10.UniTensor([u10.Bond(1, 3), u10.Bond(-1, 2), u10.Bond(-1, 2)])
+    A.putBlock(MA)
+    B = A.copy()
+    B.transpose()
+    B.putBlock(MB)
+    net = u10.Network(os.path.join(golden, "Simple.net"))
+    net.putTensor("A", A)
+    net.putTensor("B", B)
+    C = net.launch()
+    assert np.max(np.abs(C.getBlock() - MA @ MB)) <= 1e-12
+    # egN1_network with ORDER brackets on the golden U(1) tensors
+    net = u10.Network(os.path.join(golden, "egN1_network"))
+    H = u10.UniTensor.load(os.path.join(golden, "egU1_H_U1"))
+    W = u10.UniTensor.load(os.path.join(golden, "egU3_W"))
+    WT = u10.UniTensor.load(os.path.join(golden, "egU3_WT"))
+    for nm, t in (("H", H), ("W", W), ("WT", WT)):
+        net.putTensor(nm, t)
+    R = net.launch()
+    assert net.tree() == "((W H) WT)"
+    onet = ora.Network(os.path.join(golden, "egN1_network"))
+    for nm in ("H", "W", "WT"):
+        onet.put_tensor(nm, ora.load_tensor(os.path.join(golden, "eg%s" % {"H": "U1_H_U1", "W": "U3_W", "WT": "U3_WT"}[nm])))
+    Ro = onet.launch()
+    assert np.max(np.abs(R.getElem() - Ro.elem)) <= 1e-12
+
+
+def _otree(t):
+    return t if isinstance(t, str) else "(%s %s)" % (_otree(t[0]), _otree(t[1]))
+
+
+def test_network_greedy_plan_matches_oracle(u10, ora, golden):
+    chi, d = 6, 2
+    ala, blb, la, lb = ora.itebd_init(chi, d, 3)
+    b3 = [ora.Bond(ora.BD_IN, chi), ora.Bond(ora.BD_IN, chi), ora.Bond(ora.BD_OUT, d)]
+    A = ora.UniTensor(b3, elem=ala.reshape(-1))
+    B = ora.UniTensor(b3, elem=blb.reshape(-1))
+    U = ora.UniTensor([ora.Bond(ora.BD_IN, d)] * 2 + [ora.Bond(ora.BD_OUT, d)] * 2)
+    U.put_block(ora.exph(-0.01, ora.heisenberg_two_site()))
+    onet = ora.Network(os.path.join(golden, "itebd.net"))
+    pnet = u10.Network(os.path.join(golden, "itebd.net"))
+    for nm, t in (("ALa", A), ("BLb", B), ("expH", U)):
+        t.name = nm
+        onet.put_tensor(nm, t)
+        pnet.putTensor(nm, to_product(u10, t))
+    Co = onet.launch()
+    Cp = pnet.launch()
+    assert pnet.tree() == _otree(onet.tree())
+    assert Cp.label() == Co.labels
+    assert np.max(np.abs(Cp.getElem() - Co.elem)) <= 1e-14
+    # a second launch reuses the plan (leaves stay permuted, as in the reference)
+    Cp2 = pnet.launch()
+    assert np.max(np.abs(Cp2.getElem() - Co.elem)) <= 1e-14
+
+
+def test_svd_eigh_exph_api(u10, ora):
+    a = np.random.default_rng(4).uniform(-1, 1, (12, 7))
+    U, S, VT = u10.svd(a)
+    assert np.allclose(U @ np.diag(S) @ VT, a, atol=1e-13)
+    assert np.allclose(S, ora.svd(a)[1], rtol=1e-13, atol=0)
+    H = ora.heisenberg_two_site(0.7)
+    G = u10.exph(-0.05, H)
+    assert np.max(np.abs(G - ora.exph(-0.05, H))) <= 1e-15
+
+
+def test_itebd_matches_oracle_short(u10, ora, golden):
+    """chi=8, 10 steps: energies and Schmidt spectra agree with the oracle."""
+    chi, tau, seed = 8, 0.01, 11
+    P = u10.ITebd(chi, 2, 1.0, tau, seed, golden)
+    O = ora.ITEBD(chi, 2, ora.heisenberg_two_site(1.0), tau, seed, golden)
+    e0p, e0o = P.energy(), O.energy()
+    assert abs(e0p - e0o) <= 1e-12, (e0p, e0o)
+    for k in range(10):
+        P.step()
+        O.step()
+    ep, eo = P.energy(), O.energy()
+    _say("itebd chi=8: product %.15f oracle %.15f" % (ep, eo))
+    assert abs(ep - eo) <= 1e-10
+    assert np.max(np.abs(P.lam(0) - O.La)) <= 1e-10
+    assert np.max(np.abs(P.lam(1) - O.Lb)) <= 1e-10

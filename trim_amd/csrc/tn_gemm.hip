@@ -8,6 +8,13 @@
 // sector land on the same XCD (blockIdx -> XCD is round-robin over 8 XCDs),
 // which keeps their shared A rows / B columns in that XCD's L2.
 //
+// Operand addressing: every element address is (row offset) + (column
+// offset).  Row/column offsets come either from a leading dimension or from
+// gather tables, so a contraction can read tensors in their stored bond order
+// (and write its result in any order) with no separate permute pass.  Each
+// thread keeps the offsets of its fixed rows/columns in registers and fetches
+// the other index's offset once per K step.
+//
 // Arithmetic modes (storage is always FP64, row-major):
 //   F64     v_mfma_f64_16x16x4f64, FP64 accumulate            (plain drop-in)
 //   F32     v_mfma_f32_16x16x4f32 on inputs rounded to FP32, FP32 accumulate
@@ -23,7 +30,8 @@
 //     accumulator element i of lane l is C[(l/16)+4i][l%16] for f64 and
 //     C[4*(l/16)+i][l%16] for f32 (measured with a one-instruction probe).
 //   16x16x32 bf16: lane l supplies A[l%16][8*(l/16)+i] and B[8*(l/16)+i][l%16]
-//     for i = 0..7; same accumulator layout.
+//     for i = 0..7 (any k order works as long as A and B use the same one);
+//     accumulator layout as f32.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -48,20 +56,34 @@ struct GroupDev {
   int64_t M, N, K;
   int64_t a_off, b_off, c_off;
   int64_t lda, ldb, ldc;
+  int64_t a_row, a_col, b_row, b_col, c_row, c_col;   // table offsets, -1 = strided
 };
 
+// Offsets of the operand index i along a "row" or "column" of an operand.
+__device__ __forceinline__ int64_t row_off(const int64_t* __restrict__ tab, int64_t t, int64_t i, int64_t ld) {
+  return (t >= 0) ? tab[t + i] : i * ld;
+}
+__device__ __forceinline__ int64_t col_off(const int64_t* __restrict__ tab, int64_t t, int64_t i) {
+  return (t >= 0) ? tab[t + i] : i;
+}
+
 // ----------------------------------------------------------------------------
-// FP64 / FP32 kernel: BM x BN block tile, BK = 16, 256 threads = 2x2 waves.
-template <typename T, int BM, int BN>
+// FP64 / FP32 kernel: BM x BN block tile, BK deep, 256 threads = 2x2 waves.
+template <typename T, int BM, int BN, int BK>
 __global__ void __launch_bounds__(256) gemm_mfma_16x16x4(const double* __restrict__ A,
                                                          const double* __restrict__ B,
                                                          double* __restrict__ C,
                                                          const GroupDev* __restrict__ groups,
-                                                         const TileDev* __restrict__ tiles) {
-  constexpr int BK = 16;
+                                                         const TileDev* __restrict__ tiles,
+                                                         const int64_t* __restrict__ tab) {
+  static_assert(256 % BK == 0 && 256 % BN == 0, "tile shape");
   constexpr int WM = BM / 2, WN = BN / 2;          // per-wave tile
   constexpr int TM = WM / 16, TN = WN / 16;        // MFMA tiles per wave
   constexpr int PAD = (sizeof(T) == 8) ? 1 : 2;
+  constexpr int A_PER = BM * BK / 256;             // A elements per thread per K step
+  constexpr int B_PER = BN * BK / 256;
+  constexpr int A_RSTEP = 256 / BK;                // A rows covered per pass
+  constexpr int B_KSTEP = 256 / BN;                // B k-rows covered per pass
   __shared__ T As[BK][BM + PAD];
   __shared__ T Bs[BK][BN + PAD];
 
@@ -78,6 +100,19 @@ __global__ void __launch_bounds__(256) gemm_mfma_16x16x4(const double* __restric
   const int wave = tid >> 6;
   const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
 
+  // fixed per-thread offsets
+  const int a_kk = tid % BK;
+  const int a_mm0 = tid / BK;
+  int64_t a_roff[A_PER];
+#pragma unroll
+  for (int i = 0; i < A_PER; ++i) {
+    const int64_t m = m0 + a_mm0 + i * A_RSTEP;
+    a_roff[i] = (m < g.M) ? row_off(tab, g.a_row, m, g.lda) : -1;
+  }
+  const int b_nn = tid % BN;
+  const int b_kk0 = tid / BN;
+  const int64_t b_coff = (n0 + b_nn < g.N) ? col_off(tab, g.b_col, n0 + b_nn) : -1;
+
   typedef typename std::conditional<sizeof(T) == 8, d4, f4>::type acc_t;
   acc_t acc[TM][TN];
 #pragma unroll
@@ -85,39 +120,25 @@ __global__ void __launch_bounds__(256) gemm_mfma_16x16x4(const double* __restric
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  constexpr int A_PER = BM * BK / 256;  // elements per thread
-  constexpr int B_PER = BN * BK / 256;
   T ra[A_PER], rb[B_PER];
-
   auto load_tiles = [&](int64_t k0) {
+    const int64_t ka = k0 + a_kk;
+    const int64_t a_coff = (ka < g.K) ? col_off(tab, g.a_col, ka) : -1;
 #pragma unroll
-    for (int i = 0; i < A_PER; ++i) {
-      int e = tid + i * 256;
-      int kk = e % BK, mm = e / BK;
-      int64_t gm = m0 + mm, gk = k0 + kk;
-      double v = (gm < g.M && gk < g.K) ? Ab[gm * g.lda + gk] : 0.0;
-      ra[i] = (T)v;
-    }
+    for (int i = 0; i < A_PER; ++i)
+      ra[i] = (a_roff[i] >= 0 && a_coff >= 0) ? (T)Ab[a_roff[i] + a_coff] : (T)0;
 #pragma unroll
     for (int i = 0; i < B_PER; ++i) {
-      int e = tid + i * 256;
-      int nn = e % BN, kk = e / BN;
-      int64_t gk = k0 + kk, gn = n0 + nn;
-      double v = (gk < g.K && gn < g.N) ? Bb[gk * g.ldb + gn] : 0.0;
-      rb[i] = (T)v;
+      const int64_t kb = k0 + b_kk0 + i * B_KSTEP;
+      const int64_t b_roff = (kb < g.K) ? row_off(tab, g.b_row, kb, g.ldb) : -1;
+      rb[i] = (b_roff >= 0 && b_coff >= 0) ? (T)Bb[b_roff + b_coff] : (T)0;
     }
   };
   auto store_tiles = [&]() {
 #pragma unroll
-    for (int i = 0; i < A_PER; ++i) {
-      int e = tid + i * 256;
-      As[e % BK][e / BK] = ra[i];
-    }
+    for (int i = 0; i < A_PER; ++i) As[a_kk][a_mm0 + i * A_RSTEP] = ra[i];
 #pragma unroll
-    for (int i = 0; i < B_PER; ++i) {
-      int e = tid + i * 256;
-      Bs[e / BN][e % BN] = rb[i];
-    }
+    for (int i = 0; i < B_PER; ++i) Bs[b_kk0 + i * B_KSTEP][b_nn] = rb[i];
   };
 
   const int64_t nk = (g.K + BK - 1) / BK;
@@ -126,7 +147,7 @@ __global__ void __launch_bounds__(256) gemm_mfma_16x16x4(const double* __restric
     __syncthreads();
     store_tiles();
     __syncthreads();
-    if (kt + 1 < nk) load_tiles((kt + 1) * BK);  // overlap next global load with MFMAs
+    if (kt + 1 < nk) load_tiles((kt + 1) * BK);  // next global load overlaps the MFMAs
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
       T a[TM], b[TN];
@@ -148,18 +169,20 @@ __global__ void __launch_bounds__(256) gemm_mfma_16x16x4(const double* __restric
   }
   // write back
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+  for (int j = 0; j < TN; ++j) {
+    const int64_t gn = n0 + wn0 + j * 16 + (lane & 15);
+    if (gn >= g.N) continue;
+    const int64_t coff = col_off(tab, g.c_col, gn);
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      const int64_t gn = n0 + wn0 + j * 16 + (lane & 15);
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         // accumulator row: f64 16x16x4 -> (l/16) + 4v ; f32 16x16x4 -> 4(l/16) + v
         const int row = (sizeof(T) == 8) ? ((lane >> 4) + 4 * v) : (4 * (lane >> 4) + v);
         const int64_t gm = m0 + wm0 + i * 16 + row;
-        if (gm < g.M && gn < g.N) Cb[gm * g.ldc + gn] = (double)acc[i][j][v];
+        if (gm < g.M) Cb[row_off(tab, g.c_row, gm, g.ldc) + coff] = (double)acc[i][j][v];
       }
-    }
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -177,11 +200,16 @@ __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict_
                                                        const double* __restrict__ B,
                                                        double* __restrict__ C,
                                                        const GroupDev* __restrict__ groups,
-                                                       const TileDev* __restrict__ tiles) {
+                                                       const TileDev* __restrict__ tiles,
+                                                       const int64_t* __restrict__ tab) {
   constexpr int BK = 32;
   constexpr int KP = BK + 8;                       // padded row (bf16 elements)
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int A_PER = BM * BK / 256;
+  constexpr int B_PER = BN * BK / 256;
+  constexpr int A_RSTEP = 256 / BK;
+  constexpr int B_KSTEP = 256 / BN;
   __shared__ __attribute__((aligned(16))) __bf16 As[NP][BM][KP];   // [piece][m][k]
   __shared__ __attribute__((aligned(16))) __bf16 Bs[NP][BN][KP];   // [piece][n][k]
 
@@ -198,50 +226,56 @@ __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict_
   const int wave = tid >> 6;
   const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
 
+  const int a_kk = tid % BK;
+  const int a_mm0 = tid / BK;
+  int64_t a_roff[A_PER];
+#pragma unroll
+  for (int i = 0; i < A_PER; ++i) {
+    const int64_t m = m0 + a_mm0 + i * A_RSTEP;
+    a_roff[i] = (m < g.M) ? row_off(tab, g.a_row, m, g.lda) : -1;
+  }
+  const int b_nn = tid % BN;
+  const int b_kk0 = tid / BN;
+  const int64_t b_coff = (n0 + b_nn < g.N) ? col_off(tab, g.b_col, n0 + b_nn) : -1;
+
   f4 acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = f4{0, 0, 0, 0};
 
-  constexpr int A_PER = BM * BK / 256;
-  constexpr int B_PER = BN * BK / 256;
   float ra[A_PER], rb[B_PER];
-
   auto load_tiles = [&](int64_t k0) {
+    const int64_t ka = k0 + a_kk;
+    const int64_t a_coff = (ka < g.K) ? col_off(tab, g.a_col, ka) : -1;
 #pragma unroll
-    for (int i = 0; i < A_PER; ++i) {
-      int e = tid + i * 256;
-      int kk = e % BK, mm = e / BK;
-      int64_t gm = m0 + mm, gk = k0 + kk;
-      ra[i] = (gm < g.M && gk < g.K) ? (float)Ab[gm * g.lda + gk] : 0.0f;
-    }
+    for (int i = 0; i < A_PER; ++i)
+      ra[i] = (a_roff[i] >= 0 && a_coff >= 0) ? (float)Ab[a_roff[i] + a_coff] : 0.0f;
 #pragma unroll
     for (int i = 0; i < B_PER; ++i) {
-      int e = tid + i * 256;
-      int nn = e % BN, kk = e / BN;
-      int64_t gk = k0 + kk, gn = n0 + nn;
-      rb[i] = (gk < g.K && gn < g.N) ? (float)Bb[gk * g.ldb + gn] : 0.0f;
+      const int64_t kb = k0 + b_kk0 + i * B_KSTEP;
+      const int64_t b_roff = (kb < g.K) ? row_off(tab, g.b_row, kb, g.ldb) : -1;
+      rb[i] = (b_roff >= 0 && b_coff >= 0) ? (float)Bb[b_roff + b_coff] : 0.0f;
     }
   };
   auto store_tiles = [&]() {
 #pragma unroll
     for (int i = 0; i < A_PER; ++i) {
-      int e = tid + i * 256;
       __bf16 h, m, l;
       split3(ra[i], h, m, l);
-      As[0][e / BK][e % BK] = h;
-      As[1][e / BK][e % BK] = m;
-      if constexpr (NP > 2) As[2][e / BK][e % BK] = l;
+      const int mm = a_mm0 + i * A_RSTEP;
+      As[0][mm][a_kk] = h;
+      As[1][mm][a_kk] = m;
+      if constexpr (NP > 2) As[2][mm][a_kk] = l;
     }
 #pragma unroll
     for (int i = 0; i < B_PER; ++i) {
-      int e = tid + i * 256;
       __bf16 h, m, l;
       split3(rb[i], h, m, l);
-      Bs[0][e % BN][e / BN] = h;
-      Bs[1][e % BN][e / BN] = m;
-      if constexpr (NP > 2) Bs[2][e % BN][e / BN] = l;
+      const int kk = b_kk0 + i * B_KSTEP;
+      Bs[0][b_nn][kk] = h;
+      Bs[1][b_nn][kk] = m;
+      if constexpr (NP > 2) Bs[2][b_nn][kk] = l;
     }
   };
 
@@ -279,16 +313,18 @@ __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict_
       }
   }
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+  for (int j = 0; j < TN; ++j) {
+    const int64_t gn = n0 + wn0 + j * 16 + (lane & 15);
+    if (gn >= g.N) continue;
+    const int64_t coff = col_off(tab, g.c_col, gn);
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      const int64_t gn = n0 + wn0 + j * 16 + (lane & 15);
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int64_t gm = m0 + wm0 + i * 16 + 4 * (lane >> 4) + v;
-        if (gm < g.M && gn < g.N) Cb[gm * g.ldc + gn] = (double)acc[i][j][v];
+        if (gm < g.M) Cb[row_off(tab, g.c_row, gm, g.ldc) + coff] = (double)acc[i][j][v];
       }
-    }
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -299,33 +335,49 @@ struct TileCfg {
 constexpr TileCfg kCfgs[] = {{32, 32}, {64, 64}, {128, 64}, {64, 128}, {128, 128}};
 constexpr int kNumCfg = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
-typedef void (*KernFn)(const double*, const double*, double*, const GroupDev*, const TileDev*);
+typedef void (*KernFn)(const double*, const double*, double*, const GroupDev*, const TileDev*,
+                       const int64_t*);
 
 KernFn pick_kernel(int mode, int cfg) {
-#define TN_CFG_CASES(MAKE)                       \
-  switch (cfg) {                                 \
-    case 0: return MAKE(32, 32);                 \
-    case 1: return MAKE(64, 64);                 \
-    case 2: return MAKE(128, 64);                \
-    case 3: return MAKE(64, 128);                \
-    case 4: return MAKE(128, 128);               \
-  }
-#define MK_F64(a, b) gemm_mfma_16x16x4<double, a, b>
-#define MK_F32(a, b) gemm_mfma_16x16x4<float, a, b>
-#define MK_B3(a, b) gemm_split_bf16<a, b, 2>
-#define MK_B6(a, b) gemm_split_bf16<a, b, 3>
   switch (mode) {
-    case TN_GEMM_F64: TN_CFG_CASES(MK_F64); break;
-    case TN_GEMM_F32: TN_CFG_CASES(MK_F32); break;
-    case TN_GEMM_BF16X3: TN_CFG_CASES(MK_B3); break;
-    case TN_GEMM_BF16X6: TN_CFG_CASES(MK_B6); break;
+    case TN_GEMM_F64:
+      switch (cfg) {
+        case 0: return gemm_mfma_16x16x4<double, 32, 32, 64>;
+        case 1: return gemm_mfma_16x16x4<double, 64, 64, 32>;
+        case 2: return gemm_mfma_16x16x4<double, 128, 64, 32>;
+        case 3: return gemm_mfma_16x16x4<double, 64, 128, 32>;
+        case 4: return gemm_mfma_16x16x4<double, 128, 128, 16>;
+      }
+      break;
+    case TN_GEMM_F32:
+      switch (cfg) {
+        case 0: return gemm_mfma_16x16x4<float, 32, 32, 64>;
+        case 1: return gemm_mfma_16x16x4<float, 64, 64, 32>;
+        case 2: return gemm_mfma_16x16x4<float, 128, 64, 32>;
+        case 3: return gemm_mfma_16x16x4<float, 64, 128, 32>;
+        case 4: return gemm_mfma_16x16x4<float, 128, 128, 16>;
+      }
+      break;
+    case TN_GEMM_BF16X3:
+      switch (cfg) {
+        case 0: return gemm_split_bf16<32, 32, 2>;
+        case 1: return gemm_split_bf16<64, 64, 2>;
+        case 2: return gemm_split_bf16<128, 64, 2>;
+        case 3: return gemm_split_bf16<64, 128, 2>;
+        case 4: return gemm_split_bf16<128, 128, 2>;
+      }
+      break;
+    case TN_GEMM_BF16X6:
+      switch (cfg) {
+        case 0: return gemm_split_bf16<32, 32, 3>;
+        case 1: return gemm_split_bf16<64, 64, 3>;
+        case 2: return gemm_split_bf16<128, 64, 3>;
+        case 3: return gemm_split_bf16<64, 128, 3>;
+        case 4: return gemm_split_bf16<128, 128, 3>;
+      }
+      break;
   }
   return nullptr;
-#undef MK_F64
-#undef MK_F32
-#undef MK_B3
-#undef MK_B6
-#undef TN_CFG_CASES
 }
 
 }  // namespace
@@ -336,6 +388,7 @@ struct tn_gemm_plan_s {
   std::vector<GroupDev> groups;
   GroupDev* d_groups = nullptr;
   TileDev* d_tiles = nullptr;
+  int64_t* d_tab = nullptr;
   int64_t ntiles = 0;     // real tiles
   int64_t nslots = 0;     // launched blocks (with XCD padding)
   double flops = 0;
@@ -346,7 +399,7 @@ static int build_tiles(tn_gemm_plan_s* P, int cfg) {
   std::vector<TileDev> nat;
   for (size_t gi = 0; gi < P->groups.size(); ++gi) {
     const GroupDev& g = P->groups[gi];
-    if (g.M == 0 || g.N == 0) continue;
+    if (g.M == 0 || g.N == 0 || g.K == 0) continue;
     int64_t tm = (g.M + bm - 1) / bm, tn_ = (g.N + bn - 1) / bn;
     TN_REQUIRE(tm < (1 << 30) && tn_ < (1 << 30), TN_EUNSUPP, "GEMM too large");
     for (int64_t b = 0; b < tn_; ++b)
@@ -359,10 +412,7 @@ static int build_tiles(tn_gemm_plan_s* P, int cfg) {
   const int64_t per = (n + kX - 1) / kX;
   const int64_t slots = per * kX;
   std::vector<TileDev> lay(slots, TileDev{-1, 0, 0, 0});
-  for (int64_t i = 0; i < n; ++i) {
-    int64_t x = i / std::max<int64_t>(per, 1), s = i % std::max<int64_t>(per, 1);
-    lay[s * kX + x] = nat[i];
-  }
+  for (int64_t i = 0; i < n; ++i) lay[(i % per) * kX + i / per] = nat[i];
   TN_REQUIRE(slots < (int64_t(1) << 31), TN_EUNSUPP, "too many GEMM tiles");
   if (P->d_tiles) tn::pool_free(P->d_tiles);
   P->d_tiles = nullptr;
@@ -378,8 +428,8 @@ static int build_tiles(tn_gemm_plan_s* P, int cfg) {
   return TN_OK;
 }
 
-// Default tile choice: the largest tile that still gives every CU at least
-// one tile; otherwise the smallest tile.
+// Default tile: the largest tile that still gives every CU at least one
+// tile; otherwise the smallest tile.
 static int default_cfg(const tn_gemm_plan_s* P) {
   const int order[] = {4, 2, 1, 0};
   const int64_t want = tn::cu_count();
@@ -392,26 +442,41 @@ static int default_cfg(const tn_gemm_plan_s* P) {
   return 0;
 }
 
+static void destroy_plan(tn_gemm_plan_s* P) {
+  tn::pool_free(P->d_groups);
+  tn::pool_free(P->d_tiles);
+  tn::pool_free(P->d_tab);
+  delete P;
+}
+
 extern "C" {
 
-int tn_gemm_plan_create(tn_gemm_plan* plan, int mode, int64_t ngroups, const tn_gemm_desc* d) {
+int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups, const tn_gemm_desc_ex* d,
+                           int64_t ntab, const int64_t* tab) {
   TN_INIT_OR_RETURN();
   TN_REQUIRE(plan != nullptr, TN_EINVAL, "tn_gemm_plan_create: NULL out pointer");
-  TN_REQUIRE(mode >= TN_GEMM_F64 && mode <= TN_GEMM_BF16X6, TN_EINVAL, "unknown GEMM mode %d",
-             mode);
+  TN_REQUIRE(mode >= TN_GEMM_F64 && mode <= TN_GEMM_BF16X6, TN_EINVAL, "unknown GEMM mode %d", mode);
   TN_REQUIRE(ngroups >= 0 && (ngroups == 0 || d), TN_EINVAL, "tn_gemm_plan_create: NULL descs");
+  TN_REQUIRE(ntab >= 0 && (ntab == 0 || tab), TN_EINVAL, "tn_gemm_plan_create: NULL table");
   auto* P = new tn_gemm_plan_s();
   P->mode = mode;
   for (int64_t i = 0; i < ngroups; ++i) {
-    const tn_gemm_desc& x = d[i];
-    if (x.M < 0 || x.N < 0 || x.K < 0 || (x.M > 0 && x.K > 0 && x.lda < x.K) ||
-        (x.K > 0 && x.N > 0 && x.ldb < x.N) || (x.M > 0 && x.N > 0 && x.ldc < x.N) ||
-        x.a_off < 0 || x.b_off < 0 || x.c_off < 0) {
+    const tn_gemm_desc_ex& x = d[i];
+    auto tab_ok = [&](int64_t t, int64_t n) { return t < 0 || (t + n <= ntab); };
+    bool pair_ok = ((x.a_row < 0) == (x.a_col < 0)) && ((x.b_row < 0) == (x.b_col < 0)) &&
+                   ((x.c_row < 0) == (x.c_col < 0));
+    bool ld_ok = (x.a_row >= 0 || x.M == 0 || x.K == 0 || x.lda >= x.K) &&
+                 (x.b_row >= 0 || x.K == 0 || x.N == 0 || x.ldb >= x.N) &&
+                 (x.c_row >= 0 || x.M == 0 || x.N == 0 || x.ldc >= x.N);
+    if (x.M < 0 || x.N < 0 || x.K < 0 || x.a_off < 0 || x.b_off < 0 || x.c_off < 0 || !pair_ok || !ld_ok ||
+        !tab_ok(x.a_row, x.M) || !tab_ok(x.a_col, x.K) || !tab_ok(x.b_row, x.K) || !tab_ok(x.b_col, x.N) ||
+        !tab_ok(x.c_row, x.M) || !tab_ok(x.c_col, x.N)) {
       delete P;
-      tn::set_error("GEMM group %lld has an invalid shape or leading dimension", (long long)i);
+      tn::set_error("GEMM group %lld has an invalid shape, leading dimension or table range", (long long)i);
       return TN_EINVAL;
     }
-    P->groups.push_back(GroupDev{x.M, x.N, x.K, x.a_off, x.b_off, x.c_off, x.lda, x.ldb, x.ldc});
+    P->groups.push_back(GroupDev{x.M, x.N, x.K, x.a_off, x.b_off, x.c_off, x.lda, x.ldb, x.ldc, x.a_row, x.a_col,
+                                 x.b_row, x.b_col, x.c_row, x.c_col});
     P->flops += 2.0 * (double)x.M * (double)x.N * (double)x.K;
   }
   int rc = TN_OK;
@@ -420,21 +485,32 @@ int tn_gemm_plan_create(tn_gemm_plan* plan, int mode, int64_t ngroups, const tn_
     if (rc == TN_OK)
       rc = tn_memcpy_h2d(P->d_groups, P->groups.data(), sizeof(GroupDev) * P->groups.size(), nullptr);
   }
+  if (rc == TN_OK && ntab > 0) {
+    rc = tn::pool_alloc((void**)&P->d_tab, sizeof(int64_t) * ntab);
+    if (rc == TN_OK) rc = tn_memcpy_h2d(P->d_tab, tab, sizeof(int64_t) * ntab, nullptr);
+  }
   if (rc == TN_OK) rc = build_tiles(P, default_cfg(P));
   if (rc != TN_OK) {
-    tn::pool_free(P->d_groups);
-    tn::pool_free(P->d_tiles);
-    delete P;
+    destroy_plan(P);
     return rc;
   }
   *plan = P;
   return TN_OK;
 }
 
+int tn_gemm_plan_create(tn_gemm_plan* plan, int mode, int64_t ngroups, const tn_gemm_desc* d) {
+  TN_REQUIRE(ngroups >= 0 && (ngroups == 0 || d), TN_EINVAL, "tn_gemm_plan_create: NULL descs");
+  std::vector<tn_gemm_desc_ex> ex(ngroups);
+  for (int64_t i = 0; i < ngroups; ++i) {
+    const tn_gemm_desc& x = d[i];
+    ex[i] = tn_gemm_desc_ex{x.M, x.N, x.K, x.a_off, x.b_off, x.c_off, x.lda, x.ldb, x.ldc, -1, -1, -1, -1, -1, -1};
+  }
+  return tn_gemm_plan_create_ex(plan, mode, ngroups, ex.data(), 0, nullptr);
+}
+
 int tn_gemm_plan_set_tile(tn_gemm_plan P, int tile_id) {
   TN_REQUIRE(P != nullptr, TN_EINVAL, "tn_gemm_plan_set_tile: NULL plan");
-  TN_REQUIRE(tile_id >= 0 && tile_id < kNumCfg, TN_EINVAL, "tile id %d outside [0,%d)", tile_id,
-             kNumCfg);
+  TN_REQUIRE(tile_id >= 0 && tile_id < kNumCfg, TN_EINVAL, "tile id %d outside [0,%d)", tile_id, kNumCfg);
   if (tile_id == P->cfg) return TN_OK;
   return build_tiles(P, tile_id);
 }
@@ -447,44 +523,51 @@ int tn_gemm_plan_info(tn_gemm_plan P, int64_t* tiles, double* flops, int* tile_i
   return TN_OK;
 }
 
+// Fill kernel for sectors with K == 0 (dgemm with beta = 0 writes zeros).
+__global__ void zero_c_kernel(double* __restrict__ C, GroupDev g, const int64_t* __restrict__ tab) {
+  const int64_t total = g.M * g.N;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t m = e / g.N, n = e % g.N;
+    C[g.c_off + row_off(tab, g.c_row, m, g.ldc) + col_off(tab, g.c_col, n)] = 0.0;
+  }
+}
+
 int tn_gemm_exec(tn_gemm_plan P, const double* A, const double* B, double* C, void* stream) {
   TN_INIT_OR_RETURN();
   TN_REQUIRE(P != nullptr, TN_EINVAL, "tn_gemm_exec: NULL plan");
   hipStream_t s = tn::stream_of(stream);
-  // Sectors with K == 0 produce zeros (dgemm with beta = 0).
   for (const auto& g : P->groups)
     if (g.K == 0 && g.M > 0 && g.N > 0) {
-      for (int64_t r = 0; r < g.M; ++r) {
-        int rc = tn_zero(C + g.c_off + r * g.ldc, sizeof(double) * g.N, stream);
-        if (rc != TN_OK) return rc;
-      }
+      TN_REQUIRE(C != nullptr, TN_EINVAL, "tn_gemm_exec: NULL C");
+      int64_t blocks = std::min<int64_t>((g.M * g.N + 255) / 256, 1024);
+      hipLaunchKernelGGL(zero_c_kernel, dim3((unsigned)blocks), dim3(256), 0, s, C, g, P->d_tab);
+      TN_LAUNCH_CHECK();
     }
   if (P->nslots == 0) return TN_OK;
   TN_REQUIRE(A && B && C, TN_EINVAL, "tn_gemm_exec: NULL operand");
   KernFn k = pick_kernel(P->mode, P->cfg);
   TN_REQUIRE(k != nullptr, TN_EINVAL, "no kernel for mode %d tile %d", P->mode, P->cfg);
-  hipLaunchKernelGGL(k, dim3((unsigned)P->nslots), dim3(256), 0, s, A, B, C, P->d_groups,
-                     P->d_tiles);
+  hipLaunchKernelGGL(k, dim3((unsigned)P->nslots), dim3(256), 0, s, A, B, C, P->d_groups, P->d_tiles, P->d_tab);
   TN_LAUNCH_CHECK();
   return TN_OK;
 }
 
 int tn_gemm_plan_destroy(tn_gemm_plan P) {
   if (!P) return TN_OK;
-  tn::pool_free(P->d_groups);
-  tn::pool_free(P->d_tiles);
-  delete P;
+  // the plan's tables may still be read by queued launches
+  hipError_t e = hipDeviceSynchronize();
+  destroy_plan(P);
+  TN_REQUIRE(e == hipSuccess, TN_EHIP, "hipDeviceSynchronize: %s", hipGetErrorString(e));
   return TN_OK;
 }
 
-int tn_matmul_f64(const double* A, const double* B, int64_t M, int64_t N, int64_t K, double* C,
-                  int mode, void* stream) {
-  tn_gemm_desc d{M, N, K, 0, 0, 0, K, N, N};
+int tn_matmul_f64(const double* A, const double* B, int64_t M, int64_t N, int64_t K, double* C, int mode,
+                  void* stream) {
+  tn_gemm_desc d{M, N, K, 0, 0, 0, std::max<int64_t>(K, 1), std::max<int64_t>(N, 1), std::max<int64_t>(N, 1)};
   tn_gemm_plan P = nullptr;
   int rc = tn_gemm_plan_create(&P, mode, 1, &d);
   if (rc != TN_OK) return rc;
   rc = tn_gemm_exec(P, A, B, C, stream);
-  // the plan's device tables are read by the launch: keep them alive until it ends
   if (rc == TN_OK) {
     hipError_t e = hipStreamSynchronize(tn::stream_of(stream));
     if (e != hipSuccess) {

@@ -100,6 +100,7 @@ struct TileParams {
   int64_t s_inner;              // product of full inner S axes (source contiguous run unit)
   int64_t d_inner;              // product of full inner D axes
   int nS, nD;                   // tile extents (elements) in S and D
+  int ws, wd;                   // lanes along S (load phase) / along D (store phase)
   // outer axes
   int nouter;
   int64_t o_dim[kMaxRank];
@@ -172,16 +173,18 @@ __global__ void __launch_bounds__(256) tiled_permute_kernel(const double* __rest
   double* dbase = dst + base_dst_sh;
   // load: lanes run along s (contiguous in source)
   {
-    const int s = tid % kTileS;
-    for (int d = tid / kTileS; d < dv; d += 256 / kTileS) {
+    const int ws = p.ws;
+    const int s = tid % ws;
+    for (int d = tid / ws; d < dv; d += 256 / ws) {
       if (s < sv) tile[d][s] = sbase[srow_src[d] + s];
     }
   }
   __syncthreads();
   // store: lanes run along d (contiguous in destination)
   {
-    const int d = tid % kTileD;
-    for (int s = tid / kTileD; s < sv; s += 256 / kTileD) {
+    const int wd = p.wd;
+    const int d = tid % wd;
+    for (int s = tid / wd; s < sv; s += 256 / wd) {
       if (d < dv) dbase[scol_dst[s] + d] = tile[d][s];
     }
   }
@@ -196,6 +199,116 @@ struct Plan {
   TileParams tile{};
   int64_t grid = 0;
 };
+
+// Tile of a tiled transpose for a target run length tgt (64, 32 or 16).
+static int build_tile(const std::vector<int64_t>& dims, const std::vector<int>& perm,
+                      const std::vector<int64_t>& sstr, const std::vector<int64_t>& dstr, int tgt,
+                      TileParams& T, int64_t* grid_out) {
+  const int r = (int)dims.size();
+  std::memset(&T, 0, sizeof(T));
+  std::vector<int> in_s(r, 0), in_d(r, 0);
+  // S group: trailing source axes (innermost first), stop before the
+  // destination-inner axis.
+  const int dinner = perm[r - 1];
+  int64_t prod = 1;
+  int s_chunk_axis = -1;
+  for (int i = r - 1; i >= 0; --i) {
+    if (i == dinner) break;
+    if (prod * dims[i] >= tgt) {
+      s_chunk_axis = i;
+      break;
+    }
+    in_s[i] = 1;
+    T.s_dim[T.ns_axes] = dims[i];
+    T.s_full[T.ns_axes] = dims[i];
+    T.s_dst[T.ns_axes] = dstr[i];
+    T.ns_axes++;
+    prod *= dims[i];
+  }
+  T.s_inner = prod;
+  if (s_chunk_axis >= 0) {
+    int64_t len = std::max<int64_t>(1, tgt / prod);
+    len = std::min(len, dims[s_chunk_axis]);
+    in_s[s_chunk_axis] = 2;
+    T.s_chunk_src = sstr[s_chunk_axis];
+    T.s_chunk_dst = dstr[s_chunk_axis];
+    T.s_chunk_full = dims[s_chunk_axis];
+    T.s_chunk_len = len;
+    T.s_nchunks = (dims[s_chunk_axis] + len - 1) / len;
+    T.s_dim[T.ns_axes] = len;
+    T.s_full[T.ns_axes] = dims[s_chunk_axis];
+    T.s_dst[T.ns_axes] = dstr[s_chunk_axis];
+    T.ns_axes++;
+    T.nS = (int)(prod * len);
+  } else {
+    // every trailing axis up to the destination-inner one fits: treat the
+    // outermost included S axis as a single full chunk.
+    T.s_chunk_src = 0;
+    T.s_chunk_dst = 0;
+    T.s_chunk_full = 1;
+    T.s_chunk_len = 1;
+    T.s_nchunks = 1;
+    T.nS = (int)prod;
+  }
+  // D group: trailing output axes (innermost first), not in S.
+  prod = 1;
+  int d_chunk_axis = -1;
+  for (int j = r - 1; j >= 0; --j) {
+    int i = perm[j];
+    if (in_s[i]) break;
+    if (prod * dims[i] >= tgt) {
+      d_chunk_axis = i;
+      break;
+    }
+    in_d[i] = 1;
+    T.d_dim[T.nd_axes] = dims[i];
+    T.d_full[T.nd_axes] = dims[i];
+    T.d_src[T.nd_axes] = sstr[i];
+    T.nd_axes++;
+    prod *= dims[i];
+  }
+  T.d_inner = prod;
+  if (d_chunk_axis >= 0) {
+    int64_t len = std::max<int64_t>(1, tgt / prod);
+    len = std::min(len, dims[d_chunk_axis]);
+    in_d[d_chunk_axis] = 2;
+    T.d_chunk_src = sstr[d_chunk_axis];
+    T.d_chunk_dst = dstr[d_chunk_axis];
+    T.d_chunk_full = dims[d_chunk_axis];
+    T.d_chunk_len = len;
+    T.d_nchunks = (dims[d_chunk_axis] + len - 1) / len;
+    T.d_dim[T.nd_axes] = len;
+    T.d_full[T.nd_axes] = dims[d_chunk_axis];
+    T.d_src[T.nd_axes] = sstr[d_chunk_axis];
+    T.nd_axes++;
+    T.nD = (int)(prod * len);
+  } else {
+    T.d_chunk_src = 0;
+    T.d_chunk_dst = 0;
+    T.d_chunk_full = 1;
+    T.d_chunk_len = 1;
+    T.d_nchunks = 1;
+    T.nD = (int)prod;
+  }
+  // Outer axes (source order, outermost first).
+  for (int i = 0; i < r; ++i)
+    if (!in_s[i] && !in_d[i]) {
+      T.o_dim[T.nouter] = dims[i];
+      T.o_src[T.nouter] = sstr[i];
+      T.o_dst[T.nouter] = dstr[i];
+      T.nouter++;
+    }
+  int64_t grid = T.s_nchunks * T.d_nchunks;
+  for (int k = 0; k < T.nouter; ++k) grid *= T.o_dim[k];
+  *grid_out = grid;
+  T.ws = tgt;
+  T.wd = tgt;
+  TN_REQUIRE(T.nS >= 1 && T.nS <= tgt && T.nD >= 1 && T.nD <= tgt, TN_EUNSUPP,
+             "permute planner produced tile %d x %d", T.nD, T.nS);
+  TN_REQUIRE(grid < (int64_t(1) << 31), TN_EUNSUPP, "permute grid too large (%lld)",
+             (long long)grid);
+  return TN_OK;
+}
 
 int make_plan(int rank, const int64_t* dims_in, const int* perm_in, Plan* out) {
   TN_REQUIRE(rank >= 0 && rank <= kMaxRank, TN_EUNSUPP, "permute: rank %d outside [0,%d]", rank,
@@ -267,109 +380,17 @@ int make_plan(int rank, const int64_t* dims_in, const int* perm_in, Plan* out) {
     out->vec2 = (P.L % 2 == 0);
     return TN_OK;
   }
-  // Tiled transpose.
+  // Tiled transpose: the largest tile that still gives every CU two
+  // workgroups; small tensors get small tiles.
   out->kind = Plan::kTile;
-  TileParams& T = out->tile;
-  std::memset(&T, 0, sizeof(T));
-  std::vector<int> in_s(r, 0), in_d(r, 0);
-  // S group: trailing source axes (innermost first), stop before the
-  // destination-inner axis.
-  const int dinner = perm[r - 1];
-  int64_t prod = 1;
-  int s_chunk_axis = -1;
-  for (int i = r - 1; i >= 0; --i) {
-    if (i == dinner) break;
-    if (prod * dims[i] >= kTileS) {
-      s_chunk_axis = i;
-      break;
-    }
-    in_s[i] = 1;
-    T.s_dim[T.ns_axes] = dims[i];
-    T.s_full[T.ns_axes] = dims[i];
-    T.s_dst[T.ns_axes] = dstr[i];
-    T.ns_axes++;
-    prod *= dims[i];
+  const int64_t want = 2 * (int64_t)tn::cu_count();
+  for (int tgt : {64, 32, 16}) {
+    int64_t grid = 0;
+    int rc = build_tile(dims, perm, sstr, dstr, tgt, out->tile, &grid);
+    if (rc != TN_OK) return rc;
+    out->grid = grid;
+    if (grid >= want) break;
   }
-  T.s_inner = prod;
-  if (s_chunk_axis >= 0) {
-    int64_t len = std::max<int64_t>(1, kTileS / prod);
-    len = std::min(len, dims[s_chunk_axis]);
-    in_s[s_chunk_axis] = 2;
-    T.s_chunk_src = sstr[s_chunk_axis];
-    T.s_chunk_dst = dstr[s_chunk_axis];
-    T.s_chunk_full = dims[s_chunk_axis];
-    T.s_chunk_len = len;
-    T.s_nchunks = (dims[s_chunk_axis] + len - 1) / len;
-    T.s_dim[T.ns_axes] = len;
-    T.s_full[T.ns_axes] = dims[s_chunk_axis];
-    T.s_dst[T.ns_axes] = dstr[s_chunk_axis];
-    T.ns_axes++;
-    T.nS = (int)(prod * len);
-  } else {
-    // every trailing axis up to the destination-inner one fits: treat the
-    // outermost included S axis as a single full chunk.
-    T.s_chunk_src = 0;
-    T.s_chunk_dst = 0;
-    T.s_chunk_full = 1;
-    T.s_chunk_len = 1;
-    T.s_nchunks = 1;
-    T.nS = (int)prod;
-  }
-  // D group: trailing output axes (innermost first), not in S.
-  prod = 1;
-  int d_chunk_axis = -1;
-  for (int j = r - 1; j >= 0; --j) {
-    int i = perm[j];
-    if (in_s[i]) break;
-    if (prod * dims[i] >= kTileD) {
-      d_chunk_axis = i;
-      break;
-    }
-    in_d[i] = 1;
-    T.d_dim[T.nd_axes] = dims[i];
-    T.d_full[T.nd_axes] = dims[i];
-    T.d_src[T.nd_axes] = sstr[i];
-    T.nd_axes++;
-    prod *= dims[i];
-  }
-  T.d_inner = prod;
-  if (d_chunk_axis >= 0) {
-    int64_t len = std::max<int64_t>(1, kTileD / prod);
-    len = std::min(len, dims[d_chunk_axis]);
-    in_d[d_chunk_axis] = 2;
-    T.d_chunk_src = sstr[d_chunk_axis];
-    T.d_chunk_dst = dstr[d_chunk_axis];
-    T.d_chunk_full = dims[d_chunk_axis];
-    T.d_chunk_len = len;
-    T.d_nchunks = (dims[d_chunk_axis] + len - 1) / len;
-    T.d_dim[T.nd_axes] = len;
-    T.d_full[T.nd_axes] = dims[d_chunk_axis];
-    T.d_src[T.nd_axes] = sstr[d_chunk_axis];
-    T.nd_axes++;
-    T.nD = (int)(prod * len);
-  } else {
-    T.d_chunk_src = 0;
-    T.d_chunk_dst = 0;
-    T.d_chunk_full = 1;
-    T.d_chunk_len = 1;
-    T.d_nchunks = 1;
-    T.nD = (int)prod;
-  }
-  // Outer axes (source order, outermost first).
-  for (int i = 0; i < r; ++i)
-    if (!in_s[i] && !in_d[i]) {
-      T.o_dim[T.nouter] = dims[i];
-      T.o_src[T.nouter] = sstr[i];
-      T.o_dst[T.nouter] = dstr[i];
-      T.nouter++;
-    }
-  int64_t grid = T.s_nchunks * T.d_nchunks;
-  for (int k = 0; k < T.nouter; ++k) grid *= T.o_dim[k];
-  out->grid = grid;
-  TN_REQUIRE(T.nS >= 1 && T.nS <= kTileS && T.nD >= 1 && T.nD <= kTileD, TN_EUNSUPP,
-             "permute planner produced tile %d x %d", T.nD, T.nS);
-  TN_REQUIRE(grid < (int64_t(1) << 31), TN_EUNSUPP, "permute grid too large (%lld)",
-             (long long)grid);
   return TN_OK;
 }
 
@@ -385,6 +406,8 @@ int run_plan(const Plan& P, const double* src, double* dst, hipStream_t s) {
       bool vec = P.vec2 && (((uintptr_t)src | (uintptr_t)dst) & 15) == 0;
       // rows per block: aim at >= 2048 elements per block
       int rpb = (int)std::max<int64_t>(1, std::min<int64_t>(64, 2048 / std::max<int64_t>(1, R.L)));
+      // keep at least two workgroups per CU when the tensor allows it
+      while (rpb > 1 && (R.rows + rpb - 1) / rpb < 2 * (int64_t)tn::cu_count()) rpb /= 2;
       int64_t grid = (R.rows + rpb - 1) / rpb;
       TN_REQUIRE(grid < (int64_t(1) << 31), TN_EUNSUPP, "row-copy grid too large");
       if (vec) {

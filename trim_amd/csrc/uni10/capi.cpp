@@ -1,0 +1,322 @@
+// C facade of the Uni10 C++ API (include/uni10_capi.h).
+#include "uni10_capi.h"
+
+#include <cstring>
+#include <string>
+
+#include "common.h"
+#include "tebd.h"
+
+using namespace uni10;
+
+struct u10_tensor_s {
+  UniTensor T;
+};
+struct u10_network_s {
+  explicit u10_network_s(const std::string& f) : N(f) {}
+  Network N;
+};
+struct u10_itebd_s {
+  template <typename... A>
+  explicit u10_itebd_s(A&&... a) : E(std::forward<A>(a)...) {}
+  ITebd E;
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    f();
+    g_err.clear();
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  } catch (...) {
+    g_err = "unknown error";
+    return -1;
+  }
+}
+
+Qnum qn(const int* t) { return Qnum((parityFType)t[2], t[0], (parityType)t[1]); }
+
+void need(const void* p, const char* what) {
+  if (!p) detail::fail(std::string(what) + ": NULL argument");
+}
+}  // namespace
+
+extern "C" {
+
+const char* u10_last_error(void) { return g_err.c_str(); }
+
+int u10_set_gemm_mode(int mode) {
+  return guard([&] {
+    if (mode < 0 || mode > 3) detail::fail("unknown GEMM mode");
+    setGemmMode((GemmMode)mode);
+  });
+}
+
+int u10_set_stream(void* s) {
+  return guard([&] { setStream(s); });
+}
+
+int u10_tensor_create(int nbonds, const int* types, const int* nruns, const int* run_qnums, const int* run_degs,
+                      const char* name, u10_tensor* out) {
+  return guard([&] {
+    need(out, "u10_tensor_create");
+    std::vector<Bond> bonds;
+    int r = 0;
+    for (int b = 0; b < nbonds; ++b) {
+      std::vector<Qnum> full;
+      for (int k = 0; k < nruns[b]; ++k, ++r) full.insert(full.end(), run_degs[r], qn(run_qnums + 3 * r));
+      bonds.emplace_back(types[b] == 1 ? BD_IN : BD_OUT, full);
+    }
+    *out = new u10_tensor_s{UniTensor(bonds, name ? name : "")};
+  });
+}
+
+int u10_tensor_scalar(double v, u10_tensor* out) {
+  return guard([&] { *out = new u10_tensor_s{UniTensor(v)}; });
+}
+
+int u10_tensor_load(const char* fname, u10_tensor* out) {
+  return guard([&] { *out = new u10_tensor_s{UniTensor(std::string(fname))}; });
+}
+
+int u10_tensor_save(u10_tensor t, const char* fname) {
+  return guard([&] { t->T.save(fname); });
+}
+
+int u10_tensor_copy(u10_tensor t, u10_tensor* out) {
+  return guard([&] { *out = new u10_tensor_s{UniTensor(t->T)}; });
+}
+
+int u10_tensor_free(u10_tensor t) {
+  return guard([&] { delete t; });
+}
+
+int u10_tensor_info(u10_tensor t, int* nb, int* ib, int64_t* ne, int* nblk) {
+  return guard([&] {
+    if (nb) *nb = (int)t->T.bondNum();
+    if (ib) *ib = (int)t->T.inBondNum();
+    if (ne) *ne = (int64_t)t->T.elemNum();
+    if (nblk) *nblk = (int)t->T.blockNum();
+  });
+}
+
+int u10_tensor_labels(u10_tensor t, int* labels) {
+  return guard([&] {
+    auto l = t->T.label();
+    std::copy(l.begin(), l.end(), labels);
+  });
+}
+
+int u10_tensor_set_label(u10_tensor t, int n, const int* labels) {
+  return guard([&] { t->T.setLabel(std::vector<int>(labels, labels + n)); });
+}
+
+int u10_tensor_bond(u10_tensor t, int b, int* type, int* dim, int* nruns) {
+  return guard([&] {
+    Bond bd = t->T.bond((size_t)b);
+    if (type) *type = (int)bd.type();
+    if (dim) *dim = bd.dim();
+    if (nruns) *nruns = (int)bd.qnums().size();
+  });
+}
+
+int u10_tensor_bond_runs(u10_tensor t, int b, int* q, int* d) {
+  return guard([&] {
+    Bond bd = t->T.bond((size_t)b);
+    for (size_t i = 0; i < bd.qnums().size(); ++i) {
+      q[3 * i] = bd.qnums()[i].U1();
+      q[3 * i + 1] = (int)bd.qnums()[i].prt();
+      q[3 * i + 2] = (int)bd.qnums()[i].prtF();
+      d[i] = bd.qdegs()[i];
+    }
+  });
+}
+
+int u10_tensor_block(u10_tensor t, int i, int* q, int64_t* rows, int64_t* cols, int64_t* offset) {
+  return guard([&] {
+    const auto& bl = t->T.blockInfo();
+    if (i < 0 || i >= (int)bl.size()) detail::fail("block index out of range");
+    auto it = bl.begin();
+    std::advance(it, i);
+    q[0] = it->first.U1();
+    q[1] = (int)it->first.prt();
+    q[2] = (int)it->first.prtF();
+    *rows = (int64_t)it->second.Rnum;
+    *cols = (int64_t)it->second.Cnum;
+    *offset = (int64_t)it->second.offset;
+  });
+}
+
+int u10_tensor_set_elem(u10_tensor t, const double* host) {
+  return guard([&] { t->T.setElem(host, false); });
+}
+
+int u10_tensor_get_elem(u10_tensor t, double* host) {
+  return guard([&] {
+    auto h = t->T.getHostElem();
+    std::memcpy(host, h.data(), h.size() * sizeof(double));
+  });
+}
+
+int u10_tensor_set_raw(u10_tensor t, const double* raw) {
+  return guard([&] { t->T.setRawElem(raw); });
+}
+
+int u10_tensor_raw_shape(u10_tensor t, int64_t* rows, int64_t* cols) {
+  return guard([&] {
+    int64_t r = 1, c = 1;
+    for (auto& b : t->T.bond()) (b.type() == BD_IN ? r : c) *= b.dim();
+    *rows = r;
+    *cols = c;
+  });
+}
+
+int u10_tensor_get_raw(u10_tensor t, double* raw) {
+  return guard([&] {
+    Matrix m = t->T.getRawElem();
+    auto h = m.getHostElem();
+    std::memcpy(raw, h.data(), h.size() * sizeof(double));
+  });
+}
+
+int u10_tensor_at(u10_tensor t, const int64_t* idxs, double* out) {
+  return guard([&] {
+    std::vector<size_t> ix(idxs, idxs + t->T.bondNum());
+    *out = t->T.at(ix);
+  });
+}
+
+int u10_tensor_put_block(u10_tensor t, const int* q, int64_t rows, int64_t cols, const double* host) {
+  return guard([&] { t->T.putBlock(qn(q), Matrix((size_t)rows, (size_t)cols, host, false, false)); });
+}
+
+int u10_tensor_get_block(u10_tensor t, const int* q, double* host) {
+  return guard([&] {
+    auto h = t->T.getBlock(qn(q)).getHostElem();
+    std::memcpy(host, h.data(), h.size() * sizeof(double));
+  });
+}
+
+int u10_tensor_permute(u10_tensor t, int n, const int* labels, int in_bond_num) {
+  return guard([&] { t->T.permute(std::vector<int>(labels, labels + n), in_bond_num); });
+}
+
+int u10_tensor_transpose(u10_tensor t) {
+  return guard([&] { t->T.transpose(); });
+}
+
+int u10_tensor_scale(u10_tensor t, double a) {
+  return guard([&] { t->T *= a; });
+}
+
+int u10_tensor_norm(u10_tensor t, double* out) {
+  return guard([&] { *out = t->T.norm(); });
+}
+
+int u10_tensor_device_ptr(u10_tensor t, void** dptr) {
+  return guard([&] { *dptr = t->T.getElem(); });
+}
+
+int u10_contract(u10_tensor a, u10_tensor b, int fast, u10_tensor* out) {
+  return guard([&] { *out = new u10_tensor_s{contract(a->T, b->T, fast != 0)}; });
+}
+
+int u10_otimes(u10_tensor a, u10_tensor b, u10_tensor* out) {
+  return guard([&] { *out = new u10_tensor_s{otimes(a->T, b->T)}; });
+}
+
+int u10_network_create(const char* fname, u10_network* out) {
+  return guard([&] { *out = new u10_network_s(fname); });
+}
+
+int u10_network_put(u10_network net, const char* name, u10_tensor t, int transposed) {
+  return guard([&] {
+    if (transposed)
+      net->N.putTensorT(name, &t->T);
+    else
+      net->N.putTensor(name, &t->T);
+  });
+}
+
+int u10_network_launch(u10_network net, const char* name, u10_tensor* out) {
+  return guard([&] { *out = new u10_tensor_s{net->N.launch(name ? name : "")}; });
+}
+
+int u10_network_tree(u10_network net, char* buf, int len) {
+  return guard([&] {
+    std::string s = net->N.tree();
+    std::snprintf(buf, (size_t)len, "%s", s.c_str());
+  });
+}
+
+int u10_network_free(u10_network net) {
+  return guard([&] { delete net; });
+}
+
+int u10_svd(const double* A, int64_t M, int64_t N, double* U, double* S, double* VT) {
+  return guard([&] {
+    Matrix a((size_t)M, (size_t)N, A, false, false);
+    auto r = a.svd();
+    auto u = r[0].getHostElem(), s = r[1].getHostElem(), vt = r[2].getHostElem();
+    std::memcpy(U, u.data(), u.size() * 8);
+    std::memcpy(S, s.data(), s.size() * 8);
+    std::memcpy(VT, vt.data(), vt.size() * 8);
+  });
+}
+
+int u10_eigh(const double* A, int64_t N, double* eig, double* vec) {
+  return guard([&] {
+    Matrix a((size_t)N, (size_t)N, A, false, false);
+    auto r = a.eigh();
+    auto e = r[0].getHostElem(), v = r[1].getHostElem();
+    std::memcpy(eig, e.data(), e.size() * 8);
+    std::memcpy(vec, v.data(), v.size() * 8);
+  });
+}
+
+int u10_exph(double a, const double* H, int64_t N, double* out) {
+  return guard([&] {
+    Matrix h((size_t)N, (size_t)N, H, false, false);
+    auto g = exph(a, h).getHostElem();
+    std::memcpy(out, g.data(), g.size() * 8);
+  });
+}
+
+int u10_itebd_create(int chi, int d, double delta, double tau, uint64_t seed, const char* netdir, u10_itebd* out) {
+  return guard([&] { *out = new u10_itebd_s(chi, d, delta, tau, seed, std::string(netdir)); });
+}
+
+int u10_itebd_step(u10_itebd t, int n) {
+  return guard([&] {
+    for (int i = 0; i < n; ++i) t->E.step();
+  });
+}
+
+int u10_itebd_energy(u10_itebd t, double* out) {
+  return guard([&] { *out = t->E.energy(); });
+}
+
+int u10_itebd_lambda(u10_itebd t, int which, double* out, int* n) {
+  return guard([&] {
+    const Matrix& L = which == 0 ? t->E.La() : t->E.Lb();
+    auto h = L.getHostElem();
+    *n = (int)h.size();
+    if (out) std::memcpy(out, h.data(), h.size() * 8);
+  });
+}
+
+int u10_itebd_trunc_err(u10_itebd t, double* out) {
+  return guard([&] { *out = t->E.truncation_error(); });
+}
+
+int u10_itebd_free(u10_itebd t) {
+  return guard([&] { delete t; });
+}
+
+}  // extern "C"

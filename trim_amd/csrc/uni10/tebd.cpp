@@ -1,0 +1,141 @@
+// iTEBD driver written against the Uni10 API, as uni10/examples/iTEBD.cpp
+// (update :148-173, measure :189-203, expectation :205-216) with the
+// .net files shipped beside it; spin-1/2 XXZ two-site gate exp(-tau H).
+#include <algorithm>
+#include <cmath>
+
+#include "common.h"
+#include "tebd.h"
+
+namespace uni10 {
+
+Matrix xxz_two_site(double delta, double J) {
+  // J (Sx Sx + Sy Sy + delta Sz Sz), basis |uu>,|ud>,|du>,|dd>
+  std::vector<double> h(16, 0.0);
+  h[0] = h[15] = 0.25 * delta * J;
+  h[5] = h[10] = -0.25 * delta * J;
+  h[6] = h[9] = 0.5 * J;
+  return Matrix(4, 4, h.data(), false, false);
+}
+
+ITebd::ITebd(int chi, int d, double delta, double tau, uint64_t seed, const std::string& netdir)
+    : chi_(chi), d_(d), tau_(tau),
+      iTEBD_(netdir + "/itebd.net"), updateA_(netdir + "/updateA.net"), MPS_(netdir + "/MPS.net"),
+      meas_(netdir + "/measure.net") {
+  if (d != 2) detail::fail("ITebd: only the spin-1/2 chain (d = 2) is provided");
+  Matrix Hm = xxz_two_site(delta, 1.0);
+  std::vector<Bond> bH = {Bond(BD_IN, d), Bond(BD_IN, d), Bond(BD_OUT, d), Bond(BD_OUT, d)};
+  H_ = UniTensor(bH, "H");
+  H_.putBlock(Hm);
+  U_ = UniTensor(bH, "U");
+  U_.putBlock(takeExp(-tau, Hm));
+  // Gamma_A, Gamma_B (chi, d, chi) ~ U[-1,1) with seeds seed, seed+1; lambdas
+  // sorted |U[-1,1)| (seeds seed+2, seed+3) normalised; ALa = Gamma_A lambda_A
+  // in the (chi_left, chi_right | d) layout the update networks use.
+  const size_t n = (size_t)chi * d * chi;
+  std::vector<double> ga(n), gb(n), la(chi), lb(chi);
+  for (size_t i = 0; i < n; ++i) {
+    ga[i] = detail::splitmix_uniform(seed, i + 1);
+    gb[i] = detail::splitmix_uniform(seed + 1, i + 1);
+  }
+  for (int i = 0; i < chi; ++i) {
+    la[i] = std::fabs(detail::splitmix_uniform(seed + 2, i + 1));
+    lb[i] = std::fabs(detail::splitmix_uniform(seed + 3, i + 1));
+  }
+  std::sort(la.rbegin(), la.rend());
+  std::sort(lb.rbegin(), lb.rend());
+  double na = 0, nbb = 0;
+  for (int i = 0; i < chi; ++i) {
+    na += la[i] * la[i];
+    nbb += lb[i] * lb[i];
+  }
+  na = std::sqrt(na);
+  nbb = std::sqrt(nbb);
+  for (int i = 0; i < chi; ++i) {
+    la[i] /= na;
+    lb[i] /= nbb;
+  }
+  std::vector<double> ala(n), blb(n);
+  for (int a = 0; a < chi; ++a)
+    for (int s = 0; s < d; ++s)
+      for (int b = 0; b < chi; ++b) {
+        ala[((size_t)a * chi + b) * d + s] = ga[((size_t)a * d + s) * chi + b] * la[b];
+        blb[((size_t)a * chi + b) * d + s] = gb[((size_t)a * d + s) * chi + b] * lb[b];
+      }
+  std::vector<Bond> b3 = {Bond(BD_IN, chi), Bond(BD_IN, chi), Bond(BD_OUT, d)};
+  ALa_ = UniTensor(b3, "ALa");
+  BLb_ = UniTensor(b3, "BLb");
+  ALa_.setElem(ala);
+  BLb_.setElem(blb);
+  La_ = Matrix(chi, chi, la.data(), true, false);
+  Lb_ = Matrix(chi, chi, lb.data(), true, false);
+}
+
+void ITebd::update(UniTensor& ALa, UniTensor& BLb, Matrix& La, Matrix& Lb) {
+  Qnum q0(0);
+  iTEBD_.putTensor("ALa", &ALa);
+  iTEBD_.putTensor("BLb", &BLb);
+  iTEBD_.putTensor("expH", &U_);
+  UniTensor C = iTEBD_.launch();
+  UniTensor Theta(C.bond(), "Theta");
+  Theta.putBlock(q0, Lb * C.getBlock(q0));
+  Theta.permute(2);
+  std::vector<Matrix> rets = Theta.getBlock(q0).svd();
+  int dim = chi_ < (int)rets[1].row() ? chi_ : (int)rets[1].row();
+  // discarded weight (not computed by the reference; reported alongside)
+  {
+    std::vector<double> s = rets[1].getHostElem();
+    double e = 0;
+    for (size_t i = dim; i < s.size(); ++i) e += s[i] * s[i];
+    trunc_err_ = e;
+  }
+  double norm = rets[1].resize(dim, dim).norm();
+  rets[1] *= (1 / norm);
+  La = rets[1];
+  Bond bdi(BD_IN, dim);
+  std::vector<Bond> bond3 = ALa.bond();
+  bond3[0] = bdi;
+  BLb.assign(bond3);
+  Matrix blk = BLb.getBlock(q0);
+  blk.setElem(rets[2].getElem(), true);
+  BLb.putBlock(q0, blk);
+  updateA_.putTensor("BLb", &BLb);
+  updateA_.putTensor("C", &C);
+  ALa = updateA_.launch();
+  ALa *= (1 / norm);
+}
+
+void ITebd::step() {
+  update(ALa_, BLb_, La_, Lb_);
+  update(BLb_, ALa_, Lb_, La_);
+}
+
+double ITebd::expectation(UniTensor& L, UniTensor& R) {
+  Qnum q0(0);
+  MPS_.putTensor("L", &L);
+  MPS_.putTensor("R", &R);
+  UniTensor psi = MPS_.launch();
+  double norm = psi.getBlock(q0).norm();
+  norm *= norm;
+  meas_.putTensor("bra", &psi);
+  meas_.putTensorT("ket", &psi);
+  meas_.putTensor("Op", &H_);
+  return meas_.launch()[0] / norm;
+}
+
+double ITebd::energy() {
+  Qnum q0(0);
+  UniTensor A1(ALa_);
+  A1.permute(1);
+  A1.putBlock(q0, Lb_ * A1.getBlock(q0));
+  A1.permute(2);
+  UniTensor B1(BLb_);
+  B1.permute(1);
+  B1.putBlock(q0, La_ * B1.getBlock(q0));
+  B1.permute(2);
+  double val = expectation(A1, BLb_);
+  val += expectation(B1, ALa_);
+  return val / 2;
+}
+
+}  // namespace uni10

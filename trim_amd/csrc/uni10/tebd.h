@@ -1,0 +1,34 @@
+// iTEBD driver over the Uni10 API (the reference example's update/measure).
+#pragma once
+#include <cstdint>
+#include <string>
+
+#include "uni10_amd/uni10.hpp"
+
+namespace uni10 {
+
+Matrix xxz_two_site(double delta, double J);
+
+class ITebd {
+ public:
+  ITebd(int chi, int d, double delta, double tau, uint64_t seed, const std::string& netdir);
+  void step();                 // update(A,B) then update(B,A)
+  double energy();             // mean of the two bond energies
+  double truncation_error() const { return trunc_err_; }
+  const Matrix& La() const { return La_; }
+  const Matrix& Lb() const { return Lb_; }
+  UniTensor& ALa() { return ALa_; }
+  UniTensor& BLb() { return BLb_; }
+
+ private:
+  void update(UniTensor& ALa, UniTensor& BLb, Matrix& La, Matrix& Lb);
+  double expectation(UniTensor& L, UniTensor& R);
+  int chi_, d_;
+  double tau_;
+  double trunc_err_ = 0;
+  UniTensor H_, U_, ALa_, BLb_;
+  Matrix La_, Lb_;
+  Network iTEBD_, updateA_, MPS_, meas_;
+};
+
+}  // namespace uni10

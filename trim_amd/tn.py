@@ -51,6 +51,7 @@ SIGNATURES = {
     "tn_blockperm_plan_destroy": [P],
     "tn_transpose_f64": [P, I64, I64, P, P],
     "tn_gemm_plan_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, I64, P],
+    "tn_gemm_plan_create_ex": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, I64, P, I64, P],
     "tn_gemm_plan_set_tile": [P, ctypes.c_int],
     "tn_gemm_plan_info": [P, PI64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)],
     "tn_gemm_exec": [P, P, P, P, P],

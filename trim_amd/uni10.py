@@ -1,0 +1,444 @@
+"""Python binding of the Uni10 C++ API (libuni10_amd.so through include/uni10_capi.h).
+
+Plays the role of the reference's SWIG module (uni10/Swig/pyUni10.i): the
+classes hold handles of C++ objects whose elements live in HBM; every
+operation runs the C++ API and its HIP kernels.  No computation happens in
+Python and nothing falls back to a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import tn as _tn
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libuni10_amd.so")
+
+BD_IN, BD_OUT = 1, -1
+PRT_EVEN, PRT_ODD = 0, 1
+PRTF_EVEN, PRTF_ODD = 0, 1
+GEMM_F64, GEMM_F32, GEMM_BF16X3, GEMM_BF16X6 = 0, 1, 2, 3
+
+P = ctypes.c_void_p
+PP = ctypes.POINTER(ctypes.c_void_p)
+I = ctypes.c_int
+I64 = ctypes.c_int64
+D = ctypes.c_double
+PI = ctypes.POINTER(ctypes.c_int)
+PI64 = ctypes.POINTER(ctypes.c_int64)
+PD = ctypes.POINTER(ctypes.c_double)
+S = ctypes.c_char_p
+
+SIGNATURES = {
+    "u10_last_error": [],
+    "u10_set_gemm_mode": [I],
+    "u10_set_stream": [P],
+    "u10_tensor_create": [I, P, P, P, P, S, PP],
+    "u10_tensor_scalar": [D, PP],
+    "u10_tensor_load": [S, PP],
+    "u10_tensor_save": [P, S],
+    "u10_tensor_copy": [P, PP],
+    "u10_tensor_free": [P],
+    "u10_tensor_info": [P, PI, PI, PI64, PI],
+    "u10_tensor_labels": [P, P],
+    "u10_tensor_set_label": [P, I, P],
+    "u10_tensor_bond": [P, I, PI, PI, PI],
+    "u10_tensor_bond_runs": [P, I, P, P],
+    "u10_tensor_block": [P, I, P, PI64, PI64, PI64],
+    "u10_tensor_set_elem": [P, P],
+    "u10_tensor_get_elem": [P, P],
+    "u10_tensor_set_raw": [P, P],
+    "u10_tensor_raw_shape": [P, PI64, PI64],
+    "u10_tensor_get_raw": [P, P],
+    "u10_tensor_at": [P, P, PD],
+    "u10_tensor_put_block": [P, P, I64, I64, P],
+    "u10_tensor_get_block": [P, P, P],
+    "u10_tensor_permute": [P, I, P, I],
+    "u10_tensor_transpose": [P],
+    "u10_tensor_scale": [P, D],
+    "u10_tensor_norm": [P, PD],
+    "u10_tensor_device_ptr": [P, PP],
+    "u10_contract": [P, P, I, PP],
+    "u10_otimes": [P, P, PP],
+    "u10_network_create": [S, PP],
+    "u10_network_put": [P, S, P, I],
+    "u10_network_launch": [P, S, PP],
+    "u10_network_tree": [P, ctypes.c_char_p, I],
+    "u10_network_free": [P],
+    "u10_svd": [P, I64, I64, P, P, P],
+    "u10_eigh": [P, I64, P, P],
+    "u10_exph": [D, P, I64, P],
+    "u10_itebd_create": [I, I, D, D, ctypes.c_uint64, S, PP],
+    "u10_itebd_step": [P, I],
+    "u10_itebd_energy": [P, PD],
+    "u10_itebd_lambda": [P, I, P, PI],
+    "u10_itebd_trunc_err": [P, PD],
+    "u10_itebd_free": [P],
+}
+
+_lib = None
+
+
+class Uni10Error(RuntimeError):
+    pass
+
+
+def load():
+    global _lib
+    if _lib is None:
+        _tn.load()   # dependency first (same directory, rpath $ORIGIN)
+        if not os.path.exists(LIB_PATH):
+            raise Uni10Error("libuni10_amd.so is not built; run python -m trim_amd._build")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_char_p if name == "u10_last_error" else ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _call(name, *args):
+    rc = getattr(load(), name)(*args)
+    if rc != 0:
+        raise Uni10Error("%s: %s" % (name, load().u10_last_error().decode(errors="replace")))
+
+
+def _arr(a, dtype):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a, a.ctypes.data_as(ctypes.c_void_p)
+
+
+def set_gemm_mode(mode: int):
+    _call("u10_set_gemm_mode", int(mode))
+
+
+class Qnum:
+    __slots__ = ("U1", "prt", "prtF")
+
+    def __init__(self, U1=0, prt=PRT_EVEN, prtF=PRTF_EVEN):
+        self.U1, self.prt, self.prtF = int(U1), int(prt), int(prtF)
+
+    def triple(self):
+        return (self.U1, self.prt, self.prtF)
+
+    def __eq__(self, o):
+        return self.triple() == o.triple()
+
+    def __hash__(self):
+        return hash(self.triple())
+
+    def __repr__(self):
+        return "Qnum(%d,%d,%d)" % self.triple()
+
+
+class Bond:
+    """Bond descriptor: a type and one Qnum per state (combine/degeneracy are
+    C++ operations; this is what crosses the binding)."""
+
+    def __init__(self, tp, qnums_or_dim):
+        self.type = int(tp)
+        if isinstance(qnums_or_dim, (int, np.integer)):
+            self.qnums = [Qnum(0)] * int(qnums_or_dim)
+        else:
+            self.qnums = [q if isinstance(q, Qnum) else Qnum(*q) for q in qnums_or_dim]
+
+    @property
+    def dim(self):
+        return len(self.qnums)
+
+    def runs(self):
+        out = []
+        for q in self.qnums:
+            if out and out[-1][0] == q:
+                out[-1][1] += 1
+            else:
+                out.append([q, 1])
+        return out
+
+
+class UniTensor:
+    def __init__(self, bonds: Sequence[Bond] = None, name: str = "", _handle=None):
+        if _handle is not None:
+            self.h = _handle
+            return
+        bonds = list(bonds or [])
+        types = np.array([b.type for b in bonds], np.int32)
+        runs = [b.runs() for b in bonds]
+        nruns = np.array([len(r) for r in runs], np.int32)
+        q = np.array([x[0].triple() for r in runs for x in r], np.int32).reshape(-1)
+        d = np.array([x[1] for r in runs for x in r], np.int32)
+        h = ctypes.c_void_p()
+        _call("u10_tensor_create", len(bonds), types.ctypes.data_as(P), nruns.ctypes.data_as(P),
+              (q if q.size else np.zeros(1, np.int32)).ctypes.data_as(P),
+              (d if d.size else np.zeros(1, np.int32)).ctypes.data_as(P), name.encode(), ctypes.byref(h))
+        self.h = h.value
+
+    @classmethod
+    def scalar(cls, v: float):
+        h = ctypes.c_void_p()
+        _call("u10_tensor_scalar", float(v), ctypes.byref(h))
+        return cls(_handle=h.value)
+
+    @classmethod
+    def load(cls, fname: str):
+        h = ctypes.c_void_p()
+        _call("u10_tensor_load", fname.encode(), ctypes.byref(h))
+        return cls(_handle=h.value)
+
+    def save(self, fname: str):
+        _call("u10_tensor_save", self.h, fname.encode())
+
+    def copy(self):
+        h = ctypes.c_void_p()
+        _call("u10_tensor_copy", self.h, ctypes.byref(h))
+        return UniTensor(_handle=h.value)
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                load().u10_tensor_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def _info(self):
+        nb, ib, nblk = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ne = ctypes.c_int64()
+        _call("u10_tensor_info", self.h, ctypes.byref(nb), ctypes.byref(ib), ctypes.byref(ne),
+              ctypes.byref(nblk))
+        return nb.value, ib.value, ne.value, nblk.value
+
+    def bondNum(self):
+        return self._info()[0]
+
+    def inBondNum(self):
+        return self._info()[1]
+
+    def elemNum(self):
+        return self._info()[2]
+
+    def blockNum(self):
+        return self._info()[3]
+
+    def label(self) -> List[int]:
+        n = self.bondNum()
+        out = np.zeros(max(n, 1), np.int32)
+        _call("u10_tensor_labels", self.h, out.ctypes.data_as(P))
+        return [int(v) for v in out[:n]]
+
+    def setLabel(self, labels):
+        a, p = _arr(labels, np.int32)
+        _call("u10_tensor_set_label", self.h, len(a), p)
+        return self
+
+    def bond(self, b=None):
+        if b is None:
+            return [self.bond(i) for i in range(self.bondNum())]
+        tp, dim, nr = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _call("u10_tensor_bond", self.h, b, ctypes.byref(tp), ctypes.byref(dim), ctypes.byref(nr))
+        q = np.zeros(3 * nr.value, np.int32)
+        d = np.zeros(nr.value, np.int32)
+        _call("u10_tensor_bond_runs", self.h, b, q.ctypes.data_as(P), d.ctypes.data_as(P))
+        qs = []
+        for i in range(nr.value):
+            qs += [Qnum(*q[3 * i:3 * i + 3])] * int(d[i])
+        return Bond(tp.value, qs)
+
+    def blocks(self):
+        """[(Qnum, rows, cols, offset)] in block order."""
+        out = []
+        for i in range(self.blockNum()):
+            q = np.zeros(3, np.int32)
+            r, c, o = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+            _call("u10_tensor_block", self.h, i, q.ctypes.data_as(P), ctypes.byref(r), ctypes.byref(c),
+                  ctypes.byref(o))
+            out.append((Qnum(*q), r.value, c.value, o.value))
+        return out
+
+    def setElem(self, e):
+        a, p = _arr(e, np.float64)
+        if a.size < self.elemNum():
+            raise ValueError("not enough elements")
+        _call("u10_tensor_set_elem", self.h, p)
+
+    def getElem(self) -> np.ndarray:
+        out = np.empty(self.elemNum())
+        _call("u10_tensor_get_elem", self.h, out.ctypes.data_as(P))
+        return out
+
+    def setRawElem(self, raw):
+        a, p = _arr(raw, np.float64)
+        _call("u10_tensor_set_raw", self.h, p)
+
+    def getRawElem(self) -> np.ndarray:
+        r, c = ctypes.c_int64(), ctypes.c_int64()
+        _call("u10_tensor_raw_shape", self.h, ctypes.byref(r), ctypes.byref(c))
+        out = np.empty((r.value, c.value))
+        _call("u10_tensor_get_raw", self.h, out.ctypes.data_as(P))
+        return out
+
+    def at(self, idxs):
+        a, p = _arr(idxs, np.int64)
+        v = ctypes.c_double()
+        _call("u10_tensor_at", self.h, p, ctypes.byref(v))
+        return v.value
+
+    def putBlock(self, mat, q: Optional[Qnum] = None):
+        q = q or Qnum(0)
+        m, p = _arr(mat, np.float64)
+        qa, qp = _arr(q.triple(), np.int32)
+        _call("u10_tensor_put_block", self.h, qp, m.shape[0], m.shape[1], p)
+
+    def getBlock(self, q: Optional[Qnum] = None) -> np.ndarray:
+        q = q or Qnum(0)
+        for bq, r, c, o in self.blocks():
+            if bq == q:
+                out = np.empty((r, c))
+                qa, qp = _arr(q.triple(), np.int32)
+                _call("u10_tensor_get_block", self.h, qp, out.ctypes.data_as(P))
+                return out
+        raise KeyError("There is no block with the given quantum number %r" % (q,))
+
+    def permute(self, labels, in_bond_num=None):
+        if in_bond_num is None:          # permute(rowBondNum)
+            in_bond_num, labels = labels, self.label()
+        a, p = _arr(labels, np.int32)
+        _call("u10_tensor_permute", self.h, len(a), p, int(in_bond_num))
+        return self
+
+    def transpose(self):
+        _call("u10_tensor_transpose", self.h)
+        return self
+
+    def __imul__(self, a):
+        _call("u10_tensor_scale", self.h, float(a))
+        return self
+
+    def norm(self) -> float:
+        v = ctypes.c_double()
+        _call("u10_tensor_norm", self.h, ctypes.byref(v))
+        return v.value
+
+    def device_ptr(self) -> int:
+        p = ctypes.c_void_p()
+        _call("u10_tensor_device_ptr", self.h, ctypes.byref(p))
+        return p.value
+
+    def __mul__(self, other):
+        if isinstance(other, UniTensor):
+            return contract(self.copy(), other.copy(), True)
+        t = self.copy()
+        t *= other
+        return t
+
+
+def contract(a: UniTensor, b: UniTensor, fast: bool = False) -> UniTensor:
+    h = ctypes.c_void_p()
+    _call("u10_contract", a.h, b.h, 1 if fast else 0, ctypes.byref(h))
+    return UniTensor(_handle=h.value)
+
+
+def otimes(a: UniTensor, b: UniTensor) -> UniTensor:
+    h = ctypes.c_void_p()
+    _call("u10_otimes", a.h, b.h, ctypes.byref(h))
+    return UniTensor(_handle=h.value)
+
+
+class Network:
+    def __init__(self, fname: str):
+        h = ctypes.c_void_p()
+        _call("u10_network_create", fname.encode(), ctypes.byref(h))
+        self.h = h.value
+
+    def putTensor(self, name: str, T: UniTensor):
+        _call("u10_network_put", self.h, name.encode(), T.h, 0)
+
+    def putTensorT(self, name: str, T: UniTensor):
+        _call("u10_network_put", self.h, name.encode(), T.h, 1)
+
+    def launch(self, name: str = "") -> UniTensor:
+        h = ctypes.c_void_p()
+        _call("u10_network_launch", self.h, name.encode(), ctypes.byref(h))
+        return UniTensor(_handle=h.value)
+
+    def tree(self) -> str:
+        buf = ctypes.create_string_buffer(4096)
+        _call("u10_network_tree", self.h, buf, 4096)
+        return buf.value.decode()
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                load().u10_network_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def svd(A):
+    a, p = _arr(A, np.float64)
+    M, N = a.shape
+    k = min(M, N)
+    U, Sv, VT = np.empty((M, k)), np.empty(k), np.empty((k, N))
+    _call("u10_svd", p, M, N, U.ctypes.data_as(P), Sv.ctypes.data_as(P), VT.ctypes.data_as(P))
+    return U, Sv, VT
+
+
+def eigh(A):
+    a, p = _arr(A, np.float64)
+    n = a.shape[0]
+    w, v = np.empty(n), np.empty((n, n))
+    _call("u10_eigh", p, n, w.ctypes.data_as(P), v.ctypes.data_as(P))
+    return w, v
+
+
+def exph(x: float, H):
+    a, p = _arr(H, np.float64)
+    n = a.shape[0]
+    out = np.empty((n, n))
+    _call("u10_exph", float(x), p, n, out.ctypes.data_as(P))
+    return out
+
+
+class ITebd:
+    """The reference example's iTEBD (Networks + SVD truncation) in C++."""
+
+    def __init__(self, chi: int, d: int, delta: float, tau: float, seed: int, netdir: str):
+        h = ctypes.c_void_p()
+        _call("u10_itebd_create", int(chi), int(d), float(delta), float(tau), int(seed),
+              netdir.encode(), ctypes.byref(h))
+        self.h = h.value
+        self.chi = chi
+
+    def step(self, n: int = 1):
+        _call("u10_itebd_step", self.h, int(n))
+
+    def energy(self) -> float:
+        v = ctypes.c_double()
+        _call("u10_itebd_energy", self.h, ctypes.byref(v))
+        return v.value
+
+    def lam(self, which: int) -> np.ndarray:
+        n = ctypes.c_int()
+        _call("u10_itebd_lambda", self.h, which, None, ctypes.byref(n))
+        out = np.empty(n.value)
+        _call("u10_itebd_lambda", self.h, which, out.ctypes.data_as(P), ctypes.byref(n))
+        return out
+
+    def trunc_err(self) -> float:
+        v = ctypes.c_double()
+        _call("u10_itebd_trunc_err", self.h, ctypes.byref(v))
+        return v.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                load().u10_itebd_free(self.h)
+                self.h = None
+        except Exception:
+            pass
