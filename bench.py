@@ -1,26 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark of the Uni10 contraction hot path on one MI355X.
 
-Workload (one "step"): the dense FP64 contraction-and-permute
-  A(256,2,2,256) labels (0,1,2,3)  x  B(256,2,2,256) labels (3,2,4,5)
-  -> contract over bonds {2,3} -> C(256,2,2,256) labels (0,1,4,5)
-  -> permute C to positions (3,1,0,2) (labels 5,1,0,4)
-with A, B ~ SplitMix64 uniform[-1,1) (seeds 1, 2), resident in HBM.  A step
-runs exactly the device work of the drop-in contract() + permute(): the
-permute of B to (shared | free), one 512x512x512 FP64 MFMA GEMM, the output
-permute.
+Headline workload (one "step"), FP64, through the drop-in C++ API:
+    A(256,2,2,256) labels (0,1,2,3), B(256,2,2,256) labels (3,2,4,5)
+    C = contract(A, B)                      -> (256,2,2,256) labels (0,1,4,5)
+    C.permute([5,1,0,4], 2)                 -> positions (3,1,0,2)
+A, B ~ SplitMix64 uniform[-1,1) (seeds 1, 2), resident in HBM before timing.
+The step is exactly what a Uni10 caller runs; the device work is one
+512x512x512 FP64 MFMA GEMM that reads A and B in their stored bond order and
+one tiled permute.
 
-Reported value: GEMM TFLOP/s of the whole step (2*512^3 flop / step time).
-Also reported: permute GB/s (16 B per element moved) at this shape and at an
-HBM-resident (4096,2,2,4096) shape, per-kernel durations (HIP events on the
-launch stream), the roofline of the dominant kernel, and the oracle's CPU
-time for the same step on the host cores.
+value = 2*512^3 flop / step time (TFLOP/s of the whole step).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+Also measured (same JSON line):
+  * per-kernel durations with HIP events on the launch stream, and the
+    roofline of the dominant kernel (GEMM: FP64 MFMA peak; permute: HBM);
+  * permute GB/s (16 B per element) at this shape and at an HBM-resident
+    (4096,2,2,4096) shape (512 MiB each way);
+  * iTEBD steps/s, spin-1/2 Heisenberg, chi=32, d=2, tau=0.01 (a step is
+    update(A,B) + update(B,A) through Networks, SVD and truncation);
+  * the oracle's CPU time for the same contraction step and iTEBD step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -30,13 +36,15 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+NETDIR = os.path.join(ROOT, "tests", "golden")
 
 HBM_PEAK_GBPS = 8000.0
 F64_MFMA_PEAK_TFLOPS = 78.6
+DIMS = (256, 2, 2, 256)
 
 
 def splitmix_uniform(seed, n):
-    """SplitMix64 -> uniform[-1,1) (same stream as the oracle's generator)."""
+    """SplitMix64 -> uniform[-1,1): x_i = mix(seed + i*0x9E3779B97F4A7C15), i >= 1."""
     M = np.uint64
     i = np.arange(1, n + 1, dtype=np.uint64)
     with np.errstate(over="ignore"):
@@ -47,48 +55,89 @@ def splitmix_uniform(seed, n):
     return 2.0 * ((z >> M(11)).astype(np.float64) * (2.0 ** -53)) - 1.0
 
 
-def gpu_step_parts(tn, dims=(256, 2, 2, 256)):
-    chi, d1, d2, chi2 = dims
+def make_tensors(u10):
+    n = int(np.prod(DIMS))
+    bonds = [u10.Bond(u10.BD_IN, DIMS[0]), u10.Bond(u10.BD_IN, DIMS[1]),
+             u10.Bond(u10.BD_OUT, DIMS[2]), u10.Bond(u10.BD_OUT, DIMS[3])]
+    A = u10.UniTensor(bonds, "A")
+    B = u10.UniTensor(bonds, "B")
+    A.setElem(splitmix_uniform(1, n))
+    B.setElem(splitmix_uniform(2, n))
+    A.setLabel([0, 1, 2, 3])
+    B.setLabel([3, 2, 4, 5])
+    return A, B
+
+
+def api_step(u10, A, B):
+    C = u10.contract(A, B)
+    C.permute([5, 1, 0, 4], 2)
+    return C
+
+
+def kernel_parts(tn):
+    """The device work of one step, issued directly on the C ABI with the same
+    plans the API builds (gather-addressed GEMM, tiled permute)."""
+    chi, d1, d2, chi2 = DIMS
     n = chi * d1 * d2 * chi2
     A = tn.DeviceArray.from_host(splitmix_uniform(1, n))
     B = tn.DeviceArray.from_host(splitmix_uniform(2, n))
-    Bp = tn.DeviceArray((n,))
     C = tn.DeviceArray((n,))
     D = tn.DeviceArray((n,))
-    M = chi * d1          # free bonds of A (labels 0,1)
-    K = d2 * chi2         # shared bonds (labels 2,3)
-    Nn = d1 * chi2        # free bonds of B (labels 4,5)
-    desc = np.array([(M, Nn, K, 0, 0, 0, K, Nn, Nn)], dtype=tn.GEMM_DESC)
-    plan = tn.GemmPlan(tn.TN_GEMM_F64, desc)
-
-    def permute_b(stream=None):
-        # B labels (3,2,4,5) -> (2,3,4,5): positions (1,0,2,3)
-        tn.permute_dense(B, Bp, dims, (1, 0, 2, 3), stream)
+    # A (0,1,2,3): rows (0,1) stride 1024/512... row-major: strides (1024,512,256,1)
+    sa = (d1 * d2 * chi2, d2 * chi2, chi2, 1)
+    # B labels (3,2,4,5): strides (1024,512,256,1); K = (label2, label3), N = (label4, label5)
+    sb = sa
+    rowA = np.array([i0 * sa[0] + i1 * sa[1] for i0 in range(chi) for i1 in range(d1)], np.int64)
+    colA = np.array([i2 * sa[2] + i3 * sa[3] for i2 in range(d2) for i3 in range(chi2)], np.int64)
+    rowB = np.array([l2 * sb[1] + l3 * sb[0] for l2 in range(d2) for l3 in range(chi)], np.int64)
+    colB = np.array([l4 * sb[2] + l5 * sb[3] for l4 in range(d1) for l5 in range(chi2)], np.int64)
+    tab = np.concatenate([rowA, colA, rowB, colB])
+    M, K, N = len(rowA), len(colA), len(colB)
+    desc = np.zeros(1, dtype=[(k, "<i8") for k in ("M", "N", "K", "a_off", "b_off", "c_off", "lda", "ldb",
+                                                  "ldc", "a_row", "a_col", "b_row", "b_col", "c_row",
+                                                  "c_col")])
+    desc[0] = (M, N, K, 0, 0, 0, K, N, N, 0, M, M + K, M + 2 * K, -1, -1)
+    h = ctypes.c_void_p()
+    tn.call("tn_gemm_plan_create_ex", ctypes.byref(h), tn.TN_GEMM_F64, 1, desc.ctypes.data_as(ctypes.c_void_p),
+            len(tab), tab.ctypes.data_as(ctypes.c_void_p))
+    plan = h.value
 
     def gemm(stream=None):
-        plan(A, Bp, C, stream)
+        tn.call("tn_gemm_exec", ctypes.c_void_p(plan), tn._p(A), tn._p(B), tn._p(C), stream)
 
-    def permute_c(stream=None):
-        tn.permute_dense(C, D, dims, (3, 1, 0, 2), stream)
+    def permute(stream=None):
+        tn.permute_dense(C, D, DIMS, (3, 1, 0, 2), stream)
 
-    return dict(A=A, B=B, Bp=Bp, C=C, D=D, plan=plan, parts=[("permute_b", permute_b),
-                                                                ("gemm_f64", gemm),
-                                                                ("permute_c", permute_c)],
-                flops=2.0 * M * Nn * K, n=n)
+    info_t, info_c = ctypes.c_int64(), ctypes.c_int()
+    tn.call("tn_gemm_plan_info", ctypes.c_void_p(plan), ctypes.byref(info_t), None, ctypes.byref(info_c))
+    keep = (A, B, C, D, tab, desc)
+    return [("gemm_f64", gemm), ("permute", permute)], 2.0 * M * N * K, n, keep, \
+        {"gemm_tiles": info_t.value, "gemm_tile_cfg": info_c.value}
 
 
 def time_parts(tn, parts, steps):
-    """Per-part average duration with HIP events on the launch stream."""
-    evs = {name: [] for name, _ in parts}
-    for _ in range(steps):
-        for name, fn in parts:
-            e0, e1 = tn.Event(), tn.Event()
-            e0.record()
+    """Average launch duration of each part: a HIP event is recorded just
+    before and just after every launch, on the launch stream.  A few large
+    fill kernels are queued first so the device is still busy while the host
+    enqueues the (event, kernel, event) triples: the intervals then hold the
+    kernel itself, not host enqueue gaps."""
+    spacer = tn.DeviceArray((256 * 1024 * 1024,))   # 2 GiB
+    out = {}
+    for name, fn in parts:
+        for _ in range(3):
             fn()
-            e1.record()
-            evs[name].append((e0, e1))
-    tn.sync()
-    return {name: float(np.mean([a.elapsed_ms(b) for a, b in pairs])) for name, pairs in evs.items()}
+        tn.sync()
+        evs = [(tn.Event(), tn.Event()) for _ in range(steps)]
+        for _ in range(max(4, steps // 25)):
+            tn.call("tn_zero", tn._p(spacer), spacer.nbytes, None)
+        for a, b in evs:
+            a.record()
+            fn()
+            b.record()
+        tn.sync()
+        out[name] = float(np.mean([a.elapsed_ms(b) for a, b in evs]))
+    spacer.free()
+    return out
 
 
 def permute_hbm(tn, steps):
@@ -110,47 +159,65 @@ def permute_hbm(tn, steps):
     return 16.0 * n / (ms * 1e-3) / 1e9, ms
 
 
-def cpu_baseline(budget_s=12.0, max_steps=400):
-    """The oracle's CPU step: odometer permutes (uni10_cpu.c, one thread) and
-    numpy dgemm (OpenBLAS) on the host cores."""
+def tebd_rate(u10, steps, chi=32):
+    t = u10.ITebd(chi, 2, 1.0, 0.01, 7, NETDIR)
+    t.step(5)
+    t0 = time.perf_counter()
+    t.step(steps)
+    e = t.energy()
+    dt = time.perf_counter() - t0
+    return steps / dt, e, t.trunc_err()
+
+
+def cpu_baseline(budget_s=10.0, max_steps=400):
+    """The oracle's contraction step on the host: C odometer permutes (one
+    thread) and numpy/OpenBLAS dgemm (BLAS threads = cores)."""
     from oracle import uni10_oracle as ora
     try:
         from threadpoolctl import threadpool_limits
     except Exception:  # pragma: no cover
         threadpool_limits = None
     cores = min(os.cpu_count() or 1, 16)
-    dims = np.array([256, 2, 2, 256], np.int64)
+    dims = np.array(DIMS, np.int64)
     n = int(np.prod(dims))
     A = splitmix_uniform(1, n)
     B = splitmix_uniform(2, n)
-    Bp = np.empty(n)
-    D = np.empty(n)
+    Bp, Bb, D = np.empty(n), np.empty(n), np.empty(n)
     L = ora.lib()
     pb = np.array([1, 0, 2, 3], np.int32)
     pc = np.array([3, 1, 0, 2], np.int32)
 
     def step():
+        # contract(): permute B to (shared | free), dgemm, permute B back; then permute C
         L.ora_permute_dense(ora._ptr(B), ora._ptr(Bp), 4, ora._ptr(dims), ora._ptr(pb))
         C = np.ascontiguousarray(A.reshape(512, 512) @ Bp.reshape(512, 512)).reshape(-1)
+        L.ora_permute_dense(ora._ptr(Bp), ora._ptr(Bb), 4, ora._ptr(dims), ora._ptr(pb))
         L.ora_permute_dense(ora._ptr(C), ora._ptr(D), 4, ora._ptr(dims), ora._ptr(pc))
-        return C
 
-    ctx = threadpool_limits(limits=cores) if threadpool_limits else None
-    try:
+    lim = threadpool_limits(limits=cores) if threadpool_limits else None
+    step()
+    t0 = time.perf_counter()
+    k = 0
+    while k < max_steps and time.perf_counter() - t0 < budget_s:
         step()
-        t0 = time.perf_counter()
-        k = 0
-        while k < max_steps and time.perf_counter() - t0 < budget_s:
-            step()
-            k += 1
-        dt = (time.perf_counter() - t0) / k
-    finally:
-        if ctx is not None:
-            ctx.unregister() if hasattr(ctx, "unregister") else None
-    flops = 2.0 * 512 ** 3
-    return {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
-            "sample": "%d contraction-and-permute steps (256,2,2,256) FP64: C odometer permutes "
-                      "(1 thread) + numpy/OpenBLAS dgemm (%d threads); %.2f ms/step" % (k, cores, dt * 1e3)}
+        k += 1
+    dt = (time.perf_counter() - t0) / k
+    if lim is not None and hasattr(lim, "unregister"):
+        lim.unregister()
+    # iTEBD on the oracle (chi=32), a short sample
+    t = ora.ITEBD(32, 2, ora.heisenberg_two_site(1.0), 0.01, 7, NETDIR)
+    t.step()
+    t1 = time.perf_counter()
+    ns = 0
+    while ns < 200 and time.perf_counter() - t1 < 5.0:
+        t.step()
+        ns += 1
+    tebd = ns / (time.perf_counter() - t1)
+    return {"value": 2.0 * 512 ** 3 / dt / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
+            "sample": "%d contraction-and-permute steps (256,2,2,256) FP64 in %.1f s: oracle C odometer "
+                      "permutes (1 thread) + numpy/OpenBLAS dgemm (%d threads), %.3f ms/step; oracle iTEBD "
+                      "chi=32: %d steps, %.1f steps/s" % (k, k * dt, cores, dt * 1e3, ns, tebd),
+            "ms_per_step": dt * 1e3, "tebd_steps_per_s": tebd}
 
 
 def main():
@@ -158,65 +225,67 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--tebd-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(independent chains/requests would shard by instance; see the design notes). "
-              "Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path in this build (independent "
+              "chains and requests would shard by instance across GPUs). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
-    from trim_amd import tn
+    from trim_amd import tn, uni10
     tn.init(0)
+    uni10.load()
     info = tn.device_info()
-    W = gpu_step_parts(tn)
-    parts = W["parts"]
 
-    def step():
-        for _, fn in parts:
-            fn()
-
+    # headline: the drop-in API step
+    A, B = make_tensors(uni10)
     for _ in range(args.warmup):
-        step()
-    tn.sync()
+        api_step(uni10, A, B)
+    tn.call("tn_device_sync")
     e0, e1 = tn.Event(), tn.Event()
     t0 = time.perf_counter()
     e0.record()
     for _ in range(args.steps):
-        step()
+        api_step(uni10, A, B)
     e1.record()
-    tn.sync()
+    tn.call("tn_device_sync")
     wall = time.perf_counter() - t0
     ms = e0.elapsed_ms(e1) / args.steps
-    value = W["flops"] / (ms * 1e-3) / 1e12
+    flops = 2.0 * 512 ** 3
+    value = flops / (ms * 1e-3) / 1e12
 
-    per = time_parts(tn, parts, max(10, min(args.steps, 100)))
-    gemm_ms = per["gemm_f64"]
-    gemm_tflops = W["flops"] / (gemm_ms * 1e-3) / 1e12
-    perm_ms = per["permute_c"]
-    perm_gbps = 16.0 * W["n"] / (perm_ms * 1e-3) / 1e9
+    parts, kflops, n, keep, ginfo = kernel_parts(tn)
+    per = time_parts(tn, parts, max(20, min(args.steps, 200)))
+    gemm_tf = kflops / (per["gemm_f64"] * 1e-3) / 1e12
+    perm_gbps = 16.0 * n / (per["permute"] * 1e-3) / 1e9
     hbm_gbps, hbm_ms = permute_hbm(tn, 10)
-    dominant = max(per, key=per.get)
-    if dominant == "gemm_f64":
-        roof = {"bound": "mfma", "achieved": gemm_tflops, "peak": F64_MFMA_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": gemm_tflops / F64_MFMA_PEAK_TFLOPS, "traffic": None,
-                "kernel": "gemm_mfma_16x16x4<double>", "avg_ms": gemm_ms}
+    if per["gemm_f64"] >= per["permute"]:
+        roof = {"bound": "mfma", "achieved": gemm_tf, "peak": F64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": gemm_tf / F64_MFMA_PEAK_TFLOPS, "traffic": None,
+                "kernel": "gemm_mfma_16x16x4<double,32,32,64>", "avg_ms": per["gemm_f64"],
+                "flop_per_launch": kflops}
     else:
-        a = 16.0 * W["n"] / (per[dominant] * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": a / HBM_PEAK_GBPS, "traffic": None, "kernel": dominant,
-                "avg_ms": per[dominant]}
+        roof = {"bound": "hbm", "achieved": perm_gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": perm_gbps / HBM_PEAK_GBPS, "traffic": None, "kernel": "tiled_permute_kernel",
+                "avg_ms": per["permute"], "bytes_per_launch": 16 * n}
+    tebd_sps, tebd_e, tebd_te = tebd_rate(uni10, args.tebd_steps)
     out = {
         "metric": "sector-GEMM TFLOP/s + permute GB/s; TEBD steps/s, p99 latency vs plain FP64",
         "value": value, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "dense FP64 contraction A(256,2,2,256) x B(256,2,2,256) over 2 bonds "
-                               "+ permute (3,1,0,2)", "gemm": "512x512x512", "parallelism": "1 GPU"},
+        "config": {"workload": "dense FP64 contract(A(256,2,2,256), B(256,2,2,256)) over 2 bonds + "
+                               "permute (3,1,0,2), drop-in API", "gemm": "512x512x512",
+                   "parallelism": "1 GPU"},
         "roofline": roof,
         "kernels_ms": per,
+        "gemm_plan": ginfo,
         "permute_GBps_config": perm_gbps,
         "permute_GBps_hbm_4096x2x2x4096": hbm_gbps,
         "permute_frac_hbm": hbm_gbps / HBM_PEAK_GBPS,
+        "tebd_chi32": {"steps_per_s": tebd_sps, "steps": args.tebd_steps, "energy_per_site": tebd_e,
+                       "trunc_err_last": tebd_te},
         "wall_s_timed": wall,
         "device": info,
     }

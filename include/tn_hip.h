@@ -62,6 +62,16 @@ int tn_event_destroy(void* ev);
 int tn_event_record(void* ev, void* stream);
 int tn_event_elapsed_ms(void* ev_start, void* ev_stop, float* ms);
 
+/* HIP graphs: capture every call made on `stream` between begin and end into
+ * an executable graph (no host round trips inside: calls that copy from or
+ * to host memory synchronise and are not capturable).  Replaying it issues
+ * the same kernels with no host gaps.  No memset nodes are recorded: the
+ * library clears memory with fill kernels. */
+int tn_graph_begin(void* stream);
+int tn_graph_end(void* stream, void** graph_exec);
+int tn_graph_launch(void* graph_exec, void* stream);
+int tn_graph_destroy(void* graph_exec);
+
 /* ---- permute / reshapeElem ----------------------------------------------
  * Dense (non-symmetric) permute.  Output axis j takes source axis perm[j]
  * (rsp_outin of UniTensorReal.cpp:545-558); dst has dims[perm[j]].
@@ -166,6 +176,9 @@ int tn_svd_f64_batched(int64_t batch, const int64_t* m, const int64_t* n,
                        double* U, const int64_t* u_off,
                        double* S, const int64_t* s_off,
                        double* VT, const int64_t* vt_off, void* stream);
+/* Jacobi sweeps used by the last SVD call (batch: the largest) on this
+ * thread; a convergence diagnostic. */
+int tn_svd_last_sweeps(int* sweeps);
 /* eigSyDecompose (uni10_lapack_cpu.cpp:131-152): symmetric N x N;
  * eigenvalues ascending; eigenvectors returned as ROWS of EigVec. */
 int tn_eigh_f64(const double* A, int64_t N, double* eig, double* eigvec_rows,

@@ -266,6 +266,21 @@ def test_svd_random(tn, ora, M, N):
     _svd_check(tn, ora, _rand(M * 1000 + N, (M, N)))
 
 
+@pytest.mark.parametrize("M,N,r", [(64, 64, 32), (128, 128, 64), (40, 90, 7)])
+def test_svd_rank_deficient(tn, ora, M, N, r):
+    """theta of an iTEBD update has rank <= chi < min(M, N): the non-zero
+    singular triplets must match LAPACK and the sweeps must terminate."""
+    rng = np.random.default_rng(M + N + r)
+    a = rng.uniform(-1, 1, (M, r)) @ rng.uniform(-1, 1, (r, N))
+    A = tn.DeviceArray.from_host(a)
+    u, s, vt = (x.to_host() for x in tn.svd(A, M, N))
+    ur, sr, vtr = ora.svd(a)
+    assert np.max(np.abs(s[:r] - sr[:r])) <= 1e-12 * sr[0]
+    assert np.max(np.abs(s[r:])) <= 1e-12 * sr[0]
+    assert np.linalg.norm(u[:, :r] @ np.diag(s[:r]) @ vt[:r] - a) <= 1e-12 * np.linalg.norm(a)
+    assert np.allclose(u[:, :r].T @ u[:, :r], np.eye(r), atol=1e-12)
+
+
 def test_svd_golden_matrix(tn, ora, golden):
     """Rebuild the 4x5 matrix from the reference's saved U, Sigma, VT dump and
     check the device SVD returns the dumped Sigma (descending) and factors."""

@@ -311,6 +311,41 @@ int tn_event_record(void* ev, void* stream) {
   return TN_OK;
 }
 
+int tn_graph_begin(void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_HIP_CHECK(hipStreamBeginCapture(stream_of(stream), hipStreamCaptureModeThreadLocal));
+  return TN_OK;
+}
+
+int tn_graph_end(void* stream, void** graph_exec) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(graph_exec != nullptr, TN_EINVAL, "tn_graph_end: NULL out pointer");
+  hipGraph_t g = nullptr;
+  TN_HIP_CHECK(hipStreamEndCapture(stream_of(stream), &g));
+  hipGraphExec_t ge = nullptr;
+  hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
+    return TN_EHIP;
+  }
+  *graph_exec = (void*)ge;
+  return TN_OK;
+}
+
+int tn_graph_launch(void* graph_exec, void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(graph_exec != nullptr, TN_EINVAL, "tn_graph_launch: NULL graph");
+  TN_HIP_CHECK(hipGraphLaunch((hipGraphExec_t)graph_exec, stream_of(stream)));
+  return TN_OK;
+}
+
+int tn_graph_destroy(void* graph_exec) {
+  if (!graph_exec) return TN_OK;
+  TN_HIP_CHECK(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+  return TN_OK;
+}
+
 int tn_event_elapsed_ms(void* a, void* b, float* ms) {
   TN_INIT_OR_RETURN();
   TN_HIP_CHECK(hipEventSynchronize((hipEvent_t)b));

@@ -25,6 +25,7 @@
 namespace {
 
 constexpr int kThreads = 256;
+thread_local int g_last_sweeps = 0;   // sweeps of the last SVD batch on this thread
 constexpr size_t kLdsBytes = 152 * 1024;   // 160 KiB per CU minus the kernel's static arrays
 
 struct SvdProb {
@@ -33,10 +34,17 @@ struct SvdProb {
   int64_t scratch_off;       // offset (doubles) into the scratch area, -1 if LDS
 };
 
-// Group-of-tp reduction (tp a power of two <= 64, groups aligned in a wave).
-__device__ inline double group_sum(double v, int tp) {
-  for (int o = tp >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+// Sum of a, b, c over each aligned group of TP lanes (TP a power of two),
+// the three butterflies interleaved so their shuffles overlap.
+template <int TP>
+__device__ inline void group_sum3(double& a, double& b, double& c) {
+#pragma unroll
+  for (int o = TP >> 1; o > 0; o >>= 1) {
+    const double x = __shfl_xor(a, o, 64), y = __shfl_xor(b, o, 64), z = __shfl_xor(c, o, 64);
+    a += x;
+    b += y;
+    c += z;
+  }
 }
 
 // Round-robin pairing: np even; round r in [0, np-1); pair k in [0, np/2).
@@ -56,6 +64,12 @@ __device__ inline void rr_pair(int r, int k, int np, int& p, int& q) {
   }
 }
 
+// kLds: W and V live in LDS (the compiler then emits ds_* accesses); otherwise
+// in the device scratch area.  The two cases are separate instantiations so
+// that no generic (flat) pointer reaches the inner loops.  TP lanes work on
+// one column pair (few lanes per pair: short reductions, which bound the
+// latency of a round).
+template <bool kLds, int TP>
 __global__ void __launch_bounds__(kThreads) svd_jacobi_kernel(const SvdProb* __restrict__ probs,
                                                               const double* __restrict__ A,
                                                               double* __restrict__ U,
@@ -66,35 +80,62 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_kernel(const SvdProb* __r
   extern __shared__ double lds[];
   __shared__ double sig[512];
   __shared__ int rank_of[512];
+  __shared__ double red[kThreads];
   const SvdProb pr = probs[blockIdx.x];
   const int64_t M = pr.m, N = pr.n;
   const bool tr = (M < N);           // work on A^T
   const int m = (int)(tr ? N : M);   // column length
   const int n = (int)(tr ? M : N);   // number of columns
   if (m == 0 || n == 0) return;
-  double* W = (pr.scratch_off < 0) ? lds : scratch + pr.scratch_off;
-  double* V = W + (size_t)m * n;
+  // Column strides are padded by one double so that the columns handled by
+  // different pair groups in a round start on different LDS banks.
+  const int ldw = m + 1, ldv = n + 1;
+  double* W = kLds ? lds : scratch + pr.scratch_off;
+  double* V = W + (size_t)ldw * n;
   const int tid = threadIdx.x;
   const double* Ab = A + pr.a_off;
 
-  // W column-major: W[j*m + i]
+  // W column-major: W[j*ldw + i]
   for (int64_t e = tid; e < (int64_t)m * n; e += kThreads) {
     int j = (int)(e / m), i = (int)(e % m);
-    W[e] = tr ? Ab[(int64_t)j * N + i] : Ab[(int64_t)i * N + j];
+    W[(size_t)j * ldw + i] = tr ? Ab[(int64_t)j * N + i] : Ab[(int64_t)i * N + j];
   }
   for (int64_t e = tid; e < (int64_t)n * n; e += kThreads) {
     int j = (int)(e / n), i = (int)(e % n);
-    V[e] = (i == j) ? 1.0 : 0.0;
+    V[(size_t)j * ldv + i] = (i == j) ? 1.0 : 0.0;
   }
+  // ||A||_F^2 (fixed-order tree), invariant under the rotations
+  {
+    double s = 0;
+    for (int64_t e = tid; e < (int64_t)m * n; e += kThreads) {
+      int j = (int)(e / m), i = (int)(e % m);
+      double x = W[(size_t)j * ldw + i];
+      s += x * x;
+    }
+    red[tid] = s;
+  }
+  __syncthreads();
+  for (int o = kThreads / 2; o > 0; o >>= 1) {
+    if (tid < o) red[tid] += red[tid + o];
+    __syncthreads();
+  }
+  const double normF2 = red[0];
   __syncthreads();
 
   const int np = (n % 2) ? n + 1 : n;
   const int npairs = np / 2;
-  int tp = 1;
-  while (tp * 2 * npairs <= kThreads && tp < 64) tp *= 2;
-  const int groups = kThreads / tp;
+  constexpr int tp = TP;
+  constexpr int groups = kThreads / TP;
   const int g = tid / tp, t = tid % tp;
-  const double tol = DBL_EPSILON * sqrt((double)m);
+  // Rotate a pair while |w_p.w_q| > tol |w_p||w_q|.  The computed dot product
+  // of orthogonal columns carries rounding noise of order sqrt(m) eps, so the
+  // threshold sits above it (10 m eps) or sweeps never stop; singular values
+  // are then exact to O(tol^2) relative and vectors to O(tol).
+  const double tol = 10.0 * DBL_EPSILON * (double)m;
+  // Columns at rounding level (|w| ~ eps ||A||_F, e.g. the null space of a
+  // rank-deficient theta) have dot products that are pure noise: a pair is
+  // left alone when |w_p.w_q| is below m eps^2 ||A||_F^2.
+  const double abs_tol = (double)m * DBL_EPSILON * DBL_EPSILON * normF2;
 
   int sweep = 0;
   const int max_sweeps = 60;
@@ -112,8 +153,9 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_kernel(const SvdProb* __r
         }
         double a = 0, b = 0, c = 0;
         if (live) {
-          const double* wp = W + (size_t)p * m;
-          const double* wq = W + (size_t)q * m;
+          const double* wp = W + (size_t)p * ldw;
+          const double* wq = W + (size_t)q * ldw;
+#pragma unroll 4
           for (int i = t; i < m; i += tp) {
             double x = wp[i], y = wq[i];
             a += x * x;
@@ -121,24 +163,24 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_kernel(const SvdProb* __r
             c += x * y;
           }
         }
-        a = group_sum(a, tp);
-        b = group_sum(b, tp);
-        c = group_sum(c, tp);
-        if (live && fabs(c) > tol * sqrt(a * b) && c != 0.0) {
+        group_sum3<TP>(a, b, c);
+        if (live && fabs(c) > tol * sqrt(a * b) && fabs(c) > abs_tol) {
           rotated = 1;
           const double zeta = (b - a) / (2.0 * c);
           const double tt = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
           const double cs = 1.0 / sqrt(1.0 + tt * tt);
           const double sn = cs * tt;
-          double* wp = W + (size_t)p * m;
-          double* wq = W + (size_t)q * m;
+          double* wp = W + (size_t)p * ldw;
+          double* wq = W + (size_t)q * ldw;
+#pragma unroll 4
           for (int i = t; i < m; i += tp) {
             double x = wp[i], y = wq[i];
             wp[i] = cs * x - sn * y;
             wq[i] = sn * x + cs * y;
           }
-          double* vp = V + (size_t)p * n;
-          double* vq = V + (size_t)q * n;
+          double* vp = V + (size_t)p * ldv;
+          double* vq = V + (size_t)q * ldv;
+#pragma unroll 4
           for (int i = t; i < n; i += tp) {
             double x = vp[i], y = vq[i];
             vp[i] = cs * x - sn * y;
@@ -150,11 +192,14 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_kernel(const SvdProb* __r
     }
     if (!__syncthreads_or(rotated)) break;
   }
-  if (tid == 0 && sweep >= max_sweeps) atomicAdd(status, 1);
+  if (tid == 0) {
+    if (sweep >= max_sweeps) atomicAdd(status, 1);
+    atomicMax(status + 1, sweep + 1);   // sweeps run (diagnostic)
+  }
 
   // singular values = column norms (fixed order per column)
   for (int j = tid; j < n; j += kThreads) {
-    const double* w = W + (size_t)j * m;
+    const double* w = W + (size_t)j * ldw;
     double s = 0;
     for (int i = 0; i < m; ++i) s += w[i] * w[i];
     sig[j] = sqrt(s);
@@ -178,7 +223,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_kernel(const SvdProb* __r
   for (int64_t e = tid; e < (int64_t)m * n; e += kThreads) {
     int j = (int)(e / m), i = (int)(e % m);
     double sj = sig[j];
-    double v = (sj > 0.0) ? W[e] / sj : 0.0;
+    double v = (sj > 0.0) ? W[(size_t)j * ldw + i] / sj : 0.0;
     int rk = rank_of[j];
     if (!tr)
       Ub[(int64_t)i * n + rk] = v;       // U is M x n (n = N)
@@ -188,10 +233,11 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_kernel(const SvdProb* __r
   for (int64_t e = tid; e < (int64_t)n * n; e += kThreads) {
     int j = (int)(e / n), i = (int)(e % n);
     int rk = rank_of[j];
+    const double vv = V[(size_t)j * ldv + i];
     if (!tr)
-      VTb[(int64_t)rk * n + i] = V[e];   // VT is n x n
+      VTb[(int64_t)rk * n + i] = vv;     // VT is n x n
     else
-      Ub[(int64_t)i * n + rk] = V[e];    // U is M x M
+      Ub[(int64_t)i * n + rk] = vv;      // U is M x M
   }
 }
 
@@ -398,21 +444,21 @@ int svd_batched(int64_t batch, const int64_t* m, const int64_t* n, const double*
                (long long)nn);
     SvdProb p{m[i], n[i], a_off ? a_off[i] : 0, u_off ? u_off[i] : 0, s_off ? s_off[i] : 0,
               vt_off ? vt_off[i] : 0, -1};
-    size_t bytes = sizeof(double) * (size_t)(mm * nn + nn * nn);
+    size_t bytes = sizeof(double) * (size_t)((mm + 1) * nn + (nn + 1) * nn);
     if (bytes <= kLdsBytes) {
       lds_need = std::max(lds_need, bytes);
       lds_probs.push_back(p);
     } else {
       p.scratch_off = scratch_total;
-      scratch_total += mm * nn + nn * nn;
+      scratch_total += (mm + 1) * nn + (nn + 1) * nn;
       glb_probs.push_back(p);
     }
   }
   int* d_status = nullptr;
   SvdProb* d_p = nullptr;
   double* d_scratch = nullptr;
-  int rc = tn::pool_alloc((void**)&d_status, sizeof(int));
-  if (rc == TN_OK) rc = tn_zero(d_status, sizeof(int), st);
+  int rc = tn::pool_alloc((void**)&d_status, 2 * sizeof(int));
+  if (rc == TN_OK) rc = tn_zero(d_status, 2 * sizeof(int), st);
   const size_t np = lds_probs.size() + glb_probs.size();
   if (rc == TN_OK) rc = tn::pool_alloc((void**)&d_p, sizeof(SvdProb) * np);
   std::vector<SvdProb> all(lds_probs);
@@ -420,45 +466,58 @@ int svd_batched(int64_t batch, const int64_t* m, const int64_t* n, const double*
   if (rc == TN_OK) rc = tn_memcpy_h2d(d_p, all.data(), sizeof(SvdProb) * np, st);
   if (rc == TN_OK && scratch_total > 0)
     rc = tn::pool_alloc((void**)&d_scratch, sizeof(double) * scratch_total);
-  if (rc == TN_OK && !lds_probs.empty()) {
-    static bool attr_set = false;
-    if (!attr_set) {
-      (void)hipFuncSetAttribute((const void*)svd_jacobi_kernel,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes);
+  // lanes per column pair: as few as the widest problem allows with one
+  // pass over the pairs of a round (8 lanes up to 64 columns)
+  auto pick_tp = [](const std::vector<SvdProb>& v) {
+    int64_t mx = 0;
+    for (auto& p : v) mx = std::max(mx, std::min(p.m, p.n));
+    int64_t pairs = (mx + 1) / 2;
+    return pairs <= 32 ? 8 : (pairs <= 64 ? 4 : 2);
+  };
+  auto launch = [&](bool in_lds, int tp, const SvdProb* probs, size_t count, size_t lds_bytes) {
+    typedef void (*K)(const SvdProb*, const double*, double*, double*, double*, double*, int*);
+    K k = nullptr;
+    if (in_lds)
+      k = (tp == 8) ? svd_jacobi_kernel<true, 8> : (tp == 4) ? svd_jacobi_kernel<true, 4> : svd_jacobi_kernel<true, 2>;
+    else
+      k = (tp == 8) ? svd_jacobi_kernel<false, 8> : (tp == 4) ? svd_jacobi_kernel<false, 4> : svd_jacobi_kernel<false, 2>;
+    if (in_lds) {
+      (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes);
       (void)hipGetLastError();
-      attr_set = true;
     }
-    hipLaunchKernelGGL(svd_jacobi_kernel, dim3((unsigned)lds_probs.size()), dim3(kThreads),
-                       lds_need, st, d_p, A, U, S, VT, d_scratch, d_status);
+    hipLaunchKernelGGL(k, dim3((unsigned)count), dim3(kThreads), lds_bytes, st, probs, A, U, S, VT, d_scratch,
+                       d_status);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
       tn::set_error("svd kernel launch: %s", hipGetErrorString(e));
-      rc = TN_EHIP;
+      return (int)TN_EHIP;
     }
-  }
-  if (rc == TN_OK && !glb_probs.empty()) {
-    hipLaunchKernelGGL(svd_jacobi_kernel, dim3((unsigned)glb_probs.size()), dim3(kThreads), 0, st,
-                       d_p + lds_probs.size(), A, U, S, VT, d_scratch, d_status);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-      tn::set_error("svd kernel launch: %s", hipGetErrorString(e));
-      rc = TN_EHIP;
-    }
-  }
-  int h_status = 0;
-  if (rc == TN_OK) rc = tn_memcpy_d2h(&h_status, d_status, sizeof(int), st);  // synchronises
+    return (int)TN_OK;
+  };
+  if (rc == TN_OK && !lds_probs.empty()) rc = launch(true, pick_tp(lds_probs), d_p, lds_probs.size(), lds_need);
+  if (rc == TN_OK && !glb_probs.empty())
+    rc = launch(false, pick_tp(glb_probs), d_p + lds_probs.size(), glb_probs.size(), 0);
+  int h_status[2] = {0, 0};
+  if (rc == TN_OK) rc = tn_memcpy_d2h(h_status, d_status, 2 * sizeof(int), st);  // synchronises
   tn::pool_free(d_status);
   tn::pool_free(d_p);
   tn::pool_free(d_scratch);
   if (rc != TN_OK) return rc;
-  TN_REQUIRE(h_status == 0, TN_ENOCONV, "svd: %d problem(s) did not converge in 60 sweeps",
-             h_status);
+  g_last_sweeps = h_status[1];
+  TN_REQUIRE(h_status[0] == 0, TN_ENOCONV, "svd: %d problem(s) did not converge in 60 sweeps",
+             h_status[0]);
   return TN_OK;
 }
 
 }  // namespace
 
 extern "C" {
+
+int tn_svd_last_sweeps(int* sweeps) {
+  TN_REQUIRE(sweeps != nullptr, TN_EINVAL, "tn_svd_last_sweeps: NULL out pointer");
+  *sweeps = g_last_sweeps;
+  return TN_OK;
+}
 
 int tn_svd_f64(const double* A, int64_t M, int64_t N, double* U, double* S, double* VT,
                void* stream) {

@@ -44,6 +44,10 @@ SIGNATURES = {
     "tn_event_destroy": [P],
     "tn_event_record": [P, P],
     "tn_event_elapsed_ms": [P, P, ctypes.POINTER(ctypes.c_float)],
+    "tn_graph_begin": [P],
+    "tn_graph_end": [P, ctypes.POINTER(ctypes.c_void_p)],
+    "tn_graph_launch": [P, P],
+    "tn_graph_destroy": [P],
     "tn_permute_dense_f64": [P, P, ctypes.c_int, P, P, P],
     "tn_reshape_elem_f64": [P, ctypes.c_int, ctypes.c_size_t, P, P, P],
     "tn_blockperm_plan_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, I64, P, P, P, P, I64],
@@ -64,6 +68,7 @@ SIGNATURES = {
     "tn_diag_col_mul_f64": [P, P, I64, I64, P],
     "tn_svd_f64": [P, I64, I64, P, P, P, P],
     "tn_svd_f64_batched": [I64, P, P, P, P, P, P, P, P, P, P, P],
+    "tn_svd_last_sweeps": [ctypes.POINTER(ctypes.c_int)],
     "tn_eigh_f64": [P, I64, P, P, P],
     "tn_truncate_f64": [P, I64, I64, P, P, P, P],
 }
@@ -265,6 +270,38 @@ def eigh(A: DeviceArray, N: int):
     v = DeviceArray((N, N))
     call("tn_eigh_f64", _p(A), N, _p(w), _p(v), None)
     return w, v
+
+
+class Graph:
+    """Captures the library calls made by fn() on `stream` into a HIP graph."""
+
+    def __init__(self, fn, stream=None):
+        self.stream = stream
+        call("tn_graph_begin", stream)
+        try:
+            fn()
+        finally:
+            h = ctypes.c_void_p()
+            rc = load().tn_graph_end(stream, ctypes.byref(h))
+        check(rc, "tn_graph_end")
+        self.h = h.value
+
+    def launch(self):
+        call("tn_graph_launch", ctypes.c_void_p(self.h), self.stream)
+
+    def __del__(self):
+        try:
+            if self.h:
+                load().tn_graph_destroy(ctypes.c_void_p(self.h))
+                self.h = 0
+        except Exception:
+            pass
+
+
+def stream_create():
+    h = ctypes.c_void_p()
+    call("tn_stream_create", ctypes.byref(h))
+    return h.value
 
 
 class Event:
