@@ -83,36 +83,30 @@ def kernel_parts(tn):
     B = tn.DeviceArray.from_host(splitmix_uniform(2, n))
     C = tn.DeviceArray((n,))
     D = tn.DeviceArray((n,))
-    # A (0,1,2,3): rows (0,1) stride 1024/512... row-major: strides (1024,512,256,1)
-    sa = (d1 * d2 * chi2, d2 * chi2, chi2, 1)
-    # B labels (3,2,4,5): strides (1024,512,256,1); K = (label2, label3), N = (label4, label5)
-    sb = sa
-    rowA = np.array([i0 * sa[0] + i1 * sa[1] for i0 in range(chi) for i1 in range(d1)], np.int64)
-    colA = np.array([i2 * sa[2] + i3 * sa[3] for i2 in range(d2) for i3 in range(chi2)], np.int64)
-    rowB = np.array([l2 * sb[1] + l3 * sb[0] for l2 in range(d2) for l3 in range(chi)], np.int64)
-    colB = np.array([l4 * sb[2] + l5 * sb[3] for l4 in range(d1) for l5 in range(chi2)], np.int64)
-    tab = np.concatenate([rowA, colA, rowB, colB])
-    M, K, N = len(rowA), len(colA), len(colB)
-    desc = np.zeros(1, dtype=[(k, "<i8") for k in ("M", "N", "K", "a_off", "b_off", "c_off", "lda", "ldb",
-                                                  "ldc", "a_row", "a_col", "b_row", "b_col", "c_row",
-                                                  "c_col")])
-    desc[0] = (M, N, K, 0, 0, 0, K, N, N, 0, M, M + K, M + 2 * K, -1, -1)
-    h = ctypes.c_void_p()
-    tn.call("tn_gemm_plan_create_ex", ctypes.byref(h), tn.TN_GEMM_F64, 1, desc.ctypes.data_as(ctypes.c_void_p),
-            len(tab), tab.ctypes.data_as(ctypes.c_void_p))
-    plan = h.value
+    # A labels (0,1,2,3), strides (1024,512,256,1): rows (0,1) and K (2,3)
+    # are each one contiguous level.  B labels (3,2,4,5), same strides:
+    # K = (label 2, label 3) -> levels (2, 512), (256, 1024); N = (4,5) contiguous.
+    M, K, N = chi * d1, d2 * chi2, d1 * chi2
+    desc = np.zeros(1, dtype=tn.GEMM_DESC_EX)
+    desc["M"], desc["N"], desc["K"] = M, N, K
+    desc["lda"], desc["ldb"], desc["ldc"] = K, N, N
+    desc["a_row"] = tn.index_map([(M, K)])
+    desc["a_col"] = tn.index_map([(K, 1)])
+    desc["b_row"] = tn.index_map([(d2, d1 * chi2), (chi, d2 * d1 * chi2)])
+    desc["b_col"] = tn.index_map([(N, 1)])
+    plan = tn.GemmPlan(tn.TN_GEMM_F64, desc)
+    plan.autotune(A, B, C, reps=3)
 
     def gemm(stream=None):
-        tn.call("tn_gemm_exec", ctypes.c_void_p(plan), tn._p(A), tn._p(B), tn._p(C), stream)
+        plan(A, B, C, stream)
 
     def permute(stream=None):
         tn.permute_dense(C, D, DIMS, (3, 1, 0, 2), stream)
 
-    info_t, info_c = ctypes.c_int64(), ctypes.c_int()
-    tn.call("tn_gemm_plan_info", ctypes.c_void_p(plan), ctypes.byref(info_t), None, ctypes.byref(info_c))
-    keep = (A, B, C, D, tab, desc)
+    info = plan.info()
+    keep = (A, B, C, D, desc, plan)
     return [("gemm_f64", gemm), ("permute", permute)], 2.0 * M * N * K, n, keep, \
-        {"gemm_tiles": info_t.value, "gemm_tile_cfg": info_c.value}
+        {"gemm_tiles": info["tiles"], "gemm_tile_cfg": info["tile_id"]}
 
 
 def time_parts(tn, parts, steps):
@@ -157,6 +151,56 @@ def permute_hbm(tn, steps):
     src.free()
     dst.free()
     return 16.0 * n / (ms * 1e-3) / 1e9, ms
+
+
+def c3_u1(tn, u10, steps=10):
+    """U(1) two-site contraction, chi=1024 over Sz in {-8..8} (binomial sector
+    dims), d=2: Theta = (A1 . A2) . G through the drop-in API in every GEMM
+    mode, plus the sector GEMM of A1 . A2 alone as one grouped launch."""
+    from trim_amd import workloads
+    A1, A2, G, dims = workloads.u1_two_site(u10, 1024, 8, 11, 0.01)
+    res = {"sector_dims": dims}
+    ref = None
+    modes = (("f64", 0), ("f32", 1), ("bf16x3", 2), ("bf16x6", 3))
+    for name, mode in modes:
+        u10.set_gemm_mode(mode)
+        R = u10.contract(u10.contract(A1, A2), G)      # plans + autotune
+        tn.call("tn_device_sync")
+        e0, e1 = tn.Event(), tn.Event()
+        e0.record()
+        for _ in range(steps):
+            R = u10.contract(u10.contract(A1, A2), G)
+        e1.record()
+        ms = e0.elapsed_ms(e1) / steps
+        out = R.getElem()
+        if ref is None:
+            ref = out
+        err = float(np.linalg.norm(out - ref) / np.linalg.norm(ref))
+        res[name] = {"ms_per_contraction": ms, "rel_frobenius_err_vs_f64": err}
+    u10.set_gemm_mode(0)
+    # the sector GEMM of A1 . A2 as one grouped launch
+    A1p = A1.copy().permute([-1, 10, 0], 2)
+    A2p = A2.copy().permute([0, 11, -2], 1)
+    b1 = {q.triple(): (r, c, o) for q, r, c, o in A1p.blocks()}
+    b2 = {q.triple(): (r, c, o) for q, r, c, o in A2p.blocks()}
+    descs, oc = [], 0
+    for q, (r1, c1, o1) in b1.items():
+        if q in b2:
+            r2, c2, o2 = b2[q]
+            descs.append((r1, c2, c1, o1, o2, oc, max(c1, 1), max(c2, 1), max(c2, 1)))
+            oc += r1 * c2
+    C = tn.DeviceArray((oc,))
+    pa, pb = A1p.device_ptr(), A2p.device_ptr()
+    sect = {"sectors": len(descs)}
+    for name, mode in modes:
+        plan = tn.GemmPlan(mode, np.array(descs, dtype=tn.GEMM_DESC))
+        plan.autotune(pa, pb, C, reps=3)
+        info = plan.info()
+        t = time_parts(tn, [(name, lambda: plan(pa, pb, C))], 50)[name]
+        sect[name] = {"kernel_ms": t, "tflops": info["flops"] / (t * 1e-3) / 1e12, "tiles": info["tiles"],
+                      "tile_cfg": info["tile_id"], "flop": info["flops"]}
+    res["sector_gemm_A1A2"] = sect
+    return res
 
 
 def tebd_rate(u10, steps, chi=32):
@@ -270,6 +314,7 @@ def main():
                 "frac": perm_gbps / HBM_PEAK_GBPS, "traffic": None, "kernel": "tiled_permute_kernel",
                 "avg_ms": per["permute"], "bytes_per_launch": 16 * n}
     tebd_sps, tebd_e, tebd_te = tebd_rate(uni10, args.tebd_steps)
+    c3 = c3_u1(tn, uni10)
     out = {
         "metric": "sector-GEMM TFLOP/s + permute GB/s; TEBD steps/s, p99 latency vs plain FP64",
         "value": value, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -286,6 +331,7 @@ def main():
         "permute_frac_hbm": hbm_gbps / HBM_PEAK_GBPS,
         "tebd_chi32": {"steps_per_s": tebd_sps, "steps": args.tebd_steps, "energy_per_site": tebd_e,
                        "trunc_err_last": tebd_te},
+        "c3_u1_chi1024": c3,
         "wall_s_timed": wall,
         "device": info,
     }

@@ -129,22 +129,38 @@ typedef struct {
 typedef struct tn_gemm_plan_s* tn_gemm_plan;
 int tn_gemm_plan_create(tn_gemm_plan* plan, int mode, int64_t ngroups,
                         const tn_gemm_desc* descs);
-/* Gather-addressed operands: a group member may address an operand through
- * offset tables instead of a leading dimension, so a contraction reads its
- * inputs (and writes its output) in any bond order without permuting them:
- *   A(m,k) = A[a_off + tab[a_row+m] + tab[a_col+k]]   (a_row >= 0)
- *          = A[a_off + m*lda + k]                       (a_row <  0)
- * and likewise B(k,n) with b_row/b_col and C(m,n) with c_row/c_col.
- * tab holds ntab element offsets (int64), uploaded with the plan. */
+/* Multi-stride operand addressing, so a contraction reads its inputs (and
+ * writes its output) in any bond order without permuting them.  An index map
+ * sends a row or column index i of an operand to an element offset: with
+ * n >= 1 levels, i is split into digits over ext[0..n) (last level fastest)
+ * and offset(i) = sum_l digit_l * stride[l]; the product of ext[] must equal
+ * the index range (M, N or K).  n = 0 means the plain layout: row i ->
+ * i * ld, column i -> i.  Indices must stay below 2^31.
+ *   A(m,k) = A[a_off + a_row(m) + a_col(k)]
+ *   B(k,n) = B[b_off + b_row(k) + b_col(n)]
+ *   C(m,n) = C[c_off + c_row(m) + c_col(n)]                                  */
+#define TN_MAX_MAP_LEVELS 4
+typedef struct {
+  int32_t n;
+  int32_t pad;
+  int64_t ext[TN_MAX_MAP_LEVELS];
+  int64_t stride[TN_MAX_MAP_LEVELS];
+} tn_index_map;
 typedef struct {
   int64_t M, N, K;
   int64_t a_off, b_off, c_off;
   int64_t lda, ldb, ldc;
-  int64_t a_row, a_col, b_row, b_col, c_row, c_col;
+  tn_index_map a_row, a_col, b_row, b_col, c_row, c_col;
 } tn_gemm_desc_ex;
 int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups,
-                           const tn_gemm_desc_ex* descs, int64_t ntab, const int64_t* tab);
+                           const tn_gemm_desc_ex* descs);
 int tn_gemm_plan_set_tile(tn_gemm_plan plan, int tile_id); /* autotuner hook  */
+/* Autotuner: runs the plan with every tile shape on the given operands
+ * (reps timed launches each, HIP events on `stream`), keeps the fastest and
+ * reports its time.  Every tile shape accumulates K in the same order, so
+ * the choice never changes the results.  C is overwritten. */
+int tn_gemm_plan_autotune(tn_gemm_plan plan, const double* A, const double* B, double* C,
+                          int reps, void* stream, float* best_ms);
 int tn_gemm_plan_info(tn_gemm_plan plan, int64_t* tiles, double* flops, int* tile_id);
 int tn_gemm_exec(tn_gemm_plan plan, const double* A, const double* B, double* C,
                  void* stream);

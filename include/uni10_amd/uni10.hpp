@@ -35,6 +35,7 @@ enum parityFType { PRTF_EVEN = 0, PRTF_ODD = 1 };
 enum GemmMode { GEMM_F64 = 0, GEMM_F32 = 1, GEMM_BF16X3 = 2, GEMM_BF16X6 = 3 };
 void setGemmMode(GemmMode m);          // thread-local; default GEMM_F64
 GemmMode gemmMode();
+void setAutotune(bool on);             // thread-local; time tile shapes on first use (default on)
 void setStream(void* stream);          // thread-local; NULL = library default
 void* currentStream();
 void synchronize();                    // wait for the current stream

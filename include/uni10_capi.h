@@ -26,6 +26,7 @@ typedef struct u10_itebd_s* u10_itebd;
 const char* u10_last_error(void);
 int u10_set_gemm_mode(int mode);          /* GemmMode of uni10.hpp (thread-local) */
 int u10_set_stream(void* stream);         /* thread-local current stream          */
+int u10_set_autotune(int on);             /* tile autotuning of new GEMM plans    */
 
 /* UniTensor(bonds, name): bond b has type types[b] (1 = BD_IN, -1 = BD_OUT)
  * and nruns[b] runs; run r of all bonds, in order, has charge

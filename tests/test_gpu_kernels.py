@@ -236,6 +236,63 @@ def test_gemm_grouped_uneven(tn, mode, tile):
     _check_gemm(tn, mode, shapes, seed=mode * 10 + tile, tile=tile)
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_gemm_tile_choice_is_bitwise_neutral(tn, mode):
+    """Every tile shape (and so the autotuner's choice) gives identical bits:
+    each output element accumulates K in the same order."""
+    shapes = [(200, 130, 300), (17, 33, 5), (64, 64, 640)]
+    descs, A, B, nc = _group_case(77, shapes)
+    plan = tn.GemmPlan(mode, np.array(descs, dtype=tn.GEMM_DESC))
+    dA, dB = tn.DeviceArray.from_host(A), tn.DeviceArray.from_host(B)
+    outs = []
+    for tile in range(5):
+        plan.set_tile(tile)
+        dC = tn.DeviceArray((nc,))
+        plan(dA, dB, dC)
+        outs.append(dC.to_host().tobytes())
+    assert all(o == outs[0] for o in outs)
+    dC = tn.DeviceArray((nc,))
+    ms = plan.autotune(dA, dB, dC, reps=2)
+    assert ms > 0 and dC.to_host().tobytes() == outs[0]
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_gemm_index_maps(tn, mode):
+    """Multi-stride operands equal the same GEMM on explicitly permuted
+    copies: A read as the (0,2)x(1,3) matricisation of a rank-4 tensor, B
+    read transposed, C written as the transpose of a (5,N) x (M/5) layout."""
+    rng = np.random.default_rng(5 + mode)
+    d = (6, 5, 7, 4)
+    a = rng.uniform(-1, 1, d)
+    M, K, N = 6 * 7, 5 * 4, 9
+    bt = rng.uniform(-1, 1, (N, K))           # stored as B^T
+    st = [5 * 7 * 4, 7 * 4, 4, 1]
+    desc = np.zeros(1, dtype=tn.GEMM_DESC_EX)
+    desc["M"], desc["N"], desc["K"] = M, N, K
+    desc["lda"], desc["ldb"], desc["ldc"] = K, N, N
+    desc["a_row"] = tn.index_map([(6, st[0]), (7, st[2])])
+    desc["a_col"] = tn.index_map([(5, st[1]), (4, st[3])])
+    desc["b_row"] = tn.index_map([(K, 1)])
+    desc["b_col"] = tn.index_map([(N, K)])
+    desc["c_row"] = tn.index_map([(M, 1)])     # C stored transposed (N x M)
+    desc["c_col"] = tn.index_map([(N, M)])
+    plan = tn.GemmPlan(mode, desc)
+    dA, dB, dC = tn.DeviceArray.from_host(a), tn.DeviceArray.from_host(bt), tn.DeviceArray((N, M))
+    plan(dA, dB, dC)
+    ref = (np.transpose(a, (0, 2, 1, 3)).reshape(M, K) @ bt.T).T
+    assert np.linalg.norm(dC.to_host() - ref) <= GEMM_TOL[mode] * np.linalg.norm(ref)
+
+
+def test_gemm_index_map_rejects_bad_extent(tn):
+    desc = np.zeros(1, dtype=tn.GEMM_DESC_EX)
+    desc["M"], desc["N"], desc["K"] = 6, 4, 3
+    desc["lda"], desc["ldb"], desc["ldc"] = 3, 4, 4
+    desc["a_row"] = tn.index_map([(2, 1), (2, 2)])   # covers 4 rows, M is 6
+    desc["a_col"] = tn.index_map([(3, 1)])
+    with pytest.raises(tn.TNError):
+        tn.GemmPlan(0, desc)
+
+
 # ---------------------------------------------------------------------------
 # SVD / eigh
 def _svd_check(tn, ora, a, tol=1e-10):

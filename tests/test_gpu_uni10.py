@@ -268,6 +268,45 @@ def test_network_greedy_plan_matches_oracle(u10, ora, golden):
     assert np.max(np.abs(Cp2.getElem() - Co.elem)) <= 1e-14
 
 
+MODE_TOL = {0: 1e-12, 1: 2e-6, 2: 5e-5, 3: 2e-6}
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_u1_two_site_contraction_modes(u10, ora, mode):
+    """Theta = (A1 . A2) . G for U(1) tensors (chi=96 over Sz in {-4..4},
+    binomial sectors): FP64 within 1e-12 of the oracle; FP32-MFMA and
+    split-BF16 within their stated tolerances."""
+    from trim_amd import workloads
+    A1, A2, G, dims = workloads.u1_two_site(u10, 96, 4, 21, 0.01)
+    Q = ora.Qnum
+    charges = list(range(-4, 5))
+    def ob(tp, ch, ds):
+        qs = []
+        for c, d in zip(ch, ds):
+            qs += [Q(c)] * d
+        return ora.Bond(tp, qs)
+    ci, co = ob(ora.BD_IN, charges, dims), ob(ora.BD_OUT, charges, dims)
+    po, pi = ob(ora.BD_OUT, [1, -1], [1, 1]), ob(ora.BD_IN, [1, -1], [1, 1])
+    oA1 = ora.UniTensor([ci, po, co], elem=workloads.splitmix_uniform(21, A1.elemNum()))
+    oA2 = ora.UniTensor([ci, po, co], elem=workloads.splitmix_uniform(22, A2.elemNum()))
+    oA1.set_label([-1, 10, 0])
+    oA2.set_label([0, 11, -2])
+    oG = ora.UniTensor([pi, pi, po, po])
+    oG.set_raw_elem(ora.exph(-0.01, workloads.sz_hamiltonian(23)))
+    oG.set_label([10, 11, 20, 21])
+    assert same_layout(A1, oA1) and same_layout(G, oG)
+    ref = ora.contract(ora.contract(oA1, oA2), oG)
+    u10.set_gemm_mode(mode)
+    try:
+        R = u10.contract(u10.contract(A1, A2), G)
+    finally:
+        u10.set_gemm_mode(0)
+    assert R.label() == ref.labels and same_layout(R, ref)
+    err = np.linalg.norm(R.getElem() - ref.elem) / np.linalg.norm(ref.elem)
+    _say("u1 two-site mode %d rel err %.3e" % (mode, err))
+    assert err <= MODE_TOL[mode]
+
+
 def test_svd_eigh_exph_api(u10, ora):
     a = np.random.default_rng(4).uniform(-1, 1, (12, 7))
     U, S, VT = u10.svd(a)

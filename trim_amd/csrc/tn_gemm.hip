@@ -9,11 +9,12 @@
 // which keeps their shared A rows / B columns in that XCD's L2.
 //
 // Operand addressing: every element address is (row offset) + (column
-// offset).  Row/column offsets come either from a leading dimension or from
-// gather tables, so a contraction can read tensors in their stored bond order
-// (and write its result in any order) with no separate permute pass.  Each
-// thread keeps the offsets of its fixed rows/columns in registers and fetches
-// the other index's offset once per K step.
+// offset), each from an index map (a leading dimension, or up to four
+// (extent, stride) levels decoded with 32-bit magic-number division), so a
+// contraction reads tensors in their stored bond order and writes its result
+// in any order with no separate permute pass and no memory indirection.
+// Each thread keeps the offsets of its fixed rows/columns in registers and
+// decodes the K-side offsets once per K step.
 //
 // Arithmetic modes (storage is always FP64, row-major):
 //   F64     v_mfma_f64_16x16x4f64, FP64 accumulate            (plain drop-in)
@@ -22,7 +23,8 @@
 //           hh + hm + mh on v_mfma_f32_16x16x32_bf16, FP32 accumulate
 //   BF16X6  x = h + m + l; products hh+hm+mh+hl+mm+lh, FP32 accumulate
 // Every output element is computed by exactly one workgroup, accumulating
-// over K in a fixed order: results do not depend on what else is in the
+// over K in increasing order, one MFMA K-step at a time, for every tile
+// shape: results do not depend on the tile shape or on what else is in the
 // group (no split-K, no atomics).
 //
 // MFMA operand layout used (16x16 output tile, wave of 64 lanes):
@@ -52,19 +54,36 @@ struct TileDev {
   int32_t pad;
 };
 
+// Index map on the device: n levels (0 = plain), extents with their 32-bit
+// division magic (q = (umulhi(i, magic) + i) >> shift, exact for i < 2^31).
+struct MapDev {
+  int32_t n;
+  uint32_t ext[TN_MAX_MAP_LEVELS];
+  uint32_t magic[TN_MAX_MAP_LEVELS];
+  uint32_t shift[TN_MAX_MAP_LEVELS];
+  int64_t stride[TN_MAX_MAP_LEVELS];
+};
+
 struct GroupDev {
   int64_t M, N, K;
   int64_t a_off, b_off, c_off;
   int64_t lda, ldb, ldc;
-  int64_t a_row, a_col, b_row, b_col, c_row, c_col;   // table offsets, -1 = strided
+  MapDev a_row, a_col, b_row, b_col, c_row, c_col;
 };
 
-// Offsets of the operand index i along a "row" or "column" of an operand.
-__device__ __forceinline__ int64_t row_off(const int64_t* __restrict__ tab, int64_t t, int64_t i, int64_t ld) {
-  return (t >= 0) ? tab[t + i] : i * ld;
-}
-__device__ __forceinline__ int64_t col_off(const int64_t* __restrict__ tab, int64_t t, int64_t i) {
-  return (t >= 0) ? tab[t + i] : i;
+__device__ __forceinline__ int64_t map_off(const MapDev& m, int64_t i64, int64_t plain_mul) {
+  if (m.n == 0) return i64 * plain_mul;
+  uint32_t i = (uint32_t)i64;
+  int64_t off = 0;
+#pragma unroll
+  for (int l = TN_MAX_MAP_LEVELS - 1; l >= 0; --l) {
+    if (l < m.n) {
+      const uint32_t q = (__umulhi(i, m.magic[l]) + i) >> m.shift[l];
+      off += (int64_t)(i - q * m.ext[l]) * m.stride[l];
+      i = q;
+    }
+  }
+  return off;
 }
 
 // ----------------------------------------------------------------------------
@@ -74,8 +93,7 @@ __global__ void __launch_bounds__(256) gemm_mfma_16x16x4(const double* __restric
                                                          const double* __restrict__ B,
                                                          double* __restrict__ C,
                                                          const GroupDev* __restrict__ groups,
-                                                         const TileDev* __restrict__ tiles,
-                                                         const int64_t* __restrict__ tab) {
+                                                         const TileDev* __restrict__ tiles) {
   static_assert(256 % BK == 0 && 256 % BN == 0, "tile shape");
   constexpr int WM = BM / 2, WN = BN / 2;          // per-wave tile
   constexpr int TM = WM / 16, TN = WN / 16;        // MFMA tiles per wave
@@ -89,7 +107,8 @@ __global__ void __launch_bounds__(256) gemm_mfma_16x16x4(const double* __restric
 
   const TileDev tile = tiles[blockIdx.x];
   if (tile.group < 0) return;
-  const GroupDev g = groups[tile.group];
+  const GroupDev& g = groups[tile.group];
+  const int64_t M = g.M, N = g.N, K = g.K;
   const int64_t m0 = (int64_t)tile.tm * BM, n0 = (int64_t)tile.tn * BN;
   const double* __restrict__ Ab = A + g.a_off;
   const double* __restrict__ Bb = B + g.b_off;
@@ -107,11 +126,11 @@ __global__ void __launch_bounds__(256) gemm_mfma_16x16x4(const double* __restric
 #pragma unroll
   for (int i = 0; i < A_PER; ++i) {
     const int64_t m = m0 + a_mm0 + i * A_RSTEP;
-    a_roff[i] = (m < g.M) ? row_off(tab, g.a_row, m, g.lda) : -1;
+    a_roff[i] = (m < M) ? map_off(g.a_row, m, g.lda) : -1;
   }
   const int b_nn = tid % BN;
   const int b_kk0 = tid / BN;
-  const int64_t b_coff = (n0 + b_nn < g.N) ? col_off(tab, g.b_col, n0 + b_nn) : -1;
+  const int64_t b_coff = (n0 + b_nn < N) ? map_off(g.b_col, n0 + b_nn, 1) : -1;
 
   typedef typename std::conditional<sizeof(T) == 8, d4, f4>::type acc_t;
   acc_t acc[TM][TN];
@@ -123,14 +142,14 @@ __global__ void __launch_bounds__(256) gemm_mfma_16x16x4(const double* __restric
   T ra[A_PER], rb[B_PER];
   auto load_tiles = [&](int64_t k0) {
     const int64_t ka = k0 + a_kk;
-    const int64_t a_coff = (ka < g.K) ? col_off(tab, g.a_col, ka) : -1;
+    const int64_t a_coff = (ka < K) ? map_off(g.a_col, ka, 1) : -1;
 #pragma unroll
     for (int i = 0; i < A_PER; ++i)
       ra[i] = (a_roff[i] >= 0 && a_coff >= 0) ? (T)Ab[a_roff[i] + a_coff] : (T)0;
 #pragma unroll
     for (int i = 0; i < B_PER; ++i) {
       const int64_t kb = k0 + b_kk0 + i * B_KSTEP;
-      const int64_t b_roff = (kb < g.K) ? row_off(tab, g.b_row, kb, g.ldb) : -1;
+      const int64_t b_roff = (kb < K) ? map_off(g.b_row, kb, g.ldb) : -1;
       rb[i] = (b_roff >= 0 && b_coff >= 0) ? (T)Bb[b_roff + b_coff] : (T)0;
     }
   };
@@ -141,7 +160,7 @@ __global__ void __launch_bounds__(256) gemm_mfma_16x16x4(const double* __restric
     for (int i = 0; i < B_PER; ++i) Bs[b_kk0 + i * B_KSTEP][b_nn] = rb[i];
   };
 
-  const int64_t nk = (g.K + BK - 1) / BK;
+  const int64_t nk = (K + BK - 1) / BK;
   load_tiles(0);
   for (int64_t kt = 0; kt < nk; ++kt) {
     __syncthreads();
@@ -171,8 +190,8 @@ __global__ void __launch_bounds__(256) gemm_mfma_16x16x4(const double* __restric
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
     const int64_t gn = n0 + wn0 + j * 16 + (lane & 15);
-    if (gn >= g.N) continue;
-    const int64_t coff = col_off(tab, g.c_col, gn);
+    if (gn >= N) continue;
+    const int64_t coff = map_off(g.c_col, gn, 1);
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -180,13 +199,14 @@ __global__ void __launch_bounds__(256) gemm_mfma_16x16x4(const double* __restric
         // accumulator row: f64 16x16x4 -> (l/16) + 4v ; f32 16x16x4 -> 4(l/16) + v
         const int row = (sizeof(T) == 8) ? ((lane >> 4) + 4 * v) : (4 * (lane >> 4) + v);
         const int64_t gm = m0 + wm0 + i * 16 + row;
-        if (gm < g.M) Cb[row_off(tab, g.c_row, gm, g.ldc) + coff] = (double)acc[i][j][v];
+        if (gm < M) Cb[map_off(g.c_row, gm, g.ldc) + coff] = (double)acc[i][j][v];
       }
   }
 }
 
 // ----------------------------------------------------------------------------
-// Split-BF16 kernel: BM x BN tile, BK = 32, pieces NP = 2 (BF16X3) or 3 (BF16X6).
+// Split-BF16 kernel: BM x BN tile, BK deep (multiple of 32), pieces NP = 2
+// (BF16X3) or 3 (BF16X6).
 __device__ inline void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
   h = (__bf16)x;
   float r = x - (float)h;
@@ -195,14 +215,13 @@ __device__ inline void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
   l = (__bf16)r2;
 }
 
-template <int BM, int BN, int NP>
+template <int BM, int BN, int NP, int BK>
 __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict__ A,
                                                        const double* __restrict__ B,
                                                        double* __restrict__ C,
                                                        const GroupDev* __restrict__ groups,
-                                                       const TileDev* __restrict__ tiles,
-                                                       const int64_t* __restrict__ tab) {
-  constexpr int BK = 32;
+                                                       const TileDev* __restrict__ tiles) {
+  static_assert(BK % 32 == 0 && 256 % BK == 0 && 256 % BN == 0, "tile shape");
   constexpr int KP = BK + 8;                       // padded row (bf16 elements)
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
@@ -215,7 +234,8 @@ __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict_
 
   const TileDev tile = tiles[blockIdx.x];
   if (tile.group < 0) return;
-  const GroupDev g = groups[tile.group];
+  const GroupDev& g = groups[tile.group];
+  const int64_t M = g.M, N = g.N, K = g.K;
   const int64_t m0 = (int64_t)tile.tm * BM, n0 = (int64_t)tile.tn * BN;
   const double* __restrict__ Ab = A + g.a_off;
   const double* __restrict__ Bb = B + g.b_off;
@@ -232,11 +252,11 @@ __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict_
 #pragma unroll
   for (int i = 0; i < A_PER; ++i) {
     const int64_t m = m0 + a_mm0 + i * A_RSTEP;
-    a_roff[i] = (m < g.M) ? row_off(tab, g.a_row, m, g.lda) : -1;
+    a_roff[i] = (m < M) ? map_off(g.a_row, m, g.lda) : -1;
   }
   const int b_nn = tid % BN;
   const int b_kk0 = tid / BN;
-  const int64_t b_coff = (n0 + b_nn < g.N) ? col_off(tab, g.b_col, n0 + b_nn) : -1;
+  const int64_t b_coff = (n0 + b_nn < N) ? map_off(g.b_col, n0 + b_nn, 1) : -1;
 
   f4 acc[TM][TN];
 #pragma unroll
@@ -247,14 +267,14 @@ __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict_
   float ra[A_PER], rb[B_PER];
   auto load_tiles = [&](int64_t k0) {
     const int64_t ka = k0 + a_kk;
-    const int64_t a_coff = (ka < g.K) ? col_off(tab, g.a_col, ka) : -1;
+    const int64_t a_coff = (ka < K) ? map_off(g.a_col, ka, 1) : -1;
 #pragma unroll
     for (int i = 0; i < A_PER; ++i)
       ra[i] = (a_roff[i] >= 0 && a_coff >= 0) ? (float)Ab[a_roff[i] + a_coff] : 0.0f;
 #pragma unroll
     for (int i = 0; i < B_PER; ++i) {
       const int64_t kb = k0 + b_kk0 + i * B_KSTEP;
-      const int64_t b_roff = (kb < g.K) ? row_off(tab, g.b_row, kb, g.ldb) : -1;
+      const int64_t b_roff = (kb < K) ? map_off(g.b_row, kb, g.ldb) : -1;
       rb[i] = (b_roff >= 0 && b_coff >= 0) ? (float)Bb[b_roff + b_coff] : 0.0f;
     }
   };
@@ -279,50 +299,53 @@ __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict_
     }
   };
 
-  const int64_t nk = (g.K + BK - 1) / BK;
+  const int64_t nk = (K + BK - 1) / BK;
   load_tiles(0);
-  const int kq = 8 * (lane >> 4);
   for (int64_t kt = 0; kt < nk; ++kt) {
     __syncthreads();
     store_tiles();
     __syncthreads();
     if (kt + 1 < nk) load_tiles((kt + 1) * BK);
-    bf16x8 a[NP][TM], b[NP][TN];
 #pragma unroll
-    for (int p = 0; p < NP; ++p) {
+    for (int ks = 0; ks < BK; ks += 32) {
+      const int kq = ks + 8 * (lane >> 4);
+      bf16x8 a[NP][TM], b[NP][TN];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+          a[p][i] = *reinterpret_cast<const bf16x8*>(&As[p][wm0 + i * 16 + (lane & 15)][kq]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          b[p][j] = *reinterpret_cast<const bf16x8*>(&Bs[p][wn0 + j * 16 + (lane & 15)][kq]);
+      }
 #pragma unroll
       for (int i = 0; i < TM; ++i)
-        a[p][i] = *reinterpret_cast<const bf16x8*>(&As[p][wm0 + i * 16 + (lane & 15)][kq]);
 #pragma unroll
-      for (int j = 0; j < TN; ++j)
-        b[p][j] = *reinterpret_cast<const bf16x8*>(&Bs[p][wn0 + j * 16 + (lane & 15)][kq]);
-    }
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        // smallest terms first
-        if constexpr (NP > 2) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], b[0][j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[1][j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[2][j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < TN; ++j) {
+          // smallest terms first
+          if constexpr (NP > 2) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], b[0][j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[1][j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[2][j], acc[i][j], 0, 0, 0);
+          }
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[0][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[1][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], acc[i][j], 0, 0, 0);
         }
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[0][j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[1][j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], acc[i][j], 0, 0, 0);
-      }
+    }
   }
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
     const int64_t gn = n0 + wn0 + j * 16 + (lane & 15);
-    if (gn >= g.N) continue;
-    const int64_t coff = col_off(tab, g.c_col, gn);
+    if (gn >= N) continue;
+    const int64_t coff = map_off(g.c_col, gn, 1);
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int64_t gm = m0 + wm0 + i * 16 + 4 * (lane >> 4) + v;
-        if (gm < g.M) Cb[row_off(tab, g.c_row, gm, g.ldc) + coff] = (double)acc[i][j][v];
+        if (gm < M) Cb[map_off(g.c_row, gm, g.ldc) + coff] = (double)acc[i][j][v];
       }
   }
 }
@@ -335,15 +358,14 @@ struct TileCfg {
 constexpr TileCfg kCfgs[] = {{32, 32}, {64, 64}, {128, 64}, {64, 128}, {128, 128}};
 constexpr int kNumCfg = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
-typedef void (*KernFn)(const double*, const double*, double*, const GroupDev*, const TileDev*,
-                       const int64_t*);
+typedef void (*KernFn)(const double*, const double*, double*, const GroupDev*, const TileDev*);
 
 KernFn pick_kernel(int mode, int cfg) {
   switch (mode) {
     case TN_GEMM_F64:
       switch (cfg) {
-        case 0: return gemm_mfma_16x16x4<double, 32, 32, 64>;
-        case 1: return gemm_mfma_16x16x4<double, 64, 64, 32>;
+        case 0: return gemm_mfma_16x16x4<double, 32, 32, 128>;
+        case 1: return gemm_mfma_16x16x4<double, 64, 64, 64>;
         case 2: return gemm_mfma_16x16x4<double, 128, 64, 32>;
         case 3: return gemm_mfma_16x16x4<double, 64, 128, 32>;
         case 4: return gemm_mfma_16x16x4<double, 128, 128, 16>;
@@ -351,8 +373,8 @@ KernFn pick_kernel(int mode, int cfg) {
       break;
     case TN_GEMM_F32:
       switch (cfg) {
-        case 0: return gemm_mfma_16x16x4<float, 32, 32, 64>;
-        case 1: return gemm_mfma_16x16x4<float, 64, 64, 32>;
+        case 0: return gemm_mfma_16x16x4<float, 32, 32, 128>;
+        case 1: return gemm_mfma_16x16x4<float, 64, 64, 64>;
         case 2: return gemm_mfma_16x16x4<float, 128, 64, 32>;
         case 3: return gemm_mfma_16x16x4<float, 64, 128, 32>;
         case 4: return gemm_mfma_16x16x4<float, 128, 128, 16>;
@@ -360,24 +382,52 @@ KernFn pick_kernel(int mode, int cfg) {
       break;
     case TN_GEMM_BF16X3:
       switch (cfg) {
-        case 0: return gemm_split_bf16<32, 32, 2>;
-        case 1: return gemm_split_bf16<64, 64, 2>;
-        case 2: return gemm_split_bf16<128, 64, 2>;
-        case 3: return gemm_split_bf16<64, 128, 2>;
-        case 4: return gemm_split_bf16<128, 128, 2>;
+        case 0: return gemm_split_bf16<32, 32, 2, 128>;
+        case 1: return gemm_split_bf16<64, 64, 2, 64>;
+        case 2: return gemm_split_bf16<128, 64, 2, 32>;
+        case 3: return gemm_split_bf16<64, 128, 2, 32>;
+        case 4: return gemm_split_bf16<128, 128, 2, 32>;
       }
       break;
     case TN_GEMM_BF16X6:
       switch (cfg) {
-        case 0: return gemm_split_bf16<32, 32, 3>;
-        case 1: return gemm_split_bf16<64, 64, 3>;
-        case 2: return gemm_split_bf16<128, 64, 3>;
-        case 3: return gemm_split_bf16<64, 128, 3>;
-        case 4: return gemm_split_bf16<128, 128, 3>;
+        case 0: return gemm_split_bf16<32, 32, 3, 128>;
+        case 1: return gemm_split_bf16<64, 64, 3, 64>;
+        case 2: return gemm_split_bf16<128, 64, 3, 32>;
+        case 3: return gemm_split_bf16<64, 128, 3, 32>;
+        case 4: return gemm_split_bf16<128, 128, 3, 32>;
       }
       break;
   }
   return nullptr;
+}
+
+// Host: 32-bit division magic for divisor d >= 1 (exact for dividends < 2^31).
+void div_magic(uint32_t d, uint32_t* magic, uint32_t* shift) {
+  uint32_t l = 0;
+  while ((uint64_t(1) << l) < d) ++l;
+  *shift = l;
+  *magic = (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << l) - d)) / d + 1);
+}
+
+bool make_map(const tn_index_map& in, int64_t range, MapDev* out) {
+  std::memset(out, 0, sizeof(*out));
+  if (in.n == 0 || range == 0) return true;
+  if (in.n < 0 || in.n > TN_MAX_MAP_LEVELS) return false;
+  int64_t prod = 1;
+  for (int l = 0; l < in.n; ++l) {
+    if (in.ext[l] < 1 || in.ext[l] >= (int64_t(1) << 31) || in.stride[l] < 0) return false;
+    prod *= in.ext[l];
+    if (prod >= (int64_t(1) << 31)) return false;
+  }
+  if (prod != range) return false;
+  out->n = in.n;
+  for (int l = 0; l < in.n; ++l) {
+    out->ext[l] = (uint32_t)in.ext[l];
+    div_magic(out->ext[l], &out->magic[l], &out->shift[l]);
+    out->stride[l] = in.stride[l];
+  }
+  return true;
 }
 
 }  // namespace
@@ -388,7 +438,6 @@ struct tn_gemm_plan_s {
   std::vector<GroupDev> groups;
   GroupDev* d_groups = nullptr;
   TileDev* d_tiles = nullptr;
-  int64_t* d_tab = nullptr;
   int64_t ntiles = 0;     // real tiles
   int64_t nslots = 0;     // launched blocks (with XCD padding)
   double flops = 0;
@@ -445,38 +494,45 @@ static int default_cfg(const tn_gemm_plan_s* P) {
 static void destroy_plan(tn_gemm_plan_s* P) {
   tn::pool_free(P->d_groups);
   tn::pool_free(P->d_tiles);
-  tn::pool_free(P->d_tab);
   delete P;
 }
 
 extern "C" {
 
-int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups, const tn_gemm_desc_ex* d,
-                           int64_t ntab, const int64_t* tab) {
+int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups, const tn_gemm_desc_ex* d) {
   TN_INIT_OR_RETURN();
   TN_REQUIRE(plan != nullptr, TN_EINVAL, "tn_gemm_plan_create: NULL out pointer");
   TN_REQUIRE(mode >= TN_GEMM_F64 && mode <= TN_GEMM_BF16X6, TN_EINVAL, "unknown GEMM mode %d", mode);
   TN_REQUIRE(ngroups >= 0 && (ngroups == 0 || d), TN_EINVAL, "tn_gemm_plan_create: NULL descs");
-  TN_REQUIRE(ntab >= 0 && (ntab == 0 || tab), TN_EINVAL, "tn_gemm_plan_create: NULL table");
   auto* P = new tn_gemm_plan_s();
   P->mode = mode;
   for (int64_t i = 0; i < ngroups; ++i) {
     const tn_gemm_desc_ex& x = d[i];
-    auto tab_ok = [&](int64_t t, int64_t n) { return t < 0 || (t + n <= ntab); };
-    bool pair_ok = ((x.a_row < 0) == (x.a_col < 0)) && ((x.b_row < 0) == (x.b_col < 0)) &&
-                   ((x.c_row < 0) == (x.c_col < 0));
-    bool ld_ok = (x.a_row >= 0 || x.M == 0 || x.K == 0 || x.lda >= x.K) &&
-                 (x.b_row >= 0 || x.K == 0 || x.N == 0 || x.ldb >= x.N) &&
-                 (x.c_row >= 0 || x.M == 0 || x.N == 0 || x.ldc >= x.N);
-    if (x.M < 0 || x.N < 0 || x.K < 0 || x.a_off < 0 || x.b_off < 0 || x.c_off < 0 || !pair_ok || !ld_ok ||
-        !tab_ok(x.a_row, x.M) || !tab_ok(x.a_col, x.K) || !tab_ok(x.b_row, x.K) || !tab_ok(x.b_col, x.N) ||
-        !tab_ok(x.c_row, x.M) || !tab_ok(x.c_col, x.N)) {
+    GroupDev G;
+    std::memset(&G, 0, sizeof(G));
+    G.M = x.M;
+    G.N = x.N;
+    G.K = x.K;
+    G.a_off = x.a_off;
+    G.b_off = x.b_off;
+    G.c_off = x.c_off;
+    G.lda = x.lda;
+    G.ldb = x.ldb;
+    G.ldc = x.ldc;
+    const bool ld_ok = (x.a_row.n || x.M == 0 || x.K == 0 || x.lda >= x.K) &&
+                       (x.b_row.n || x.K == 0 || x.N == 0 || x.ldb >= x.N) &&
+                       (x.c_row.n || x.M == 0 || x.N == 0 || x.ldc >= x.N);
+    const bool range_ok = x.M < (int64_t(1) << 31) && x.N < (int64_t(1) << 31) && x.K < (int64_t(1) << 31);
+    const bool maps_ok = make_map(x.a_row, x.M, &G.a_row) && make_map(x.a_col, x.K, &G.a_col) &&
+                         make_map(x.b_row, x.K, &G.b_row) && make_map(x.b_col, x.N, &G.b_col) &&
+                         make_map(x.c_row, x.M, &G.c_row) && make_map(x.c_col, x.N, &G.c_col);
+    if (x.M < 0 || x.N < 0 || x.K < 0 || x.a_off < 0 || x.b_off < 0 || x.c_off < 0 || !ld_ok || !range_ok ||
+        !maps_ok) {
       delete P;
-      tn::set_error("GEMM group %lld has an invalid shape, leading dimension or table range", (long long)i);
+      tn::set_error("GEMM group %lld has an invalid shape, leading dimension or index map", (long long)i);
       return TN_EINVAL;
     }
-    P->groups.push_back(GroupDev{x.M, x.N, x.K, x.a_off, x.b_off, x.c_off, x.lda, x.ldb, x.ldc, x.a_row, x.a_col,
-                                 x.b_row, x.b_col, x.c_row, x.c_col});
+    P->groups.push_back(G);
     P->flops += 2.0 * (double)x.M * (double)x.N * (double)x.K;
   }
   int rc = TN_OK;
@@ -484,10 +540,6 @@ int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups, const 
     rc = tn::pool_alloc((void**)&P->d_groups, sizeof(GroupDev) * P->groups.size());
     if (rc == TN_OK)
       rc = tn_memcpy_h2d(P->d_groups, P->groups.data(), sizeof(GroupDev) * P->groups.size(), nullptr);
-  }
-  if (rc == TN_OK && ntab > 0) {
-    rc = tn::pool_alloc((void**)&P->d_tab, sizeof(int64_t) * ntab);
-    if (rc == TN_OK) rc = tn_memcpy_h2d(P->d_tab, tab, sizeof(int64_t) * ntab, nullptr);
   }
   if (rc == TN_OK) rc = build_tiles(P, default_cfg(P));
   if (rc != TN_OK) {
@@ -503,9 +555,18 @@ int tn_gemm_plan_create(tn_gemm_plan* plan, int mode, int64_t ngroups, const tn_
   std::vector<tn_gemm_desc_ex> ex(ngroups);
   for (int64_t i = 0; i < ngroups; ++i) {
     const tn_gemm_desc& x = d[i];
-    ex[i] = tn_gemm_desc_ex{x.M, x.N, x.K, x.a_off, x.b_off, x.c_off, x.lda, x.ldb, x.ldc, -1, -1, -1, -1, -1, -1};
+    std::memset(&ex[i], 0, sizeof(tn_gemm_desc_ex));
+    ex[i].M = x.M;
+    ex[i].N = x.N;
+    ex[i].K = x.K;
+    ex[i].a_off = x.a_off;
+    ex[i].b_off = x.b_off;
+    ex[i].c_off = x.c_off;
+    ex[i].lda = x.lda;
+    ex[i].ldb = x.ldb;
+    ex[i].ldc = x.ldc;
   }
-  return tn_gemm_plan_create_ex(plan, mode, ngroups, ex.data(), 0, nullptr);
+  return tn_gemm_plan_create_ex(plan, mode, ngroups, ex.data());
 }
 
 int tn_gemm_plan_set_tile(tn_gemm_plan P, int tile_id) {
@@ -513,6 +574,44 @@ int tn_gemm_plan_set_tile(tn_gemm_plan P, int tile_id) {
   TN_REQUIRE(tile_id >= 0 && tile_id < kNumCfg, TN_EINVAL, "tile id %d outside [0,%d)", tile_id, kNumCfg);
   if (tile_id == P->cfg) return TN_OK;
   return build_tiles(P, tile_id);
+}
+
+int tn_gemm_plan_autotune(tn_gemm_plan P, const double* A, const double* B, double* C, int reps,
+                          void* stream, float* best_ms) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(P != nullptr, TN_EINVAL, "tn_gemm_plan_autotune: NULL plan");
+  if (P->groups.empty()) return TN_OK;
+  if (reps < 1) reps = 1;
+  hipStream_t s = tn::stream_of(stream);
+  hipEvent_t e0, e1;
+  TN_HIP_CHECK(hipEventCreate(&e0));
+  TN_HIP_CHECK(hipEventCreate(&e1));
+  int best = P->cfg;
+  float best_t = 1e30f;
+  int rc = TN_OK;
+  for (int c = 0; c < kNumCfg && rc == TN_OK; ++c) {
+    rc = build_tiles(P, c);
+    if (rc != TN_OK) break;
+    rc = tn_gemm_exec(P, A, B, C, stream);   // warm-up
+    if (rc != TN_OK) break;
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps && rc == TN_OK; ++r) rc = tn_gemm_exec(P, A, B, C, stream);
+    (void)hipEventRecord(e1, s);
+    if (rc != TN_OK) break;
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    ms /= reps;
+    if (ms < best_t) {
+      best_t = ms;
+      best = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != TN_OK) return rc;
+  if (best_ms) *best_ms = best_t;
+  return build_tiles(P, best);
 }
 
 int tn_gemm_plan_info(tn_gemm_plan P, int64_t* tiles, double* flops, int* tile_id) {
@@ -524,11 +623,12 @@ int tn_gemm_plan_info(tn_gemm_plan P, int64_t* tiles, double* flops, int* tile_i
 }
 
 // Fill kernel for sectors with K == 0 (dgemm with beta = 0 writes zeros).
-__global__ void zero_c_kernel(double* __restrict__ C, GroupDev g, const int64_t* __restrict__ tab) {
+__global__ void zero_c_kernel(double* __restrict__ C, const GroupDev* __restrict__ gp) {
+  const GroupDev& g = *gp;
   const int64_t total = g.M * g.N;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     int64_t m = e / g.N, n = e % g.N;
-    C[g.c_off + row_off(tab, g.c_row, m, g.ldc) + col_off(tab, g.c_col, n)] = 0.0;
+    C[g.c_off + map_off(g.c_row, m, g.ldc) + map_off(g.c_col, n, 1)] = 0.0;
   }
 }
 
@@ -536,25 +636,27 @@ int tn_gemm_exec(tn_gemm_plan P, const double* A, const double* B, double* C, vo
   TN_INIT_OR_RETURN();
   TN_REQUIRE(P != nullptr, TN_EINVAL, "tn_gemm_exec: NULL plan");
   hipStream_t s = tn::stream_of(stream);
-  for (const auto& g : P->groups)
+  for (size_t gi = 0; gi < P->groups.size(); ++gi) {
+    const GroupDev& g = P->groups[gi];
     if (g.K == 0 && g.M > 0 && g.N > 0) {
       TN_REQUIRE(C != nullptr, TN_EINVAL, "tn_gemm_exec: NULL C");
       int64_t blocks = std::min<int64_t>((g.M * g.N + 255) / 256, 1024);
-      hipLaunchKernelGGL(zero_c_kernel, dim3((unsigned)blocks), dim3(256), 0, s, C, g, P->d_tab);
+      hipLaunchKernelGGL(zero_c_kernel, dim3((unsigned)blocks), dim3(256), 0, s, C, P->d_groups + gi);
       TN_LAUNCH_CHECK();
     }
+  }
   if (P->nslots == 0) return TN_OK;
   TN_REQUIRE(A && B && C, TN_EINVAL, "tn_gemm_exec: NULL operand");
   KernFn k = pick_kernel(P->mode, P->cfg);
   TN_REQUIRE(k != nullptr, TN_EINVAL, "no kernel for mode %d tile %d", P->mode, P->cfg);
-  hipLaunchKernelGGL(k, dim3((unsigned)P->nslots), dim3(256), 0, s, A, B, C, P->d_groups, P->d_tiles, P->d_tab);
+  hipLaunchKernelGGL(k, dim3((unsigned)P->nslots), dim3(256), 0, s, A, B, C, P->d_groups, P->d_tiles);
   TN_LAUNCH_CHECK();
   return TN_OK;
 }
 
 int tn_gemm_plan_destroy(tn_gemm_plan P) {
   if (!P) return TN_OK;
-  // the plan's tables may still be read by queued launches
+  // the plan's descriptors may still be read by queued launches
   hipError_t e = hipDeviceSynchronize();
   destroy_plan(P);
   TN_REQUIRE(e == hipSuccess, TN_EHIP, "hipDeviceSynchronize: %s", hipGetErrorString(e));

@@ -62,6 +62,10 @@ int u10_set_stream(void* s) {
   return guard([&] { setStream(s); });
 }
 
+int u10_set_autotune(int on) {
+  return guard([&] { setAutotune(on != 0); });
+}
+
 int u10_tensor_create(int nbonds, const int* types, const int* nruns, const int* run_qnums, const int* run_degs,
                       const char* name, u10_tensor* out) {
   return guard([&] {

@@ -781,6 +781,25 @@ std::unordered_map<std::string, std::shared_ptr<GemmPlanEntry>> g_gemm_cache;
 // One grouped GEMM over the charges present in both A and B (the per-Qnum
 // loop of UniTensorTools.cpp:191-203); returns the plan and whether every
 // block of C is written (else C must be zero-filled first).
+// Tile autotuning: the first time a cached GEMM plan runs, every tile shape
+// is timed on the real operands and the fastest kept (results are identical
+// for every tile shape).  Thread-local switch; on by default.
+namespace {
+thread_local bool t_autotune = true;
+std::unordered_map<tn_gemm_plan, bool> g_tuned;   // guarded by g_gemm_mu
+}  // namespace
+
+void setAutotune(bool on) { t_autotune = on; }
+
+static void maybe_autotune(tn_gemm_plan plan, const double* A, const double* B, double* C) {
+  if (!t_autotune) return;
+  {
+    std::lock_guard<std::mutex> g(g_gemm_mu);
+    if (!g_tuned.emplace(plan, true).second) return;
+  }
+  detail::check(tn_gemm_plan_autotune(plan, A, B, C, 3, detail::stream(), nullptr), "tn_gemm_plan_autotune");
+}
+
 static tn_gemm_plan contract_plan(const Layout& A, const Layout& B, const Layout& C, int mode, bool* covers_c) {
   std::string key = A.key + "#" + B.key + "#" + C.key + "#" + std::to_string(mode);
   size_t covered = 0;
@@ -851,43 +870,49 @@ static tn_gemm_plan gather_plan(const UniTensor& Ta, const std::vector<int>& a_f
     auto it = g_gemm_cache.find(key);
     if (it != g_gemm_cache.end()) return it->second->plan;
   }
-  // offsets of the multi-index over `axes` (last fastest)
-  auto table = [](const UniTensor& T, const std::vector<int>& axes, const std::vector<int64_t>& st,
-                  std::vector<int64_t>& out) {
-    int64_t n = 1;
-    for (int a : axes) n *= T.bond(a).dim();
-    size_t base = out.size();
-    out.resize(base + n, 0);
-    int64_t span = 1;
-    for (int i = (int)axes.size() - 1; i >= 0; --i) {
-      const int64_t d = T.bond(axes[i]).dim();
-      for (int64_t e = 0; e < n; ++e) out[base + e] += ((e / span) % d) * st[axes[i]];
-      span *= d;
+  // Index map of the multi-index over `axes` (last fastest): unit axes drop,
+  // an axis merges into the previous level when they are adjacent in storage.
+  auto make_map = [](const UniTensor& T, const std::vector<int>& axes, const std::vector<int64_t>& st,
+                     tn_index_map& m, int64_t& range) {
+    std::vector<std::pair<int64_t, int64_t>> lv;   // (extent, stride), outermost first
+    range = 1;
+    for (int a : axes) {
+      const int64_t e = T.bond(a).dim();
+      range *= e;
+      if (e == 1) continue;
+      if (!lv.empty() && lv.back().second == st[a] * e) {
+        lv.back().first *= e;
+        lv.back().second = st[a];
+      } else {
+        lv.emplace_back(e, st[a]);
+      }
     }
-    return n;
+    if (lv.empty()) lv.emplace_back(1, 0);
+    if (lv.size() > TN_MAX_MAP_LEVELS) return false;
+    std::memset(&m, 0, sizeof(m));
+    m.n = (int32_t)lv.size();
+    for (size_t l = 0; l < lv.size(); ++l) {
+      m.ext[l] = lv[l].first;
+      m.stride[l] = lv[l].second;
+    }
+    return true;
   };
-  std::vector<int64_t> tab;
   tn_gemm_desc_ex d;
-  d.a_row = 0;
-  d.M = table(Ta, a_free, sa, tab);
-  d.a_col = (int64_t)tab.size();
-  d.K = table(Ta, a_sh, sa, tab);
-  d.b_row = (int64_t)tab.size();
-  int64_t kb = table(Tb, b_sh, sb, tab);
-  d.b_col = (int64_t)tab.size();
-  d.N = table(Tb, b_free, sb, tab);
+  std::memset(&d, 0, sizeof(d));
+  int64_t kb = 0;
+  if (!make_map(Ta, a_free, sa, d.a_row, d.M) || !make_map(Ta, a_sh, sa, d.a_col, d.K) ||
+      !make_map(Tb, b_sh, sb, d.b_row, kb) || !make_map(Tb, b_free, sb, d.b_col, d.N))
+    return nullptr;   // more than TN_MAX_MAP_LEVELS strided levels: use the permute path
   if (kb != d.K) detail::fail("The dimensions the bonds to be contracted out are different.");
+  if (d.M >= (int64_t(1) << 31) || d.N >= (int64_t(1) << 31) || d.K >= (int64_t(1) << 31)) return nullptr;
   const BlockInfo& cb = C.blocks.begin()->second;
   if (C.blocks.size() != 1 || (int64_t)cb.Rnum != d.M || (int64_t)cb.Cnum != d.N)
     detail::fail("contract: unexpected output layout for dense operands");
-  d.a_off = d.b_off = d.c_off = 0;
   d.lda = std::max<int64_t>(d.K, 1);
   d.ldb = std::max<int64_t>(d.N, 1);
   d.ldc = std::max<int64_t>(d.N, 1);
-  d.c_row = d.c_col = -1;
   auto ent = std::make_shared<GemmPlanEntry>();
-  detail::check(tn_gemm_plan_create_ex(&ent->plan, mode, 1, &d, (int64_t)tab.size(), tab.data()),
-                "tn_gemm_plan_create_ex");
+  detail::check(tn_gemm_plan_create_ex(&ent->plan, mode, 1, &d), "tn_gemm_plan_create_ex");
   std::lock_guard<std::mutex> g(g_gemm_mu);
   auto ins = g_gemm_cache.emplace(key, ent);
   if (!ins.second) tn_gemm_plan_destroy(ent->plan);
@@ -930,12 +955,12 @@ UniTensor contract_impl(UniTensor& Ta, UniTensor& Tb, bool fast, bool allow_gath
         newC.push_back(Tb.labels[b]);
       }
     const int con = (int)inter.size();
-    const bool gather = allow_gather && is_dense(Ta) && is_dense(Tb);
+    bool gather = allow_gather && is_dense(Ta) && is_dense(Tb);
     UniTensor Tc(std::vector<Bond>(), "", UniTensor::NoZero);
     if (gather) {
       // Dense operands: read A and B in their stored bond order through
-      // offset tables; no permute of the inputs (their observable state is
-      // what the reference leaves after its permute + restore).
+      // multi-stride index maps; no permute of the inputs (their observable
+      // state is what the reference leaves after its permute + restore).
       std::vector<int> a_free, a_sh, b_sh, b_free;
       for (int a = 0; a < na; ++a)
         (std::find(inter.begin(), inter.end(), Ta.labels[a]) == inter.end() ? a_free : a_sh).push_back(a);
@@ -951,8 +976,14 @@ UniTensor contract_impl(UniTensor& Ta, UniTensor& Tb, bool fast, bool allow_gath
       Tc = UniTensor(cBonds, "", UniTensor::NoZero);
       if (!cBonds.empty()) Tc.setLabel(newC);
       tn_gemm_plan plan = gather_plan(Ta, a_free, a_sh, Tb, b_sh, b_free, *Tc.m_layout, detail::gemm_mode());
-      detail::check(tn_gemm_exec(plan, Ta.buf.get(), Tb.buf.get(), Tc.buf.get(), detail::stream()), "contract");
-    } else {
+      if (plan) {
+        maybe_autotune(plan, Ta.buf.get(), Tb.buf.get(), Tc.buf.get());
+        detail::check(tn_gemm_exec(plan, Ta.buf.get(), Tb.buf.get(), Tc.buf.get(), detail::stream()), "contract");
+      } else {
+        gather = false;
+      }
+    }
+    if (!gather) {
       Ta.permute(newA, na - con);
       Tb.permute(newB, con);
       std::vector<Bond> cBonds;
@@ -962,6 +993,7 @@ UniTensor contract_impl(UniTensor& Ta, UniTensor& Tb, bool fast, bool allow_gath
       if (!cBonds.empty()) Tc.setLabel(newC);
       bool covers = false;
       tn_gemm_plan plan = contract_plan(*Ta.m_layout, *Tb.m_layout, *Tc.m_layout, detail::gemm_mode(), &covers);
+      maybe_autotune(plan, Ta.buf.get(), Tb.buf.get(), Tc.buf.get());
       if (!covers)
         detail::check(tn_zero(Tc.buf.get(), Tc.m_elemNum * sizeof(double), detail::stream()), "contract zero");
       detail::check(tn_gemm_exec(plan, Ta.buf.get(), Tb.buf.get(), Tc.buf.get(), detail::stream()), "contract");

@@ -55,8 +55,9 @@ SIGNATURES = {
     "tn_blockperm_plan_destroy": [P],
     "tn_transpose_f64": [P, I64, I64, P, P],
     "tn_gemm_plan_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, I64, P],
-    "tn_gemm_plan_create_ex": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, I64, P, I64, P],
+    "tn_gemm_plan_create_ex": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, I64, P],
     "tn_gemm_plan_set_tile": [P, ctypes.c_int],
+    "tn_gemm_plan_autotune": [P, P, P, P, ctypes.c_int, P, ctypes.POINTER(ctypes.c_float)],
     "tn_gemm_plan_info": [P, PI64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)],
     "tn_gemm_exec": [P, P, P, P, P],
     "tn_gemm_plan_destroy": [P],
@@ -225,15 +226,44 @@ GEMM_DESC = np.dtype([("M", "<i8"), ("N", "<i8"), ("K", "<i8"), ("a_off", "<i8")
                       ("ldc", "<i8")])
 
 
+INDEX_MAP = np.dtype([("n", "<i4"), ("pad", "<i4"), ("ext", "<i8", (4,)), ("stride", "<i8", (4,))])
+GEMM_DESC_EX = np.dtype([("M", "<i8"), ("N", "<i8"), ("K", "<i8"), ("a_off", "<i8"), ("b_off", "<i8"),
+                         ("c_off", "<i8"), ("lda", "<i8"), ("ldb", "<i8"), ("ldc", "<i8"),
+                         ("a_row", INDEX_MAP), ("a_col", INDEX_MAP), ("b_row", INDEX_MAP),
+                         ("b_col", INDEX_MAP), ("c_row", INDEX_MAP), ("c_col", INDEX_MAP)])
+
+
+def index_map(levels):
+    """[(extent, stride), ...] outermost first -> an INDEX_MAP record (empty = plain)."""
+    m = np.zeros((), dtype=INDEX_MAP)
+    m["n"] = len(levels)
+    for i, (e, s) in enumerate(levels):
+        m["ext"][i] = e
+        m["stride"][i] = s
+    return m
+
+
 class GemmPlan:
     def __init__(self, mode, descs):
-        self.descs = np.ascontiguousarray(descs, dtype=GEMM_DESC)
+        descs = np.asarray(descs)
+        if descs.dtype == GEMM_DESC_EX:
+            self.descs = np.ascontiguousarray(descs)
+            fn = "tn_gemm_plan_create_ex"
+        else:
+            self.descs = np.ascontiguousarray(descs, dtype=GEMM_DESC)
+            fn = "tn_gemm_plan_create"
         h = ctypes.c_void_p()
-        call("tn_gemm_plan_create", ctypes.byref(h), int(mode), len(self.descs), _p(self.descs))
+        call(fn, ctypes.byref(h), int(mode), len(self.descs), _p(self.descs))
         self.h = h.value
 
     def set_tile(self, tile_id: int):
         call("tn_gemm_plan_set_tile", ctypes.c_void_p(self.h), int(tile_id))
+
+    def autotune(self, A, B, C, reps: int = 5, stream=None) -> float:
+        ms = ctypes.c_float()
+        call("tn_gemm_plan_autotune", ctypes.c_void_p(self.h), _p(A), _p(B), _p(C), int(reps), stream,
+             ctypes.byref(ms))
+        return ms.value
 
     def info(self):
         t = ctypes.c_int64()

@@ -37,6 +37,7 @@ SIGNATURES = {
     "u10_last_error": [],
     "u10_set_gemm_mode": [I],
     "u10_set_stream": [P],
+    "u10_set_autotune": [I],
     "u10_tensor_create": [I, P, P, P, P, S, PP],
     "u10_tensor_scalar": [D, PP],
     "u10_tensor_load": [S, PP],
@@ -115,6 +116,10 @@ def _arr(a, dtype):
 
 def set_gemm_mode(mode: int):
     _call("u10_set_gemm_mode", int(mode))
+
+
+def set_autotune(on: bool):
+    _call("u10_set_autotune", 1 if on else 0)
 
 
 class Qnum:
