@@ -13,7 +13,7 @@ Floating-point kernels call the same LAPACK/BLAS routines the reference calls
 (``dgemm`` through numpy's matmul, ``dgesvd`` and ``dsyev`` through SciPy);
 integer/index work (block layout, permutes) is restated exactly.
 
-Pinning (what each part is checked against; see tests/test_oracle_*.py):
+Pinning (what each part is checked against; see tests/test_oracle.py):
   * Qnum / Bond           - reference unit-test known answers (testQnum.cpp,
                             testBond.cpp).
   * grouping/setRawElem   - golden tensor dumps egU1_H_U1 (rank-4 U(1)),
@@ -31,6 +31,11 @@ Pinning (what each part is checked against; see tests/test_oracle_*.py):
                             brackets vs greedy plan on the same tensors.
   * iTEBD energy          - Bethe-ansatz bound 1/4 - ln 2 and the two-site
                             exact ground energy -3/4 at chi=1.
+  * rec_swap              - inversion-set identity and permutation sign.
+  * swap_signs_setup      - PARITY UNPINNED: the convention (column bonds
+                            taken in reverse order) follows
+                            UniTensorReal.cpp:657-681; the reference holds no
+                            fermionic permute fixture or test to pin it.
 """
 from __future__ import annotations
 

@@ -232,6 +232,36 @@ def test_symmetric_permute_densify(ora, seed):
     assert back.elem.tobytes() == T.elem.tobytes()
 
 
+def _perm_sign_by_cycles(order):
+    seen, sign = [False] * len(order), 1
+    for s in range(len(order)):
+        length, i = 0, s
+        while not seen[i]:
+            seen[i] = True
+            i = order[i]
+            length += 1
+        if length and length % 2 == 0:
+            sign = -sign
+    return sign
+
+
+def test_rec_swap_lists_exactly_the_inverted_pairs(ora):
+    """Bubble sort swaps every inverted pair of the order exactly once, so the
+    swap list that sets the fermionic signs is the inversion set (mapped
+    through ordF); with every bond odd its length has the parity of the
+    permutation, whose sign the cycle decomposition gives independently."""
+    rng = np.random.default_rng(11)
+    for n in range(1, 8):
+        for _ in range(30):
+            order = [int(v) for v in rng.permutation(n)]
+            ordF = [int(v) for v in rng.permutation(n)]
+            got = ora.rec_swap(order, ordF)
+            want = {(ordF[order[j]], ordF[order[i]])
+                    for i in range(n) for j in range(i + 1, n) if order[i] > order[j]}
+            assert len(got) == len(want) and set(got) == want
+            assert (-1) ** len(got) == _perm_sign_by_cycles(order)
+
+
 def test_contract_vs_tensordot(ora):
     for seed in range(6):
         rng = np.random.default_rng(seed)
