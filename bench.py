@@ -41,6 +41,8 @@ NETDIR = os.path.join(ROOT, "tests", "golden")
 HBM_PEAK_GBPS = 8000.0
 F64_MFMA_PEAK_TFLOPS = 78.6
 DIMS = (256, 2, 2, 256)
+# FP64 GEMM tile shapes by autotuner id: BM x BN block tile, KC k per stage
+F64_TILES = {0: "32x32,kc128", 1: "64x64,kc64", 2: "128x64,kc32", 3: "64x128,kc32", 4: "64x32,kc64"}
 
 
 def splitmix_uniform(seed, n):
@@ -307,7 +309,8 @@ def main():
     if per["gemm_f64"] >= per["permute"]:
         roof = {"bound": "mfma", "achieved": gemm_tf, "peak": F64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": gemm_tf / F64_MFMA_PEAK_TFLOPS, "traffic": None,
-                "kernel": "gemm_mfma_16x16x4<double,32,32,64>", "avg_ms": per["gemm_f64"],
+                "kernel": "gemm_mfma_k32<double,%s>" % F64_TILES.get(ginfo["gemm_tile_cfg"], "?"),
+                "avg_ms": per["gemm_f64"],
                 "flop_per_launch": kflops}
     else:
         roof = {"bound": "hbm", "achieved": perm_gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",

@@ -22,10 +22,9 @@
 //   BF16X3  inputs split x = h + m (+ rest), each BF16; products
 //           hh + hm + mh on v_mfma_f32_16x16x32_bf16, FP32 accumulate
 //   BF16X6  x = h + m + l; products hh+hm+mh+hl+mm+lh, FP32 accumulate
-// Every output element is computed by exactly one workgroup, accumulating
-// over K in increasing order, one MFMA K-step at a time, for every tile
-// shape: results do not depend on the tile shape or on what else is in the
-// group (no split-K, no atomics).
+// Every output element is computed by exactly one workgroup, in a K order
+// fixed per arithmetic mode (not per tile shape): results do not depend on
+// the tile shape or on what else is in the group (no split-K, no atomics).
 //
 // MFMA operand layout used (16x16 output tile, wave of 64 lanes):
 //   16x16x4 (f64/f32): lane l supplies A[l%16][l/16] and B[l/16][l%16];
@@ -54,10 +53,12 @@ struct TileDev {
   int32_t pad;
 };
 
-// Index map on the device: n levels (0 = plain), extents with their 32-bit
-// division magic (q = (umulhi(i, magic) + i) >> shift, exact for i < 2^31).
+// Index map on the device: always TN_MAX_MAP_LEVELS levels, the real ones
+// right-aligned (last = fastest) and the rest of extent 1 and stride 0, so
+// decoding is branch-free; the plain layout is the single level
+// (range, leading dimension or 1).  Each extent carries its 32-bit division
+// magic: q = (umulhi(i, magic) + i) >> shift, exact for i < 2^31.
 struct MapDev {
-  int32_t n;
   uint32_t ext[TN_MAX_MAP_LEVELS];
   uint32_t magic[TN_MAX_MAP_LEVELS];
   uint32_t shift[TN_MAX_MAP_LEVELS];
@@ -71,39 +72,60 @@ struct GroupDev {
   MapDev a_row, a_col, b_row, b_col, c_row, c_col;
 };
 
-__device__ __forceinline__ int64_t map_off(const MapDev& m, int64_t i64, int64_t plain_mul) {
-  if (m.n == 0) return i64 * plain_mul;
+__device__ __forceinline__ int64_t map_off(const MapDev& m, int64_t i64) {
   uint32_t i = (uint32_t)i64;
   int64_t off = 0;
 #pragma unroll
   for (int l = TN_MAX_MAP_LEVELS - 1; l >= 0; --l) {
-    if (l < m.n) {
-      const uint32_t q = (__umulhi(i, m.magic[l]) + i) >> m.shift[l];
-      off += (int64_t)(i - q * m.ext[l]) * m.stride[l];
-      i = q;
-    }
+    const uint32_t q = (__umulhi(i, m.magic[l]) + i) >> m.shift[l];
+    off += (int64_t)(i - q * m.ext[l]) * m.stride[l];
+    i = q;
   }
   return off;
 }
 
 // ----------------------------------------------------------------------------
-// FP64 / FP32 kernel: BM x BN block tile, BK deep, 256 threads = 2x2 waves.
-template <typename T, int BM, int BN, int BK>
-__global__ void __launch_bounds__(256) gemm_mfma_16x16x4(const double* __restrict__ A,
-                                                         const double* __restrict__ B,
-                                                         double* __restrict__ C,
-                                                         const GroupDev* __restrict__ groups,
-                                                         const TileDev* __restrict__ tiles) {
-  static_assert(256 % BK == 0 && 256 % BN == 0, "tile shape");
-  constexpr int WM = BM / 2, WN = BN / 2;          // per-wave tile
-  constexpr int TM = WM / 16, TN = WN / 16;        // MFMA tiles per wave
-  constexpr int PAD = (sizeof(T) == 8) ? 1 : 2;
-  constexpr int A_PER = BM * BK / 256;             // A elements per thread per K step
-  constexpr int B_PER = BN * BK / 256;
-  constexpr int A_RSTEP = 256 / BK;                // A rows covered per pass
-  constexpr int B_KSTEP = 256 / BN;                // B k-rows covered per pass
-  __shared__ T As[BK][BM + PAD];
-  __shared__ T Bs[BK][BN + PAD];
+// FP64 / FP32 kernel.  256 threads = 2x2 waves; the block owns a BM x BN
+// output tile, each wave a (BM/2) x (BN/2) sub-tile of TM x TN MFMA tiles.
+//
+// K order (the same for every tile shape, so results never depend on it):
+// K runs in 32-wide chunks; in MFMA step s (0..7) of chunk c, lane group
+// g = lane/16 supplies k = 32c + 8g + s, and step s accumulates into chain
+// s mod 2; C = chain0 + chain1 at the end.  Two independent accumulation
+// chains keep the matrix core busy when a wave owns a single 16x16 tile.
+//
+// Pipeline: a stage is KC (a multiple of 32) consecutive k.  Global loads
+// of stage t+D are in flight (registers, D slots) while stage t is
+// multiplied out of LDS; LDS is double-buffered with one barrier per stage,
+// so a deeper stage amortises the barrier and the LDS read latency over
+// more MFMAs.  Each thread loads element pairs along the contiguous axis:
+// A pairs (k, k+1), B pairs (n, n+1); VEC = one 16-byte load per pair (the
+// host checks that every pair is contiguous and 16-byte aligned).
+constexpr int kKTab = 2048;   // longest K whose offsets are tabulated in LDS
+
+template <typename T, int BM, int BN, int KC, int D, bool VEC, bool KT>
+__global__ void __launch_bounds__(256) gemm_mfma_k32(const double* __restrict__ A,
+                                                     const double* __restrict__ B,
+                                                     double* __restrict__ C,
+                                                     const GroupDev* __restrict__ groups,
+                                                     const TileDev* __restrict__ tiles) {
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int NPA = KC / 2;                 // A pairs per row of a stage
+  constexpr int AROWS = 256 / NPA;            // A rows per pass of the block
+  constexpr int PA = BM / AROWS;              // A pairs per thread per stage
+  constexpr int NPB = BN / 2;                 // B pairs per k row
+  constexpr int BROWS = 256 / NPB;            // k rows per pass of the block
+  constexpr int PB = KC / BROWS;              // B pairs per thread per stage
+  constexpr int PAD = 16 / (int)sizeof(T);    // keeps rows 16-byte aligned
+  static_assert(KC % 32 == 0 && TM >= 1 && TN >= 1 && 256 % NPA == 0 && BM % AROWS == 0 &&
+                    256 % NPB == 0 && KC % BROWS == 0,
+                "tile shape");
+  typedef typename std::conditional<sizeof(T) == 8, d4, f4>::type acc_t;
+  typedef T t2 __attribute__((ext_vector_type(2)));
+  __shared__ __attribute__((aligned(16))) T As[2][BM][KC + PAD];
+  __shared__ __attribute__((aligned(16))) T Bs[2][KC][BN + PAD];
+  __shared__ int32_t tabA[KT ? kKTab : 1], tabB[KT ? kKTab : 1];
 
   const TileDev tile = tiles[blockIdx.x];
   if (tile.group < 0) return;
@@ -117,89 +139,188 @@ __global__ void __launch_bounds__(256) gemm_mfma_16x16x4(const double* __restric
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
+  const int lg = lane >> 4, lr = lane & 15;
   const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
 
-  // fixed per-thread offsets
-  const int a_kk = tid % BK;
-  const int a_mm0 = tid / BK;
-  int64_t a_roff[A_PER];
+  // Out-of-range rows, columns and k are clamped to the last valid index for
+  // addressing (every load reads valid memory, with no branches) and their
+  // values are zeroed when the stage is written to LDS.
+  const int a_kp = tid % NPA;                 // pair (2*a_kp, 2*a_kp+1) of the stage
+  const int a_r0 = tid / NPA;                 // rows a_r0 + AROWS*i
+  int64_t a_roff[PA];
+  bool a_rv[PA];
 #pragma unroll
-  for (int i = 0; i < A_PER; ++i) {
-    const int64_t m = m0 + a_mm0 + i * A_RSTEP;
-    a_roff[i] = (m < M) ? map_off(g.a_row, m, g.lda) : -1;
+  for (int i = 0; i < PA; ++i) {
+    const int64_t m = m0 + a_r0 + AROWS * i;
+    a_rv[i] = m < M;
+    a_roff[i] = map_off(g.a_row, a_rv[i] ? m : M - 1);
   }
-  const int b_nn = tid % BN;
-  const int b_kk0 = tid / BN;
-  const int64_t b_coff = (n0 + b_nn < N) ? map_off(g.b_col, n0 + b_nn, 1) : -1;
+  const int b_np = tid % NPB;
+  const int b_kr0 = tid / NPB;                // k rows b_kr0 + BROWS*i
+  const int64_t bn = n0 + 2 * b_np;
+  const bool b_ok0 = bn < N, b_ok1 = bn + 1 < N;
+  const int64_t b_coff0 = map_off(g.b_col, std::min<int64_t>(bn, VEC ? N - 2 : N - 1));
+  const int64_t b_coff1 = VEC ? 0 : map_off(g.b_col, std::min<int64_t>(bn + 1, N - 1));
+  const int64_t a_kmax = VEC ? K - 2 : K - 1;
 
-  typedef typename std::conditional<sizeof(T) == 8, d4, f4>::type acc_t;
-  acc_t acc[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+  // K-side offsets: with KT (K <= kKTab, offsets below 2^31) the block
+  // tabulates a_col(k) and b_row(k) in LDS once, so a stage costs one LDS
+  // read per offset instead of a multi-level decode.
+  const MapDev a_col = g.a_col, b_row = g.b_row;
+  if constexpr (KT) {
+    for (int k = tid; k < K; k += 256) {
+      tabA[k] = (int32_t)map_off(a_col, k);
+      tabB[k] = (int32_t)map_off(b_row, k);
+    }
+    __syncthreads();
+  }
+  auto koff_a = [&](int64_t k) -> int64_t { return KT ? (int64_t)tabA[k] : map_off(a_col, k); };
+  auto koff_b = [&](int64_t k) -> int64_t { return KT ? (int64_t)tabB[k] : map_off(b_row, k); };
 
-  T ra[A_PER], rb[B_PER];
-  auto load_tiles = [&](int64_t k0) {
-    const int64_t ka = k0 + a_kk;
-    const int64_t a_coff = (ka < K) ? map_off(g.a_col, ka, 1) : -1;
+  acc_t acc[2][TM][TN];
 #pragma unroll
-    for (int i = 0; i < A_PER; ++i)
-      ra[i] = (a_roff[i] >= 0 && a_coff >= 0) ? (T)Ab[a_roff[i] + a_coff] : (T)0;
+  for (int h = 0; h < 2; ++h)
 #pragma unroll
-    for (int i = 0; i < B_PER; ++i) {
-      const int64_t kb = k0 + b_kk0 + i * B_KSTEP;
-      const int64_t b_roff = (kb < K) ? map_off(g.b_row, kb, g.ldb) : -1;
-      rb[i] = (b_roff >= 0 && b_coff >= 0) ? (T)Bb[b_roff + b_coff] : (T)0;
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[h][i][j] = acc_t{0, 0, 0, 0};
+
+  // Every stage issues the same loads, so the waits count them exactly; the
+  // zeroing happens at the LDS write, so no wait follows a load.
+  double ra[D][PA][2], rb[D][PB][2];
+  auto load = [&](int d, int64_t t) {
+    const int64_t ka = t * KC + 2 * a_kp;
+    const int64_t ao0 = koff_a(std::min<int64_t>(ka, a_kmax));
+    if constexpr (VEC) {
+#pragma unroll
+      for (int i = 0; i < PA; ++i) {
+        const double2 v = *reinterpret_cast<const double2*>(Ab + a_roff[i] + ao0);
+        ra[d][i][0] = v.x;
+        ra[d][i][1] = v.y;
+      }
+    } else {
+      const int64_t ao1 = koff_a(std::min<int64_t>(ka + 1, K - 1));
+#pragma unroll
+      for (int i = 0; i < PA; ++i) {
+        ra[d][i][0] = Ab[a_roff[i] + ao0];
+        ra[d][i][1] = Ab[a_roff[i] + ao1];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < PB; ++i) {
+      const int64_t kb = t * KC + b_kr0 + BROWS * i;
+      const int64_t bo = koff_b(std::min<int64_t>(kb, K - 1));
+      if constexpr (VEC) {
+        const double2 v = *reinterpret_cast<const double2*>(Bb + bo + b_coff0);
+        rb[d][i][0] = v.x;
+        rb[d][i][1] = v.y;
+      } else {
+        rb[d][i][0] = Bb[bo + b_coff0];
+        rb[d][i][1] = Bb[bo + b_coff1];
+      }
     }
   };
-  auto store_tiles = [&]() {
+  // write register slot d (holding stage t) to LDS buffer st, zeroing the
+  // elements outside the matrix
+  auto store = [&](int d, int st, int64_t t) {
+    const int64_t ka = t * KC + 2 * a_kp;
+    const bool kv0 = ka < K, kv1 = ka + 1 < K;
 #pragma unroll
-    for (int i = 0; i < A_PER; ++i) As[a_kk][a_mm0 + i * A_RSTEP] = ra[i];
+    for (int i = 0; i < PA; ++i)
+      *reinterpret_cast<t2*>(&As[st][a_r0 + AROWS * i][2 * a_kp]) =
+          t2{(T)(a_rv[i] && kv0 ? ra[d][i][0] : 0.0), (T)(a_rv[i] && kv1 ? ra[d][i][1] : 0.0)};
 #pragma unroll
-    for (int i = 0; i < B_PER; ++i) Bs[b_kk0 + i * B_KSTEP][b_nn] = rb[i];
+    for (int i = 0; i < PB; ++i) {
+      const bool kv = t * KC + b_kr0 + BROWS * i < K;
+      *reinterpret_cast<t2*>(&Bs[st][b_kr0 + BROWS * i][2 * b_np]) =
+          t2{(T)(kv && b_ok0 ? rb[d][i][0] : 0.0), (T)(kv && b_ok1 ? rb[d][i][1] : 0.0)};
+    }
   };
-
-  const int64_t nk = (K + BK - 1) / BK;
-  load_tiles(0);
-  for (int64_t kt = 0; kt < nk; ++kt) {
-    __syncthreads();
-    store_tiles();
-    __syncthreads();
-    if (kt + 1 < nk) load_tiles((kt + 1) * BK);  // next global load overlaps the MFMAs
+  // Operands of one 32-k chunk: A (TM x 8 values) and B (TN x 8 values) per
+  // lane.  The next chunk's operands are read from LDS into separate
+  // registers before the current chunk's MFMAs issue, so LDS latency hides
+  // behind the matrix core and no read waits on an MFMA's source registers.
+  struct Frag {
+    t2 a[4][TM];
+    T b[4][2][TN];
+  };
+  auto read_frag = [&](Frag& f, int st, int u) {
 #pragma unroll
-    for (int ks = 0; ks < BK; ks += 4) {
-      T a[TM], b[TN];
-      const int kr = ks + (lane >> 4);
-#pragma unroll
-      for (int i = 0; i < TM; ++i) a[i] = As[kr][wm0 + i * 16 + (lane & 15)];
-#pragma unroll
-      for (int j = 0; j < TN; ++j) b[j] = Bs[kr][wn0 + j * 16 + (lane & 15)];
+    for (int q = 0; q < 4; ++q) {
 #pragma unroll
       for (int i = 0; i < TM; ++i)
+        f.a[q][i] = *reinterpret_cast<const t2*>(&As[st][wm0 + 16 * i + lr][32 * u + 8 * lg + 2 * q]);
 #pragma unroll
-        for (int j = 0; j < TN; ++j) {
-          if constexpr (sizeof(T) == 8)
-            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
-          else
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) f.b[q][h][j] = Bs[st][32 * u + 8 * lg + 2 * q + h][wn0 + 16 * j + lr];
+    }
+  };
+  auto mfma_frag = [&](const Frag& f) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            if constexpr (sizeof(T) == 8)
+              acc[h][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(f.a[q][i][h], f.b[q][h][j], acc[h][i][j], 0, 0, 0);
+            else
+              acc[h][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(f.a[q][i][h], f.b[q][h][j], acc[h][i][j], 0, 0, 0);
+          }
+  };
+  // (sched_barrier keeps the scheduler from sinking the reads back next to
+  // their MFMAs, which would serialise every MFMA behind an LDS round trip.)
+  auto compute = [&](int st) {
+    Frag f[2];
+    read_frag(f[0], st, 0);
+#pragma unroll
+    for (int u = 0; u < KC / 32; ++u) {
+      if (u + 1 < KC / 32) read_frag(f[(u + 1) & 1], st, u + 1);
+      __builtin_amdgcn_sched_barrier(0);
+      mfma_frag(f[u & 1]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  };
+
+  const int64_t ns = (K + KC - 1) / KC;
+#pragma unroll
+  for (int d = 0; d < D; ++d) load(d, d);
+  store(0, 0, 0);
+  load(0, D);
+  __syncthreads();
+  for (int64_t tb = 0; tb < ns; tb += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int64_t t = tb + d;
+      // Only the multiply is skipped past the last stage: every iteration
+      // stores stage t+1 (zeros past the end, never read) to the other LDS
+      // buffer and loads stage t+1+D into the freed register slot, so all
+      // iterations issue the same memory operations.
+      if (t < ns) compute((int)(t & 1));
+      store((d + 1) % D, (int)((t + 1) & 1), t + 1);
+      load((d + 1) % D, t + 1 + D);
+      __syncthreads();
     }
   }
-  // write back
+
+  // write back: C = chain0 + chain1
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
-    const int64_t gn = n0 + wn0 + j * 16 + (lane & 15);
+    const int64_t gn = n0 + wn0 + j * 16 + lr;
     if (gn >= N) continue;
-    const int64_t coff = map_off(g.c_col, gn, 1);
+    const int64_t coff = map_off(g.c_col, gn);
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         // accumulator row: f64 16x16x4 -> (l/16) + 4v ; f32 16x16x4 -> 4(l/16) + v
-        const int row = (sizeof(T) == 8) ? ((lane >> 4) + 4 * v) : (4 * (lane >> 4) + v);
+        const int row = (sizeof(T) == 8) ? (lg + 4 * v) : (4 * lg + v);
         const int64_t gm = m0 + wm0 + i * 16 + row;
-        if (gm < M) Cb[map_off(g.c_row, gm, g.ldc) + coff] = (double)acc[i][j][v];
+        if (gm < M)
+          Cb[map_off(g.c_row, gm) + coff] = (double)acc[0][i][j][v] + (double)acc[1][i][j][v];
       }
   }
 }
@@ -252,11 +373,11 @@ __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict_
 #pragma unroll
   for (int i = 0; i < A_PER; ++i) {
     const int64_t m = m0 + a_mm0 + i * A_RSTEP;
-    a_roff[i] = (m < M) ? map_off(g.a_row, m, g.lda) : -1;
+    a_roff[i] = (m < M) ? map_off(g.a_row, m) : -1;
   }
   const int b_nn = tid % BN;
   const int b_kk0 = tid / BN;
-  const int64_t b_coff = (n0 + b_nn < N) ? map_off(g.b_col, n0 + b_nn, 1) : -1;
+  const int64_t b_coff = (n0 + b_nn < N) ? map_off(g.b_col, n0 + b_nn) : -1;
 
   f4 acc[TM][TN];
 #pragma unroll
@@ -267,14 +388,14 @@ __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict_
   float ra[A_PER], rb[B_PER];
   auto load_tiles = [&](int64_t k0) {
     const int64_t ka = k0 + a_kk;
-    const int64_t a_coff = (ka < K) ? map_off(g.a_col, ka, 1) : -1;
+    const int64_t a_coff = (ka < K) ? map_off(g.a_col, ka) : -1;
 #pragma unroll
     for (int i = 0; i < A_PER; ++i)
       ra[i] = (a_roff[i] >= 0 && a_coff >= 0) ? (float)Ab[a_roff[i] + a_coff] : 0.0f;
 #pragma unroll
     for (int i = 0; i < B_PER; ++i) {
       const int64_t kb = k0 + b_kk0 + i * B_KSTEP;
-      const int64_t b_roff = (kb < K) ? map_off(g.b_row, kb, g.ldb) : -1;
+      const int64_t b_roff = (kb < K) ? map_off(g.b_row, kb) : -1;
       rb[i] = (b_roff >= 0 && b_coff >= 0) ? (float)Bb[b_roff + b_coff] : 0.0f;
     }
   };
@@ -339,47 +460,54 @@ __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict_
   for (int j = 0; j < TN; ++j) {
     const int64_t gn = n0 + wn0 + j * 16 + (lane & 15);
     if (gn >= N) continue;
-    const int64_t coff = map_off(g.c_col, gn, 1);
+    const int64_t coff = map_off(g.c_col, gn);
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int64_t gm = m0 + wm0 + i * 16 + 4 * (lane >> 4) + v;
-        if (gm < M) Cb[map_off(g.c_row, gm, g.ldc) + coff] = (double)acc[i][j][v];
+        if (gm < M) Cb[map_off(g.c_row, gm) + coff] = (double)acc[i][j][v];
       }
   }
 }
 
 // ----------------------------------------------------------------------------
-// Tile configurations (autotuner ids).
+// Tile configurations (autotuner ids), per arithmetic mode.
 struct TileCfg {
   int bm, bn;
 };
-constexpr TileCfg kCfgs[] = {{32, 32}, {64, 64}, {128, 64}, {64, 128}, {128, 128}};
-constexpr int kNumCfg = sizeof(kCfgs) / sizeof(kCfgs[0]);
+constexpr int kNumCfg = 5;
+constexpr TileCfg kCfgMfma[kNumCfg] = {{32, 32}, {64, 64}, {128, 64}, {64, 128}, {64, 32}};
+constexpr TileCfg kCfgBf16[kNumCfg] = {{32, 32}, {64, 64}, {128, 64}, {64, 128}, {128, 128}};
+
+TileCfg cfg_shape(int mode, int cfg) {
+  return (mode == TN_GEMM_F64 || mode == TN_GEMM_F32) ? kCfgMfma[cfg] : kCfgBf16[cfg];
+}
 
 typedef void (*KernFn)(const double*, const double*, double*, const GroupDev*, const TileDev*);
 
-KernFn pick_kernel(int mode, int cfg) {
+template <typename T, bool VEC, bool KT>
+KernFn pick_mfma(int cfg) {
+  switch (cfg) {
+    case 0: return gemm_mfma_k32<T, 32, 32, 128, 2, VEC, KT>;
+    case 1: return gemm_mfma_k32<T, 64, 64, 64, 2, VEC, KT>;
+    case 2: return gemm_mfma_k32<T, 128, 64, 32, 2, VEC, KT>;
+    case 3: return gemm_mfma_k32<T, 64, 128, 32, 2, VEC, KT>;
+    case 4: return gemm_mfma_k32<T, 64, 32, 64, 2, VEC, KT>;
+  }
+  return nullptr;
+}
+
+template <typename T>
+KernFn pick_mfma(int cfg, bool vec, bool kt) {
+  if (vec) return kt ? pick_mfma<T, true, true>(cfg) : pick_mfma<T, true, false>(cfg);
+  return kt ? pick_mfma<T, false, true>(cfg) : pick_mfma<T, false, false>(cfg);
+}
+
+KernFn pick_kernel(int mode, int cfg, bool vec, bool kt) {
   switch (mode) {
-    case TN_GEMM_F64:
-      switch (cfg) {
-        case 0: return gemm_mfma_16x16x4<double, 32, 32, 128>;
-        case 1: return gemm_mfma_16x16x4<double, 64, 64, 64>;
-        case 2: return gemm_mfma_16x16x4<double, 128, 64, 32>;
-        case 3: return gemm_mfma_16x16x4<double, 64, 128, 32>;
-        case 4: return gemm_mfma_16x16x4<double, 128, 128, 16>;
-      }
-      break;
-    case TN_GEMM_F32:
-      switch (cfg) {
-        case 0: return gemm_mfma_16x16x4<float, 32, 32, 128>;
-        case 1: return gemm_mfma_16x16x4<float, 64, 64, 64>;
-        case 2: return gemm_mfma_16x16x4<float, 128, 64, 32>;
-        case 3: return gemm_mfma_16x16x4<float, 64, 128, 32>;
-        case 4: return gemm_mfma_16x16x4<float, 128, 128, 16>;
-      }
-      break;
+    case TN_GEMM_F64: return pick_mfma<double>(cfg, vec, kt);
+    case TN_GEMM_F32: return pick_mfma<float>(cfg, vec, kt);
     case TN_GEMM_BF16X3:
       switch (cfg) {
         case 0: return gemm_split_bf16<32, 32, 2, 128>;
@@ -402,6 +530,35 @@ KernFn pick_kernel(int mode, int cfg) {
   return nullptr;
 }
 
+// Every (k, k+1) pair of A and (n, n+1) pair of B that the FP64/FP32 kernel
+// loads is contiguous and at an even element offset: then one 16-byte load
+// reads it (given 16-byte aligned base pointers, checked at launch).
+bool map_pairs_ok(const tn_index_map& m, int64_t range) {
+  if (m.n == 0) return range % 2 == 0;
+  if (m.stride[m.n - 1] != 1 || m.ext[m.n - 1] % 2 != 0) return false;
+  for (int l = 0; l + 1 < m.n; ++l)
+    if (m.ext[l] > 1 && m.stride[l] % 2 != 0) return false;
+  return true;
+}
+bool map_even(const tn_index_map& m, int64_t ld, int64_t range) {
+  if (m.n == 0) return range <= 1 || ld % 2 == 0;
+  for (int l = 0; l < m.n; ++l)
+    if (m.ext[l] > 1 && m.stride[l] % 2 != 0) return false;
+  return true;
+}
+// Largest offset an index map produces over [0, range).
+int64_t map_max_off(const tn_index_map& m, int64_t range, int64_t plain_stride) {
+  if (range <= 0) return 0;
+  if (m.n == 0) return (range - 1) * plain_stride;
+  int64_t o = 0;
+  for (int l = 0; l < m.n; ++l) o += (m.ext[l] - 1) * m.stride[l];
+  return o;
+}
+bool desc_vec_ok(const tn_gemm_desc_ex& x) {
+  return x.a_off % 2 == 0 && x.b_off % 2 == 0 && map_pairs_ok(x.a_col, x.K) && map_even(x.a_row, x.lda, x.M) &&
+         map_pairs_ok(x.b_col, x.N) && map_even(x.b_row, x.ldb, x.K);
+}
+
 // Host: 32-bit division magic for divisor d >= 1 (exact for dividends < 2^31).
 void div_magic(uint32_t d, uint32_t* magic, uint32_t* shift) {
   uint32_t l = 0;
@@ -410,22 +567,34 @@ void div_magic(uint32_t d, uint32_t* magic, uint32_t* shift) {
   *magic = (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << l) - d)) / d + 1);
 }
 
-bool make_map(const tn_index_map& in, int64_t range, MapDev* out) {
+bool make_map(const tn_index_map& in, int64_t range, int64_t plain_stride, MapDev* out) {
   std::memset(out, 0, sizeof(*out));
-  if (in.n == 0 || range == 0) return true;
-  if (in.n < 0 || in.n > TN_MAX_MAP_LEVELS) return false;
-  int64_t prod = 1;
-  for (int l = 0; l < in.n; ++l) {
-    if (in.ext[l] < 1 || in.ext[l] >= (int64_t(1) << 31) || in.stride[l] < 0) return false;
-    prod *= in.ext[l];
-    if (prod >= (int64_t(1) << 31)) return false;
+  int64_t ext[TN_MAX_MAP_LEVELS], str[TN_MAX_MAP_LEVELS];
+  int n = in.n;
+  if (n < 0 || n > TN_MAX_MAP_LEVELS) return false;
+  if (range == 0) n = 0;
+  if (n == 0) {
+    n = 1;
+    ext[0] = std::max<int64_t>(range, 1);
+    str[0] = plain_stride;
+  } else {
+    int64_t prod = 1;
+    for (int l = 0; l < n; ++l) {
+      if (in.ext[l] < 1 || in.ext[l] >= (int64_t(1) << 31) || in.stride[l] < 0) return false;
+      prod *= in.ext[l];
+      if (prod >= (int64_t(1) << 31)) return false;
+      ext[l] = in.ext[l];
+      str[l] = in.stride[l];
+    }
+    if (prod != range) return false;
   }
-  if (prod != range) return false;
-  out->n = in.n;
-  for (int l = 0; l < in.n; ++l) {
-    out->ext[l] = (uint32_t)in.ext[l];
+  if (ext[0] >= (int64_t(1) << 31)) return false;
+  const int pad = TN_MAX_MAP_LEVELS - n;
+  for (int l = 0; l < TN_MAX_MAP_LEVELS; ++l) {
+    const bool real = l >= pad;
+    out->ext[l] = real ? (uint32_t)ext[l - pad] : 1u;
+    out->stride[l] = real ? str[l - pad] : 0;
     div_magic(out->ext[l], &out->magic[l], &out->shift[l]);
-    out->stride[l] = in.stride[l];
   }
   return true;
 }
@@ -441,10 +610,14 @@ struct tn_gemm_plan_s {
   int64_t ntiles = 0;     // real tiles
   int64_t nslots = 0;     // launched blocks (with XCD padding)
   double flops = 0;
+  bool vec_ok = true;     // every group admits 16-byte pair loads
+  int64_t max_k = 0;      // longest K over the groups
+  bool koff32 = true;     // every K-side offset fits in int32 (LDS tables)
 };
 
 static int build_tiles(tn_gemm_plan_s* P, int cfg) {
-  const int bm = kCfgs[cfg].bm, bn = kCfgs[cfg].bn;
+  const TileCfg sh = cfg_shape(P->mode, cfg);
+  const int bm = sh.bm, bn = sh.bn;
   std::vector<TileDev> nat;
   for (size_t gi = 0; gi < P->groups.size(); ++gi) {
     const GroupDev& g = P->groups[gi];
@@ -480,15 +653,21 @@ static int build_tiles(tn_gemm_plan_s* P, int cfg) {
 // Default tile: the largest tile that still gives every CU at least one
 // tile; otherwise the smallest tile.
 static int default_cfg(const tn_gemm_plan_s* P) {
-  const int order[] = {4, 2, 1, 0};
+  int order[kNumCfg];
+  for (int c = 0; c < kNumCfg; ++c) order[c] = c;
+  std::sort(order, order + kNumCfg, [&](int a, int b) {
+    const TileCfg x = cfg_shape(P->mode, a), y = cfg_shape(P->mode, b);
+    return x.bm * x.bn > y.bm * y.bn || (x.bm * x.bn == y.bm * y.bn && a < b);
+  });
   const int64_t want = tn::cu_count();
   for (int c : order) {
+    const TileCfg sh = cfg_shape(P->mode, c);
     int64_t t = 0;
     for (const auto& g : P->groups)
-      t += ((g.M + kCfgs[c].bm - 1) / kCfgs[c].bm) * ((g.N + kCfgs[c].bn - 1) / kCfgs[c].bn);
+      t += ((g.M + sh.bm - 1) / sh.bm) * ((g.N + sh.bn - 1) / sh.bn);
     if (t >= want) return c;
   }
-  return 0;
+  return order[kNumCfg - 1];
 }
 
 static void destroy_plan(tn_gemm_plan_s* P) {
@@ -523,9 +702,9 @@ int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups, const 
                        (x.b_row.n || x.K == 0 || x.N == 0 || x.ldb >= x.N) &&
                        (x.c_row.n || x.M == 0 || x.N == 0 || x.ldc >= x.N);
     const bool range_ok = x.M < (int64_t(1) << 31) && x.N < (int64_t(1) << 31) && x.K < (int64_t(1) << 31);
-    const bool maps_ok = make_map(x.a_row, x.M, &G.a_row) && make_map(x.a_col, x.K, &G.a_col) &&
-                         make_map(x.b_row, x.K, &G.b_row) && make_map(x.b_col, x.N, &G.b_col) &&
-                         make_map(x.c_row, x.M, &G.c_row) && make_map(x.c_col, x.N, &G.c_col);
+    const bool maps_ok = make_map(x.a_row, x.M, x.lda, &G.a_row) && make_map(x.a_col, x.K, 1, &G.a_col) &&
+                         make_map(x.b_row, x.K, x.ldb, &G.b_row) && make_map(x.b_col, x.N, 1, &G.b_col) &&
+                         make_map(x.c_row, x.M, x.ldc, &G.c_row) && make_map(x.c_col, x.N, 1, &G.c_col);
     if (x.M < 0 || x.N < 0 || x.K < 0 || x.a_off < 0 || x.b_off < 0 || x.c_off < 0 || !ld_ok || !range_ok ||
         !maps_ok) {
       delete P;
@@ -534,6 +713,12 @@ int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups, const 
     }
     P->groups.push_back(G);
     P->flops += 2.0 * (double)x.M * (double)x.N * (double)x.K;
+    if (x.M > 0 && x.N > 0 && x.K > 0 && !desc_vec_ok(x)) P->vec_ok = false;
+    if (x.M > 0 && x.N > 0) {
+      P->max_k = std::max(P->max_k, x.K);
+      if (map_max_off(x.a_col, x.K, 1) >= (int64_t(1) << 31) || map_max_off(x.b_row, x.K, x.ldb) >= (int64_t(1) << 31))
+        P->koff32 = false;
+    }
   }
   int rc = TN_OK;
   if (!P->groups.empty()) {
@@ -628,7 +813,7 @@ __global__ void zero_c_kernel(double* __restrict__ C, const GroupDev* __restrict
   const int64_t total = g.M * g.N;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     int64_t m = e / g.N, n = e % g.N;
-    C[g.c_off + map_off(g.c_row, m, g.ldc) + map_off(g.c_col, n, 1)] = 0.0;
+    C[g.c_off + map_off(g.c_row, m) + map_off(g.c_col, n)] = 0.0;
   }
 }
 
@@ -647,7 +832,8 @@ int tn_gemm_exec(tn_gemm_plan P, const double* A, const double* B, double* C, vo
   }
   if (P->nslots == 0) return TN_OK;
   TN_REQUIRE(A && B && C, TN_EINVAL, "tn_gemm_exec: NULL operand");
-  KernFn k = pick_kernel(P->mode, P->cfg);
+  const bool vec = P->vec_ok && ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) & 15) == 0;
+  KernFn k = pick_kernel(P->mode, P->cfg, vec, P->koff32 && P->max_k <= kKTab);
   TN_REQUIRE(k != nullptr, TN_EINVAL, "no kernel for mode %d tile %d", P->mode, P->cfg);
   hipLaunchKernelGGL(k, dim3((unsigned)P->nslots), dim3(256), 0, s, A, B, C, P->d_groups, P->d_tiles);
   TN_LAUNCH_CHECK();
