@@ -195,6 +195,32 @@ int tn_svd_f64_batched(int64_t batch, const int64_t* m, const int64_t* n,
 /* Jacobi sweeps used by the last SVD call (batch: the largest) on this
  * thread; a convergence diagnostic. */
 int tn_svd_last_sweeps(int* sweeps);
+
+/* Warm start for a nearby matrix (M >= N): V0T is an N x N matrix with
+ * orthonormal rows, e.g. the VT of a previous SVD of a matrix close to A;
+ * the Jacobi sweeps start from W = A V0 instead of A, which cuts the sweeps
+ * when A changes little between calls (successive TEBD updates).  The result
+ * is the SVD of A, as from tn_svd_f64.  No counterpart in the reference. */
+int tn_svd_f64_warm(const double* A, int64_t M, int64_t N, const double* V0T, double* U,
+                    double* S, double* VT, void* stream);
+
+/* A reusable batch of SVD problems (shapes and offsets fixed at creation;
+ * v0_off[i] >= 0 gives problem i a warm start from V0T + v0_off[i], NULL or
+ * -1 = cold).  tn_svd_plan_exec only launches kernels: it neither allocates
+ * nor synchronises, so it can be captured into a graph (tn_graph_begin).
+ * tn_svd_plan_check waits for the stream and reports the problems of the
+ * last execution that did not converge and the most sweeps any needed.
+ * The batched matrixSVD of many chains (uni10_lapack_cpu.cpp:289-313 per
+ * chain in the reference). */
+typedef struct tn_svd_plan_s* tn_svd_plan;
+int tn_svd_plan_create(tn_svd_plan* plan, int64_t batch, const int64_t* m, const int64_t* n,
+                       const int64_t* a_off, const int64_t* v0_off, const int64_t* u_off,
+                       const int64_t* s_off, const int64_t* vt_off);
+int tn_svd_plan_exec(tn_svd_plan plan, const double* A, const double* V0T, double* U, double* S,
+                     double* VT, void* stream);
+int tn_svd_plan_check(tn_svd_plan plan, int* nonconverged, int* sweeps, void* stream);
+int tn_svd_plan_destroy(tn_svd_plan plan);
+
 /* eigSyDecompose (uni10_lapack_cpu.cpp:131-152): symmetric N x N;
  * eigenvalues ascending; eigenvectors returned as ROWS of EigVec. */
 int tn_eigh_f64(const double* A, int64_t N, double* eig, double* eigvec_rows,

@@ -377,6 +377,95 @@ def test_svd_batched_uneven(tn, ora):
         assert np.max(np.abs(Sall[s_off[i]:s_off[i] + ks[i]] - sr)) <= 1e-12 * sr.max()
 
 
+def _sweeps(tn):
+    v = ctypes.c_int()
+    tn.call("tn_svd_last_sweeps", ctypes.byref(v))
+    return v.value
+
+
+@pytest.mark.parametrize("n", [16, 64, 100])
+def test_svd_warm_start_nearby_matrix(tn, ora, n):
+    """Warm start from the VT of a nearby matrix: the same SVD (LAPACK
+    singular values, reconstruction, orthonormal factors) in fewer sweeps."""
+    _say("svd warm n=%d" % n)
+    rng = np.random.default_rng(n)
+    a0 = rng.uniform(-1, 1, (n, n))
+    a1 = a0 + 1e-3 * rng.uniform(-1, 1, (n, n))
+    A0 = tn.DeviceArray.from_host(a0)
+    _, _, VT0 = tn.svd(A0, n, n)
+    A1 = tn.DeviceArray.from_host(a1)
+    tn.svd(A1, n, n)
+    cold = _sweeps(tn)
+    U, S, VT = tn.DeviceArray((n, n)), tn.DeviceArray((n,)), tn.DeviceArray((n, n))
+    tn.call("tn_svd_f64_warm", tn._p(A1), n, n, tn._p(VT0), tn._p(U), tn._p(S), tn._p(VT), None)
+    warm = _sweeps(tn)
+    u, s, vt = U.to_host(), S.to_host(), VT.to_host()
+    sr = ora.svd(a1)[1]
+    assert np.max(np.abs(s - sr)) <= 1e-12 * sr[0]
+    assert np.linalg.norm(u @ np.diag(s) @ vt - a1) <= 1e-12 * np.linalg.norm(a1)
+    assert np.allclose(u.T @ u, np.eye(n), atol=1e-12)
+    assert np.allclose(vt @ vt.T, np.eye(n), atol=1e-12)
+    assert warm < cold, (warm, cold)
+
+
+def test_svd_plan_replay_and_graph(tn, ora):
+    """A batch plan over every kernel path (small, LDS, scratch), with one
+    warm-started problem: direct execution, a second execution and graph
+    replays all give the LAPACK singular values."""
+    shapes = [(64, 64), (30, 20), (100, 90), (160, 150), (64, 64)]
+    mats = [_rand(i + 70, s) for i, s in enumerate(shapes)]
+    A = np.concatenate([m.reshape(-1) for m in mats])
+    offs = np.cumsum([0] + [m.size for m in mats])[:-1].astype(np.int64)
+    ks = [min(s) for s in shapes]
+    u_off = np.cumsum([0] + [s[0] * k for s, k in zip(shapes, ks)])[:-1].astype(np.int64)
+    s_off = np.cumsum([0] + ks)[:-1].astype(np.int64)
+    vt_off = np.cumsum([0] + [k * s[1] for s, k in zip(shapes, ks)])[:-1].astype(np.int64)
+    m = np.array([s[0] for s in shapes], np.int64)
+    n = np.array([s[1] for s in shapes], np.int64)
+    # the last problem starts from the right singular vectors of the first
+    v0t = ora.svd(mats[0])[2]
+    v0_off = np.array([-1, -1, -1, -1, 0], np.int64)
+    dA = tn.DeviceArray.from_host(A)
+    dV0 = tn.DeviceArray.from_host(v0t)
+    dU = tn.DeviceArray((int(sum(s[0] * k for s, k in zip(shapes, ks))),))
+    dS = tn.DeviceArray((int(sum(ks)),))
+    dV = tn.DeviceArray((int(sum(k * s[1] for s, k in zip(shapes, ks))),))
+    h = ctypes.c_void_p()
+    tn.call("tn_svd_plan_create", ctypes.byref(h), len(shapes), tn._p(m), tn._p(n), tn._p(offs), tn._p(v0_off),
+            tn._p(u_off), tn._p(s_off), tn._p(vt_off))
+    plan = ctypes.c_void_p(h.value)
+    stream = tn.stream_create()
+
+    def run():
+        tn.call("tn_svd_plan_exec", plan, tn._p(dA), tn._p(dV0), tn._p(dU), tn._p(dS), tn._p(dV), stream)
+
+    def check():
+        nc, sw = ctypes.c_int(), ctypes.c_int()
+        tn.call("tn_svd_plan_check", plan, ctypes.byref(nc), ctypes.byref(sw), stream)
+        assert nc.value == 0 and sw.value > 0
+        Sall = dS.to_host()
+        for i, mat in enumerate(mats):
+            sr = ora.svd(mat)[1]
+            assert np.max(np.abs(Sall[s_off[i]:s_off[i] + ks[i]] - sr)) <= 1e-12 * sr.max()
+
+    def clear():
+        tn.call("tn_zero", tn._p(dS), dS.nbytes, stream)
+
+    try:
+        for _ in range(2):
+            clear()
+            run()
+            check()
+        g = tn.Graph(run, stream)
+        for _ in range(3):
+            clear()
+            g.launch()
+            check()
+        del g
+    finally:
+        tn.call("tn_svd_plan_destroy", plan)
+
+
 def test_eigh_heisenberg_gate(tn, ora):
     H = ora.heisenberg_two_site(1.0)
     dH = tn.DeviceArray.from_host(H)

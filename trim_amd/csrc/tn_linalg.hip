@@ -32,6 +32,7 @@ struct SvdProb {
   int64_t m, n;              // A is m x n row-major
   int64_t a_off, u_off, s_off, vt_off;
   int64_t scratch_off;       // offset (doubles) into the scratch area, -1 if LDS
+  int64_t v0_off;            // warm start: offset of an n x n V0^T (m >= n), -1 = none
 };
 
 // Sum of a, b, c over each aligned group of TP lanes (TP a power of two),
@@ -72,6 +73,7 @@ __device__ inline void rr_pair(int r, int k, int np, int& p, int& q) {
 template <bool kLds, int TP>
 __global__ void __launch_bounds__(kThreads) svd_jacobi_kernel(const SvdProb* __restrict__ probs,
                                                               const double* __restrict__ A,
+                                                              const double* __restrict__ V0T,
                                                               double* __restrict__ U,
                                                               double* __restrict__ S,
                                                               double* __restrict__ VT,
@@ -96,13 +98,22 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_kernel(const SvdProb* __r
   const double* Ab = A + pr.a_off;
 
   // W column-major: W[j*ldw + i]
-  for (int64_t e = tid; e < (int64_t)m * n; e += kThreads) {
-    int j = (int)(e / m), i = (int)(e % m);
-    W[(size_t)j * ldw + i] = tr ? Ab[(int64_t)j * N + i] : Ab[(int64_t)i * N + j];
-  }
+  // warm start (m >= n only): V = V0, W = A V0; otherwise V = I, W = A
+  const bool warm = pr.v0_off >= 0 && !tr;
   for (int64_t e = tid; e < (int64_t)n * n; e += kThreads) {
     int j = (int)(e / n), i = (int)(e % n);
-    V[(size_t)j * ldv + i] = (i == j) ? 1.0 : 0.0;
+    V[(size_t)j * ldv + i] = warm ? V0T[pr.v0_off + (int64_t)j * n + i] : ((i == j) ? 1.0 : 0.0);
+  }
+  if (warm) __syncthreads();
+  for (int64_t e = tid; e < (int64_t)m * n; e += kThreads) {
+    int j = (int)(e / m), i = (int)(e % m);
+    if (warm) {
+      double x = 0.0;
+      for (int k = 0; k < n; ++k) x += Ab[(int64_t)i * N + k] * V[(size_t)j * ldv + k];
+      W[(size_t)j * ldw + i] = x;
+    } else {
+      W[(size_t)j * ldw + i] = tr ? Ab[(int64_t)j * N + i] : Ab[(int64_t)i * N + j];
+    }
   }
   // ||A||_F^2 (fixed-order tree), invariant under the rotations
   {
@@ -429,24 +440,339 @@ unsigned grid_for(int64_t n) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 8192));
 }
 
-// Run a batch of SVD problems: LDS when W and V fit, device scratch otherwise.
-int svd_batched(int64_t batch, const int64_t* m, const int64_t* n, const double* A,
-                const int64_t* a_off, double* U, const int64_t* u_off, double* S,
-                const int64_t* s_off, double* VT, const int64_t* vt_off, hipStream_t st) {
-  if (batch == 0) return TN_OK;
-  std::vector<SvdProb> lds_probs, glb_probs;
+// ---------------------------------------------------------------------------
+// One-sided Jacobi for small matrices (n <= 64 columns of length m <= 8*EMX):
+// 32 groups of 8 lanes, one column pair per group and round.  A lane holds a
+// contiguous chunk of EMX elements of each of its two columns (and EN = 8 of
+// the two rotation columns) in registers for the round; columns live in LDS
+// between rounds, each lane chunk padded by 16 bytes so the 8 lanes of a
+// group hit disjoint banks.  The squared column norms are kept in LDS and
+// updated with each rotation (a' = a - t c, b' = b + t c), so a round needs
+// one dot product per pair; they are recomputed exactly at every sweep.
+constexpr int kSmallN = 64;      // columns
+constexpr int kSmallTP = 8;      // lanes per pair
+constexpr int kSmallEN = 8;      // V chunk per lane (n <= 64)
+
+// DPP moves within a row of 16 lanes: quad_perm [1,0,3,2] (lane ^ 1),
+// quad_perm [2,3,0,1] (lane ^ 2) and row_half_mirror (lane i <-> 7 - i in
+// each half row), so an 8-lane group sum takes three register moves instead
+// of LDS permutes; every lane ends with the same value (each step adds a
+// commuted pair).
+template <int CTRL>
+__device__ inline double dpp_mov(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline double group8_sum(double x) {
+  x += dpp_mov<0xB1>(x);    // quad_perm [1,0,3,2]
+  x += dpp_mov<0x4E>(x);    // quad_perm [2,3,0,1]
+  x += dpp_mov<0x141>(x);   // row_half_mirror
+  return x;
+}
+
+// 1/x and 1/sqrt(x) for x > 0: hardware estimate refined by Newton steps to
+// full double precision (each step squares the relative error).
+__device__ inline double fast_rcp(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return y;
+}
+__device__ inline double fast_rsqrt(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * fma(-0.5 * x * y, y, 1.5);
+  y = y * fma(-0.5 * x * y, y, 1.5);
+  return y;
+}
+
+// rr_pair without integer division: p = (r + k) mod L, q = (r - k) mod L for
+// 0 <= r, k < L (k > 0), L = np - 1.
+__device__ inline void rr_pair_fast(int r, int k, int np, int& p, int& q) {
+  const int L = np - 1;
+  if (k == 0) {
+    p = r;
+    q = L;
+  } else {
+    p = r + k;
+    p -= (p >= L) ? L : 0;
+    q = r - k;
+    q += (q < 0) ? L : 0;
+  }
+  if (p > q) {
+    const int t = p;
+    p = q;
+    q = t;
+  }
+}
+
+template <int EMX>
+__global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __restrict__ probs,
+                                                             const double* __restrict__ A,
+                                                             const double* __restrict__ V0T,
+                                                             double* __restrict__ U,
+                                                             double* __restrict__ S,
+                                                             double* __restrict__ VT,
+                                                             int* __restrict__ status) {
+  constexpr int CHW = EMX + 2;                   // W chunk stride (doubles)
+  constexpr int CW = kSmallTP * CHW + 2;         // W column stride
+  constexpr int CHV = kSmallEN + 2;
+  constexpr int CV = kSmallTP * CHV + 2;
+  __shared__ __attribute__((aligned(16))) double W[kSmallN * CW];
+  __shared__ __attribute__((aligned(16))) double V[kSmallN * CV];
+  __shared__ double nrm[kSmallN];
+  __shared__ int rank_of[kSmallN];
+  __shared__ int col_at[kSmallN];
+  const SvdProb pr = probs[blockIdx.x];
+  const int64_t M = pr.m, N = pr.n;
+  const bool tr = (M < N);
+  const int m = (int)(tr ? N : M);
+  const int n = (int)(tr ? M : N);
+  if (m == 0 || n == 0) return;
+  const int tid = threadIdx.x;
+  const int g = tid / kSmallTP, t = tid % kSmallTP;
+  const double* Ab = A + pr.a_off;
+  auto widx = [](int c, int i) { return c * CW + (i / EMX) * CHW + (i % EMX); };
+  auto vidx = [](int c, int i) { return c * CV + (i / kSmallEN) * CHV + (i % kSmallEN); };
+
+  // V = I (or V0 for a warm start, m >= n only); W = A or A^T (or A V0),
+  // rows m..8*EMX zero
+  const bool warm = pr.v0_off >= 0 && !tr;
+  for (int e = tid; e < kSmallN * kSmallTP * kSmallEN; e += kThreads) {
+    const int c = e / (kSmallTP * kSmallEN), i = e % (kSmallTP * kSmallEN);
+    double v = (c == i && c < n) ? 1.0 : 0.0;
+    if (warm) v = (c < n && i < n) ? V0T[pr.v0_off + (int64_t)c * n + i] : 0.0;
+    V[vidx(c, i)] = v;
+  }
+  if (warm) __syncthreads();
+  for (int e = tid; e < kSmallN * kSmallTP * EMX; e += kThreads) {
+    const int c = e / (kSmallTP * EMX), i = e % (kSmallTP * EMX);
+    double x = 0.0;
+    if (c < n && i < m) {
+      if (warm) {
+        for (int k = 0; k < n; ++k) x += Ab[(int64_t)i * N + k] * V[vidx(c, k)];
+      } else {
+        x = tr ? Ab[(int64_t)c * N + i] : Ab[(int64_t)i * N + c];
+      }
+    }
+    W[widx(c, i)] = x;
+  }
+  __syncthreads();
+
+  // ||A||_F^2 from the exact column norms (group sums, then a fixed-order
+  // sum over the columns by one thread)
+  auto exact_norms = [&]() {
+    for (int c = g; c < n; c += kThreads / kSmallTP) {
+      const double* w = W + c * CW + t * CHW;
+      double s = 0.0;
+#pragma unroll
+      for (int e = 0; e < EMX; ++e) s += w[e] * w[e];
+      s = group8_sum(s);
+      if (t == 0) nrm[c] = s;
+    }
+    __syncthreads();
+  };
+  exact_norms();
+  double normF2 = 0.0;
+  for (int c = 0; c < n; ++c) normF2 += nrm[c];
+
+  const int np = (n % 2) ? n + 1 : n;
+  const int npairs = np / 2;
+  const double tol = 10.0 * DBL_EPSILON * (double)m;
+  const double abs_tol = (double)m * DBL_EPSILON * DBL_EPSILON * normF2;
+
+  // Each sweep starts from the columns sorted by decreasing norm (de Rijk):
+  // position k of the round-robin schedule holds column col_at[k]; sorting
+  // moves no data and cuts the sweeps needed for graded spectra.
+  auto sort_positions = [&]() {
+    for (int j = tid; j < np; j += kThreads) {
+      if (j >= n) {
+        col_at[j] = j;   // the padding position of an odd n
+        continue;
+      }
+      const double sj = nrm[j];
+      int rk = 0;
+      for (int i = 0; i < n; ++i) {
+        const double si = nrm[i];
+        rk += (si > sj) || (si == sj && i < j);
+      }
+      col_at[rk] = j;
+    }
+    __syncthreads();
+  };
+  int sweep = 0;
+  const int max_sweeps = 60;
+  for (; sweep < max_sweeps; ++sweep) {
+    if (sweep > 0) exact_norms();
+    sort_positions();
+    int rotated = 0;
+    for (int r = 0; r < np - 1; ++r) {
+      int p = 0, q = 0;
+      bool live = g < npairs;
+      if (live) {
+        int pp, qq;
+        rr_pair_fast(r, g, np, pp, qq);
+        p = col_at[pp];
+        q = col_at[qq];
+        live = (p < n && q < n);
+      }
+      if (live) {
+        double* wp = W + p * CW + t * CHW;
+        double* wq = W + q * CW + t * CHW;
+        double* vp = V + p * CV + t * CHV;
+        double* vq = V + q * CV + t * CHV;
+        // the rotation columns are read with the working columns, so their
+        // LDS latency overlaps the dot product and its reduction
+        double yp[kSmallEN], yq[kSmallEN];
+#pragma unroll
+        for (int e = 0; e < kSmallEN; e += 2) {
+          const double2 u = *reinterpret_cast<const double2*>(vp + e);
+          const double2 v = *reinterpret_cast<const double2*>(vq + e);
+          yp[e] = u.x;
+          yp[e + 1] = u.y;
+          yq[e] = v.x;
+          yq[e + 1] = v.y;
+        }
+        double xp[EMX], xq[EMX];
+#pragma unroll
+        for (int e = 0; e < EMX; e += 2) {
+          const double2 u = *reinterpret_cast<const double2*>(wp + e);
+          const double2 v = *reinterpret_cast<const double2*>(wq + e);
+          xp[e] = u.x;
+          xp[e + 1] = u.y;
+          xq[e] = v.x;
+          xq[e + 1] = v.y;
+        }
+        double c = 0.0;
+#pragma unroll
+        for (int e = 0; e < EMX; ++e) c += xp[e] * xq[e];
+        c = group8_sum(c);
+        const double a = nrm[p], b = nrm[q];
+        if (c * c > tol * tol * (a * b) && fabs(c) > abs_tol) {
+          rotated = 1;
+          const double zeta = (b - a) * (0.5 * fast_rcp(c));
+          const double z2 = fma(zeta, zeta, 1.0);
+          const double tt = copysign(1.0, zeta) * fast_rcp(fabs(zeta) + z2 * fast_rsqrt(z2));
+          const double cs = fast_rsqrt(fma(tt, tt, 1.0));
+          const double sn = cs * tt;
+#pragma unroll
+          for (int e = 0; e < EMX; e += 2) {
+            *reinterpret_cast<double2*>(wp + e) =
+                double2{cs * xp[e] - sn * xq[e], cs * xp[e + 1] - sn * xq[e + 1]};
+            *reinterpret_cast<double2*>(wq + e) =
+                double2{sn * xp[e] + cs * xq[e], sn * xp[e + 1] + cs * xq[e + 1]};
+          }
+#pragma unroll
+          for (int e = 0; e < kSmallEN; e += 2) {
+            *reinterpret_cast<double2*>(vp + e) =
+                double2{cs * yp[e] - sn * yq[e], cs * yp[e + 1] - sn * yq[e + 1]};
+            *reinterpret_cast<double2*>(vq + e) =
+                double2{sn * yp[e] + cs * yq[e], sn * yp[e + 1] + cs * yq[e + 1]};
+          }
+          if (t == 0) {
+            nrm[p] = fmax(a - tt * c, 0.0);
+            nrm[q] = fmax(b + tt * c, 0.0);
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (!__syncthreads_or(rotated)) break;
+  }
+  if (tid == 0) {
+    if (sweep >= max_sweeps) atomicAdd(status, 1);
+    atomicMax(status + 1, sweep + 1);
+  }
+
+  // singular values = exact column norms; descending rank (ties by index)
+  exact_norms();
+  for (int j = tid; j < n; j += kThreads) {
+    const double sj = nrm[j];
+    int rk = 0;
+    for (int i = 0; i < n; ++i) {
+      const double si = nrm[i];
+      rk += (si > sj) || (si == sj && i < j);
+    }
+    rank_of[j] = rk;
+  }
+  __syncthreads();
+  double* Ub = U + pr.u_off;
+  double* Sb = S + pr.s_off;
+  double* VTb = VT + pr.vt_off;
+  for (int j = tid; j < n; j += kThreads) Sb[rank_of[j]] = sqrt(nrm[j]);
+  for (int e = tid; e < m * n; e += kThreads) {
+    const int j = e / m, i = e % m;
+    const double sj = sqrt(nrm[j]);
+    const double v = (sj > 0.0) ? W[widx(j, i)] / sj : 0.0;
+    const int rk = rank_of[j];
+    if (!tr)
+      Ub[(int64_t)i * n + rk] = v;
+    else
+      VTb[(int64_t)rk * N + i] = v;
+  }
+  for (int e = tid; e < n * n; e += kThreads) {
+    const int j = e / n, i = e % n;
+    const int rk = rank_of[j];
+    const double vv = V[vidx(j, i)];
+    if (!tr)
+      VTb[(int64_t)rk * n + i] = vv;
+    else
+      Ub[(int64_t)i * n + rk] = vv;
+  }
+}
+
+// A batch of SVD problems with its device-side description, scratch and
+// status uploaded once: executing it only launches kernels (no allocation,
+// no host round trip), so it can be replayed and captured into a graph.
+}  // namespace
+
+struct tn_svd_plan_s {
+  std::vector<SvdProb> probs;   // small-kernel problems, then LDS, then scratch
+  size_t n_small = 0, n_lds = 0, n_glb = 0;
+  int64_t small_mm = 0;
   size_t lds_need = 0;
+  int tp_lds = 8, tp_glb = 8;
+  SvdProb* d_p = nullptr;
+  double* d_scratch = nullptr;
+  int* d_status = nullptr;      // [0] problems not converged, [1] max sweeps
+};
+
+namespace {
+
+void svd_plan_free(tn_svd_plan_s* P) {
+  tn::pool_free(P->d_status);
+  tn::pool_free(P->d_p);
+  tn::pool_free(P->d_scratch);
+  delete P;
+}
+
+int svd_plan_build(tn_svd_plan_s** out, int64_t batch, const int64_t* m, const int64_t* n,
+                   const int64_t* a_off, const int64_t* v0_off, const int64_t* u_off,
+                   const int64_t* s_off, const int64_t* vt_off) {
+  std::vector<SvdProb> small_probs, lds_probs, glb_probs;
+  auto* P = new tn_svd_plan_s();
   int64_t scratch_total = 0;
   for (int64_t i = 0; i < batch; ++i) {
-    TN_REQUIRE(m[i] >= 0 && n[i] >= 0, TN_EINVAL, "svd: negative shape");
-    int64_t mm = std::max(m[i], n[i]), nn = std::min(m[i], n[i]);
-    TN_REQUIRE(nn <= 512, TN_EUNSUPP, "svd: min(M,N)=%lld > 512 is not supported yet",
-               (long long)nn);
+    if (m[i] < 0 || n[i] < 0) {
+      delete P;
+      tn::set_error("svd: negative shape");
+      return TN_EINVAL;
+    }
+    const int64_t mm = std::max(m[i], n[i]), nn = std::min(m[i], n[i]);
+    if (nn > 512) {
+      delete P;
+      tn::set_error("svd: min(M,N)=%lld > 512 is not supported yet", (long long)nn);
+      return TN_EUNSUPP;
+    }
     SvdProb p{m[i], n[i], a_off ? a_off[i] : 0, u_off ? u_off[i] : 0, s_off ? s_off[i] : 0,
-              vt_off ? vt_off[i] : 0, -1};
-    size_t bytes = sizeof(double) * (size_t)((mm + 1) * nn + (nn + 1) * nn);
-    if (bytes <= kLdsBytes) {
-      lds_need = std::max(lds_need, bytes);
+              vt_off ? vt_off[i] : 0, -1, v0_off ? v0_off[i] : -1};
+    const size_t bytes = sizeof(double) * (size_t)((mm + 1) * nn + (nn + 1) * nn);
+    if (nn <= kSmallN && mm <= kSmallTP * 16) {
+      small_probs.push_back(p);
+      P->small_mm = std::max(P->small_mm, mm);
+    } else if (bytes <= kLdsBytes) {
+      P->lds_need = std::max(P->lds_need, bytes);
       lds_probs.push_back(p);
     } else {
       p.scratch_off = scratch_total;
@@ -454,62 +780,103 @@ int svd_batched(int64_t batch, const int64_t* m, const int64_t* n, const double*
       glb_probs.push_back(p);
     }
   }
-  int* d_status = nullptr;
-  SvdProb* d_p = nullptr;
-  double* d_scratch = nullptr;
-  int rc = tn::pool_alloc((void**)&d_status, 2 * sizeof(int));
-  if (rc == TN_OK) rc = tn_zero(d_status, 2 * sizeof(int), st);
-  const size_t np = lds_probs.size() + glb_probs.size();
-  if (rc == TN_OK) rc = tn::pool_alloc((void**)&d_p, sizeof(SvdProb) * np);
-  std::vector<SvdProb> all(lds_probs);
-  all.insert(all.end(), glb_probs.begin(), glb_probs.end());
-  if (rc == TN_OK) rc = tn_memcpy_h2d(d_p, all.data(), sizeof(SvdProb) * np, st);
-  if (rc == TN_OK && scratch_total > 0)
-    rc = tn::pool_alloc((void**)&d_scratch, sizeof(double) * scratch_total);
   // lanes per column pair: as few as the widest problem allows with one
   // pass over the pairs of a round (8 lanes up to 64 columns)
   auto pick_tp = [](const std::vector<SvdProb>& v) {
     int64_t mx = 0;
     for (auto& p : v) mx = std::max(mx, std::min(p.m, p.n));
-    int64_t pairs = (mx + 1) / 2;
+    const int64_t pairs = (mx + 1) / 2;
     return pairs <= 32 ? 8 : (pairs <= 64 ? 4 : 2);
   };
-  auto launch = [&](bool in_lds, int tp, const SvdProb* probs, size_t count, size_t lds_bytes) {
-    typedef void (*K)(const SvdProb*, const double*, double*, double*, double*, double*, int*);
-    K k = nullptr;
-    if (in_lds)
-      k = (tp == 8) ? svd_jacobi_kernel<true, 8> : (tp == 4) ? svd_jacobi_kernel<true, 4> : svd_jacobi_kernel<true, 2>;
-    else
-      k = (tp == 8) ? svd_jacobi_kernel<false, 8> : (tp == 4) ? svd_jacobi_kernel<false, 4> : svd_jacobi_kernel<false, 2>;
-    if (in_lds) {
-      (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes);
-      (void)hipGetLastError();
-    }
-    hipLaunchKernelGGL(k, dim3((unsigned)count), dim3(kThreads), lds_bytes, st, probs, A, U, S, VT, d_scratch,
-                       d_status);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-      tn::set_error("svd kernel launch: %s", hipGetErrorString(e));
-      return (int)TN_EHIP;
-    }
-    return (int)TN_OK;
-  };
-  if (rc == TN_OK && !lds_probs.empty()) rc = launch(true, pick_tp(lds_probs), d_p, lds_probs.size(), lds_need);
-  if (rc == TN_OK && !glb_probs.empty())
-    rc = launch(false, pick_tp(glb_probs), d_p + lds_probs.size(), glb_probs.size(), 0);
-  int h_status[2] = {0, 0};
-  if (rc == TN_OK) rc = tn_memcpy_d2h(h_status, d_status, 2 * sizeof(int), st);  // synchronises
-  tn::pool_free(d_status);
-  tn::pool_free(d_p);
-  tn::pool_free(d_scratch);
+  P->tp_lds = pick_tp(lds_probs);
+  P->tp_glb = pick_tp(glb_probs);
+  P->n_small = small_probs.size();
+  P->n_lds = lds_probs.size();
+  P->n_glb = glb_probs.size();
+  P->probs = small_probs;
+  P->probs.insert(P->probs.end(), lds_probs.begin(), lds_probs.end());
+  P->probs.insert(P->probs.end(), glb_probs.begin(), glb_probs.end());
+  int rc = tn::pool_alloc((void**)&P->d_status, 2 * sizeof(int));
+  if (rc == TN_OK && !P->probs.empty()) rc = tn::pool_alloc((void**)&P->d_p, sizeof(SvdProb) * P->probs.size());
+  if (rc == TN_OK && !P->probs.empty())
+    rc = tn_memcpy_h2d(P->d_p, P->probs.data(), sizeof(SvdProb) * P->probs.size(), nullptr);
+  if (rc == TN_OK && scratch_total > 0) rc = tn::pool_alloc((void**)&P->d_scratch, sizeof(double) * scratch_total);
+  if (rc != TN_OK) {
+    svd_plan_free(P);
+    return rc;
+  }
+  *out = P;
+  return TN_OK;
+}
+
+typedef void (*SvdKern)(const SvdProb*, const double*, const double*, double*, double*, double*, double*,
+                        int*);
+typedef void (*SvdSmallKern)(const SvdProb*, const double*, const double*, double*, double*, double*, int*);
+
+SvdKern pick_svd_kernel(bool in_lds, int tp) {
+  if (in_lds)
+    return (tp == 8) ? svd_jacobi_kernel<true, 8> : (tp == 4) ? svd_jacobi_kernel<true, 4> : svd_jacobi_kernel<true, 2>;
+  return (tp == 8) ? svd_jacobi_kernel<false, 8> : (tp == 4) ? svd_jacobi_kernel<false, 4> : svd_jacobi_kernel<false, 2>;
+}
+
+int svd_plan_launch(const tn_svd_plan_s* P, const double* A, const double* V0T, double* U, double* S,
+                    double* VT, hipStream_t st) {
+  int rc = tn_zero(P->d_status, 2 * sizeof(int), st);
   if (rc != TN_OK) return rc;
-  g_last_sweeps = h_status[1];
-  TN_REQUIRE(h_status[0] == 0, TN_ENOCONV, "svd: %d problem(s) did not converge in 60 sweeps",
-             h_status[0]);
+  if (P->n_small > 0) {
+    SvdSmallKern k = (P->small_mm <= 32) ? svd_jacobi_small<4>
+                     : (P->small_mm <= 64) ? svd_jacobi_small<8> : svd_jacobi_small<16>;
+    hipLaunchKernelGGL(k, dim3((unsigned)P->n_small), dim3(kThreads), 0, st, P->d_p, A, V0T, U, S, VT, P->d_status);
+    TN_LAUNCH_CHECK();
+  }
+  if (P->n_lds > 0) {
+    SvdKern k = pick_svd_kernel(true, P->tp_lds);
+    // raise the dynamic LDS limit to what this batch needs (a failure here
+    // surfaces as a launch error below)
+    (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds_need);
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(k, dim3((unsigned)P->n_lds), dim3(kThreads), P->lds_need, st, P->d_p + P->n_small, A, V0T, U,
+                       S, VT, P->d_scratch, P->d_status);
+    TN_LAUNCH_CHECK();
+  }
+  if (P->n_glb > 0) {
+    SvdKern k = pick_svd_kernel(false, P->tp_glb);
+    hipLaunchKernelGGL(k, dim3((unsigned)P->n_glb), dim3(kThreads), 0, st, P->d_p + P->n_small + P->n_lds, A, V0T,
+                       U, S, VT, P->d_scratch, P->d_status);
+    TN_LAUNCH_CHECK();
+  }
+  return TN_OK;
+}
+
+int svd_plan_status(const tn_svd_plan_s* P, int* nonconv, int* sweeps, hipStream_t st) {
+  int h[2] = {0, 0};
+  const int rc = tn_memcpy_d2h(h, P->d_status, 2 * sizeof(int), st);   // synchronises
+  if (rc != TN_OK) return rc;
+  g_last_sweeps = h[1];
+  if (nonconv) *nonconv = h[0];
+  if (sweeps) *sweeps = h[1];
+  return TN_OK;
+}
+
+// One-shot batch: plan, launch, wait, check.
+int svd_batched(int64_t batch, const int64_t* m, const int64_t* n, const double* A, const int64_t* a_off,
+                const double* V0T, const int64_t* v0_off, double* U, const int64_t* u_off, double* S,
+                const int64_t* s_off, double* VT, const int64_t* vt_off, hipStream_t st) {
+  if (batch == 0) return TN_OK;
+  tn_svd_plan_s* P = nullptr;
+  int rc = svd_plan_build(&P, batch, m, n, a_off, v0_off, u_off, s_off, vt_off);
+  if (rc != TN_OK) return rc;
+  rc = svd_plan_launch(P, A, V0T, U, S, VT, st);
+  int nonconv = 0;
+  if (rc == TN_OK) rc = svd_plan_status(P, &nonconv, nullptr, st);
+  svd_plan_free(P);
+  if (rc != TN_OK) return rc;
+  TN_REQUIRE(nonconv == 0, TN_ENOCONV, "svd: %d problem(s) did not converge in 60 sweeps", nonconv);
   return TN_OK;
 }
 
 }  // namespace
+
 
 extern "C" {
 
@@ -524,7 +891,16 @@ int tn_svd_f64(const double* A, int64_t M, int64_t N, double* U, double* S, doub
   TN_INIT_OR_RETURN();
   TN_REQUIRE(A && U && S && VT, TN_EINVAL, "tn_svd_f64: NULL buffer");
   int64_t m = M, n = N, z = 0;
-  return svd_batched(1, &m, &n, A, &z, U, &z, S, &z, VT, &z, tn::stream_of(stream));
+  return svd_batched(1, &m, &n, A, &z, nullptr, nullptr, U, &z, S, &z, VT, &z, tn::stream_of(stream));
+}
+
+int tn_svd_f64_warm(const double* A, int64_t M, int64_t N, const double* V0T, double* U, double* S,
+                    double* VT, void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(A && U && S && VT && V0T, TN_EINVAL, "tn_svd_f64_warm: NULL buffer");
+  TN_REQUIRE(M >= N, TN_EINVAL, "tn_svd_f64_warm: needs M >= N (got %lld x %lld)", (long long)M, (long long)N);
+  int64_t m = M, n = N, z = 0;
+  return svd_batched(1, &m, &n, A, &z, V0T, &z, U, &z, S, &z, VT, &z, tn::stream_of(stream));
 }
 
 int tn_svd_f64_batched(int64_t batch, const int64_t* m, const int64_t* n, const double* A,
@@ -533,8 +909,46 @@ int tn_svd_f64_batched(int64_t batch, const int64_t* m, const int64_t* n, const 
   TN_INIT_OR_RETURN();
   TN_REQUIRE(batch >= 0 && (batch == 0 || (m && n && A && U && S && VT)), TN_EINVAL,
              "tn_svd_f64_batched: NULL argument");
-  return svd_batched(batch, m, n, A, a_off, U, u_off, S, s_off, VT, vt_off,
+  return svd_batched(batch, m, n, A, a_off, nullptr, nullptr, U, u_off, S, s_off, VT, vt_off,
                      tn::stream_of(stream));
+}
+
+int tn_svd_plan_create(tn_svd_plan* plan, int64_t batch, const int64_t* m, const int64_t* n,
+                       const int64_t* a_off, const int64_t* v0_off, const int64_t* u_off,
+                       const int64_t* s_off, const int64_t* vt_off) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(plan != nullptr, TN_EINVAL, "tn_svd_plan_create: NULL out pointer");
+  TN_REQUIRE(batch >= 0 && (batch == 0 || (m && n)), TN_EINVAL, "tn_svd_plan_create: NULL shapes");
+  if (v0_off)
+    for (int64_t i = 0; i < batch; ++i)
+      TN_REQUIRE(v0_off[i] < 0 || m[i] >= n[i], TN_EINVAL, "tn_svd_plan_create: warm start needs M >= N");
+  return svd_plan_build(plan, batch, m, n, a_off, v0_off, u_off, s_off, vt_off);
+}
+
+int tn_svd_plan_exec(tn_svd_plan P, const double* A, const double* V0T, double* U, double* S, double* VT,
+                     void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(P != nullptr, TN_EINVAL, "tn_svd_plan_exec: NULL plan");
+  if (P->probs.empty()) return TN_OK;
+  TN_REQUIRE(A && U && S && VT, TN_EINVAL, "tn_svd_plan_exec: NULL buffer");
+  bool needs_v0 = false;
+  for (const auto& p : P->probs) needs_v0 |= p.v0_off >= 0;
+  TN_REQUIRE(!needs_v0 || V0T, TN_EINVAL, "tn_svd_plan_exec: the plan has warm starts but V0T is NULL");
+  return svd_plan_launch(P, A, V0T, U, S, VT, tn::stream_of(stream));
+}
+
+int tn_svd_plan_check(tn_svd_plan P, int* nonconverged, int* sweeps, void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(P != nullptr, TN_EINVAL, "tn_svd_plan_check: NULL plan");
+  return svd_plan_status(P, nonconverged, sweeps, tn::stream_of(stream));
+}
+
+int tn_svd_plan_destroy(tn_svd_plan P) {
+  if (!P) return TN_OK;
+  const hipError_t e = hipDeviceSynchronize();   // launches may still read the plan
+  svd_plan_free(P);
+  TN_REQUIRE(e == hipSuccess, TN_EHIP, "hipDeviceSynchronize: %s", hipGetErrorString(e));
+  return TN_OK;
 }
 
 int tn_eigh_f64(const double* A, int64_t N, double* eig, double* vec_rows, void* stream) {

@@ -70,6 +70,11 @@ SIGNATURES = {
     "tn_svd_f64": [P, I64, I64, P, P, P, P],
     "tn_svd_f64_batched": [I64, P, P, P, P, P, P, P, P, P, P, P],
     "tn_svd_last_sweeps": [ctypes.POINTER(ctypes.c_int)],
+    "tn_svd_f64_warm": [P, I64, I64, P, P, P, P, P],
+    "tn_svd_plan_create": [ctypes.POINTER(ctypes.c_void_p), I64, P, P, P, P, P, P, P],
+    "tn_svd_plan_exec": [P, P, P, P, P, P, P],
+    "tn_svd_plan_check": [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), P],
+    "tn_svd_plan_destroy": [P],
     "tn_eigh_f64": [P, I64, P, P, P],
     "tn_truncate_f64": [P, I64, I64, P, P, P, P],
 }
