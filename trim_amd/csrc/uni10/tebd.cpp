@@ -71,6 +71,30 @@ ITebd::ITebd(int chi, int d, double delta, double tau, uint64_t seed, const std:
   Lb_ = Matrix(chi, chi, lb.data(), true, false);
 }
 
+// SVD of theta started from the right singular vectors of the previous
+// theta on the same bond (the two updates of a step alternate between two
+// bonds): successive thetas differ by one small imaginary-time step, so the
+// Jacobi sweeps start close to convergence.  Same factorisation as
+// Matrix::svd (U R x k, S k x k diagonal, VT k x C).
+static std::vector<Matrix> svd_warm(const Matrix& T, Matrix& vt_prev) {
+  const size_t R = T.row(), C = T.col(), k = std::min(R, C);
+  std::vector<Matrix> outs;
+  outs.emplace_back(R, k, false);
+  outs.emplace_back(k, k, true);
+  outs.emplace_back(k, C, false);
+  if (k == 0) return outs;
+  if (R >= C && vt_prev.row() == C && vt_prev.col() == C)
+    detail::check(tn_svd_f64_warm(T.getElem(), (int64_t)R, (int64_t)C, vt_prev.getElem(), outs[0].getElem(),
+                                  outs[1].getElem(), outs[2].getElem(), detail::stream()),
+                  "ITebd svd");
+  else
+    detail::check(tn_svd_f64(T.getElem(), (int64_t)R, (int64_t)C, outs[0].getElem(), outs[1].getElem(),
+                             outs[2].getElem(), detail::stream()),
+                  "ITebd svd");
+  vt_prev = outs[2];
+  return outs;
+}
+
 void ITebd::update(UniTensor& ALa, UniTensor& BLb, Matrix& La, Matrix& Lb) {
   Qnum q0(0);
   iTEBD_.putTensor("ALa", &ALa);
@@ -80,7 +104,8 @@ void ITebd::update(UniTensor& ALa, UniTensor& BLb, Matrix& La, Matrix& Lb) {
   UniTensor Theta(C.bond(), "Theta");
   Theta.putBlock(q0, Lb * C.getBlock(q0));
   Theta.permute(2);
-  std::vector<Matrix> rets = Theta.getBlock(q0).svd();
+  std::vector<Matrix> rets = svd_warm(Theta.getBlock(q0), vt_prev_[parity_]);
+  parity_ ^= 1;
   int dim = chi_ < (int)rets[1].row() ? chi_ : (int)rets[1].row();
   // discarded weight (not computed by the reference; reported alongside)
   {

@@ -26,6 +26,8 @@ class ITebd {
   int chi_, d_;
   double tau_;
   double trunc_err_ = 0;
+  Matrix vt_prev_[2];          // last VT of each bond's theta (warm starts)
+  int parity_ = 0;
   UniTensor H_, U_, ALa_, BLb_;
   Matrix La_, Lb_;
   Network iTEBD_, updateA_, MPS_, meas_;
