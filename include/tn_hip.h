@@ -234,6 +234,33 @@ int tn_eigh_f64(const double* A, int64_t N, double* eig, double* eigvec_rows,
 int tn_truncate_f64(const double* S, int64_t k, int64_t chi, double* lambda_out,
                     double* norm_out, double* trunc_err_out, void* stream);
 
+/* ---- batched vector operations ---------------------------------------------
+ * Many independent short vector operations (one item per chain and step of
+ * a batched TEBD update) in one launch; all offsets are in doubles from one
+ * arena base pointer.  Items of one plan must not read what another item of
+ * the same plan writes.  Executing a plan neither allocates nor
+ * synchronises (graph-capturable).  Batched forms of diagRowMul
+ * (uni10_lapack_cpu.cpp:46-57), the truncation of iTEBD.cpp:157-161, the VT
+ * row copy of iTEBD.cpp:162-166 and vectorScal (uni10_lapack_cpu.cpp:75-89).
+ *   ROWSCALE   dst[i] = src[i] * base[aux + i / rep]          (i < n)
+ *   COPY       dst[i] = src[i]                                (i < n)
+ *   SCALE_INV  dst[i] = src[i] / base[aux]                    (i < n)
+ *   TRUNCATE   src = S (n values, descending), keep rep of them:
+ *              dst[i] = S[i] / ||S[:rep]|| (i < rep),
+ *              base[aux] = ||S[:rep]||, base[aux+1] = sum_{i>=rep} S_i^2 */
+#define TN_VECOP_ROWSCALE 0
+#define TN_VECOP_COPY 1
+#define TN_VECOP_SCALE_INV 2
+#define TN_VECOP_TRUNCATE 3
+typedef struct {
+  int32_t op, pad;
+  int64_t n, src, dst, aux, rep;
+} tn_vecop;
+typedef struct tn_vecops_plan_s* tn_vecops_plan;
+int tn_vecops_plan_create(tn_vecops_plan* plan, int64_t nitems, const tn_vecop* items);
+int tn_vecops_plan_exec(tn_vecops_plan plan, double* base, void* stream);
+int tn_vecops_plan_destroy(tn_vecops_plan plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,6 +22,7 @@ extern "C" {
 typedef struct u10_tensor_s* u10_tensor;
 typedef struct u10_network_s* u10_network;
 typedef struct u10_itebd_s* u10_itebd;
+typedef struct u10_batch_tebd_s* u10_batch_tebd;
 
 const char* u10_last_error(void);
 int u10_set_gemm_mode(int mode);          /* GemmMode of uni10.hpp (thread-local) */
@@ -79,6 +80,21 @@ int u10_itebd_energy(u10_itebd t, double* out);
 int u10_itebd_lambda(u10_itebd t, int which, double* out, int* n);
 int u10_itebd_trunc_err(u10_itebd t, double* out);
 int u10_itebd_free(u10_itebd t);
+
+/* Batched iTEBD: n independent XXZ chains (bond dimension chi[i] <= 256,
+ * anisotropy delta[i], seed[i]) advanced together, each update stage one
+ * launch for all chains; gemm_mode is a GemmMode of uni10.hpp (SVD and
+ * truncation stay FP64).  set_graph(1) replays steps as a captured graph.
+ * Per chain, the update of the example driver (iTEBD.cpp:148-173). */
+int u10_batch_tebd_create(int n, const int* chi, const double* delta, const uint64_t* seed, double tau,
+                          int gemm_mode, const char* netdir, u10_batch_tebd* out);
+int u10_batch_tebd_step(u10_batch_tebd b, int nsteps);
+int u10_batch_tebd_set_graph(u10_batch_tebd b, int on);
+int u10_batch_tebd_energy(u10_batch_tebd b, int chain, double* out);
+int u10_batch_tebd_lambda(u10_batch_tebd b, int chain, int which, double* out, int* n);
+int u10_batch_tebd_trunc_err(u10_batch_tebd b, int chain, double* out);
+int u10_batch_tebd_info(u10_batch_tebd b, double* gemm_flops_per_step, int* last_sweeps);
+int u10_batch_tebd_free(u10_batch_tebd b);
 
 #ifdef __cplusplus
 }

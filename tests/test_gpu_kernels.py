@@ -517,3 +517,36 @@ def test_vector_ops(tn):
     assert abs(nrm.to_host()[0] - ref_n) <= 1e-14 * ref_n
     assert np.allclose(lam.to_host(), s[:32] / ref_n, rtol=1e-14, atol=0)
     assert abs(err.to_host()[0] - np.sum(s[32:] ** 2)) <= 1e-14
+
+
+# ---------------------------------------------------------------------------
+# batched vector operations
+def test_vecops_plan(tn):
+    rng = np.random.default_rng(9)
+    base = rng.uniform(-1, 1, 20000)
+    base[100:140] = np.sort(np.abs(base[100:140]))[::-1]     # a descending S
+    VEC = np.dtype([("op", "<i4"), ("pad", "<i4"), ("n", "<i8"), ("src", "<i8"), ("dst", "<i8"),
+                    ("aux", "<i8"), ("rep", "<i8")])
+    items = np.array([(0, 0, 9000, 0, 10000, 200, 300),     # rows of 300 scaled by base[200 + r]
+                      (1, 0, 500, 9000, 19000, 0, 0),       # copy
+                      (2, 0, 40, 300, 19600, 150, 0),       # / base[150]
+                      (3, 0, 40, 100, 19700, 19800, 25)],   # truncate S to 25
+                     dtype=VEC)
+    d = tn.DeviceArray.from_host(base)
+    h = ctypes.c_void_p()
+    tn.call("tn_vecops_plan_create", ctypes.byref(h), len(items), items.ctypes.data_as(ctypes.c_void_p))
+    try:
+        tn.call("tn_vecops_plan_exec", ctypes.c_void_p(h.value), tn._p(d), None)
+        out = d.to_host()
+    finally:
+        tn.call("tn_vecops_plan_destroy", ctypes.c_void_p(h.value))
+    ref = base.copy()
+    ref[10000:19000] = base[0:9000] * np.repeat(base[200:230], 300)
+    ref[19000:19500] = base[9000:9500]
+    ref[19600:19640] = base[300:340] / base[150]
+    S = base[100:140]
+    nrm = np.sqrt(np.sum(S[:25] ** 2))
+    ref[19700:19725] = S[:25] / nrm
+    ref[19800] = nrm
+    ref[19801] = np.sum(S[25:] ** 2)
+    assert np.max(np.abs(out - ref)) <= 1e-14

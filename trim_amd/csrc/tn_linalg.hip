@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "tn_internal.h"
@@ -747,6 +748,18 @@ void svd_plan_free(tn_svd_plan_s* P) {
   delete P;
 }
 
+typedef void (*SvdKern)(const SvdProb*, const double*, const double*, double*, double*, double*, double*,
+                        int*);
+typedef void (*SvdSmallKern)(const SvdProb*, const double*, const double*, double*, double*, double*, int*);
+
+SvdKern pick_svd_kernel(bool in_lds, int tp) {
+  if (in_lds)
+    return (tp == 8) ? svd_jacobi_kernel<true, 8> : (tp == 4) ? svd_jacobi_kernel<true, 4> : svd_jacobi_kernel<true, 2>;
+  return (tp == 8) ? svd_jacobi_kernel<false, 8> : (tp == 4) ? svd_jacobi_kernel<false, 4> : svd_jacobi_kernel<false, 2>;
+}
+
+void raise_lds_limit(int tp, size_t bytes);
+
 int svd_plan_build(tn_svd_plan_s** out, int64_t batch, const int64_t* m, const int64_t* n,
                    const int64_t* a_off, const int64_t* v0_off, const int64_t* u_off,
                    const int64_t* s_off, const int64_t* vt_off) {
@@ -790,6 +803,7 @@ int svd_plan_build(tn_svd_plan_s** out, int64_t batch, const int64_t* m, const i
   };
   P->tp_lds = pick_tp(lds_probs);
   P->tp_glb = pick_tp(glb_probs);
+  if (!lds_probs.empty()) raise_lds_limit(P->tp_lds, P->lds_need);
   P->n_small = small_probs.size();
   P->n_lds = lds_probs.size();
   P->n_glb = glb_probs.size();
@@ -809,14 +823,19 @@ int svd_plan_build(tn_svd_plan_s** out, int64_t batch, const int64_t* m, const i
   return TN_OK;
 }
 
-typedef void (*SvdKern)(const SvdProb*, const double*, const double*, double*, double*, double*, double*,
-                        int*);
-typedef void (*SvdSmallKern)(const SvdProb*, const double*, const double*, double*, double*, double*, int*);
-
-SvdKern pick_svd_kernel(bool in_lds, int tp) {
-  if (in_lds)
-    return (tp == 8) ? svd_jacobi_kernel<true, 8> : (tp == 4) ? svd_jacobi_kernel<true, 4> : svd_jacobi_kernel<true, 2>;
-  return (tp == 8) ? svd_jacobi_kernel<false, 8> : (tp == 4) ? svd_jacobi_kernel<false, 4> : svd_jacobi_kernel<false, 2>;
+// The dynamic LDS limit of each LDS-resident kernel only ever grows (set at
+// plan creation, so that launches, which may be captured into a graph, make
+// no attribute calls); a failure surfaces as a launch error.
+void raise_lds_limit(int tp, size_t bytes) {
+  static std::mutex mu;
+  static size_t limit[3] = {0, 0, 0};
+  const int slot = (tp == 8) ? 0 : (tp == 4) ? 1 : 2;
+  std::lock_guard<std::mutex> g(mu);
+  if (bytes <= limit[slot]) return;
+  (void)hipFuncSetAttribute((const void*)pick_svd_kernel(true, tp), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bytes);
+  (void)hipGetLastError();
+  limit[slot] = bytes;
 }
 
 int svd_plan_launch(const tn_svd_plan_s* P, const double* A, const double* V0T, double* U, double* S,
@@ -831,10 +850,6 @@ int svd_plan_launch(const tn_svd_plan_s* P, const double* A, const double* V0T, 
   }
   if (P->n_lds > 0) {
     SvdKern k = pick_svd_kernel(true, P->tp_lds);
-    // raise the dynamic LDS limit to what this batch needs (a failure here
-    // surfaces as a launch error below)
-    (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P->lds_need);
-    (void)hipGetLastError();
     hipLaunchKernelGGL(k, dim3((unsigned)P->n_lds), dim3(kThreads), P->lds_need, st, P->d_p + P->n_small, A, V0T, U,
                        S, VT, P->d_scratch, P->d_status);
     TN_LAUNCH_CHECK();

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <string>
 
+#include "batch_tebd.h"
 #include "common.h"
 #include "tebd.h"
 
@@ -20,6 +21,11 @@ struct u10_itebd_s {
   template <typename... A>
   explicit u10_itebd_s(A&&... a) : E(std::forward<A>(a)...) {}
   ITebd E;
+};
+struct u10_batch_tebd_s {
+  template <typename... A>
+  explicit u10_batch_tebd_s(A&&... a) : B(std::forward<A>(a)...) {}
+  BatchTebd B;
 };
 
 namespace {
@@ -321,6 +327,50 @@ int u10_itebd_trunc_err(u10_itebd t, double* out) {
 
 int u10_itebd_free(u10_itebd t) {
   return guard([&] { delete t; });
+}
+
+int u10_batch_tebd_create(int n, const int* chi, const double* delta, const uint64_t* seed, double tau,
+                          int gemm_mode, const char* netdir, u10_batch_tebd* out) {
+  return guard([&] {
+    if (n < 1 || !chi || !delta || !seed || !netdir || !out) detail::fail("u10_batch_tebd_create: bad argument");
+    *out = new u10_batch_tebd_s(std::vector<int>(chi, chi + n), std::vector<double>(delta, delta + n),
+                                std::vector<uint64_t>(seed, seed + n), tau, gemm_mode, std::string(netdir));
+  });
+}
+
+int u10_batch_tebd_step(u10_batch_tebd b, int n) {
+  return guard([&] { b->B.step(n); });
+}
+
+int u10_batch_tebd_set_graph(u10_batch_tebd b, int on) {
+  return guard([&] { b->B.set_graph(on != 0); });
+}
+
+int u10_batch_tebd_energy(u10_batch_tebd b, int chain, double* out) {
+  return guard([&] { *out = b->B.energy(chain); });
+}
+
+int u10_batch_tebd_lambda(u10_batch_tebd b, int chain, int which, double* out, int* n) {
+  return guard([&] {
+    auto h = b->B.lambda(chain, which);
+    *n = (int)h.size();
+    if (out) std::memcpy(out, h.data(), h.size() * 8);
+  });
+}
+
+int u10_batch_tebd_trunc_err(u10_batch_tebd b, int chain, double* out) {
+  return guard([&] { *out = b->B.trunc_err(chain); });
+}
+
+int u10_batch_tebd_info(u10_batch_tebd b, double* flops, int* sweeps) {
+  return guard([&] {
+    if (flops) *flops = b->B.gemm_flops_per_step();
+    if (sweeps) *sweeps = b->B.last_sweeps();
+  });
+}
+
+int u10_batch_tebd_free(u10_batch_tebd b) {
+  return guard([&] { delete b; });
 }
 
 }  // extern "C"

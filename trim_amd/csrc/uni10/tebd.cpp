@@ -135,32 +135,37 @@ void ITebd::step() {
   update(BLb_, ALa_, Lb_, La_);
 }
 
-double ITebd::expectation(UniTensor& L, UniTensor& R) {
+// <psi|H|psi> / <psi|psi> of the two-site state L R (iTEBD.cpp:205-216).
+static double expectation(UniTensor& L, UniTensor& R, UniTensor& H, Network& MPS, Network& meas) {
   Qnum q0(0);
-  MPS_.putTensor("L", &L);
-  MPS_.putTensor("R", &R);
-  UniTensor psi = MPS_.launch();
+  MPS.putTensor("L", &L);
+  MPS.putTensor("R", &R);
+  UniTensor psi = MPS.launch();
   double norm = psi.getBlock(q0).norm();
   norm *= norm;
-  meas_.putTensor("bra", &psi);
-  meas_.putTensorT("ket", &psi);
-  meas_.putTensor("Op", &H_);
-  return meas_.launch()[0] / norm;
+  meas.putTensor("bra", &psi);
+  meas.putTensorT("ket", &psi);
+  meas.putTensor("Op", &H);
+  return meas.launch()[0] / norm;
 }
 
-double ITebd::energy() {
+double itebd_energy(UniTensor& ALa, UniTensor& BLb, const Matrix& La, const Matrix& Lb, UniTensor& H,
+                    Network& MPS, Network& meas) {
+  // iTEBD.cpp:189-203: mean of the two bond energies
   Qnum q0(0);
-  UniTensor A1(ALa_);
+  UniTensor A1(ALa);
   A1.permute(1);
-  A1.putBlock(q0, Lb_ * A1.getBlock(q0));
+  A1.putBlock(q0, Lb * A1.getBlock(q0));
   A1.permute(2);
-  UniTensor B1(BLb_);
+  UniTensor B1(BLb);
   B1.permute(1);
-  B1.putBlock(q0, La_ * B1.getBlock(q0));
+  B1.putBlock(q0, La * B1.getBlock(q0));
   B1.permute(2);
-  double val = expectation(A1, BLb_);
-  val += expectation(B1, ALa_);
+  double val = expectation(A1, BLb, H, MPS, meas);
+  val += expectation(B1, ALa, H, MPS, meas);
   return val / 2;
 }
+
+double ITebd::energy() { return itebd_energy(ALa_, BLb_, La_, Lb_, H_, MPS_, meas_); }
 
 }  // namespace uni10

@@ -8,6 +8,10 @@
 namespace uni10 {
 
 Matrix xxz_two_site(double delta, double J);
+// Energy per site of the iTEBD state (ALa, BLb, La, Lb) for the two-site
+// Hamiltonian H, through the MPS.net / measure.net networks.
+double itebd_energy(UniTensor& ALa, UniTensor& BLb, const Matrix& La, const Matrix& Lb, UniTensor& H,
+                    Network& MPS, Network& meas);
 
 class ITebd {
  public:
@@ -22,7 +26,6 @@ class ITebd {
 
  private:
   void update(UniTensor& ALa, UniTensor& BLb, Matrix& La, Matrix& Lb);
-  double expectation(UniTensor& L, UniTensor& R);
   int chi_, d_;
   double tau_;
   double trunc_err_ = 0;

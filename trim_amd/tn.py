@@ -75,6 +75,9 @@ SIGNATURES = {
     "tn_svd_plan_exec": [P, P, P, P, P, P, P],
     "tn_svd_plan_check": [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), P],
     "tn_svd_plan_destroy": [P],
+    "tn_vecops_plan_create": [ctypes.POINTER(ctypes.c_void_p), I64, P],
+    "tn_vecops_plan_exec": [P, P, P],
+    "tn_vecops_plan_destroy": [P],
     "tn_eigh_f64": [P, I64, P, P, P],
     "tn_truncate_f64": [P, I64, I64, P, P, P, P],
 }

@@ -79,6 +79,14 @@ SIGNATURES = {
     "u10_itebd_lambda": [P, I, P, PI],
     "u10_itebd_trunc_err": [P, PD],
     "u10_itebd_free": [P],
+    "u10_batch_tebd_create": [I, P, P, P, D, I, S, PP],
+    "u10_batch_tebd_step": [P, I],
+    "u10_batch_tebd_set_graph": [P, I],
+    "u10_batch_tebd_energy": [P, I, PD],
+    "u10_batch_tebd_lambda": [P, I, I, P, PI],
+    "u10_batch_tebd_trunc_err": [P, I, PD],
+    "u10_batch_tebd_info": [P, PD, PI],
+    "u10_batch_tebd_free": [P],
 }
 
 _lib = None
@@ -408,6 +416,57 @@ def exph(x: float, H):
     out = np.empty((n, n))
     _call("u10_exph", float(x), p, n, out.ctypes.data_as(P))
     return out
+
+
+class BatchTebd:
+    """Many independent XXZ chains advanced together: every stage of an
+    update is one launch for all chains (grouped GEMMs, batched SVD)."""
+
+    def __init__(self, chi, delta, seed, tau: float, netdir: str, gemm_mode: int = 0):
+        chi = np.ascontiguousarray(chi, dtype=np.int32)
+        delta = np.ascontiguousarray(delta, dtype=np.float64)
+        seed = np.ascontiguousarray(seed, dtype=np.uint64)
+        h = ctypes.c_void_p()
+        _call("u10_batch_tebd_create", len(chi), chi.ctypes.data_as(P), delta.ctypes.data_as(P),
+              seed.ctypes.data_as(P), float(tau), int(gemm_mode), netdir.encode(), ctypes.byref(h))
+        self.h = h.value
+        self.chi = chi
+
+    def step(self, n: int = 1):
+        _call("u10_batch_tebd_step", self.h, int(n))
+
+    def set_graph(self, on: bool = True):
+        _call("u10_batch_tebd_set_graph", self.h, int(bool(on)))
+
+    def energy(self, chain: int) -> float:
+        v = ctypes.c_double()
+        _call("u10_batch_tebd_energy", self.h, int(chain), ctypes.byref(v))
+        return v.value
+
+    def lam(self, chain: int, which: int) -> np.ndarray:
+        n = ctypes.c_int()
+        _call("u10_batch_tebd_lambda", self.h, int(chain), int(which), None, ctypes.byref(n))
+        out = np.empty(n.value)
+        _call("u10_batch_tebd_lambda", self.h, int(chain), int(which), out.ctypes.data_as(P), ctypes.byref(n))
+        return out
+
+    def trunc_err(self, chain: int) -> float:
+        v = ctypes.c_double()
+        _call("u10_batch_tebd_trunc_err", self.h, int(chain), ctypes.byref(v))
+        return v.value
+
+    def info(self):
+        f, s = ctypes.c_double(), ctypes.c_int()
+        _call("u10_batch_tebd_info", self.h, ctypes.byref(f), ctypes.byref(s))
+        return {"gemm_flops_per_step": f.value, "last_sweeps": s.value}
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                load().u10_batch_tebd_free(self.h)
+                self.h = None
+        except Exception:
+            pass
 
 
 class ITebd:
