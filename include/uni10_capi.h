@@ -90,6 +90,8 @@ int u10_batch_tebd_create(int n, const int* chi, const double* delta, const uint
                           int gemm_mode, const char* netdir, u10_batch_tebd* out);
 int u10_batch_tebd_step(u10_batch_tebd b, int nsteps);
 int u10_batch_tebd_set_graph(u10_batch_tebd b, int on);
+/* restart one chain (slot reuse): the seeded start of (delta, seed) */
+int u10_batch_tebd_load(u10_batch_tebd b, int chain, double delta, uint64_t seed);
 int u10_batch_tebd_energy(u10_batch_tebd b, int chain, double* out);
 int u10_batch_tebd_lambda(u10_batch_tebd b, int chain, int which, double* out, int* n);
 int u10_batch_tebd_trunc_err(u10_batch_tebd b, int chain, double* out);

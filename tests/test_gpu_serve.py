@@ -1,0 +1,55 @@
+"""Continuous-batching request loop (run with -m gpu): every result is the
+same alone and in a mix of requests that arrive while others run, and the
+TEBD requests follow the oracle's single-chain iTEBD."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def serve(tn):
+    from trim_amd import serve as s
+    from trim_amd import uni10
+    uni10.load()
+    return s
+
+
+def _mix(serve):
+    reqs = serve.poisson_requests(24, rate=4000.0, seed=5, chi_classes=[8, 16], steps_max=6)
+    return reqs
+
+
+def test_result_same_alone_and_in_mix(serve, golden):
+    reqs = _mix(serve)
+    budget = 4 * serve.slot_bytes(16) * 2          # about 4 slots per class: requests queue
+    srv = serve.Server([8, 16], budget, golden)
+    assert srv.slots(16) >= 1 and srv.slots(16) < 12
+    stats = srv.run(reqs, realtime=False)
+    _say("mix: %d requests, p50 %.2f ms p99 %.2f ms" % (stats["requests"], stats["p50_ms"], stats["p99_ms"]))
+    assert stats["requests"] == len(reqs)
+    mixed = {r.rid: r.result for r in stats["done"]}
+    for target in (reqs[3], reqs[10], reqs[17]):
+        alone = serve.Server([8, 16], budget, golden)
+        solo = serve.poisson_requests(24, rate=4000.0, seed=5, chi_classes=[8, 16], steps_max=6)[target.rid]
+        out = alone.run([solo], realtime=False)["done"][0].result
+        _say("request %d (%s chi=%d steps=%d)" % (target.rid, target.kind, target.chi, target.steps))
+        assert out.tobytes() == mixed[target.rid].tobytes()
+
+
+def test_tebd_request_follows_oracle(serve, ora, golden):
+    reqs = [serve.Request(rid=0, t_arrival=0.0, kind="tebd", chi=8, delta=0.8, seed=77, steps=5)]
+    srv = serve.Server([8], 4 * serve.slot_bytes(8), golden)
+    r = srv.run(reqs, realtime=False)["done"][0]
+    O = ora.ITEBD(8, 2, ora.heisenberg_two_site(0.8), 0.01, 77, golden)
+    for _ in range(5):
+        O.step()
+    assert np.max(np.abs(r.result[:8] - O.La)) <= 1e-10
+    assert np.max(np.abs(r.result[8:] - O.Lb)) <= 1e-10

@@ -94,42 +94,13 @@ BatchTebd::BatchTebd(const std::vector<int>& chi, const std::vector<double>& del
   }
   arena_.reset((size_t)o, true);
 
-  // initial state (as ITebd: splitmix seeds seed..seed+3) and gates
+  // initial state of every chain (as ITebd) and its gate
+  tau_ = tau;
   std::vector<double> host((size_t)o, 0.0);
   for (size_t i = 0; i < nc; ++i) {
-    const int c = chi[i];
-    const Off& f = off_[i];
-    const size_t n = (size_t)c * kD * c;
-    std::vector<double> ga(n), gb(n), la(c), lb(c);
-    for (size_t e = 0; e < n; ++e) {
-      ga[e] = detail::splitmix_uniform(seed[i], e + 1);
-      gb[e] = detail::splitmix_uniform(seed[i] + 1, e + 1);
-    }
-    for (int e = 0; e < c; ++e) {
-      la[e] = std::fabs(detail::splitmix_uniform(seed[i] + 2, e + 1));
-      lb[e] = std::fabs(detail::splitmix_uniform(seed[i] + 3, e + 1));
-    }
-    std::sort(la.rbegin(), la.rend());
-    std::sort(lb.rbegin(), lb.rend());
-    double na = 0, nb = 0;
-    for (int e = 0; e < c; ++e) {
-      na += la[e] * la[e];
-      nb += lb[e] * lb[e];
-    }
-    na = std::sqrt(na);
-    nb = std::sqrt(nb);
-    for (int e = 0; e < c; ++e) {
-      host[f.l[0] + e] = la[e] / na;
-      host[f.l[1] + e] = lb[e] / nb;
-    }
-    for (int a = 0; a < c; ++a)
-      for (int s = 0; s < kD; ++s)
-        for (int b = 0; b < c; ++b) {
-          host[f.x[0] + ((size_t)a * c + b) * kD + s] = ga[((size_t)a * kD + s) * c + b] * host[f.l[0] + b];
-          host[f.x[1] + ((size_t)a * c + b) * kD + s] = gb[((size_t)a * kD + s) * c + b] * host[f.l[1] + b];
-        }
-    const std::vector<double> g = takeExp(-tau, xxz_two_site(delta[i], 1.0)).getHostElem();
-    std::copy(g.begin(), g.end(), host.begin() + f.g);
+    const int64_t end = (i + 1 < nc) ? off_[i + 1].x[0] : o;
+    region_.push_back(end - off_[i].x[0]);
+    init_chain((int)i, delta[i], seed[i], host.data() + off_[i].x[0]);
   }
   detail::check(tn_memcpy_h2d(arena_.get(), host.data(), sizeof(double) * (size_t)o, stream_), "BatchTebd upload");
 
@@ -193,6 +164,64 @@ BatchTebd::BatchTebd(const std::vector<int>& chi, const std::vector<double>& del
     detail::check(tn_vecops_plan_create(&trunc_[p], (int64_t)trunc.size(), trunc.data()), "BatchTebd trunc");
     detail::check(tn_vecops_plan_create(&scale_[p], (int64_t)scale.size(), scale.data()), "BatchTebd scale");
   }
+}
+
+// Host image of chain i's arena region (base = its x[0]): Gamma lambda
+// tensors and lambdas from the seeded start of ITebd (splitmix seeds
+// seed..seed+3), the gate exp(-tau H(delta)), and identity warm-start bases
+// (so a chain's SVD sequence never depends on an earlier occupant).
+void BatchTebd::init_chain(int i, double delta, uint64_t seed, double* base) {
+  const int c = chi_[i];
+  const Off& f = off_[i];
+  auto at = [&](int64_t off) { return base + (off - f.x[0]); };
+  std::fill(base, base + region_[i], 0.0);
+  const size_t n = (size_t)c * kD * c;
+  std::vector<double> ga(n), gb(n), la(c), lb(c);
+  for (size_t e = 0; e < n; ++e) {
+    ga[e] = detail::splitmix_uniform(seed, e + 1);
+    gb[e] = detail::splitmix_uniform(seed + 1, e + 1);
+  }
+  for (int e = 0; e < c; ++e) {
+    la[e] = std::fabs(detail::splitmix_uniform(seed + 2, e + 1));
+    lb[e] = std::fabs(detail::splitmix_uniform(seed + 3, e + 1));
+  }
+  std::sort(la.rbegin(), la.rend());
+  std::sort(lb.rbegin(), lb.rend());
+  double na = 0, nb = 0;
+  for (int e = 0; e < c; ++e) {
+    na += la[e] * la[e];
+    nb += lb[e] * lb[e];
+  }
+  na = std::sqrt(na);
+  nb = std::sqrt(nb);
+  double* l0 = at(f.l[0]);
+  double* l1 = at(f.l[1]);
+  for (int e = 0; e < c; ++e) {
+    l0[e] = la[e] / na;
+    l1[e] = lb[e] / nb;
+  }
+  double* x0 = at(f.x[0]);
+  double* x1 = at(f.x[1]);
+  for (int a = 0; a < c; ++a)
+    for (int s = 0; s < kD; ++s)
+      for (int b = 0; b < c; ++b) {
+        x0[((size_t)a * c + b) * kD + s] = ga[((size_t)a * kD + s) * c + b] * l0[b];
+        x1[((size_t)a * c + b) * kD + s] = gb[((size_t)a * kD + s) * c + b] * l1[b];
+      }
+  const std::vector<double> g = takeExp(-tau_, xxz_two_site(delta, 1.0)).getHostElem();
+  std::copy(g.begin(), g.end(), at(f.g));
+  const int64_t D = (int64_t)c * kD;
+  for (int p = 0; p < 2; ++p)
+    for (int64_t e = 0; e < D; ++e) at(f.vt[p])[e * D + e] = 1.0;
+  delta_[i] = delta;
+}
+
+void BatchTebd::load(int chain, double delta, uint64_t seed) {
+  if (chain < 0 || chain >= chains()) detail::fail("BatchTebd::load: bad index");
+  std::vector<double> host((size_t)region_[chain]);
+  init_chain(chain, delta, seed, host.data());
+  detail::check(tn_memcpy_h2d(arena_.get() + off_[chain].x[0], host.data(), sizeof(double) * host.size(), stream_),
+                "BatchTebd load");
 }
 
 BatchTebd::~BatchTebd() {

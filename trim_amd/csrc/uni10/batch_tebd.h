@@ -25,6 +25,9 @@ class BatchTebd {
   BatchTebd& operator=(const BatchTebd&) = delete;
 
   void step(int n = 1);            // n x (update(A,B), update(B,A)) on every chain
+  // restart chain `chain` from the seeded start of (delta, seed): the slot
+  // reuse of continuous batching; other chains are untouched
+  void load(int chain, double delta, uint64_t seed);
   void set_graph(bool on);         // replay steps through a captured HIP graph
   double energy(int chain);        // energy per site (networks, one chain)
   std::vector<double> lambda(int chain, int which) const;   // 0: lambda_A, 1: lambda_B
@@ -38,6 +41,9 @@ class BatchTebd {
     int64_t x[2], l[2], g, t1, cb, th, u, s, vt[2], sc[2];
   };
   void update(int p, bool warm);
+  void init_chain(int i, double delta, uint64_t seed, double* base);
+  std::vector<int64_t> region_;   // arena doubles of each chain (from its x[0])
+  double tau_ = 0;
   void run_step();
   std::vector<int> chi_;
   std::vector<double> delta_;

@@ -346,6 +346,10 @@ int u10_batch_tebd_set_graph(u10_batch_tebd b, int on) {
   return guard([&] { b->B.set_graph(on != 0); });
 }
 
+int u10_batch_tebd_load(u10_batch_tebd b, int chain, double delta, uint64_t seed) {
+  return guard([&] { b->B.load(chain, delta, seed); });
+}
+
 int u10_batch_tebd_energy(u10_batch_tebd b, int chain, double* out) {
   return guard([&] { *out = b->B.energy(chain); });
 }

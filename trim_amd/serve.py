@@ -1,0 +1,179 @@
+"""Continuous-batching request loop over the batched iTEBD engine.
+
+Requests arrive on a Poisson clock and are of two kinds:
+
+* ``tebd``: run ``steps`` iTEBD steps of one spin-1/2 XXZ chain (bond
+  dimension ``chi``, anisotropy ``delta``, seeded start) and return its
+  Schmidt spectra;
+* ``contract``: contract two seeded (chi, d, chi) tensors over one bond
+  through the drop-in ``contract`` and return the result's norm.
+
+The server keeps one batched engine per bond-dimension class, each with a
+fixed number of slots whose tensors stay resident in HBM; the slot count of
+each class is what its share of the HBM budget holds.  Every tick it admits
+the requests that have arrived into free slots (first come, first served per
+class), advances every class with a busy slot by one step in one batched,
+graph-replayed launch sequence, and retires finished requests.  Requests that
+arrive while others run join at the next tick: a chain's computation never
+depends on which other slots are busy (slots are restarted with identity
+warm-start bases), so each result is the same alone and in the mix.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import time
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import uni10
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    t_arrival: float          # seconds from the start of the stream
+    kind: str                 # "tebd" or "contract"
+    chi: int
+    delta: float
+    seed: int
+    steps: int
+    d: int = 2
+    t_admit: float = -1.0
+    t_done: float = -1.0
+    result: Optional[np.ndarray] = None
+
+
+def poisson_requests(n: int, rate: float, seed: int, chi_classes, steps_max: int = 50,
+                     tebd_fraction: float = 0.5) -> List[Request]:
+    """n requests with exponential inter-arrival times (mean 1/rate), bond
+    dimension log-uniform over the classes' range and rounded to the nearest
+    class, 1..steps_max steps, anisotropy uniform in [0, 2]."""
+    rng = np.random.default_rng(seed)
+    classes = np.asarray(sorted(chi_classes), dtype=np.float64)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    lo, hi = math.log(classes[0]), math.log(classes[-1])
+    out = []
+    for i in range(n):
+        chi = float(np.exp(rng.uniform(lo, hi)))
+        chi = int(classes[np.argmin(np.abs(np.log(classes) - math.log(chi)))])
+        kind = "tebd" if rng.random() < tebd_fraction else "contract"
+        out.append(Request(rid=i, t_arrival=float(t[i]), kind=kind, chi=chi,
+                           delta=float(rng.uniform(0.0, 2.0)), seed=int(10_000 + i),
+                           steps=int(rng.integers(1, steps_max + 1))))
+    return out
+
+
+def slot_bytes(chi: int, d: int = 2) -> int:
+    """HBM bytes one chain of the batched engine keeps resident."""
+    D = chi * d
+    doubles = 2 * chi * chi * d + 2 * chi + d ** 4 + 3 * chi * chi * d * d + 3 * D * D + D + 4
+    return 8 * doubles
+
+
+class Server:
+    def __init__(self, chi_classes, hbm_budget: float, netdir: str, gemm_mode: int = 0,
+                 max_slots: int = 256, tau: float = 0.01):
+        self.netdir = netdir
+        self.gemm_mode = gemm_mode
+        self.classes = sorted(chi_classes)
+        share = hbm_budget / len(self.classes)
+        self.engines: Dict[int, uni10.BatchTebd] = {}
+        self.free: Dict[int, List[int]] = {}
+        self.busy: Dict[int, Dict[int, list]] = {}
+        for c in self.classes:
+            k = int(max(1, min(max_slots, share // slot_bytes(c))))
+            eng = uni10.BatchTebd([c] * k, [1.0] * k, [1 + s for s in range(k)], tau, netdir, gemm_mode)
+            eng.set_graph(True)
+            self.engines[c] = eng
+            self.free[c] = list(range(k))
+            self.busy[c] = {}
+        self.chain_steps = 0
+
+    def slots(self, chi: int) -> int:
+        return len(self.engines[chi].chi)
+
+    def _contract(self, r: Request) -> np.ndarray:
+        uni10.set_gemm_mode(self.gemm_mode)
+        try:
+            c, d = r.chi, r.d
+            bA = [uni10.Bond(uni10.BD_IN, c), uni10.Bond(uni10.BD_IN, d), uni10.Bond(uni10.BD_OUT, c)]
+            A = uni10.UniTensor(bA)
+            B = uni10.UniTensor(bA)
+            rng = np.random.default_rng(r.seed)
+            A.setRawElem(rng.uniform(-1, 1, c * d * c))
+            B.setRawElem(rng.uniform(-1, 1, c * d * c))
+            A.setLabel([0, 1, 2])
+            B.setLabel([2, 3, 4])
+            C = uni10.contract(A, B)
+            return np.array([C.norm()])
+        finally:
+            uni10.set_gemm_mode(0)
+
+    def run(self, requests: List[Request], realtime: bool = True) -> Dict:
+        """Serve the requests; returns latency and throughput figures."""
+        pending = sorted(requests, key=lambda r: r.t_arrival)
+        waiting: Dict[int, List[Request]] = {c: [] for c in self.classes}
+        contracts: List[Request] = []
+        done: List[Request] = []
+        t0 = time.perf_counter()
+        i = 0
+        clock = 0.0
+        while i < len(pending) or contracts or any(waiting.values()) or any(self.busy.values()):
+            now = (time.perf_counter() - t0) if realtime else clock
+            if not realtime and i < len(pending) and not any(self.busy.values()) and not contracts \
+                    and not any(waiting.values()):
+                now = clock = max(clock, pending[i].t_arrival)
+            while i < len(pending) and pending[i].t_arrival <= now:
+                r = pending[i]
+                (contracts if r.kind == "contract" else waiting[r.chi]).append(r)
+                i += 1
+            # admission: free slots, first come first served per class
+            for c in self.classes:
+                while waiting[c] and self.free[c]:
+                    r = waiting[c].pop(0)
+                    s = self.free[c].pop(0)
+                    self.engines[c].load(s, r.delta, r.seed)
+                    r.t_admit = now
+                    self.busy[c][s] = [r, r.steps]
+            # contractions run between steps
+            for r in contracts:
+                r.t_admit = now
+                r.result = self._contract(r)
+                r.t_done = (time.perf_counter() - t0) if realtime else now
+                done.append(r)
+            contracts = []
+            # one batched step of every class with a busy slot
+            for c in self.classes:
+                if not self.busy[c]:
+                    continue
+                self.engines[c].step(1)
+                self.chain_steps += len(self.busy[c])
+                for s in list(self.busy[c]):
+                    ent = self.busy[c][s]
+                    ent[1] -= 1
+                    if ent[1] == 0:
+                        r = ent[0]
+                        r.result = np.concatenate([self.engines[c].lam(s, 0), self.engines[c].lam(s, 1)])
+                        r.t_done = (time.perf_counter() - t0) if realtime else now
+                        done.append(r)
+                        del self.busy[c][s]
+                        self.free[c].append(s)
+            if not realtime:
+                clock = now + 1e-3
+            elif i < len(pending) and not any(self.busy.values()) and not contracts \
+                    and not any(waiting.values()):
+                time.sleep(max(0.0, min(0.01, pending[i].t_arrival - (time.perf_counter() - t0))))
+        wall = time.perf_counter() - t0
+        lat = np.array([r.t_done - r.t_arrival for r in done])
+        tebd_steps = sum(r.steps for r in done if r.kind == "tebd")
+        return {
+            "requests": len(done),
+            "wall_s": wall,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3) if len(lat) else None,
+            "p99_ms": float(np.percentile(lat, 99) * 1e3) if len(lat) else None,
+            "tebd_steps_per_s": tebd_steps / wall,
+            "chain_steps": self.chain_steps,
+            "done": done,
+        }

@@ -82,6 +82,7 @@ SIGNATURES = {
     "u10_batch_tebd_create": [I, P, P, P, D, I, S, PP],
     "u10_batch_tebd_step": [P, I],
     "u10_batch_tebd_set_graph": [P, I],
+    "u10_batch_tebd_load": [P, I, D, ctypes.c_uint64],
     "u10_batch_tebd_energy": [P, I, PD],
     "u10_batch_tebd_lambda": [P, I, I, P, PI],
     "u10_batch_tebd_trunc_err": [P, I, PD],
@@ -437,6 +438,9 @@ class BatchTebd:
 
     def set_graph(self, on: bool = True):
         _call("u10_batch_tebd_set_graph", self.h, int(bool(on)))
+
+    def load(self, chain: int, delta: float, seed: int):
+        _call("u10_batch_tebd_load", self.h, int(chain), float(delta), int(seed))
 
     def energy(self, chain: int) -> float:
         v = ctypes.c_double()
