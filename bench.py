@@ -210,9 +210,60 @@ def tebd_rate(u10, steps, chi=32):
     t.step(5)
     t0 = time.perf_counter()
     t.step(steps)
-    e = t.energy()
     dt = time.perf_counter() - t0
-    return steps / dt, e, t.trunc_err()
+    return steps / dt, t.energy(), t.trunc_err()
+
+
+def c4_batch(u10, chis, steps, label):
+    """Batched iTEBD: 512 XXZ chains (delta ~ U[0,2]) with the given bond
+    dimensions, every update stage one launch for all chains, steps replayed
+    as a captured graph.  One step = update(A,B) + update(B,A) of every chain."""
+    n = 512
+    chi = [int(chis[i % len(chis)]) for i in range(n)]
+    delta = np.random.default_rng(4).uniform(0.0, 2.0, n)
+    B = u10.BatchTebd(chi, delta, [1000 + 4 * i for i in range(n)], 0.01, NETDIR)
+    B.set_graph(True)
+    B.step(1)   # first SVDs from scratch
+    B.step(1)   # capture
+    t0 = time.perf_counter()
+    B.step(steps)
+    dt = time.perf_counter() - t0
+    info = B.info()
+    return {"chains": n, "chi": label, "symmetry": "dense", "steps": steps, "ms_per_step": dt / steps * 1e3,
+            "steps_per_s": steps / dt, "chain_updates_per_s": 2.0 * n * steps / dt,
+            "gemm_tflops": info["gemm_flops_per_step"] * steps / dt / 1e12,
+            "svd_sweeps_last_update": info["last_sweeps"]}
+
+
+def mixed_precision_energy(u10, steps=20):
+    """Energy per site after `steps` batched steps in each GEMM mode, against
+    FP64 (SVD, truncation and energies FP64 in every mode)."""
+    chi, delta, seed = [8, 16, 24, 32], [0.5, 1.0, 1.5, 1.0], [21, 22, 23, 24]
+    e = {}
+    for mode, name in ((0, "f64"), (1, "f32"), (2, "bf16x3"), (3, "bf16x6")):
+        B = u10.BatchTebd(chi, delta, seed, 0.01, NETDIR, mode)
+        B.step(steps)
+        e[name] = np.array([B.energy(i) for i in range(len(chi))])
+    return {k: float(np.max(np.abs(v - e["f64"]))) for k, v in e.items() if k != "f64"}
+
+
+def c5_serve(u10, n_req, rate, classes=(16, 32)):
+    """Poisson stream of TEBD-step and contraction requests served by
+    continuous batching over resident slots, FP64 and FP32-MFMA GEMMs."""
+    from trim_amd import serve
+    out, res = {}, {}
+    for mode, name in ((0, "f64"), (1, "f32")):
+        srv = serve.Server(list(classes), 8e9, NETDIR, gemm_mode=mode, max_slots=256)
+        srv.warmup()
+        reqs = serve.poisson_requests(n_req, rate, seed=5, chi_classes=list(classes))
+        st = srv.run(reqs, realtime=True)
+        res[name] = {r.rid: r.result for r in st["done"]}
+        out[name] = {k: st[k] for k in ("requests", "wall_s", "p50_ms", "p99_ms", "tebd_steps_per_s")}
+        out[name]["slots"] = {str(c): srv.slots(c) for c in classes}
+        del srv
+    diff = max(float(np.max(np.abs(res["f32"][k] - res["f64"][k]))) for k in res["f64"])
+    out["max_result_diff_f32_vs_f64"] = diff
+    return out
 
 
 def cpu_baseline(budget_s=10.0, max_steps=400):
@@ -266,6 +317,10 @@ def cpu_baseline(budget_s=10.0, max_steps=400):
             "ms_per_step": dt * 1e3, "tebd_steps_per_s": tebd}
 
 
+def progress(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -273,6 +328,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--tebd-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-chain and serving legs")
+    ap.add_argument("--serve-requests", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path in this build (independent "
@@ -301,6 +358,7 @@ def main():
     flops = 2.0 * 512 ** 3
     value = flops / (ms * 1e-3) / 1e12
 
+    progress("headline step done: %.4f ms/step" % ms)
     parts, kflops, n, keep, ginfo = kernel_parts(tn)
     per = time_parts(tn, parts, max(20, min(args.steps, 200)))
     gemm_tf = kflops / (per["gemm_f64"] * 1e-3) / 1e12
@@ -316,8 +374,22 @@ def main():
         roof = {"bound": "hbm", "achieved": perm_gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": perm_gbps / HBM_PEAK_GBPS, "traffic": None, "kernel": "tiled_permute_kernel",
                 "avg_ms": per["permute"], "bytes_per_launch": 16 * n}
+    progress("kernel parts and HBM permute done")
     tebd_sps, tebd_e, tebd_te = tebd_rate(uni10, args.tebd_steps)
+    progress("iTEBD chi=32: %.1f steps/s" % tebd_sps)
     c3 = c3_u1(tn, uni10)
+    progress("U(1) chi=1024 contraction modes done")
+    c4 = {}
+    if not args.no_batch:
+        c4["chi_16_32"] = c4_batch(uni10, [16, 32], 10, "16 and 32, 256 chains each")
+        progress("batched iTEBD chi 16/32: %.1f steps/s" % c4["chi_16_32"]["steps_per_s"])
+        c4["chi_16_to_256"] = c4_batch(uni10, list(range(16, 257, 16)), 1, "16..256 step 16, 32 chains each")
+        progress("batched iTEBD chi 16..256: %.3f steps/s" % c4["chi_16_to_256"]["steps_per_s"])
+        c4["energy_err_vs_f64_after_20_steps"] = mixed_precision_energy(uni10)
+        progress("mixed-precision energies done")
+        c5 = c5_serve(uni10, args.serve_requests, 2000.0)
+        progress("serving done")
+        c5["speedup_f32_serving_vs_f64_dropin"] = c5["f32"]["tebd_steps_per_s"] / tebd_sps
     out = {
         "metric": "sector-GEMM TFLOP/s + permute GB/s; TEBD steps/s, p99 latency vs plain FP64",
         "value": value, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -335,11 +407,14 @@ def main():
         "tebd_chi32": {"steps_per_s": tebd_sps, "steps": args.tebd_steps, "energy_per_site": tebd_e,
                        "trunc_err_last": tebd_te},
         "c3_u1_chi1024": c3,
+        "c4_batched_itebd": c4,
+        "c5_serving": (c5 if not args.no_batch else None),
         "wall_s_timed": wall,
         "device": info,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
+        progress("cpu baseline done")
     print(json.dumps(out), flush=True)
 
 

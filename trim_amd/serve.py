@@ -91,6 +91,12 @@ class Server:
             self.busy[c] = {}
         self.chain_steps = 0
 
+    def warmup(self):
+        """Run the first (from-scratch) step and the graph capture of every
+        engine before requests arrive; slots are restarted on admission."""
+        for eng in self.engines.values():
+            eng.step(2)
+
     def slots(self, chi: int) -> int:
         return len(self.engines[chi].chi)
 
