@@ -155,6 +155,35 @@ def permute_hbm(tn, steps):
     return 16.0 * n / (ms * 1e-3) / 1e9, ms
 
 
+def copy_calibration(tn, mib=256):
+    """One plain device-to-device copy of a known size: the reference the
+    FETCH_SIZE / WRITE_SIZE counters are scaled by (bytes the counter reports
+    per byte actually moved)."""
+    n = mib * 1024 * 1024 // 8
+    a = tn.DeviceArray((n,), zero=True)
+    b = tn.DeviceArray((n,))
+    tn.call("tn_memcpy_d2d", tn._p(b), tn._p(a), a.nbytes, None)
+    tn.sync()
+    a.free()
+    b.free()
+    return mib * 1024 * 1024
+
+
+def recorded_traffic(kernel):
+    """HBM bytes per launch of the headline GEMM from the committed counter
+    summary (two rocprofv3 --pmc passes, FETCH_SIZE then WRITE_SIZE, scaled by
+    the calibration copy), or None when there is none for this kernel."""
+    path = os.path.join(ROOT, "profiles", "traffic_gemm_f64_512x512x512.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != kernel:
+        return None
+    return t.get("bytes_per_launch")
+
+
 def c3_u1(tn, u10, steps=10):
     """U(1) two-site contraction, chi=1024 over Sz in {-8..8} (binomial sector
     dims), d=2: Theta = (A1 . A2) . G through the drop-in API in every GEMM
@@ -330,6 +359,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-chain and serving legs")
     ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline step, its kernels and the copy calibration (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path in this build (independent "
@@ -374,7 +405,13 @@ def main():
         roof = {"bound": "hbm", "achieved": perm_gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": perm_gbps / HBM_PEAK_GBPS, "traffic": None, "kernel": "tiled_permute_kernel",
                 "avg_ms": per["permute"], "bytes_per_launch": 16 * n}
+    copy_bytes = copy_calibration(tn)
+    roof["traffic"] = recorded_traffic(roof["kernel"])
     progress("kernel parts and HBM permute done")
+    if args.headline_only:
+        print(json.dumps({"metric": "headline step (counter run)", "ms_per_step": ms, "roofline": roof,
+                          "kernels_ms": per, "copy_calibration_bytes": copy_bytes}), flush=True)
+        return
     tebd_sps, tebd_e, tebd_te = tebd_rate(uni10, args.tebd_steps)
     progress("iTEBD chi=32: %.1f steps/s" % tebd_sps)
     c3 = c3_u1(tn, uni10)

@@ -36,6 +36,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <vector>
 
@@ -624,8 +625,26 @@ static int build_tiles(tn_gemm_plan_s* P, int cfg) {
     if (g.M == 0 || g.N == 0 || g.K == 0) continue;
     int64_t tm = (g.M + bm - 1) / bm, tn_ = (g.N + bn - 1) / bn;
     TN_REQUIRE(tm < (1 << 30) && tn_ < (1 << 30), TN_EUNSUPP, "GEMM too large");
-    for (int64_t b = 0; b < tn_; ++b)
-      for (int64_t a = 0; a < tm; ++a) nat.push_back(TileDev{(int32_t)gi, (int32_t)a, (int32_t)b, 0});
+    // The group's tile grid is cut into px x py regions (px * py = 8, the
+    // split that minimises the rows of A plus columns of B one region
+    // reads), listed region by region: with the XCD dealing below each XCD
+    // then computes one compact block of C instead of a full-height strip.
+    int64_t px = 1, py = 8, best = INT64_MAX;
+    for (int64_t cx : {1, 2, 4, 8}) {
+      const int64_t cy = 8 / cx;
+      const int64_t cost = ((tm + cx - 1) / cx) * bm + ((tn_ + cy - 1) / cy) * bn;
+      if (cost < best) {
+        best = cost;
+        px = cx;
+        py = cy;
+      }
+    }
+    const int64_t rm = (tm + px - 1) / px, rn = (tn_ + py - 1) / py;
+    for (int64_t rx = 0; rx < px; ++rx)
+      for (int64_t ry = 0; ry < py; ++ry)
+        for (int64_t b = ry * rn; b < std::min(tn_, (ry + 1) * rn); ++b)
+          for (int64_t a = rx * rm; a < std::min(tm, (rx + 1) * rm); ++a)
+            nat.push_back(TileDev{(int32_t)gi, (int32_t)a, (int32_t)b, 0});
   }
   const int64_t n = (int64_t)nat.size();
   // XCD-aware placement: natural tile i goes to XCD i / per, slot i % per,
