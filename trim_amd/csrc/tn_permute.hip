@@ -108,6 +108,10 @@ struct TileParams {
   int64_t o_dst[kMaxRank];
 };
 
+// VL: rows are read as 16-byte pairs along s; VS: columns are written as
+// 16-byte pairs along d (the planner checks that every pair is in range and
+// aligned).
+template <bool VL, bool VS>
 __global__ void __launch_bounds__(256) tiled_permute_kernel(const double* __restrict__ src,
                                                             double* __restrict__ dst,
                                                             TileParams p) {
@@ -172,7 +176,17 @@ __global__ void __launch_bounds__(256) tiled_permute_kernel(const double* __rest
   const double* sbase = src + base_src_sh;
   double* dbase = dst + base_dst_sh;
   // load: lanes run along s (contiguous in source)
-  {
+  if constexpr (VL) {
+    const int ws = p.ws / 2;
+    const int s = 2 * (tid % ws);
+    for (int d = tid / ws; d < dv; d += 256 / ws) {
+      if (s < sv) {
+        const double2 v = *reinterpret_cast<const double2*>(sbase + srow_src[d] + s);
+        tile[d][s] = v.x;
+        tile[d][s + 1] = v.y;
+      }
+    }
+  } else {
     const int ws = p.ws;
     const int s = tid % ws;
     for (int d = tid / ws; d < dv; d += 256 / ws) {
@@ -181,7 +195,13 @@ __global__ void __launch_bounds__(256) tiled_permute_kernel(const double* __rest
   }
   __syncthreads();
   // store: lanes run along d (contiguous in destination)
-  {
+  if constexpr (VS) {
+    const int wd = p.wd / 2;
+    const int d = 2 * (tid % wd);
+    for (int s = tid / wd; s < sv; s += 256 / wd) {
+      if (d < dv) *reinterpret_cast<double2*>(dbase + scol_dst[s] + d) = double2{tile[d][s], tile[d + 1][s]};
+    }
+  } else {
     const int wd = p.wd;
     const int d = tid % wd;
     for (int s = tid / wd; s < sv; s += 256 / wd) {
@@ -198,6 +218,7 @@ struct Plan {
   bool vec2 = false;
   TileParams tile{};
   int64_t grid = 0;
+  bool tile_vl = false, tile_vs = false;   // 16-byte pairs along s / d
 };
 
 // Tile of a tiled transpose for a target run length tgt (64, 32 or 16).
@@ -391,6 +412,12 @@ int make_plan(int rank, const int64_t* dims_in, const int* perm_in, Plan* out) {
     out->grid = grid;
     if (grid >= want) break;
   }
+  // Pairs along s stay inside a row when the S group's full extent and every
+  // chunk's extent are even (then each row start and valid length is even);
+  // likewise along d.
+  const TileParams& T = out->tile;
+  out->tile_vl = (T.s_inner * T.s_chunk_full) % 2 == 0 && (T.s_inner * T.s_chunk_len) % 2 == 0;
+  out->tile_vs = (T.d_inner * T.d_chunk_full) % 2 == 0 && (T.d_inner * T.d_chunk_len) % 2 == 0;
   return TN_OK;
 }
 
@@ -421,11 +448,15 @@ int run_plan(const Plan& P, const double* src, double* dst, hipStream_t s) {
       TN_LAUNCH_CHECK();
       return TN_OK;
     }
-    case Plan::kTile:
-      hipLaunchKernelGGL(tiled_permute_kernel, dim3((unsigned)P.grid), dim3(256), 0, s, src, dst,
-                         P.tile);
+    case Plan::kTile: {
+      const bool aligned = (((uintptr_t)src | (uintptr_t)dst) & 15) == 0;
+      const bool vl = P.tile_vl && aligned, vs = P.tile_vs && aligned;
+      auto k = vl ? (vs ? tiled_permute_kernel<true, true> : tiled_permute_kernel<true, false>)
+                  : (vs ? tiled_permute_kernel<false, true> : tiled_permute_kernel<false, false>);
+      hipLaunchKernelGGL(k, dim3((unsigned)P.grid), dim3(256), 0, s, src, dst, P.tile);
       TN_LAUNCH_CHECK();
       return TN_OK;
+    }
   }
   return TN_EINVAL;
 }
