@@ -383,7 +383,7 @@ def _sweeps(tn):
     return v.value
 
 
-@pytest.mark.parametrize("n", [16, 64, 100])
+@pytest.mark.parametrize("n", [16, 64, 100, 200])
 def test_svd_warm_start_nearby_matrix(tn, ora, n):
     """Warm start from the VT of a nearby matrix: the same SVD (LAPACK
     singular values, reconstruction, orthonormal factors) in fewer sweeps."""
@@ -408,11 +408,81 @@ def test_svd_warm_start_nearby_matrix(tn, ora, n):
     assert warm < cold, (warm, cold)
 
 
+@pytest.mark.parametrize("M,N", [(100, 80), (80, 100), (96, 96), (70, 65), (130, 67)])
+def test_svd_block_jacobi_forced(tn, ora, M, N):
+    """Matrices that fit one workgroup's LDS, sent down the block Jacobi path
+    (the path of every larger matrix): LAPACK's singular values and vectors."""
+    _say("svd block %dx%d" % (M, N))
+    tn.call("tn_svd_set_block_min", 65)
+    try:
+        _svd_check(tn, ora, _rand(M * 1000 + N + 7, (M, N)))
+    finally:
+        tn.call("tn_svd_set_block_min", 0)
+
+
+def _orth(rng, n):
+    q, r = np.linalg.qr(rng.normal(size=(n, n)))
+    return q * np.sign(np.diag(r))
+
+
+@pytest.mark.parametrize("n", [192, 320])
+def test_svd_block_graded_spectrum(tn, ora, n):
+    """A theta-like spectrum decaying over ten decades, cold start: every
+    singular value to 1e-12 of the largest, the factors reconstruct A, and
+    the well-separated leading vectors match LAPACK's."""
+    _say("svd graded n=%d" % n)
+    rng = np.random.default_rng(n)
+    s = np.exp(-np.arange(n) * (23.0 / n))
+    a = _orth(rng, n) @ np.diag(s) @ _orth(rng, n).T
+    A = tn.DeviceArray.from_host(a)
+    u, sg, vt = (x.to_host() for x in tn.svd(A, n, n))
+    ur, sr, vtr = ora.svd(a)
+    assert np.max(np.abs(sg - sr)) <= 1e-12 * sr[0]
+    assert np.linalg.norm(u @ np.diag(sg) @ vt - a) <= 1e-12 * np.linalg.norm(a)
+    for j in range(n):
+        if sr[j] < 1e-6:
+            break
+        sgn = np.sign(np.dot(u[:, j], ur[:, j]))
+        assert np.max(np.abs(sgn * u[:, j] - ur[:, j])) <= 1e-8
+        assert np.max(np.abs(sgn * vt[j] - vtr[j])) <= 1e-8
+
+
+def test_svd_block_large_warm(tn, ora):
+    """512 x 512 (the theta of bond dimension 256): cold, then warm-started
+    from the right singular vectors of a nearby matrix, as the iTEBD update
+    does; both give LAPACK's singular values, and the warm start needs fewer
+    sweeps."""
+    n = 512
+    rng = np.random.default_rng(5)
+    s = np.exp(-np.arange(n) * (20.0 / n))
+    a0 = _orth(rng, n) @ np.diag(s) @ _orth(rng, n).T
+    a1 = a0 + 1e-2 * (rng.uniform(-1, 1, (n, n)) / np.sqrt(n)) @ a0
+    A0 = tn.DeviceArray.from_host(a0)
+    _say("svd 512 cold")
+    _, S0, VT0 = tn.svd(A0, n, n)
+    cold = _sweeps(tn)
+    assert np.max(np.abs(S0.to_host() - ora.svd(a0)[1])) <= 1e-12
+    A1 = tn.DeviceArray.from_host(a1)
+    U, S, VT = tn.DeviceArray((n, n)), tn.DeviceArray((n,)), tn.DeviceArray((n, n))
+    _say("svd 512 warm")
+    tn.call("tn_svd_f64_warm", tn._p(A1), n, n, tn._p(VT0), tn._p(U), tn._p(S), tn._p(VT), None)
+    warm = _sweeps(tn)
+    _say("sweeps cold %d warm %d" % (cold, warm))
+    u, sg, vt = U.to_host(), S.to_host(), VT.to_host()
+    sr = ora.svd(a1)[1]
+    assert np.max(np.abs(sg - sr)) <= 1e-12 * sr[0]
+    assert np.linalg.norm(u @ np.diag(sg) @ vt - a1) <= 1e-12 * np.linalg.norm(a1)
+    assert np.allclose(vt @ vt.T, np.eye(n), atol=1e-12)
+    k = int(np.sum(sr > 1e-6))
+    assert np.allclose(u[:, :k].T @ u[:, :k], np.eye(k), atol=1e-10)
+    assert warm < cold, (warm, cold)
+
+
 def test_svd_plan_replay_and_graph(tn, ora):
-    """A batch plan over every kernel path (small, LDS, scratch), with one
+    """A batch plan over every kernel path (small, LDS, block Jacobi), with one
     warm-started problem: direct execution, a second execution and graph
     replays all give the LAPACK singular values."""
-    shapes = [(64, 64), (30, 20), (100, 90), (160, 150), (64, 64)]
+    shapes = [(64, 64), (30, 20), (100, 90), (160, 150), (64, 64), (200, 200)]
     mats = [_rand(i + 70, s) for i, s in enumerate(shapes)]
     A = np.concatenate([m.reshape(-1) for m in mats])
     offs = np.cumsum([0] + [m.size for m in mats])[:-1].astype(np.int64)
@@ -422,11 +492,13 @@ def test_svd_plan_replay_and_graph(tn, ora):
     vt_off = np.cumsum([0] + [k * s[1] for s, k in zip(shapes, ks)])[:-1].astype(np.int64)
     m = np.array([s[0] for s in shapes], np.int64)
     n = np.array([s[1] for s in shapes], np.int64)
-    # the last problem starts from the right singular vectors of the first
+    # problem 4 starts from the right singular vectors of problem 0, problem 5
+    # (block Jacobi) from those of a nearby matrix
     v0t = ora.svd(mats[0])[2]
-    v0_off = np.array([-1, -1, -1, -1, 0], np.int64)
+    v5 = ora.svd(mats[5] + 1e-3 * _rand(99, (200, 200)))[2]
+    v0_off = np.array([-1, -1, -1, -1, 0, v0t.size], np.int64)
     dA = tn.DeviceArray.from_host(A)
-    dV0 = tn.DeviceArray.from_host(v0t)
+    dV0 = tn.DeviceArray.from_host(np.concatenate([v0t.reshape(-1), v5.reshape(-1)]))
     dU = tn.DeviceArray((int(sum(s[0] * k for s, k in zip(shapes, ks))),))
     dS = tn.DeviceArray((int(sum(ks)),))
     dV = tn.DeviceArray((int(sum(k * s[1] for s, k in zip(shapes, ks))),))

@@ -490,7 +490,7 @@ typedef void (*KernFn)(const double*, const double*, double*, const GroupDev*, c
 template <typename T, bool VEC, bool KT>
 KernFn pick_mfma(int cfg) {
   switch (cfg) {
-    case 0: return gemm_mfma_k32<T, 32, 32, 128, 2, VEC, KT>;
+    case 0: return gemm_mfma_k32<T, 32, 32, 32, 3, VEC, KT>;
     case 1: return gemm_mfma_k32<T, 64, 64, 64, 2, VEC, KT>;
     case 2: return gemm_mfma_k32<T, 128, 64, 32, 2, VEC, KT>;
     case 3: return gemm_mfma_k32<T, 64, 128, 32, 2, VEC, KT>;

@@ -5,9 +5,10 @@
 //    on device (uni10_lapack_gpu.cu:418-500).  Jacobi is used here because it
 //    is parallel within a round (disjoint column pairs), gives singular values
 //    to high relative accuracy, and is deterministic (fixed pairing order,
-//    fixed reduction trees).  One workgroup per matrix; the working columns
-//    (W = A or A^T, column-major) and the accumulated rotations V live in LDS
-//    when they fit, otherwise in a device scratch area of the same layout.
+//    fixed reduction trees).  Matrices that fit one workgroup's LDS are
+//    solved by one workgroup each, the working columns (W = A or A^T,
+//    column-major) and the accumulated rotations V in LDS; larger ones by
+//    the block Jacobi of tn_svd_block.hip (many workgroups per matrix).
 //  * eigh: parallel two-sided Jacobi (replaces dsyev, uni10_lapack_cpu.cpp:131-152,
 //    which the reference's GPU build also ran on the host,
 //    uni10_lapack_gpu.cu:380-416).
@@ -22,6 +23,7 @@
 #include <vector>
 
 #include "tn_internal.h"
+#include "tn_svd_block.h"
 
 namespace {
 
@@ -32,7 +34,6 @@ constexpr size_t kLdsBytes = 152 * 1024;   // 160 KiB per CU minus the kernel's 
 struct SvdProb {
   int64_t m, n;              // A is m x n row-major
   int64_t a_off, u_off, s_off, vt_off;
-  int64_t scratch_off;       // offset (doubles) into the scratch area, -1 if LDS
   int64_t v0_off;            // warm start: offset of an n x n V0^T (m >= n), -1 = none
 };
 
@@ -66,19 +67,16 @@ __device__ inline void rr_pair(int r, int k, int np, int& p, int& q) {
   }
 }
 
-// kLds: W and V live in LDS (the compiler then emits ds_* accesses); otherwise
-// in the device scratch area.  The two cases are separate instantiations so
-// that no generic (flat) pointer reaches the inner loops.  TP lanes work on
-// one column pair (few lanes per pair: short reductions, which bound the
-// latency of a round).
-template <bool kLds, int TP>
+// One workgroup per matrix, W and V in LDS.  TP lanes work on one column
+// pair (few lanes per pair: short reductions, which bound the latency of a
+// round).
+template <int TP>
 __global__ void __launch_bounds__(kThreads) svd_jacobi_kernel(const SvdProb* __restrict__ probs,
                                                               const double* __restrict__ A,
                                                               const double* __restrict__ V0T,
                                                               double* __restrict__ U,
                                                               double* __restrict__ S,
                                                               double* __restrict__ VT,
-                                                              double* __restrict__ scratch,
                                                               int* __restrict__ status) {
   extern __shared__ double lds[];
   __shared__ double sig[512];
@@ -93,7 +91,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_kernel(const SvdProb* __r
   // Column strides are padded by one double so that the columns handled by
   // different pair groups in a round start on different LDS banks.
   const int ldw = m + 1, ldv = n + 1;
-  double* W = kLds ? lds : scratch + pr.scratch_off;
+  double* W = lds;
   double* V = W + (size_t)ldw * n;
   const int tid = threadIdx.x;
   const double* Ab = A + pr.a_off;
@@ -729,14 +727,15 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
 }  // namespace
 
 struct tn_svd_plan_s {
-  std::vector<SvdProb> probs;   // small-kernel problems, then LDS, then scratch
-  size_t n_small = 0, n_lds = 0, n_glb = 0;
+  std::vector<SvdProb> probs;   // small-kernel problems, then LDS ones
+  size_t n_small = 0, n_lds = 0;
   int64_t small_mm = 0;
   size_t lds_need = 0;
-  int tp_lds = 8, tp_glb = 8;
+  int tp_lds = 8;
   SvdProb* d_p = nullptr;
-  double* d_scratch = nullptr;
+  tn::bj::Plan* blk = nullptr;  // block Jacobi: problems too large for LDS
   int* d_status = nullptr;      // [0] problems not converged, [1] max sweeps
+  bool v0_any = false;          // some problem has a warm start
 };
 
 namespace {
@@ -744,28 +743,29 @@ namespace {
 void svd_plan_free(tn_svd_plan_s* P) {
   tn::pool_free(P->d_status);
   tn::pool_free(P->d_p);
-  tn::pool_free(P->d_scratch);
+  tn::bj::destroy(P->blk);
   delete P;
 }
 
-typedef void (*SvdKern)(const SvdProb*, const double*, const double*, double*, double*, double*, double*,
-                        int*);
+typedef void (*SvdKern)(const SvdProb*, const double*, const double*, double*, double*, double*, int*);
 typedef void (*SvdSmallKern)(const SvdProb*, const double*, const double*, double*, double*, double*, int*);
 
-SvdKern pick_svd_kernel(bool in_lds, int tp) {
-  if (in_lds)
-    return (tp == 8) ? svd_jacobi_kernel<true, 8> : (tp == 4) ? svd_jacobi_kernel<true, 4> : svd_jacobi_kernel<true, 2>;
-  return (tp == 8) ? svd_jacobi_kernel<false, 8> : (tp == 4) ? svd_jacobi_kernel<false, 4> : svd_jacobi_kernel<false, 2>;
+SvdKern pick_svd_kernel(int tp) {
+  return (tp == 8) ? svd_jacobi_kernel<8> : (tp == 4) ? svd_jacobi_kernel<4> : svd_jacobi_kernel<2>;
 }
 
 void raise_lds_limit(int tp, size_t bytes);
 
+// Problems with min(M, N) >= this go to the block path even when they would
+// fit one workgroup's LDS (0: only those that do not fit).
+int64_t g_block_min = 0;
+
 int svd_plan_build(tn_svd_plan_s** out, int64_t batch, const int64_t* m, const int64_t* n,
                    const int64_t* a_off, const int64_t* v0_off, const int64_t* u_off,
                    const int64_t* s_off, const int64_t* vt_off) {
-  std::vector<SvdProb> small_probs, lds_probs, glb_probs;
+  std::vector<SvdProb> small_probs, lds_probs;
+  std::vector<tn::bj::Spec> blk;
   auto* P = new tn_svd_plan_s();
-  int64_t scratch_total = 0;
   for (int64_t i = 0; i < batch; ++i) {
     if (m[i] < 0 || n[i] < 0) {
       delete P;
@@ -773,24 +773,18 @@ int svd_plan_build(tn_svd_plan_s** out, int64_t batch, const int64_t* m, const i
       return TN_EINVAL;
     }
     const int64_t mm = std::max(m[i], n[i]), nn = std::min(m[i], n[i]);
-    if (nn > 512) {
-      delete P;
-      tn::set_error("svd: min(M,N)=%lld > 512 is not supported yet", (long long)nn);
-      return TN_EUNSUPP;
-    }
     SvdProb p{m[i], n[i], a_off ? a_off[i] : 0, u_off ? u_off[i] : 0, s_off ? s_off[i] : 0,
-              vt_off ? vt_off[i] : 0, -1, v0_off ? v0_off[i] : -1};
+              vt_off ? vt_off[i] : 0, v0_off ? v0_off[i] : -1};
+    P->v0_any |= p.v0_off >= 0;
     const size_t bytes = sizeof(double) * (size_t)((mm + 1) * nn + (nn + 1) * nn);
     if (nn <= kSmallN && mm <= kSmallTP * 16) {
       small_probs.push_back(p);
       P->small_mm = std::max(P->small_mm, mm);
-    } else if (bytes <= kLdsBytes) {
+    } else if (bytes <= kLdsBytes && (g_block_min <= 0 || nn < g_block_min)) {
       P->lds_need = std::max(P->lds_need, bytes);
       lds_probs.push_back(p);
-    } else {
-      p.scratch_off = scratch_total;
-      scratch_total += (mm + 1) * nn + (nn + 1) * nn;
-      glb_probs.push_back(p);
+    } else if (mm > 0 && nn > 0) {
+      blk.push_back(tn::bj::Spec{p.m, p.n, p.a_off, p.u_off, p.s_off, p.vt_off, p.v0_off});
     }
   }
   // lanes per column pair: as few as the widest problem allows with one
@@ -802,19 +796,16 @@ int svd_plan_build(tn_svd_plan_s** out, int64_t batch, const int64_t* m, const i
     return pairs <= 32 ? 8 : (pairs <= 64 ? 4 : 2);
   };
   P->tp_lds = pick_tp(lds_probs);
-  P->tp_glb = pick_tp(glb_probs);
   if (!lds_probs.empty()) raise_lds_limit(P->tp_lds, P->lds_need);
   P->n_small = small_probs.size();
   P->n_lds = lds_probs.size();
-  P->n_glb = glb_probs.size();
   P->probs = small_probs;
   P->probs.insert(P->probs.end(), lds_probs.begin(), lds_probs.end());
-  P->probs.insert(P->probs.end(), glb_probs.begin(), glb_probs.end());
-  int rc = tn::pool_alloc((void**)&P->d_status, 2 * sizeof(int));
+  int rc = tn::bj::build(&P->blk, blk);
+  if (rc == TN_OK) rc = tn::pool_alloc((void**)&P->d_status, 2 * sizeof(int));
   if (rc == TN_OK && !P->probs.empty()) rc = tn::pool_alloc((void**)&P->d_p, sizeof(SvdProb) * P->probs.size());
   if (rc == TN_OK && !P->probs.empty())
     rc = tn_memcpy_h2d(P->d_p, P->probs.data(), sizeof(SvdProb) * P->probs.size(), nullptr);
-  if (rc == TN_OK && scratch_total > 0) rc = tn::pool_alloc((void**)&P->d_scratch, sizeof(double) * scratch_total);
   if (rc != TN_OK) {
     svd_plan_free(P);
     return rc;
@@ -832,11 +823,13 @@ void raise_lds_limit(int tp, size_t bytes) {
   const int slot = (tp == 8) ? 0 : (tp == 4) ? 1 : 2;
   std::lock_guard<std::mutex> g(mu);
   if (bytes <= limit[slot]) return;
-  (void)hipFuncSetAttribute((const void*)pick_svd_kernel(true, tp), hipFuncAttributeMaxDynamicSharedMemorySize,
+  (void)hipFuncSetAttribute((const void*)pick_svd_kernel(tp), hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)bytes);
   (void)hipGetLastError();
   limit[slot] = bytes;
 }
+
+bool svd_plan_empty(const tn_svd_plan_s* P) { return P->probs.empty() && tn::bj::empty(P->blk); }
 
 int svd_plan_launch(const tn_svd_plan_s* P, const double* A, const double* V0T, double* U, double* S,
                     double* VT, hipStream_t st) {
@@ -849,18 +842,12 @@ int svd_plan_launch(const tn_svd_plan_s* P, const double* A, const double* V0T, 
     TN_LAUNCH_CHECK();
   }
   if (P->n_lds > 0) {
-    SvdKern k = pick_svd_kernel(true, P->tp_lds);
+    SvdKern k = pick_svd_kernel(P->tp_lds);
     hipLaunchKernelGGL(k, dim3((unsigned)P->n_lds), dim3(kThreads), P->lds_need, st, P->d_p + P->n_small, A, V0T, U,
-                       S, VT, P->d_scratch, P->d_status);
+                       S, VT, P->d_status);
     TN_LAUNCH_CHECK();
   }
-  if (P->n_glb > 0) {
-    SvdKern k = pick_svd_kernel(false, P->tp_glb);
-    hipLaunchKernelGGL(k, dim3((unsigned)P->n_glb), dim3(kThreads), 0, st, P->d_p + P->n_small + P->n_lds, A, V0T,
-                       U, S, VT, P->d_scratch, P->d_status);
-    TN_LAUNCH_CHECK();
-  }
-  return TN_OK;
+  return tn::bj::launch(P->blk, A, V0T, U, S, VT, P->d_status, st);
 }
 
 int svd_plan_status(const tn_svd_plan_s* P, int* nonconv, int* sweeps, hipStream_t st) {
@@ -894,6 +881,12 @@ int svd_batched(int64_t batch, const int64_t* m, const int64_t* n, const double*
 
 
 extern "C" {
+
+int tn_svd_set_block_min(int64_t n) {
+  TN_REQUIRE(n >= 0, TN_EINVAL, "tn_svd_set_block_min: negative size");
+  g_block_min = n;
+  return TN_OK;
+}
 
 int tn_svd_last_sweeps(int* sweeps) {
   TN_REQUIRE(sweeps != nullptr, TN_EINVAL, "tn_svd_last_sweeps: NULL out pointer");
@@ -944,10 +937,9 @@ int tn_svd_plan_exec(tn_svd_plan P, const double* A, const double* V0T, double* 
                      void* stream) {
   TN_INIT_OR_RETURN();
   TN_REQUIRE(P != nullptr, TN_EINVAL, "tn_svd_plan_exec: NULL plan");
-  if (P->probs.empty()) return TN_OK;
+  if (svd_plan_empty(P)) return TN_OK;
   TN_REQUIRE(A && U && S && VT, TN_EINVAL, "tn_svd_plan_exec: NULL buffer");
-  bool needs_v0 = false;
-  for (const auto& p : P->probs) needs_v0 |= p.v0_off >= 0;
+  bool needs_v0 = P->v0_any;
   TN_REQUIRE(!needs_v0 || V0T, TN_EINVAL, "tn_svd_plan_exec: the plan has warm starts but V0T is NULL");
   return svd_plan_launch(P, A, V0T, U, S, VT, tn::stream_of(stream));
 }

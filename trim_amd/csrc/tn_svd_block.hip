@@ -1,0 +1,644 @@
+// Block one-sided Jacobi SVD for matrices that do not fit one workgroup's
+// LDS (the iTEBD theta of bond dimension 48 and up).
+//
+// The working matrix W (A, A^T or A V0; column-major, rows padded to a
+// multiple of 16, columns to a multiple of 32 with zero columns) is split
+// into column blocks of 16.  A sweep is a round-robin tournament over the
+// blocks; in each round every block pair (P, Q) of every problem is handled
+// by one workgroup, all in one launch:
+//   G = [W_P W_Q]^T [W_P W_Q]                 32 x 32, FP64 MFMA over the rows
+//   if a column pair of the panel is not yet orthogonal:
+//     Q^T G Q = diag                            cyclic Jacobi on G in LDS
+//     [W_P W_Q] <- [W_P W_Q] Q, [V_P V_Q] <- [V_P V_Q] Q    FP64 MFMA
+// Each sweep first orders the columns by decreasing norm (de Rijk), so that
+// a block holds columns of similar norm; this halves the sweeps a graded
+// spectrum needs.  Every column pair meets in some block pair of a sweep,
+// so a sweep that rotates nothing leaves W with orthogonal columns: the
+// singular values are the column norms of W, U its normalised columns and V
+// the accumulated rotations (A = W V^T).  The reference computes the same factorisation with
+// dgesvd (uni10_lapack_cpu.cpp:289-313) or cuSOLVER gesvd
+// (uni10_lapack_gpu.cu:418-500).
+//
+// Convergence: a pair (p, q) is left alone when |w_p.w_q| is below
+// 10 m eps |w_p||w_q| (the rounding noise of a dot product of orthogonal
+// columns) or below m eps^2 ||A||_F^2 (columns at rounding level), the
+// criterion of the one-workgroup kernels.  The Jacobi rotations of the
+// panel's Gram matrix keep the relative structure of a graded panel (a
+// rotation between a large and a small column has a small angle, computed
+// to relative accuracy), so small columns are not swamped by the panel's
+// large ones and the same criterion is met.
+#include "tn_svd_block.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+
+#include "tn_internal.h"
+
+namespace tn {
+namespace bj {
+namespace {
+
+constexpr int kB = 16;            // columns per block
+constexpr int kPW = 2 * kB;       // panel width (two blocks)
+constexpr int kThreads = 256;
+constexpr int kMaxSweeps = 40;
+constexpr int kMaxSweepsCaptured = 16;
+constexpr int kInnerSweeps = 10;
+constexpr int kSetupBlocks = 64;
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+struct Prob {
+  int64_t M, N;                           // caller's A: M x N row-major
+  int64_t a_off, u_off, s_off, vt_off, v0_off;
+  int64_t w_off, v_off, sig_off;          // scratch (doubles): W, V, column norms + ||A||_F^2
+  int32_t m, n;                           // working W: m x n, m >= n
+  int32_t mp, nvp, np, nb;                // padded rows of W / V, padded columns, blocks
+  int32_t tr;                             // W = A^T
+  int32_t flag_off, rank_off, pos_off;    // int scratch: rotated-in-sweep flags, ranks, sweep order
+};
+
+__device__ inline void rr_pair(int r, int k, int np, int& p, int& q) {
+  const int L = np - 1;
+  if (k == 0) {
+    p = r;
+    q = L;
+  } else {
+    p = (r + k) % L;
+    q = (r - k + L) % L;
+  }
+  if (p > q) {
+    const int t = p;
+    p = q;
+    q = t;
+  }
+}
+
+__device__ inline bool needs_rot(double gpq, double gpp, double gqq, double tol, double abs_tol) {
+  const double a = fabs(gpq);
+  return a > tol * sqrt(fmax(gpp, 0.0) * fmax(gqq, 0.0)) && a > abs_tol;
+}
+
+__device__ inline double block_sum(double v, double* red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+#pragma unroll
+  for (int o = kThreads / 2; o > 0; o >>= 1) {
+    if (tid < o) red[tid] += red[tid + o];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// W (and V) of every problem, padding included: W = A, A^T or zero (a warm
+// start's W = A V0 is written next by the GEMM), V = V0 or I.  Clears the
+// problem's sweep flags and the launch-wide ones.
+__global__ void __launch_bounds__(kThreads) bj_setup(const Prob* __restrict__ probs, const double* __restrict__ A,
+                                                     const double* __restrict__ V0T, double* __restrict__ scr,
+                                                     int* __restrict__ iscr) {
+  const Prob pr = probs[blockIdx.y];
+  const bool warm = pr.v0_off >= 0;
+  double* W = scr + pr.w_off;
+  double* V = scr + pr.v_off;
+  const int64_t totW = (int64_t)pr.mp * pr.np, totV = (int64_t)pr.nvp * pr.np;
+  const int64_t stride = (int64_t)gridDim.x * kThreads;
+  for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < totW + totV; e += stride) {
+    if (e < totW) {
+      const int64_t c = e / pr.mp, r = e % pr.mp;
+      double v = 0.0;
+      if (!warm && r < pr.m && c < pr.n) v = pr.tr ? A[pr.a_off + c * pr.N + r] : A[pr.a_off + r * pr.N + c];
+      W[e] = v;
+    } else {
+      const int64_t e2 = e - totW;
+      const int64_t c = e2 / pr.nvp, r = e2 % pr.nvp;
+      double v = 0.0;
+      if (r < pr.n && c < pr.n) v = warm ? V0T[pr.v0_off + c * pr.n + r] : (r == c ? 1.0 : 0.0);
+      V[e2] = v;
+    }
+  }
+  if (blockIdx.x == 0) {
+    for (int s = threadIdx.x; s < kMaxSweeps; s += kThreads) {
+      iscr[pr.flag_off + s] = 0;
+      if (blockIdx.y == 0) iscr[s] = 0;
+    }
+  }
+}
+
+// ||W||_F^2 (= ||A||_F^2 up to rounding), fixed-order reduction.
+__global__ void __launch_bounds__(kThreads) bj_norm(const Prob* __restrict__ probs, double* __restrict__ scr) {
+  __shared__ double red[kThreads];
+  const Prob pr = probs[blockIdx.x];
+  const double* W = scr + pr.w_off;
+  const int64_t tot = (int64_t)pr.mp * pr.np;
+  double s = 0.0;
+  for (int64_t e = threadIdx.x; e < tot; e += kThreads) s += W[e] * W[e];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) scr[pr.sig_off + pr.n] = s;
+}
+
+// Start of a sweep: the columns ordered by decreasing norm (ties by index;
+// padding columns last).  Block b of the sweep is the columns at positions
+// [16b, 16b + 16) of this order, so a block holds columns of similar norm.
+__global__ void __launch_bounds__(kThreads) bj_sort(const Prob* __restrict__ probs, double* __restrict__ scr,
+                                                    int* __restrict__ iscr, int sweep) {
+  const Prob pr = probs[blockIdx.x];
+  if (sweep > 0 && iscr[pr.flag_off + sweep - 1] == 0) return;
+  const double* W = scr + pr.w_off;
+  double* nrm = scr + pr.sig_off;
+  int* pos = iscr + pr.pos_off;
+  const int tid = threadIdx.x;
+  for (int j = tid; j < pr.n; j += kThreads) {
+    const double* wj = W + (int64_t)j * pr.mp;
+    double s = 0.0;
+    for (int i = 0; i < pr.m; ++i) s += wj[i] * wj[i];
+    nrm[j] = s;
+  }
+  __syncthreads();
+  for (int j = tid; j < pr.np; j += kThreads) {
+    if (j >= pr.n) {
+      pos[j] = j;
+      continue;
+    }
+    const double sj = nrm[j];
+    int rk = 0;
+    for (int i = 0; i < pr.n; ++i) {
+      const double si = nrm[i];
+      rk += (si > sj) || (si == sj && i < j);
+    }
+    pos[rk] = j;
+  }
+}
+
+// One round of a sweep: workgroup = (problem, block pair).
+__global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ probs, const int2* __restrict__ wmap,
+                                                     double* __restrict__ scr, int* __restrict__ iscr, int sweep,
+                                                     int round) {
+  __shared__ double part[4][3][256];
+  __shared__ double G[kPW][kPW + 1];
+  __shared__ double Qm[kPW][kPW + 1];
+  __shared__ double r_c[kB], r_s[kB], r_t[kB];
+  __shared__ int r_p[kB], r_q[kB];
+
+  const int2 wm = wmap[blockIdx.x];
+  const Prob pr = probs[wm.x];
+  if (round >= pr.nb - 1) return;
+  int* flags = iscr + pr.flag_off;
+  if (sweep > 0 && flags[sweep - 1] == 0) return;   // converged in an earlier sweep
+  int P, Q;
+  rr_pair(round, wm.y, pr.nb, P, Q);
+  const int64_t mp = pr.mp, nvp = pr.nvp;
+  double* W = scr + pr.w_off;
+  double* V = scr + pr.v_off;
+  const double normF2 = scr[pr.sig_off + pr.n];
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, lr = l & 15, lg = l >> 4;
+  // panel column c < 16 is W column pos[16P + c], c >= 16 is pos[16Q + c - 16]
+  const int* posP = iscr + pr.pos_off + P * kB;
+  const int* posQ = iscr + pr.pos_off + Q * kB;
+  const int64_t myP = posP[lr], myQ = posQ[lr];
+
+  // ---- G = X^T X: three 16 x 16 tiles (PP, PQ, QQ), rows split over the waves
+  {
+    // lane (lr, lg) of k-step s supplies X[4s + lg][col lr] of each block
+    const double* xp = W + myP * mp + lg;
+    const double* xq = W + myQ * mp + lg;
+    d4 aPP = {0.0, 0.0, 0.0, 0.0}, aPQ = aPP, aQQ = aPP;
+    const int per = (int)(mp >> 4);   // 4-row steps per wave (mp is a multiple of 16)
+    int s = w * per;
+    const int e = s + per;
+    for (; s + 4 <= e; s += 4) {
+      double p[4], q[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        p[u] = xp[4 * (s + u)];
+        q[u] = xq[4 * (s + u)];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        aPP = __builtin_amdgcn_mfma_f64_16x16x4f64(p[u], p[u], aPP, 0, 0, 0);
+        aPQ = __builtin_amdgcn_mfma_f64_16x16x4f64(p[u], q[u], aPQ, 0, 0, 0);
+        aQQ = __builtin_amdgcn_mfma_f64_16x16x4f64(q[u], q[u], aQQ, 0, 0, 0);
+      }
+    }
+    for (; s < e; ++s) {
+      const double p = xp[4 * s], q = xq[4 * s];
+      aPP = __builtin_amdgcn_mfma_f64_16x16x4f64(p, p, aPP, 0, 0, 0);
+      aPQ = __builtin_amdgcn_mfma_f64_16x16x4f64(p, q, aPQ, 0, 0, 0);
+      aQQ = __builtin_amdgcn_mfma_f64_16x16x4f64(q, q, aQQ, 0, 0, 0);
+    }
+    // accumulator element v of lane (lr, lg) is tile entry (lg + 4v, lr)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int idx = (lg + 4 * v) * 16 + lr;
+      part[w][0][idx] = aPP[v];
+      part[w][1][idx] = aPQ[v];
+      part[w][2][idx] = aQQ[v];
+    }
+  }
+  __syncthreads();
+  // sum the waves' partials in a fixed order; the diagonal tiles are read
+  // from their upper triangle so that G is exactly symmetric
+  for (int e = tid; e < kPW * kPW; e += kThreads) {
+    const int i = e / kPW, j = e % kPW;
+    int t, a, b;
+    if (i < kB && j < kB) {
+      t = 0;
+      a = min(i, j);
+      b = max(i, j);
+    } else if (i >= kB && j >= kB) {
+      t = 2;
+      a = min(i, j) - kB;
+      b = max(i, j) - kB;
+    } else if (i < kB) {
+      t = 1;
+      a = i;
+      b = j - kB;
+    } else {
+      t = 1;
+      a = j;
+      b = i - kB;
+    }
+    const int idx = a * 16 + b;
+    G[i][j] = ((part[0][t][idx] + part[1][t][idx]) + part[2][t][idx]) + part[3][t][idx];
+  }
+  __syncthreads();
+
+  const double tol = 10.0 * DBL_EPSILON * (double)pr.m;
+  const double abs_tol = (double)pr.m * DBL_EPSILON * DBL_EPSILON * normF2;
+  int need = 0;
+  for (int e = tid; e < kPW * kPW; e += kThreads) {
+    const int i = e / kPW, j = e % kPW;
+    if (i < j && needs_rot(G[i][j], G[i][i], G[j][j], tol, abs_tol)) need = 1;
+  }
+  if (!__syncthreads_or(need)) return;
+
+  // ---- cyclic Jacobi on G (16 disjoint rotations per step), Q accumulated
+  for (int e = tid; e < kPW * kPW; e += kThreads) Qm[e / kPW][e % kPW] = (e / kPW == e % kPW) ? 1.0 : 0.0;
+  __syncthreads();
+  int any_rot = 0;
+  for (int isw = 0; isw < kInnerSweeps; ++isw) {
+    int rot = 0;
+    for (int r = 0; r < kPW - 1; ++r) {
+      if (tid < kB) {
+        int p, q;
+        rr_pair(r, tid, kPW, p, q);
+        const double app = G[p][p], aqq = G[q][q], apq = G[p][q];
+        double c = 1.0, s = 0.0, t = 0.0;
+        if (needs_rot(apq, app, aqq, tol, abs_tol)) {
+          const double zeta = (aqq - app) / (2.0 * apq);
+          t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          c = 1.0 / sqrt(1.0 + t * t);
+          s = c * t;
+          rot = 1;
+        }
+        r_c[tid] = c;
+        r_s[tid] = s;
+        r_t[tid] = t;
+        r_p[tid] = p;
+        r_q[tid] = q;
+      }
+      __syncthreads();
+      {
+        // G <- J^T G J, one 2 x 2 block {p_a, q_a} x {p_b, q_b} per thread
+        // (a < b; the thread also writes the transposed block)
+        const int a = tid >> 4, b = tid & 15;
+        if (a < b) {
+          const double c1 = r_c[a], s1 = r_s[a], c2 = r_c[b], s2 = r_s[b];
+          if (s1 != 0.0 || s2 != 0.0) {
+            const int p1 = r_p[a], q1 = r_q[a], p2 = r_p[b], q2 = r_q[b];
+            const double x = G[p1][p2], y = G[p1][q2], z = G[q1][p2], u = G[q1][q2];
+            const double rp0 = c1 * x - s1 * z, rp1 = c1 * y - s1 * u;
+            const double rq0 = s1 * x + c1 * z, rq1 = s1 * y + c1 * u;
+            const double npp = c2 * rp0 - s2 * rp1, npq = s2 * rp0 + c2 * rp1;
+            const double nqp = c2 * rq0 - s2 * rq1, nqq = s2 * rq0 + c2 * rq1;
+            G[p1][p2] = npp;
+            G[p2][p1] = npp;
+            G[p1][q2] = npq;
+            G[q2][p1] = npq;
+            G[q1][p2] = nqp;
+            G[p2][q1] = nqp;
+            G[q1][q2] = nqq;
+            G[q2][q1] = nqq;
+          }
+        } else if (a == b && r_s[a] != 0.0) {
+          const int p = r_p[a], q = r_q[a];
+          const double apq = G[p][q], t = r_t[a];
+          G[p][p] -= t * apq;
+          G[q][q] += t * apq;
+          G[p][q] = 0.0;
+          G[q][p] = 0.0;
+        }
+      }
+      // Q <- Q J: columns p, q of every rotated pair
+      for (int e = tid; e < kB * kPW; e += kThreads) {
+        const int k = e / kPW, i = e % kPW;
+        const double s = r_s[k];
+        if (s != 0.0) {
+          const int p = r_p[k], q = r_q[k];
+          const double c = r_c[k];
+          const double x = Qm[i][p], y = Qm[i][q];
+          Qm[i][p] = c * x - s * y;
+          Qm[i][q] = s * x + c * y;
+        }
+      }
+      __syncthreads();
+    }
+    rot = __syncthreads_or(rot);
+    any_rot |= rot;
+    if (!rot) break;
+  }
+  if (!any_rot) return;
+  if (tid == 0) {
+    atomicOr(&flags[sweep], 1);
+    atomicOr(&iscr[sweep], 1);
+  }
+
+  // ---- [X_P X_Q] <- [X_P X_Q] Q for X = W (mp rows) and V (nvp rows),
+  // 16-row tiles dealt to the waves; a tile is read whole before it is
+  // written, and no other wave touches its rows
+  double qb[2][8];
+#pragma unroll
+  for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+    for (int s = 0; s < 8; ++s) qb[jt][s] = Qm[4 * s + lg][16 * jt + lr];
+  int64_t gcol[8];   // W column of panel column 4s + lg
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const int col = 4 * s + lg;
+    gcol[s] = (col < kB) ? posP[col] : posQ[col - kB];
+  }
+  const int tW = (int)(mp >> 4), tV = (int)(nvp >> 4);
+  for (int t = w; t < tW + tV; t += 4) {
+    double* base;
+    int64_t ld, r0;
+    if (t < tW) {
+      base = W;
+      ld = mp;
+      r0 = (int64_t)t * 16;
+    } else {
+      base = V;
+      ld = nvp;
+      r0 = (int64_t)(t - tW) * 16;
+    }
+    double a[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) a[s] = base[gcol[s] * ld + r0 + lr];
+    d4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = c0;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], qb[0][s], c0, 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], qb[1][s], c1, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t row = r0 + lg + 4 * v;
+      base[myP * ld + row] = c0[v];
+      base[myQ * ld + row] = c1[v];
+    }
+  }
+}
+
+// Singular values (exact column norms, descending; ties by column index),
+// their ranks, and the convergence status.
+__global__ void __launch_bounds__(kThreads) bj_sigma(const Prob* __restrict__ probs, double* __restrict__ scr,
+                                                     int* __restrict__ iscr, double* __restrict__ S, int nsweeps,
+                                                     int* __restrict__ status) {
+  const Prob pr = probs[blockIdx.x];
+  const double* W = scr + pr.w_off;
+  double* sig = scr + pr.sig_off;
+  int* rank = iscr + pr.rank_off;
+  const int tid = threadIdx.x;
+  for (int j = tid; j < pr.n; j += kThreads) {
+    const double* wj = W + (int64_t)j * pr.mp;
+    double s = 0.0;
+    for (int i = 0; i < pr.m; ++i) s += wj[i] * wj[i];
+    sig[j] = s;
+  }
+  __syncthreads();
+  for (int j = tid; j < pr.n; j += kThreads) {
+    const double sj = sig[j];
+    int rk = 0;
+    for (int i = 0; i < pr.n; ++i) {
+      const double si = sig[i];
+      rk += (si > sj) || (si == sj && i < j);
+    }
+    rank[j] = rk;
+    S[pr.s_off + rk] = sqrt(sj);
+  }
+  if (tid == 0) {
+    int sw = -1;
+    for (int s = 0; s < nsweeps; ++s)
+      if (iscr[pr.flag_off + s] == 0) {
+        sw = s + 1;
+        break;
+      }
+    if (sw < 0) {
+      atomicAdd(status, 1);
+      sw = nsweeps;
+    }
+    atomicMax(status + 1, sw);
+  }
+}
+
+// U and VT from the normalised columns of W and from V, in rank order.
+__global__ void __launch_bounds__(kThreads) bj_vectors(const Prob* __restrict__ probs, const double* __restrict__ scr,
+                                                       const int* __restrict__ iscr, double* __restrict__ U,
+                                                       double* __restrict__ VT) {
+  const Prob pr = probs[blockIdx.y];
+  const double* W = scr + pr.w_off;
+  const double* V = scr + pr.v_off;
+  const double* sig = scr + pr.sig_off;
+  const int* rank = iscr + pr.rank_off;
+  const int64_t m = pr.m, n = pr.n;
+  const int64_t totU = m * n, totV = n * n;
+  const int64_t stride = (int64_t)gridDim.x * kThreads;
+  for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < totU + totV; e += stride) {
+    if (e < totU) {
+      const int64_t j = e / m, i = e % m;
+      const double sj = sqrt(sig[j]);
+      const double v = (sj > 0.0) ? W[j * pr.mp + i] / sj : 0.0;
+      const int rk = rank[j];
+      if (!pr.tr)
+        U[pr.u_off + i * n + rk] = v;        // U is M x n
+      else
+        VT[pr.vt_off + rk * pr.N + i] = v;   // VT is n x N
+    } else {
+      const int64_t e2 = e - totU;
+      const int64_t j = e2 / n, i = e2 % n;
+      const int rk = rank[j];
+      const double vv = V[j * pr.nvp + i];
+      if (!pr.tr)
+        VT[pr.vt_off + rk * n + i] = vv;     // VT is n x n
+      else
+        U[pr.u_off + i * n + rk] = vv;       // U is M x M
+    }
+  }
+}
+
+}  // namespace
+
+struct Plan {
+  std::vector<Prob> probs;
+  Prob* d_probs = nullptr;
+  int2* d_map = nullptr;
+  int64_t pairs = 0;
+  int max_nb = 0;
+  double* d_scr = nullptr;
+  int* d_iscr = nullptr;
+  tn_gemm_plan warm = nullptr;    // W = A V0 of the warm-started problems
+};
+
+static int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+void destroy(Plan* P) {
+  if (!P) return;
+  if (P->warm) tn_gemm_plan_destroy(P->warm);
+  tn::pool_free(P->d_probs);
+  tn::pool_free(P->d_map);
+  tn::pool_free(P->d_scr);
+  tn::pool_free(P->d_iscr);
+  delete P;
+}
+
+bool empty(const Plan* P) { return P == nullptr || P->probs.empty(); }
+
+int build(Plan** out, const std::vector<Spec>& specs) {
+  auto* P = new Plan();
+  int64_t scr = 0, iscr = kMaxSweeps;
+  std::vector<int2> map;
+  std::vector<tn_gemm_desc_ex> gd;
+  for (size_t i = 0; i < specs.size(); ++i) {
+    const Spec& s = specs[i];
+    Prob p{};
+    p.M = s.M;
+    p.N = s.N;
+    p.a_off = s.a_off;
+    p.u_off = s.u_off;
+    p.s_off = s.s_off;
+    p.vt_off = s.vt_off;
+    p.v0_off = s.v0_off;
+    p.tr = s.M < s.N;
+    const int64_t m = std::max(s.M, s.N), n = std::min(s.M, s.N);
+    const int64_t np = round_up(n, kPW), mp = round_up(m, 16), nvp = round_up(n, 16);
+    if (p.tr && s.v0_off >= 0) {
+      destroy(P);
+      tn::set_error("svd: a warm start needs M >= N");
+      return TN_EINVAL;
+    }
+    if (mp * np >= (int64_t(1) << 31) || np > (1 << 15)) {
+      destroy(P);
+      tn::set_error("svd: %lld x %lld is too large", (long long)s.M, (long long)s.N);
+      return TN_EUNSUPP;
+    }
+    p.m = (int32_t)m;
+    p.n = (int32_t)n;
+    p.mp = (int32_t)mp;
+    p.nvp = (int32_t)nvp;
+    p.np = (int32_t)np;
+    p.nb = (int32_t)(np / kB);
+    p.w_off = scr;
+    scr += mp * np;
+    p.v_off = scr;
+    scr += nvp * np;
+    p.sig_off = scr;
+    scr += round_up(n + 1, 2);
+    p.flag_off = (int32_t)iscr;
+    iscr += kMaxSweeps;
+    p.rank_off = (int32_t)iscr;
+    iscr += n;
+    p.pos_off = (int32_t)iscr;
+    iscr += np;
+    for (int k = 0; k < p.nb / 2; ++k) map.push_back(make_int2((int)i, k));
+    P->max_nb = std::max(P->max_nb, p.nb);
+    if (s.v0_off >= 0) {
+      // W(i, j) = sum_k A(i, k) V0T(j, k), W column-major with leading dimension mp
+      tn_gemm_desc_ex g{};
+      g.M = m;
+      g.N = n;
+      g.K = n;
+      g.a_off = s.a_off;
+      g.lda = s.N;
+      g.b_off = s.v0_off;
+      g.c_off = p.w_off;
+      g.b_row = tn_index_map{1, 0, {n, 0, 0, 0}, {1, 0, 0, 0}};
+      g.b_col = tn_index_map{1, 0, {n, 0, 0, 0}, {n, 0, 0, 0}};
+      g.c_row = tn_index_map{1, 0, {m, 0, 0, 0}, {1, 0, 0, 0}};
+      g.c_col = tn_index_map{1, 0, {n, 0, 0, 0}, {mp, 0, 0, 0}};
+      g.ldb = n;
+      g.ldc = mp;
+      gd.push_back(g);
+    }
+    P->probs.push_back(p);
+  }
+  P->pairs = (int64_t)map.size();
+  if (P->probs.empty()) {
+    *out = P;
+    return TN_OK;
+  }
+  int rc = tn::pool_alloc((void**)&P->d_probs, sizeof(Prob) * P->probs.size());
+  if (rc == TN_OK) rc = tn::pool_alloc((void**)&P->d_map, sizeof(int2) * map.size());
+  if (rc == TN_OK) rc = tn::pool_alloc((void**)&P->d_scr, sizeof(double) * scr);
+  if (rc == TN_OK) rc = tn::pool_alloc((void**)&P->d_iscr, sizeof(int) * iscr);
+  if (rc == TN_OK) rc = tn_memcpy_h2d(P->d_probs, P->probs.data(), sizeof(Prob) * P->probs.size(), nullptr);
+  if (rc == TN_OK) rc = tn_memcpy_h2d(P->d_map, map.data(), sizeof(int2) * map.size(), nullptr);
+  if (rc == TN_OK && !gd.empty()) rc = tn_gemm_plan_create_ex(&P->warm, TN_GEMM_F64, (int64_t)gd.size(), gd.data());
+  if (rc != TN_OK) {
+    destroy(P);
+    return rc;
+  }
+  *out = P;
+  return TN_OK;
+}
+
+int launch(Plan* P, const double* A, const double* V0T, double* U, double* S, double* VT, int* status,
+           hipStream_t st) {
+  if (empty(P)) return TN_OK;
+  const unsigned np = (unsigned)P->probs.size();
+  hipLaunchKernelGGL(bj_setup, dim3(kSetupBlocks, np), dim3(kThreads), 0, st, P->d_probs, A, V0T, P->d_scr,
+                     P->d_iscr);
+  TN_LAUNCH_CHECK();
+  if (P->warm) {
+    const int rc = tn_gemm_exec(P->warm, A, V0T, P->d_scr, st);
+    if (rc != TN_OK) return rc;
+  }
+  hipLaunchKernelGGL(bj_norm, dim3(np), dim3(kThreads), 0, st, P->d_probs, P->d_scr);
+  TN_LAUNCH_CHECK();
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  TN_HIP_CHECK(hipStreamIsCapturing(st, &cap));
+  const bool capturing = cap != hipStreamCaptureStatusNone;
+  // Under capture every sweep is launched (a converged problem's workgroups
+  // return at once); otherwise the host stops after a sweep that rotated
+  // nothing in any problem.
+  const int max_sw = capturing ? kMaxSweepsCaptured : kMaxSweeps;
+  int ran = 0;
+  for (int s = 0; s < max_sw; ++s) {
+    hipLaunchKernelGGL(bj_sort, dim3(np), dim3(kThreads), 0, st, P->d_probs, P->d_scr, P->d_iscr, s);
+    TN_LAUNCH_CHECK();
+    for (int r = 0; r < P->max_nb - 1; ++r) {
+      hipLaunchKernelGGL(bj_round, dim3((unsigned)P->pairs), dim3(kThreads), 0, st, P->d_probs, P->d_map, P->d_scr,
+                         P->d_iscr, s, r);
+      TN_LAUNCH_CHECK();
+    }
+    ran = s + 1;
+    if (!capturing && s >= 1) {
+      int any = 1;
+      TN_HIP_CHECK(hipMemcpyAsync(&any, P->d_iscr + s, sizeof(int), hipMemcpyDeviceToHost, st));
+      TN_HIP_CHECK(hipStreamSynchronize(st));
+      if (!any) break;
+    }
+  }
+  hipLaunchKernelGGL(bj_sigma, dim3(np), dim3(kThreads), 0, st, P->d_probs, P->d_scr, P->d_iscr, S, ran, status);
+  TN_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bj_vectors, dim3(kSetupBlocks, np), dim3(kThreads), 0, st, P->d_probs, P->d_scr, P->d_iscr, U,
+                     VT);
+  TN_LAUNCH_CHECK();
+  return TN_OK;
+}
+
+}  // namespace bj
+}  // namespace tn

@@ -70,6 +70,7 @@ SIGNATURES = {
     "tn_svd_f64": [P, I64, I64, P, P, P, P],
     "tn_svd_f64_batched": [I64, P, P, P, P, P, P, P, P, P, P, P],
     "tn_svd_last_sweeps": [ctypes.POINTER(ctypes.c_int)],
+    "tn_svd_set_block_min": [ctypes.c_int64],
     "tn_svd_f64_warm": [P, I64, I64, P, P, P, P, P],
     "tn_svd_plan_create": [ctypes.POINTER(ctypes.c_void_p), I64, P, P, P, P, P, P, P],
     "tn_svd_plan_exec": [P, P, P, P, P, P, P],
