@@ -197,11 +197,12 @@ int tn_svd_f64_batched(int64_t batch, const int64_t* m, const int64_t* n,
 int tn_svd_last_sweeps(int* sweeps);
 /* Solver choice (tuning hook, no counterpart in the reference): a problem
  * with min(M, N) <= 64 and max(M, N) <= 128 is solved by one wavefront-
- * parallel workgroup; a larger one that fits one workgroup's LDS by one
- * workgroup in LDS, unless min(M, N) >= n_min; the rest by block Jacobi
- * (32-column panels, FP64 MFMA Gram and update products, many workgroups
- * per matrix).  n_min = 0 (the default) sends only the problems that do not
- * fit LDS to block Jacobi.  Applies to plans created afterwards. */
+ * parallel workgroup; a larger one with min(M, N) < n_min that fits one
+ * workgroup's LDS by one workgroup in LDS; the rest (min(M, N) <= 8192) by
+ * block Jacobi (32-column panels, FP64 MFMA Gram and update products, many
+ * workgroups per matrix).  n_min = 0 restores the default, 65 (block Jacobi
+ * for everything beyond the small kernel).  Applies to plans created
+ * afterwards. */
 int tn_svd_set_block_min(int64_t n_min);
 
 /* Warm start for a nearby matrix (M >= N): V0T is an N x N matrix with

@@ -409,11 +409,12 @@ def test_svd_warm_start_nearby_matrix(tn, ora, n):
 
 
 @pytest.mark.parametrize("M,N", [(100, 80), (80, 100), (96, 96), (70, 65), (130, 67)])
-def test_svd_block_jacobi_forced(tn, ora, M, N):
-    """Matrices that fit one workgroup's LDS, sent down the block Jacobi path
-    (the path of every larger matrix): LAPACK's singular values and vectors."""
-    _say("svd block %dx%d" % (M, N))
-    tn.call("tn_svd_set_block_min", 65)
+def test_svd_one_workgroup_lds_path(tn, ora, M, N):
+    """Matrices beyond the small kernel that fit one workgroup's LDS, kept on
+    the one-workgroup LDS kernel instead of block Jacobi: LAPACK's singular
+    values and vectors."""
+    _say("svd lds %dx%d" % (M, N))
+    tn.call("tn_svd_set_block_min", 1 << 20)
     try:
         _svd_check(tn, ora, _rand(M * 1000 + N + 7, (M, N)))
     finally:
@@ -503,8 +504,12 @@ def test_svd_plan_replay_and_graph(tn, ora):
     dS = tn.DeviceArray((int(sum(ks)),))
     dV = tn.DeviceArray((int(sum(k * s[1] for s, k in zip(shapes, ks))),))
     h = ctypes.c_void_p()
-    tn.call("tn_svd_plan_create", ctypes.byref(h), len(shapes), tn._p(m), tn._p(n), tn._p(offs), tn._p(v0_off),
-            tn._p(u_off), tn._p(s_off), tn._p(vt_off))
+    tn.call("tn_svd_set_block_min", 128)   # 100 x 90 on the LDS kernel, 160 x 150 and up on block Jacobi
+    try:
+        tn.call("tn_svd_plan_create", ctypes.byref(h), len(shapes), tn._p(m), tn._p(n), tn._p(offs),
+                tn._p(v0_off), tn._p(u_off), tn._p(s_off), tn._p(vt_off))
+    finally:
+        tn.call("tn_svd_set_block_min", 0)
     plan = ctypes.c_void_p(h.value)
     stream = tn.stream_create()
 

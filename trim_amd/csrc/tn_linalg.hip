@@ -756,9 +756,10 @@ SvdKern pick_svd_kernel(int tp) {
 
 void raise_lds_limit(int tp, size_t bytes);
 
-// Problems with min(M, N) >= this go to the block path even when they would
-// fit one workgroup's LDS (0: only those that do not fit).
-int64_t g_block_min = 0;
+// Problems beyond the small kernel with min(M, N) >= this go to the block
+// path; smaller ones that fit one workgroup's LDS to the LDS kernel.
+constexpr int64_t kBlockMinDefault = kSmallN + 1;
+int64_t g_block_min = kBlockMinDefault;
 
 int svd_plan_build(tn_svd_plan_s** out, int64_t batch, const int64_t* m, const int64_t* n,
                    const int64_t* a_off, const int64_t* v0_off, const int64_t* u_off,
@@ -780,7 +781,7 @@ int svd_plan_build(tn_svd_plan_s** out, int64_t batch, const int64_t* m, const i
     if (nn <= kSmallN && mm <= kSmallTP * 16) {
       small_probs.push_back(p);
       P->small_mm = std::max(P->small_mm, mm);
-    } else if (bytes <= kLdsBytes && (g_block_min <= 0 || nn < g_block_min)) {
+    } else if (bytes <= kLdsBytes && nn < g_block_min) {
       P->lds_need = std::max(P->lds_need, bytes);
       lds_probs.push_back(p);
     } else if (mm > 0 && nn > 0) {
@@ -884,7 +885,7 @@ extern "C" {
 
 int tn_svd_set_block_min(int64_t n) {
   TN_REQUIRE(n >= 0, TN_EINVAL, "tn_svd_set_block_min: negative size");
-  g_block_min = n;
+  g_block_min = (n == 0) ? kBlockMinDefault : n;
   return TN_OK;
 }
 

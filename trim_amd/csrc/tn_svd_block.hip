@@ -45,9 +45,11 @@ constexpr int kB = 16;            // columns per block
 constexpr int kPW = 2 * kB;       // panel width (two blocks)
 constexpr int kThreads = 256;
 constexpr int kMaxSweeps = 40;
-constexpr int kMaxSweepsCaptured = 16;
-constexpr int kInnerSweeps = 10;
+constexpr int kMaxSweepsCaptured = kMaxSweeps;
+constexpr int kInnerSweeps = 2;
 constexpr int kSetupBlocks = 64;
+constexpr int kNormBlocks = 16;
+constexpr int kMaxN = 8192;       // the sort keeps n norms in LDS (64 KiB)
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -80,6 +82,53 @@ __device__ inline void rr_pair(int r, int k, int np, int& p, int& q) {
 __device__ inline bool needs_rot(double gpq, double gpp, double gqq, double tol, double abs_tol) {
   const double a = fabs(gpq);
   return a > tol * sqrt(fmax(gpp, 0.0) * fmax(gqq, 0.0)) && a > abs_tol;
+}
+
+__device__ inline double fast_rcp(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return y;
+}
+
+__device__ inline double fast_rsqrt(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * fma(-0.5 * x * y, y, 1.5);
+  y = y * fma(-0.5 * x * y, y, 1.5);
+  return y;
+}
+
+// Pair k of step r of the round-robin over the 32 panel columns (p < q) and
+// its Jacobi rotation from G (c = 1, s = t = 0 when the pair is orthogonal
+// enough): J^T G J zeroes G[p][q] with J = [c s; -s c] on rows/columns p, q.
+__device__ inline void rotation(const double (*G)[kPW + 1], int r, int k, double tol, double abs_tol, int& p,
+                                int& q, double& c, double& s, double& t) {
+  constexpr int L = kPW - 1;
+  if (k == 0) {
+    p = r;
+    q = L;
+  } else {
+    p = r + k;
+    p -= (p >= L) ? L : 0;
+    q = r - k;
+    q += (q < 0) ? L : 0;
+    if (p > q) {
+      const int x = p;
+      p = q;
+      q = x;
+    }
+  }
+  const double app = G[p][p], aqq = G[q][q], apq = G[p][q];
+  c = 1.0;
+  s = 0.0;
+  t = 0.0;
+  if (needs_rot(apq, app, aqq, tol, abs_tol)) {
+    const double zeta = (aqq - app) * (0.5 * fast_rcp(apq));
+    const double z2 = fma(zeta, zeta, 1.0);
+    t = copysign(1.0, zeta) * fast_rcp(fabs(zeta) + z2 * fast_rsqrt(z2));
+    c = fast_rsqrt(fma(t, t, 1.0));
+    s = c * t;
+  }
 }
 
 __device__ inline double block_sum(double v, double* red) {
@@ -130,49 +179,67 @@ __global__ void __launch_bounds__(kThreads) bj_setup(const Prob* __restrict__ pr
   }
 }
 
-// ||W||_F^2 (= ||A||_F^2 up to rounding), fixed-order reduction.
-__global__ void __launch_bounds__(kThreads) bj_norm(const Prob* __restrict__ probs, double* __restrict__ scr) {
-  __shared__ double red[kThreads];
-  const Prob pr = probs[blockIdx.x];
-  const double* W = scr + pr.w_off;
-  const int64_t tot = (int64_t)pr.mp * pr.np;
-  double s = 0.0;
-  for (int64_t e = threadIdx.x; e < tot; e += kThreads) s += W[e] * W[e];
-  s = block_sum(s, red);
-  if (threadIdx.x == 0) scr[pr.sig_off + pr.n] = s;
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
 }
 
-// Start of a sweep: the columns ordered by decreasing norm (ties by index;
-// padding columns last).  Block b of the sweep is the columns at positions
-// [16b, 16b + 16) of this order, so a block holds columns of similar norm.
-__global__ void __launch_bounds__(kThreads) bj_sort(const Prob* __restrict__ probs, double* __restrict__ scr,
-                                                    int* __restrict__ iscr, int sweep) {
-  const Prob pr = probs[blockIdx.x];
-  if (sweep > 0 && iscr[pr.flag_off + sweep - 1] == 0) return;
+// Exact column norms of W (squared), one wavefront per column, the rows
+// strided over the lanes (coalesced) and summed by a fixed butterfly.
+__global__ void __launch_bounds__(kThreads) bj_colnorm(const Prob* __restrict__ probs, double* __restrict__ scr) {
+  const Prob pr = probs[blockIdx.y];
   const double* W = scr + pr.w_off;
   double* nrm = scr + pr.sig_off;
-  int* pos = iscr + pr.pos_off;
-  const int tid = threadIdx.x;
-  for (int j = tid; j < pr.n; j += kThreads) {
+  const int lane = threadIdx.x & 63;
+  const int gw = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  const int nw = gridDim.x * (kThreads / 64);
+  for (int j = gw; j < pr.n; j += nw) {
     const double* wj = W + (int64_t)j * pr.mp;
     double s = 0.0;
-    for (int i = 0; i < pr.m; ++i) s += wj[i] * wj[i];
-    nrm[j] = s;
+    for (int i = lane; i < pr.m; i += 64) s += wj[i] * wj[i];
+    s = wave_sum(s);
+    if (lane == 0) nrm[j] = s;
   }
+}
+
+// Rank of every column by decreasing norm (ties by index), from the norms
+// in LDS: pos[rank] = column (sort) or rank[column] = rank with S written
+// (final).  n doubles of dynamic LDS.
+__device__ inline int norm_rank(const double* sh, int n, int j) {
+  const double sj = sh[j];
+  int rk = 0;
+  for (int i = 0; i < n; ++i) {
+    const double si = sh[i];
+    rk += (si > sj) || (si == sj && i < j);
+  }
+  return rk;
+}
+
+// Start of a sweep: the columns ordered by decreasing norm (padding columns
+// last).  Block b of the sweep is the columns at positions [16b, 16b + 16)
+// of this order, so a block holds columns of similar norm.  The norms are
+// the exact ones (first sweep) or the diagonals of the panels' rotated Gram
+// matrices left by the last round.  The first sweep also records
+// ||A||_F^2 = sum of the column norms (fixed order).
+__global__ void __launch_bounds__(kThreads) bj_sort(const Prob* __restrict__ probs, double* __restrict__ scr,
+                                                    int* __restrict__ iscr, int sweep) {
+  extern __shared__ double sh[];
+  __shared__ double red[kThreads];
+  const Prob pr = probs[blockIdx.x];
+  if (sweep > 0 && iscr[pr.flag_off + sweep - 1] == 0) return;
+  const double* nrm = scr + pr.sig_off;
+  int* pos = iscr + pr.pos_off;
+  const int tid = threadIdx.x;
+  for (int j = tid; j < pr.n; j += kThreads) sh[j] = nrm[j];
   __syncthreads();
-  for (int j = tid; j < pr.np; j += kThreads) {
-    if (j >= pr.n) {
-      pos[j] = j;
-      continue;
-    }
-    const double sj = nrm[j];
-    int rk = 0;
-    for (int i = 0; i < pr.n; ++i) {
-      const double si = nrm[i];
-      rk += (si > sj) || (si == sj && i < j);
-    }
-    pos[rk] = j;
+  if (sweep == 0) {
+    double s = 0.0;
+    for (int j = tid; j < pr.n; j += kThreads) s += sh[j];
+    s = block_sum(s, red);
+    if (tid == 0) scr[pr.sig_off + pr.n] = s;
   }
+  for (int j = tid; j < pr.np; j += kThreads) pos[j < pr.n ? norm_rank(sh, pr.n, j) : j] = j;
 }
 
 // One round of a sweep: workgroup = (problem, block pair).
@@ -180,10 +247,8 @@ __global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ pr
                                                      double* __restrict__ scr, int* __restrict__ iscr, int sweep,
                                                      int round) {
   __shared__ double part[4][3][256];
-  __shared__ double G[kPW][kPW + 1];
+  __shared__ double G[2][kPW][kPW + 1];   // double-buffered for the inner sweeps
   __shared__ double Qm[kPW][kPW + 1];
-  __shared__ double r_c[kB], r_s[kB], r_t[kB];
-  __shared__ int r_p[kB], r_q[kB];
 
   const int2 wm = wmap[blockIdx.x];
   const Prob pr = probs[wm.x];
@@ -209,14 +274,26 @@ __global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ pr
     const double* xq = W + myQ * mp + lg;
     d4 aPP = {0.0, 0.0, 0.0, 0.0}, aPQ = aPP, aQQ = aPP;
     const int per = (int)(mp >> 4);   // 4-row steps per wave (mp is a multiple of 16)
-    int s = w * per;
-    const int e = s + per;
-    for (; s + 4 <= e; s += 4) {
-      double p[4], q[4];
+    const int s0 = w * per, ng = per >> 2;
+    // groups of four steps, the next group's loads issued before this
+    // group's MFMAs
+    double p[4], q[4];
+    if (ng > 0) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        p[u] = xp[4 * (s + u)];
-        q[u] = xq[4 * (s + u)];
+        p[u] = xp[4 * (s0 + u)];
+        q[u] = xq[4 * (s0 + u)];
+      }
+    }
+    for (int g = 0; g < ng; ++g) {
+      double pn[4], qn[4];
+      const int sn = s0 + 4 * (g + 1);
+      if (g + 1 < ng) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          pn[u] = xp[4 * (sn + u)];
+          qn[u] = xq[4 * (sn + u)];
+        }
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -224,12 +301,19 @@ __global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ pr
         aPQ = __builtin_amdgcn_mfma_f64_16x16x4f64(p[u], q[u], aPQ, 0, 0, 0);
         aQQ = __builtin_amdgcn_mfma_f64_16x16x4f64(q[u], q[u], aQQ, 0, 0, 0);
       }
+      if (g + 1 < ng) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          p[u] = pn[u];
+          q[u] = qn[u];
+        }
+      }
     }
-    for (; s < e; ++s) {
-      const double p = xp[4 * s], q = xq[4 * s];
-      aPP = __builtin_amdgcn_mfma_f64_16x16x4f64(p, p, aPP, 0, 0, 0);
-      aPQ = __builtin_amdgcn_mfma_f64_16x16x4f64(p, q, aPQ, 0, 0, 0);
-      aQQ = __builtin_amdgcn_mfma_f64_16x16x4f64(q, q, aQQ, 0, 0, 0);
+    for (int s = s0 + 4 * ng; s < s0 + per; ++s) {
+      const double pv = xp[4 * s], qv = xq[4 * s];
+      aPP = __builtin_amdgcn_mfma_f64_16x16x4f64(pv, pv, aPP, 0, 0, 0);
+      aPQ = __builtin_amdgcn_mfma_f64_16x16x4f64(pv, qv, aPQ, 0, 0, 0);
+      aQQ = __builtin_amdgcn_mfma_f64_16x16x4f64(qv, qv, aQQ, 0, 0, 0);
     }
     // accumulator element v of lane (lr, lg) is tile entry (lg + 4v, lr)
 #pragma unroll
@@ -264,7 +348,7 @@ __global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ pr
       b = i - kB;
     }
     const int idx = a * 16 + b;
-    G[i][j] = ((part[0][t][idx] + part[1][t][idx]) + part[2][t][idx]) + part[3][t][idx];
+    G[0][i][j] = ((part[0][t][idx] + part[1][t][idx]) + part[2][t][idx]) + part[3][t][idx];
   }
   __syncthreads();
 
@@ -273,85 +357,77 @@ __global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ pr
   int need = 0;
   for (int e = tid; e < kPW * kPW; e += kThreads) {
     const int i = e / kPW, j = e % kPW;
-    if (i < j && needs_rot(G[i][j], G[i][i], G[j][j], tol, abs_tol)) need = 1;
+    if (i < j && needs_rot(G[0][i][j], G[0][i][i], G[0][j][j], tol, abs_tol)) need = 1;
   }
-  if (!__syncthreads_or(need)) return;
+  // the panel's column norms (diagonal of G; of the rotated G below) are
+  // what the next sweep orders the columns by
+  double* nrm = scr + pr.sig_off;
+  const int64_t dcol = (tid < kB) ? posP[tid] : (tid < kPW ? posQ[tid - kB] : 0);
+  if (!__syncthreads_or(need)) {
+    if (tid < kPW && dcol < pr.n) nrm[dcol] = G[0][tid][tid];
+    return;
+  }
 
-  // ---- cyclic Jacobi on G (16 disjoint rotations per step), Q accumulated
+  // ---- cyclic Jacobi on G (16 disjoint rotations per step), Q accumulated.
+  // Thread (a, b) rotates the 2 x 2 block {p_a, q_a} x {p_b, q_b} of G from
+  // one buffer into the other (a <= b, with its transpose; every entry is
+  // rewritten each step) and rows 2b, 2b + 1 of Q's columns p_a, q_a in
+  // place.  It computes the two rotations it needs itself, so a step ends in
+  // a single barrier.  Two sweeps: the panel's columns need not be fully
+  // orthogonalised here, the next outer sweep recomputes G from W.
   for (int e = tid; e < kPW * kPW; e += kThreads) Qm[e / kPW][e % kPW] = (e / kPW == e % kPW) ? 1.0 : 0.0;
   __syncthreads();
-  int any_rot = 0;
+  const int ba = tid >> 4, bb = tid & 15;
+  int any_rot = 0, cur = 0;
   for (int isw = 0; isw < kInnerSweeps; ++isw) {
     int rot = 0;
     for (int r = 0; r < kPW - 1; ++r) {
-      if (tid < kB) {
-        int p, q;
-        rr_pair(r, tid, kPW, p, q);
-        const double app = G[p][p], aqq = G[q][q], apq = G[p][q];
-        double c = 1.0, s = 0.0, t = 0.0;
-        if (needs_rot(apq, app, aqq, tol, abs_tol)) {
-          const double zeta = (aqq - app) / (2.0 * apq);
-          t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-          c = 1.0 / sqrt(1.0 + t * t);
-          s = c * t;
-          rot = 1;
-        }
-        r_c[tid] = c;
-        r_s[tid] = s;
-        r_t[tid] = t;
-        r_p[tid] = p;
-        r_q[tid] = q;
+      const double(*Gc)[kPW + 1] = G[cur];
+      double(*Gn)[kPW + 1] = G[cur ^ 1];
+      int p1, q1, p2, q2;
+      double c1, s1, t1, c2, s2, t2;
+      rotation(Gc, r, ba, tol, abs_tol, p1, q1, c1, s1, t1);
+      rotation(Gc, r, bb, tol, abs_tol, p2, q2, c2, s2, t2);
+      rot |= (s1 != 0.0);
+      if (ba < bb) {
+        const double x = Gc[p1][p2], y = Gc[p1][q2], z = Gc[q1][p2], u = Gc[q1][q2];
+        const double rp0 = c1 * x - s1 * z, rp1 = c1 * y - s1 * u;
+        const double rq0 = s1 * x + c1 * z, rq1 = s1 * y + c1 * u;
+        const double npp = c2 * rp0 - s2 * rp1, npq = s2 * rp0 + c2 * rp1;
+        const double nqp = c2 * rq0 - s2 * rq1, nqq = s2 * rq0 + c2 * rq1;
+        Gn[p1][p2] = npp;
+        Gn[p2][p1] = npp;
+        Gn[p1][q2] = npq;
+        Gn[q2][p1] = npq;
+        Gn[q1][p2] = nqp;
+        Gn[p2][q1] = nqp;
+        Gn[q1][q2] = nqq;
+        Gn[q2][q1] = nqq;
+      } else if (ba == bb) {
+        const double app = Gc[p1][p1], aqq = Gc[q1][q1], apq = Gc[p1][q1];
+        const bool rt = s1 != 0.0;
+        Gn[p1][p1] = rt ? app - t1 * apq : app;
+        Gn[q1][q1] = rt ? aqq + t1 * apq : aqq;
+        Gn[p1][q1] = rt ? 0.0 : apq;
+        Gn[q1][p1] = rt ? 0.0 : apq;
       }
-      __syncthreads();
-      {
-        // G <- J^T G J, one 2 x 2 block {p_a, q_a} x {p_b, q_b} per thread
-        // (a < b; the thread also writes the transposed block)
-        const int a = tid >> 4, b = tid & 15;
-        if (a < b) {
-          const double c1 = r_c[a], s1 = r_s[a], c2 = r_c[b], s2 = r_s[b];
-          if (s1 != 0.0 || s2 != 0.0) {
-            const int p1 = r_p[a], q1 = r_q[a], p2 = r_p[b], q2 = r_q[b];
-            const double x = G[p1][p2], y = G[p1][q2], z = G[q1][p2], u = G[q1][q2];
-            const double rp0 = c1 * x - s1 * z, rp1 = c1 * y - s1 * u;
-            const double rq0 = s1 * x + c1 * z, rq1 = s1 * y + c1 * u;
-            const double npp = c2 * rp0 - s2 * rp1, npq = s2 * rp0 + c2 * rp1;
-            const double nqp = c2 * rq0 - s2 * rq1, nqq = s2 * rq0 + c2 * rq1;
-            G[p1][p2] = npp;
-            G[p2][p1] = npp;
-            G[p1][q2] = npq;
-            G[q2][p1] = npq;
-            G[q1][p2] = nqp;
-            G[p2][q1] = nqp;
-            G[q1][q2] = nqq;
-            G[q2][q1] = nqq;
-          }
-        } else if (a == b && r_s[a] != 0.0) {
-          const int p = r_p[a], q = r_q[a];
-          const double apq = G[p][q], t = r_t[a];
-          G[p][p] -= t * apq;
-          G[q][q] += t * apq;
-          G[p][q] = 0.0;
-          G[q][p] = 0.0;
+      if (s1 != 0.0) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int i = 2 * bb + h;
+          const double x = Qm[i][p1], y = Qm[i][q1];
+          Qm[i][p1] = c1 * x - s1 * y;
+          Qm[i][q1] = s1 * x + c1 * y;
         }
       }
-      // Q <- Q J: columns p, q of every rotated pair
-      for (int e = tid; e < kB * kPW; e += kThreads) {
-        const int k = e / kPW, i = e % kPW;
-        const double s = r_s[k];
-        if (s != 0.0) {
-          const int p = r_p[k], q = r_q[k];
-          const double c = r_c[k];
-          const double x = Qm[i][p], y = Qm[i][q];
-          Qm[i][p] = c * x - s * y;
-          Qm[i][q] = s * x + c * y;
-        }
-      }
+      cur ^= 1;
       __syncthreads();
     }
     rot = __syncthreads_or(rot);
     any_rot |= rot;
     if (!rot) break;
   }
+  if (tid < kPW && dcol < pr.n) nrm[dcol] = G[cur][tid][tid];
   if (!any_rot) return;
   if (tid == 0) {
     atomicOr(&flags[sweep], 1);
@@ -373,9 +449,8 @@ __global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ pr
     gcol[s] = (col < kB) ? posP[col] : posQ[col - kB];
   }
   const int tW = (int)(mp >> 4), tV = (int)(nvp >> 4);
-  for (int t = w; t < tW + tV; t += 4) {
-    double* base;
-    int64_t ld, r0;
+  // tile t: rows [16t, 16t + 16) of W (t < tW) or of V
+  auto tile = [&](int t, double*& base, int64_t& ld, int64_t& r0) {
     if (t < tW) {
       base = W;
       ld = mp;
@@ -385,9 +460,28 @@ __global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ pr
       ld = nvp;
       r0 = (int64_t)(t - tW) * 16;
     }
-    double a[8];
+  };
+  double a[8];
+  if (w < tW + tV) {
+    double* base;
+    int64_t ld, r0;
+    tile(w, base, ld, r0);
 #pragma unroll
     for (int s = 0; s < 8; ++s) a[s] = base[gcol[s] * ld + r0 + lr];
+  }
+  for (int t = w; t < tW + tV; t += 4) {
+    double* base;
+    int64_t ld, r0;
+    tile(t, base, ld, r0);
+    // the next tile's loads go out before this tile's MFMAs
+    double an[8];
+    if (t + 4 < tW + tV) {
+      double* nb;
+      int64_t nld, nr0;
+      tile(t + 4, nb, nld, nr0);
+#pragma unroll
+      for (int s = 0; s < 8; ++s) an[s] = nb[gcol[s] * nld + nr0 + lr];
+    }
     d4 c0 = {0.0, 0.0, 0.0, 0.0}, c1 = c0;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
@@ -400,35 +494,29 @@ __global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ pr
       base[myP * ld + row] = c0[v];
       base[myQ * ld + row] = c1[v];
     }
+    if (t + 4 < tW + tV) {
+#pragma unroll
+      for (int s = 0; s < 8; ++s) a[s] = an[s];
+    }
   }
 }
 
-// Singular values (exact column norms, descending; ties by column index),
-// their ranks, and the convergence status.
+// Singular values (square roots of the exact column norms, descending; ties
+// by column index), their ranks, and the convergence status.
 __global__ void __launch_bounds__(kThreads) bj_sigma(const Prob* __restrict__ probs, double* __restrict__ scr,
                                                      int* __restrict__ iscr, double* __restrict__ S, int nsweeps,
                                                      int* __restrict__ status) {
+  extern __shared__ double sh[];
   const Prob pr = probs[blockIdx.x];
-  const double* W = scr + pr.w_off;
-  double* sig = scr + pr.sig_off;
+  const double* nrm = scr + pr.sig_off;
   int* rank = iscr + pr.rank_off;
   const int tid = threadIdx.x;
-  for (int j = tid; j < pr.n; j += kThreads) {
-    const double* wj = W + (int64_t)j * pr.mp;
-    double s = 0.0;
-    for (int i = 0; i < pr.m; ++i) s += wj[i] * wj[i];
-    sig[j] = s;
-  }
+  for (int j = tid; j < pr.n; j += kThreads) sh[j] = nrm[j];
   __syncthreads();
   for (int j = tid; j < pr.n; j += kThreads) {
-    const double sj = sig[j];
-    int rk = 0;
-    for (int i = 0; i < pr.n; ++i) {
-      const double si = sig[i];
-      rk += (si > sj) || (si == sj && i < j);
-    }
+    const int rk = norm_rank(sh, pr.n, j);
     rank[j] = rk;
-    S[pr.s_off + rk] = sqrt(sj);
+    S[pr.s_off + rk] = sqrt(sh[j]);
   }
   if (tid == 0) {
     int sw = -1;
@@ -488,6 +576,7 @@ struct Plan {
   int2* d_map = nullptr;
   int64_t pairs = 0;
   int max_nb = 0;
+  size_t lds = 0;                 // dynamic LDS of the sort: the largest n
   double* d_scr = nullptr;
   int* d_iscr = nullptr;
   tn_gemm_plan warm = nullptr;    // W = A V0 of the warm-started problems
@@ -530,7 +619,7 @@ int build(Plan** out, const std::vector<Spec>& specs) {
       tn::set_error("svd: a warm start needs M >= N");
       return TN_EINVAL;
     }
-    if (mp * np >= (int64_t(1) << 31) || np > (1 << 15)) {
+    if (mp * np >= (int64_t(1) << 31) || n > kMaxN) {
       destroy(P);
       tn::set_error("svd: %lld x %lld is too large", (long long)s.M, (long long)s.N);
       return TN_EUNSUPP;
@@ -555,6 +644,7 @@ int build(Plan** out, const std::vector<Spec>& specs) {
     iscr += np;
     for (int k = 0; k < p.nb / 2; ++k) map.push_back(make_int2((int)i, k));
     P->max_nb = std::max(P->max_nb, p.nb);
+    P->lds = std::max(P->lds, sizeof(double) * (size_t)n);
     if (s.v0_off >= 0) {
       // W(i, j) = sum_k A(i, k) V0T(j, k), W column-major with leading dimension mp
       tn_gemm_desc_ex g{};
@@ -606,7 +696,7 @@ int launch(Plan* P, const double* A, const double* V0T, double* U, double* S, do
     const int rc = tn_gemm_exec(P->warm, A, V0T, P->d_scr, st);
     if (rc != TN_OK) return rc;
   }
-  hipLaunchKernelGGL(bj_norm, dim3(np), dim3(kThreads), 0, st, P->d_probs, P->d_scr);
+  hipLaunchKernelGGL(bj_colnorm, dim3(kNormBlocks, np), dim3(kThreads), 0, st, P->d_probs, P->d_scr);
   TN_LAUNCH_CHECK();
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   TN_HIP_CHECK(hipStreamIsCapturing(st, &cap));
@@ -617,7 +707,7 @@ int launch(Plan* P, const double* A, const double* V0T, double* U, double* S, do
   const int max_sw = capturing ? kMaxSweepsCaptured : kMaxSweeps;
   int ran = 0;
   for (int s = 0; s < max_sw; ++s) {
-    hipLaunchKernelGGL(bj_sort, dim3(np), dim3(kThreads), 0, st, P->d_probs, P->d_scr, P->d_iscr, s);
+    hipLaunchKernelGGL(bj_sort, dim3(np), dim3(kThreads), P->lds, st, P->d_probs, P->d_scr, P->d_iscr, s);
     TN_LAUNCH_CHECK();
     for (int r = 0; r < P->max_nb - 1; ++r) {
       hipLaunchKernelGGL(bj_round, dim3((unsigned)P->pairs), dim3(kThreads), 0, st, P->d_probs, P->d_map, P->d_scr,
@@ -632,7 +722,10 @@ int launch(Plan* P, const double* A, const double* V0T, double* U, double* S, do
       if (!any) break;
     }
   }
-  hipLaunchKernelGGL(bj_sigma, dim3(np), dim3(kThreads), 0, st, P->d_probs, P->d_scr, P->d_iscr, S, ran, status);
+  hipLaunchKernelGGL(bj_colnorm, dim3(kNormBlocks, np), dim3(kThreads), 0, st, P->d_probs, P->d_scr);
+  TN_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bj_sigma, dim3(np), dim3(kThreads), P->lds, st, P->d_probs, P->d_scr, P->d_iscr, S, ran,
+                     status);
   TN_LAUNCH_CHECK();
   hipLaunchKernelGGL(bj_vectors, dim3(kSetupBlocks, np), dim3(kThreads), 0, st, P->d_probs, P->d_scr, P->d_iscr, U,
                      VT);
