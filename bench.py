@@ -264,16 +264,37 @@ def c4_batch(u10, chis, steps, label):
             "svd_sweeps_last_update": info["last_sweeps"]}
 
 
-def mixed_precision_energy(u10, steps=20):
-    """Energy per site after `steps` batched steps in each GEMM mode, against
-    FP64 (SVD, truncation and energies FP64 in every mode)."""
+def mixed_precision_energy(u10, checkpoints=(20, 2000)):
+    """Energy per site of four chains after each checkpoint's number of
+    batched steps in each GEMM mode, against the FP64 run (SVD, truncation
+    and energies FP64 in every mode): early in the evolution, where the
+    energy still moves at first order, and near the converged ground state,
+    where it is stationary."""
     chi, delta, seed = [8, 16, 24, 32], [0.5, 1.0, 1.5, 1.0], [21, 22, 23, 24]
-    e = {}
+    e = {c: {} for c in checkpoints}
+    last_move = {}
     for mode, name in ((0, "f64"), (1, "f32"), (2, "bf16x3"), (3, "bf16x6")):
         B = u10.BatchTebd(chi, delta, seed, 0.01, NETDIR, mode)
-        B.step(steps)
-        e[name] = np.array([B.energy(i) for i in range(len(chi))])
-    return {k: float(np.max(np.abs(v - e["f64"]))) for k, v in e.items() if k != "f64"}
+        B.set_graph(True)
+        done = 0
+        for c in checkpoints:
+            if c - done > 100 and c == checkpoints[-1]:
+                B.step(c - 100 - done)
+                e0 = np.array([B.energy(i) for i in range(len(chi))])
+                done = c - 100
+                B.step(c - done)
+                e[c][name] = np.array([B.energy(i) for i in range(len(chi))])
+                last_move[name] = float(np.max(np.abs(e[c][name] - e0)))
+            else:
+                B.step(c - done)
+                e[c][name] = np.array([B.energy(i) for i in range(len(chi))])
+            done = c
+        del B
+    out = {}
+    for c in checkpoints:
+        out["after_%d_steps" % c] = {k: float(np.max(np.abs(v - e[c]["f64"]))) for k, v in e[c].items() if k != "f64"}
+    out["f64_energy_change_over_last_100_steps"] = last_move.get("f64")
+    return out
 
 
 def c5_serve(u10, n_req, rate, classes=(16, 32)):
@@ -420,9 +441,9 @@ def main():
     if not args.no_batch:
         c4["chi_16_32"] = c4_batch(uni10, [16, 32], 10, "16 and 32, 256 chains each")
         progress("batched iTEBD chi 16/32: %.1f steps/s" % c4["chi_16_32"]["steps_per_s"])
-        c4["chi_16_to_256"] = c4_batch(uni10, list(range(16, 257, 16)), 1, "16..256 step 16, 32 chains each")
+        c4["chi_16_to_256"] = c4_batch(uni10, list(range(16, 257, 16)), 3, "16..256 step 16, 32 chains each")
         progress("batched iTEBD chi 16..256: %.3f steps/s" % c4["chi_16_to_256"]["steps_per_s"])
-        c4["energy_err_vs_f64_after_20_steps"] = mixed_precision_energy(uni10)
+        c4["energy_err_vs_f64"] = mixed_precision_energy(uni10)
         progress("mixed-precision energies done")
         c5 = c5_serve(uni10, args.serve_requests, 2000.0)
         progress("serving done")

@@ -1343,3 +1343,195 @@ class ITEBD:
         v = self._expectation(A1, self.BLb)
         v += self._expectation(B1, self.ALa)
         return v / 2.0
+
+
+# ---------------------------------------------------------------------------
+# Spin-1 chains and U(1)-symmetric iTEBD (the batched workloads' symmetric
+# chains).  The reference's iTEBD example is dense; its symmetric tensors
+# (UniTensorReal.cpp:1169-1385) hold one block per total charge, and
+# Block::svd (BlockReal.cpp:226-250) factorises one block.  A U(1) update is
+# the dense update of iTEBD.cpp:148-173 with theta block-diagonal in the
+# charge of the middle bond: every charge block is factorised on its own and
+# keeps a fixed number of singular values (the bond's sector dimension), so
+# every tensor keeps its block shapes from step to step.
+# ---------------------------------------------------------------------------
+def xxz_two_site_spin1(delta: float = 1.0, J: float = 1.0) -> np.ndarray:
+    """Two-site spin-1 XXZ J (Sx Sx + Sy Sy + delta Sz Sz) = J ((S+S- + S-S+)/2
+    + delta Sz Sz), basis |s t> with s, t in (+1, 0, -1), s the slower index."""
+    sz = np.diag([1.0, 0.0, -1.0])
+    sp = np.sqrt(2.0) * np.array([[0.0, 1.0, 0.0], [0.0, 0.0, 1.0], [0.0, 0.0, 0.0]])
+    sm = sp.T
+    return J * (0.5 * (np.kron(sp, sm) + np.kron(sm, sp)) + delta * np.kron(sz, sz))
+
+
+def xxz_two_site_d(delta: float, d: int) -> np.ndarray:
+    if d == 2:
+        return heisenberg_two_site(delta)
+    if d == 3:
+        return xxz_two_site_spin1(delta)
+    raise ValueError("d must be 2 or 3")
+
+
+def phys_charges(d: int) -> List[int]:
+    """2 Sz of the physical states, in basis order: (+1, -1) or (+2, 0, -2)."""
+    return {2: [1, -1], 3: [2, 0, -2]}[d]
+
+
+def _largest_remainder(total: int, w: np.ndarray) -> np.ndarray:
+    """Integers >= 1 summing to total, proportional to w (largest remainder;
+    ties to the lower index)."""
+    n = len(w)
+    x = w / w.sum() * (total - n)
+    base = np.floor(x).astype(np.int64)
+    rem = x - base
+    short = (total - n) - int(base.sum())
+    order = sorted(range(n), key=lambda i: (-rem[i], i))
+    for i in order[:short]:
+        base[i] += 1
+    return base + 1
+
+
+def u1_bond_sectors(chi: int, d: int, qmax: int = 12):
+    """Charges (ascending) and dimensions of the two bonds of a U(1) chain of
+    bond dimension chi: bond 1 (lambda_B, left of site A) holds even charges,
+    bond 0 (lambda_A) the charges q + s reachable through one site, both
+    within [-qmax, qmax].  Dimensions follow a Gaussian of width
+    sigma = 2 (1 + log2(chi) / 4) over the charge, rounded by largest
+    remainder with at least 1 per sector, on the charges where the Gaussian
+    share is at least 1/2; a sector never holds more states than the theta
+    block that produces it (sum over s of the outer bond's sector q - s),
+    excess being moved to the sector with the most room."""
+    phys = phys_charges(d)
+    sigma = 2.0 * (1.0 + math.log2(chi) / 4.0)
+
+    def pick(cands):
+        cands = np.array(sorted(cands))
+        w = np.exp(-cands.astype(np.float64) ** 2 / (2 * sigma * sigma))
+        keep = chi * w / w.sum() >= 0.5
+        keep[np.argmax(w)] = True
+        cands, w = cands[keep], w[keep]
+        if len(cands) > chi:
+            order = np.argsort(-w, kind="stable")[:chi]
+            cands, w = cands[np.sort(order)], w[np.sort(order)]
+        return [int(c) for c in cands], [int(v) for v in _largest_remainder(chi, w)]
+
+    q1, n1 = pick([q for q in range(-qmax, qmax + 1) if q % 2 == 0])
+    q0, n0 = pick({q + s for q in q1 for s in phys if abs(q + s) <= qmax})
+    bonds = [[q0, n0], [q1, n1]]
+    for _ in range(64):
+        changed = False
+        for p in (0, 1):
+            qR, nR = bonds[p]
+            qL, nL = bonds[1 - p]
+            dimL = dict(zip(qL, nL))
+            room = [sum(dimL.get(q - s, 0) for s in phys) for q in qR]
+            for i in range(len(qR)):
+                while nR[i] > room[i]:
+                    nR[i] -= 1
+                    j = max(range(len(qR)), key=lambda k: (room[k] - nR[k], -abs(qR[k]), -qR[k]))
+                    if room[j] - nR[j] <= 0:
+                        raise ValueError("u1_bond_sectors: chi=%d does not fit the charge range" % chi)
+                    nR[j] += 1
+                    changed = True
+        if not changed:
+            break
+    return (bonds[0][0], bonds[0][1]), (bonds[1][0], bonds[1][1])
+
+
+class ITEBDU1:
+    """U(1)-symmetric iTEBD on a spin-1/2 (d = 2) or spin-1 (d = 3) XXZ chain,
+    restated with dense arrays and charge masks (the GPU engine stores the
+    blocks).  Tensor p (Gamma_p lambda_p, as ALa/BLb of iTEBD.cpp) has left
+    bond 1 - p and right bond p, entries only where q_right = q_left + s.
+    The seeded start: Gamma_p entries splitmix64(seed + p) at the dense index
+    (a d + s) chi + m, masked; lambda_p = |splitmix64(seed + 2 + p)| sorted
+    descending within each sector, normalised to unit 2-norm."""
+
+    def __init__(self, chi: int, d: int, delta: float, tau: float, seed: int, netdir: str, qmax: int = 12):
+        self.chi, self.d, self.tau, self.delta, self.netdir = chi, d, tau, delta, netdir
+        self.phys = np.array(phys_charges(d))
+        self.sect = list(u1_bond_sectors(chi, d, qmax))
+        self.q = [np.repeat(np.array(s[0]), s[1]) for s in self.sect]   # charge of every bond index
+        self.H = xxz_two_site_d(delta, d)
+        self.G = exph(-tau, self.H).reshape(d, d, d, d)
+        self.lam = []
+        for p in (0, 1):
+            raw = np.abs(splitmix64_uniform(seed + 2 + p, chi))
+            lam = np.empty(chi)
+            o = 0
+            for n in self.sect[p][1]:
+                lam[o:o + n] = np.sort(raw[o:o + n])[::-1]
+                o += n
+            self.lam.append(lam / np.linalg.norm(lam))
+        self.X = []
+        for p in (0, 1):
+            qL, qR = self.q[1 - p], self.q[p]
+            g = splitmix64_uniform(seed + p, chi * d * chi).reshape(chi, d, chi)
+            mask = (qR[None, None, :] == qL[:, None, None] + self.phys[None, :, None])
+            g = np.where(mask, g, 0.0)
+            self.X.append(np.ascontiguousarray((g * self.lam[p][None, None, :]).transpose(0, 2, 1)))
+        self.trunc_err = 0.0
+
+    def _update(self, p: int, keep_all: bool = False):
+        """iTEBD.cpp:148-173 with theta factorised per charge block of the
+        middle bond p (Block::svd, BlockReal.cpp:226-250); the outer bond is
+        1 - p.  keep_all keeps every singular value (the bond p sectors then
+        take the block sizes)."""
+        d, phys = self.d, self.phys
+        X, Y = self.X[p], self.X[1 - p]
+        qL = self.q[1 - p]
+        cL = len(qL)
+        T1 = np.einsum("ams,mbt->asbt", X, Y)
+        C = np.einsum("asbt,stuv->aubv", T1, self.G)
+        Th = self.lam[1 - p][:, None, None, None] * C
+        Cm = C.reshape(cL * d, cL * d)
+        Thm = Th.reshape(cL * d, cL * d)
+        rowq = (qL[:, None] + phys[None, :]).reshape(-1)
+        colq = (qL[:, None] - phys[None, :]).reshape(-1)
+        charges, dims = self.sect[p]
+        newS, newVT, newq, disc = [], [], [], 0.0
+        for q, k in zip(charges, dims):
+            rows = np.nonzero(rowq == q)[0]
+            cols = np.nonzero(colq == q)[0]
+            blk = Thm[np.ix_(rows, cols)]
+            s, vt = svd(blk)[1:]
+            kk = len(s) if keep_all else k
+            if kk > len(s):
+                raise ValueError("sector %d keeps %d of %d singular values" % (q, kk, len(s)))
+            disc += float(np.sum(s[kk:] ** 2))
+            full_vt = np.zeros((kk, cL * d))
+            full_vt[:, cols] = vt[:kk]
+            newS.append(s[:kk])
+            newVT.append(full_vt)
+            newq.append(np.full(kk, q))
+        S = np.concatenate(newS)
+        VT = np.concatenate(newVT)
+        nrm = float(np.sqrt(np.sum(S * S)))
+        if keep_all:
+            self.sect[p] = [list(charges), [len(s) for s in newS]]
+            self.q[p] = np.concatenate(newq)
+        cR = len(S)
+        self.lam[p] = S / nrm
+        self.X[1 - p] = VT.reshape(cR, cL, d)
+        self.X[p] = np.ascontiguousarray((Cm @ VT.T).reshape(cL, d, cR).transpose(0, 2, 1)) / nrm
+        self.trunc_err = disc
+
+    def step(self):
+        """update(A,B) then update(B,A), as iTEBD.cpp:128-129."""
+        self._update(0)
+        self._update(1)
+
+    def dense(self):
+        """(ALa, BLb, La, Lb) in the dense layout of ITEBD."""
+        return self.X[0], self.X[1], self.lam[0], self.lam[1]
+
+    def energy(self) -> float:
+        """Energy per site of the state, through ITEBD's networks
+        (iTEBD.cpp:189-216) on the dense tensors."""
+        E = ITEBD(1, self.d, self.H, self.tau, 1, self.netdir)
+        c = self.chi
+        b3 = [Bond(BD_IN, c), Bond(BD_IN, c), Bond(BD_OUT, self.d)]
+        E.ALa = UniTensor(b3, "ALa", elem=self.X[0].reshape(-1))
+        E.BLb = UniTensor(b3, "BLb", elem=self.X[1].reshape(-1))
+        E.La, E.Lb = self.lam[0], self.lam[1]
+        return E.energy()

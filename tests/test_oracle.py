@@ -433,3 +433,95 @@ def test_itebd_converges_towards_bethe(ora, golden):
     e = t.energy()
     bethe = 0.25 - math.log(2)
     assert bethe - 1e-9 <= e <= bethe + 2e-3
+
+
+def test_spin1_two_site_closed_form(ora):
+    """Two spin-1 Heisenberg (delta = 1): S1.S2 = (S(S+1) - 4) / 2 over total
+    spin S = 2, 1, 0, i.e. eigenvalues 1 (x5), -1 (x3), -2 (x1); at delta = 0
+    the Sz Sz term drops and the spectrum is symmetric under Sz -> -Sz."""
+    ev = np.linalg.eigvalsh(ora.xxz_two_site_spin1(1.0))
+    assert np.allclose(ev, [-2.0] + [-1.0] * 3 + [1.0] * 5, atol=1e-13)
+    H0 = ora.xxz_two_site_spin1(0.0)
+    flip = np.kron(np.eye(3)[::-1], np.eye(3)[::-1])
+    assert np.allclose(flip @ H0 @ flip, H0)
+    # charge conservation: H only couples |s t> and |s' t'> with s + t = s' + t'
+    q = np.add.outer([2, 0, -2], [2, 0, -2]).reshape(-1)
+    assert np.all(ora.xxz_two_site_spin1(0.7)[q[:, None] != q[None, :]] == 0)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_u1_sectors_fit_their_theta_blocks(ora, d):
+    """Every sector sum equals chi, every sector holds at least one state, and
+    no sector holds more states than the theta block that produces it."""
+    phys = ora.phys_charges(d)
+    for chi in (4, 8, 16, 24, 32, 48, 64, 100, 128, 256, 512, 1024):
+        b = ora.u1_bond_sectors(chi, d)
+        for p in (0, 1):
+            (qR, nR), (qL, nL) = b[p], b[1 - p]
+            assert sum(nR) == chi and min(nR) >= 1 and list(qR) == sorted(qR)
+            dimL = dict(zip(qL, nL))
+            for q, n in zip(qR, nR):
+                assert n <= sum(dimL.get(q - s, 0) for s in phys)
+
+
+@pytest.mark.parametrize("d,chi,delta", [(2, 8, 0.7), (3, 8, 1.3), (2, 16, 1.0)])
+def test_u1_update_matches_dense_without_truncation(ora, golden, d, chi, delta):
+    """Keeping every singular value, the per-charge-block update and a dense
+    update describe the same state: the same starting energy; the singular
+    values of one SVD of the whole theta (the dense update of the reference
+    example through the oracle's networks when every charge theta reaches is
+    a sector of the middle bond, as for d = 2; numpy's SVD of theta with the
+    rows and columns of unkept charges zeroed, as the blocks drop them, for
+    d = 3); and the rebuilt tensors reproduce the gated theta:
+    sum_m X'[a,m,s] Y'[m,b,t] = C[a,s,b,t] / ||S|| on the kept charges."""
+    U = ora.ITEBDU1(chi, d, delta, 0.05, 11, golden)
+    D = ora.ITEBD(chi, d, ora.xxz_two_site_d(delta, d), 0.05, 1, golden)
+    b3 = [ora.Bond(ora.BD_IN, chi), ora.Bond(ora.BD_IN, chi), ora.Bond(ora.BD_OUT, d)]
+    D.ALa = ora.UniTensor(b3, "ALa", elem=U.X[0].reshape(-1))
+    D.BLb = ora.UniTensor(b3, "BLb", elem=U.X[1].reshape(-1))
+    D.La, D.Lb = U.lam[0].copy(), U.lam[1].copy()
+    assert abs(U.energy() - D.energy()) <= 1e-13
+    X, Y = U.X[0].copy(), U.X[1].copy()
+    C = np.einsum("asbt,stuv->aubv", np.einsum("ams,mbt->asbt", X, Y), U.G)
+    kept = np.array(U.sect[0][0])
+    rowq = (U.q[1][:, None] + np.array(U.phys)[None, :]).reshape(-1)
+    colq = (U.q[1][:, None] - np.array(U.phys)[None, :]).reshape(-1)
+    rin, cin = np.isin(rowq, kept), np.isin(colq, kept)
+    theta = (U.lam[1][:, None, None, None] * C).reshape(chi * d, chi * d) * rin[:, None] * cin[None, :]
+    sv = np.linalg.svd(theta, compute_uv=False)
+    D.chi = 10 ** 6
+    U._update(0, keep_all=True)
+    mine = np.sort(U.lam[0])[::-1]
+    nrm = np.linalg.norm(sv)
+    assert len(mine) <= len(sv)
+    assert np.max(np.abs(mine - sv[:len(mine)] / nrm)) <= 1e-13
+    assert np.all(sv[len(mine):] <= 1e-13 * nrm)
+    if rin.all() and cin.all():
+        La = D._update(D.ALa, D.BLb, D.La, D.Lb)[2]
+        assert np.max(np.abs(mine - La[:len(mine)])) <= 1e-13
+        assert np.all(np.abs(La[len(mine):]) <= 1e-13)
+    else:
+        assert d == 3
+    T = np.einsum("ams,mbt->asbt", U.X[0], U.X[1])
+    Ck = C * rin.reshape(chi, d)[:, :, None, None] * cin.reshape(chi, d)[None, None, :, :]
+    assert np.max(np.abs(T * nrm - Ck)) <= 1e-13 * np.max(np.abs(C))
+
+
+def test_u1_truncated_chain_stays_symmetric(ora, golden):
+    """With truncation, the tensors keep their charge structure (no entry
+    where q_right != q_left + s), the lambdas stay normalised and sorted
+    within each sector, and the energy decreases under imaginary time."""
+    U = ora.ITEBDU1(16, 2, 1.0, 0.05, 3, golden)
+    e0 = U.energy()
+    for _ in range(10):
+        U.step()
+    for p in (0, 1):
+        qL, qR = U.q[1 - p], U.q[p]
+        mask = qR[None, :, None] == qL[:, None, None] + np.array(U.phys)[None, None, :]
+        assert np.all(U.X[p][~mask] == 0)
+        assert abs(np.linalg.norm(U.lam[p]) - 1.0) <= 1e-14
+        o = 0
+        for n in U.sect[p][1]:
+            assert np.all(np.diff(U.lam[p][o:o + n]) <= 0)
+            o += n
+    assert U.energy() < e0
