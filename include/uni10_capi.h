@@ -81,13 +81,30 @@ int u10_itebd_lambda(u10_itebd t, int which, double* out, int* n);
 int u10_itebd_trunc_err(u10_itebd t, double* out);
 int u10_itebd_free(u10_itebd t);
 
-/* Batched iTEBD: n independent XXZ chains (bond dimension chi[i] <= 256,
+/* Batched iTEBD: n independent XXZ chains (bond dimension chi[i] <= 2048,
  * anisotropy delta[i], seed[i]) advanced together, each update stage one
  * launch for all chains; gemm_mode is a GemmMode of uni10.hpp (SVD and
  * truncation stay FP64).  set_graph(1) replays steps as a captured graph.
  * Per chain, the update of the example driver (iTEBD.cpp:148-173). */
 int u10_batch_tebd_create(int n, const int* chi, const double* delta, const uint64_t* seed, double tau,
                           int gemm_mode, const char* netdir, u10_batch_tebd* out);
+/* As create, with per-chain physical dimension d[i] (2: spin-1/2 XXZ, 3:
+ * spin-1 XXZ; NULL = all 2) and symmetry sym[i] (0: dense, 1: U(1) with
+ * conserved Sz, block-sparse tensors, per-charge-block SVD and fixed sector
+ * dimensions; NULL = all dense).  No counterpart in the reference, whose
+ * iTEBD example is dense spin-1/2; a U(1) chain is that update run per
+ * Qnum block (Block::svd per block, BlockReal.cpp:226-250). */
+int u10_batch_tebd_create_ex(int n, const int* chi, const double* delta, const uint64_t* seed, const int* d,
+                             const int* sym, double tau, int gemm_mode, const char* netdir, u10_batch_tebd* out);
+/* Charge sectors of bond `bond` (0: lambda_A, 1: lambda_B) of a U(1) chain:
+ * sets *count to the number of sectors (0 for a dense chain).  With charges
+ * or dims non-NULL, *count must hold their capacity on input, and the
+ * sectors' charges (2 Sz, ascending) and dimensions are written. */
+int u10_batch_tebd_sectors(u10_batch_tebd b, int chain, int bond, int* charges, int* dims, int* count);
+/* The sector rule itself (host only, no device work): the charges and
+ * dimensions of bond `bond` of a U(1) chain of bond dimension chi and
+ * physical dimension d, with the same count/capacity convention. */
+int u10_u1_bond_sectors(int chi, int d, int bond, int* charges, int* dims, int* count);
 int u10_batch_tebd_step(u10_batch_tebd b, int nsteps);
 int u10_batch_tebd_set_graph(u10_batch_tebd b, int on);
 /* restart one chain (slot reuse): the seeded start of (delta, seed) */

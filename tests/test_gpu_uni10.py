@@ -340,9 +340,9 @@ BATCH = dict(chi=[4, 8, 12, 40], delta=[0.5, 1.0, 1.5, 1.0], seed=[11, 12, 13, 1
 
 
 def test_batch_tebd_matches_oracle_per_chain(u10, ora, golden):
-    """Every chain of a batch of uneven bond dimensions (small and LDS SVD
-    paths) follows the oracle's single-chain iTEBD: energies and Schmidt
-    spectra to 1e-10 after 6 steps."""
+    """Every chain of a batch of uneven bond dimensions (small-kernel and
+    block-Jacobi SVD paths) follows the oracle's single-chain iTEBD: energies
+    and Schmidt spectra to 1e-10 after 6 steps."""
     tau = 0.01
     B = u10.BatchTebd(BATCH["chi"], BATCH["delta"], BATCH["seed"], tau, golden)
     B.step(6)
@@ -380,3 +380,64 @@ def test_batch_tebd_fp32_mode_error_is_stated(u10, golden):
     b.step(10)
     for i in range(len(BATCH["chi"])):
         assert abs(a.energy(i) - b.energy(i)) <= 1e-6
+
+
+MIXED = dict(chi=[8, 16, 8, 8, 12, 40, 24], delta=[0.7, 1.0, 1.3, 0.5, 1.0, 0.9, 0.4],
+             seed=[31, 32, 33, 34, 35, 36, 37], d=[2, 2, 3, 3, 2, 2, 3], sym=[1, 1, 1, 0, 0, 1, 1])
+
+
+def _mixed(u10, golden, tau, **kw):
+    return u10.BatchTebd(MIXED["chi"], MIXED["delta"], MIXED["seed"], tau, golden, d=MIXED["d"],
+                         sym=MIXED["sym"], **kw)
+
+
+def test_batch_tebd_mixed_kinds_match_oracle(u10, ora, golden):
+    """One batch of dense and U(1) chains, spin 1/2 and spin 1: every chain
+    follows its oracle (the dense iTEBD of the reference example, or the
+    per-charge-block U(1) update): sector tables, energies, Schmidt spectra
+    (sector by sector for U(1)) and discarded weight to 1e-10 after 5 steps."""
+    tau = 0.02
+    B = _mixed(u10, golden, tau)
+    B.step(5)
+    for i, (chi, delta, seed, d, sym) in enumerate(zip(*(MIXED[k] for k in ("chi", "delta", "seed", "d", "sym")))):
+        if sym:
+            O = ora.ITEBDU1(chi, d, delta, tau, seed, golden)
+            for b in (0, 1):
+                q, n = B.sectors(i, b)
+                assert list(q) == list(O.sect[b][0]) and list(n) == list(O.sect[b][1])
+        else:
+            O = ora.ITEBD(chi, d, ora.xxz_two_site_d(delta, d), tau, seed, golden)
+            assert len(B.sectors(i, 0)[0]) == 0
+        for _ in range(5):
+            O.step()
+        la, lb = (O.lam[0], O.lam[1]) if sym else (O.La, O.Lb)
+        e, eo = B.energy(i), O.energy()
+        _say("mixed chain %d chi=%d d=%d %s: %.15f oracle %.15f" % (i, chi, d, "U(1)" if sym else "dense", e, eo))
+        assert abs(e - eo) <= 1e-10
+        assert np.max(np.abs(B.lam(i, 0) - la)) <= 1e-10
+        assert np.max(np.abs(B.lam(i, 1) - lb)) <= 1e-10
+        assert abs(B.trunc_err(i) - O.trunc_err) <= 1e-10
+
+
+def test_batch_tebd_mixed_graph_replay_and_slot_reload(u10, golden):
+    """The mixed batch replayed as a graph gives the same bits as issued
+    step by step; reloading one U(1) slot restarts it exactly (same bits as
+    a fresh engine after the same steps) without touching the others."""
+    a = _mixed(u10, golden, 0.02)
+    b = _mixed(u10, golden, 0.02)
+    b.set_graph(True)
+    for _ in range(2):
+        a.step(2)
+        b.step(2)
+    for i in range(len(MIXED["chi"])):
+        for w in (0, 1):
+            assert a.lam(i, w).tobytes() == b.lam(i, w).tobytes()
+    keep = [b.lam(i, 0).tobytes() for i in range(len(MIXED["chi"]))]
+    b.load(1, MIXED["delta"][1], MIXED["seed"][1])
+    b.step(3)
+    c = _mixed(u10, golden, 0.02)
+    c.set_graph(True)
+    c.step(3)
+    assert b.lam(1, 0).tobytes() == c.lam(1, 0).tobytes()
+    assert b.lam(1, 1).tobytes() == c.lam(1, 1).tobytes()
+    assert b.lam(0, 0).tobytes() != keep[0]   # the others kept evolving

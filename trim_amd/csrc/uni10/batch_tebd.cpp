@@ -1,15 +1,26 @@
 // Batched iTEBD over the kernel C ABI.
 //
-// Per chain (bond dimension c, d = 2, D = c d) and per update, with X the
-// Gamma*lambda tensor being rebuilt (ALa in update(A,B)), Y the other one,
-// LX / LY the lambdas between and outside them (iTEBD.cpp:148-173 with
-// itebd.net, updateA.net):
+// Dense chain (bond dimension c, physical dimension d, D = c d), per update,
+// with X the Gamma*lambda tensor being rebuilt (ALa in update(A,B)), Y the
+// other one, LX / LY the lambdas between and outside them (iTEBD.cpp:148-173
+// with itebd.net, updateA.net):
 //   T1[a,s,b,t]   = sum_m X[a,m,s] Y[m,b,t]                     GEMM  D x c x D
 //   C[a,s',b,t']  = sum_{s,t} T1[a,s,b,t] G[(s,t),(s',t')]      GEMM  c^2 x d^2 x d^2
 //   Theta         = LY[a] C[(a,s'),(b,t')]                      row scaling
 //   Theta         = U S VT                                      SVD (D x D)
 //   LX            = S[:c] / ||S[:c]||,  Y = VT[:c, :]           truncation
 //   X[a,m',s']    = sum_{b,t'} C[a,s',b,t'] VT[m',(b,t')] / ||S[:c]||
+//
+// U(1) chain: the same update on block-sparse tensors.  Theta is
+// block-diagonal in the charge q of the middle bond; block q has rows
+// (s, a with q_a = q - s) and columns (t, b with q_b = q + t).  T1 is built
+// per middle sector (one GEMM per (s, t)) into arrays keyed by (q_a, s + t),
+// where the gate, which conserves s + t, is one GEMM per output (s', t')
+// writing straight into theta's blocks.  Every block is factorised on its
+// own (Block::svd per charge, BlockReal.cpp:226-250) and keeps its sector's
+// number of singular values, so shapes never change; LX is normalised over
+// all kept values of all blocks.
+//
 // Every stage is one launch over all chains: the GEMMs are grouped plans
 // whose index maps read each operand in its stored layout, the SVD is a
 // batched plan warm-started from the bond's previous VT, and the vector
@@ -19,6 +30,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 
 #include "common.h"
 #include "tebd.h"
@@ -26,7 +38,8 @@
 namespace uni10 {
 
 namespace {
-constexpr int kD = 2;
+constexpr int kQmax = 12;       // U(1) charges (2 Sz) of a bond stay within [-12, 12]
+constexpr int kMaxChi = 2048;
 
 tn_index_map imap(std::initializer_list<std::pair<int64_t, int64_t>> lv) {
   tn_index_map m;
@@ -55,17 +68,242 @@ tn_gemm_desc_ex desc(int64_t M, int64_t N, int64_t K, int64_t a_off, int64_t b_o
   d.ldc = N;
   return d;
 }
+
+// 2 Sz of the physical states in basis order
+std::vector<int> phys_charges(int d) { return d == 2 ? std::vector<int>{1, -1} : std::vector<int>{2, 0, -2}; }
+
+// The sums s + t of two physical charges, descending, and for each the
+// (s, t) index pairs with that sum, s ascending.
+void sigma_combos(const std::vector<int>& phys, std::vector<int>& sig, std::vector<std::vector<int>>& cs) {
+  const int d = (int)phys.size();
+  sig.clear();
+  for (int s = 0; s < d; ++s)
+    for (int t = 0; t < d; ++t) sig.push_back(phys[s] + phys[t]);
+  std::sort(sig.rbegin(), sig.rend());
+  sig.erase(std::unique(sig.begin(), sig.end()), sig.end());
+  cs.assign(sig.size(), {});
+  for (size_t k = 0; k < sig.size(); ++k)
+    for (int s = 0; s < d; ++s)
+      for (int t = 0; t < d; ++t)
+        if (phys[s] + phys[t] == sig[k]) cs[k].push_back(s);   // t follows from s
+}
+
+std::vector<int> largest_remainder(int total, const std::vector<double>& w) {
+  const int n = (int)w.size();
+  double W = 0.0;
+  for (double v : w) W += v;
+  std::vector<double> rem(n);
+  std::vector<int64_t> base(n);
+  int64_t sum = 0;
+  for (int i = 0; i < n; ++i) {
+    const double x = w[i] / W * (double)(total - n);
+    base[i] = (int64_t)std::floor(x);
+    rem[i] = x - (double)base[i];
+    sum += base[i];
+  }
+  const int64_t shortfall = (int64_t)(total - n) - sum;
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return rem[a] > rem[b]; });
+  for (int64_t k = 0; k < shortfall && k < n; ++k) base[order[k]] += 1;
+  std::vector<int> out(n);
+  for (int i = 0; i < n; ++i) out[i] = (int)base[i] + 1;
+  return out;
+}
 }  // namespace
 
+int U1Sectors::find(int charge) const {
+  const auto it = std::lower_bound(q.begin(), q.end(), charge);
+  return (it != q.end() && *it == charge) ? (int)(it - q.begin()) : -1;
+}
+
+void u1_bond_sectors(int chi, int d, int qmax, U1Sectors& bond0, U1Sectors& bond1) {
+  const std::vector<int> phys = phys_charges(d);
+  const double sigma = 2.0 * (1.0 + std::log2((double)chi) / 4.0);
+  auto pick = [&](std::vector<int> cands, U1Sectors& out) {
+    std::sort(cands.begin(), cands.end());
+    cands.erase(std::unique(cands.begin(), cands.end()), cands.end());
+    std::vector<double> w(cands.size());
+    double W = 0.0;
+    for (size_t i = 0; i < cands.size(); ++i) {
+      const double c = (double)cands[i];
+      w[i] = std::exp(-(c * c) / (2.0 * sigma * sigma));
+      W += w[i];
+    }
+    const size_t imax = std::max_element(w.begin(), w.end()) - w.begin();
+    std::vector<int> kq;
+    std::vector<double> kw;
+    for (size_t i = 0; i < cands.size(); ++i)
+      if ((double)chi * w[i] / W >= 0.5 || i == imax) {
+        kq.push_back(cands[i]);
+        kw.push_back(w[i]);
+      }
+    if ((int)kq.size() > chi) {
+      std::vector<int> order(kq.size());
+      std::iota(order.begin(), order.end(), 0);
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return kw[a] > kw[b]; });
+      order.resize(chi);
+      std::sort(order.begin(), order.end());
+      std::vector<int> q2;
+      std::vector<double> w2;
+      for (int o : order) {
+        q2.push_back(kq[o]);
+        w2.push_back(kw[o]);
+      }
+      kq = q2;
+      kw = w2;
+    }
+    out.q = kq;
+    out.n = largest_remainder(chi, kw);
+  };
+  std::vector<int> c1, c0;
+  for (int q = -qmax; q <= qmax; ++q)
+    if (q % 2 == 0) c1.push_back(q);
+  pick(c1, bond1);
+  for (int q : bond1.q)
+    for (int s : phys)
+      if (std::abs(q + s) <= qmax) c0.push_back(q + s);
+  pick(c0, bond0);
+  U1Sectors* b[2] = {&bond0, &bond1};
+  for (int it = 0; it < 64; ++it) {
+    bool changed = false;
+    for (int p = 0; p < 2; ++p) {
+      U1Sectors& R = *b[p];
+      const U1Sectors& L = *b[1 - p];
+      std::vector<int> room(R.q.size(), 0);
+      for (size_t i = 0; i < R.q.size(); ++i)
+        for (int s : phys) {
+          const int j = L.find(R.q[i] - s);
+          if (j >= 0) room[i] += L.n[j];
+        }
+      for (size_t i = 0; i < R.q.size(); ++i)
+        while (R.n[i] > room[i]) {
+          R.n[i] -= 1;
+          size_t j = 0;
+          for (size_t k = 1; k < R.q.size(); ++k) {
+            const long sk = room[k] - R.n[k], sj = room[j] - R.n[j];
+            if (sk > sj || (sk == sj && (std::abs(R.q[k]) < std::abs(R.q[j]) ||
+                                         (std::abs(R.q[k]) == std::abs(R.q[j]) && R.q[k] < R.q[j]))))
+              j = k;
+          }
+          if (room[j] - R.n[j] <= 0) detail::fail("u1_bond_sectors: chi does not fit the charge range");
+          R.n[j] += 1;
+          changed = true;
+        }
+    }
+    if (!changed) break;
+  }
+  for (U1Sectors* s : b) {
+    s->off.assign(s->q.size(), 0);
+    for (size_t i = 1; i < s->q.size(); ++i) s->off[i] = s->off[i - 1] + s->n[i - 1];
+  }
+}
+
+void BatchTebd::layout_u1(int i) {
+  U1Chain& u = u1_[i];
+  const int d = d_[i];
+  u.phys = phys_charges(d);
+  u1_bond_sectors(chi_[i], d, kQmax, u.bond[0], u.bond[1]);
+  std::vector<int> sig;
+  std::vector<std::vector<int>> cs;
+  sigma_combos(u.phys, sig, cs);
+  for (int p = 0; p < 2; ++p) {
+    const U1Sectors& L = u.bond[1 - p];
+    const U1Sectors& R = u.bond[p];
+    // tensor p: blocks by left-bond sector
+    int64_t o = 0;
+    const size_t nL = L.q.size(), nR = R.q.size();
+    u.xblk[p].assign(nL, 0);
+    u.xw[p].assign(nL, 0);
+    u.xpart[p].assign(nL, std::vector<int64_t>(d, -1));
+    for (size_t a = 0; a < nL; ++a) {
+      int64_t w = 0;
+      for (int s = 0; s < d; ++s) {
+        const int j = R.find(L.q[a] + u.phys[s]);
+        if (j >= 0) {
+          u.xpart[p][a][s] = w;
+          w += R.n[j];
+        }
+      }
+      u.xblk[p][a] = o;
+      u.xw[p][a] = w;
+      o += (int64_t)L.n[a] * w;
+    }
+    u.xsize[p] = o;
+    // update p: theta blocks by sector of bond p
+    u.D[p].assign(nR, 0);
+    u.sq[p].assign(nR, 0);
+    u.sv[p].assign(nR, 0);
+    u.disc[p].assign(nR, 0);
+    u.part[p].assign(nR, std::vector<int64_t>(2 * d, -1));
+    int64_t sqo = 0, svo = 0, dio = 0;
+    for (size_t j = 0; j < nR; ++j) {
+      const int q = R.q[j];
+      int64_t D = 0, Dc = 0;
+      for (int s = 0; s < d; ++s) {
+        const int a = L.find(q - u.phys[s]);
+        if (a >= 0) {
+          u.part[p][j][s] = D;
+          D += L.n[a];
+        }
+        const int b = L.find(q + u.phys[s]);
+        if (b >= 0) {
+          u.part[p][j][d + s] = Dc;
+          Dc += L.n[b];
+        }
+      }
+      if (D != Dc) detail::fail("BatchTebd: theta block is not square");
+      if (D < R.n[j]) detail::fail("BatchTebd: a sector is larger than its theta block");
+      u.D[p][j] = D;
+      u.sq[p][j] = sqo;
+      u.sv[p][j] = svo;
+      u.disc[p][j] = dio;
+      sqo += D * D;
+      svo += D;
+      dio += D - R.n[j];
+    }
+    u.sqsize[p] = sqo;
+    u.svsize[p] = svo;
+    u.tmpsize[p] = R.total() + dio;
+    // T1 arrays of (outer sector a, sigma): rows (a, b in sector q_a + sigma), combos fastest
+    const size_t ns = sig.size();
+    u.pair[p].assign(nL * ns, -1);
+    int64_t t1o = 0;
+    for (size_t a = 0; a < nL; ++a)
+      for (size_t k = 0; k < ns; ++k) {
+        const int b = L.find(L.q[a] + sig[k]);
+        if (b < 0) continue;
+        u.pair[p][a * ns + k] = t1o;
+        t1o += (int64_t)L.n[a] * L.n[b] * (int64_t)cs[k].size();
+      }
+    u.t1size[p] = t1o;
+  }
+  // the block rows of VT are the blocks of the tensor they become
+  for (int p = 0; p < 2; ++p)
+    for (size_t j = 0; j < u.bond[p].q.size(); ++j)
+      if (u.xw[1 - p][j] != u.D[p][j]) detail::fail("BatchTebd: VT block and tensor block differ");
+}
+
 BatchTebd::BatchTebd(const std::vector<int>& chi, const std::vector<double>& delta,
-                     const std::vector<uint64_t>& seed, double tau, int gemm_mode, const std::string& netdir)
+                     const std::vector<uint64_t>& seed, double tau, int gemm_mode, const std::string& netdir,
+                     const std::vector<int>& d, const std::vector<int>& sym)
     : chi_(chi), delta_(delta), netdir_(netdir) {
   const size_t nc = chi.size();
   if (delta.size() != nc || seed.size() != nc) detail::fail("BatchTebd: chi, delta and seed differ in length");
+  if (!d.empty() && d.size() != nc) detail::fail("BatchTebd: d and chi differ in length");
+  if (!sym.empty() && sym.size() != nc) detail::fail("BatchTebd: sym and chi differ in length");
   if (nc == 0) detail::fail("BatchTebd: no chains");
-  for (int c : chi)
-    if (c < 1 || c > 256) detail::fail("BatchTebd: chi must be in [1, 256]");
+  d_ = d.empty() ? std::vector<int>(nc, 2) : d;
+  sym_ = sym.empty() ? std::vector<int>(nc, 0) : sym;
+  for (size_t i = 0; i < nc; ++i) {
+    if (chi[i] < 1 || chi[i] > kMaxChi) detail::fail("BatchTebd: chi must be in [1, 2048]");
+    if (d_[i] != 2 && d_[i] != 3) detail::fail("BatchTebd: d must be 2 (spin 1/2) or 3 (spin 1)");
+    if (sym_[i] != 0 && sym_[i] != 1) detail::fail("BatchTebd: sym must be 0 (dense) or 1 (U(1))");
+  }
   stream_ = detail::stream();
+  u1_.resize(nc);
+  for (size_t i = 0; i < nc; ++i)
+    if (sym_[i]) layout_u1((int)i);
   // arena layout
   int64_t o = 0;
   auto take = [&](int64_t n) {
@@ -75,26 +313,46 @@ BatchTebd::BatchTebd(const std::vector<int>& chi, const std::vector<double>& del
   };
   off_.resize(nc);
   for (size_t i = 0; i < nc; ++i) {
-    const int64_t c = chi[i], D = c * kD, t = c * c * kD * kD;
+    const int64_t c = chi[i], dd = d_[i], D = c * dd, t = c * c * dd * dd;
     Off& f = off_[i];
-    f.x[0] = take(c * c * kD);
-    f.x[1] = take(c * c * kD);
-    f.l[0] = take(c);
-    f.l[1] = take(c);
-    f.g = take(kD * kD * kD * kD);
-    f.t1 = take(t);
-    f.cb = take(t);
-    f.th = take(t);
-    f.u = take(D * D);
-    f.s = take(D);
-    f.vt[0] = take(D * D);
-    f.vt[1] = take(D * D);
-    f.sc[0] = take(2);
-    f.sc[1] = take(2);
+    if (!sym_[i]) {
+      f.x[0] = take(c * c * dd);
+      f.x[1] = take(c * c * dd);
+      f.l[0] = take(c);
+      f.l[1] = take(c);
+      f.g = take(dd * dd * dd * dd);
+      f.t1[0] = f.t1[1] = take(t);
+      f.cb[0] = f.cb[1] = take(t);
+      f.th[0] = f.th[1] = take(t);
+      f.u[0] = f.u[1] = take(D * D);
+      f.s[0] = f.s[1] = take(D);
+      f.vt[0] = take(D * D);
+      f.vt[1] = take(D * D);
+      f.sc[0] = take(2);
+      f.sc[1] = take(2);
+      f.tmp[0] = f.tmp[1] = -1;
+    } else {
+      const U1Chain& u = u1_[i];
+      f.x[0] = take(u.xsize[0]);
+      f.x[1] = take(u.xsize[1]);
+      f.l[0] = take(c);
+      f.l[1] = take(c);
+      f.g = take(dd * dd * dd * dd);
+      for (int p = 0; p < 2; ++p) {
+        f.t1[p] = take(u.t1size[p]);
+        f.cb[p] = take(u.sqsize[p]);
+        f.th[p] = take(u.sqsize[p]);
+        f.u[p] = take(u.sqsize[p]);
+        f.s[p] = take(u.svsize[p]);
+        f.vt[p] = take(u.sqsize[p]);
+        f.tmp[p] = take(u.tmpsize[p]);
+        f.sc[p] = take(2);
+      }
+    }
   }
   arena_.reset((size_t)o, true);
 
-  // initial state of every chain (as ITebd) and its gate
+  // initial state of every chain and its gate
   tau_ = tau;
   std::vector<double> host((size_t)o, 0.0);
   for (size_t i = 0; i < nc; ++i) {
@@ -107,86 +365,218 @@ BatchTebd::BatchTebd(const std::vector<int>& chi, const std::vector<double>& del
   // plans, one set per update parity p (p = 0: X = ALa, Y = BLb; p = 1: swapped)
   for (int p = 0; p < 2; ++p) {
     std::vector<tn_gemm_desc_ex> d1, d2, d3;
-    std::vector<int64_t> sm, sn, sa, su, ss, svt, sv0;
-    std::vector<tn_vecop> rows, trunc, scale;
+    std::vector<int64_t> svd[7];   // m, n, a, u, s, vt, v0
+    std::vector<tn_vecop> rows, gather, trunc, scale;
     for (size_t i = 0; i < nc; ++i) {
-      const int64_t c = chi[i], D = c * kD, dd = kD * kD;
-      const Off& f = off_[i];
-      const int64_t X = f.x[p], Y = f.x[1 - p], LX = f.l[p], LY = f.l[1 - p];
-      // T1[(a,s),(b,t)] = sum_m X[a,m,s] Y[m,(b,t)]
-      tn_gemm_desc_ex g1 = desc(D, D, c, X, Y, f.t1);
-      g1.a_row = imap({{c, c * kD}, {kD, 1}});
-      g1.a_col = imap({{c, kD}});
-      d1.push_back(g1);
-      // C[a,s',b,t'] = sum_{s,t} T1[a,s,b,t] G[(s,t),(s',t')]; rows (a,b)
-      tn_gemm_desc_ex g2 = desc(c * c, dd, dd, f.t1, f.g, f.cb);
-      g2.a_row = imap({{c, kD * c * kD}, {c, kD}});
-      g2.a_col = imap({{kD, c * kD}, {kD, 1}});
-      g2.c_row = imap({{c, kD * c * kD}, {c, kD}});
-      g2.c_col = imap({{kD, c * kD}, {kD, 1}});
-      d2.push_back(g2);
-      // Theta = LY[a] C[(a,s'),(b,t')]
-      rows.push_back(tn_vecop{TN_VECOP_ROWSCALE, 0, D * D, f.cb, f.th, LY, c * kD * kD});
-      // SVD of Theta (D x D) into U, S, VT[p]
-      sm.push_back(D);
-      sn.push_back(D);
-      sa.push_back(f.th);
-      su.push_back(f.u);
-      ss.push_back(f.s);
-      svt.push_back(f.vt[p]);
-      sv0.push_back(f.vt[p]);
-      // LX = S[:c]/||S[:c]||, sc = {norm, discarded}; Y = VT[:c, :]
-      trunc.push_back(tn_vecop{TN_VECOP_TRUNCATE, 0, D, f.s, LX, f.sc[p], c});
-      trunc.push_back(tn_vecop{TN_VECOP_COPY, 0, c * D, f.vt[p], Y, 0, 0});
-      // X[a,m',s'] = sum_{(b,t')} C[(a,s'),(b,t')] VT[m',(b,t')]
-      tn_gemm_desc_ex g3 = desc(D, c, D, f.cb, f.vt[p], X);
-      g3.b_row = imap({{D, 1}});
-      g3.b_col = imap({{c, D}});
-      g3.c_row = imap({{c, c * kD}, {kD, 1}});
-      g3.c_col = imap({{c, kD}});
-      d3.push_back(g3);
-      // X /= ||S[:c]||
-      scale.push_back(tn_vecop{TN_VECOP_SCALE_INV, 0, c * c * kD, X, X, f.sc[p], 0});
-      if (p == 0)
-        flops_ += 2.0 * (2.0 * D * D * c + 2.0 * c * c * dd * dd + 2.0 * D * c * D);
+      if (sym_[i])
+        plans_u1((int)i, p, d1, d2, d3, rows, gather, trunc, scale, svd);
+      else
+        plans_dense((int)i, p, d1, d2, d3, rows, gather, trunc, scale, svd);
     }
-    const int64_t n = (int64_t)nc;
-    detail::check(tn_gemm_plan_create_ex(&g1_[p], gemm_mode, n, d1.data()), "BatchTebd gemm1");
-    detail::check(tn_gemm_plan_create_ex(&g2_[p], gemm_mode, n, d2.data()), "BatchTebd gemm2");
-    detail::check(tn_gemm_plan_create_ex(&g3_[p], gemm_mode, n, d3.data()), "BatchTebd gemm3");
-    detail::check(tn_svd_plan_create(&svd_cold_[p], n, sm.data(), sn.data(), sa.data(), nullptr, su.data(),
-                                     ss.data(), svt.data()),
+    auto gemm = [&](tn_gemm_plan* g, const std::vector<tn_gemm_desc_ex>& ds, const char* what) {
+      detail::check(tn_gemm_plan_create_ex(g, gemm_mode, (int64_t)ds.size(), ds.data()), what);
+    };
+    gemm(&g1_[p], d1, "BatchTebd gemm1");
+    gemm(&g2_[p], d2, "BatchTebd gemm2");
+    gemm(&g3_[p], d3, "BatchTebd gemm3");
+    const int64_t ns = (int64_t)svd[0].size();
+    detail::check(tn_svd_plan_create(&svd_cold_[p], ns, svd[0].data(), svd[1].data(), svd[2].data(), nullptr,
+                                     svd[3].data(), svd[4].data(), svd[5].data()),
                   "BatchTebd svd");
-    detail::check(tn_svd_plan_create(&svd_warm_[p], n, sm.data(), sn.data(), sa.data(), sv0.data(), su.data(),
-                                     ss.data(), svt.data()),
+    detail::check(tn_svd_plan_create(&svd_warm_[p], ns, svd[0].data(), svd[1].data(), svd[2].data(), svd[6].data(),
+                                     svd[3].data(), svd[4].data(), svd[5].data()),
                   "BatchTebd svd");
-    detail::check(tn_vecops_plan_create(&rows_[p], (int64_t)rows.size(), rows.data()), "BatchTebd rows");
-    detail::check(tn_vecops_plan_create(&trunc_[p], (int64_t)trunc.size(), trunc.data()), "BatchTebd trunc");
-    detail::check(tn_vecops_plan_create(&scale_[p], (int64_t)scale.size(), scale.data()), "BatchTebd scale");
+    auto vec = [&](tn_vecops_plan* v, const std::vector<tn_vecop>& items, const char* what) {
+      detail::check(tn_vecops_plan_create(v, (int64_t)items.size(), items.data()), what);
+    };
+    vec(&rows_[p], rows, "BatchTebd rows");
+    vec(&gather_[p], gather, "BatchTebd gather");
+    vec(&trunc_[p], trunc, "BatchTebd trunc");
+    vec(&scale_[p], scale, "BatchTebd scale");
   }
 }
 
+void BatchTebd::plans_dense(int i, int p, std::vector<tn_gemm_desc_ex>& d1, std::vector<tn_gemm_desc_ex>& d2,
+                            std::vector<tn_gemm_desc_ex>& d3, std::vector<tn_vecop>& rows, std::vector<tn_vecop>&,
+                            std::vector<tn_vecop>& trunc, std::vector<tn_vecop>& scale, std::vector<int64_t>* svd) {
+  const int64_t c = chi_[i], kd = d_[i], D = c * kd, dd = kd * kd;
+  const Off& f = off_[i];
+  const int64_t X = f.x[p], Y = f.x[1 - p], LX = f.l[p], LY = f.l[1 - p];
+  // T1[(a,s),(b,t)] = sum_m X[a,m,s] Y[m,(b,t)]
+  tn_gemm_desc_ex g1 = desc(D, D, c, X, Y, f.t1[p]);
+  g1.a_row = imap({{c, c * kd}, {kd, 1}});
+  g1.a_col = imap({{c, kd}});
+  d1.push_back(g1);
+  // C[a,s',b,t'] = sum_{s,t} T1[a,s,b,t] G[(s,t),(s',t')]; rows (a,b)
+  tn_gemm_desc_ex g2 = desc(c * c, dd, dd, f.t1[p], f.g, f.cb[p]);
+  g2.a_row = imap({{c, kd * c * kd}, {c, kd}});
+  g2.a_col = imap({{kd, c * kd}, {kd, 1}});
+  g2.c_row = imap({{c, kd * c * kd}, {c, kd}});
+  g2.c_col = imap({{kd, c * kd}, {kd, 1}});
+  d2.push_back(g2);
+  // Theta = LY[a] C[(a,s'),(b,t')]
+  rows.push_back(tn_vecop{TN_VECOP_ROWSCALE, 0, D * D, f.cb[p], f.th[p], LY, c * kd * kd});
+  // SVD of Theta (D x D) into U, S, VT[p]
+  const int64_t sv[7] = {D, D, f.th[p], f.u[p], f.s[p], f.vt[p], f.vt[p]};
+  for (int k = 0; k < 7; ++k) svd[k].push_back(sv[k]);
+  // LX = S[:c]/||S[:c]||, sc = {norm, discarded}; Y = VT[:c, :]
+  trunc.push_back(tn_vecop{TN_VECOP_TRUNCATE, 0, D, f.s[p], LX, f.sc[p], c});
+  trunc.push_back(tn_vecop{TN_VECOP_COPY, 0, c * D, f.vt[p], Y, 0, 0});
+  // X[a,m',s'] = sum_{b,t'} C[a,s',b,t'] VT[m',(b,t')]
+  tn_gemm_desc_ex g3 = desc(D, c, D, f.cb[p], f.vt[p], X);
+  g3.b_row = imap({{D, 1}});
+  g3.b_col = imap({{c, D}});
+  g3.c_row = imap({{c, c * kd}, {kd, 1}});
+  g3.c_col = imap({{c, kd}});
+  d3.push_back(g3);
+  // X /= ||S[:c]||
+  scale.push_back(tn_vecop{TN_VECOP_SCALE_INV, 0, c * c * kd, X, X, f.sc[p], 0});
+  if (p == 0) flops_ += 2.0 * (2.0 * D * D * c + 2.0 * c * c * dd * dd + 2.0 * D * c * D);
+}
+
+void BatchTebd::plans_u1(int i, int p, std::vector<tn_gemm_desc_ex>& d1, std::vector<tn_gemm_desc_ex>& d2,
+                         std::vector<tn_gemm_desc_ex>& d3, std::vector<tn_vecop>& rows, std::vector<tn_vecop>& gather,
+                         std::vector<tn_vecop>& trunc, std::vector<tn_vecop>& scale, std::vector<int64_t>* svd) {
+  const U1Chain& u = u1_[i];
+  const Off& f = off_[i];
+  const int d = d_[i];
+  const U1Sectors& L = u.bond[1 - p];
+  const U1Sectors& R = u.bond[p];
+  std::vector<int> sig;
+  std::vector<std::vector<int>> cs;
+  sigma_combos(u.phys, sig, cs);
+  const size_t ns = sig.size();
+  auto sig_index = [&](int sum) { return (size_t)(std::find(sig.begin(), sig.end(), sum) - sig.begin()); };
+  double fl = 0.0;
+  // T1 of middle sector j: rows (s, a), columns (t, b), into the (a, sigma) arrays
+  for (size_t j = 0; j < R.q.size(); ++j) {
+    const int q = R.q[j];
+    for (int s = 0; s < d; ++s) {
+      const int a = L.find(q - u.phys[s]);
+      if (a < 0) continue;
+      for (int t = 0; t < d; ++t) {
+        const int b = L.find(q + u.phys[t]);
+        if (b < 0) continue;
+        const size_t k = sig_index(u.phys[s] + u.phys[t]);
+        const int64_t nc = (int64_t)cs[k].size();
+        const int64_t combo = std::find(cs[k].begin(), cs[k].end(), s) - cs[k].begin();
+        const int64_t M = L.n[a], N = L.n[b], K = R.n[j];
+        tn_gemm_desc_ex g = desc(M, N, K, f.x[p] + u.xblk[p][a] + u.xpart[p][a][s],
+                                 f.x[1 - p] + u.xblk[1 - p][j] + u.xpart[1 - p][j][t],
+                                 f.t1[p] + u.pair[p][a * ns + k] + combo);
+        g.lda = u.xw[p][a];
+        g.ldb = u.xw[1 - p][j];
+        g.c_row = imap({{M, N * nc}});
+        g.c_col = imap({{N, nc}});
+        d1.push_back(g);
+        fl += 2.0 * M * N * K;
+      }
+    }
+  }
+  // gate: C block (s', t') of theta block q_a + s' from the (a, sigma) array
+  for (size_t a = 0; a < L.q.size(); ++a)
+    for (size_t k = 0; k < ns; ++k) {
+      const int64_t po = u.pair[p][a * ns + k];
+      if (po < 0) continue;
+      const int b = L.find(L.q[a] + sig[k]);
+      const int64_t nc = (int64_t)cs[k].size();
+      // input combo c is (s0 + c, t0 - c): G rows in0 + c (d - 1)
+      const int s0 = cs[k][0];
+      int t0 = 0;
+      while (u.phys[s0] + u.phys[t0] != sig[k]) ++t0;
+      const int64_t in0 = (int64_t)s0 * d + t0;
+      for (int so : cs[k]) {
+        int to = 0;
+        while (u.phys[so] + u.phys[to] != sig[k]) ++to;
+        const int j = R.find(L.q[a] + u.phys[so]);
+        if (j < 0) continue;   // the charge lies outside the middle bond's sectors
+        const int64_t D = u.D[p][j];
+        const int64_t M = (int64_t)L.n[a] * L.n[b];
+        tn_gemm_desc_ex g = desc(M, 1, nc, f.t1[p] + po, f.g + in0 * d * d + (int64_t)so * d + to,
+                                 f.cb[p] + u.sq[p][j] + u.part[p][j][so] * D + u.part[p][j][d + to]);
+        g.lda = nc;
+        g.b_row = imap({{nc, (int64_t)(d - 1) * d * d}});
+        g.b_col = imap({{1, 1}});
+        g.c_row = imap({{L.n[a], D}, {L.n[b], 1}});
+        g.c_col = imap({{1, 1}});
+        d2.push_back(g);
+        fl += 2.0 * M * nc;
+      }
+    }
+  int64_t xn = 0;
+  for (size_t j = 0; j < R.q.size(); ++j) {
+    const int q = R.q[j];
+    const int64_t D = u.D[p][j], k = R.n[j], sq = u.sq[p][j];
+    // theta = LY[a] C, row part by row part
+    for (int s = 0; s < d; ++s) {
+      const int a = L.find(q - u.phys[s]);
+      if (a < 0) continue;
+      const int64_t r0 = u.part[p][j][s] * D;
+      rows.push_back(tn_vecop{TN_VECOP_ROWSCALE, 0, (int64_t)L.n[a] * D, f.cb[p] + sq + r0, f.th[p] + sq + r0,
+                              f.l[1 - p] + L.off[a], D});
+    }
+    const int64_t sv[7] = {D, D, f.th[p] + sq, f.u[p] + sq, f.s[p] + u.sv[p][j], f.vt[p] + sq, f.vt[p] + sq};
+    for (int e = 0; e < 7; ++e) svd[e].push_back(sv[e]);
+    // kept singular values to the sector's place, the rest after all kept
+    // ones; kept VT rows are the new block j of the other tensor
+    gather.push_back(tn_vecop{TN_VECOP_COPY, 0, k, f.s[p] + u.sv[p][j], f.tmp[p] + R.off[j], 0, 0});
+    if (D > k)
+      gather.push_back(
+          tn_vecop{TN_VECOP_COPY, 0, D - k, f.s[p] + u.sv[p][j] + k, f.tmp[p] + R.total() + u.disc[p][j], 0, 0});
+    gather.push_back(tn_vecop{TN_VECOP_COPY, 0, k * D, f.vt[p] + sq, f.x[1 - p] + u.xblk[1 - p][j], 0, 0});
+    // X block of (s, a): C rows of part s times VT^T
+    for (int s = 0; s < d; ++s) {
+      const int a = L.find(q - u.phys[s]);
+      if (a < 0) continue;
+      const int64_t M = L.n[a];
+      tn_gemm_desc_ex g = desc(M, k, D, f.cb[p] + sq + u.part[p][j][s] * D, f.vt[p] + sq,
+                               f.x[p] + u.xblk[p][a] + u.xpart[p][a][s]);
+      g.lda = D;
+      g.b_row = imap({{D, 1}});
+      g.b_col = imap({{k, D}});
+      g.ldc = u.xw[p][a];
+      d3.push_back(g);
+      fl += 2.0 * M * k * D;
+      xn += M * k;
+    }
+  }
+  trunc.push_back(tn_vecop{TN_VECOP_TRUNCATE, 0, u.tmpsize[p], f.tmp[p], f.l[p], f.sc[p], R.total()});
+  scale.push_back(tn_vecop{TN_VECOP_SCALE_INV, 0, u.xsize[p], f.x[p], f.x[p], f.sc[p], 0});
+  if (xn != u.xsize[p]) detail::fail("BatchTebd: U(1) update does not rebuild its whole tensor");
+  flops_ += fl;
+}
+
+std::vector<double> BatchTebd::gate(int i, double delta) const {
+  const Matrix H = (d_[i] == 2) ? xxz_two_site(delta, 1.0) : xxz_two_site_spin1(delta, 1.0);
+  return takeExp(-tau_, H).getHostElem();
+}
+
 // Host image of chain i's arena region (base = its x[0]): Gamma lambda
-// tensors and lambdas from the seeded start of ITebd (splitmix seeds
-// seed..seed+3), the gate exp(-tau H(delta)), and identity warm-start bases
-// (so a chain's SVD sequence never depends on an earlier occupant).
+// tensors and lambdas from the seeded start, the gate exp(-tau H(delta)),
+// and identity warm-start bases (so a chain's SVD sequence never depends on
+// an earlier occupant).  A dense chain starts as ITebd (splitmix seeds
+// seed..seed+3); a U(1) chain from the same dense Gammas restricted to its
+// charge structure, with each lambda sorted within its sectors.
 void BatchTebd::init_chain(int i, double delta, uint64_t seed, double* base) {
-  const int c = chi_[i];
+  const int c = chi_[i], kd = d_[i];
   const Off& f = off_[i];
   auto at = [&](int64_t off) { return base + (off - f.x[0]); };
   std::fill(base, base + region_[i], 0.0);
-  const size_t n = (size_t)c * kD * c;
-  std::vector<double> ga(n), gb(n), la(c), lb(c);
-  for (size_t e = 0; e < n; ++e) {
-    ga[e] = detail::splitmix_uniform(seed, e + 1);
-    gb[e] = detail::splitmix_uniform(seed + 1, e + 1);
-  }
+  std::vector<double> la(c), lb(c);
   for (int e = 0; e < c; ++e) {
     la[e] = std::fabs(detail::splitmix_uniform(seed + 2, e + 1));
     lb[e] = std::fabs(detail::splitmix_uniform(seed + 3, e + 1));
   }
-  std::sort(la.rbegin(), la.rend());
-  std::sort(lb.rbegin(), lb.rend());
+  if (!sym_[i]) {
+    std::sort(la.rbegin(), la.rend());
+    std::sort(lb.rbegin(), lb.rend());
+  } else {
+    const U1Chain& u = u1_[i];
+    for (size_t k = 0; k < u.bond[0].q.size(); ++k)
+      std::sort(la.begin() + u.bond[0].off[k], la.begin() + u.bond[0].off[k] + u.bond[0].n[k], std::greater<double>());
+    for (size_t k = 0; k < u.bond[1].q.size(); ++k)
+      std::sort(lb.begin() + u.bond[1].off[k], lb.begin() + u.bond[1].off[k] + u.bond[1].n[k], std::greater<double>());
+  }
   double na = 0, nb = 0;
   for (int e = 0; e < c; ++e) {
     na += la[e] * la[e];
@@ -200,19 +590,47 @@ void BatchTebd::init_chain(int i, double delta, uint64_t seed, double* base) {
     l0[e] = la[e] / na;
     l1[e] = lb[e] / nb;
   }
-  double* x0 = at(f.x[0]);
-  double* x1 = at(f.x[1]);
-  for (int a = 0; a < c; ++a)
-    for (int s = 0; s < kD; ++s)
-      for (int b = 0; b < c; ++b) {
-        x0[((size_t)a * c + b) * kD + s] = ga[((size_t)a * kD + s) * c + b] * l0[b];
-        x1[((size_t)a * c + b) * kD + s] = gb[((size_t)a * kD + s) * c + b] * l1[b];
+  const double* lam[2] = {l0, l1};
+  if (!sym_[i]) {
+    double* x0 = at(f.x[0]);
+    double* x1 = at(f.x[1]);
+    for (int a = 0; a < c; ++a)
+      for (int s = 0; s < kd; ++s)
+        for (int b = 0; b < c; ++b) {
+          const uint64_t e = ((uint64_t)a * kd + s) * c + b + 1;
+          x0[((size_t)a * c + b) * kd + s] = detail::splitmix_uniform(seed, e) * l0[b];
+          x1[((size_t)a * c + b) * kd + s] = detail::splitmix_uniform(seed + 1, e) * l1[b];
+        }
+    const int64_t D = (int64_t)c * kd;
+    for (int p = 0; p < 2; ++p)
+      for (int64_t e = 0; e < D; ++e) at(f.vt[p])[e * D + e] = 1.0;
+  } else {
+    const U1Chain& u = u1_[i];
+    for (int p = 0; p < 2; ++p) {
+      const U1Sectors& L = u.bond[1 - p];
+      const U1Sectors& R = u.bond[p];
+      double* x = at(f.x[p]);
+      for (size_t a = 0; a < L.q.size(); ++a)
+        for (int s = 0; s < kd; ++s) {
+          if (u.xpart[p][a][s] < 0) continue;
+          const int j = R.find(L.q[a] + u.phys[s]);
+          for (int ar = 0; ar < L.n[a]; ++ar)
+            for (int mr = 0; mr < R.n[j]; ++mr) {
+              const int64_t ga = L.off[a] + ar, gm = R.off[j] + mr;
+              const uint64_t e = ((uint64_t)ga * kd + s) * c + gm + 1;
+              x[u.xblk[p][a] + ar * u.xw[p][a] + u.xpart[p][a][s] + mr] =
+                  detail::splitmix_uniform(seed + p, e) * lam[p][gm];
+            }
+        }
+      for (size_t j = 0; j < R.q.size(); ++j) {
+        const int64_t D = u.D[p][j];
+        double* v = at(f.vt[p]) + u.sq[p][j];
+        for (int64_t e = 0; e < D; ++e) v[e * D + e] = 1.0;
       }
-  const std::vector<double> g = takeExp(-tau_, xxz_two_site(delta, 1.0)).getHostElem();
+    }
+  }
+  const std::vector<double> g = gate(i, delta);
   std::copy(g.begin(), g.end(), at(f.g));
-  const int64_t D = (int64_t)c * kD;
-  for (int p = 0; p < 2; ++p)
-    for (int64_t e = 0; e < D; ++e) at(f.vt[p])[e * D + e] = 1.0;
   delta_[i] = delta;
 }
 
@@ -233,6 +651,7 @@ BatchTebd::~BatchTebd() {
     tn_svd_plan_destroy(svd_cold_[p]);
     tn_svd_plan_destroy(svd_warm_[p]);
     tn_vecops_plan_destroy(rows_[p]);
+    tn_vecops_plan_destroy(gather_[p]);
     tn_vecops_plan_destroy(trunc_[p]);
     tn_vecops_plan_destroy(scale_[p]);
   }
@@ -245,6 +664,7 @@ void BatchTebd::update(int p, bool warm) {
   detail::check(tn_vecops_plan_exec(rows_[p], A, stream_), "BatchTebd rows");
   last_svd_ = warm ? svd_warm_[p] : svd_cold_[p];
   detail::check(tn_svd_plan_exec(last_svd_, A, A, A, A, A, stream_), "BatchTebd svd");
+  detail::check(tn_vecops_plan_exec(gather_[p], A, stream_), "BatchTebd gather");
   detail::check(tn_vecops_plan_exec(trunc_[p], A, stream_), "BatchTebd trunc");
   detail::check(tn_gemm_exec(g3_[p], A, A, A, stream_), "BatchTebd gemm3");
   detail::check(tn_vecops_plan_exec(scale_[p], A, stream_), "BatchTebd scale");
@@ -299,18 +719,45 @@ double BatchTebd::trunc_err(int chain) const {
   return h[1];
 }
 
+const U1Sectors* BatchTebd::sectors(int chain, int bond) const {
+  if (chain < 0 || chain >= chains() || bond < 0 || bond > 1) detail::fail("BatchTebd::sectors: bad index");
+  return sym_[chain] ? &u1_[chain].bond[bond] : nullptr;
+}
+
 double BatchTebd::energy(int chain) {
   if (chain < 0 || chain >= chains()) detail::fail("BatchTebd::energy: bad index");
-  const int c = chi_[chain];
+  const int c = chi_[chain], kd = d_[chain];
   const Off& f = off_[chain];
-  std::vector<Bond> b3 = {Bond(BD_IN, c), Bond(BD_IN, c), Bond(BD_OUT, kD)};
+  std::vector<Bond> b3 = {Bond(BD_IN, c), Bond(BD_IN, c), Bond(BD_OUT, kd)};
   UniTensor ALa(b3, "ALa"), BLb(b3, "BLb");
-  ALa.setElem(arena_.get() + f.x[0], true);
-  BLb.setElem(arena_.get() + f.x[1], true);
+  if (!sym_[chain]) {
+    ALa.setElem(arena_.get() + f.x[0], true);
+    BLb.setElem(arena_.get() + f.x[1], true);
+  } else {
+    // the dense (chi, chi, d) layout of the block tensors
+    const U1Chain& u = u1_[chain];
+    for (int p = 0; p < 2; ++p) {
+      std::vector<double> blk((size_t)u.xsize[p]), dense((size_t)c * c * kd, 0.0);
+      detail::check(tn_memcpy_d2h(blk.data(), arena_.get() + f.x[p], sizeof(double) * blk.size(), stream_),
+                    "BatchTebd energy");
+      const U1Sectors& L = u.bond[1 - p];
+      const U1Sectors& R = u.bond[p];
+      for (size_t a = 0; a < L.q.size(); ++a)
+        for (int s = 0; s < kd; ++s) {
+          if (u.xpart[p][a][s] < 0) continue;
+          const int j = R.find(L.q[a] + u.phys[s]);
+          for (int ar = 0; ar < L.n[a]; ++ar)
+            for (int mr = 0; mr < R.n[j]; ++mr)
+              dense[((size_t)(L.off[a] + ar) * c + R.off[j] + mr) * kd + s] =
+                  blk[u.xblk[p][a] + ar * u.xw[p][a] + u.xpart[p][a][s] + mr];
+        }
+      (p == 0 ? ALa : BLb).setRawElem(dense);
+    }
+  }
   Matrix La(c, c, arena_.get() + f.l[0], true, true), Lb(c, c, arena_.get() + f.l[1], true, true);
-  std::vector<Bond> bH = {Bond(BD_IN, kD), Bond(BD_IN, kD), Bond(BD_OUT, kD), Bond(BD_OUT, kD)};
+  std::vector<Bond> bH = {Bond(BD_IN, kd), Bond(BD_IN, kd), Bond(BD_OUT, kd), Bond(BD_OUT, kd)};
   UniTensor H(bH, "H");
-  H.putBlock(xxz_two_site(delta_[chain], 1.0));
+  H.putBlock(kd == 2 ? xxz_two_site(delta_[chain], 1.0) : xxz_two_site_spin1(delta_[chain], 1.0));
   Network MPS(netdir_ + "/MPS.net"), meas(netdir_ + "/measure.net");
   return itebd_energy(ALa, BLb, La, Lb, H, MPS, meas);
 }

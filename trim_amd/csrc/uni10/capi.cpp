@@ -338,6 +338,48 @@ int u10_batch_tebd_create(int n, const int* chi, const double* delta, const uint
   });
 }
 
+int u10_batch_tebd_create_ex(int n, const int* chi, const double* delta, const uint64_t* seed, const int* d,
+                             const int* sym, double tau, int gemm_mode, const char* netdir, u10_batch_tebd* out) {
+  return guard([&] {
+    if (n < 1 || !chi || !delta || !seed || !netdir || !out) detail::fail("u10_batch_tebd_create_ex: bad argument");
+    *out = new u10_batch_tebd_s(std::vector<int>(chi, chi + n), std::vector<double>(delta, delta + n),
+                                std::vector<uint64_t>(seed, seed + n), tau, gemm_mode, std::string(netdir),
+                                d ? std::vector<int>(d, d + n) : std::vector<int>(),
+                                sym ? std::vector<int>(sym, sym + n) : std::vector<int>());
+  });
+}
+
+int u10_batch_tebd_sectors(u10_batch_tebd b, int chain, int bond, int* charges, int* dims, int* count) {
+  return guard([&] {
+    if (!count) detail::fail("u10_batch_tebd_sectors: NULL count");
+    const U1Sectors* s = b->B.sectors(chain, bond);
+    const int n = s ? (int)s->q.size() : 0;
+    if ((charges || dims) && *count < n) detail::fail("u10_batch_tebd_sectors: output arrays too short");
+    for (int i = 0; i < n; ++i) {
+      if (charges) charges[i] = s->q[i];
+      if (dims) dims[i] = s->n[i];
+    }
+    *count = n;
+  });
+}
+
+int u10_u1_bond_sectors(int chi, int d, int bond, int* charges, int* dims, int* count) {
+  return guard([&] {
+    if (!count || chi < 1 || (d != 2 && d != 3) || bond < 0 || bond > 1)
+      detail::fail("u10_u1_bond_sectors: bad argument");
+    U1Sectors b[2];
+    u1_bond_sectors(chi, d, 12, b[0], b[1]);
+    const U1Sectors& s = b[bond];
+    const int n = (int)s.q.size();
+    if ((charges || dims) && *count < n) detail::fail("u10_u1_bond_sectors: output arrays too short");
+    for (int i = 0; i < n; ++i) {
+      if (charges) charges[i] = s.q[i];
+      if (dims) dims[i] = s.n[i];
+    }
+    *count = n;
+  });
+}
+
 int u10_batch_tebd_step(u10_batch_tebd b, int n) {
   return guard([&] { b->B.step(n); });
 }

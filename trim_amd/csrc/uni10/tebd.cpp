@@ -18,6 +18,23 @@ Matrix xxz_two_site(double delta, double J) {
   return Matrix(4, 4, h.data(), false, false);
 }
 
+Matrix xxz_two_site_spin1(double delta, double J) {
+  // S+ = sqrt(2) (|+1><0| + |0><-1|); (S+S- + S-S+)/2 moves one unit of Sz
+  // between the sites with amplitude 1 (sqrt(2) sqrt(2) / 2)
+  std::vector<double> h(81, 0.0);
+  auto idx = [](int s, int t) { return 3 * s + t; };   // state index 0: +1, 1: 0, 2: -1
+  const double sz[3] = {1.0, 0.0, -1.0};
+  for (int s = 0; s < 3; ++s)
+    for (int t = 0; t < 3; ++t) h[idx(s, t) * 9 + idx(s, t)] = J * delta * sz[s] * sz[t];
+  for (int s = 1; s < 3; ++s)      // site 1 raised (s -> s - 1), site 2 lowered (t -> t + 1)
+    for (int t = 0; t < 2; ++t) {
+      const int a = idx(s, t), b = idx(s - 1, t + 1);
+      h[b * 9 + a] = J;
+      h[a * 9 + b] = J;
+    }
+  return Matrix(9, 9, h.data(), false, false);
+}
+
 ITebd::ITebd(int chi, int d, double delta, double tau, uint64_t seed, const std::string& netdir)
     : chi_(chi), d_(d), tau_(tau),
       iTEBD_(netdir + "/itebd.net"), updateA_(netdir + "/updateA.net"), MPS_(netdir + "/MPS.net"),

@@ -8,6 +8,9 @@
 namespace uni10 {
 
 Matrix xxz_two_site(double delta, double J);
+// Two-site spin-1 XXZ J ((S+S- + S-S+)/2 + delta Sz Sz), basis |s t> with
+// s, t in (+1, 0, -1), s the slower index.
+Matrix xxz_two_site_spin1(double delta, double J);
 // Energy per site of the iTEBD state (ALa, BLb, La, Lb) for the two-site
 // Hamiltonian H, through the MPS.net / measure.net networks.
 double itebd_energy(UniTensor& ALa, UniTensor& BLb, const Matrix& La, const Matrix& Lb, UniTensor& H,

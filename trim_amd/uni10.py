@@ -80,6 +80,9 @@ SIGNATURES = {
     "u10_itebd_trunc_err": [P, PD],
     "u10_itebd_free": [P],
     "u10_batch_tebd_create": [I, P, P, P, D, I, S, PP],
+    "u10_batch_tebd_create_ex": [I, P, P, P, P, P, D, I, S, PP],
+    "u10_batch_tebd_sectors": [P, I, I, P, P, PI],
+    "u10_u1_bond_sectors": [I, I, I, P, P, PI],
     "u10_batch_tebd_step": [P, I],
     "u10_batch_tebd_set_graph": [P, I],
     "u10_batch_tebd_load": [P, I, D, ctypes.c_uint64],
@@ -419,19 +422,50 @@ def exph(x: float, H):
     return out
 
 
+def u1_bond_sectors(chi: int, d: int, bond: int):
+    """(charges, dims) the U(1) chains of BatchTebd give bond `bond` (0:
+    lambda_A, 1: lambda_B) at bond dimension chi (host-side rule)."""
+    n = ctypes.c_int(0)
+    _call("u10_u1_bond_sectors", int(chi), int(d), int(bond), None, None, ctypes.byref(n))
+    q = np.empty(n.value, np.int32)
+    m = np.empty(n.value, np.int32)
+    _call("u10_u1_bond_sectors", int(chi), int(d), int(bond), q.ctypes.data_as(P), m.ctypes.data_as(P),
+          ctypes.byref(n))
+    return q, m
+
+
 class BatchTebd:
     """Many independent XXZ chains advanced together: every stage of an
-    update is one launch for all chains (grouped GEMMs, batched SVD)."""
+    update is one launch for all chains (grouped GEMMs, batched SVD).
+    d[i] = 2 (spin 1/2) or 3 (spin 1), sym[i] = 0 (dense) or 1 (U(1));
+    None = all 2 / all dense."""
 
-    def __init__(self, chi, delta, seed, tau: float, netdir: str, gemm_mode: int = 0):
+    def __init__(self, chi, delta, seed, tau: float, netdir: str, gemm_mode: int = 0, d=None, sym=None):
         chi = np.ascontiguousarray(chi, dtype=np.int32)
         delta = np.ascontiguousarray(delta, dtype=np.float64)
         seed = np.ascontiguousarray(seed, dtype=np.uint64)
+        dd = None if d is None else np.ascontiguousarray(d, dtype=np.int32)
+        sy = None if sym is None else np.ascontiguousarray(sym, dtype=np.int32)
         h = ctypes.c_void_p()
-        _call("u10_batch_tebd_create", len(chi), chi.ctypes.data_as(P), delta.ctypes.data_as(P),
-              seed.ctypes.data_as(P), float(tau), int(gemm_mode), netdir.encode(), ctypes.byref(h))
+        _call("u10_batch_tebd_create_ex", len(chi), chi.ctypes.data_as(P), delta.ctypes.data_as(P),
+              seed.ctypes.data_as(P), None if dd is None else dd.ctypes.data_as(P),
+              None if sy is None else sy.ctypes.data_as(P), float(tau), int(gemm_mode), netdir.encode(),
+              ctypes.byref(h))
         self.h = h.value
         self.chi = chi
+        self.d = np.full(len(chi), 2, np.int32) if dd is None else dd
+        self.sym = np.zeros(len(chi), np.int32) if sy is None else sy
+
+    def sectors(self, chain: int, bond: int):
+        """(charges, dims) of a U(1) chain's bond (0: lambda_A, 1: lambda_B);
+        empty arrays for a dense chain."""
+        n = ctypes.c_int(0)
+        _call("u10_batch_tebd_sectors", self.h, int(chain), int(bond), None, None, ctypes.byref(n))
+        q = np.empty(n.value, np.int32)
+        m = np.empty(n.value, np.int32)
+        _call("u10_batch_tebd_sectors", self.h, int(chain), int(bond), q.ctypes.data_as(P),
+              m.ctypes.data_as(P), ctypes.byref(n))
+        return q, m
 
     def step(self, n: int = 1):
         _call("u10_batch_tebd_step", self.h, int(n))
