@@ -243,14 +243,16 @@ def tebd_rate(u10, steps, chi=32):
     return steps / dt, t.energy(), t.trunc_err()
 
 
-def c4_batch(u10, chis, steps, label):
-    """Batched iTEBD: 512 XXZ chains (delta ~ U[0,2]) with the given bond
-    dimensions, every update stage one launch for all chains, steps replayed
-    as a captured graph.  One step = update(A,B) + update(B,A) of every chain."""
+def c4_batch(u10, chis, steps, label, u1_half=False):
+    """Batched iTEBD: 512 spin-1/2 XXZ chains (delta ~ U[0,2]) with bond
+    dimensions cycling through chis, every update stage one launch for all
+    chains, steps replayed as a captured graph.  u1_half: chains 0-255 U(1),
+    256-511 dense.  One step = update(A,B) + update(B,A) of every chain."""
     n = 512
     chi = [int(chis[i % len(chis)]) for i in range(n)]
+    sym = [1 if (u1_half and i < n // 2) else 0 for i in range(n)]
     delta = np.random.default_rng(4).uniform(0.0, 2.0, n)
-    B = u10.BatchTebd(chi, delta, [1000 + 4 * i for i in range(n)], 0.01, NETDIR)
+    B = u10.BatchTebd(chi, delta, [1000 + 4 * i for i in range(n)], 0.01, NETDIR, sym=sym)
     B.set_graph(True)
     B.step(1)   # first SVDs from scratch
     B.step(1)   # capture
@@ -258,10 +260,32 @@ def c4_batch(u10, chis, steps, label):
     B.step(steps)
     dt = time.perf_counter() - t0
     info = B.info()
-    return {"chains": n, "chi": label, "symmetry": "dense", "steps": steps, "ms_per_step": dt / steps * 1e3,
+    return {"chains": n, "chi": label, "symmetry": "chains 0-255 U(1), 256-511 dense" if u1_half else "dense",
+            "steps": steps, "ms_per_step": dt / steps * 1e3,
             "steps_per_s": steps / dt, "chain_updates_per_s": 2.0 * n * steps / dt,
             "gemm_tflops": info["gemm_flops_per_step"] * steps / dt / 1e12,
             "svd_sweeps_last_update": info["last_sweeps"]}
+
+
+def large_chi_steps(u10):
+    """One chain per (chi, d, symmetry) at the top of the serving range: time
+    of the first (cold-SVD) step and of a warm step."""
+    out = {}
+    for chi in (256, 512, 1024):
+        for d in (2, 3):
+            for sym in (0, 1):
+                B = u10.BatchTebd([chi], [1.0], [7], 0.01, NETDIR, d=[d], sym=[sym])
+                t0 = time.perf_counter()
+                B.step(1)
+                t1 = time.perf_counter()
+                B.step(1)
+                t2 = time.perf_counter()
+                out["chi%d_d%d_%s" % (chi, d, "u1" if sym else "dense")] = {
+                    "first_step_ms": (t1 - t0) * 1e3, "warm_step_ms": (t2 - t1) * 1e3,
+                    "svd_sweeps": B.info()["last_sweeps"]}
+                progress("chi=%d d=%d %s: warm step %.1f ms" % (chi, d, "U(1)" if sym else "dense", (t2 - t1) * 1e3))
+                del B
+    return out
 
 
 def mixed_precision_energy(u10, checkpoints=(20, 2000)):
@@ -297,22 +321,35 @@ def mixed_precision_energy(u10, checkpoints=(20, 2000)):
     return out
 
 
-def c5_serve(u10, n_req, rate, classes=(16, 32)):
-    """Poisson stream of TEBD-step and contraction requests served by
-    continuous batching over resident slots, FP64 and FP32-MFMA GEMMs."""
+def c5_serve(u10, n_req, rate, chis=(32, 64, 128), mixed=True):
+    """Poisson stream of TEBD-step and contraction requests (bond dimension
+    log-uniform over chis, 1..50 steps; mixed: d in {2, 3} and 60 % U(1),
+    else dense spin 1/2) served by continuous batching over resident slots,
+    FP64 and FP32-MFMA GEMMs."""
     from trim_amd import serve
+    if mixed:
+        classes = [(c, d, s) for c in chis for d in (2, 3) for s in (0, 1)]
+        gen = dict(d_choices=(2, 3), u1_fraction=0.6)
+    else:
+        classes = [(c, 2, 0) for c in chis]
+        gen = {}
     out, res = {}, {}
     for mode, name in ((0, "f64"), (1, "f32")):
-        srv = serve.Server(list(classes), 8e9, NETDIR, gemm_mode=mode, max_slots=256)
+        srv = serve.Server(classes, 16e9, NETDIR, gemm_mode=mode, max_slots=64 if mixed else 256)
         srv.warmup()
-        reqs = serve.poisson_requests(n_req, rate, seed=5, chi_classes=list(classes))
+        reqs = serve.poisson_requests(n_req, rate, seed=5, chi_classes=list(chis), **gen)
         st = srv.run(reqs, realtime=True)
         res[name] = {r.rid: r.result for r in st["done"]}
         out[name] = {k: st[k] for k in ("requests", "wall_s", "p50_ms", "p99_ms", "tebd_steps_per_s")}
-        out[name]["slots"] = {str(c): srv.slots(c) for c in classes}
+        out[name]["slots"] = {"chi%d_d%d_%s" % (c, d, "u1" if s else "dense"): srv.slots((c, d, s))
+                              for c, d, s in classes}
         del srv
+        progress("serving %s: p99 %.1f ms" % (name, out[name]["p99_ms"]))
     diff = max(float(np.max(np.abs(res["f32"][k] - res["f64"][k]))) for k in res["f64"])
     out["max_result_diff_f32_vs_f64"] = diff
+    out["workload"] = ("Poisson %.0f req/s, %d requests, chi log-uniform over %s, %s, 1..50 steps, half TEBD "
+                       "half contraction" % (rate, n_req, list(chis),
+                                             "d in {2,3}, 60% U(1) (charges -12..12)" if mixed else "dense spin 1/2"))
     return out
 
 
@@ -441,13 +478,21 @@ def main():
     if not args.no_batch:
         c4["chi_16_32"] = c4_batch(uni10, [16, 32], 10, "16 and 32, 256 chains each")
         progress("batched iTEBD chi 16/32: %.1f steps/s" % c4["chi_16_32"]["steps_per_s"])
-        c4["chi_16_to_256"] = c4_batch(uni10, list(range(16, 257, 16)), 3, "16..256 step 16, 32 chains each")
-        progress("batched iTEBD chi 16..256: %.3f steps/s" % c4["chi_16_to_256"]["steps_per_s"])
+        c4["chi_16_to_256_half_u1"] = c4_batch(uni10, list(range(16, 257, 16)), 3,
+                                               "16..256 step 16, 32 chains each", u1_half=True)
+        progress("batched iTEBD chi 16..256 half U(1): %.3f steps/s" % c4["chi_16_to_256_half_u1"]["steps_per_s"])
+        c4["chi_16_to_256_dense"] = c4_batch(uni10, list(range(16, 257, 16)), 3, "16..256 step 16, 32 chains each")
+        progress("batched iTEBD chi 16..256 dense: %.3f steps/s" % c4["chi_16_to_256_dense"]["steps_per_s"])
         c4["energy_err_vs_f64"] = mixed_precision_energy(uni10)
         progress("mixed-precision energies done")
         c5 = c5_serve(uni10, args.serve_requests, 2000.0)
         progress("serving done")
+        c5["spin_half_dense_chi_16_32"] = c5_serve(uni10, args.serve_requests, 2000.0, chis=(16, 32), mixed=False)
+        progress("spin-1/2 dense serving done")
+        c5["large_chi_single_chain_steps"] = large_chi_steps(uni10)
         c5["speedup_f32_serving_vs_f64_dropin"] = c5["f32"]["tebd_steps_per_s"] / tebd_sps
+        c5["spin_half_dense_chi_16_32"]["speedup_f32_serving_vs_f64_dropin"] = \
+            c5["spin_half_dense_chi_16_32"]["f32"]["tebd_steps_per_s"] / tebd_sps
     out = {
         "metric": "sector-GEMM TFLOP/s + permute GB/s; TEBD steps/s, p99 latency vs plain FP64",
         "value": value, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

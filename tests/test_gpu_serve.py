@@ -53,3 +53,33 @@ def test_tebd_request_follows_oracle(serve, ora, golden):
         O.step()
     assert np.max(np.abs(r.result[:8] - O.La)) <= 1e-10
     assert np.max(np.abs(r.result[8:] - O.Lb)) <= 1e-10
+
+
+def test_mixed_classes_same_alone_and_in_mix(serve, ora, golden):
+    """Requests of spin 1/2 and spin 1, dense and U(1) (60 %), TEBD steps and
+    contractions, arriving while others run: each result is the same alone
+    and in the mix, and a U(1) spin-1 TEBD request follows the oracle."""
+    classes = [(c, d, s) for c in (8, 16) for d in (2, 3) for s in (0, 1)]
+    gen = dict(n=32, rate=4000.0, seed=9, chi_classes=[8, 16], steps_max=5, d_choices=(2, 3), u1_fraction=0.6)
+    reqs = serve.poisson_requests(**gen)
+    assert {(r.d, r.sym) for r in reqs} == {(2, 0), (2, 1), (3, 0), (3, 1)}
+    budget = 3 * serve.slot_bytes(16, 3) * len(classes)
+    srv = serve.Server(classes, budget, golden)
+    stats = srv.run(reqs, realtime=False)
+    _say("mixed classes: %d requests, p50 %.2f ms p99 %.2f ms" % (stats["requests"], stats["p50_ms"], stats["p99_ms"]))
+    assert stats["requests"] == len(reqs)
+    mixed = {r.rid: r.result for r in stats["done"]}
+    picks = [next(r for r in reqs if r.kind == k and r.sym == 1 and r.d == d) for k in ("tebd", "contract")
+             for d in (2, 3)]
+    for target in picks:
+        solo = serve.poisson_requests(**gen)[target.rid]
+        out = serve.Server(classes, budget, golden).run([solo], realtime=False)["done"][0].result
+        _say("request %d (%s chi=%d d=%d sym=%d)" % (target.rid, target.kind, target.chi, target.d, target.sym))
+        assert out.tobytes() == mixed[target.rid].tobytes()
+    t = next(r for r in picks if r.kind == "tebd" and r.d == 3)
+    O = ora.ITEBDU1(t.chi, 3, t.delta, 0.01, t.seed, golden)
+    for _ in range(t.steps):
+        O.step()
+    res = mixed[t.rid]
+    assert np.max(np.abs(res[:t.chi] - O.lam[0])) <= 1e-10
+    assert np.max(np.abs(res[t.chi:] - O.lam[1])) <= 1e-10
