@@ -155,6 +155,27 @@ def permute_hbm(tn, steps):
     return 16.0 * n / (ms * 1e-3) / 1e9, ms
 
 
+def copy_rate(tn, steps=10):
+    """Plain device-to-device copy of the same 1 GiB moved by the HBM-shape
+    permute (512 MiB read + 512 MiB written): what the library's copy path
+    reaches on this HBM, the practical ceiling the permute is set against."""
+    n = 4096 * 2 * 2 * 4096
+    a = tn.DeviceArray((n,), zero=True)
+    b = tn.DeviceArray((n,))
+    for _ in range(2):
+        tn.call("tn_memcpy_d2d", tn._p(b), tn._p(a), a.nbytes, None)
+    tn.sync()
+    e0, e1 = tn.Event(), tn.Event()
+    e0.record()
+    for _ in range(steps):
+        tn.call("tn_memcpy_d2d", tn._p(b), tn._p(a), a.nbytes, None)
+    e1.record()
+    ms = e0.elapsed_ms(e1) / steps
+    a.free()
+    b.free()
+    return 16.0 * n / (ms * 1e-3) / 1e9
+
+
 def copy_calibration(tn, mib=256):
     """One plain device-to-device copy of a known size: the reference the
     FETCH_SIZE / WRITE_SIZE counters are scaled by (bytes the counter reports
@@ -453,6 +474,7 @@ def main():
     gemm_tf = kflops / (per["gemm_f64"] * 1e-3) / 1e12
     perm_gbps = 16.0 * n / (per["permute"] * 1e-3) / 1e9
     hbm_gbps, hbm_ms = permute_hbm(tn, 10)
+    copy_gbps = copy_rate(tn)
     if per["gemm_f64"] >= per["permute"]:
         roof = {"bound": "mfma", "achieved": gemm_tf, "peak": F64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": gemm_tf / F64_MFMA_PEAK_TFLOPS, "traffic": None,
@@ -468,7 +490,9 @@ def main():
     progress("kernel parts and HBM permute done")
     if args.headline_only:
         print(json.dumps({"metric": "headline step (counter run)", "ms_per_step": ms, "roofline": roof,
-                          "kernels_ms": per, "copy_calibration_bytes": copy_bytes}), flush=True)
+                          "kernels_ms": per, "copy_calibration_bytes": copy_bytes,
+                          "permute_GBps_hbm_4096x2x2x4096": hbm_gbps, "copy_GBps_same_bytes": copy_gbps}),
+              flush=True)
         return
     tebd_sps, tebd_e, tebd_te = tebd_rate(uni10, args.tebd_steps)
     progress("iTEBD chi=32: %.1f steps/s" % tebd_sps)
@@ -507,6 +531,8 @@ def main():
         "permute_GBps_config": perm_gbps,
         "permute_GBps_hbm_4096x2x2x4096": hbm_gbps,
         "permute_frac_hbm": hbm_gbps / HBM_PEAK_GBPS,
+        "copy_GBps_same_bytes": copy_gbps,
+        "permute_frac_of_copy": hbm_gbps / copy_gbps,
         "tebd_chi32": {"steps_per_s": tebd_sps, "steps": args.tebd_steps, "energy_per_site": tebd_e,
                        "trunc_err_last": tebd_te},
         "c3_u1_chi1024": c3,
