@@ -46,7 +46,7 @@ constexpr int kPW = 2 * kB;       // panel width (two blocks)
 constexpr int kThreads = 256;
 constexpr int kMaxSweeps = 40;
 constexpr int kMaxSweepsCaptured = kMaxSweeps;
-constexpr int kInnerSweeps = 2;
+constexpr int kInnerSweeps = 1;
 constexpr int kSetupBlocks = 64;
 constexpr int kNormBlocks = 16;
 constexpr int kMaxN = 8192;       // the sort keeps n norms in LDS (64 KiB)
@@ -80,8 +80,8 @@ __device__ inline void rr_pair(int r, int k, int np, int& p, int& q) {
 }
 
 __device__ inline bool needs_rot(double gpq, double gpp, double gqq, double tol, double abs_tol) {
-  const double a = fabs(gpq);
-  return a > tol * sqrt(fmax(gpp, 0.0) * fmax(gqq, 0.0)) && a > abs_tol;
+  // |g_pq| > tol sqrt(g_pp g_qq), compared in squares
+  return gpq * gpq > (tol * tol) * (fmax(gpp, 0.0) * fmax(gqq, 0.0)) && fabs(gpq) > abs_tol;
 }
 
 __device__ inline double fast_rcp(double x) {
@@ -98,11 +98,8 @@ __device__ inline double fast_rsqrt(double x) {
   return y;
 }
 
-// Pair k of step r of the round-robin over the 32 panel columns (p < q) and
-// its Jacobi rotation from G (c = 1, s = t = 0 when the pair is orthogonal
-// enough): J^T G J zeroes G[p][q] with J = [c s; -s c] on rows/columns p, q.
-__device__ inline void rotation(const double (*G)[kPW + 1], int r, int k, double tol, double abs_tol, int& p,
-                                int& q, double& c, double& s, double& t) {
+// Pair k of step r of the round-robin over the 32 panel columns (p < q).
+__device__ inline void panel_pair(int r, int k, int& p, int& q) {
   constexpr int L = kPW - 1;
   if (k == 0) {
     p = r;
@@ -118,14 +115,24 @@ __device__ inline void rotation(const double (*G)[kPW + 1], int r, int k, double
       q = x;
     }
   }
-  const double app = G[p][p], aqq = G[q][q], apq = G[p][q];
+}
+
+// Jacobi rotation of the pair from its G entries (c = 1, s = t = 0 when the
+// pair is orthogonal enough): J^T G J zeroes G[p][q] with J = [c s; -s c] on
+// rows/columns p, q.
+__device__ inline void rotation(double app, double aqq, double apq, double tol, double abs_tol, double& c,
+                                double& s, double& t) {
   c = 1.0;
   s = 0.0;
   t = 0.0;
   if (needs_rot(apq, app, aqq, tol, abs_tol)) {
-    const double zeta = (aqq - app) * (0.5 * fast_rcp(apq));
+    // The angle need not be exact (the sweeps converge on what is left), so
+    // zeta and t use the hardware reciprocal estimates as they are; c is
+    // refined to full precision so that c^2 + s^2 = 1 to rounding and every
+    // rotation is orthogonal.
+    const double zeta = (aqq - app) * (0.5 * __builtin_amdgcn_rcp(apq));
     const double z2 = fma(zeta, zeta, 1.0);
-    t = copysign(1.0, zeta) * fast_rcp(fabs(zeta) + z2 * fast_rsqrt(z2));
+    t = copysign(1.0, zeta) * __builtin_amdgcn_rcp(fabs(zeta) + z2 * __builtin_amdgcn_rsq(z2));
     c = fast_rsqrt(fma(t, t, 1.0));
     s = c * t;
   }
@@ -249,6 +256,7 @@ __global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ pr
   __shared__ double part[4][3][256];
   __shared__ double G[2][kPW][kPW + 1];   // double-buffered for the inner sweeps
   __shared__ double Qm[kPW][kPW + 1];
+  __shared__ double r_c[kB], r_s[kB], r_t[kB];   // the step's rotations
 
   const int2 wm = wmap[blockIdx.x];
   const Prob pr = probs[wm.x];
@@ -269,51 +277,48 @@ __global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ pr
 
   // ---- G = X^T X: three 16 x 16 tiles (PP, PQ, QQ), rows split over the waves
   {
-    // lane (lr, lg) of k-step s supplies X[4s + lg][col lr] of each block
-    const double* xp = W + myP * mp + lg;
-    const double* xq = W + myQ * mp + lg;
+    // Rows go in 8-row groups, dealt round-robin to the waves.  In a group
+    // starting at row r, lane (lr, lg) loads rows r + 2lg and r + 2lg + 1 of
+    // panel column lr (one 16-byte load per block) and supplies them to two
+    // MFMA k-steps: the sum over rows of G = X^T X takes them in any order.
+    const double* xp = W + myP * mp + 2 * lg;
+    const double* xq = W + myQ * mp + 2 * lg;
     d4 aPP = {0.0, 0.0, 0.0, 0.0}, aPQ = aPP, aQQ = aPP;
-    const int per = (int)(mp >> 4);   // 4-row steps per wave (mp is a multiple of 16)
-    const int s0 = w * per, ng = per >> 2;
-    // groups of four steps, the next group's loads issued before this
-    // group's MFMAs
-    double p[4], q[4];
-    if (ng > 0) {
+    const int ngr = (int)(mp >> 3);
+    auto ld2 = [](const double* p) { return *reinterpret_cast<const double2*>(p); };
+    auto mm = [&](double p, double q) {
+      aPP = __builtin_amdgcn_mfma_f64_16x16x4f64(p, p, aPP, 0, 0, 0);
+      aPQ = __builtin_amdgcn_mfma_f64_16x16x4f64(p, q, aPQ, 0, 0, 0);
+      aQQ = __builtin_amdgcn_mfma_f64_16x16x4f64(q, q, aQQ, 0, 0, 0);
+    };
+    // two groups per iteration (g, g + 4); the next two are loaded first
+    double2 pv[2], qv[2];
+    int g = w;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        p[u] = xp[4 * (s0 + u)];
-        q[u] = xq[4 * (s0 + u)];
+    for (int h = 0; h < 2; ++h)
+      if (g + 4 * h < ngr) {
+        pv[h] = ld2(xp + 8 * (g + 4 * h));
+        qv[h] = ld2(xq + 8 * (g + 4 * h));
       }
-    }
-    for (int g = 0; g < ng; ++g) {
-      double pn[4], qn[4];
-      const int sn = s0 + 4 * (g + 1);
-      if (g + 1 < ng) {
+    for (; g < ngr; g += 8) {
+      double2 pn[2], qn[2];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          pn[u] = xp[4 * (sn + u)];
-          qn[u] = xq[4 * (sn + u)];
+      for (int h = 0; h < 2; ++h)
+        if (g + 8 + 4 * h < ngr) {
+          pn[h] = ld2(xp + 8 * (g + 8 + 4 * h));
+          qn[h] = ld2(xq + 8 * (g + 8 + 4 * h));
         }
-      }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        aPP = __builtin_amdgcn_mfma_f64_16x16x4f64(p[u], p[u], aPP, 0, 0, 0);
-        aPQ = __builtin_amdgcn_mfma_f64_16x16x4f64(p[u], q[u], aPQ, 0, 0, 0);
-        aQQ = __builtin_amdgcn_mfma_f64_16x16x4f64(q[u], q[u], aQQ, 0, 0, 0);
-      }
-      if (g + 1 < ng) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          p[u] = pn[u];
-          q[u] = qn[u];
+      for (int h = 0; h < 2; ++h)
+        if (g + 4 * h < ngr) {
+          mm(pv[h].x, qv[h].x);
+          mm(pv[h].y, qv[h].y);
         }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        pv[h] = pn[h];
+        qv[h] = qn[h];
       }
-    }
-    for (int s = s0 + 4 * ng; s < s0 + per; ++s) {
-      const double pv = xp[4 * s], qv = xq[4 * s];
-      aPP = __builtin_amdgcn_mfma_f64_16x16x4f64(pv, pv, aPP, 0, 0, 0);
-      aPQ = __builtin_amdgcn_mfma_f64_16x16x4f64(pv, qv, aPQ, 0, 0, 0);
-      aQQ = __builtin_amdgcn_mfma_f64_16x16x4f64(qv, qv, aQQ, 0, 0, 0);
     }
     // accumulator element v of lane (lr, lg) is tile entry (lg + 4v, lr)
 #pragma unroll
@@ -369,12 +374,12 @@ __global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ pr
   }
 
   // ---- cyclic Jacobi on G (16 disjoint rotations per step), Q accumulated.
-  // Thread (a, b) rotates the 2 x 2 block {p_a, q_a} x {p_b, q_b} of G from
-  // one buffer into the other (a <= b, with its transpose; every entry is
-  // rewritten each step) and rows 2b, 2b + 1 of Q's columns p_a, q_a in
-  // place.  It computes the two rotations it needs itself, so a step ends in
-  // a single barrier.  Two sweeps: the panel's columns need not be fully
-  // orthogonalised here, the next outer sweep recomputes G from W.
+  // Threads 0-15 compute the step's rotations from one buffer of G; then
+  // thread (a, b) rotates the 2 x 2 block {p_a, q_a} x {p_b, q_b} into the
+  // other buffer (a <= b, with its transpose; every entry is rewritten each
+  // step) and rows 2b, 2b + 1 of Q's columns p_a, q_a in place.  Two sweeps:
+  // the panel's columns need not be fully orthogonalised here, the next
+  // outer sweep recomputes G from W.
   for (int e = tid; e < kPW * kPW; e += kThreads) Qm[e / kPW][e % kPW] = (e / kPW == e % kPW) ? 1.0 : 0.0;
   __syncthreads();
   const int ba = tid >> 4, bb = tid & 15;
@@ -384,13 +389,33 @@ __global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ pr
     for (int r = 0; r < kPW - 1; ++r) {
       const double(*Gc)[kPW + 1] = G[cur];
       double(*Gn)[kPW + 1] = G[cur ^ 1];
+      // everything this thread reads of G and Q is fixed for the step: it
+      // is loaded before the rotations are known, so only c and s cross the
+      // barrier
       int p1, q1, p2, q2;
-      double c1, s1, t1, c2, s2, t2;
-      rotation(Gc, r, ba, tol, abs_tol, p1, q1, c1, s1, t1);
-      rotation(Gc, r, bb, tol, abs_tol, p2, q2, c2, s2, t2);
-      rot |= (s1 != 0.0);
+      panel_pair(r, ba, p1, q1);
+      panel_pair(r, bb, p2, q2);
+      const double x = Gc[p1][p2], y = Gc[p1][q2], z = Gc[q1][p2], u = Gc[q1][q2];
+      double qx[2], qy[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        qx[h] = Qm[2 * bb + h][p1];
+        qy[h] = Qm[2 * bb + h][q1];
+      }
+      if (tid < kB) {
+        int p, q;
+        panel_pair(r, tid, p, q);
+        double c, sn, t;
+        rotation(Gc[p][p], Gc[q][q], Gc[p][q], tol, abs_tol, c, sn, t);
+        rot |= (sn != 0.0);
+        r_c[tid] = c;
+        r_s[tid] = sn;
+        r_t[tid] = t;
+      }
+      __syncthreads();
+      const double c1 = r_c[ba], s1 = r_s[ba];
       if (ba < bb) {
-        const double x = Gc[p1][p2], y = Gc[p1][q2], z = Gc[q1][p2], u = Gc[q1][q2];
+        const double c2 = r_c[bb], s2 = r_s[bb];
         const double rp0 = c1 * x - s1 * z, rp1 = c1 * y - s1 * u;
         const double rq0 = s1 * x + c1 * z, rq1 = s1 * y + c1 * u;
         const double npp = c2 * rp0 - s2 * rp1, npq = s2 * rp0 + c2 * rp1;
@@ -404,20 +429,19 @@ __global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ pr
         Gn[q1][q2] = nqq;
         Gn[q2][q1] = nqq;
       } else if (ba == bb) {
-        const double app = Gc[p1][p1], aqq = Gc[q1][q1], apq = Gc[p1][q1];
+        // (x, y; z, u) = (a_pp, a_pq; a_qp, a_qq)
+        const double t1 = r_t[ba];
         const bool rt = s1 != 0.0;
-        Gn[p1][p1] = rt ? app - t1 * apq : app;
-        Gn[q1][q1] = rt ? aqq + t1 * apq : aqq;
-        Gn[p1][q1] = rt ? 0.0 : apq;
-        Gn[q1][p1] = rt ? 0.0 : apq;
+        Gn[p1][p1] = rt ? x - t1 * y : x;
+        Gn[q1][q1] = rt ? u + t1 * y : u;
+        Gn[p1][q1] = rt ? 0.0 : y;
+        Gn[q1][p1] = rt ? 0.0 : y;
       }
       if (s1 != 0.0) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-          const int i = 2 * bb + h;
-          const double x = Qm[i][p1], y = Qm[i][q1];
-          Qm[i][p1] = c1 * x - s1 * y;
-          Qm[i][q1] = s1 * x + c1 * y;
+          Qm[2 * bb + h][p1] = c1 * qx[h] - s1 * qy[h];
+          Qm[2 * bb + h][q1] = s1 * qx[h] + c1 * qy[h];
         }
       }
       cur ^= 1;
