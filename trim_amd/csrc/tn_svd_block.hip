@@ -249,14 +249,22 @@ __global__ void __launch_bounds__(kThreads) bj_sort(const Prob* __restrict__ pro
   for (int j = tid; j < pr.np; j += kThreads) pos[j < pr.n ? norm_rank(sh, pr.n, j) : j] = j;
 }
 
-// One round of a sweep: workgroup = (problem, block pair).
-__global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ probs, const int2* __restrict__ wmap,
+// One round of a sweep: workgroup = (problem, block pair).  Registers are
+// capped for four waves per SIMD (four workgroups per CU with 26 KB of LDS
+// each): a round is latency-bound per workgroup, so the CU needs several.
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) bj_round(const Prob* __restrict__ probs, const int2* __restrict__ wmap,
                                                      double* __restrict__ scr, int* __restrict__ iscr, int sweep,
                                                      int round) {
-  __shared__ double part[4][3][256];
-  __shared__ double G[2][kPW][kPW + 1];   // double-buffered for the inner sweeps
-  __shared__ double Qm[kPW][kPW + 1];
+  // LDS (26 KB, so that several workgroups share a CU): G's first buffer,
+  // and one region that holds the waves' partial Gram tiles until G is
+  // formed and then G's second buffer and Q
+  __shared__ double g0[kPW][kPW + 1];
+  __shared__ double un[2][kPW][kPW + 1];
   __shared__ double r_c[kB], r_s[kB], r_t[kB];   // the step's rotations
+  auto Gb = [&](int b) { return b ? un[0] : g0; };   // G's two buffers
+  double(*Qm)[kPW + 1] = un[1];
+  double(*part)[3][256] = reinterpret_cast<double(*)[3][256]>(&un[0][0][0]);   // [2][3][256]
+  static_assert(2 * 3 * 256 <= 2 * kPW * (kPW + 1), "partial tiles must fit the shared region");
 
   const int2 wm = wmap[blockIdx.x];
   const Prob pr = probs[wm.x];
@@ -320,17 +328,31 @@ __global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ pr
         qv[h] = qn[h];
       }
     }
-    // accumulator element v of lane (lr, lg) is tile entry (lg + 4v, lr)
+    // accumulator element v of lane (lr, lg) is tile entry (lg + 4v, lr).
+    // Waves 2 and 3 publish their tiles, waves 0 and 1 add them to theirs
+    // (w + 2 into w) and publish the sums.
+    if (w >= 2) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int idx = (lg + 4 * v) * 16 + lr;
-      part[w][0][idx] = aPP[v];
-      part[w][1][idx] = aPQ[v];
-      part[w][2][idx] = aQQ[v];
+      for (int v = 0; v < 4; ++v) {
+        const int idx = (lg + 4 * v) * 16 + lr;
+        part[w - 2][0][idx] = aPP[v];
+        part[w - 2][1][idx] = aPQ[v];
+        part[w - 2][2][idx] = aQQ[v];
+      }
+    }
+    __syncthreads();
+    if (w < 2) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int idx = (lg + 4 * v) * 16 + lr;
+        part[w][0][idx] = aPP[v] + part[w][0][idx];
+        part[w][1][idx] = aPQ[v] + part[w][1][idx];
+        part[w][2][idx] = aQQ[v] + part[w][2][idx];
+      }
     }
   }
   __syncthreads();
-  // sum the waves' partials in a fixed order; the diagonal tiles are read
+  // the last two partials, in a fixed order; the diagonal tiles are read
   // from their upper triangle so that G is exactly symmetric
   for (int e = tid; e < kPW * kPW; e += kThreads) {
     const int i = e / kPW, j = e % kPW;
@@ -353,7 +375,7 @@ __global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ pr
       b = i - kB;
     }
     const int idx = a * 16 + b;
-    G[0][i][j] = ((part[0][t][idx] + part[1][t][idx]) + part[2][t][idx]) + part[3][t][idx];
+    g0[i][j] = part[0][t][idx] + part[1][t][idx];   // (wave 0 + wave 2) + (wave 1 + wave 3)
   }
   __syncthreads();
 
@@ -362,14 +384,14 @@ __global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ pr
   int need = 0;
   for (int e = tid; e < kPW * kPW; e += kThreads) {
     const int i = e / kPW, j = e % kPW;
-    if (i < j && needs_rot(G[0][i][j], G[0][i][i], G[0][j][j], tol, abs_tol)) need = 1;
+    if (i < j && needs_rot(g0[i][j], g0[i][i], g0[j][j], tol, abs_tol)) need = 1;
   }
   // the panel's column norms (diagonal of G; of the rotated G below) are
   // what the next sweep orders the columns by
   double* nrm = scr + pr.sig_off;
   const int64_t dcol = (tid < kB) ? posP[tid] : (tid < kPW ? posQ[tid - kB] : 0);
   if (!__syncthreads_or(need)) {
-    if (tid < kPW && dcol < pr.n) nrm[dcol] = G[0][tid][tid];
+    if (tid < kPW && dcol < pr.n) nrm[dcol] = g0[tid][tid];
     return;
   }
 
@@ -387,8 +409,8 @@ __global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ pr
   for (int isw = 0; isw < kInnerSweeps; ++isw) {
     int rot = 0;
     for (int r = 0; r < kPW - 1; ++r) {
-      const double(*Gc)[kPW + 1] = G[cur];
-      double(*Gn)[kPW + 1] = G[cur ^ 1];
+      const double(*Gc)[kPW + 1] = Gb(cur);
+      double(*Gn)[kPW + 1] = Gb(cur ^ 1);
       // everything this thread reads of G and Q is fixed for the step: it
       // is loaded before the rotations are known, so only c and s cross the
       // barrier
@@ -451,7 +473,7 @@ __global__ void __launch_bounds__(kThreads) bj_round(const Prob* __restrict__ pr
     any_rot |= rot;
     if (!rot) break;
   }
-  if (tid < kPW && dcol < pr.n) nrm[dcol] = G[cur][tid][tid];
+  if (tid < kPW && dcol < pr.n) nrm[dcol] = Gb(cur)[tid][tid];
   if (!any_rot) return;
   if (tid == 0) {
     atomicOr(&flags[sweep], 1);
