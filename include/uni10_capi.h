@@ -106,6 +106,11 @@ int u10_batch_tebd_sectors(u10_batch_tebd b, int chain, int bond, int* charges, 
  * physical dimension d, with the same count/capacity convention. */
 int u10_u1_bond_sectors(int chi, int d, int bond, int* charges, int* dims, int* count);
 int u10_batch_tebd_step(u10_batch_tebd b, int nsteps);
+/* step without waiting: each engine has its own stream, so the steps of
+ * several engines overlap on the device; wait finishes the engine's work and
+ * checks that its SVDs converged */
+int u10_batch_tebd_launch(u10_batch_tebd b, int nsteps);
+int u10_batch_tebd_wait(u10_batch_tebd b);
 int u10_batch_tebd_set_graph(u10_batch_tebd b, int on);
 /* restart one chain (slot reuse): the seeded start of (delta, seed) */
 int u10_batch_tebd_load(u10_batch_tebd b, int chain, double delta, uint64_t seed);

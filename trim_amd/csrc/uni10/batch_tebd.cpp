@@ -300,7 +300,8 @@ BatchTebd::BatchTebd(const std::vector<int>& chi, const std::vector<double>& del
     if (d_[i] != 2 && d_[i] != 3) detail::fail("BatchTebd: d must be 2 (spin 1/2) or 3 (spin 1)");
     if (sym_[i] != 0 && sym_[i] != 1) detail::fail("BatchTebd: sym must be 0 (dense) or 1 (U(1))");
   }
-  stream_ = detail::stream();
+  // the engine's own stream (default priority): independent engines overlap
+  detail::check(tn_stream_create(&stream_), "BatchTebd stream");
   u1_.resize(nc);
   for (size_t i = 0; i < nc; ++i)
     if (sym_[i]) layout_u1((int)i);
@@ -351,6 +352,7 @@ BatchTebd::BatchTebd(const std::vector<int>& chi, const std::vector<double>& del
     }
   }
   arena_.reset((size_t)o, true);
+  detail::sync();   // the arena is cleared on the caller's stream
 
   // initial state of every chain and its gate
   tau_ = tau;
@@ -643,6 +645,7 @@ void BatchTebd::load(int chain, double delta, uint64_t seed) {
 }
 
 BatchTebd::~BatchTebd() {
+  tn_stream_sync(stream_);
   if (graph_) tn_graph_destroy(graph_);
   for (int p = 0; p < 2; ++p) {
     tn_gemm_plan_destroy(g1_[p]);
@@ -655,6 +658,7 @@ BatchTebd::~BatchTebd() {
     tn_vecops_plan_destroy(trunc_[p]);
     tn_vecops_plan_destroy(scale_[p]);
   }
+  tn_stream_destroy(stream_);
 }
 
 void BatchTebd::update(int p, bool warm) {
@@ -685,6 +689,11 @@ void BatchTebd::set_graph(bool on) {
 }
 
 void BatchTebd::step(int n) {
+  launch(n);
+  wait();
+}
+
+void BatchTebd::launch(int n) {
   for (int k = 0; k < n; ++k) {
     if (use_graph_ && warm_[0] && warm_[1]) {
       // every later step issues the same launches: capture once, replay
@@ -698,6 +707,10 @@ void BatchTebd::step(int n) {
       run_step();
     }
   }
+}
+
+void BatchTebd::wait() {
+  if (!last_svd_) return;
   int nonconv = 0;
   // status of the last SVD launch of the step (the second update's)
   detail::check(tn_svd_plan_check(last_svd_, &nonconv, &sweeps_, stream_), "BatchTebd svd");
@@ -726,6 +739,7 @@ const U1Sectors* BatchTebd::sectors(int chain, int bond) const {
 
 double BatchTebd::energy(int chain) {
   if (chain < 0 || chain >= chains()) detail::fail("BatchTebd::energy: bad index");
+  detail::check(tn_stream_sync(stream_), "BatchTebd energy");   // the networks run on the caller's stream
   const int c = chi_[chain], kd = d_[chain];
   const Off& f = off_[chain];
   std::vector<Bond> b3 = {Bond(BD_IN, c), Bond(BD_IN, c), Bond(BD_OUT, kd)};

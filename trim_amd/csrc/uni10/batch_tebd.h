@@ -44,6 +44,11 @@ class BatchTebd {
   BatchTebd& operator=(const BatchTebd&) = delete;
 
   void step(int n = 1);            // n x (update(A,B), update(B,A)) on every chain
+  // step without waiting: the engine's work goes to its own stream, so the
+  // steps of several engines overlap on the device; wait() finishes them
+  // and checks the SVDs' convergence
+  void launch(int n = 1);
+  void wait();
   // restart chain `chain` from the seeded start of (delta, seed): the slot
   // reuse of continuous batching; other chains are untouched
   void load(int chain, double delta, uint64_t seed);

@@ -384,6 +384,14 @@ int u10_batch_tebd_step(u10_batch_tebd b, int n) {
   return guard([&] { b->B.step(n); });
 }
 
+int u10_batch_tebd_launch(u10_batch_tebd b, int n) {
+  return guard([&] { b->B.launch(n); });
+}
+
+int u10_batch_tebd_wait(u10_batch_tebd b) {
+  return guard([&] { b->B.wait(); });
+}
+
 int u10_batch_tebd_set_graph(u10_batch_tebd b, int on) {
   return guard([&] { b->B.set_graph(on != 0); });
 }

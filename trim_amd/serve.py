@@ -13,11 +13,13 @@ symmetry (dense, or U(1) with conserved Sz: block-sparse tensors whose
 sectors follow the U(1) chains' sector rule).  The server keeps one batched
 engine per class (bond dimension, d, symmetry), each with a fixed number of
 slots whose tensors stay resident in HBM; the slot count of each class is
-what its share of the HBM budget holds.  Every tick it admits
-the requests that have arrived into free slots (first come, first served per
-class), advances every class with a busy slot by one step in one batched,
-graph-replayed launch sequence, and retires finished requests.  Requests that
-arrive while others run join at the next tick: a chain's computation never
+what its share of the HBM budget holds.  Every tick it admits the requests
+that have arrived into free slots (first come, first served per class),
+advances every class with a busy slot by one step in one batched,
+graph-replayed launch sequence (every class on its own stream, so the
+classes' work overlaps on the device), and retires finished requests.
+Requests that arrive while others run join at the next tick: a chain's
+computation never
 depends on which other slots are busy (slots are restarted with identity
 warm-start bases), so each result is the same alone and in the mix.
 """
@@ -181,11 +183,15 @@ class Server:
                 r.t_done = (time.perf_counter() - t0) if realtime else now
                 done.append(r)
             contracts = []
-            # one batched step of every class with a busy slot
-            for c in self.classes:
-                if not self.busy[c]:
-                    continue
-                self.engines[c].step(1)
+            # one batched step of every class with a busy slot: all classes
+            # are launched first (each engine has its own stream, so their
+            # work overlaps on the device), then waited for
+            active = [c for c in self.classes if self.busy[c]]
+            for c in active:
+                self.engines[c].launch(1)
+            for c in active:
+                self.engines[c].wait()
+            for c in active:
                 self.chain_steps += len(self.busy[c])
                 for s in list(self.busy[c]):
                     ent = self.busy[c][s]

@@ -84,6 +84,8 @@ SIGNATURES = {
     "u10_batch_tebd_sectors": [P, I, I, P, P, PI],
     "u10_u1_bond_sectors": [I, I, I, P, P, PI],
     "u10_batch_tebd_step": [P, I],
+    "u10_batch_tebd_launch": [P, I],
+    "u10_batch_tebd_wait": [P],
     "u10_batch_tebd_set_graph": [P, I],
     "u10_batch_tebd_load": [P, I, D, ctypes.c_uint64],
     "u10_batch_tebd_energy": [P, I, PD],
@@ -469,6 +471,13 @@ class BatchTebd:
 
     def step(self, n: int = 1):
         _call("u10_batch_tebd_step", self.h, int(n))
+
+    def launch(self, n: int = 1):
+        """Enqueue n steps on the engine's own stream without waiting."""
+        _call("u10_batch_tebd_launch", self.h, int(n))
+
+    def wait(self):
+        _call("u10_batch_tebd_wait", self.h)
 
     def set_graph(self, on: bool = True):
         _call("u10_batch_tebd_set_graph", self.h, int(bool(on)))
