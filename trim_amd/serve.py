@@ -19,9 +19,9 @@ advances every class with a busy slot by one step in one batched,
 graph-replayed launch sequence (every class on its own stream, so the
 classes' work overlaps on the device), and retires finished requests.
 Requests that arrive while others run join at the next tick: a chain's
-computation never
-depends on which other slots are busy (slots are restarted with identity
-warm-start bases), so each result is the same alone and in the mix.
+computation never depends on which other slots are busy (slots are
+restarted with identity warm-start bases), so each result is the same alone
+and in the mix.
 """
 from __future__ import annotations
 
