@@ -42,7 +42,8 @@ HBM_PEAK_GBPS = 8000.0
 F64_MFMA_PEAK_TFLOPS = 78.6
 DIMS = (256, 2, 2, 256)
 # FP64 GEMM tile shapes by autotuner id: BM x BN block tile, KC k per stage
-F64_TILES = {0: "32x32,kc32", 1: "64x64,kc64", 2: "128x64,kc32", 3: "64x128,kc32", 4: "64x32,kc64"}
+F64_TILES = {0: "32x32,kc32", 1: "64x64,kc64", 2: "128x64,kc32", 3: "64x128,kc32", 4: "64x32,kc64",
+             5: "32x32,kc32,512 threads"}
 
 
 def splitmix_uniform(seed, n):

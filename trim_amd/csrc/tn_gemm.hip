@@ -104,8 +104,11 @@ __device__ __forceinline__ int64_t map_off(const MapDev& m, int64_t i64) {
 // host checks that every pair is contiguous and 16-byte aligned).
 constexpr int kKTab = 2048;   // longest K whose offsets are tabulated in LDS
 
-template <typename T, int BM, int BN, int KC, int D, bool VEC, bool KT>
-__global__ void __launch_bounds__(256) gemm_mfma_k32(const double* __restrict__ A,
+// NT = 512: two waves per 16x16 sub-tile, one per K chain (steps s even /
+// odd of every chunk), so each SIMD holds two waves; the chains are summed
+// in the same order as with one wave holding both.
+template <typename T, int BM, int BN, int KC, int D, bool VEC, bool KT, int NT = 256>
+__global__ void __launch_bounds__(NT) gemm_mfma_k32(const double* __restrict__ A,
                                                      const double* __restrict__ B,
                                                      double* __restrict__ C,
                                                      const GroupDev* __restrict__ groups,
@@ -113,15 +116,16 @@ __global__ void __launch_bounds__(256) gemm_mfma_k32(const double* __restrict__ 
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
   constexpr int NPA = KC / 2;                 // A pairs per row of a stage
-  constexpr int AROWS = 256 / NPA;            // A rows per pass of the block
+  constexpr int AROWS = NT / NPA;             // A rows per pass of the block
   constexpr int PA = BM / AROWS;              // A pairs per thread per stage
   constexpr int NPB = BN / 2;                 // B pairs per k row
-  constexpr int BROWS = 256 / NPB;            // k rows per pass of the block
+  constexpr int BROWS = NT / NPB;             // k rows per pass of the block
   constexpr int PB = KC / BROWS;              // B pairs per thread per stage
   constexpr int PAD = 16 / (int)sizeof(T);    // keeps rows 16-byte aligned
-  static_assert(KC % 32 == 0 && TM >= 1 && TN >= 1 && 256 % NPA == 0 && BM % AROWS == 0 &&
-                    256 % NPB == 0 && KC % BROWS == 0,
+  static_assert(KC % 32 == 0 && TM >= 1 && TN >= 1 && NT % NPA == 0 && BM % AROWS == 0 &&
+                    NT % NPB == 0 && KC % BROWS == 0 && (NT == 256 || NT == 512),
                 "tile shape");
+  constexpr int NCH = (NT == 512) ? 1 : 2;    // K chains held by one wave
   typedef typename std::conditional<sizeof(T) == 8, d4, f4>::type acc_t;
   typedef T t2 __attribute__((ext_vector_type(2)));
   __shared__ __attribute__((aligned(16))) T As[2][BM][KC + PAD];
@@ -141,7 +145,9 @@ __global__ void __launch_bounds__(256) gemm_mfma_k32(const double* __restrict__ 
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int lg = lane >> 4, lr = lane & 15;
-  const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
+  const int tw = wave & 3;                    // the wave's 16x16 sub-tiles
+  const int ch = (NT == 512) ? (wave >> 2) : 0;   // its K chain when NCH == 1
+  const int wm0 = (tw >> 1) * WM, wn0 = (tw & 1) * WN;
 
   // Out-of-range rows, columns and k are clamped to the last valid index for
   // addressing (every load reads valid memory, with no branches) and their
@@ -169,7 +175,7 @@ __global__ void __launch_bounds__(256) gemm_mfma_k32(const double* __restrict__ 
   // read per offset instead of a multi-level decode.
   const MapDev a_col = g.a_col, b_row = g.b_row;
   if constexpr (KT) {
-    for (int k = tid; k < K; k += 256) {
+    for (int k = tid; k < K; k += NT) {
       tabA[k] = (int32_t)map_off(a_col, k);
       tabB[k] = (int32_t)map_off(b_row, k);
     }
@@ -178,9 +184,9 @@ __global__ void __launch_bounds__(256) gemm_mfma_k32(const double* __restrict__ 
   auto koff_a = [&](int64_t k) -> int64_t { return KT ? (int64_t)tabA[k] : map_off(a_col, k); };
   auto koff_b = [&](int64_t k) -> int64_t { return KT ? (int64_t)tabB[k] : map_off(b_row, k); };
 
-  acc_t acc[2][TM][TN];
+  acc_t acc[NCH][TM][TN];
 #pragma unroll
-  for (int h = 0; h < 2; ++h)
+  for (int h = 0; h < NCH; ++h)
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -241,35 +247,45 @@ __global__ void __launch_bounds__(256) gemm_mfma_k32(const double* __restrict__ 
   // lane.  The next chunk's operands are read from LDS into separate
   // registers before the current chunk's MFMAs issue, so LDS latency hides
   // behind the matrix core and no read waits on an MFMA's source registers.
+  // (step s = 2q + h of a chunk accumulates into chain h)
   struct Frag {
-    t2 a[4][TM];
-    T b[4][2][TN];
+    T a[4][NCH][TM];
+    T b[4][NCH][TN];
   };
   auto read_frag = [&](Frag& f, int st, int u) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
+      if constexpr (NCH == 2) {
 #pragma unroll
-      for (int i = 0; i < TM; ++i)
-        f.a[q][i] = *reinterpret_cast<const t2*>(&As[st][wm0 + 16 * i + lr][32 * u + 8 * lg + 2 * q]);
+        for (int i = 0; i < TM; ++i) {
+          const t2 v = *reinterpret_cast<const t2*>(&As[st][wm0 + 16 * i + lr][32 * u + 8 * lg + 2 * q]);
+          f.a[q][0][i] = v.x;
+          f.a[q][1][i] = v.y;
+        }
+      } else {
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
+        for (int i = 0; i < TM; ++i) f.a[q][0][i] = As[st][wm0 + 16 * i + lr][32 * u + 8 * lg + 2 * q + ch];
+      }
 #pragma unroll
-        for (int j = 0; j < TN; ++j) f.b[q][h][j] = Bs[st][32 * u + 8 * lg + 2 * q + h][wn0 + 16 * j + lr];
+      for (int h = 0; h < NCH; ++h)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          f.b[q][h][j] = Bs[st][32 * u + 8 * lg + 2 * q + (NCH == 2 ? h : ch)][wn0 + 16 * j + lr];
     }
   };
   auto mfma_frag = [&](const Frag& f) {
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
+      for (int h = 0; h < NCH; ++h)
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
           for (int j = 0; j < TN; ++j) {
             if constexpr (sizeof(T) == 8)
-              acc[h][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(f.a[q][i][h], f.b[q][h][j], acc[h][i][j], 0, 0, 0);
+              acc[h][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(f.a[q][h][i], f.b[q][h][j], acc[h][i][j], 0, 0, 0);
             else
-              acc[h][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(f.a[q][i][h], f.b[q][h][j], acc[h][i][j], 0, 0, 0);
+              acc[h][i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(f.a[q][h][i], f.b[q][h][j], acc[h][i][j], 0, 0, 0);
           }
   };
   // (sched_barrier keeps the scheduler from sinking the reads back next to
@@ -307,7 +323,31 @@ __global__ void __launch_bounds__(256) gemm_mfma_k32(const double* __restrict__ 
     }
   }
 
-  // write back: C = chain0 + chain1
+  // write back: C = chain0 + chain1.  With NCH == 1 the chain-1 waves hand
+  // their accumulators to the chain-0 waves through LDS (the stage buffers
+  // are free after the last barrier of the loop).
+  acc_t other[TM][TN];
+  if constexpr (NCH == 1) {
+    acc_t* xch = reinterpret_cast<acc_t*>(&As[0][0][0]);
+    static_assert(sizeof(acc_t) * 4 * 64 * TM * TN <= sizeof(As), "exchange fits the A stages");
+    if (ch == 1) {
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) xch[((tw * TM + i) * TN + j) * 64 + lane] = acc[0][i][j];
+    }
+    __syncthreads();
+    if (ch == 1) return;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) other[i][j] = xch[((tw * TM + i) * TN + j) * 64 + lane];
+  } else {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) other[i][j] = acc[NCH - 1][i][j];
+  }
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
     const int64_t gn = n0 + wn0 + j * 16 + lr;
@@ -321,7 +361,7 @@ __global__ void __launch_bounds__(256) gemm_mfma_k32(const double* __restrict__ 
         const int row = (sizeof(T) == 8) ? (lg + 4 * v) : (4 * lg + v);
         const int64_t gm = m0 + wm0 + i * 16 + row;
         if (gm < M)
-          Cb[map_off(g.c_row, gm) + coff] = (double)acc[0][i][j][v] + (double)acc[1][i][j][v];
+          Cb[map_off(g.c_row, gm) + coff] = (double)acc[0][i][j][v] + (double)other[i][j][v];
       }
   }
 }
@@ -477,13 +517,17 @@ __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict_
 struct TileCfg {
   int bm, bn;
 };
-constexpr int kNumCfg = 5;
-constexpr TileCfg kCfgMfma[kNumCfg] = {{32, 32}, {64, 64}, {128, 64}, {64, 128}, {64, 32}};
-constexpr TileCfg kCfgBf16[kNumCfg] = {{32, 32}, {64, 64}, {128, 64}, {64, 128}, {128, 128}};
+// FP64/FP32: five 256-thread shapes and the 32x32 tile with 512 threads
+// (id 5, two waves per sub-tile); split-BF16: five shapes.
+constexpr int kMaxCfg = 6;
+constexpr TileCfg kCfgMfma[kMaxCfg] = {{32, 32}, {64, 64}, {128, 64}, {64, 128}, {64, 32}, {32, 32}};
+constexpr TileCfg kCfgBf16[kMaxCfg] = {{32, 32}, {64, 64}, {128, 64}, {64, 128}, {128, 128}, {128, 128}};
 
-TileCfg cfg_shape(int mode, int cfg) {
-  return (mode == TN_GEMM_F64 || mode == TN_GEMM_F32) ? kCfgMfma[cfg] : kCfgBf16[cfg];
-}
+bool mfma_mode(int mode) { return mode == TN_GEMM_F64 || mode == TN_GEMM_F32; }
+int num_cfg(int mode) { return mfma_mode(mode) ? 6 : 5; }
+int cfg_threads(int mode, int cfg) { return (mfma_mode(mode) && cfg == 5) ? 512 : 256; }
+
+TileCfg cfg_shape(int mode, int cfg) { return mfma_mode(mode) ? kCfgMfma[cfg] : kCfgBf16[cfg]; }
 
 typedef void (*KernFn)(const double*, const double*, double*, const GroupDev*, const TileDev*);
 
@@ -495,6 +539,7 @@ KernFn pick_mfma(int cfg) {
     case 2: return gemm_mfma_k32<T, 128, 64, 32, 2, VEC, KT>;
     case 3: return gemm_mfma_k32<T, 64, 128, 32, 2, VEC, KT>;
     case 4: return gemm_mfma_k32<T, 64, 32, 64, 2, VEC, KT>;
+    case 5: return gemm_mfma_k32<T, 32, 32, 32, 3, VEC, KT, 512>;
   }
   return nullptr;
 }
@@ -672,9 +717,10 @@ static int build_tiles(tn_gemm_plan_s* P, int cfg) {
 // Default tile: the largest tile that still gives every CU at least one
 // tile; otherwise the smallest tile.
 static int default_cfg(const tn_gemm_plan_s* P) {
-  int order[kNumCfg];
-  for (int c = 0; c < kNumCfg; ++c) order[c] = c;
-  std::sort(order, order + kNumCfg, [&](int a, int b) {
+  const int nc = num_cfg(P->mode);
+  int order[kMaxCfg];
+  for (int c = 0; c < nc; ++c) order[c] = c;
+  std::sort(order, order + nc, [&](int a, int b) {
     const TileCfg x = cfg_shape(P->mode, a), y = cfg_shape(P->mode, b);
     return x.bm * x.bn > y.bm * y.bn || (x.bm * x.bn == y.bm * y.bn && a < b);
   });
@@ -686,7 +732,7 @@ static int default_cfg(const tn_gemm_plan_s* P) {
       t += ((g.M + sh.bm - 1) / sh.bm) * ((g.N + sh.bn - 1) / sh.bn);
     if (t >= want) return c;
   }
-  return order[kNumCfg - 1];
+  return order[nc - 1];
 }
 
 static void destroy_plan(tn_gemm_plan_s* P) {
@@ -775,7 +821,8 @@ int tn_gemm_plan_create(tn_gemm_plan* plan, int mode, int64_t ngroups, const tn_
 
 int tn_gemm_plan_set_tile(tn_gemm_plan P, int tile_id) {
   TN_REQUIRE(P != nullptr, TN_EINVAL, "tn_gemm_plan_set_tile: NULL plan");
-  TN_REQUIRE(tile_id >= 0 && tile_id < kNumCfg, TN_EINVAL, "tile id %d outside [0,%d)", tile_id, kNumCfg);
+  TN_REQUIRE(tile_id >= 0 && tile_id < num_cfg(P->mode), TN_EINVAL, "tile id %d outside [0,%d)", tile_id,
+             num_cfg(P->mode));
   if (tile_id == P->cfg) return TN_OK;
   return build_tiles(P, tile_id);
 }
@@ -793,7 +840,7 @@ int tn_gemm_plan_autotune(tn_gemm_plan P, const double* A, const double* B, doub
   int best = P->cfg;
   float best_t = 1e30f;
   int rc = TN_OK;
-  for (int c = 0; c < kNumCfg && rc == TN_OK; ++c) {
+  for (int c = 0; c < num_cfg(P->mode) && rc == TN_OK; ++c) {
     rc = build_tiles(P, c);
     if (rc != TN_OK) break;
     rc = tn_gemm_exec(P, A, B, C, stream);   // warm-up
@@ -854,7 +901,8 @@ int tn_gemm_exec(tn_gemm_plan P, const double* A, const double* B, double* C, vo
   const bool vec = P->vec_ok && ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) & 15) == 0;
   KernFn k = pick_kernel(P->mode, P->cfg, vec, P->koff32 && P->max_k <= kKTab);
   TN_REQUIRE(k != nullptr, TN_EINVAL, "no kernel for mode %d tile %d", P->mode, P->cfg);
-  hipLaunchKernelGGL(k, dim3((unsigned)P->nslots), dim3(256), 0, s, A, B, C, P->d_groups, P->d_tiles);
+  hipLaunchKernelGGL(k, dim3((unsigned)P->nslots), dim3(cfg_threads(P->mode, P->cfg)), 0, s, A, B, C, P->d_groups,
+                     P->d_tiles);
   TN_LAUNCH_CHECK();
   return TN_OK;
 }
