@@ -182,7 +182,9 @@ int tn_diag_col_mul_f64(double* mat, const double* diag, int64_t M, int64_t N, v
 /* ---- dense FP64 decompositions -------------------------------------------
  * matrixSVD (uni10_lapack_cpu.cpp:289-313): thin SVD of the row-major
  * M x N matrix A: U (M x k), S (k, descending), VT (k x N), k = min(M, N).
- * One-sided Jacobi; singular vectors are unique up to a sign per pair. */
+ * One-sided Jacobi; singular vectors are unique up to a sign per pair.
+ * U may be NULL (only S and VT are formed; for M >= N and min(M, N) <= 64
+ * this takes the faster QR-preconditioned sweeps). */
 int tn_svd_f64(const double* A, int64_t M, int64_t N, double* U, double* S,
                double* VT, void* stream);
 /* Batched: problem i reads A + a_off[i] (m[i] x n[i]) and writes
@@ -209,13 +211,15 @@ int tn_svd_set_block_min(int64_t n_min);
  * orthonormal rows, e.g. the VT of a previous SVD of a matrix close to A;
  * the Jacobi sweeps start from W = A V0 instead of A, which cuts the sweeps
  * when A changes little between calls (successive TEBD updates).  The result
- * is the SVD of A, as from tn_svd_f64.  No counterpart in the reference. */
+ * is the SVD of A, as from tn_svd_f64 (U may be NULL likewise).  No
+ * counterpart in the reference. */
 int tn_svd_f64_warm(const double* A, int64_t M, int64_t N, const double* V0T, double* U,
                     double* S, double* VT, void* stream);
 
 /* A reusable batch of SVD problems (shapes and offsets fixed at creation;
  * v0_off[i] >= 0 gives problem i a warm start from V0T + v0_off[i], NULL or
- * -1 = cold).  tn_svd_plan_exec only launches kernels: it neither allocates
+ * -1 = cold; u_off[i] = -1: U of problem i is not formed, and U may be NULL
+ * at execution when no problem forms it).  tn_svd_plan_exec only launches kernels: it neither allocates
  * nor synchronises, so it can be captured into a graph (tn_graph_begin).
  * tn_svd_plan_check waits for the stream and reports the problems of the
  * last execution that did not converge and the most sweeps any needed.

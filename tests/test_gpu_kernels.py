@@ -407,6 +407,156 @@ def test_svd_warm_start_nearby_matrix(tn, ora, n):
     assert warm < cold, (warm, cold)
 
 
+def _svd_no_u(tn, a, vt0=None):
+    """tn_svd_f64 / tn_svd_f64_warm with U = NULL: S and VT only."""
+    M, N = a.shape
+    k = min(M, N)
+    A = tn.DeviceArray.from_host(a)
+    S, VT = tn.DeviceArray((k,)), tn.DeviceArray((k, N))
+    if vt0 is None:
+        tn.call("tn_svd_f64", tn._p(A), M, N, None, tn._p(S), tn._p(VT), None)
+    else:
+        V0 = tn.DeviceArray.from_host(np.ascontiguousarray(vt0))
+        tn.call("tn_svd_f64_warm", tn._p(A), M, N, tn._p(V0), None, tn._p(S), tn._p(VT), None)
+    return S.to_host(), VT.to_host()
+
+
+def _check_s_vt(ora, a, s, vt, rank=None):
+    """LAPACK's singular values; VT with orthonormal rows such that A VT^T
+    has orthogonal columns of norms S (so A = (A VT^T) VT when M >= N); the
+    rows of separated singular values equal LAPACK's up to sign."""
+    M, N = a.shape
+    k = min(M, N)
+    r = k if rank is None else rank
+    _, sr, vtr = ora.svd(a)
+    s0 = max(sr[0], 1e-300)
+    assert np.max(np.abs(s[:r] - sr[:r])) <= 1e-12 * s0
+    assert np.max(np.abs(s[r:]), initial=0.0) <= 1e-12 * s0
+    assert np.all(np.diff(s) <= 0)
+    assert np.max(np.abs(vt @ vt.T - np.eye(k))) <= 1e-12
+    us = a @ vt.T
+    assert np.max(np.abs(us.T @ us - np.diag(s * s))) <= 1e-12 * s0 * s0
+    if M >= N:
+        assert np.linalg.norm(us @ vt - a) <= 1e-12 * max(np.linalg.norm(a), 1e-300)
+    gaps = np.abs(np.diff(sr)) if k > 1 else np.array([np.inf])
+    for j in range(r):
+        sep = min(gaps[j - 1] if j > 0 else np.inf, gaps[j] if j < k - 1 else np.inf)
+        if sr[j] < 1e-8 * s0 or sep < 1e-6 * s0:
+            continue
+        sg = np.sign(np.dot(vt[j], vtr[j])) or 1.0
+        assert np.max(np.abs(sg * vt[j] - vtr[j])) <= 1e-8
+
+
+@pytest.mark.parametrize("M,N", [(64, 64), (5, 4), (100, 64), (128, 33), (7, 1), (1, 1), (64, 63),
+                                 (33, 5), (20, 50)])
+def test_svd_without_u(tn, ora, M, N):
+    """U not wanted: for M >= N (n <= 64) the kernel factorises A = Q R by
+    Householder reflections and sweeps R^T, reading VT off its normalised
+    columns; M < N takes the plain sweeps without writing U."""
+    _say("svd no U %dx%d" % (M, N))
+    a = _rand(M * 1000 + N + 3, (M, N))
+    s, vt = _svd_no_u(tn, a)
+    _check_s_vt(ora, a, s, vt)
+
+
+@pytest.mark.parametrize("case", ["rank20", "zero columns", "zero", "graded"])
+def test_svd_without_u_special(tn, ora, case):
+    """Rank-deficient inputs on the QR-preconditioned path: noise columns are
+    orthogonalised relative to their own norms (VT stays orthonormal); an
+    exactly zero column cannot be normalised and sends the problem through
+    the plain sweeps.  A spectrum over twelve decades keeps its small values
+    to LAPACK's absolute accuracy."""
+    rng = np.random.default_rng(11)
+    r = None
+    if case == "rank20":
+        a, r = rng.uniform(-1, 1, (64, 20)) @ rng.uniform(-1, 1, (20, 64)), 20
+    elif case == "zero columns":
+        a, r = np.hstack([rng.uniform(-1, 1, (64, 40)), np.zeros((64, 24))]), 40
+    elif case == "zero":
+        a, r = np.zeros((16, 16)), 0
+    else:
+        a = _orth(rng, 64) @ np.diag(10.0 ** -np.linspace(0, 12, 64)) @ _orth(rng, 64).T
+    s, vt = _svd_no_u(tn, a)
+    if r == 0:
+        assert np.all(s == 0) and np.max(np.abs(vt @ vt.T - np.eye(16))) <= 1e-12
+        return
+    _check_s_vt(ora, a, s, vt, rank=r)
+
+
+def test_svd_without_u_warm_theta_like(tn, ora):
+    """Warm start on a theta-like matrix (rows scaled by a lambda decaying
+    over nine decades, as theta = lambda C) after a 1e-3 change of C: the
+    QR-preconditioned sweeps give the same factors as the plain warm sweeps,
+    in fewer sweeps (4 against 18 in a CPU emulation)."""
+    rng = np.random.default_rng(5)
+    n = 64
+    lam = np.exp(-np.arange(n) / 3.0)
+    c0 = rng.uniform(-1, 1, (n, n))
+    a0 = lam[:, None] * c0
+    a1 = lam[:, None] * (c0 + 1e-3 * rng.uniform(-1, 1, (n, n)))
+    vt0 = ora.svd(a0)[2]
+    s, vt = _svd_no_u(tn, a1, vt0)
+    qr_sweeps = _sweeps(tn)
+    _check_s_vt(ora, a1, s, vt)
+    A1, V0 = tn.DeviceArray.from_host(a1), tn.DeviceArray.from_host(np.ascontiguousarray(vt0))
+    U, S, VT = tn.DeviceArray((n, n)), tn.DeviceArray((n,)), tn.DeviceArray((n, n))
+    tn.call("tn_svd_f64_warm", tn._p(A1), n, n, tn._p(V0), tn._p(U), tn._p(S), tn._p(VT), None)
+    plain_sweeps = _sweeps(tn)
+    assert np.max(np.abs(S.to_host() - s)) <= 1e-12 * s[0]
+    assert qr_sweeps < plain_sweeps, (qr_sweeps, plain_sweeps)
+
+
+def test_svd_plan_without_u(tn, ora):
+    """A plan mixing problems that form U with problems that do not
+    (u_off = -1) over the small, LDS and block paths, one warm-started; and
+    a plan where none does, executed with U = NULL."""
+    shapes = [(64, 64), (30, 20), (100, 90), (64, 64), (160, 150)]
+    mats = [_rand(i + 170, s) for i, s in enumerate(shapes)]
+    A = np.concatenate([m.reshape(-1) for m in mats])
+    offs = np.cumsum([0] + [m.size for m in mats])[:-1].astype(np.int64)
+    ks = [min(s) for s in shapes]
+    s_off = np.cumsum([0] + ks)[:-1].astype(np.int64)
+    vt_off = np.cumsum([0] + [k * s[1] for s, k in zip(shapes, ks)])[:-1].astype(np.int64)
+    m = np.array([s[0] for s in shapes], np.int64)
+    n = np.array([s[1] for s in shapes], np.int64)
+    v0t = ora.svd(mats[0] + 1e-3 * _rand(5, (64, 64)))[2]
+    v0_off = np.array([-1, -1, -1, 0, -1], np.int64)
+    dA = tn.DeviceArray.from_host(A)
+    dV0 = tn.DeviceArray.from_host(v0t)
+    dS = tn.DeviceArray((int(sum(ks)),))
+    dV = tn.DeviceArray((int(sum(k * s[1] for s, k in zip(shapes, ks))),))
+    for with_u in (True, False):
+        u_off = np.array([-1, 0, -1, -1, -1] if with_u else [-1] * 5, np.int64)
+        dU = tn.DeviceArray((30 * 20,)) if with_u else None
+        h = ctypes.c_void_p()
+        tn.call("tn_svd_set_block_min", 128)
+        try:
+            tn.call("tn_svd_plan_create", ctypes.byref(h), len(shapes), tn._p(m), tn._p(n), tn._p(offs),
+                    tn._p(v0_off), tn._p(u_off), tn._p(s_off), tn._p(vt_off))
+        finally:
+            tn.call("tn_svd_set_block_min", 0)
+        plan = ctypes.c_void_p(h.value)
+        try:
+            tn.call("tn_svd_plan_exec", plan, tn._p(dA), tn._p(dV0), tn._p(dU) if with_u else None, tn._p(dS),
+                    tn._p(dV), None)
+            nc, sw = ctypes.c_int(), ctypes.c_int()
+            tn.call("tn_svd_plan_check", plan, ctypes.byref(nc), ctypes.byref(sw), None)
+            assert nc.value == 0
+        finally:
+            tn.call("tn_svd_plan_destroy", plan)
+        Sall, Vall = dS.to_host(), dV.to_host()
+        for i, mat in enumerate(mats):
+            k = ks[i]
+            s = Sall[s_off[i]:s_off[i] + k]
+            vt = Vall[vt_off[i]:vt_off[i] + k * shapes[i][1]].reshape(k, shapes[i][1])
+            _check_s_vt(ora, mat, s, vt)
+        if with_u:
+            u = dU.to_host().reshape(30, 20)
+            s = Sall[s_off[1]:s_off[1] + 20]
+            vt = Vall[vt_off[1]:vt_off[1] + 400].reshape(20, 20)
+            assert np.linalg.norm(u @ np.diag(s) @ vt - mats[1]) <= 1e-12 * np.linalg.norm(mats[1])
+
+
 @pytest.mark.parametrize("M,N", [(100, 80), (80, 100), (96, 96), (70, 65), (130, 67)])
 def test_svd_one_workgroup_lds_path(tn, ora, M, N):
     """Matrices beyond the small kernel that fit one workgroup's LDS, kept on

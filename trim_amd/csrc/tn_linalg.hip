@@ -235,10 +235,11 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_kernel(const SvdProb* __r
     double sj = sig[j];
     double v = (sj > 0.0) ? W[(size_t)j * ldw + i] / sj : 0.0;
     int rk = rank_of[j];
-    if (!tr)
-      Ub[(int64_t)i * n + rk] = v;       // U is M x n (n = N)
-    else
+    if (!tr) {
+      if (pr.u_off >= 0) Ub[(int64_t)i * n + rk] = v;   // U is M x n (n = N)
+    } else {
       VTb[(int64_t)rk * N + i] = v;      // VT is n x N (n = M)
+    }
   }
   for (int64_t e = tid; e < (int64_t)n * n; e += kThreads) {
     int j = (int)(e / n), i = (int)(e % n);
@@ -246,7 +247,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_kernel(const SvdProb* __r
     const double vv = V[(size_t)j * ldv + i];
     if (!tr)
       VTb[(int64_t)rk * n + i] = vv;     // VT is n x n
-    else
+    else if (pr.u_off >= 0)
       Ub[(int64_t)i * n + rk] = vv;      // U is M x M
   }
 }
@@ -506,6 +507,18 @@ __device__ inline void rr_pair_fast(int r, int k, int np, int& p, int& q) {
   }
 }
 
+// When only S and VT are wanted (no U) and M >= N, the kernel first
+// factorises W = Q R by Householder reflections in LDS and runs the sweeps on
+// X = R^T, whose column Gram matrix is close to diagonal (Drmac and Veselic,
+// "New fast and accurate Jacobi SVD algorithm", SIAM J. Matrix Anal. Appl.
+// 29 (2008)): on TEBD thetas this halves the sweeps.  From X = U_X S V_X^T,
+// R = V_X S U_X^T and A V0 P = Q R give A = (Q V_X) S (V0 P U_X)^T, so the
+// right singular vectors are V0 P U_X with U_X the normalised columns of X;
+// no rotation is accumulated.  This needs every column of X nonzero, so the
+// sweeps run without the absolute floor (pairs are orthogonal relative to
+// their own norms, which keeps U_X orthonormal down to the noise columns of
+// a rank-deficient A); a column norm at the underflow level (an exactly
+// rank-deficient A) sends the problem back through the plain sweeps.
 template <int EMX>
 __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __restrict__ probs,
                                                              const double* __restrict__ A,
@@ -518,11 +531,13 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
   constexpr int CW = kSmallTP * CHW + 2;         // W column stride
   constexpr int CHV = kSmallEN + 2;
   constexpr int CV = kSmallTP * CHV + 2;
+  constexpr int kRows = kSmallTP * EMX;          // W rows held (m <= kRows)
   __shared__ __attribute__((aligned(16))) double W[kSmallN * CW];
   __shared__ __attribute__((aligned(16))) double V[kSmallN * CV];
   __shared__ double nrm[kSmallN];
   __shared__ int rank_of[kSmallN];
   __shared__ int col_at[kSmallN];
+  __shared__ int qr_col[kSmallN];
   const SvdProb pr = probs[blockIdx.x];
   const int64_t M = pr.m, N = pr.n;
   const bool tr = (M < N);
@@ -532,6 +547,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
   const int tid = threadIdx.x;
   const int g = tid / kSmallTP, t = tid % kSmallTP;
   const double* Ab = A + pr.a_off;
+  const bool want_u = pr.u_off >= 0;
   auto widx = [](int c, int i) { return c * CW + (i / EMX) * CHW + (i % EMX); };
   auto vidx = [](int c, int i) { return c * CV + (i / kSmallEN) * CHV + (i % kSmallEN); };
 
@@ -545,19 +561,22 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
     V[vidx(c, i)] = v;
   }
   if (warm) __syncthreads();
-  for (int e = tid; e < kSmallN * kSmallTP * EMX; e += kThreads) {
-    const int c = e / (kSmallTP * EMX), i = e % (kSmallTP * EMX);
-    double x = 0.0;
-    if (c < n && i < m) {
-      if (warm) {
-        for (int k = 0; k < n; ++k) x += Ab[(int64_t)i * N + k] * V[vidx(c, k)];
-      } else {
-        x = tr ? Ab[(int64_t)c * N + i] : Ab[(int64_t)i * N + c];
+  auto load_w = [&]() {
+    for (int e = tid; e < kSmallN * kRows; e += kThreads) {
+      const int c = e / kRows, i = e % kRows;
+      double x = 0.0;
+      if (c < n && i < m) {
+        if (warm) {
+          for (int k = 0; k < n; ++k) x += Ab[(int64_t)i * N + k] * V[vidx(c, k)];
+        } else {
+          x = tr ? Ab[(int64_t)c * N + i] : Ab[(int64_t)i * N + c];
+        }
       }
+      W[widx(c, i)] = x;
     }
-    W[widx(c, i)] = x;
-  }
-  __syncthreads();
+    __syncthreads();
+  };
+  load_w();
 
   // ||A||_F^2 from the exact column norms (group sums, then a fixed-order
   // sum over the columns by one thread)
@@ -578,8 +597,6 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
 
   const int np = (n % 2) ? n + 1 : n;
   const int npairs = np / 2;
-  const double tol = 10.0 * DBL_EPSILON * (double)m;
-  const double abs_tol = (double)m * DBL_EPSILON * DBL_EPSILON * normF2;
 
   // Each sweep starts from the columns sorted by decreasing norm (de Rijk):
   // position k of the round-robin schedule holds column col_at[k]; sorting
@@ -600,92 +617,214 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
     }
     __syncthreads();
   };
-  int sweep = 0;
-  const int max_sweeps = 60;
-  for (; sweep < max_sweeps; ++sweep) {
-    if (sweep > 0) exact_norms();
-    sort_positions();
-    int rotated = 0;
-    for (int r = 0; r < np - 1; ++r) {
-      int p = 0, q = 0;
-      bool live = g < npairs;
-      if (live) {
-        int pp, qq;
-        rr_pair_fast(r, g, np, pp, qq);
-        p = col_at[pp];
-        q = col_at[qq];
-        live = (p < n && q < n);
+
+  // W = Q R by Householder reflections over the columns in the order
+  // qr_col (decreasing norm for a cold start, a pivot-free stand-in for
+  // column pivoting; the given order for a warm start, where A V0 is
+  // already ordered by the previous singular values), then X = R^T in the
+  // first n columns of W.  Step j: every group reads the pivot column (rows
+  // >= j), and the groups reflect the later columns, one column each.
+  auto householder_rt = [&]() {
+    if (warm) {
+      for (int j = tid; j < n; j += kThreads) qr_col[j] = j;
+    } else {
+      sort_positions();
+      for (int j = tid; j < n; j += kThreads) qr_col[j] = col_at[j];
+    }
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+      const int cj = qr_col[j];
+      const double* xc = W + cj * CW + t * CHW;
+      double xr[EMX];
+      double xx = 0.0;
+#pragma unroll
+      for (int e = 0; e < EMX; ++e) {
+        xr[e] = (t * EMX + e >= j) ? xc[e] : 0.0;
+        xx += xr[e] * xr[e];
       }
-      if (live) {
-        double* wp = W + p * CW + t * CHW;
-        double* wq = W + q * CW + t * CHW;
-        double* vp = V + p * CV + t * CHV;
-        double* vq = V + q * CV + t * CHV;
-        // the rotation columns are read with the working columns, so their
-        // LDS latency overlaps the dot product and its reduction
-        double yp[kSmallEN], yq[kSmallEN];
+      xx = group8_sum(xx);
+      const double xj = W[widx(cj, j)];
+      const double alpha = -copysign(sqrt(xx), xj);
+      const double den = alpha * alpha - alpha * xj;   // v^T v / 2 for v = x - alpha e_j
+      for (int k = j + 1 + g; k < n; k += kThreads / kSmallTP) {
+        const int ck = qr_col[k];
+        double* wc = W + ck * CW + t * CHW;
+        double wr[EMX];
+        double xw = 0.0;
 #pragma unroll
-        for (int e = 0; e < kSmallEN; e += 2) {
-          const double2 u = *reinterpret_cast<const double2*>(vp + e);
-          const double2 v = *reinterpret_cast<const double2*>(vq + e);
-          yp[e] = u.x;
-          yp[e + 1] = u.y;
-          yq[e] = v.x;
-          yq[e + 1] = v.y;
+        for (int e = 0; e < EMX; ++e) {
+          wr[e] = wc[e];
+          xw += xr[e] * wr[e];
         }
-        double xp[EMX], xq[EMX];
+        xw = group8_sum(xw);
+        const double wj = W[widx(ck, j)];
+        if (xx > 0.0) {
+          const double coef = (xw - alpha * wj) / den;
 #pragma unroll
-        for (int e = 0; e < EMX; e += 2) {
-          const double2 u = *reinterpret_cast<const double2*>(wp + e);
-          const double2 v = *reinterpret_cast<const double2*>(wq + e);
-          xp[e] = u.x;
-          xp[e + 1] = u.y;
-          xq[e] = v.x;
-          xq[e + 1] = v.y;
-        }
-        double c = 0.0;
-#pragma unroll
-        for (int e = 0; e < EMX; ++e) c += xp[e] * xq[e];
-        c = group8_sum(c);
-        const double a = nrm[p], b = nrm[q];
-        if (c * c > tol * tol * (a * b) && fabs(c) > abs_tol) {
-          rotated = 1;
-          const double zeta = (b - a) * (0.5 * fast_rcp(c));
-          const double z2 = fma(zeta, zeta, 1.0);
-          const double tt = copysign(1.0, zeta) * fast_rcp(fabs(zeta) + z2 * fast_rsqrt(z2));
-          const double cs = fast_rsqrt(fma(tt, tt, 1.0));
-          const double sn = cs * tt;
-#pragma unroll
-          for (int e = 0; e < EMX; e += 2) {
-            *reinterpret_cast<double2*>(wp + e) =
-                double2{cs * xp[e] - sn * xq[e], cs * xp[e + 1] - sn * xq[e + 1]};
-            *reinterpret_cast<double2*>(wq + e) =
-                double2{sn * xp[e] + cs * xq[e], sn * xp[e + 1] + cs * xq[e + 1]};
-          }
-#pragma unroll
-          for (int e = 0; e < kSmallEN; e += 2) {
-            *reinterpret_cast<double2*>(vp + e) =
-                double2{cs * yp[e] - sn * yq[e], cs * yp[e + 1] - sn * yq[e + 1]};
-            *reinterpret_cast<double2*>(vq + e) =
-                double2{sn * yp[e] + cs * yq[e], sn * yp[e + 1] + cs * yq[e + 1]};
-          }
-          if (t == 0) {
-            nrm[p] = fmax(a - tt * c, 0.0);
-            nrm[q] = fmax(b + tt * c, 0.0);
+          for (int e = 0; e < EMX; ++e) {
+            const int r = t * EMX + e;
+            if (r >= j) wc[e] = wr[e] - coef * ((r == j) ? xj - alpha : xr[e]);
           }
         }
       }
       __syncthreads();
+      // the pivot column becomes column j of R (no one reads it before the
+      // transpose below)
+      if (g == 0) {
+        double* pc = W + cj * CW + t * CHW;
+#pragma unroll
+        for (int e = 0; e < EMX; ++e) {
+          const int r = t * EMX + e;
+          if (r == j) pc[e] = (xx > 0.0) ? alpha : xj;
+          else if (r > j) pc[e] = 0.0;
+        }
+      }
     }
-    if (!__syncthreads_or(rotated)) break;
+    __syncthreads();
+    // X[i][k] = R[k][i] = W[k][qr_col[i]] (k <= i): staged in registers
+    constexpr int kPer = (kSmallN * kSmallN + kThreads - 1) / kThreads;
+    double xv[kPer];
+#pragma unroll
+    for (int s = 0; s < kPer; ++s) {
+      const int e = tid + s * kThreads;
+      xv[s] = 0.0;
+      if (e < n * n) {
+        const int i = e / n, k = e % n;
+        if (k <= i) xv[s] = W[widx(qr_col[i], k)];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kPer; ++s) {
+      const int e = tid + s * kThreads;
+      if (e < n * n) W[widx(e % n, e / n)] = xv[s];
+    }
+    for (int e = tid; e < n * (kRows - n); e += kThreads) W[widx(e / (kRows - n), n + e % (kRows - n))] = 0.0;
+    __syncthreads();
+  };
+
+  // a plain sweep run also accumulates the rotations in V
+  auto sweeps = [&](bool qr) {
+    const int mr = qr ? n : m;
+    const double tol = 10.0 * DBL_EPSILON * (double)mr;
+    const double abs_tol = qr ? 0.0 : (double)m * DBL_EPSILON * DBL_EPSILON * normF2;
+    const int max_sweeps = 60;
+    int sweep = 0;
+    for (; sweep < max_sweeps; ++sweep) {
+      if (sweep > 0) exact_norms();
+      sort_positions();
+      int rotated = 0;
+      for (int r = 0; r < np - 1; ++r) {
+        int p = 0, q = 0;
+        bool live = g < npairs;
+        if (live) {
+          int pp, qq;
+          rr_pair_fast(r, g, np, pp, qq);
+          p = col_at[pp];
+          q = col_at[qq];
+          live = (p < n && q < n);
+        }
+        if (live) {
+          double* wp = W + p * CW + t * CHW;
+          double* wq = W + q * CW + t * CHW;
+          double* vp = V + p * CV + t * CHV;
+          double* vq = V + q * CV + t * CHV;
+          // the rotation columns are read with the working columns, so their
+          // LDS latency overlaps the dot product and its reduction
+          double yp[kSmallEN], yq[kSmallEN];
+          if (!qr) {
+#pragma unroll
+            for (int e = 0; e < kSmallEN; e += 2) {
+              const double2 u = *reinterpret_cast<const double2*>(vp + e);
+              const double2 v = *reinterpret_cast<const double2*>(vq + e);
+              yp[e] = u.x;
+              yp[e + 1] = u.y;
+              yq[e] = v.x;
+              yq[e + 1] = v.y;
+            }
+          }
+          double xp[EMX], xq[EMX];
+#pragma unroll
+          for (int e = 0; e < EMX; e += 2) {
+            const double2 u = *reinterpret_cast<const double2*>(wp + e);
+            const double2 v = *reinterpret_cast<const double2*>(wq + e);
+            xp[e] = u.x;
+            xp[e + 1] = u.y;
+            xq[e] = v.x;
+            xq[e + 1] = v.y;
+          }
+          double c = 0.0;
+#pragma unroll
+          for (int e = 0; e < EMX; ++e) c += xp[e] * xq[e];
+          c = group8_sum(c);
+          const double a = nrm[p], b = nrm[q];
+          if (c * c > tol * tol * (a * b) && fabs(c) > abs_tol) {
+            rotated = 1;
+            const double zeta = (b - a) * (0.5 * fast_rcp(c));
+            const double z2 = fma(zeta, zeta, 1.0);
+            const double tt = copysign(1.0, zeta) * fast_rcp(fabs(zeta) + z2 * fast_rsqrt(z2));
+            const double cs = fast_rsqrt(fma(tt, tt, 1.0));
+            const double sn = cs * tt;
+#pragma unroll
+            for (int e = 0; e < EMX; e += 2) {
+              *reinterpret_cast<double2*>(wp + e) =
+                  double2{cs * xp[e] - sn * xq[e], cs * xp[e + 1] - sn * xq[e + 1]};
+              *reinterpret_cast<double2*>(wq + e) =
+                  double2{sn * xp[e] + cs * xq[e], sn * xp[e + 1] + cs * xq[e + 1]};
+            }
+            if (!qr) {
+#pragma unroll
+              for (int e = 0; e < kSmallEN; e += 2) {
+                *reinterpret_cast<double2*>(vp + e) =
+                    double2{cs * yp[e] - sn * yq[e], cs * yp[e + 1] - sn * yq[e + 1]};
+                *reinterpret_cast<double2*>(vq + e) =
+                    double2{sn * yp[e] + cs * yq[e], sn * yp[e + 1] + cs * yq[e + 1]};
+              }
+            }
+            if (t == 0) {
+              nrm[p] = fmax(a - tt * c, 0.0);
+              nrm[q] = fmax(b + tt * c, 0.0);
+            }
+          }
+        }
+        __syncthreads();
+      }
+      if (!__syncthreads_or(rotated)) break;
+    }
+    return sweep;
+  };
+
+  bool qr = !tr && !want_u;
+  int total = 0;
+  bool nonconv = false;
+  if (qr) {
+    householder_rt();
+    exact_norms();
+    const int sw = sweeps(true);
+    total = sw + 1;
+    nonconv = sw >= 60;
+    exact_norms();
+    // a column at the underflow level cannot be normalised: plain sweeps
+    int tiny = 0;
+    for (int c = tid; c < n; c += kThreads) tiny |= !(nrm[c] >= 1e-270);
+    if (__syncthreads_or(tiny)) {
+      qr = false;
+      load_w();
+      exact_norms();
+    }
+  }
+  if (!qr) {
+    const int sw = sweeps(false);
+    total += sw + 1;
+    nonconv = sw >= 60;
+    exact_norms();
   }
   if (tid == 0) {
-    if (sweep >= max_sweeps) atomicAdd(status, 1);
-    atomicMax(status + 1, sweep + 1);
+    if (nonconv) atomicAdd(status, 1);
+    atomicMax(status + 1, total);
   }
 
   // singular values = exact column norms; descending rank (ties by index)
-  exact_norms();
   for (int j = tid; j < n; j += kThreads) {
     const double sj = nrm[j];
     int rk = 0;
@@ -700,15 +839,27 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
   double* Sb = S + pr.s_off;
   double* VTb = VT + pr.vt_off;
   for (int j = tid; j < n; j += kThreads) Sb[rank_of[j]] = sqrt(nrm[j]);
+  if (qr) {
+    // VT row rank(k) = (V0 P U_X)[:, k]: sum over QR positions i of
+    // V0[r][qr_col[i]] X[i][k] / sigma_k
+    for (int e = tid; e < n * n; e += kThreads) {
+      const int k = e / n, r = e % n;
+      double acc = 0.0;
+      for (int i = 0; i < n; ++i) acc += V[vidx(qr_col[i], r)] * W[widx(k, i)];
+      VTb[(int64_t)rank_of[k] * n + r] = acc / sqrt(nrm[k]);
+    }
+    return;
+  }
   for (int e = tid; e < m * n; e += kThreads) {
     const int j = e / m, i = e % m;
     const double sj = sqrt(nrm[j]);
     const double v = (sj > 0.0) ? W[widx(j, i)] / sj : 0.0;
     const int rk = rank_of[j];
-    if (!tr)
-      Ub[(int64_t)i * n + rk] = v;
-    else
+    if (!tr) {
+      if (want_u) Ub[(int64_t)i * n + rk] = v;
+    } else {
       VTb[(int64_t)rk * N + i] = v;
+    }
   }
   for (int e = tid; e < n * n; e += kThreads) {
     const int j = e / n, i = e % n;
@@ -716,7 +867,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
     const double vv = V[vidx(j, i)];
     if (!tr)
       VTb[(int64_t)rk * n + i] = vv;
-    else
+    else if (want_u)
       Ub[(int64_t)i * n + rk] = vv;
   }
 }
@@ -736,6 +887,7 @@ struct tn_svd_plan_s {
   tn::bj::Plan* blk = nullptr;  // block Jacobi: problems too large for LDS
   int* d_status = nullptr;      // [0] problems not converged, [1] max sweeps
   bool v0_any = false;          // some problem has a warm start
+  bool u_any = false;           // some problem writes U (u_off >= 0)
 };
 
 namespace {
@@ -777,6 +929,7 @@ int svd_plan_build(tn_svd_plan_s** out, int64_t batch, const int64_t* m, const i
     SvdProb p{m[i], n[i], a_off ? a_off[i] : 0, u_off ? u_off[i] : 0, s_off ? s_off[i] : 0,
               vt_off ? vt_off[i] : 0, v0_off ? v0_off[i] : -1};
     P->v0_any |= p.v0_off >= 0;
+    P->u_any |= p.u_off >= 0;
     const size_t bytes = sizeof(double) * (size_t)((mm + 1) * nn + (nn + 1) * nn);
     if (nn <= kSmallN && mm <= kSmallTP * 16) {
       small_probs.push_back(p);
@@ -898,18 +1051,18 @@ int tn_svd_last_sweeps(int* sweeps) {
 int tn_svd_f64(const double* A, int64_t M, int64_t N, double* U, double* S, double* VT,
                void* stream) {
   TN_INIT_OR_RETURN();
-  TN_REQUIRE(A && U && S && VT, TN_EINVAL, "tn_svd_f64: NULL buffer");
-  int64_t m = M, n = N, z = 0;
-  return svd_batched(1, &m, &n, A, &z, nullptr, nullptr, U, &z, S, &z, VT, &z, tn::stream_of(stream));
+  TN_REQUIRE(A && S && VT, TN_EINVAL, "tn_svd_f64: NULL buffer");
+  int64_t m = M, n = N, z = 0, uo = U ? 0 : -1;
+  return svd_batched(1, &m, &n, A, &z, nullptr, nullptr, U, &uo, S, &z, VT, &z, tn::stream_of(stream));
 }
 
 int tn_svd_f64_warm(const double* A, int64_t M, int64_t N, const double* V0T, double* U, double* S,
                     double* VT, void* stream) {
   TN_INIT_OR_RETURN();
-  TN_REQUIRE(A && U && S && VT && V0T, TN_EINVAL, "tn_svd_f64_warm: NULL buffer");
+  TN_REQUIRE(A && S && VT && V0T, TN_EINVAL, "tn_svd_f64_warm: NULL buffer");
   TN_REQUIRE(M >= N, TN_EINVAL, "tn_svd_f64_warm: needs M >= N (got %lld x %lld)", (long long)M, (long long)N);
-  int64_t m = M, n = N, z = 0;
-  return svd_batched(1, &m, &n, A, &z, V0T, &z, U, &z, S, &z, VT, &z, tn::stream_of(stream));
+  int64_t m = M, n = N, z = 0, uo = U ? 0 : -1;
+  return svd_batched(1, &m, &n, A, &z, V0T, &z, U, &uo, S, &z, VT, &z, tn::stream_of(stream));
 }
 
 int tn_svd_f64_batched(int64_t batch, const int64_t* m, const int64_t* n, const double* A,
@@ -939,7 +1092,7 @@ int tn_svd_plan_exec(tn_svd_plan P, const double* A, const double* V0T, double* 
   TN_INIT_OR_RETURN();
   TN_REQUIRE(P != nullptr, TN_EINVAL, "tn_svd_plan_exec: NULL plan");
   if (svd_plan_empty(P)) return TN_OK;
-  TN_REQUIRE(A && U && S && VT, TN_EINVAL, "tn_svd_plan_exec: NULL buffer");
+  TN_REQUIRE(A && S && VT && (U || !P->u_any), TN_EINVAL, "tn_svd_plan_exec: NULL buffer");
   bool needs_v0 = P->v0_any;
   TN_REQUIRE(!needs_v0 || V0T, TN_EINVAL, "tn_svd_plan_exec: the plan has warm starts but V0T is NULL");
   return svd_plan_launch(P, A, V0T, U, S, VT, tn::stream_of(stream));

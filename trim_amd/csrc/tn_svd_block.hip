@@ -597,10 +597,11 @@ __global__ void __launch_bounds__(kThreads) bj_vectors(const Prob* __restrict__ 
       const double sj = sqrt(sig[j]);
       const double v = (sj > 0.0) ? W[j * pr.mp + i] / sj : 0.0;
       const int rk = rank[j];
-      if (!pr.tr)
-        U[pr.u_off + i * n + rk] = v;        // U is M x n
-      else
+      if (!pr.tr) {
+        if (pr.u_off >= 0) U[pr.u_off + i * n + rk] = v;   // U is M x n
+      } else {
         VT[pr.vt_off + rk * pr.N + i] = v;   // VT is n x N
+      }
     } else {
       const int64_t e2 = e - totU;
       const int64_t j = e2 / n, i = e2 % n;
@@ -608,7 +609,7 @@ __global__ void __launch_bounds__(kThreads) bj_vectors(const Prob* __restrict__ 
       const double vv = V[j * pr.nvp + i];
       if (!pr.tr)
         VT[pr.vt_off + rk * n + i] = vv;     // VT is n x n
-      else
+      else if (pr.u_off >= 0)
         U[pr.u_off + i * n + rk] = vv;       // U is M x M
     }
   }

@@ -325,7 +325,6 @@ BatchTebd::BatchTebd(const std::vector<int>& chi, const std::vector<double>& del
       f.t1[0] = f.t1[1] = take(t);
       f.cb[0] = f.cb[1] = take(t);
       f.th[0] = f.th[1] = take(t);
-      f.u[0] = f.u[1] = take(D * D);
       f.s[0] = f.s[1] = take(D);
       f.vt[0] = take(D * D);
       f.vt[1] = take(D * D);
@@ -343,7 +342,6 @@ BatchTebd::BatchTebd(const std::vector<int>& chi, const std::vector<double>& del
         f.t1[p] = take(u.t1size[p]);
         f.cb[p] = take(u.sqsize[p]);
         f.th[p] = take(u.sqsize[p]);
-        f.u[p] = take(u.sqsize[p]);
         f.s[p] = take(u.svsize[p]);
         f.vt[p] = take(u.sqsize[p]);
         f.tmp[p] = take(u.tmpsize[p]);
@@ -382,10 +380,10 @@ BatchTebd::BatchTebd(const std::vector<int>& chi, const std::vector<double>& del
     gemm(&g2_[p], d2, "BatchTebd gemm2");
     gemm(&g3_[p], d3, "BatchTebd gemm3");
     const int64_t ns = (int64_t)svd[0].size();
-    detail::check(tn_svd_plan_create(&svd_cold_[p], ns, svd[0].data(), svd[1].data(), svd[2].data(), nullptr,
-                                     svd[3].data(), svd[4].data(), svd[5].data()),
-                  "BatchTebd svd");
-    detail::check(tn_svd_plan_create(&svd_warm_[p], ns, svd[0].data(), svd[1].data(), svd[2].data(), svd[6].data(),
+    // every SVD starts from the VT of the previous update on its bond, in
+    // place; a chain's start (and a reloaded slot) holds VT = I, so its first
+    // update is a cold start through the same launches
+    detail::check(tn_svd_plan_create(&svd_[p], ns, svd[0].data(), svd[1].data(), svd[2].data(), svd[6].data(),
                                      svd[3].data(), svd[4].data(), svd[5].data()),
                   "BatchTebd svd");
     auto vec = [&](tn_vecops_plan* v, const std::vector<tn_vecop>& items, const char* what) {
@@ -419,7 +417,7 @@ void BatchTebd::plans_dense(int i, int p, std::vector<tn_gemm_desc_ex>& d1, std:
   // Theta = LY[a] C[(a,s'),(b,t')]
   rows.push_back(tn_vecop{TN_VECOP_ROWSCALE, 0, D * D, f.cb[p], f.th[p], LY, c * kd * kd});
   // SVD of Theta (D x D) into U, S, VT[p]
-  const int64_t sv[7] = {D, D, f.th[p], f.u[p], f.s[p], f.vt[p], f.vt[p]};
+  const int64_t sv[7] = {D, D, f.th[p], -1, f.s[p], f.vt[p], f.vt[p]};   // U is not formed
   for (int k = 0; k < 7; ++k) svd[k].push_back(sv[k]);
   // LX = S[:c]/||S[:c]||, sc = {norm, discarded}; Y = VT[:c, :]
   trunc.push_back(tn_vecop{TN_VECOP_TRUNCATE, 0, D, f.s[p], LX, f.sc[p], c});
@@ -517,7 +515,7 @@ void BatchTebd::plans_u1(int i, int p, std::vector<tn_gemm_desc_ex>& d1, std::ve
       rows.push_back(tn_vecop{TN_VECOP_ROWSCALE, 0, (int64_t)L.n[a] * D, f.cb[p] + sq + r0, f.th[p] + sq + r0,
                               f.l[1 - p] + L.off[a], D});
     }
-    const int64_t sv[7] = {D, D, f.th[p] + sq, f.u[p] + sq, f.s[p] + u.sv[p][j], f.vt[p] + sq, f.vt[p] + sq};
+    const int64_t sv[7] = {D, D, f.th[p] + sq, -1, f.s[p] + u.sv[p][j], f.vt[p] + sq, f.vt[p] + sq};
     for (int e = 0; e < 7; ++e) svd[e].push_back(sv[e]);
     // kept singular values to the sector's place, the rest after all kept
     // ones; kept VT rows are the new block j of the other tensor
@@ -651,8 +649,7 @@ BatchTebd::~BatchTebd() {
     tn_gemm_plan_destroy(g1_[p]);
     tn_gemm_plan_destroy(g2_[p]);
     tn_gemm_plan_destroy(g3_[p]);
-    tn_svd_plan_destroy(svd_cold_[p]);
-    tn_svd_plan_destroy(svd_warm_[p]);
+    tn_svd_plan_destroy(svd_[p]);
     tn_vecops_plan_destroy(rows_[p]);
     tn_vecops_plan_destroy(gather_[p]);
     tn_vecops_plan_destroy(trunc_[p]);
@@ -661,12 +658,12 @@ BatchTebd::~BatchTebd() {
   tn_stream_destroy(stream_);
 }
 
-void BatchTebd::update(int p, bool warm) {
+void BatchTebd::update(int p) {
   double* A = arena_.get();
   detail::check(tn_gemm_exec(g1_[p], A, A, A, stream_), "BatchTebd gemm1");
   detail::check(tn_gemm_exec(g2_[p], A, A, A, stream_), "BatchTebd gemm2");
   detail::check(tn_vecops_plan_exec(rows_[p], A, stream_), "BatchTebd rows");
-  last_svd_ = warm ? svd_warm_[p] : svd_cold_[p];
+  last_svd_ = svd_[p];
   detail::check(tn_svd_plan_exec(last_svd_, A, A, A, A, A, stream_), "BatchTebd svd");
   detail::check(tn_vecops_plan_exec(gather_[p], A, stream_), "BatchTebd gather");
   detail::check(tn_vecops_plan_exec(trunc_[p], A, stream_), "BatchTebd trunc");
@@ -675,8 +672,8 @@ void BatchTebd::update(int p, bool warm) {
 }
 
 void BatchTebd::run_step() {
-  update(0, warm_[0]);
-  update(1, warm_[1]);
+  update(0);
+  update(1);
   warm_[0] = warm_[1] = true;
 }
 

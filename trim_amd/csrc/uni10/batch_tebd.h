@@ -64,7 +64,7 @@ class BatchTebd {
 
  private:
   struct Off {
-    int64_t x[2], l[2], g, t1[2], cb[2], th[2], u[2], s[2], vt[2], sc[2], tmp[2];
+    int64_t x[2], l[2], g, t1[2], cb[2], th[2], s[2], vt[2], sc[2], tmp[2];
   };
   // Block layout of a U(1) chain.  Tensor p (Gamma_p lambda_p) has left bond
   // 1 - p and right bond p and is stored by left-bond sector: block i holds
@@ -89,7 +89,7 @@ class BatchTebd {
   void plans_u1(int i, int p, std::vector<tn_gemm_desc_ex>& d1, std::vector<tn_gemm_desc_ex>& d2,
                 std::vector<tn_gemm_desc_ex>& d3, std::vector<tn_vecop>& rows, std::vector<tn_vecop>& gather,
                 std::vector<tn_vecop>& trunc, std::vector<tn_vecop>& scale, std::vector<int64_t>* svd);
-  void update(int p, bool warm);
+  void update(int p);
   void init_chain(int i, double delta, uint64_t seed, double* base);
   std::vector<double> gate(int i, double delta) const;
   std::vector<int64_t> region_;   // arena doubles of each chain (from its x[0])
@@ -102,7 +102,7 @@ class BatchTebd {
   DevBuf arena_;
   void* stream_ = nullptr;
   tn_gemm_plan g1_[2] = {nullptr, nullptr}, g2_[2] = {nullptr, nullptr}, g3_[2] = {nullptr, nullptr};
-  tn_svd_plan svd_cold_[2] = {nullptr, nullptr}, svd_warm_[2] = {nullptr, nullptr};
+  tn_svd_plan svd_[2] = {nullptr, nullptr};
   tn_vecops_plan rows_[2] = {nullptr, nullptr}, gather_[2] = {nullptr, nullptr}, trunc_[2] = {nullptr, nullptr},
                  scale_[2] = {nullptr, nullptr};
   tn_svd_plan last_svd_ = nullptr;

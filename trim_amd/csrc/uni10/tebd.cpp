@@ -91,24 +91,23 @@ ITebd::ITebd(int chi, int d, double delta, double tau, uint64_t seed, const std:
 // SVD of theta started from the right singular vectors of the previous
 // theta on the same bond (the two updates of a step alternate between two
 // bonds): successive thetas differ by one small imaginary-time step, so the
-// Jacobi sweeps start close to convergence.  Same factorisation as
-// Matrix::svd (U R x k, S k x k diagonal, VT k x C).
+// Jacobi sweeps start close to convergence.  The factors of Matrix::svd that
+// the update uses: S (k x k diagonal) and VT (k x C); U is not formed.
 static std::vector<Matrix> svd_warm(const Matrix& T, Matrix& vt_prev) {
   const size_t R = T.row(), C = T.col(), k = std::min(R, C);
   std::vector<Matrix> outs;
-  outs.emplace_back(R, k, false);
   outs.emplace_back(k, k, true);
   outs.emplace_back(k, C, false);
   if (k == 0) return outs;
   if (R >= C && vt_prev.row() == C && vt_prev.col() == C)
-    detail::check(tn_svd_f64_warm(T.getElem(), (int64_t)R, (int64_t)C, vt_prev.getElem(), outs[0].getElem(),
-                                  outs[1].getElem(), outs[2].getElem(), detail::stream()),
+    detail::check(tn_svd_f64_warm(T.getElem(), (int64_t)R, (int64_t)C, vt_prev.getElem(), nullptr,
+                                  outs[0].getElem(), outs[1].getElem(), detail::stream()),
                   "ITebd svd");
   else
-    detail::check(tn_svd_f64(T.getElem(), (int64_t)R, (int64_t)C, outs[0].getElem(), outs[1].getElem(),
-                             outs[2].getElem(), detail::stream()),
+    detail::check(tn_svd_f64(T.getElem(), (int64_t)R, (int64_t)C, nullptr, outs[0].getElem(),
+                             outs[1].getElem(), detail::stream()),
                   "ITebd svd");
-  vt_prev = outs[2];
+  vt_prev = outs[1];
   return outs;
 }
 
@@ -123,23 +122,23 @@ void ITebd::update(UniTensor& ALa, UniTensor& BLb, Matrix& La, Matrix& Lb) {
   Theta.permute(2);
   std::vector<Matrix> rets = svd_warm(Theta.getBlock(q0), vt_prev_[parity_]);
   parity_ ^= 1;
-  int dim = chi_ < (int)rets[1].row() ? chi_ : (int)rets[1].row();
+  int dim = chi_ < (int)rets[0].row() ? chi_ : (int)rets[0].row();
   // discarded weight (not computed by the reference; reported alongside)
   {
-    std::vector<double> s = rets[1].getHostElem();
+    std::vector<double> s = rets[0].getHostElem();
     double e = 0;
     for (size_t i = dim; i < s.size(); ++i) e += s[i] * s[i];
     trunc_err_ = e;
   }
-  double norm = rets[1].resize(dim, dim).norm();
-  rets[1] *= (1 / norm);
-  La = rets[1];
+  double norm = rets[0].resize(dim, dim).norm();
+  rets[0] *= (1 / norm);
+  La = rets[0];
   Bond bdi(BD_IN, dim);
   std::vector<Bond> bond3 = ALa.bond();
   bond3[0] = bdi;
   BLb.assign(bond3);
   Matrix blk = BLb.getBlock(q0);
-  blk.setElem(rets[2].getElem(), true);
+  blk.setElem(rets[1].getElem(), true);
   BLb.putBlock(q0, blk);
   updateA_.putTensor("BLb", &BLb);
   updateA_.putTensor("C", &C);
