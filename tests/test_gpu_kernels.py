@@ -434,8 +434,10 @@ def _check_s_vt(ora, a, s, vt, rank=None):
     assert np.max(np.abs(s[r:]), initial=0.0) <= 1e-12 * s0
     assert np.all(np.diff(s) <= 0)
     assert np.max(np.abs(vt @ vt.T - np.eye(k))) <= 1e-12
+    # columns orthogonal to the sweeps' tolerance, 10 n eps relative
     us = a @ vt.T
-    assert np.max(np.abs(us.T @ us - np.diag(s * s))) <= 1e-12 * s0 * s0
+    gtol = max(1e-12, 20 * k * np.finfo(float).eps)
+    assert np.max(np.abs(us.T @ us - np.diag(s * s))) <= gtol * s0 * s0
     if M >= N:
         assert np.linalg.norm(us @ vt - a) <= 1e-12 * max(np.linalg.norm(a), 1e-300)
     gaps = np.abs(np.diff(sr)) if k > 1 else np.array([np.inf])
@@ -448,37 +450,44 @@ def _check_s_vt(ora, a, s, vt, rank=None):
 
 
 @pytest.mark.parametrize("M,N", [(64, 64), (5, 4), (100, 64), (128, 33), (7, 1), (1, 1), (64, 63),
-                                 (33, 5), (20, 50)])
+                                 (33, 5), (20, 50), (96, 96), (200, 150), (300, 300), (700, 513),
+                                 (1030, 1030)])
 def test_svd_without_u(tn, ora, M, N):
-    """U not wanted: for M >= N (n <= 64) the kernel factorises A = Q R by
-    Householder reflections and sweeps R^T, reading VT off its normalised
-    columns; M < N takes the plain sweeps without writing U."""
+    """U not wanted: for M >= N the solver factorises A = Q R by Householder
+    reflections (in LDS for n <= 64; blocked, 32-column panels, for the
+    block path up to M = 1024) and sweeps R^T, reading VT off its normalised
+    columns; M < N, and M > 1024 on the block path, take the plain sweeps
+    without writing U."""
     _say("svd no U %dx%d" % (M, N))
     a = _rand(M * 1000 + N + 3, (M, N))
     s, vt = _svd_no_u(tn, a)
     _check_s_vt(ora, a, s, vt)
 
 
+@pytest.mark.parametrize("n", [64, 160])
 @pytest.mark.parametrize("case", ["rank20", "zero columns", "zero", "graded"])
-def test_svd_without_u_special(tn, ora, case):
+def test_svd_without_u_special(tn, ora, case, n):
     """Rank-deficient inputs on the QR-preconditioned path: noise columns are
     orthogonalised relative to their own norms (VT stays orthonormal); an
     exactly zero column cannot be normalised and sends the problem through
     the plain sweeps.  A spectrum over twelve decades keeps its small values
     to LAPACK's absolute accuracy."""
-    rng = np.random.default_rng(11)
+    _say("svd no U %s n=%d" % (case, n))
+    rng = np.random.default_rng(11 + n)
     r = None
     if case == "rank20":
-        a, r = rng.uniform(-1, 1, (64, 20)) @ rng.uniform(-1, 1, (20, 64)), 20
+        a, r = rng.uniform(-1, 1, (n, 20)) @ rng.uniform(-1, 1, (20, n)), 20
     elif case == "zero columns":
-        a, r = np.hstack([rng.uniform(-1, 1, (64, 40)), np.zeros((64, 24))]), 40
+        k = (5 * n) // 8
+        a, r = np.hstack([rng.uniform(-1, 1, (n, k)), np.zeros((n, n - k))]), k
     elif case == "zero":
-        a, r = np.zeros((16, 16)), 0
+        a, r = np.zeros((n // 4, n // 4)), 0
     else:
-        a = _orth(rng, 64) @ np.diag(10.0 ** -np.linspace(0, 12, 64)) @ _orth(rng, 64).T
+        a = _orth(rng, n) @ np.diag(10.0 ** -np.linspace(0, 12, n)) @ _orth(rng, n).T
     s, vt = _svd_no_u(tn, a)
     if r == 0:
-        assert np.all(s == 0) and np.max(np.abs(vt @ vt.T - np.eye(16))) <= 1e-12
+        k = a.shape[0]
+        assert np.all(s == 0) and np.max(np.abs(vt @ vt.T - np.eye(k))) <= 1e-12
         return
     _check_s_vt(ora, a, s, vt, rank=r)
 
@@ -504,6 +513,28 @@ def test_svd_without_u_warm_theta_like(tn, ora):
     plain_sweeps = _sweeps(tn)
     assert np.max(np.abs(S.to_host() - s)) <= 1e-12 * s[0]
     assert qr_sweeps < plain_sweeps, (qr_sweeps, plain_sweeps)
+
+
+@pytest.mark.parametrize("n", [64, 160])
+def test_svd_without_u_chained_warm_starts(tn, ora, n):
+    """Fifty warm starts in a row, each from the VT the previous call wrote
+    (in place, as the TEBD engines do): the rows stay orthonormal to
+    rounding (no drift from call to call) and S stays LAPACK's."""
+    _say("svd no U chained n=%d" % n)
+    rng = np.random.default_rng(3 + n)
+    lam = np.exp(-np.arange(n) / 6.0)
+    c = rng.uniform(-1, 1, (n, n))
+    VT = tn.DeviceArray.from_host(np.eye(n))
+    S = tn.DeviceArray((n,))
+    for k in range(50):
+        c = c + 1e-3 * rng.uniform(-1, 1, (n, n))
+        a = lam[:, None] * c
+        A = tn.DeviceArray.from_host(a)
+        tn.call("tn_svd_f64_warm", tn._p(A), n, n, tn._p(VT), None, tn._p(S), tn._p(VT), None)
+    vt, s = VT.to_host(), S.to_host()
+    sr = ora.svd(a)[1]
+    assert np.max(np.abs(s - sr)) <= 1e-12 * sr[0]
+    assert np.max(np.abs(vt @ vt.T - np.eye(n))) <= 1e-13
 
 
 def test_svd_plan_without_u(tn, ora):

@@ -538,6 +538,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
   __shared__ int rank_of[kSmallN];
   __shared__ int col_at[kSmallN];
   __shared__ int qr_col[kSmallN];
+  __shared__ double hh_xx[2];
   const SvdProb pr = probs[blockIdx.x];
   const int64_t M = pr.m, N = pr.n;
   const bool tr = (M < N);
@@ -553,28 +554,69 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
 
   // V = I (or V0 for a warm start, m >= n only); W = A or A^T (or A V0),
   // rows m..8*EMX zero
+  // (all global loads of a thread are issued before any is used)
   const bool warm = pr.v0_off >= 0 && !tr;
-  for (int e = tid; e < kSmallN * kSmallTP * kSmallEN; e += kThreads) {
-    const int c = e / (kSmallTP * kSmallEN), i = e % (kSmallTP * kSmallEN);
-    double v = (c == i && c < n) ? 1.0 : 0.0;
-    if (warm) v = (c < n && i < n) ? V0T[pr.v0_off + (int64_t)c * n + i] : 0.0;
-    V[vidx(c, i)] = v;
+  {
+    constexpr int kPerV = kSmallN * kSmallTP * kSmallEN / kThreads;
+    double vv[kPerV];
+#pragma unroll
+    for (int u = 0; u < kPerV; ++u) {
+      const int e = tid + u * kThreads, c = e / (kSmallTP * kSmallEN), i = e % (kSmallTP * kSmallEN);
+      vv[u] = (c == i && c < n) ? 1.0 : 0.0;
+      if (warm) vv[u] = (c < n && i < n) ? V0T[pr.v0_off + (int64_t)c * n + i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPerV; ++u) {
+      const int e = tid + u * kThreads;
+      V[vidx(e / (kSmallTP * kSmallEN), e % (kSmallTP * kSmallEN))] = vv[u];
+    }
   }
   if (warm) __syncthreads();
+  // W = A (A^T for M < N), read in the source's row order so that
+  // neighbouring lanes read neighbouring addresses; for a warm start W = A V0
+  // is then formed from LDS, each thread a block of EMX / 2 rows by 4 columns
   auto load_w = [&]() {
-    for (int e = tid; e < kSmallN * kRows; e += kThreads) {
-      const int c = e / kRows, i = e % kRows;
-      double x = 0.0;
-      if (c < n && i < m) {
-        if (warm) {
-          for (int k = 0; k < n; ++k) x += Ab[(int64_t)i * N + k] * V[vidx(c, k)];
-        } else {
-          x = tr ? Ab[(int64_t)c * N + i] : Ab[(int64_t)i * N + c];
-        }
-      }
-      W[widx(c, i)] = x;
+    constexpr int kPerA = kSmallN * kRows / kThreads;
+    double xa[kPerA];
+#pragma unroll
+    for (int u = 0; u < kPerA; ++u) {
+      const int e = tid + u * kThreads;
+      const int i = tr ? e % kRows : e / kSmallN;
+      const int c = tr ? e / kRows : e % kSmallN;
+      xa[u] = (c < n && i < m) ? (tr ? Ab[(int64_t)c * N + i] : Ab[(int64_t)i * N + c]) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPerA; ++u) {
+      const int e = tid + u * kThreads;
+      W[widx(tr ? e / kRows : e % kSmallN, tr ? e % kRows : e / kSmallN)] = xa[u];
     }
     __syncthreads();
+    if (warm) {
+      constexpr int RB = EMX / 2;
+      const int r0 = (tid / 16) * RB, c0 = (tid % 16) * 4;
+      double acc[RB][4];
+#pragma unroll
+      for (int u = 0; u < RB; ++u)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[u][j] = 0.0;
+      for (int k = 0; k < n; ++k) {
+        double a[RB], v[4];
+#pragma unroll
+        for (int u = 0; u < RB; ++u) a[u] = W[widx(k, r0 + u)];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = V[vidx(c0 + j, k)];
+#pragma unroll
+        for (int u = 0; u < RB; ++u)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[u][j] = fma(a[u], v[j], acc[u][j]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < RB; ++u)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) W[widx(c0 + j, r0 + u)] = acc[u][j];
+      __syncthreads();
+    }
   };
   load_w();
 
@@ -618,60 +660,115 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
     __syncthreads();
   };
 
-  // W = Q R by Householder reflections over the columns in the order
-  // qr_col (decreasing norm for a cold start, a pivot-free stand-in for
-  // column pivoting; the given order for a warm start, where A V0 is
-  // already ordered by the previous singular values), then X = R^T in the
-  // first n columns of W.  Step j: every group reads the pivot column (rows
-  // >= j), and the groups reflect the later columns, one column each.
+  // W = Q R by Householder reflections, then X = R^T in the first n columns
+  // of W.  Cold start: the columns are first moved into decreasing-norm
+  // order (a pivot-free stand-in for column pivoting); warm start: A V0 is
+  // already ordered by the previous singular values.  qr_col[j] is the
+  // column of A V0 at position j.  Step j: every group reads the pivot
+  // column (rows >= j) and reflects up to two later columns; the group that
+  // owns column j + 1 also leaves the squared norm of its rows > j for the
+  // next step.
   auto householder_rt = [&]() {
     if (warm) {
       for (int j = tid; j < n; j += kThreads) qr_col[j] = j;
     } else {
       sort_positions();
       for (int j = tid; j < n; j += kThreads) qr_col[j] = col_at[j];
+      constexpr int kPerW = kSmallN * kRows / kThreads;
+      double st[kPerW];
+#pragma unroll
+      for (int u = 0; u < kPerW; ++u) {
+        const int e = tid + u * kThreads, c = e / kRows, i = e % kRows;
+        st[u] = (c < n) ? W[widx(col_at[c], i)] : 0.0;
+      }
+      const double nv = (tid < n) ? nrm[col_at[tid]] : 0.0;
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < kPerW; ++u) {
+        const int e = tid + u * kThreads, c = e / kRows, i = e % kRows;
+        if (c < n) W[widx(c, i)] = st[u];
+      }
+      if (tid < n) nrm[tid] = nv;
     }
     __syncthreads();
+    if (tid == 0) hh_xx[0] = nrm[0];
+    __syncthreads();
+    constexpr int kGroups = kThreads / kSmallTP;
     for (int j = 0; j < n; ++j) {
-      const int cj = qr_col[j];
-      const double* xc = W + cj * CW + t * CHW;
-      double xr[EMX];
-      double xx = 0.0;
+      // every LDS read of the step first, then the two dot products, and the
+      // reflector's scalars (branch-free) while their reductions run
+      const double xx = hh_xx[j & 1];
+      const double xj = W[widx(j, j)];
+      const int k0 = j + 1 + g, k1 = k0 + kGroups;
+      const bool h0 = k0 < n, h1 = k1 < n;
+      const double* xc = W + j * CW + t * CHW;
+      double* wc0 = W + (h0 ? k0 : j) * CW + t * CHW;
+      double* wc1 = W + (h1 ? k1 : j) * CW + t * CHW;
+      double xr[EMX], w0[EMX], w1[EMX];
 #pragma unroll
-      for (int e = 0; e < EMX; ++e) {
-        xr[e] = (t * EMX + e >= j) ? xc[e] : 0.0;
-        xx += xr[e] * xr[e];
+      for (int e = 0; e < EMX; e += 2) {
+        const double2 u = *reinterpret_cast<const double2*>(xc + e);
+        const double2 v0 = *reinterpret_cast<const double2*>(wc0 + e);
+        const double2 v1 = *reinterpret_cast<const double2*>(wc1 + e);
+        xr[e] = u.x;
+        xr[e + 1] = u.y;
+        w0[e] = v0.x;
+        w0[e + 1] = v0.y;
+        w1[e] = v1.x;
+        w1[e + 1] = v1.y;
       }
-      xx = group8_sum(xx);
-      const double xj = W[widx(cj, j)];
-      const double alpha = -copysign(sqrt(xx), xj);
-      const double den = alpha * alpha - alpha * xj;   // v^T v / 2 for v = x - alpha e_j
-      for (int k = j + 1 + g; k < n; k += kThreads / kSmallTP) {
-        const int ck = qr_col[k];
-        double* wc = W + ck * CW + t * CHW;
-        double wr[EMX];
-        double xw = 0.0;
+      const double wj0 = W[widx(h0 ? k0 : j, j)];
+      const double wj1 = W[widx(h1 ? k1 : j, j)];
+#pragma unroll
+      for (int e = 0; e < EMX; ++e) xr[e] = (t * EMX + e >= j) ? xr[e] : 0.0;
+      double d0a = 0.0, d0b = 0.0, d1a = 0.0, d1b = 0.0;
+#pragma unroll
+      for (int e = 0; e < EMX; e += 2) {
+        d0a = fma(xr[e], w0[e], d0a);
+        d0b = fma(xr[e + 1], w0[e + 1], d0b);
+        d1a = fma(xr[e], w1[e], d1a);
+        d1b = fma(xr[e + 1], w1[e + 1], d1b);
+      }
+      double d0 = d0a + d0b, d1 = d1a + d1b;
+      d0 = group8_sum(d0);
+      d1 = group8_sum(d1);
+      const bool refl = xx > 0.0;
+      const double nrm_x = refl ? xx * fast_rsqrt(refl ? xx : 1.0) : 0.0;
+      const double alpha = -copysign(nrm_x, xj);
+      const double den = alpha * (alpha - xj);   // v^T v / 2 for v = x - alpha e_j
+      const double rden = refl ? fast_rcp(refl ? den : 1.0) : 0.0;
+      const double vj = xj - alpha;
+      if (h0) {
+        const double coef = (d0 - alpha * wj0) * rden;
+        double nx = 0.0;
 #pragma unroll
         for (int e = 0; e < EMX; ++e) {
-          wr[e] = wc[e];
-          xw += xr[e] * wr[e];
+          const int r = t * EMX + e;
+          if (r >= j) w0[e] -= coef * ((r == j) ? vj : xr[e]);
+          if (r > j) nx += w0[e] * w0[e];
         }
-        xw = group8_sum(xw);
-        const double wj = W[widx(ck, j)];
-        if (xx > 0.0) {
-          const double coef = (xw - alpha * wj) / den;
 #pragma unroll
-          for (int e = 0; e < EMX; ++e) {
-            const int r = t * EMX + e;
-            if (r >= j) wc[e] = wr[e] - coef * ((r == j) ? xj - alpha : xr[e]);
-          }
+        for (int e = 0; e < EMX; e += 2) *reinterpret_cast<double2*>(wc0 + e) = double2{w0[e], w0[e + 1]};
+        if (g == 0) {
+          nx = group8_sum(nx);
+          if (t == 0) hh_xx[(j + 1) & 1] = nx;
         }
       }
+      if (h1) {
+        const double coef = (d1 - alpha * wj1) * rden;
+#pragma unroll
+        for (int e = 0; e < EMX; ++e) {
+          const int r = t * EMX + e;
+          if (r >= j) w1[e] -= coef * ((r == j) ? vj : xr[e]);
+        }
+#pragma unroll
+        for (int e = 0; e < EMX; e += 2) *reinterpret_cast<double2*>(wc1 + e) = double2{w1[e], w1[e + 1]};
+      }
       __syncthreads();
-      // the pivot column becomes column j of R (no one reads it before the
+      // the pivot column becomes column j of R (read again only by the
       // transpose below)
       if (g == 0) {
-        double* pc = W + cj * CW + t * CHW;
+        double* pc = W + j * CW + t * CHW;
 #pragma unroll
         for (int e = 0; e < EMX; ++e) {
           const int r = t * EMX + e;
@@ -681,23 +778,23 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
       }
     }
     __syncthreads();
-    // X[i][k] = R[k][i] = W[k][qr_col[i]] (k <= i): staged in registers
+    // X[i][k] = R[k][i] = W[k][i] (k <= i): staged in registers
     constexpr int kPer = (kSmallN * kSmallN + kThreads - 1) / kThreads;
     double xv[kPer];
 #pragma unroll
-    for (int s = 0; s < kPer; ++s) {
-      const int e = tid + s * kThreads;
-      xv[s] = 0.0;
+    for (int u = 0; u < kPer; ++u) {
+      const int e = tid + u * kThreads;
+      xv[u] = 0.0;
       if (e < n * n) {
         const int i = e / n, k = e % n;
-        if (k <= i) xv[s] = W[widx(qr_col[i], k)];
+        if (k <= i) xv[u] = W[widx(i, k)];
       }
     }
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < kPer; ++s) {
-      const int e = tid + s * kThreads;
-      if (e < n * n) W[widx(e % n, e / n)] = xv[s];
+    for (int u = 0; u < kPer; ++u) {
+      const int e = tid + u * kThreads;
+      if (e < n * n) W[widx(e % n, e / n)] = xv[u];
     }
     for (int e = tid; e < n * (kRows - n); e += kThreads) W[widx(e / (kRows - n), n + e % (kRows - n))] = 0.0;
     __syncthreads();
@@ -714,16 +811,21 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
       if (sweep > 0) exact_norms();
       sort_positions();
       int rotated = 0;
+      // the pair of the next round is looked up before the barrier (the
+      // positions do not change within a sweep)
+      auto pair_of = [&](int r, int& p, int& q) {
+        if (g >= npairs) return false;
+        int pp, qq;
+        rr_pair_fast(r, g, np, pp, qq);
+        p = col_at[pp];
+        q = col_at[qq];
+        return p < n && q < n;
+      };
+      int pn = 0, qn = 0;
+      bool ln = pair_of(0, pn, qn);
       for (int r = 0; r < np - 1; ++r) {
-        int p = 0, q = 0;
-        bool live = g < npairs;
-        if (live) {
-          int pp, qq;
-          rr_pair_fast(r, g, np, pp, qq);
-          p = col_at[pp];
-          q = col_at[qq];
-          live = (p < n && q < n);
-        }
+        const int p = pn, q = qn;
+        const bool live = ln;
         if (live) {
           double* wp = W + p * CW + t * CHW;
           double* wq = W + q * CW + t * CHW;
@@ -753,16 +855,22 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
             xq[e] = v.x;
             xq[e + 1] = v.y;
           }
-          double c = 0.0;
+          double ca = 0.0, cb = 0.0;
 #pragma unroll
-          for (int e = 0; e < EMX; ++e) c += xp[e] * xq[e];
-          c = group8_sum(c);
+          for (int e = 0; e < EMX; e += 2) {
+            ca = fma(xp[e], xq[e], ca);
+            cb = fma(xp[e + 1], xq[e + 1], cb);
+          }
+          const double c = group8_sum(ca + cb);
           const double a = nrm[p], b = nrm[q];
           if (c * c > tol * tol * (a * b) && fabs(c) > abs_tol) {
             rotated = 1;
-            const double zeta = (b - a) * (0.5 * fast_rcp(c));
+            // the tangent from the raw reciprocal and square-root estimates
+            // (its error only moves the angle); c = 1 / sqrt(1 + t^2) refined,
+            // so that (c, c t) is a rotation to working accuracy
+            const double zeta = (b - a) * (0.5 * __builtin_amdgcn_rcp(c));
             const double z2 = fma(zeta, zeta, 1.0);
-            const double tt = copysign(1.0, zeta) * fast_rcp(fabs(zeta) + z2 * fast_rsqrt(z2));
+            const double tt = copysign(1.0, zeta) * __builtin_amdgcn_rcp(fabs(zeta) + z2 * __builtin_amdgcn_rsq(z2));
             const double cs = fast_rsqrt(fma(tt, tt, 1.0));
             const double sn = cs * tt;
 #pragma unroll
@@ -787,6 +895,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
             }
           }
         }
+        if (r + 1 < np - 1) ln = pair_of(r + 1, pn, qn);
         __syncthreads();
       }
       if (!__syncthreads_or(rotated)) break;
@@ -840,13 +949,99 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
   double* VTb = VT + pr.vt_off;
   for (int j = tid; j < n; j += kThreads) Sb[rank_of[j]] = sqrt(nrm[j]);
   if (qr) {
-    // VT row rank(k) = (V0 P U_X)[:, k]: sum over QR positions i of
-    // V0[r][qr_col[i]] X[i][k] / sigma_k
-    for (int e = tid; e < n * n; e += kThreads) {
-      const int k = e / n, r = e % n;
-      double acc = 0.0;
-      for (int i = 0; i < n; ++i) acc += V[vidx(qr_col[i], r)] * W[widx(k, i)];
-      VTb[(int64_t)rank_of[k] * n + r] = acc / sqrt(nrm[k]);
+    // P = U_X^T (V0 Pi)^T, row k = the sum over QR positions i of
+    // X[i][k] / sigma_k times V0[r][qr_col[i]]; then one Newton-Schulz step
+    // VT = (3 P - (P P^T) P) / 2, which makes the rows orthonormal to
+    // rounding (U_X is orthogonal only to the sweeps' tolerance, and a warm
+    // start would carry that error from one call to the next).  Each thread
+    // holds a 4 x 4 block (rows k0.., columns r0..) of each product.
+    const int r0 = (tid % 16) * 4, k0 = (tid / 16) * 4;
+    const bool mine = r0 < n && k0 < n;
+    double pk[4][4], acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) pk[a][b] = 0.0;
+    if (mine) {
+      for (int i = 0; i < n; ++i) {
+        const int ci = qr_col[i];
+        double x[4], v[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) x[a] = W[widx(k0 + a, i)];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) v[b] = V[vidx(ci, r0 + b)];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) pk[a][b] = fma(x[a], v[b], pk[a][b]);
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const double sk = (k0 + a < n) ? sqrt(nrm[k0 + a]) : 1.0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) pk[a][b] /= sk;
+      }
+    }
+    __syncthreads();
+    // P row k -> W column k
+    if (mine) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if (k0 + a < n && r0 + b < n) W[widx(k0 + a, r0 + b)] = pk[a][b];
+    }
+    __syncthreads();
+    // G = P P^T (rows k0.., columns l = r0..) -> V column k
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    if (mine) {
+      for (int r = 0; r < n; ++r) {
+        double x[4], y[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) x[a] = W[widx(k0 + a, r)];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) y[b] = W[widx(r0 + b, r)];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) acc[a][b] = fma(x[a], y[b], acc[a][b]);
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if (k0 + a < n && r0 + b < n) V[vidx(k0 + a, r0 + b)] = acc[a][b];
+    }
+    __syncthreads();
+    if (mine) {
+      // H = G P; VT row rank(k) = (3 P - H) / 2
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+      for (int l = 0; l < n; ++l) {
+        double gv[4], pv[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) gv[a] = V[vidx(k0 + a, l)];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) pv[b] = W[widx(l, r0 + b)];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) acc[a][b] = fma(gv[a], pv[b], acc[a][b]);
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int k = k0 + a;
+        if (k >= n) continue;
+        const int64_t row = (int64_t)rank_of[k] * n;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if (r0 + b < n) VTb[row + r0 + b] = 1.5 * pk[a][b] - 0.5 * acc[a][b];
+      }
     }
     return;
   }

@@ -19,6 +19,19 @@
 // dgesvd (uni10_lapack_cpu.cpp:289-313) or cuSOLVER gesvd
 // (uni10_lapack_gpu.cu:418-500).
 //
+// Problems that need only S and VT (no U; M >= N, M <= 1024) are first
+// factorised A V0 = Q R by a blocked Householder QR (32-column panels: one
+// workgroup factors a panel and builds its compact-WY factor T, many
+// workgroups apply I - Y T^T Y^T to the trailing columns), and the sweeps
+// run on X = R^T, whose column Gram matrix is close to diagonal (Drmac and
+// Veselic, SIAM J. Matrix Anal. Appl. 29 (2008)): TEBD thetas need about
+// half the sweeps.  No rotation is accumulated: from X = U_X S V_X^T,
+// A = (Q V_X) S (V0 U_X)^T, so VT = U_X^T V0^T (one GEMM at the end), with
+// U_X the normalised columns of X (an exactly zero column is completed to
+// an orthonormal basis by Gram-Schmidt).  These problems are orthogonalised
+// relative to their own column norms only (no absolute floor), which keeps
+// U_X orthonormal down to the rounding-level columns of a rank-deficient A.
+//
 // Convergence: a pair (p, q) is left alone when |w_p.w_q| is below
 // 10 m eps |w_p||w_q| (the rounding noise of a dot product of orthogonal
 // columns) or below m eps^2 ||A||_F^2 (columns at rounding level), the
@@ -50,6 +63,8 @@ constexpr int kInnerSweeps = 1;
 constexpr int kSetupBlocks = 64;
 constexpr int kNormBlocks = 16;
 constexpr int kMaxN = 8192;       // the sort keeps n norms in LDS (64 KiB)
+constexpr int kQrMaxM = 1024;     // QR panel rows held in registers (4 per thread)
+constexpr int kQrRows = kQrMaxM / kThreads;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -57,10 +72,13 @@ struct Prob {
   int64_t M, N;                           // caller's A: M x N row-major
   int64_t a_off, u_off, s_off, vt_off, v0_off;
   int64_t w_off, v_off, sig_off;          // scratch (doubles): W, V, column norms + ||A||_F^2
+  int64_t y_off, t_off, ut_off;           // QR: panel reflectors (mp x 32), T (32 x 32), U_X^T (n x n)
   int32_t m, n;                           // working W: m x n, m >= n
   int32_t mp, nvp, np, nb;                // padded rows of W / V, padded columns, blocks
   int32_t tr;                             // W = A^T
   int32_t flag_off, rank_off, pos_off;    // int scratch: rotated-in-sweep flags, ranks, sweep order
+  int32_t qr;                             // QR-preconditioned: sweeps on R^T, V holds V0
+  int32_t ms;                             // rows the sweeps touch (mp; n rounded up to 16 for QR)
 };
 
 __device__ inline void rr_pair(int r, int k, int np, int& p, int& q) {
@@ -204,7 +222,8 @@ __global__ void __launch_bounds__(kThreads) bj_colnorm(const Prob* __restrict__ 
   for (int j = gw; j < pr.n; j += nw) {
     const double* wj = W + (int64_t)j * pr.mp;
     double s = 0.0;
-    for (int i = lane; i < pr.m; i += 64) s += wj[i] * wj[i];
+    const int rows = pr.qr ? pr.n : pr.m;
+    for (int i = lane; i < rows; i += 64) s += wj[i] * wj[i];
     s = wave_sum(s);
     if (lane == 0) nrm[j] = s;
   }
@@ -292,7 +311,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
     const double* xp = W + myP * mp + 2 * lg;
     const double* xq = W + myQ * mp + 2 * lg;
     d4 aPP = {0.0, 0.0, 0.0, 0.0}, aPQ = aPP, aQQ = aPP;
-    const int ngr = (int)(mp >> 3);
+    const int ngr = pr.ms >> 3;
     auto ld2 = [](const double* p) { return *reinterpret_cast<const double2*>(p); };
     auto mm = [&](double p, double q) {
       aPP = __builtin_amdgcn_mfma_f64_16x16x4f64(p, p, aPP, 0, 0, 0);
@@ -379,8 +398,8 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
   }
   __syncthreads();
 
-  const double tol = 10.0 * DBL_EPSILON * (double)pr.m;
-  const double abs_tol = (double)pr.m * DBL_EPSILON * DBL_EPSILON * normF2;
+  const double tol = 10.0 * DBL_EPSILON * (double)(pr.qr ? pr.n : pr.m);
+  const double abs_tol = pr.qr ? 0.0 : (double)pr.m * DBL_EPSILON * DBL_EPSILON * normF2;
   int need = 0;
   for (int e = tid; e < kPW * kPW; e += kThreads) {
     const int i = e / kPW, j = e % kPW;
@@ -494,7 +513,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
     const int col = 4 * s + lg;
     gcol[s] = (col < kB) ? posP[col] : posQ[col - kB];
   }
-  const int tW = (int)(mp >> 4), tV = (int)(nvp >> 4);
+  const int tW = pr.ms >> 4, tV = pr.qr ? 0 : (int)(nvp >> 4);   // QR: V holds V0
   // tile t: rows [16t, 16t + 16) of W (t < tW) or of V
   auto tile = [&](int t, double*& base, int64_t& ld, int64_t& r0) {
     if (t < tW) {
@@ -547,6 +566,281 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
   }
 }
 
+// ---- QR preconditioning ------------------------------------------------
+// Panel `panel` (columns kb .. kb + nb, kb = 32 panel) of the Householder QR
+// of W, one workgroup per problem: thread t holds rows kb + t + 256u of the
+// panel in registers.  Step j: one block reduction gives x.a_c for every
+// panel column c, x = column j from row kb + j down; with alpha = -sign(x_j)
+// |x| the reflector is H_j = I - tau v v^T, v = x - alpha e_j, tau = 1 /
+// (alpha (alpha - x_j)), and v.a_c = x.a_c - alpha a_c[kb + j] gives both the
+// update of the later columns and, for the earlier reflectors y_c, the new
+// column of T (H_0 .. H_{nb-1} = I - Y T Y^T; T(0:j, j) = -tau_j T(0:j, 0:j)
+// Y(:, 0:j)^T v_j, the forward recurrence of LAPACK's dlarft).  Out: R in
+// the panel's columns of W (rows >= kb), Y (zero above its diagonal) and T.
+__global__ void __launch_bounds__(kThreads) bj_qr_panel(const Prob* __restrict__ probs, const int* __restrict__ qlist,
+                                                        double* __restrict__ scr, int panel) {
+  __shared__ double part[kPW][kThreads + 1];
+  __shared__ double part2[kPW][9];
+  __shared__ double res[kPW], row[kPW], rdiag[kPW];
+  __shared__ double Ts[kPW][kPW + 1];
+  const Prob pr = probs[qlist[blockIdx.x]];
+  const int kb = panel * kPW;
+  if (kb >= pr.n) return;
+  const int nb = min(kPW, pr.n - kb);
+  const int m = pr.m, tid = threadIdx.x;
+  const int64_t mp = pr.mp;
+  double* W = scr + pr.w_off;
+  for (int e = tid; e < kPW * kPW; e += kThreads) Ts[e / kPW][e % kPW] = 0.0;
+  double a[kQrRows][kPW];
+#pragma unroll
+  for (int u = 0; u < kQrRows; ++u) {
+    const int r = kb + tid + kThreads * u;
+#pragma unroll
+    for (int c = 0; c < kPW; ++c) a[u][c] = (r < m && c < nb) ? W[(int64_t)(kb + c) * mp + r] : 0.0;
+  }
+  for (int j = 0; j < nb; ++j) {
+    const int rj = kb + j;
+    double x[kQrRows];
+#pragma unroll
+    for (int u = 0; u < kQrRows; ++u) {
+      const int r = kb + tid + kThreads * u;
+      double v = 0.0;
+#pragma unroll
+      for (int c = 0; c < kPW; ++c) v = (c == j) ? a[u][c] : v;
+      x[u] = (r >= rj) ? v : 0.0;
+    }
+    if (tid == j) {   // row rj is row 0 of thread j
+#pragma unroll
+      for (int c = 0; c < kPW; ++c) row[c] = a[0][c];
+    }
+#pragma unroll
+    for (int c = 0; c < kPW; ++c) {
+      double d = 0.0;
+#pragma unroll
+      for (int u = 0; u < kQrRows; ++u) d = fma(x[u], a[u][c], d);
+      part[c][tid] = d;
+    }
+    __syncthreads();
+    {
+      const int c = tid >> 3, s0 = (tid & 7) * (kThreads / 8);
+      double acc = 0.0;
+      for (int i = 0; i < kThreads / 8; ++i) acc += part[c][s0 + i];
+      part2[c][tid & 7] = acc;
+    }
+    __syncthreads();
+    if (tid < kPW) {
+      double acc = 0.0;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) acc += part2[tid][i];
+      res[tid] = acc;
+    }
+    __syncthreads();
+    const double xx = res[j], xj = row[j];
+    const bool refl = xx > 0.0;
+    const double alpha = refl ? -copysign(sqrt(xx), xj) : xj;
+    const double tau = refl ? 1.0 / (alpha * (alpha - xj)) : 0.0;
+    const double vj = xj - alpha;
+#pragma unroll
+    for (int c = 0; c < kPW; ++c) {
+      if (c <= j || c >= nb) continue;
+      const double coef = tau * (res[c] - alpha * row[c]);
+#pragma unroll
+      for (int u = 0; u < kQrRows; ++u) {
+        const int r = kb + tid + kThreads * u;
+        if (r >= rj) a[u][c] -= coef * ((r == rj) ? vj : x[u]);
+      }
+    }
+    // column j becomes the reflector (rows >= rj)
+#pragma unroll
+    for (int u = 0; u < kQrRows; ++u) {
+      const int r = kb + tid + kThreads * u;
+#pragma unroll
+      for (int c = 0; c < kPW; ++c)
+        if (c == j && r == rj) a[u][c] = vj;
+    }
+    if (tid < j) {
+      double sum = 0.0;
+      for (int l = tid; l < j; ++l) sum += Ts[tid][l] * (res[l] - alpha * row[l]);
+      Ts[tid][j] = -tau * sum;
+    }
+    if (tid == j) {
+      Ts[j][j] = tau;
+      rdiag[j] = alpha;
+    }
+    __syncthreads();
+  }
+  double* Y = scr + pr.y_off;
+#pragma unroll
+  for (int u = 0; u < kQrRows; ++u) {
+    const int r = kb + tid + kThreads * u;
+    if (r >= pr.mp) continue;
+    const int i = r - kb;
+#pragma unroll
+    for (int c = 0; c < kPW; ++c) {
+      if (c >= nb) continue;
+      W[(int64_t)(kb + c) * mp + r] = (c > i) ? a[u][c] : (c == i ? rdiag[c] : 0.0);
+      Y[(int64_t)c * mp + r] = (i >= c) ? a[u][c] : 0.0;
+    }
+  }
+  for (int e = tid; e < kPW * kPW; e += kThreads) scr[pr.t_off + e] = Ts[e / kPW][e % kPW];
+}
+
+// Trailing update after panel `panel`: workgroup (tile, problem) applies
+// Q_panel^T = I - Y T^T Y^T to the 32 columns c0 .. c0 + 32 of W (rows >= kb):
+// Z = Y^T C (rows in 64-row chunks through LDS, a fixed order), Z2 = T^T Z,
+// C -= Y Z2.
+__global__ void __launch_bounds__(kThreads) bj_qr_update(const Prob* __restrict__ probs, const int* __restrict__ qlist,
+                                                         double* __restrict__ scr, int panel) {
+  __shared__ double Ys[64][kPW + 1], Cs[64][kPW + 1];
+  __shared__ double Zs[kPW][kPW + 1], Z2[kPW][kPW + 1], Tt[kPW][kPW + 1];
+  const Prob pr = probs[qlist[blockIdx.y]];
+  const int kb = panel * kPW;
+  const int c0 = kb + kPW + kPW * (int)blockIdx.x;
+  if (c0 >= pr.n) return;
+  const int m = pr.m, tid = threadIdx.x;
+  const int64_t mp = pr.mp;
+  double* W = scr + pr.w_off;
+  const double* Y = scr + pr.y_off;
+  for (int e = tid; e < kPW * kPW; e += kThreads) Tt[e / kPW][e % kPW] = scr[pr.t_off + e];
+  const int zi = tid >> 3, zc = (tid & 7) * 4;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int r0 = kb; r0 < m; r0 += 64) {
+    for (int e = tid; e < 64 * kPW; e += kThreads) {
+      const int c = e / 64, rr = e % 64, r = r0 + rr;
+      Ys[rr][c] = (r < m) ? Y[(int64_t)c * mp + r] : 0.0;
+      Cs[rr][c] = (r < m) ? W[(int64_t)(c0 + c) * mp + r] : 0.0;
+    }
+    __syncthreads();
+    for (int rr = 0; rr < 64; ++rr) {
+      const double y = Ys[rr][zi];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = fma(y, Cs[rr][zc + q], acc[q]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) Zs[zi][zc + q] = acc[q];
+  __syncthreads();
+  // Z2 = T^T Z (T upper triangular)
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double z = 0.0;
+    for (int l = 0; l <= zi; ++l) z = fma(Tt[l][zi], Zs[l][zc + q], z);
+    acc[q] = z;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) Z2[zi][zc + q] = acc[q];
+  __syncthreads();
+  // C -= Y Z2: thread (row rr, 8 columns)
+  const int rr = tid & 63, cg = (tid >> 6) * 8;
+  for (int r = kb + rr; r < m; r += 64) {
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = W[(int64_t)(c0 + cg + q) * mp + r];
+#pragma unroll 4
+    for (int i = 0; i < kPW; ++i) {
+      const double yi = Y[(int64_t)i * mp + r];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = fma(-yi, Z2[i][cg + q], v[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) W[(int64_t)(c0 + cg + q) * mp + r] = v[q];
+  }
+}
+
+// X = R^T in place: R's strict upper triangle moves to the lower one.
+__global__ void __launch_bounds__(kThreads) bj_qr_transpose(const Prob* __restrict__ probs,
+                                                            const int* __restrict__ qlist, double* __restrict__ scr) {
+  const Prob pr = probs[qlist[blockIdx.y]];
+  double* W = scr + pr.w_off;
+  const int64_t n = pr.n, mp = pr.mp;
+  const int64_t stride = (int64_t)gridDim.x * kThreads;
+  for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < n * n; e += stride) {
+    const int64_t i = e / n, k = e % n;
+    if (k >= i) continue;
+    const double v = W[i * mp + k];   // R[k][i]
+    W[k * mp + i] = v;                // X[i][k]
+    W[i * mp + k] = 0.0;
+  }
+}
+
+// QR problems: U_X^T in rank order, row k = column of X / sigma (zero for
+// sigma = 0, completed below).
+__global__ void __launch_bounds__(kThreads) bj_qr_ut(const Prob* __restrict__ probs, const int* __restrict__ qlist,
+                                                     double* __restrict__ scr, const int* __restrict__ iscr) {
+  const Prob pr = probs[qlist[blockIdx.y]];
+  const double* W = scr + pr.w_off;
+  const double* sig = scr + pr.sig_off;
+  const int* rank = iscr + pr.rank_off;
+  double* Ut = scr + pr.ut_off;
+  const int64_t n = pr.n;
+  const int64_t stride = (int64_t)gridDim.x * kThreads;
+  for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < n * n; e += stride) {
+    const int64_t k = e / n, i = e % n;
+    const double sk = sqrt(sig[k]);
+    Ut[rank[k] * n + i] = (sk > 0.0) ? W[k * pr.mp + i] / sk : 0.0;
+  }
+}
+
+// Rows of U_X^T whose singular value is zero (an exactly rank-deficient A)
+// are completed to an orthonormal basis: for each, the first unit vector e_i
+// that keeps more than half its length after two Gram-Schmidt passes
+// against the rows already orthonormal.  One workgroup per problem; returns
+// at once when every singular value is positive.
+__global__ void __launch_bounds__(kThreads) bj_qr_complete(const Prob* __restrict__ probs,
+                                                           const int* __restrict__ qlist, double* __restrict__ scr,
+                                                           const double* __restrict__ S) {
+  __shared__ double u[kQrMaxM], d[kQrMaxM], red[kThreads];
+  const Prob pr = probs[qlist[blockIdx.x]];
+  const int n = pr.n, tid = threadIdx.x;
+  double* Ut = scr + pr.ut_off;
+  int good = n;   // rows [0, good) are orthonormal (S is descending)
+  while (good > 0 && !(S[pr.s_off + good - 1] > 0.0)) --good;
+  for (int k = good; k < n; ++k) {
+    for (int cand = 0; cand < n; ++cand) {
+      for (int i = tid; i < n; i += kThreads) u[i] = (i == cand) ? 1.0 : 0.0;
+      __syncthreads();
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int j = tid; j < k; j += kThreads) {
+          double s = 0.0;
+          for (int i = 0; i < n; ++i) s = fma(Ut[(int64_t)j * n + i], u[i], s);
+          d[j] = s;
+        }
+        __syncthreads();
+        for (int i = tid; i < n; i += kThreads) {
+          double v = u[i];
+          for (int j = 0; j < k; ++j) v = fma(-d[j], Ut[(int64_t)j * n + i], v);
+          u[i] = v;
+        }
+        __syncthreads();
+      }
+      double s = 0.0;
+      for (int i = tid; i < n; i += kThreads) s = fma(u[i], u[i], s);
+      s = block_sum(s, red);
+      if (s > 0.25) {
+        const double inv = 1.0 / sqrt(s);
+        for (int i = tid; i < n; i += kThreads) Ut[(int64_t)k * n + i] = u[i] * inv;
+        __syncthreads();
+        break;
+      }
+    }
+  }
+}
+
+// The Newton-Schulz step's last part: VT = (3 VT - H) / 2, H = (VT VT^T) VT
+// (in W's storage, leading dimension n).  U_X is orthogonal only to the
+// sweeps' tolerance; this makes VT's rows orthonormal to rounding, so a warm
+// start does not carry the error from one call to the next.
+__global__ void __launch_bounds__(kThreads) bj_qr_polish(const Prob* __restrict__ probs, const int* __restrict__ qlist,
+                                                         const double* __restrict__ scr, double* __restrict__ VT) {
+  const Prob pr = probs[qlist[blockIdx.y]];
+  const double* H = scr + pr.w_off;
+  const int64_t nn = (int64_t)pr.n * pr.n;
+  const int64_t stride = (int64_t)gridDim.x * kThreads;
+  for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < nn; e += stride)
+    VT[pr.vt_off + e] = 1.5 * VT[pr.vt_off + e] - 0.5 * H[e];
+}
+
 // Singular values (square roots of the exact column norms, descending; ties
 // by column index), their ranks, and the convergence status.
 __global__ void __launch_bounds__(kThreads) bj_sigma(const Prob* __restrict__ probs, double* __restrict__ scr,
@@ -584,6 +878,7 @@ __global__ void __launch_bounds__(kThreads) bj_vectors(const Prob* __restrict__ 
                                                        const int* __restrict__ iscr, double* __restrict__ U,
                                                        double* __restrict__ VT) {
   const Prob pr = probs[blockIdx.y];
+  if (pr.qr) return;   // VT by the U_X^T V0^T product
   const double* W = scr + pr.w_off;
   const double* V = scr + pr.v_off;
   const double* sig = scr + pr.sig_off;
@@ -627,6 +922,11 @@ struct Plan {
   double* d_scr = nullptr;
   int* d_iscr = nullptr;
   tn_gemm_plan warm = nullptr;    // W = A V0 of the warm-started problems
+  int* d_qlist = nullptr;         // QR-preconditioned problems
+  int nqr = 0, max_qr_n = 0;
+  tn_gemm_plan qr_vt = nullptr;   // their VT = U_X^T V0^T
+  tn_gemm_plan qr_g = nullptr;    // G = VT VT^T
+  tn_gemm_plan qr_h = nullptr;    // H = G VT (then VT = (3 VT - H) / 2)
 };
 
 static int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
@@ -634,6 +934,10 @@ static int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 void destroy(Plan* P) {
   if (!P) return;
   if (P->warm) tn_gemm_plan_destroy(P->warm);
+  if (P->qr_vt) tn_gemm_plan_destroy(P->qr_vt);
+  if (P->qr_g) tn_gemm_plan_destroy(P->qr_g);
+  if (P->qr_h) tn_gemm_plan_destroy(P->qr_h);
+  tn::pool_free(P->d_qlist);
   tn::pool_free(P->d_probs);
   tn::pool_free(P->d_map);
   tn::pool_free(P->d_scr);
@@ -647,7 +951,8 @@ int build(Plan** out, const std::vector<Spec>& specs) {
   auto* P = new Plan();
   int64_t scr = 0, iscr = kMaxSweeps;
   std::vector<int2> map;
-  std::vector<tn_gemm_desc_ex> gd;
+  std::vector<tn_gemm_desc_ex> gd, qd, qg, qh;
+  std::vector<int> qlist;
   for (size_t i = 0; i < specs.size(); ++i) {
     const Spec& s = specs[i];
     Prob p{};
@@ -689,6 +994,58 @@ int build(Plan** out, const std::vector<Spec>& specs) {
     iscr += n;
     p.pos_off = (int32_t)iscr;
     iscr += np;
+    // only S and VT wanted: QR-preconditioned sweeps (W held by a panel
+    // workgroup's registers up to kQrMaxM rows)
+    p.qr = (s.u_off < 0 && !p.tr && m <= kQrMaxM) ? 1 : 0;
+    p.ms = p.qr ? (int32_t)round_up(n, 16) : p.mp;
+    p.y_off = p.t_off = p.ut_off = -1;
+    if (p.qr) {
+      p.y_off = scr;
+      scr += mp * kPW;
+      p.t_off = scr;
+      scr += kPW * kPW;
+      p.ut_off = scr;
+      scr += n * n;
+      qlist.push_back((int)i);
+      P->max_qr_n = std::max(P->max_qr_n, (int)n);
+      // VT(k, r) = sum_i Ut(k, i) V0T(i, r), V0T kept in V (row stride nvp)
+      tn_gemm_desc_ex g{};
+      g.M = n;
+      g.N = n;
+      g.K = n;
+      g.a_off = p.ut_off;
+      g.lda = n;
+      g.b_off = p.v_off;
+      g.ldb = nvp;
+      g.c_off = s.vt_off;
+      g.ldc = n;
+      qd.push_back(g);
+      // Newton-Schulz step on VT: G = VT VT^T into Ut's storage, H = G VT
+      // into W's (both free by then)
+      tn_gemm_desc_ex h{};
+      h.M = n;
+      h.N = n;
+      h.K = n;
+      h.a_off = s.vt_off;
+      h.lda = n;
+      h.b_off = s.vt_off;
+      h.b_row = tn_index_map{1, 0, {n, 0, 0, 0}, {1, 0, 0, 0}};
+      h.b_col = tn_index_map{1, 0, {n, 0, 0, 0}, {n, 0, 0, 0}};
+      h.c_off = p.ut_off;
+      h.ldc = n;
+      qg.push_back(h);
+      tn_gemm_desc_ex f{};
+      f.M = n;
+      f.N = n;
+      f.K = n;
+      f.a_off = p.ut_off;
+      f.lda = n;
+      f.b_off = s.vt_off;
+      f.ldb = n;
+      f.c_off = p.w_off;
+      f.ldc = n;
+      qh.push_back(f);
+    }
     for (int k = 0; k < p.nb / 2; ++k) map.push_back(make_int2((int)i, k));
     P->max_nb = std::max(P->max_nb, p.nb);
     P->lds = std::max(P->lds, sizeof(double) * (size_t)n);
@@ -724,6 +1081,12 @@ int build(Plan** out, const std::vector<Spec>& specs) {
   if (rc == TN_OK) rc = tn_memcpy_h2d(P->d_probs, P->probs.data(), sizeof(Prob) * P->probs.size(), nullptr);
   if (rc == TN_OK) rc = tn_memcpy_h2d(P->d_map, map.data(), sizeof(int2) * map.size(), nullptr);
   if (rc == TN_OK && !gd.empty()) rc = tn_gemm_plan_create_ex(&P->warm, TN_GEMM_F64, (int64_t)gd.size(), gd.data());
+  P->nqr = (int)qlist.size();
+  if (rc == TN_OK && P->nqr > 0) rc = tn::pool_alloc((void**)&P->d_qlist, sizeof(int) * qlist.size());
+  if (rc == TN_OK && P->nqr > 0) rc = tn_memcpy_h2d(P->d_qlist, qlist.data(), sizeof(int) * qlist.size(), nullptr);
+  if (rc == TN_OK && P->nqr > 0) rc = tn_gemm_plan_create_ex(&P->qr_vt, TN_GEMM_F64, (int64_t)qd.size(), qd.data());
+  if (rc == TN_OK && P->nqr > 0) rc = tn_gemm_plan_create_ex(&P->qr_g, TN_GEMM_F64, (int64_t)qg.size(), qg.data());
+  if (rc == TN_OK && P->nqr > 0) rc = tn_gemm_plan_create_ex(&P->qr_h, TN_GEMM_F64, (int64_t)qh.size(), qh.data());
   if (rc != TN_OK) {
     destroy(P);
     return rc;
@@ -742,6 +1105,23 @@ int launch(Plan* P, const double* A, const double* V0T, double* U, double* S, do
   if (P->warm) {
     const int rc = tn_gemm_exec(P->warm, A, V0T, P->d_scr, st);
     if (rc != TN_OK) return rc;
+  }
+  if (P->nqr > 0) {
+    const unsigned nq = (unsigned)P->nqr;
+    const int panels = (P->max_qr_n + kPW - 1) / kPW;
+    for (int p = 0; p < panels; ++p) {
+      hipLaunchKernelGGL(bj_qr_panel, dim3(nq), dim3(kThreads), 0, st, P->d_probs, P->d_qlist, P->d_scr, p);
+      TN_LAUNCH_CHECK();
+      const int tiles = (P->max_qr_n - kPW * (p + 1) + kPW - 1) / kPW;
+      if (tiles > 0) {
+        hipLaunchKernelGGL(bj_qr_update, dim3((unsigned)tiles, nq), dim3(kThreads), 0, st, P->d_probs, P->d_qlist,
+                           P->d_scr, p);
+        TN_LAUNCH_CHECK();
+      }
+    }
+    hipLaunchKernelGGL(bj_qr_transpose, dim3(kSetupBlocks, nq), dim3(kThreads), 0, st, P->d_probs, P->d_qlist,
+                       P->d_scr);
+    TN_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(bj_colnorm, dim3(kNormBlocks, np), dim3(kThreads), 0, st, P->d_probs, P->d_scr);
   TN_LAUNCH_CHECK();
@@ -777,6 +1157,21 @@ int launch(Plan* P, const double* A, const double* V0T, double* U, double* S, do
   hipLaunchKernelGGL(bj_vectors, dim3(kSetupBlocks, np), dim3(kThreads), 0, st, P->d_probs, P->d_scr, P->d_iscr, U,
                      VT);
   TN_LAUNCH_CHECK();
+  if (P->nqr > 0) {
+    const unsigned nq = (unsigned)P->nqr;
+    hipLaunchKernelGGL(bj_qr_ut, dim3(kSetupBlocks, nq), dim3(kThreads), 0, st, P->d_probs, P->d_qlist, P->d_scr,
+                       P->d_iscr);
+    TN_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bj_qr_complete, dim3(nq), dim3(kThreads), 0, st, P->d_probs, P->d_qlist, P->d_scr, S);
+    TN_LAUNCH_CHECK();
+    int rc = tn_gemm_exec(P->qr_vt, P->d_scr, P->d_scr, VT, st);
+    if (rc == TN_OK) rc = tn_gemm_exec(P->qr_g, VT, VT, P->d_scr, st);
+    if (rc == TN_OK) rc = tn_gemm_exec(P->qr_h, P->d_scr, VT, P->d_scr, st);
+    if (rc != TN_OK) return rc;
+    hipLaunchKernelGGL(bj_qr_polish, dim3(kSetupBlocks, nq), dim3(kThreads), 0, st, P->d_probs, P->d_qlist, P->d_scr,
+                       VT);
+    TN_LAUNCH_CHECK();
+  }
   return TN_OK;
 }
 
