@@ -449,6 +449,7 @@ unsigned grid_for(int64_t n) {
 // group hit disjoint banks.  The squared column norms are kept in LDS and
 // updated with each rotation (a' = a - t c, b' = b + t c), so a round needs
 // one dot product per pair; they are recomputed exactly at every sweep.
+typedef double d4s __attribute__((ext_vector_type(4)));
 constexpr int kSmallN = 64;      // columns
 constexpr int kSmallTP = 8;      // lanes per pair
 constexpr int kSmallEN = 8;      // V chunk per lane (n <= 64)
@@ -528,7 +529,11 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
                                                              double* __restrict__ VT,
                                                              int* __restrict__ status) {
   constexpr int CHW = EMX + 2;                   // W chunk stride (doubles)
-  constexpr int CW = kSmallTP * CHW + 2;         // W column stride
+  // W column stride: 8 CHW doubles, 16 mod 32, so that two neighbouring
+  // columns sit half the LDS banks apart (with the columns in sweep order,
+  // the groups of a round read neighbouring columns)
+  constexpr int CW = kSmallTP * CHW;
+  static_assert(CW % 32 == 16, "column stride must be 16 mod 32 doubles");
   constexpr int CHV = kSmallEN + 2;
   constexpr int CV = kSmallTP * CHV + 2;
   constexpr int kRows = kSmallTP * EMX;          // W rows held (m <= kRows)
@@ -660,6 +665,28 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
     __syncthreads();
   };
 
+  // Moves column col_at[j] (and its norm) to position j, so that position =
+  // column for the rest of the sweep; ends with col_at = identity.
+  auto permute_sorted = [&]() {
+    constexpr int kPerW = kSmallN * kRows / kThreads;
+    double st[kPerW];
+#pragma unroll
+    for (int u = 0; u < kPerW; ++u) {
+      const int e = tid + u * kThreads, c = e / kRows, i = e % kRows;
+      st[u] = (c < n) ? W[widx(col_at[c], i)] : 0.0;
+    }
+    const double nv = (tid < n) ? nrm[col_at[tid]] : 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kPerW; ++u) {
+      const int e = tid + u * kThreads, c = e / kRows, i = e % kRows;
+      if (c < n) W[widx(c, i)] = st[u];
+    }
+    if (tid < n) nrm[tid] = nv;
+    for (int j = tid; j < np; j += kThreads) col_at[j] = j;
+    __syncthreads();
+  };
+
   // W = Q R by Householder reflections, then X = R^T in the first n columns
   // of W.  Cold start: the columns are first moved into decreasing-norm
   // order (a pivot-free stand-in for column pivoting); warm start: A V0 is
@@ -674,21 +701,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
     } else {
       sort_positions();
       for (int j = tid; j < n; j += kThreads) qr_col[j] = col_at[j];
-      constexpr int kPerW = kSmallN * kRows / kThreads;
-      double st[kPerW];
-#pragma unroll
-      for (int u = 0; u < kPerW; ++u) {
-        const int e = tid + u * kThreads, c = e / kRows, i = e % kRows;
-        st[u] = (c < n) ? W[widx(col_at[c], i)] : 0.0;
-      }
-      const double nv = (tid < n) ? nrm[col_at[tid]] : 0.0;
-      __syncthreads();
-#pragma unroll
-      for (int u = 0; u < kPerW; ++u) {
-        const int e = tid + u * kThreads, c = e / kRows, i = e % kRows;
-        if (c < n) W[widx(c, i)] = st[u];
-      }
-      if (tid < n) nrm[tid] = nv;
+      permute_sorted();
     }
     __syncthreads();
     if (tid == 0) hh_xx[0] = nrm[0];
@@ -810,6 +823,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
     for (; sweep < max_sweeps; ++sweep) {
       if (sweep > 0) exact_norms();
       sort_positions();
+      if (qr) permute_sorted();   // no V to carry along
       int rotated = 0;
       // the pair of the next round is looked up before the barrier (the
       // positions do not change within a sweep)
@@ -953,96 +967,60 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
     // X[i][k] / sigma_k times V0[r][qr_col[i]]; then one Newton-Schulz step
     // VT = (3 P - (P P^T) P) / 2, which makes the rows orthonormal to
     // rounding (U_X is orthogonal only to the sweeps' tolerance, and a warm
-    // start would carry that error from one call to the next).  Each thread
-    // holds a 4 x 4 block (rows k0.., columns r0..) of each product.
-    const int r0 = (tid % 16) * 4, k0 = (tid / 16) * 4;
-    const bool mine = r0 < n && k0 < n;
-    double pk[4][4], acc[4][4];
+    // start would carry that error from one call to the next).  The three
+    // D x D products run on the FP64 matrix cores from LDS: wave w owns a
+    // (D/2)-square quadrant of 16 x 16 tiles; lane l supplies A[l%16][l/16]
+    // and B[l/16][l%16], accumulator element v is C[l/16 + 4v][l%16].
+    constexpr int D = (kRows >= 64) ? 64 : 32;
+    constexpr int NT = (D == 64) ? 4 : 1;
+    const int w = tid >> 6, l = tid & 63, lr = l & 15, lk = l >> 4;
+    const int kmax = (n + 3) & ~3;
+    auto tile_r = [&](int tt) { return (D == 64) ? ((w >> 1) * 32 + 16 * (tt >> 1)) : 16 * (w >> 1); };
+    auto tile_c = [&](int tt) { return (D == 64) ? ((w & 1) * 32 + 16 * (tt & 1)) : 16 * (w & 1); };
+    auto mm = [&](auto a_at, auto b_at, d4s(&acc)[NT]) {
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+      for (int tt = 0; tt < NT; ++tt) acc[tt] = d4s{0.0, 0.0, 0.0, 0.0};
+      for (int kk = 0; kk < kmax; kk += 4) {
+        const int k = kk + lk;
 #pragma unroll
-      for (int b = 0; b < 4; ++b) pk[a][b] = 0.0;
-    if (mine) {
-      for (int i = 0; i < n; ++i) {
-        const int ci = qr_col[i];
-        double x[4], v[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) x[a] = W[widx(k0 + a, i)];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) v[b] = V[vidx(ci, r0 + b)];
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int b = 0; b < 4; ++b) pk[a][b] = fma(x[a], v[b], pk[a][b]);
+        for (int tt = 0; tt < NT; ++tt)
+          acc[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_at(tile_r(tt) + lr, k), b_at(k, tile_c(tt) + lr), acc[tt],
+                                                         0, 0, 0);
       }
+    };
+    d4s pk[NT], hk[NT];
+    mm([&](int k, int i) { return W[widx(k, i)]; },
+       [&](int i, int r) { return (i < n && r < n) ? V[vidx(qr_col[i], r)] : 0.0; }, pk);
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const double sk = (k0 + a < n) ? sqrt(nrm[k0 + a]) : 1.0;
+    for (int tt = 0; tt < NT; ++tt)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) pk[a][b] /= sk;
+      for (int v = 0; v < 4; ++v) {
+        const int k = tile_r(tt) + lk + 4 * v;
+        pk[tt][v] = (k < n) ? pk[tt][v] / sqrt(nrm[k]) : 0.0;
       }
-    }
     __syncthreads();
     // P row k -> W column k
-    if (mine) {
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+    for (int tt = 0; tt < NT; ++tt)
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
-          if (k0 + a < n && r0 + b < n) W[widx(k0 + a, r0 + b)] = pk[a][b];
-    }
+      for (int v = 0; v < 4; ++v) W[widx(tile_r(tt) + lk + 4 * v, tile_c(tt) + lr)] = pk[tt][v];
     __syncthreads();
-    // G = P P^T (rows k0.., columns l = r0..) -> V column k
+    // G = P P^T -> V column k
+    mm([&](int k, int r) { return W[widx(k, r)]; }, [&](int r, int c) { return W[widx(c, r)]; }, hk);
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int tt = 0; tt < NT; ++tt)
 #pragma unroll
-      for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-    if (mine) {
-      for (int r = 0; r < n; ++r) {
-        double x[4], y[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) x[a] = W[widx(k0 + a, r)];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) y[b] = W[widx(r0 + b, r)];
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int b = 0; b < 4; ++b) acc[a][b] = fma(x[a], y[b], acc[a][b]);
-      }
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-          if (k0 + a < n && r0 + b < n) V[vidx(k0 + a, r0 + b)] = acc[a][b];
-    }
+      for (int v = 0; v < 4; ++v) V[vidx(tile_r(tt) + lk + 4 * v, tile_c(tt) + lr)] = hk[tt][v];
     __syncthreads();
-    if (mine) {
-      // H = G P; VT row rank(k) = (3 P - H) / 2
+    // H = G P; VT row rank(k) = (3 P - H) / 2
+    mm([&](int k, int c) { return V[vidx(k, c)]; }, [&](int c, int r) { return W[widx(c, r)]; }, hk);
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+    for (int tt = 0; tt < NT; ++tt)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-      for (int l = 0; l < n; ++l) {
-        double gv[4], pv[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) gv[a] = V[vidx(k0 + a, l)];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) pv[b] = W[widx(l, r0 + b)];
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int b = 0; b < 4; ++b) acc[a][b] = fma(gv[a], pv[b], acc[a][b]);
+      for (int v = 0; v < 4; ++v) {
+        const int k = tile_r(tt) + lk + 4 * v, r = tile_c(tt) + lr;
+        if (k < n && r < n) VTb[(int64_t)rank_of[k] * n + r] = 1.5 * pk[tt][v] - 0.5 * hk[tt][v];
       }
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const int k = k0 + a;
-        if (k >= n) continue;
-        const int64_t row = (int64_t)rank_of[k] * n;
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-          if (r0 + b < n) VTb[row + r0 + b] = 1.5 * pk[a][b] - 0.5 * acc[a][b];
-      }
-    }
     return;
   }
   for (int e = tid; e < m * n; e += kThreads) {

@@ -122,17 +122,15 @@ void ITebd::update(UniTensor& ALa, UniTensor& BLb, Matrix& La, Matrix& Lb) {
   Theta.permute(2);
   std::vector<Matrix> rets = svd_warm(Theta.getBlock(q0), vt_prev_[parity_]);
   parity_ ^= 1;
-  int dim = chi_ < (int)rets[0].row() ? chi_ : (int)rets[0].row();
-  // discarded weight (not computed by the reference; reported alongside)
-  {
-    std::vector<double> s = rets[0].getHostElem();
-    double e = 0;
-    for (size_t i = dim; i < s.size(); ++i) e += s[i] * s[i];
-    trunc_err_ = e;
-  }
-  double norm = rets[0].resize(dim, dim).norm();
-  rets[0] *= (1 / norm);
-  La = rets[0];
+  const int k = (int)rets[0].row();
+  const int dim = chi_ < k ? chi_ : k;
+  // lambda = S[:dim] / ||S[:dim]|| (iTEBD.cpp:157-161), the norm and the
+  // discarded weight stay on the device: the update never waits for the host
+  Matrix lam(dim, dim, true);
+  detail::check(tn_truncate_f64(rets[0].getElem(), k, chi_, lam.getElem(), scal_.get(), scal_.get() + 1,
+                                detail::stream()),
+                "ITebd truncate");
+  La = lam;
   Bond bdi(BD_IN, dim);
   std::vector<Bond> bond3 = ALa.bond();
   bond3[0] = bdi;
@@ -143,7 +141,14 @@ void ITebd::update(UniTensor& ALa, UniTensor& BLb, Matrix& La, Matrix& Lb) {
   updateA_.putTensor("BLb", &BLb);
   updateA_.putTensor("C", &C);
   ALa = updateA_.launch();
-  ALa *= (1 / norm);
+  detail::check(tn_scale_inv_dev_f64(ALa.getElem(), (int64_t)ALa.elemNum(), scal_.get(), detail::stream()),
+                "ITebd scale");
+}
+
+double ITebd::truncation_error() const {
+  double e = 0;
+  detail::check(tn_memcpy_d2h(&e, scal_.get() + 1, sizeof(double), detail::stream()), "ITebd trunc_err");
+  return e;
 }
 
 void ITebd::step() {

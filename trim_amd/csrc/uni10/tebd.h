@@ -21,7 +21,7 @@ class ITebd {
   ITebd(int chi, int d, double delta, double tau, uint64_t seed, const std::string& netdir);
   void step();                 // update(A,B) then update(B,A)
   double energy();             // mean of the two bond energies
-  double truncation_error() const { return trunc_err_; }
+  double truncation_error() const;   // discarded weight of the last update
   const Matrix& La() const { return La_; }
   const Matrix& Lb() const { return Lb_; }
   UniTensor& ALa() { return ALa_; }
@@ -31,7 +31,7 @@ class ITebd {
   void update(UniTensor& ALa, UniTensor& BLb, Matrix& La, Matrix& Lb);
   int chi_, d_;
   double tau_;
-  double trunc_err_ = 0;
+  DevBuf scal_{2, true};        // ||S[:dim]|| and the discarded weight of the last update
   Matrix vt_prev_[2];          // last VT of each bond's theta (warm starts)
   int parity_ = 0;
   UniTensor H_, U_, ALa_, BLb_;
