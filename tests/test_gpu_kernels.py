@@ -227,7 +227,7 @@ def test_gemm_single_512(tn, mode):
     _check_gemm(tn, mode, [(512, 512, 512)])
 
 
-@pytest.mark.parametrize("mode,tile", [(m, t) for m in (0, 1, 2, 3) for t in range(6 if m < 2 else 5)])
+@pytest.mark.parametrize("mode,tile", [(m, t) for m in (0, 1, 2, 3) for t in range(8 if m < 2 else 5)])
 def test_gemm_grouped_uneven(tn, mode, tile):
     shapes = [(1, 1, 1), (17, 33, 5), (64, 64, 64), (130, 70, 200), (3, 200, 1), (200, 3, 300),
               (0, 5, 5), (5, 0, 5), (5, 5, 0), (96, 128, 17)]
@@ -244,7 +244,7 @@ def test_gemm_tile_choice_is_bitwise_neutral(tn, mode):
     plan = tn.GemmPlan(mode, np.array(descs, dtype=tn.GEMM_DESC))
     dA, dB = tn.DeviceArray.from_host(A), tn.DeviceArray.from_host(B)
     outs = []
-    for tile in range(6 if mode < 2 else 5):   # FP64/FP32 have a sixth, 512-thread tile
+    for tile in range(8 if mode < 2 else 5):   # FP64/FP32 add three 512-thread tiles
         plan.set_tile(tile)
         dC = tn.DeviceArray((nc,))
         plan(dA, dB, dC)

@@ -518,14 +518,17 @@ struct TileCfg {
   int bm, bn;
 };
 // FP64/FP32: five 256-thread shapes and the 32x32 tile with 512 threads
-// (id 5, two waves per sub-tile); split-BF16: five shapes.
-constexpr int kMaxCfg = 6;
-constexpr TileCfg kCfgMfma[kMaxCfg] = {{32, 32}, {64, 64}, {128, 64}, {64, 128}, {64, 32}, {32, 32}};
-constexpr TileCfg kCfgBf16[kMaxCfg] = {{32, 32}, {64, 64}, {128, 64}, {64, 128}, {128, 128}, {128, 128}};
+// (two waves per sub-tile) at stage depths 32 (id 5), 64 (id 6) and 128
+// (id 7): a deeper stage means fewer barriers per K and a longer prefetch
+// distance in time; split-BF16: five shapes.
+constexpr int kMaxCfg = 8;
+constexpr TileCfg kCfgMfma[kMaxCfg] = {{32, 32}, {64, 64}, {128, 64}, {64, 128}, {64, 32}, {32, 32}, {32, 32}, {32, 32}};
+constexpr TileCfg kCfgBf16[kMaxCfg] = {{32, 32}, {64, 64}, {128, 64}, {64, 128}, {128, 128}, {128, 128}, {128, 128},
+                                       {128, 128}};
 
 bool mfma_mode(int mode) { return mode == TN_GEMM_F64 || mode == TN_GEMM_F32; }
-int num_cfg(int mode) { return mfma_mode(mode) ? 6 : 5; }
-int cfg_threads(int mode, int cfg) { return (mfma_mode(mode) && cfg == 5) ? 512 : 256; }
+int num_cfg(int mode) { return mfma_mode(mode) ? 8 : 5; }
+int cfg_threads(int mode, int cfg) { return (mfma_mode(mode) && cfg >= 5) ? 512 : 256; }
 
 TileCfg cfg_shape(int mode, int cfg) { return mfma_mode(mode) ? kCfgMfma[cfg] : kCfgBf16[cfg]; }
 
@@ -540,6 +543,8 @@ KernFn pick_mfma(int cfg) {
     case 3: return gemm_mfma_k32<T, 64, 128, 32, 2, VEC, KT>;
     case 4: return gemm_mfma_k32<T, 64, 32, 64, 2, VEC, KT>;
     case 5: return gemm_mfma_k32<T, 32, 32, 32, 3, VEC, KT, 512>;
+    case 6: return gemm_mfma_k32<T, 32, 32, 64, 4, VEC, KT, 512>;
+    case 7: return gemm_mfma_k32<T, 32, 32, 128, 2, VEC, KT, 512>;
   }
   return nullptr;
 }
@@ -725,7 +730,8 @@ static int default_cfg(const tn_gemm_plan_s* P) {
     return x.bm * x.bn > y.bm * y.bn || (x.bm * x.bn == y.bm * y.bn && a < b);
   });
   const int64_t want = tn::cu_count();
-  for (int c : order) {
+  for (int i = 0; i < nc; ++i) {
+    const int c = order[i];
     const TileCfg sh = cfg_shape(P->mode, c);
     int64_t t = 0;
     for (const auto& g : P->groups)
