@@ -360,7 +360,6 @@ BatchTebd::BatchTebd(const std::vector<int>& chi, const std::vector<double>& del
     region_.push_back(end - off_[i].x[0]);
     init_chain((int)i, delta[i], seed[i], host.data() + off_[i].x[0]);
   }
-  detail::check(tn_memcpy_h2d(arena_.get(), host.data(), sizeof(double) * (size_t)o, stream_), "BatchTebd upload");
 
   // plans, one set per update parity p (p = 0: X = ALa, Y = BLb; p = 1: swapped)
   for (int p = 0; p < 2; ++p) {
@@ -379,6 +378,13 @@ BatchTebd::BatchTebd(const std::vector<int>& chi, const std::vector<double>& del
     gemm(&g1_[p], d1, "BatchTebd gemm1");
     gemm(&g2_[p], d2, "BatchTebd gemm2");
     gemm(&g3_[p], d3, "BatchTebd gemm3");
+    // tile shapes picked by timing each plan on the (still zero) arena; every
+    // shape gives the same bits, so the choice never changes a result
+    for (tn_gemm_plan g : {g1_[p], g2_[p], g3_[p]}) {
+      float ms = 0;
+      double* A = arena_.get();
+      detail::check(tn_gemm_plan_autotune(g, A, A, A, 2, stream_, &ms), "BatchTebd autotune");
+    }
     const int64_t ns = (int64_t)svd[0].size();
     // every SVD starts from the VT of the previous update on its bond, in
     // place; a chain's start (and a reloaded slot) holds VT = I, so its first
@@ -394,6 +400,7 @@ BatchTebd::BatchTebd(const std::vector<int>& chi, const std::vector<double>& del
     vec(&trunc_[p], trunc, "BatchTebd trunc");
     vec(&scale_[p], scale, "BatchTebd scale");
   }
+  detail::check(tn_memcpy_h2d(arena_.get(), host.data(), sizeof(double) * (size_t)o, stream_), "BatchTebd upload");
 }
 
 void BatchTebd::plans_dense(int i, int p, std::vector<tn_gemm_desc_ex>& d1, std::vector<tn_gemm_desc_ex>& d2,
