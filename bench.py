@@ -43,7 +43,7 @@ F64_MFMA_PEAK_TFLOPS = 78.6
 DIMS = (256, 2, 2, 256)
 # FP64 GEMM tile shapes by autotuner id: BM x BN block tile, KC k per stage
 F64_TILES = {0: "32x32,kc32", 1: "64x64,kc64", 2: "128x64,kc32", 3: "64x128,kc32", 4: "64x32,kc64",
-             5: "32x32,kc32,512 threads"}
+             5: "32x32,kc32,512 threads", 6: "32x32,kc64,512 threads", 7: "32x32,kc128,512 threads"}
 
 
 def splitmix_uniform(seed, n):
@@ -201,9 +201,7 @@ def recorded_traffic(kernel):
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("kernel") != kernel:
-        return None
-    return t.get("bytes_per_launch")
+    return t.get("kernels", {}).get(kernel)
 
 
 def c3_u1(tn, u10, steps=10):
