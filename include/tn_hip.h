@@ -184,7 +184,8 @@ int tn_diag_col_mul_f64(double* mat, const double* diag, int64_t M, int64_t N, v
  * M x N matrix A: U (M x k), S (k, descending), VT (k x N), k = min(M, N).
  * One-sided Jacobi; singular vectors are unique up to a sign per pair.
  * U may be NULL (only S and VT are formed; for M >= N and min(M, N) <= 64
- * this takes the faster QR-preconditioned sweeps). */
+ * this takes the faster QR-preconditioned sweeps, as does the block path
+ * for M >= N, M <= 2048). */
 int tn_svd_f64(const double* A, int64_t M, int64_t N, double* U, double* S,
                double* VT, void* stream);
 /* Batched: problem i reads A + a_off[i] (m[i] x n[i]) and writes

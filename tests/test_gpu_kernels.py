@@ -434,9 +434,10 @@ def _check_s_vt(ora, a, s, vt, rank=None):
     assert np.max(np.abs(s[r:]), initial=0.0) <= 1e-12 * s0
     assert np.all(np.diff(s) <= 0)
     assert np.max(np.abs(vt @ vt.T - np.eye(k))) <= 1e-12
-    # columns orthogonal to the sweeps' tolerance, 10 n eps relative
+    # columns orthogonal to the sweeps' tolerance: 10 eps times the rows of
+    # the swept matrix (n for the QR-preconditioned path, max(M, N) otherwise)
     us = a @ vt.T
-    gtol = max(1e-12, 20 * k * np.finfo(float).eps)
+    gtol = max(1e-12, 20 * max(M, N) * np.finfo(float).eps)
     assert np.max(np.abs(us.T @ us - np.diag(s * s))) <= gtol * s0 * s0
     if M >= N:
         assert np.linalg.norm(us @ vt - a) <= 1e-12 * max(np.linalg.norm(a), 1e-300)
@@ -451,13 +452,13 @@ def _check_s_vt(ora, a, s, vt, rank=None):
 
 @pytest.mark.parametrize("M,N", [(64, 64), (5, 4), (100, 64), (128, 33), (7, 1), (1, 1), (64, 63),
                                  (33, 5), (20, 50), (96, 96), (200, 150), (300, 300), (700, 513),
-                                 (1030, 1030)])
+                                 (1030, 1030), (1500, 1100), (2100, 300)])
 def test_svd_without_u(tn, ora, M, N):
     """U not wanted: for M >= N the solver factorises A = Q R by Householder
-    reflections (in LDS for n <= 64; blocked, 32-column panels, for the
-    block path up to M = 1024) and sweeps R^T, reading VT off its normalised
-    columns; M < N, and M > 1024 on the block path, take the plain sweeps
-    without writing U."""
+    reflections (in LDS for n <= 64; blocked for the block path: 32-column
+    panels up to M = 1024, 16-column ones up to 2048) and sweeps R^T, reading
+    VT off its normalised columns; M < N, and M > 2048 on the block path,
+    take the plain sweeps without writing U."""
     _say("svd no U %dx%d" % (M, N))
     a = _rand(M * 1000 + N + 3, (M, N))
     s, vt = _svd_no_u(tn, a)

@@ -19,8 +19,9 @@
 // dgesvd (uni10_lapack_cpu.cpp:289-313) or cuSOLVER gesvd
 // (uni10_lapack_gpu.cu:418-500).
 //
-// Problems that need only S and VT (no U; M >= N, M <= 1024) are first
-// factorised A V0 = Q R by a blocked Householder QR (32-column panels: one
+// Problems that need only S and VT (no U; M >= N, M <= 2048) are first
+// factorised A V0 = Q R by a blocked Householder QR (32-column panels, 16
+// beyond 1024 rows: one
 // workgroup factors a panel and builds its compact-WY factor T, many
 // workgroups apply I - Y T^T Y^T to the trailing columns), and the sweeps
 // run on X = R^T, whose column Gram matrix is close to diagonal (Drmac and
@@ -63,8 +64,10 @@ constexpr int kInnerSweeps = 1;
 constexpr int kSetupBlocks = 64;
 constexpr int kNormBlocks = 16;
 constexpr int kMaxN = 8192;       // the sort keeps n norms in LDS (64 KiB)
-constexpr int kQrMaxM = 1024;     // QR panel rows held in registers (4 per thread)
-constexpr int kQrRows = kQrMaxM / kThreads;
+// QR panels are held in registers, 128 doubles per thread: 32 columns x 4
+// rows (M <= 1024) or 16 columns x 8 rows (M <= 2048)
+constexpr int kQrMaxM = 2048;
+constexpr int kQrWideMaxM = 1024;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -78,6 +81,7 @@ struct Prob {
   int32_t tr;                             // W = A^T
   int32_t flag_off, rank_off, pos_off;    // int scratch: rotated-in-sweep flags, ranks, sweep order
   int32_t qr;                             // QR-preconditioned: sweeps on R^T, V holds V0
+  int32_t qpw;                            // its QR panel width (32, or 16 beyond 1024 rows)
   int32_t ms;                             // rows the sweeps touch (mp; n rounded up to 16 for QR)
 };
 
@@ -577,61 +581,63 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 // column of T (H_0 .. H_{nb-1} = I - Y T Y^T; T(0:j, j) = -tau_j T(0:j, 0:j)
 // Y(:, 0:j)^T v_j, the forward recurrence of LAPACK's dlarft).  Out: R in
 // the panel's columns of W (rows >= kb), Y (zero above its diagonal) and T.
+template <int PW, int ROWS>
 __global__ void __launch_bounds__(kThreads) bj_qr_panel(const Prob* __restrict__ probs, const int* __restrict__ qlist,
                                                         double* __restrict__ scr, int panel) {
-  __shared__ double part[kPW][kThreads + 1];
-  __shared__ double part2[kPW][9];
-  __shared__ double res[kPW], row[kPW], rdiag[kPW];
-  __shared__ double Ts[kPW][kPW + 1];
+  constexpr int TPC = kThreads / PW;   // threads summing one column's partials
+  __shared__ double part[PW][kThreads + 1];
+  __shared__ double part2[PW][TPC + 1];
+  __shared__ double res[PW], row[PW], rdiag[PW];
+  __shared__ double Ts[PW][PW + 1];
   const Prob pr = probs[qlist[blockIdx.x]];
-  const int kb = panel * kPW;
+  const int kb = panel * PW;
   if (kb >= pr.n) return;
-  const int nb = min(kPW, pr.n - kb);
+  const int nb = min(PW, pr.n - kb);
   const int m = pr.m, tid = threadIdx.x;
   const int64_t mp = pr.mp;
   double* W = scr + pr.w_off;
-  for (int e = tid; e < kPW * kPW; e += kThreads) Ts[e / kPW][e % kPW] = 0.0;
-  double a[kQrRows][kPW];
+  for (int e = tid; e < PW * PW; e += kThreads) Ts[e / PW][e % PW] = 0.0;
+  double a[ROWS][PW];
 #pragma unroll
-  for (int u = 0; u < kQrRows; ++u) {
+  for (int u = 0; u < ROWS; ++u) {
     const int r = kb + tid + kThreads * u;
 #pragma unroll
-    for (int c = 0; c < kPW; ++c) a[u][c] = (r < m && c < nb) ? W[(int64_t)(kb + c) * mp + r] : 0.0;
+    for (int c = 0; c < PW; ++c) a[u][c] = (r < m && c < nb) ? W[(int64_t)(kb + c) * mp + r] : 0.0;
   }
   for (int j = 0; j < nb; ++j) {
     const int rj = kb + j;
-    double x[kQrRows];
+    double x[ROWS];
 #pragma unroll
-    for (int u = 0; u < kQrRows; ++u) {
+    for (int u = 0; u < ROWS; ++u) {
       const int r = kb + tid + kThreads * u;
       double v = 0.0;
 #pragma unroll
-      for (int c = 0; c < kPW; ++c) v = (c == j) ? a[u][c] : v;
+      for (int c = 0; c < PW; ++c) v = (c == j) ? a[u][c] : v;
       x[u] = (r >= rj) ? v : 0.0;
     }
     if (tid == j) {   // row rj is row 0 of thread j
 #pragma unroll
-      for (int c = 0; c < kPW; ++c) row[c] = a[0][c];
+      for (int c = 0; c < PW; ++c) row[c] = a[0][c];
     }
 #pragma unroll
-    for (int c = 0; c < kPW; ++c) {
+    for (int c = 0; c < PW; ++c) {
       double d = 0.0;
 #pragma unroll
-      for (int u = 0; u < kQrRows; ++u) d = fma(x[u], a[u][c], d);
+      for (int u = 0; u < ROWS; ++u) d = fma(x[u], a[u][c], d);
       part[c][tid] = d;
     }
     __syncthreads();
     {
-      const int c = tid >> 3, s0 = (tid & 7) * (kThreads / 8);
+      const int c = tid / TPC, s0 = (tid % TPC) * (kThreads / TPC);
       double acc = 0.0;
-      for (int i = 0; i < kThreads / 8; ++i) acc += part[c][s0 + i];
-      part2[c][tid & 7] = acc;
+      for (int i = 0; i < kThreads / TPC; ++i) acc += part[c][s0 + i];
+      part2[c][tid % TPC] = acc;
     }
     __syncthreads();
-    if (tid < kPW) {
+    if (tid < PW) {
       double acc = 0.0;
 #pragma unroll
-      for (int i = 0; i < 8; ++i) acc += part2[tid][i];
+      for (int i = 0; i < TPC; ++i) acc += part2[tid][i];
       res[tid] = acc;
     }
     __syncthreads();
@@ -641,21 +647,21 @@ __global__ void __launch_bounds__(kThreads) bj_qr_panel(const Prob* __restrict__
     const double tau = refl ? 1.0 / (alpha * (alpha - xj)) : 0.0;
     const double vj = xj - alpha;
 #pragma unroll
-    for (int c = 0; c < kPW; ++c) {
+    for (int c = 0; c < PW; ++c) {
       if (c <= j || c >= nb) continue;
       const double coef = tau * (res[c] - alpha * row[c]);
 #pragma unroll
-      for (int u = 0; u < kQrRows; ++u) {
+      for (int u = 0; u < ROWS; ++u) {
         const int r = kb + tid + kThreads * u;
         if (r >= rj) a[u][c] -= coef * ((r == rj) ? vj : x[u]);
       }
     }
     // column j becomes the reflector (rows >= rj)
 #pragma unroll
-    for (int u = 0; u < kQrRows; ++u) {
+    for (int u = 0; u < ROWS; ++u) {
       const int r = kb + tid + kThreads * u;
 #pragma unroll
-      for (int c = 0; c < kPW; ++c)
+      for (int c = 0; c < PW; ++c)
         if (c == j && r == rj) a[u][c] = vj;
     }
     if (tid < j) {
@@ -671,80 +677,89 @@ __global__ void __launch_bounds__(kThreads) bj_qr_panel(const Prob* __restrict__
   }
   double* Y = scr + pr.y_off;
 #pragma unroll
-  for (int u = 0; u < kQrRows; ++u) {
+  for (int u = 0; u < ROWS; ++u) {
     const int r = kb + tid + kThreads * u;
     if (r >= pr.mp) continue;
     const int i = r - kb;
 #pragma unroll
-    for (int c = 0; c < kPW; ++c) {
+    for (int c = 0; c < PW; ++c) {
       if (c >= nb) continue;
       W[(int64_t)(kb + c) * mp + r] = (c > i) ? a[u][c] : (c == i ? rdiag[c] : 0.0);
       Y[(int64_t)c * mp + r] = (i >= c) ? a[u][c] : 0.0;
     }
   }
-  for (int e = tid; e < kPW * kPW; e += kThreads) scr[pr.t_off + e] = Ts[e / kPW][e % kPW];
+  for (int e = tid; e < PW * PW; e += kThreads) scr[pr.t_off + e] = Ts[e / PW][e % PW];
 }
 
 // Trailing update after panel `panel`: workgroup (tile, problem) applies
 // Q_panel^T = I - Y T^T Y^T to the 32 columns c0 .. c0 + 32 of W (rows >= kb):
 // Z = Y^T C (rows in 64-row chunks through LDS, a fixed order), Z2 = T^T Z,
 // C -= Y Z2.
+template <int PW>
 __global__ void __launch_bounds__(kThreads) bj_qr_update(const Prob* __restrict__ probs, const int* __restrict__ qlist,
                                                          double* __restrict__ scr, int panel) {
-  __shared__ double Ys[64][kPW + 1], Cs[64][kPW + 1];
-  __shared__ double Zs[kPW][kPW + 1], Z2[kPW][kPW + 1], Tt[kPW][kPW + 1];
+  constexpr int ZQ = PW * 32 / kThreads;   // entries of Z (PW x 32) per thread
+  __shared__ double Ys[64][PW + 1], Cs[64][33];
+  __shared__ double Zs[PW][33], Z2[PW][33], Tt[PW][PW + 1];
   const Prob pr = probs[qlist[blockIdx.y]];
-  const int kb = panel * kPW;
-  const int c0 = kb + kPW + kPW * (int)blockIdx.x;
+  const int kb = panel * PW;
+  const int c0 = kb + PW + 32 * (int)blockIdx.x;
   if (c0 >= pr.n) return;
+  const int ncol = min(32, pr.np - c0);   // a 16-wide panel's tiles may end past W's padded columns
   const int m = pr.m, tid = threadIdx.x;
   const int64_t mp = pr.mp;
   double* W = scr + pr.w_off;
   const double* Y = scr + pr.y_off;
-  for (int e = tid; e < kPW * kPW; e += kThreads) Tt[e / kPW][e % kPW] = scr[pr.t_off + e];
-  const int zi = tid >> 3, zc = (tid & 7) * 4;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int e = tid; e < PW * PW; e += kThreads) Tt[e / PW][e % PW] = scr[pr.t_off + e];
+  const int zi = tid / (32 / ZQ), zc = (tid % (32 / ZQ)) * ZQ;
+  double acc[ZQ];
+#pragma unroll
+  for (int q = 0; q < ZQ; ++q) acc[q] = 0.0;
   for (int r0 = kb; r0 < m; r0 += 64) {
-    for (int e = tid; e < 64 * kPW; e += kThreads) {
+    for (int e = tid; e < 64 * PW; e += kThreads) {
       const int c = e / 64, rr = e % 64, r = r0 + rr;
       Ys[rr][c] = (r < m) ? Y[(int64_t)c * mp + r] : 0.0;
-      Cs[rr][c] = (r < m) ? W[(int64_t)(c0 + c) * mp + r] : 0.0;
+    }
+    for (int e = tid; e < 64 * 32; e += kThreads) {
+      const int c = e / 64, rr = e % 64, r = r0 + rr;
+      Cs[rr][c] = (r < m && c < ncol) ? W[(int64_t)(c0 + c) * mp + r] : 0.0;
     }
     __syncthreads();
     for (int rr = 0; rr < 64; ++rr) {
       const double y = Ys[rr][zi];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[q] = fma(y, Cs[rr][zc + q], acc[q]);
+      for (int q = 0; q < ZQ; ++q) acc[q] = fma(y, Cs[rr][zc + q], acc[q]);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) Zs[zi][zc + q] = acc[q];
+  for (int q = 0; q < ZQ; ++q) Zs[zi][zc + q] = acc[q];
   __syncthreads();
   // Z2 = T^T Z (T upper triangular)
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < ZQ; ++q) {
     double z = 0.0;
     for (int l = 0; l <= zi; ++l) z = fma(Tt[l][zi], Zs[l][zc + q], z);
     acc[q] = z;
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) Z2[zi][zc + q] = acc[q];
+  for (int q = 0; q < ZQ; ++q) Z2[zi][zc + q] = acc[q];
   __syncthreads();
   // C -= Y Z2: thread (row rr, 8 columns)
   const int rr = tid & 63, cg = (tid >> 6) * 8;
   for (int r = kb + rr; r < m; r += 64) {
     double v[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) v[q] = W[(int64_t)(c0 + cg + q) * mp + r];
+    for (int q = 0; q < 8; ++q) v[q] = (cg + q < ncol) ? W[(int64_t)(c0 + cg + q) * mp + r] : 0.0;
 #pragma unroll 4
-    for (int i = 0; i < kPW; ++i) {
+    for (int i = 0; i < PW; ++i) {
       const double yi = Y[(int64_t)i * mp + r];
 #pragma unroll
       for (int q = 0; q < 8; ++q) v[q] = fma(-yi, Z2[i][cg + q], v[q]);
     }
 #pragma unroll
-    for (int q = 0; q < 8; ++q) W[(int64_t)(c0 + cg + q) * mp + r] = v[q];
+    for (int q = 0; q < 8; ++q)
+      if (cg + q < ncol) W[(int64_t)(c0 + cg + q) * mp + r] = v[q];
   }
 }
 
@@ -922,8 +937,8 @@ struct Plan {
   double* d_scr = nullptr;
   int* d_iscr = nullptr;
   tn_gemm_plan warm = nullptr;    // W = A V0 of the warm-started problems
-  int* d_qlist = nullptr;         // QR-preconditioned problems
-  int nqr = 0, max_qr_n = 0;
+  int* d_qlist = nullptr;         // QR-preconditioned problems: 32-wide panels first, then 16-wide
+  int nqr = 0, nqr32 = 0, max_n32 = 0, max_n16 = 0;
   tn_gemm_plan qr_vt = nullptr;   // their VT = U_X^T V0^T
   tn_gemm_plan qr_g = nullptr;    // G = VT VT^T
   tn_gemm_plan qr_h = nullptr;    // H = G VT (then VT = (3 VT - H) / 2)
@@ -952,7 +967,7 @@ int build(Plan** out, const std::vector<Spec>& specs) {
   int64_t scr = 0, iscr = kMaxSweeps;
   std::vector<int2> map;
   std::vector<tn_gemm_desc_ex> gd, qd, qg, qh;
-  std::vector<int> qlist;
+  std::vector<int> qlist, qlist16;
   for (size_t i = 0; i < specs.size(); ++i) {
     const Spec& s = specs[i];
     Prob p{};
@@ -997,6 +1012,7 @@ int build(Plan** out, const std::vector<Spec>& specs) {
     // only S and VT wanted: QR-preconditioned sweeps (W held by a panel
     // workgroup's registers up to kQrMaxM rows)
     p.qr = (s.u_off < 0 && !p.tr && m <= kQrMaxM) ? 1 : 0;
+    p.qpw = (m <= kQrWideMaxM) ? kPW : kPW / 2;
     p.ms = p.qr ? (int32_t)round_up(n, 16) : p.mp;
     p.y_off = p.t_off = p.ut_off = -1;
     if (p.qr) {
@@ -1006,8 +1022,11 @@ int build(Plan** out, const std::vector<Spec>& specs) {
       scr += kPW * kPW;
       p.ut_off = scr;
       scr += n * n;
-      qlist.push_back((int)i);
-      P->max_qr_n = std::max(P->max_qr_n, (int)n);
+      (p.qpw == kPW ? qlist : qlist16).push_back((int)i);
+      if (p.qpw == kPW)
+        P->max_n32 = std::max(P->max_n32, (int)n);
+      else
+        P->max_n16 = std::max(P->max_n16, (int)n);
       // VT(k, r) = sum_i Ut(k, i) V0T(i, r), V0T kept in V (row stride nvp)
       tn_gemm_desc_ex g{};
       g.M = n;
@@ -1081,6 +1100,8 @@ int build(Plan** out, const std::vector<Spec>& specs) {
   if (rc == TN_OK) rc = tn_memcpy_h2d(P->d_probs, P->probs.data(), sizeof(Prob) * P->probs.size(), nullptr);
   if (rc == TN_OK) rc = tn_memcpy_h2d(P->d_map, map.data(), sizeof(int2) * map.size(), nullptr);
   if (rc == TN_OK && !gd.empty()) rc = tn_gemm_plan_create_ex(&P->warm, TN_GEMM_F64, (int64_t)gd.size(), gd.data());
+  P->nqr32 = (int)qlist.size();
+  qlist.insert(qlist.end(), qlist16.begin(), qlist16.end());
   P->nqr = (int)qlist.size();
   if (rc == TN_OK && P->nqr > 0) rc = tn::pool_alloc((void**)&P->d_qlist, sizeof(int) * qlist.size());
   if (rc == TN_OK && P->nqr > 0) rc = tn_memcpy_h2d(P->d_qlist, qlist.data(), sizeof(int) * qlist.size(), nullptr);
@@ -1108,14 +1129,28 @@ int launch(Plan* P, const double* A, const double* V0T, double* U, double* S, do
   }
   if (P->nqr > 0) {
     const unsigned nq = (unsigned)P->nqr;
-    const int panels = (P->max_qr_n + kPW - 1) / kPW;
-    for (int p = 0; p < panels; ++p) {
-      hipLaunchKernelGGL(bj_qr_panel, dim3(nq), dim3(kThreads), 0, st, P->d_probs, P->d_qlist, P->d_scr, p);
+    // panels of the 32-wide problems (list head), then of the 16-wide ones
+    const int n32 = P->nqr32, n16 = P->nqr - P->nqr32;
+    for (int p = 0; n32 > 0 && p < (P->max_n32 + kPW - 1) / kPW; ++p) {
+      hipLaunchKernelGGL((bj_qr_panel<kPW, 4>), dim3((unsigned)n32), dim3(kThreads), 0, st, P->d_probs, P->d_qlist,
+                         P->d_scr, p);
       TN_LAUNCH_CHECK();
-      const int tiles = (P->max_qr_n - kPW * (p + 1) + kPW - 1) / kPW;
+      const int tiles = (P->max_n32 - kPW * (p + 1) + 31) / 32;
       if (tiles > 0) {
-        hipLaunchKernelGGL(bj_qr_update, dim3((unsigned)tiles, nq), dim3(kThreads), 0, st, P->d_probs, P->d_qlist,
-                           P->d_scr, p);
+        hipLaunchKernelGGL(bj_qr_update<kPW>, dim3((unsigned)tiles, (unsigned)n32), dim3(kThreads), 0, st, P->d_probs,
+                           P->d_qlist, P->d_scr, p);
+        TN_LAUNCH_CHECK();
+      }
+    }
+    constexpr int kPN = kPW / 2;
+    for (int p = 0; n16 > 0 && p < (P->max_n16 + kPN - 1) / kPN; ++p) {
+      hipLaunchKernelGGL((bj_qr_panel<kPN, 8>), dim3((unsigned)n16), dim3(kThreads), 0, st, P->d_probs,
+                         P->d_qlist + n32, P->d_scr, p);
+      TN_LAUNCH_CHECK();
+      const int tiles = (P->max_n16 - kPN * (p + 1) + 31) / 32;
+      if (tiles > 0) {
+        hipLaunchKernelGGL(bj_qr_update<kPN>, dim3((unsigned)tiles, (unsigned)n16), dim3(kThreads), 0, st, P->d_probs,
+                           P->d_qlist + n32, P->d_scr, p);
         TN_LAUNCH_CHECK();
       }
     }
