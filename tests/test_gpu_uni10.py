@@ -419,6 +419,26 @@ def test_batch_tebd_mixed_kinds_match_oracle(u10, ora, golden):
         assert abs(B.trunc_err(i) - O.trunc_err) <= 1e-10
 
 
+@pytest.mark.parametrize("chi", [300, 520])
+def test_batch_tebd_large_dense_chain_matches_oracle(u10, ora, golden, chi):
+    """A dense spin-1/2 chain whose theta (2 chi square) goes through the
+    block SVD with QR preconditioning: 32-column panels at chi = 300 (600
+    rows), 16-column ones at chi = 520 (1040 rows).  Two steps against the
+    oracle's dense iTEBD: energy, both Schmidt spectra and the discarded
+    weight to 1e-10."""
+    tau, delta, seed = 0.02, 0.8, 41
+    _say("large dense chain chi=%d" % chi)
+    B = u10.BatchTebd([chi], [delta], [seed], tau, golden)
+    B.step(2)
+    O = ora.ITEBD(chi, 2, ora.xxz_two_site_d(delta, 2), tau, seed, golden)
+    for _ in range(2):
+        O.step()
+    assert abs(B.energy(0) - O.energy()) <= 1e-10
+    assert np.max(np.abs(B.lam(0, 0) - O.La)) <= 1e-10
+    assert np.max(np.abs(B.lam(0, 1) - O.Lb)) <= 1e-10
+    assert abs(B.trunc_err(0) - O.trunc_err) <= 1e-10
+
+
 def test_batch_tebd_mixed_graph_replay_and_slot_reload(u10, golden):
     """The mixed batch replayed as a graph gives the same bits as issued
     step by step; reloading one U(1) slot restarts it exactly (same bits as
