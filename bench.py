@@ -439,6 +439,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--headline-only", action="store_true",
                     help="only the headline step, its kernels and the copy calibration (for counter runs)")
+    ap.add_argument("--batch-only", type=int, default=0, metavar="STEPS",
+                    help="only the 512-chain chi 16..256 half-U(1) batch for STEPS steps, one JSON line "
+                         "(for kernel-trace runs of the batched update)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path in this build (independent "
@@ -449,6 +452,12 @@ def main():
     tn.init(0)
     uni10.load()
     info = tn.device_info()
+    if args.batch_only:
+        out = c4_batch(uni10, list(range(16, 257, 16)), args.batch_only, "16..256 step 16, 32 chains each",
+                       u1_half=True)
+        print(json.dumps({"metric": "batched iTEBD steps/s (512 chains, chi 16..256, half U(1))",
+                          "value": out["steps_per_s"], "unit": "steps/s", "detail": out}))
+        return
 
     # headline: the drop-in API step
     A, B = make_tensors(uni10)
