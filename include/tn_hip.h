@@ -55,6 +55,9 @@ int tn_zero(void* dst, size_t bytes, void* stream);  /* elemBzero (fill kernel) 
 int tn_stream_create(void** stream);
 int tn_stream_destroy(void* stream);
 int tn_stream_sync(void* stream);
+/* *done = 1 when every piece of work enqueued on the stream has finished,
+ * 0 while some is still running (never blocks) */
+int tn_stream_query(void* stream, int* done);
 int tn_device_sync(void);
 /* Events for timing on a given stream (milliseconds between two records). */
 int tn_event_create(void** ev);

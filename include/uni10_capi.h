@@ -111,6 +111,9 @@ int u10_batch_tebd_step(u10_batch_tebd b, int nsteps);
  * checks that its SVDs converged */
 int u10_batch_tebd_launch(u10_batch_tebd b, int nsteps);
 int u10_batch_tebd_wait(u10_batch_tebd b);
+/* *ready = 1 once the launched steps have finished, 0 while they run (a
+ * poll that never blocks; wait still checks the SVDs) */
+int u10_batch_tebd_ready(u10_batch_tebd b, int* ready);
 int u10_batch_tebd_set_graph(u10_batch_tebd b, int on);
 /* restart one chain (slot reuse): the seeded start of (delta, seed) */
 int u10_batch_tebd_load(u10_batch_tebd b, int chain, double delta, uint64_t seed);

@@ -284,6 +284,20 @@ int tn_stream_sync(void* stream) {
   return TN_OK;
 }
 
+int tn_stream_query(void* stream, int* done) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(done != nullptr, TN_EINVAL, "tn_stream_query: NULL out pointer");
+  const hipError_t e = hipStreamQuery(stream_of(stream));
+  if (e == hipErrorNotReady) {
+    (void)hipGetLastError();  // not an error: clear it so later checks do not see it
+    *done = 0;
+    return TN_OK;
+  }
+  TN_HIP_CHECK(e);
+  *done = 1;
+  return TN_OK;
+}
+
 int tn_device_sync(void) {
   TN_INIT_OR_RETURN();
   TN_HIP_CHECK(hipDeviceSynchronize());

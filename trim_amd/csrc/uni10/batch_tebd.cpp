@@ -721,6 +721,12 @@ void BatchTebd::wait() {
   if (nonconv) detail::fail("BatchTebd: an SVD did not converge");
 }
 
+bool BatchTebd::ready() const {
+  int done = 0;
+  detail::check(tn_stream_query(stream_, &done), "BatchTebd ready");
+  return done != 0;
+}
+
 std::vector<double> BatchTebd::lambda(int chain, int which) const {
   if (chain < 0 || chain >= chains() || which < 0 || which > 1) detail::fail("BatchTebd::lambda: bad index");
   std::vector<double> h(chi_[chain]);

@@ -49,6 +49,8 @@ class BatchTebd {
   // and checks the SVDs' convergence
   void launch(int n = 1);
   void wait();
+  // true once every launched step has finished (never blocks)
+  bool ready() const;
   // restart chain `chain` from the seeded start of (delta, seed): the slot
   // reuse of continuous batching; other chains are untouched
   void load(int chain, double delta, uint64_t seed);

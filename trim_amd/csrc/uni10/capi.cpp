@@ -392,6 +392,13 @@ int u10_batch_tebd_wait(u10_batch_tebd b) {
   return guard([&] { b->B.wait(); });
 }
 
+int u10_batch_tebd_ready(u10_batch_tebd b, int* ready) {
+  return guard([&] {
+    if (!ready) detail::fail("u10_batch_tebd_ready: NULL out pointer");
+    *ready = b->B.ready() ? 1 : 0;
+  });
+}
+
 int u10_batch_tebd_set_graph(u10_batch_tebd b, int on) {
   return guard([&] { b->B.set_graph(on != 0); });
 }

@@ -15,10 +15,12 @@ engine per class (bond dimension, d, symmetry), each with a fixed number of
 slots whose tensors stay resident in HBM; the slot count of each class is
 what its share of the HBM budget holds.  Every tick it admits the requests
 that have arrived into free slots (first come, first served per class),
-advances every class with a busy slot by one step in one batched,
-graph-replayed launch sequence (every class on its own stream, so the
-classes' work overlaps on the device), and retires finished requests.
-Requests that arrive while others run join at the next tick: a chain's
+and every class steps at its own pace: a class with a busy slot and no step
+in flight launches one batched, graph-replayed step on its own stream (so
+the classes' work overlaps on the device), and a class whose step has
+finished retires its finished requests and admits new ones without waiting
+for slower classes.  Requests that arrive while others run join at their
+class's next step: a chain's
 computation never depends on which other slots are busy (slots are
 restarted with identity warm-start bases), so each result is the same alone
 and in the mix.
@@ -159,6 +161,7 @@ class Server:
         t0 = time.perf_counter()
         i = 0
         clock = 0.0
+        inflight = set()       # classes with a launched step not yet retired
         while i < len(pending) or contracts or any(waiting.values()) or any(self.busy.values()):
             now = (time.perf_counter() - t0) if realtime else clock
             if not realtime and i < len(pending) and not any(self.busy.values()) and not contracts \
@@ -168,8 +171,11 @@ class Server:
                 r = pending[i]
                 (contracts if r.kind == "contract" else waiting[(r.chi, r.d, r.sym)]).append(r)
                 i += 1
-            # admission: free slots, first come first served per class
+            # admission: free slots of classes between steps, first come first
+            # served per class
             for c in self.classes:
+                if c in inflight:
+                    continue
                 while waiting[c] and self.free[c]:
                     r = waiting[c].pop(0)
                     s = self.free[c].pop(0)
@@ -183,15 +189,22 @@ class Server:
                 r.t_done = (time.perf_counter() - t0) if realtime else now
                 done.append(r)
             contracts = []
-            # one batched step of every class with a busy slot: all classes
-            # are launched first (each engine has its own stream, so their
-            # work overlaps on the device), then waited for
-            active = [c for c in self.classes if self.busy[c]]
-            for c in active:
-                self.engines[c].launch(1)
-            for c in active:
+            # every class steps at its own pace: a class with busy slots and
+            # no step in flight launches one batched step on its own stream
+            # (the classes' steps overlap on the device), and a class whose
+            # step has finished is retired without waiting for slower ones
+            for c in self.classes:
+                if self.busy[c] and c not in inflight:
+                    self.engines[c].launch(1)
+                    inflight.add(c)
+            retired = False
+            for c in self.classes:
+                if c not in inflight or not self.engines[c].ready():
+                    continue
                 self.engines[c].wait()
-            for c in active:
+                inflight.discard(c)
+                retired = True
+                t_done = (time.perf_counter() - t0) if realtime else now
                 self.chain_steps += len(self.busy[c])
                 for s in list(self.busy[c]):
                     ent = self.busy[c][s]
@@ -199,12 +212,13 @@ class Server:
                     if ent[1] == 0:
                         r = ent[0]
                         r.result = np.concatenate([self.engines[c].lam(s, 0), self.engines[c].lam(s, 1)])
-                        r.t_done = (time.perf_counter() - t0) if realtime else now
+                        r.t_done = t_done
                         done.append(r)
                         del self.busy[c][s]
                         self.free[c].append(s)
             if not realtime:
-                clock = now + 1e-3
+                if retired:
+                    clock = now + 1e-3
             elif i < len(pending) and not any(self.busy.values()) and not contracts \
                     and not any(waiting.values()):
                 time.sleep(max(0.0, min(0.01, pending[i].t_arrival - (time.perf_counter() - t0))))

@@ -39,6 +39,7 @@ SIGNATURES = {
     "tn_stream_create": [ctypes.POINTER(ctypes.c_void_p)],
     "tn_stream_destroy": [P],
     "tn_stream_sync": [P],
+    "tn_stream_query": [P, ctypes.POINTER(ctypes.c_int)],
     "tn_device_sync": [],
     "tn_event_create": [ctypes.POINTER(ctypes.c_void_p)],
     "tn_event_destroy": [P],

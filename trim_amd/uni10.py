@@ -86,6 +86,7 @@ SIGNATURES = {
     "u10_batch_tebd_step": [P, I],
     "u10_batch_tebd_launch": [P, I],
     "u10_batch_tebd_wait": [P],
+    "u10_batch_tebd_ready": [P, PI],
     "u10_batch_tebd_set_graph": [P, I],
     "u10_batch_tebd_load": [P, I, D, ctypes.c_uint64],
     "u10_batch_tebd_energy": [P, I, PD],
@@ -478,6 +479,12 @@ class BatchTebd:
 
     def wait(self):
         _call("u10_batch_tebd_wait", self.h)
+
+    def ready(self) -> bool:
+        """True once the launched steps have finished (a poll; never blocks)."""
+        out = ctypes.c_int(0)
+        _call("u10_batch_tebd_ready", self.h, ctypes.byref(out))
+        return bool(out.value)
 
     def set_graph(self, on: bool = True):
         _call("u10_batch_tebd_set_graph", self.h, int(bool(on)))
