@@ -373,6 +373,48 @@ def c5_serve(u10, n_req, rate, chis=(32, 64, 128), mixed=True):
     return out
 
 
+LANCZOS_N = 8192
+LANCZOS_STEPS = 60
+
+
+def lanczos_matrix(n):
+    """Symmetric X + X^T, X ~ U[-1, 1) (seed 21): the Lanczos leg's input."""
+    X = np.random.default_rng(21).uniform(-1.0, 1.0, (n, n))
+    X += X.T.copy()
+    return X
+
+
+def lanczos_leg(tn):
+    """lanczosEigh on the device: a fixed number of Lanczos steps (err_tol 0,
+    so the run never stops early) on an 8192 x 8192 symmetric matrix (512
+    MiB in HBM).  Every step reads the matrix once (matrix-vector product),
+    so n^2 * 8 bytes per step over the step time is the HBM rate the step
+    reaches, including its vector work and the two doubles copied back."""
+    import ctypes
+    n = LANCZOS_N
+    A = tn.DeviceArray.from_host(lanczos_matrix(n))
+    start = np.random.default_rng(22).uniform(-1.0, 1.0, n)
+    res = {}
+    for label, steps in (("warm", 5), ("timed", LANCZOS_STEPS)):
+        psi = tn.DeviceArray.from_host(start)
+        it = ctypes.c_int64(steps)
+        e0 = ctypes.c_double()
+        conv = ctypes.c_int()
+        tn.sync()
+        t0 = time.perf_counter()
+        tn.call("tn_lanczos_f64", tn._p(A), n, tn._p(psi), ctypes.byref(it), 0.0, ctypes.byref(e0),
+                ctypes.byref(conv), None)
+        dt = time.perf_counter() - t0
+        res[label] = (dt, it.value, e0.value)
+    dt, steps, e0 = res["timed"]
+    per = dt / steps
+    gbps = n * n * 8 / per / 1e9
+    del A
+    return {"n": n, "steps": steps, "ms_per_step": per * 1e3, "lowest_ritz_value": e0,
+            "matrix_GBps_per_step": gbps, "frac_hbm": gbps / HBM_PEAK_GBPS,
+            "note": "matrix bytes per step / step time (whole step: matvec, vector work, host bisection)"}
+
+
 def cpu_baseline(budget_s=10.0, max_steps=400):
     """The oracle's contraction step on the host: C odometer permutes (one
     thread) and numpy/OpenBLAS dgemm (BLAS threads = cores)."""
@@ -417,11 +459,22 @@ def cpu_baseline(budget_s=10.0, max_steps=400):
         t.step()
         ns += 1
     tebd = ns / (time.perf_counter() - t1)
+    # Lanczos on the oracle (numpy dgemv, BLAS threads = cores), a few steps
+    lim = threadpool_limits(limits=cores) if threadpool_limits else None
+    S = lanczos_matrix(LANCZOS_N)
+    start = np.random.default_rng(22).uniform(-1.0, 1.0, LANCZOS_N)
+    ora.lanczos_ev(S, start, max_iter=3, err_tol=0.0)
+    t2 = time.perf_counter()
+    _, _, lz_steps, _ = ora.lanczos_ev(S, start, max_iter=20, err_tol=0.0)
+    lz = (time.perf_counter() - t2) / lz_steps
+    del S
+    if lim is not None and hasattr(lim, "unregister"):
+        lim.unregister()
     return {"value": 2.0 * 512 ** 3 / dt / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
             "sample": "%d contraction-and-permute steps (256,2,2,256) FP64 in %.1f s: oracle C odometer "
                       "permutes (1 thread) + numpy/OpenBLAS dgemm (%d threads), %.3f ms/step; oracle iTEBD "
                       "chi=32: %d steps, %.1f steps/s" % (k, k * dt, cores, dt * 1e3, ns, tebd),
-            "ms_per_step": dt * 1e3, "tebd_steps_per_s": tebd}
+            "ms_per_step": dt * 1e3, "tebd_steps_per_s": tebd, "lanczos_ms_per_step_n8192": lz * 1e3}
 
 
 def progress(msg):
@@ -442,6 +495,8 @@ def main():
     ap.add_argument("--batch-only", type=int, default=0, metavar="STEPS",
                     help="only the 512-chain chi 16..256 half-U(1) batch for STEPS steps, one JSON line "
                          "(for kernel-trace runs of the batched update)")
+    ap.add_argument("--lanczos-only", action="store_true",
+                    help="only the Lanczos leg (8192 x 8192), one JSON line (for kernel-trace runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path in this build (independent "
@@ -452,6 +507,9 @@ def main():
     tn.init(0)
     uni10.load()
     info = tn.device_info()
+    if args.lanczos_only:
+        print(json.dumps(lanczos_leg(tn)), flush=True)
+        return
     if args.batch_only:
         out = c4_batch(uni10, list(range(16, 257, 16)), args.batch_only, "16..256 step 16, 32 chains each",
                        u1_half=True)
@@ -506,6 +564,8 @@ def main():
     progress("iTEBD chi=32: %.1f steps/s" % tebd_sps)
     c3 = c3_u1(tn, uni10)
     progress("U(1) chi=1024 contraction modes done")
+    lz = lanczos_leg(tn)
+    progress("Lanczos n=%d: %.3f ms/step" % (lz["n"], lz["ms_per_step"]))
     c4 = {}
     if not args.no_batch:
         c4["chi_16_32"] = c4_batch(uni10, [16, 32], 10, "16 and 32, 256 chains each")
@@ -544,6 +604,7 @@ def main():
         "tebd_chi32": {"steps_per_s": tebd_sps, "steps": args.tebd_steps, "energy_per_site": tebd_e,
                        "trunc_err_last": tebd_te},
         "c3_u1_chi1024": c3,
+        "lanczos": lz,
         "c4_batched_itebd": c4,
         "c5_serving": (c5 if not args.no_batch else None),
         "wall_s_timed": wall,

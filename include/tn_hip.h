@@ -243,6 +243,20 @@ int tn_svd_plan_destroy(tn_svd_plan plan);
 int tn_eigh_f64(const double* A, int64_t N, double* eig, double* eigvec_rows,
                 void* stream);
 
+/* lanczosEV (uni10_lapack_cpu.cpp:416-509; the reference's GPU entry
+ * uni10_lapack_gpu.cu:681-687 only throws): lowest eigenpair of the n x n
+ * row-major matrix A by the plain Lanczos recurrence, every vector operation
+ * on the device.  psi (n doubles, device) is the start vector on entry and
+ * the ground state sum_k z_k v_k on return (sign: <start|psi> >= 0).
+ * *iters: the maximum step count on entry (must exceed 2), the steps taken
+ * on return.  Stops when the lowest Ritz value changes by at most err_tol
+ * relative to max(|E0|, 1), when the Krylov space closes (beta <= 1e-15),
+ * or at the maximum; *converged = 0 only in the last case.  A space that
+ * closes after one step reports *E0 = 0 and leaves psi as given, as the
+ * reference does.  Synchronises the stream. */
+int tn_lanczos_f64(const double* A, int64_t n, double* psi, int64_t* iters, double err_tol,
+                   double* E0, int* converged, void* stream);
+
 /* ---- TEBD truncation ------------------------------------------------------
  * The truncate-and-renormalise of iTEBD.cpp:157-161 plus the discarded
  * weight: given S (k, descending) on the device, keep dim = min(chi, k)

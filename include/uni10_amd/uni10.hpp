@@ -181,6 +181,11 @@ typedef Matrix Block;  // read-only views are returned as copies
 Matrix takeExp(double a, const Matrix& mat);
 Matrix exph(double a, const Matrix& mat);
 Matrix RDotR(const Matrix& a, const Matrix& b);
+// lanczosEigh (Block.h:389, Block.cpp:474-506): lowest eigenvalue E0 of a
+// square matrix by Lanczos; psi is the start vector on entry and the ground
+// state on return.  Returns the steps taken; throws when max_iter steps do
+// not converge.
+size_t lanczosEigh(Matrix& ori_mat, double& E0, Matrix& psi, size_t max_iter = 1000, double err_tol = 5e-15);
 
 // ---------------------------------------------------------------------------
 // UniTensor (UniTensor.h:58-971), real FP64.

@@ -71,6 +71,10 @@ int u10_network_free(u10_network net);
 int u10_svd(const double* A, int64_t M, int64_t N, double* U, double* S, double* VT);
 int u10_eigh(const double* A, int64_t N, double* eig, double* vec_rows);
 int u10_exph(double a, const double* H, int64_t N, double* out);
+/* lanczosEigh (Block.cpp:474-506) on host arrays: psi (N) is the start
+ * vector on entry and the ground state on return; *iters = steps taken. */
+int u10_lanczos_eigh(const double* A, int64_t N, double* psi, uint64_t max_iter, double err_tol, double* E0,
+                     uint64_t* iters);
 
 /* iTEBD of the spin-1/2 XXZ chain through Networks (the example driver). */
 int u10_itebd_create(int chi, int d, double delta, double tau, uint64_t seed, const char* netdir,

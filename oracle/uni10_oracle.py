@@ -27,6 +27,10 @@ Pinning (what each part is checked against; see tests/test_oracle.py):
                             reconstruction, orthonormality, ordering.
   * eigh / exph           - closed form of the two-site spin-1/2 Heisenberg
                             gate (eigenvalues 1/4 x3, -3/4).
+  * lanczos_ev            - the lowest eigenvalue from numpy.linalg.eigh
+                            (textbook routine) and the closed form -2 of the
+                            four-site Heisenberg ring; the eigenvector by its
+                            residual.
   * Network               - Simple.net and egN1_network known answers; ORDER
                             brackets vs greedy plan on the same tensors.
   * iTEBD energy          - Bethe-ansatz bound 1/4 - ln 2 and the two-site
@@ -806,6 +810,90 @@ def exph(a: float, H: np.ndarray) -> np.ndarray:
 
 def vector_norm(x: np.ndarray) -> float:
     return float(math.sqrt(float(np.dot(x, x))))
+
+
+def lanczos_ev(A: np.ndarray, psi: np.ndarray, max_iter: int = 1000, err_tol: float = 5e-15):
+    """lanczosEV (uni10_lapack_cpu.cpp:416-509) behind lanczosEigh (Block.cpp:474-506).
+
+    Plain Lanczos from psi / ||psi|| without reorthogonalisation: dgemv (the
+    row-major matrix times the vector, :444), ddot / daxpy (:445-447), norm
+    and scale (:449-455); after every step the smallest eigenvalue of the
+    tridiagonal matrix (dstev, :464-477; here SciPy's eigh_tridiagonal);
+    stop on a relative change <= err_tol against max(|E0|, 1), on
+    beta <= 1e-15, or at max_iter, never before 2 steps (:418-419, :431).
+    Returns (E0, psi, steps, converged).  The eigenvector's sign is left to
+    dstev in the reference; here it is fixed so that <start|psi> >= 0.
+    A space that closes after one step returns E0 = 0 and psi unchanged
+    (:501-504).
+    """
+    import scipy.linalg as sla
+    if not max_iter > 2:
+        raise ValueError("Maximum iteration number should be set greater than 2.")
+    A = np.asarray(A, dtype=np.float64)
+    N = A.shape[0]
+    psi = np.asarray(psi, dtype=np.float64).reshape(-1)
+    V = np.zeros((max_iter + 1, N))
+    V[0] = psi * (1.0 / vector_norm(psi))
+    As, Bs = [], []
+    beta, e_diff, e0_old, it, conv = 1.0, 1.0, 0.0, 0, False
+    beta_err = 1e-15
+
+    def lowest(m, vec=False):
+        d = np.array(As[:m])
+        e = np.array(Bs[:m - 1])
+        if vec:
+            w, z = sla.eigh_tridiagonal(d, e, select="i", select_range=(0, 0))
+            z = z[:, 0]
+            return w[0], (z if z[0] >= 0 else -z)
+        return sla.eigh_tridiagonal(d, e, eigvals_only=True, select="i", select_range=(0, 0))[0]
+
+    while (((e_diff > err_tol) and it < max_iter) or it < 2) and beta > beta_err:
+        w = A @ V[it] - (beta * V[it - 1] if it > 0 else 0.0)
+        alpha = float(np.dot(V[it], w))
+        w = w - alpha * V[it]
+        beta = vector_norm(w)
+        As.append(alpha)
+        if beta > beta_err:
+            V[it + 1] = w * (1.0 / beta)
+            Bs.append(beta if it < max_iter - 1 else 0.0)
+        else:
+            V[it + 1] = w
+            Bs.append(0.0)
+            conv = True
+        it += 1
+        if it > 1:
+            d0 = lowest(it)
+            base = abs(d0) if abs(d0) > 1 else 1.0
+            e_diff = abs(d0 - e0_old) / base
+            e0_old = d0
+            if e_diff <= err_tol:
+                conv = True
+    if it > 1:
+        lam, z = lowest(it, vec=True)
+        return float(lam), z @ V[:it], it, conv
+    return 0.0, psi.copy(), 1, conv
+
+
+def heisenberg_ring(L: int, J: float = 1.0) -> np.ndarray:
+    """H = J sum_i S_i . S_{i+1} on a periodic ring of L spins 1/2 (2^L x 2^L).
+    Its ground energy at L = 4 is exactly -2 J."""
+    h2 = heisenberg_two_site(1.0, J)
+    n = 2 ** L
+    H = np.zeros((n, n))
+    for i in range(L):
+        j = (i + 1) % L
+        for s in range(n):
+            bi, bj = (s >> (L - 1 - i)) & 1, (s >> (L - 1 - j)) & 1
+            for a in range(2):
+                for b in range(2):
+                    v = h2[2 * a + b, 2 * bi + bj]
+                    if v == 0.0:
+                        continue
+                    t = s
+                    t = (t & ~(1 << (L - 1 - i))) | (a << (L - 1 - i))
+                    t = (t & ~(1 << (L - 1 - j))) | (b << (L - 1 - j))
+                    H[t, s] += v
+    return H
 
 
 # ---------------------------------------------------------------------------

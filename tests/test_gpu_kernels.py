@@ -808,3 +808,32 @@ def test_vecops_plan(tn):
     ref[19800] = nrm
     ref[19801] = np.sum(S[25:] ** 2)
     assert np.max(np.abs(out - ref)) <= 1e-14
+
+
+def test_lanczos_c_abi(tn, ora):
+    # tn_lanczos_f64 on device buffers against the oracle's lanczosEV
+    import ctypes
+    n = 500
+    rng = np.random.default_rng(11)
+    X = rng.uniform(-1, 1, (n, n))
+    S = X @ X.T + X.T @ X
+    start = rng.uniform(-1, 1, n)
+    A = tn.DeviceArray.from_host(S)
+    psi = tn.DeviceArray.from_host(start)
+    it = ctypes.c_int64(1000)
+    e0 = ctypes.c_double()
+    conv = ctypes.c_int()
+    tn.call("tn_lanczos_f64", tn._p(A), n, tn._p(psi), ctypes.byref(it), 5e-15, ctypes.byref(e0),
+            ctypes.byref(conv), None)
+    e_o, psi_o, it_o, conv_o = ora.lanczos_ev(S, start)
+    assert conv.value == 1 and conv_o
+    assert abs(e0.value - e_o) <= 1e-12 * np.abs(np.linalg.eigvalsh(S)).max()
+    assert abs(it.value - it_o) <= 3
+    v = psi.to_host()
+    assert np.dot(v, psi_o) / (np.linalg.norm(v) * np.linalg.norm(psi_o)) > 1 - 1e-8
+    # the start vector's norm does not matter
+    psi2 = tn.DeviceArray.from_host(start * 1e3)
+    it2 = ctypes.c_int64(1000)
+    tn.call("tn_lanczos_f64", tn._p(A), n, tn._p(psi2), ctypes.byref(it2), 5e-15, ctypes.byref(e0),
+            ctypes.byref(conv), None)
+    assert abs(e0.value - e_o) <= 1e-12 * np.abs(np.linalg.eigvalsh(S)).max()

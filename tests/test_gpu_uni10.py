@@ -479,3 +479,51 @@ def test_batch_tebd_mixed_graph_replay_and_slot_reload(u10, golden):
     assert b.lam(1, 0).tobytes() == c.lam(1, 0).tobytes()
     assert b.lam(1, 1).tobytes() == c.lam(1, 1).tobytes()
     assert b.lam(0, 0).tobytes() != keep[0]   # the others kept evolving
+
+
+# ---------------------------------------------------------------------------
+# lanczosEigh (Block.cpp:474-506 -> lanczosEV) on the device
+@pytest.mark.parametrize("n", [10, 64, 300, 1000])
+def test_lanczos_matches_oracle(u10, ora, n):
+    _say("lanczos n=%d" % n)
+    rng = np.random.default_rng(100 + n)
+    X = rng.uniform(-1, 1, (n, n))
+    S = X @ X.T + X.T @ X
+    start = rng.uniform(-1, 1, n)
+    e_o, psi_o, it_o, conv_o = ora.lanczos_ev(S, start)
+    e_g, psi_g, it_g = u10.lanczosEigh(S, start)
+    assert conv_o
+    scale = np.abs(np.linalg.eigvalsh(S)).max()
+    assert abs(e_g - e_o) <= 1e-12 * scale
+    assert abs(it_g - it_o) <= 3
+    # same Ritz vector and sign convention (<start|psi> >= 0); the vector is
+    # as accurate as the stopping rule makes it, so compare at that level
+    assert np.dot(psi_g, start) >= 0
+    rg = np.linalg.norm(S @ psi_g - e_g * psi_g) / np.linalg.norm(psi_g)
+    assert rg <= 1e-4 * scale
+    cos = np.dot(psi_g, psi_o) / (np.linalg.norm(psi_g) * np.linalg.norm(psi_o))
+    assert cos > 1 - 1e-8
+
+
+def test_lanczos_four_site_ring(u10, ora):
+    H = ora.heisenberg_ring(4)
+    e, psi, it = u10.lanczosEigh(H, np.random.default_rng(2).uniform(-1, 1, 16))
+    assert abs(e + 2.0) < 1e-13
+    assert np.linalg.norm(H @ psi + 2.0 * psi) < 1e-12
+
+
+def test_lanczos_errors_and_edge_cases(u10):
+    with pytest.raises(RuntimeError, match="greater than 2"):
+        u10.lanczosEigh(np.eye(4), np.ones(4), max_iter=2)
+    with pytest.raises(RuntimeError, match="non-square"):
+        u10.lanczosEigh(np.ones((3, 4)), np.ones(3))
+    with pytest.raises(ValueError):
+        u10.lanczosEigh(np.eye(4), np.ones(3))
+    # start vector = eigenvector: the reference's E0 = 0, psi unchanged
+    e, psi, it = u10.lanczosEigh(np.diag([1.0, 2.0, 3.0]), [0.0, 1.0, 0.0])
+    assert (e, it) == (0.0, 1)
+    assert np.array_equal(psi, [0.0, 1.0, 0.0])
+    rng = np.random.default_rng(7)
+    X = rng.uniform(-1, 1, (200, 200))
+    with pytest.raises(RuntimeError, match="fails in converging"):
+        u10.lanczosEigh(X + X.T, rng.uniform(-1, 1, 200), max_iter=3)

@@ -525,3 +525,47 @@ def test_u1_truncated_chain_stays_symmetric(ora, golden):
             assert np.all(np.diff(U.lam[p][o:o + n]) <= 0)
             o += n
     assert U.energy() < e0
+
+
+# ---------------------------------------------------------------------------
+# Lanczos (lanczosEV, uni10_lapack_cpu.cpp:416-509)
+def test_lanczos_four_site_ring_closed_form(ora):
+    # four-site periodic Heisenberg ring: ground energy exactly -2 J
+    H = ora.heisenberg_ring(4)
+    assert np.array_equal(H, H.T)
+    e0, psi, it, conv = ora.lanczos_ev(H, np.random.default_rng(1).uniform(-1, 1, 16))
+    assert conv and it >= 2
+    assert abs(e0 + 2.0) < 1e-13
+    assert np.linalg.norm(H @ psi + 2.0 * psi) < 1e-12
+
+
+@pytest.mark.parametrize("n", [10, 64, 300])
+def test_lanczos_lowest_eigenvalue_matches_eigh(ora, n):
+    # S = H H^T + H^T H as in the reference's egAP example; the lowest
+    # eigenvalue from the textbook dense routine, the vector by its residual
+    rng = np.random.default_rng(n)
+    X = rng.uniform(-1, 1, (n, n))
+    S = X @ X.T + X.T @ X
+    start = rng.uniform(-1, 1, n)
+    e0, psi, it, conv = ora.lanczos_ev(S, start)
+    w = np.linalg.eigvalsh(S)
+    assert conv
+    assert abs(e0 - w[0]) <= 1e-12 * abs(w[-1])
+    assert np.dot(psi, start) >= 0
+    assert np.linalg.norm(S @ psi - e0 * psi) <= 1e-4 * abs(w[-1]) * np.linalg.norm(psi)
+
+
+def test_lanczos_edge_cases(ora):
+    with pytest.raises(ValueError):
+        ora.lanczos_ev(np.eye(4), np.ones(4), max_iter=2)
+    # a start vector that is an eigenvector closes the space after one step:
+    # the reference then reports 0 and leaves psi unchanged
+    H = np.diag([1.0, 2.0, 3.0])
+    e0, psi, it, conv = ora.lanczos_ev(H, np.array([0.0, 1.0, 0.0]))
+    assert (e0, it, conv) == (0.0, 1, True)
+    assert np.array_equal(psi, [0.0, 1.0, 0.0])
+    # max_iter reached before the energy settles: not converged
+    rng = np.random.default_rng(7)
+    X = rng.uniform(-1, 1, (200, 200))
+    e0, psi, it, conv = ora.lanczos_ev(X + X.T, rng.uniform(-1, 1, 200), max_iter=3)
+    assert it == 3 and not conv

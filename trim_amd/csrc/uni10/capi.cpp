@@ -290,6 +290,21 @@ int u10_eigh(const double* A, int64_t N, double* eig, double* vec) {
   });
 }
 
+int u10_lanczos_eigh(const double* A, int64_t N, double* psi, uint64_t max_iter, double err_tol, double* E0,
+                     uint64_t* iters) {
+  return guard([&] {
+    if (!A || !psi || !E0 || !iters) detail::fail("u10_lanczos_eigh: NULL argument");
+    Matrix a((size_t)N, (size_t)N, A, false, false);
+    Matrix v((size_t)N, 1, psi, false, false);
+    double e0 = 0.0;
+    const size_t it = lanczosEigh(a, e0, v, (size_t)max_iter, err_tol);
+    auto h = v.getHostElem();
+    std::memcpy(psi, h.data(), h.size() * 8);
+    *E0 = e0;
+    *iters = it;
+  });
+}
+
 int u10_exph(double a, const double* H, int64_t N, double* out) {
   return guard([&] {
     Matrix h((size_t)N, (size_t)N, H, false, false);

@@ -379,4 +379,24 @@ Matrix exph(double a, const Matrix& mat) {
 
 Matrix takeExp(double a, const Matrix& mat) { return exph(a, mat); }
 
+size_t lanczosEigh(Matrix& ori_mat, double& E0, Matrix& psi, size_t max_iter, double err_tol) {
+  // Block.cpp:474-506 -> lanczosEV (uni10_lapack_cpu.cpp:416-509), on the device
+  if (ori_mat.row() != ori_mat.col())
+    detail::fail("Cannot perform Lanczos algorithm to find the lowest eigen value and eigen vector on a "
+                 "non-square matrix.");
+  if (ori_mat.row() != psi.elemNum())
+    detail::fail("Error in Lanczos initial vector psi. The vector dimension does not match with the number of "
+                 "the columns.");
+  if (ori_mat.isDiag() || psi.isDiag()) detail::fail("lanczosEigh: diagonal matrices are not supported");
+  int64_t it = (int64_t)max_iter;
+  double e0 = 0.0;
+  int conv = 0;
+  detail::check(tn_lanczos_f64(ori_mat.getElem(), (int64_t)ori_mat.row(), psi.getElem(), &it, err_tol, &e0,
+                               &conv, detail::stream()),
+                "lanczosEigh");
+  if (!conv) detail::fail("Lanczos algorithm fails in converging.");
+  E0 = e0;
+  return (size_t)it;
+}
+
 }  // namespace uni10

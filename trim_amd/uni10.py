@@ -72,6 +72,8 @@ SIGNATURES = {
     "u10_network_free": [P],
     "u10_svd": [P, I64, I64, P, P, P],
     "u10_eigh": [P, I64, P, P],
+    "u10_lanczos_eigh": [P, I64, P, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                         ctypes.POINTER(ctypes.c_uint64)],
     "u10_exph": [D, P, I64, P],
     "u10_itebd_create": [I, I, D, D, ctypes.c_uint64, S, PP],
     "u10_itebd_step": [P, I],
@@ -415,6 +417,25 @@ def eigh(A):
     w, v = np.empty(n), np.empty((n, n))
     _call("u10_eigh", p, n, w.ctypes.data_as(P), v.ctypes.data_as(P))
     return w, v
+
+
+def lanczosEigh(A, psi, max_iter: int = 1000, err_tol: float = 5e-15):
+    """Lowest eigenpair of a square matrix by Lanczos (Block.cpp:474-506):
+    returns (E0, ground state, steps taken); psi is the start vector."""
+    a, p = _arr(A, np.float64)
+    if a.ndim != 2 or a.shape[0] != a.shape[1]:
+        raise Uni10Error("lanczosEigh: Cannot perform Lanczos algorithm to find the lowest eigen value and "
+                         "eigen vector on a non-square matrix.")
+    n = a.shape[0]
+    v = np.array(psi, dtype=np.float64).reshape(-1).copy()
+    if v.size != n:
+        raise ValueError("Error in Lanczos initial vector psi. The vector dimension does not match with the "
+                         "number of the columns.")
+    e0 = ctypes.c_double()
+    it = ctypes.c_uint64()
+    _call("u10_lanczos_eigh", p, n, v.ctypes.data_as(P), int(max_iter), float(err_tol), ctypes.byref(e0),
+          ctypes.byref(it))
+    return e0.value, v, int(it.value)
 
 
 def exph(x: float, H):
