@@ -18,7 +18,7 @@
 // on the host is what the reference also does there: the eigenvalue of the
 // small tridiagonal matrix after every step (here by Sturm bisection, and
 // the final eigenvector by inverse iteration), fed by two doubles copied
-// back per step.  Every vector lives in one pooled buffer of
+// back per step (read in batches of steps).  Every vector lives in one pooled buffer of
 // (max_iter + 1) x n doubles, as the reference's Vm.
 #include <hip/hip_runtime.h>
 
@@ -36,16 +36,19 @@ constexpr int kStepThreads = 1024;
 constexpr double kBetaErr = 1e-15;         // uni10_lapack_cpu.cpp:419
 constexpr int kMinIter = 2;                // uni10_lapack_cpu.cpp:418
 
-// w[r] = A[r, :] . v - beta * vprev[r]; one wave per row, eight partial
+// w[r] = A[r, :] . v - beta * vprev[r] (beta = *beta_prev, the previous
+// step's, 0 at the first step); one wave per row, eight partial
 // sums per lane (a fixed order), then a butterfly over the wave.  VEC: the
 // rows are 16-byte aligned (n even, A aligned), so every lane loads pairs,
 // 4 KiB per wave in flight per iteration.
 template <bool VEC>
 __global__ void __launch_bounds__(kMvThreads) lz_matvec(const double* __restrict__ A, int64_t n,
                                                         const double* __restrict__ v,
-                                                        const double* __restrict__ vprev, double beta,
+                                                        const double* __restrict__ vprev,
+                                                        const double* __restrict__ beta_prev,
                                                         double* __restrict__ w) {
   const int lane = threadIdx.x & 63;
+  const double beta = beta_prev ? *beta_prev : 0.0;
   const int64_t row = (int64_t)blockIdx.x * (kMvThreads / 64) + (threadIdx.x >> 6);
   if (row >= n) return;
   const double* __restrict__ a = A + row * n;
@@ -81,16 +84,18 @@ __global__ void __launch_bounds__(kMvThreads) lz_matvec(const double* __restrict
   if (lane == 0) w[row] = t - beta * vprev[row];
 }
 
+// Sum over the block in a fixed order: a butterfly inside each wave, then
+// the wave totals in wave order.
 __device__ double block_sum(double x, double* red) {
-  const int t = threadIdx.x;
-  red[t] = x;
-  __syncthreads();
+  constexpr int kWaves = kStepThreads / 64;
 #pragma unroll
-  for (int h = kStepThreads / 2; h > 0; h >>= 1) {
-    if (t < h) red[t] += red[t + h];
-    __syncthreads();
-  }
-  const double r = red[0];
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  const int t = threadIdx.x;
+  if ((t & 63) == 0) red[t >> 6] = x;
+  __syncthreads();
+  double r = 0.0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) r += red[w];
   __syncthreads();
   return r;
 }
@@ -255,10 +260,10 @@ int tn_lanczos_f64(const double* A, int64_t n, double* psi, int64_t* iters, doub
              "tn_lanczos_f64: max_iter too large");
   hipStream_t st = tn::stream_of(stream);
   double* Vm = nullptr;
-  double* dsc = nullptr;   // [0..1] alpha, beta of the step; [2] start norm; [3..] z
+  double* dsc = nullptr;   // [2] start norm; [3, 3 + M) z; [3 + M, 3 + 3M) alpha, beta per step
   int rc = tn::pool_alloc((void**)&Vm, sizeof(double) * (size_t)(M + 1) * (size_t)n);
   if (rc != TN_OK) return rc;
-  rc = tn::pool_alloc((void**)&dsc, sizeof(double) * (size_t)(M + 3));
+  rc = tn::pool_alloc((void**)&dsc, sizeof(double) * (size_t)(3 * M + 3));
   if (rc != TN_OK) {
     tn::pool_free(Vm);
     return rc;
@@ -280,26 +285,40 @@ int tn_lanczos_f64(const double* A, int64_t n, double* psi, int64_t* iters, doub
   std::vector<double> As, Bs;
   As.reserve(M);
   Bs.reserve(M);
+  // Steps are launched kBatch at a time, each step's alpha and beta kept on
+  // the device (dsc[3 + M + 2i ...]), and read back together: the host then
+  // applies the stopping rule step by step and ignores steps computed past
+  // the stopping point (their vectors are never combined).
+  constexpr int64_t kBatch = 8;
+  double* d_ab = dsc + 3 + M;
+  std::vector<double> hab((size_t)(2 * M));
+  int64_t done = 0;
   double beta = 1.0, e_diff = 1.0, e0_old = 0.0;
   bool conv = false;
   int64_t it = 0;
   while ((((e_diff > err_tol) && it < M) || it < kMinIter) && beta > kBetaErr) {
-    const double* v = Vm + it * n;
-    const double* vprev = (it == 0) ? v : Vm + (it - 1) * n;
-    double* w = Vm + (it + 1) * n;
-    if (vec)
-      hipLaunchKernelGGL(lz_matvec<true>, dim3((unsigned)((n + 3) / 4)), dim3(kMvThreads), 0, st, A, n, v, vprev,
-                         it == 0 ? 0.0 : beta, w);
-    else
-      hipLaunchKernelGGL(lz_matvec<false>, dim3((unsigned)((n + 3) / 4)), dim3(kMvThreads), 0, st, A, n, v,
-                         vprev, it == 0 ? 0.0 : beta, w);
-    hipLaunchKernelGGL(lz_step, dim3(1), dim3(kStepThreads), 0, st, v, w, n, dsc);
-    if (hipGetLastError() != hipSuccess) return fail((tn::set_error("lanczos: launch failed"), TN_EHIP));
-    double ab[2];
-    rc = tn_memcpy_d2h(ab, dsc, sizeof(ab), st);
-    if (rc != TN_OK) return fail(rc);
-    const double alpha = ab[0];
-    beta = ab[1];
+    if (it == done) {
+      const int64_t k = std::min<int64_t>(kBatch, M - done);
+      for (int64_t i = done; i < done + k; ++i) {
+        const double* v = Vm + i * n;
+        const double* vprev = (i == 0) ? v : Vm + (i - 1) * n;
+        const double* bprev = (i == 0) ? nullptr : d_ab + 2 * (i - 1) + 1;
+        double* w = Vm + (i + 1) * n;
+        if (vec)
+          hipLaunchKernelGGL(lz_matvec<true>, dim3((unsigned)((n + 3) / 4)), dim3(kMvThreads), 0, st, A, n, v,
+                             vprev, bprev, w);
+        else
+          hipLaunchKernelGGL(lz_matvec<false>, dim3((unsigned)((n + 3) / 4)), dim3(kMvThreads), 0, st, A, n, v,
+                             vprev, bprev, w);
+        hipLaunchKernelGGL(lz_step, dim3(1), dim3(kStepThreads), 0, st, v, w, n, d_ab + 2 * i);
+      }
+      if (hipGetLastError() != hipSuccess) return fail((tn::set_error("lanczos: launch failed"), TN_EHIP));
+      rc = tn_memcpy_d2h(hab.data() + 2 * done, d_ab + 2 * done, sizeof(double) * 2 * k, st);
+      if (rc != TN_OK) return fail(rc);
+      done += k;
+    }
+    const double alpha = hab[2 * it];
+    beta = hab[2 * it + 1];
     As.push_back(alpha);
     if (beta > kBetaErr) {
       Bs.push_back(it < M - 1 ? beta : 0.0);
