@@ -171,6 +171,16 @@ int tn_gemm_plan_destroy(tn_gemm_plan plan);
 /* One matrixMul (a group of one), same argument order as the reference. */
 int tn_matmul_f64(const double* A, const double* B, int64_t M, int64_t N, int64_t K,
                   double* C, int mode, void* stream);
+/* Complex FP64, interleaved (re, im) pairs.  tn_permute_dense_c128: the
+ * complex dense permute (UniTensorComplex.cpp) as the real permute with the
+ * pair axis kept last, bit-exact.  tn_matmul_c128: row-major C = A B with
+ * A (M x K), B (K x N) complex (matrixMul on std::complex<double>,
+ * uni10_lapack_cpu.cpp), as one FP64 MFMA product of A (M x 2K real) with
+ * the 2K x 2N real expansion of B; synchronises the stream. */
+int tn_permute_dense_c128(const double* src, double* dst, int rank, const int64_t* dims,
+                          const int* perm, void* stream);
+int tn_matmul_c128(const double* A, const double* B, int64_t M, int64_t N, int64_t K, double* C,
+                   void* stream);
 
 /* ---- vectors / diagonal scaling -----------------------------------------
  * vectorNorm (uni10_lapack_cpu.cpp:397-414), vectorScal (:75-89),

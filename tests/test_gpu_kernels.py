@@ -837,3 +837,38 @@ def test_lanczos_c_abi(tn, ora):
     tn.call("tn_lanczos_f64", tn._p(A), n, tn._p(psi2), ctypes.byref(it2), 5e-15, ctypes.byref(e0),
             ctypes.byref(conv), None)
     assert abs(e0.value - e_o) <= 1e-12 * np.abs(np.linalg.eigvalsh(S)).max()
+
+
+@pytest.mark.parametrize("dims,perm", [((7,), (0,)), ((3, 5), (1, 0)), ((4, 3, 5), (2, 0, 1)),
+                                       ((256, 2, 2, 256), (3, 1, 0, 2)), ((2, 3, 1, 4, 5), (4, 2, 0, 3, 1))])
+def test_complex_permute_bit_exact(tn, dims, perm):
+    # complex dense permute: numpy's transpose of the complex array, bit for bit
+    rng = np.random.default_rng(len(dims))
+    n = int(np.prod(dims))
+    z = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    src = tn.DeviceArray.from_host(z.view(np.float64))
+    dst = tn.DeviceArray((2 * n,))
+    d = np.array(dims, np.int64)
+    p = np.array(perm, np.int32)
+    tn.call("tn_permute_dense_c128", tn._p(src), tn._p(dst), len(dims), tn._p(d), tn._p(p), None)
+    out = dst.to_host().view(np.complex128)
+    want = np.ascontiguousarray(z.reshape(dims).transpose(perm)).reshape(-1)
+    assert np.array_equal(out.view(np.float64), want.view(np.float64))
+
+
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (17, 33, 9), (128, 64, 256), (300, 200, 100), (5, 7, 0)])
+def test_complex_matmul(tn, M, N, K):
+    # complex product against numpy's (the textbook routine), 1e-12 relative
+    rng = np.random.default_rng(M + N + K)
+    a = rng.uniform(-1, 1, (M, K)) + 1j * rng.uniform(-1, 1, (M, K))
+    b = rng.uniform(-1, 1, (K, N)) + 1j * rng.uniform(-1, 1, (K, N))
+    A = tn.DeviceArray.from_host(np.ascontiguousarray(a).view(np.float64).reshape(-1)) if K else None
+    B = tn.DeviceArray.from_host(np.ascontiguousarray(b).view(np.float64).reshape(-1)) if K else None
+    C = tn.DeviceArray.from_host(np.full(2 * M * N, np.nan))
+    tn.call("tn_matmul_c128", tn._p(A), tn._p(B), M, N, K, tn._p(C), None)
+    c = C.to_host().view(np.complex128).reshape(M, N)
+    want = a @ b
+    if K == 0:
+        assert np.array_equal(c, np.zeros((M, N), complex))
+    else:
+        assert np.linalg.norm(c - want) <= 1e-12 * np.linalg.norm(want)

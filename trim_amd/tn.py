@@ -63,6 +63,8 @@ SIGNATURES = {
     "tn_gemm_exec": [P, P, P, P, P],
     "tn_gemm_plan_destroy": [P],
     "tn_matmul_f64": [P, P, I64, I64, I64, P, ctypes.c_int, P],
+    "tn_permute_dense_c128": [P, P, ctypes.c_int, P, P, P],
+    "tn_matmul_c128": [P, P, I64, I64, I64, P, P],
     "tn_norm2_f64": [P, I64, P, P],
     "tn_scale_f64": [P, I64, ctypes.c_double, P],
     "tn_scale_inv_dev_f64": [P, I64, P, P],
