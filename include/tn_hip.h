@@ -266,6 +266,12 @@ int tn_eigh_f64(const double* A, int64_t N, double* eig, double* eigvec_rows,
  * reference does.  Synchronises the stream. */
 int tn_lanczos_f64(const double* A, int64_t n, double* psi, int64_t* iters, double err_tol,
                    double* E0, int* converged, void* stream);
+/* The complex form (lanczosEV on std::complex<double>,
+ * uni10_lapack_cpu.cpp:905-1000): A (n x n) and psi (n) complex, interleaved
+ * (re, im) pairs, 16-byte aligned; alpha = Re(v^H w); same arguments,
+ * stopping rule and sign convention (<start|psi> real and >= 0). */
+int tn_lanczos_c128(const double* A, int64_t n, double* psi, int64_t* iters, double err_tol,
+                    double* E0, int* converged, void* stream);
 
 /* ---- TEBD truncation ------------------------------------------------------
  * The truncate-and-renormalise of iTEBD.cpp:157-161 plus the discarded

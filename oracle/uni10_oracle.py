@@ -821,6 +821,7 @@ def lanczos_ev(A: np.ndarray, psi: np.ndarray, max_iter: int = 1000, err_tol: fl
     tridiagonal matrix (dstev, :464-477; here SciPy's eigh_tridiagonal);
     stop on a relative change <= err_tol against max(|E0|, 1), on
     beta <= 1e-15, or at max_iter, never before 2 steps (:418-419, :431).
+    Complex input follows the complex form (:905-1000): alpha = Re(v^H w).
     Returns (E0, psi, steps, converged).  The eigenvector's sign is left to
     dstev in the reference; here it is fixed so that <start|psi> >= 0.
     A space that closes after one step returns E0 = 0 and psi unchanged
@@ -829,11 +830,13 @@ def lanczos_ev(A: np.ndarray, psi: np.ndarray, max_iter: int = 1000, err_tol: fl
     import scipy.linalg as sla
     if not max_iter > 2:
         raise ValueError("Maximum iteration number should be set greater than 2.")
-    A = np.asarray(A, dtype=np.float64)
+    dt = np.complex128 if (np.iscomplexobj(A) or np.iscomplexobj(psi)) else np.float64
+    A = np.asarray(A, dtype=dt)
     N = A.shape[0]
-    psi = np.asarray(psi, dtype=np.float64).reshape(-1)
-    V = np.zeros((max_iter + 1, N))
-    V[0] = psi * (1.0 / vector_norm(psi))
+    psi = np.asarray(psi, dtype=dt).reshape(-1)
+    V = np.zeros((max_iter + 1, N), dtype=dt)
+    nrm = lambda x: float(math.sqrt(float(np.vdot(x, x).real)))
+    V[0] = psi * (1.0 / nrm(psi))
     As, Bs = [], []
     beta, e_diff, e0_old, it, conv = 1.0, 1.0, 0.0, 0, False
     beta_err = 1e-15
@@ -849,9 +852,9 @@ def lanczos_ev(A: np.ndarray, psi: np.ndarray, max_iter: int = 1000, err_tol: fl
 
     while (((e_diff > err_tol) and it < max_iter) or it < 2) and beta > beta_err:
         w = A @ V[it] - (beta * V[it - 1] if it > 0 else 0.0)
-        alpha = float(np.dot(V[it], w))
+        alpha = float(np.vdot(V[it], w).real)      # zdotc's real part for complex (:937-938)
         w = w - alpha * V[it]
-        beta = vector_norm(w)
+        beta = nrm(w)
         As.append(alpha)
         if beta > beta_err:
             V[it + 1] = w * (1.0 / beta)

@@ -872,3 +872,30 @@ def test_complex_matmul(tn, M, N, K):
         assert np.array_equal(c, np.zeros((M, N), complex))
     else:
         assert np.linalg.norm(c - want) <= 1e-12 * np.linalg.norm(want)
+
+
+@pytest.mark.parametrize("n", [10, 64, 300])
+def test_lanczos_complex_c_abi(tn, ora, n):
+    # tn_lanczos_c128 against the oracle's complex lanczosEV
+    import ctypes
+    rng = np.random.default_rng(60 + n)
+    X = rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n))
+    S = np.ascontiguousarray(X @ X.conj().T + X.conj().T @ X)
+    start = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    A = tn.DeviceArray.from_host(S.view(np.float64).reshape(-1))
+    psi = tn.DeviceArray.from_host(start.view(np.float64))
+    it = ctypes.c_int64(1000)
+    e0 = ctypes.c_double()
+    conv = ctypes.c_int()
+    tn.call("tn_lanczos_c128", tn._p(A), n, tn._p(psi), ctypes.byref(it), 5e-15, ctypes.byref(e0),
+            ctypes.byref(conv), None)
+    e_o, psi_o, it_o, conv_o = ora.lanczos_ev(S, start)
+    scale = np.abs(np.linalg.eigvalsh(S)).max()
+    assert conv.value == 1 and conv_o
+    assert abs(e0.value - e_o) <= 1e-12 * scale
+    assert abs(it.value - it_o) <= 3
+    v = psi.to_host().view(np.complex128)
+    assert np.vdot(start, v).real > 0
+    cos = abs(np.vdot(v, psi_o)) / (np.linalg.norm(v) * np.linalg.norm(psi_o))
+    assert cos > 1 - 1e-8
+    assert np.linalg.norm(S @ v - e0.value * v) <= 1e-4 * scale * np.linalg.norm(v)

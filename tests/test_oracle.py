@@ -569,3 +569,19 @@ def test_lanczos_edge_cases(ora):
     X = rng.uniform(-1, 1, (200, 200))
     e0, psi, it, conv = ora.lanczos_ev(X + X.T, rng.uniform(-1, 1, 200), max_iter=3)
     assert it == 3 and not conv
+
+
+@pytest.mark.parametrize("n", [10, 64])
+def test_lanczos_complex_hermitian_matches_eigh(ora, n):
+    # complex form (uni10_lapack_cpu.cpp:905-1000) on S = H H^+ + H^+ H
+    rng = np.random.default_rng(50 + n)
+    X = rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n))
+    S = X @ X.conj().T + X.conj().T @ X
+    start = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    e0, psi, it, conv = ora.lanczos_ev(S, start)
+    w = np.linalg.eigvalsh(S)
+    assert conv
+    assert abs(e0 - w[0]) <= 1e-12 * abs(w[-1])
+    ov = np.vdot(start, psi)
+    assert ov.real > 0 and abs(ov.imag) <= 1e-6 * abs(ov)
+    assert np.linalg.norm(S @ psi - e0 * psi) <= 1e-4 * abs(w[-1]) * np.linalg.norm(psi)

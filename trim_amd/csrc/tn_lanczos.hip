@@ -84,6 +84,52 @@ __global__ void __launch_bounds__(kMvThreads) lz_matvec(const double* __restrict
   if (lane == 0) w[row] = t - beta * vprev[row];
 }
 
+// Complex rows (interleaved re, im): w[r] = A[r, :] v - beta * vprev[r], no
+// conjugation (zgemv 'T' on the column-major view, uni10_lapack_cpu.cpp:935).
+// Pairs are 16-byte aligned; four partial sums per lane and component.
+__global__ void __launch_bounds__(kMvThreads) lzc_matvec(const double* __restrict__ A, int64_t n,
+                                                         const double* __restrict__ v,
+                                                         const double* __restrict__ vprev,
+                                                         const double* __restrict__ beta_prev,
+                                                         double* __restrict__ w) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * (kMvThreads / 64) + (threadIdx.x >> 6);
+  if (row >= n) return;
+  const double beta = beta_prev ? *beta_prev : 0.0;
+  const double2* __restrict__ a = reinterpret_cast<const double2*>(A + row * 2 * n);
+  const double2* __restrict__ x = reinterpret_cast<const double2*>(v);
+  double sr[4] = {0.0, 0.0, 0.0, 0.0}, si[4] = {0.0, 0.0, 0.0, 0.0};
+  int64_t j = lane;
+  for (; j + 192 < n; j += 256) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double2 p = a[j + 64 * u], q = x[j + 64 * u];
+      sr[u] = fma(p.x, q.x, sr[u]);
+      sr[u] = fma(-p.y, q.y, sr[u]);
+      si[u] = fma(p.x, q.y, si[u]);
+      si[u] = fma(p.y, q.x, si[u]);
+    }
+  }
+  for (; j < n; j += 64) {
+    const double2 p = a[j], q = x[j];
+    sr[0] = fma(p.x, q.x, sr[0]);
+    sr[0] = fma(-p.y, q.y, sr[0]);
+    si[0] = fma(p.x, q.y, si[0]);
+    si[0] = fma(p.y, q.x, si[0]);
+  }
+  double tr = (sr[0] + sr[1]) + (sr[2] + sr[3]);
+  double ti = (si[0] + si[1]) + (si[2] + si[3]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    tr += __shfl_xor(tr, off);
+    ti += __shfl_xor(ti, off);
+  }
+  if (lane == 0) {
+    w[2 * row] = tr - beta * vprev[2 * row];
+    w[2 * row + 1] = ti - beta * vprev[2 * row + 1];
+  }
+}
+
 // Sum over the block in a fixed order: a butterfly inside each wave, then
 // the wave totals in wave order.
 __device__ double block_sum(double x, double* red) {
@@ -249,19 +295,24 @@ std::vector<double> tridiag_eigvec(const std::vector<double>& d, const std::vect
 
 }  // namespace
 
-int tn_lanczos_f64(const double* A, int64_t n, double* psi, int64_t* iters, double err_tol, double* E0,
-                   int* converged, void* stream) {
-  TN_INIT_OR_RETURN();
+// The complex form differs only in the product: alpha = Re(v^H w) (zdotc,
+// :937-938), the axpy with a real alpha, the norm and the scaling are the
+// real operations on the 2n interleaved doubles.
+static int lanczos_impl(const double* A, int64_t n, bool cplx, double* psi, int64_t* iters, double err_tol,
+                        double* E0, int* converged, void* stream) {
   TN_REQUIRE(A && psi && iters && E0 && converged, TN_EINVAL, "tn_lanczos_f64: NULL argument");
   TN_REQUIRE(n >= 1, TN_EINVAL, "tn_lanczos_f64: n=%lld", (long long)n);
   TN_REQUIRE(*iters > kMinIter, TN_EINVAL, "Maximum iteration number should be set greater than 2.");
   const int64_t M = *iters;
-  TN_REQUIRE(M < (int64_t(1) << 31) && (M + 1) <= INT64_MAX / n / 8, TN_EUNSUPP,
-             "tn_lanczos_f64: max_iter too large");
+  const int64_t L = cplx ? 2 * n : n;   // doubles per vector
+  TN_REQUIRE(M < (int64_t(1) << 31) && (M + 1) <= INT64_MAX / L / 8, TN_EUNSUPP,
+             "tn_lanczos: max_iter too large");
+  TN_REQUIRE(!cplx || ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(psi)) & 15) == 0,
+             TN_EINVAL, "tn_lanczos_c128: A and psi must be 16-byte aligned");
   hipStream_t st = tn::stream_of(stream);
   double* Vm = nullptr;
   double* dsc = nullptr;   // [2] start norm; [3, 3 + M) z; [3 + M, 3 + 3M) alpha, beta per step
-  int rc = tn::pool_alloc((void**)&Vm, sizeof(double) * (size_t)(M + 1) * (size_t)n);
+  int rc = tn::pool_alloc((void**)&Vm, sizeof(double) * (size_t)(M + 1) * (size_t)L);
   if (rc != TN_OK) return rc;
   rc = tn::pool_alloc((void**)&dsc, sizeof(double) * (size_t)(3 * M + 3));
   if (rc != TN_OK) {
@@ -274,10 +325,10 @@ int tn_lanczos_f64(const double* A, int64_t n, double* psi, int64_t* iters, doub
     tn::pool_free(dsc);
     return code;
   };
-  const unsigned vb = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+  const unsigned vb = (unsigned)std::min<int64_t>((L + 255) / 256, 4096);
   // v0 = psi / ||psi||  (uni10_lapack_cpu.cpp:437-438)
-  hipLaunchKernelGGL(lz_norm, dim3(1), dim3(kStepThreads), 0, st, psi, n, dsc + 2);
-  hipLaunchKernelGGL(lz_scale_copy, dim3(vb), dim3(256), 0, st, psi, dsc + 2, n, Vm);
+  hipLaunchKernelGGL(lz_norm, dim3(1), dim3(kStepThreads), 0, st, psi, L, dsc + 2);
+  hipLaunchKernelGGL(lz_scale_copy, dim3(vb), dim3(256), 0, st, psi, dsc + 2, L, Vm);
   if (hipGetLastError() != hipSuccess) return fail((tn::set_error("lanczos: launch failed"), TN_EHIP));
 
   // rows of A and the Lanczos vectors (row stride n) are 16-byte aligned
@@ -300,17 +351,20 @@ int tn_lanczos_f64(const double* A, int64_t n, double* psi, int64_t* iters, doub
     if (it == done) {
       const int64_t k = std::min<int64_t>(kBatch, M - done);
       for (int64_t i = done; i < done + k; ++i) {
-        const double* v = Vm + i * n;
-        const double* vprev = (i == 0) ? v : Vm + (i - 1) * n;
+        const double* v = Vm + i * L;
+        const double* vprev = (i == 0) ? v : Vm + (i - 1) * L;
         const double* bprev = (i == 0) ? nullptr : d_ab + 2 * (i - 1) + 1;
-        double* w = Vm + (i + 1) * n;
-        if (vec)
+        double* w = Vm + (i + 1) * L;
+        if (cplx)
+          hipLaunchKernelGGL(lzc_matvec, dim3((unsigned)((n + 3) / 4)), dim3(kMvThreads), 0, st, A, n, v, vprev,
+                             bprev, w);
+        else if (vec)
           hipLaunchKernelGGL(lz_matvec<true>, dim3((unsigned)((n + 3) / 4)), dim3(kMvThreads), 0, st, A, n, v,
                              vprev, bprev, w);
         else
           hipLaunchKernelGGL(lz_matvec<false>, dim3((unsigned)((n + 3) / 4)), dim3(kMvThreads), 0, st, A, n, v,
                              vprev, bprev, w);
-        hipLaunchKernelGGL(lz_step, dim3(1), dim3(kStepThreads), 0, st, v, w, n, d_ab + 2 * i);
+        hipLaunchKernelGGL(lz_step, dim3(1), dim3(kStepThreads), 0, st, v, w, L, d_ab + 2 * i);
       }
       if (hipGetLastError() != hipSuccess) return fail((tn::set_error("lanczos: launch failed"), TN_EHIP));
       rc = tn_memcpy_d2h(hab.data() + 2 * done, d_ab + 2 * done, sizeof(double) * 2 * k, st);
@@ -340,7 +394,7 @@ int tn_lanczos_f64(const double* A, int64_t n, double* psi, int64_t* iters, doub
     const std::vector<double> z = tridiag_eigvec(As, Bs, (int)it, lam);
     rc = tn_memcpy_h2d(dsc + 3, z.data(), sizeof(double) * (size_t)it, st);
     if (rc != TN_OK) return fail(rc);
-    hipLaunchKernelGGL(lz_combine, dim3(vb), dim3(256), 0, st, Vm, dsc + 3, it, n, psi);
+    hipLaunchKernelGGL(lz_combine, dim3(vb), dim3(256), 0, st, Vm, dsc + 3, it, L, psi);
     if (hipGetLastError() != hipSuccess) return fail((tn::set_error("lanczos: launch failed"), TN_EHIP));
     *iters = it;
     *E0 = lam;
@@ -356,4 +410,16 @@ int tn_lanczos_f64(const double* A, int64_t n, double* psi, int64_t* iters, doub
   tn::pool_free(Vm);
   tn::pool_free(dsc);
   return rc;
+}
+
+int tn_lanczos_f64(const double* A, int64_t n, double* psi, int64_t* iters, double err_tol, double* E0,
+                   int* converged, void* stream) {
+  TN_INIT_OR_RETURN();
+  return lanczos_impl(A, n, false, psi, iters, err_tol, E0, converged, stream);
+}
+
+int tn_lanczos_c128(const double* A, int64_t n, double* psi, int64_t* iters, double err_tol, double* E0,
+                    int* converged, void* stream) {
+  TN_INIT_OR_RETURN();
+  return lanczos_impl(A, n, true, psi, iters, err_tol, E0, converged, stream);
 }

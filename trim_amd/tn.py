@@ -85,6 +85,8 @@ SIGNATURES = {
     "tn_eigh_f64": [P, I64, P, P, P],
     "tn_lanczos_f64": [P, I64, P, PI64, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                        ctypes.POINTER(ctypes.c_int), P],
+    "tn_lanczos_c128": [P, I64, P, PI64, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                        ctypes.POINTER(ctypes.c_int), P],
     "tn_truncate_f64": [P, I64, I64, P, P, P, P],
 }
 
