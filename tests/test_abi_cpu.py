@@ -85,3 +85,47 @@ def test_no_oracle_in_product():
             if f.endswith((".py", ".cpp", ".hip", ".h")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in text.lower().replace("oracle-free", ""), os.path.join(dirpath, f)
+
+
+CALL_SITES = r'''
+// The reference's own call sites for the hot path, unchanged
+// (iTEBD.cpp:148-216, UniTensor.h:583-592,973, Network.h:74-127, Block.h:389)
+#include "uni10_amd/uni10.hpp"
+using namespace uni10;
+int main() {
+  std::vector<Bond> bonds = {Bond(BD_IN, 4), Bond(BD_IN, 2), Bond(BD_OUT, 4)};
+  UniTensor A(bonds, "A"), B(bonds, "B");
+  std::vector<int> la = {0, 1, 2}, lb = {2, 3, 4};
+  A.setLabel(la);
+  B.setLabel(lb);
+  UniTensor C = contract(A, B, true);
+  std::vector<int> order = {4, 1, 0, 3};
+  C.permute(order, 2);
+  Network net("itebd.net");
+  net.putTensor("ALa", &A);
+  net.putTensor("BLb", B);
+  UniTensor T = net.launch();
+  Matrix M = T.getBlock();
+  std::vector<Matrix> usv = M.svd();
+  std::vector<Matrix> eig = usv[0].eigh();
+  Matrix psi(M.row(), 1);
+  double E0 = 0.0;
+  size_t it = lanczosEigh(eig[1], E0, psi, 100, 1e-12);
+  return (int)it + (int)T.elemNum() + (int)takeExp(-0.01, eig[1]).row();
+}
+'''
+
+
+def test_cpp_api_compiles_and_links_as_the_reference_call_sites(built, tmp_path):
+    # the C++ drop-in keeps the reference's signatures: a program written
+    # against them builds and links against libuni10_amd.so (not run here:
+    # no GPU in this container)
+    src = tmp_path / "call_sites.cpp"
+    src.write_text(CALL_SITES)
+    exe = tmp_path / "call_sites"
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    pkg = os.path.dirname(built.U10_LIB)
+    r = subprocess.run(["g++", "-std=c++17", "-I", os.path.join(root, "include"), str(src), "-o", str(exe),
+                        "-L", pkg, "-luni10_amd", "-ltn_hip", "-Wl,-rpath," + pkg],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout
