@@ -300,8 +300,8 @@ std::vector<double> tridiag_eigvec(const std::vector<double>& d, const std::vect
 // real operations on the 2n interleaved doubles.
 static int lanczos_impl(const double* A, int64_t n, bool cplx, double* psi, int64_t* iters, double err_tol,
                         double* E0, int* converged, void* stream) {
-  TN_REQUIRE(A && psi && iters && E0 && converged, TN_EINVAL, "tn_lanczos_f64: NULL argument");
-  TN_REQUIRE(n >= 1, TN_EINVAL, "tn_lanczos_f64: n=%lld", (long long)n);
+  TN_REQUIRE(A && psi && iters && E0 && converged, TN_EINVAL, "tn_lanczos: NULL argument");
+  TN_REQUIRE(n >= 1, TN_EINVAL, "tn_lanczos: n=%lld", (long long)n);
   TN_REQUIRE(*iters > kMinIter, TN_EINVAL, "Maximum iteration number should be set greater than 2.");
   const int64_t M = *iters;
   const int64_t L = cplx ? 2 * n : n;   // doubles per vector
