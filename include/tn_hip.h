@@ -110,6 +110,17 @@ int tn_blockperm_exec(tn_blockperm_plan plan, const double* src, double* dst,
                       void* stream);
 int tn_blockperm_plan_destroy(tn_blockperm_plan plan);
 
+/* In-place scaling of rectangular sub-blocks of an element buffer: the
+ * swap-gate sign loop of addGate (UniTensorReal.cpp:868-921), which
+ * multiplies every Qidx sub-block whose fermion-parity product is odd by -1.
+ * rects[4*i..4*i+3] = {element offset of the corner, rows, cols, row pitch}
+ * (host array, uploaded once); exec multiplies every listed element by a.
+ * Rectangles must not overlap. */
+typedef struct tn_rectscale_plan_s* tn_rectscale_plan;
+int tn_rectscale_plan_create(tn_rectscale_plan* plan, int64_t nrects, const int64_t* rects);
+int tn_rectscale_exec(tn_rectscale_plan plan, double* base, double a, void* stream);
+int tn_rectscale_plan_destroy(tn_rectscale_plan plan);
+
 /* Row-major transpose AT(N x M) = A(M x N)^T; replaces setTranspose
  * (uni10_lapack_cpu.cpp:350-355; GPU uni10_lapack_gpu.cu:568-598). */
 int tn_transpose_f64(const double* A, int64_t M, int64_t N, double* AT, void* stream);
@@ -220,6 +231,12 @@ int tn_svd_last_sweeps(int* sweeps);
  * for everything beyond the small kernel).  Applies to plans created
  * afterwards. */
 int tn_svd_set_block_min(int64_t n_min);
+/* Sweep cap (test and tuning hook, no counterpart in the reference): plans
+ * and calls created afterwards report a problem as not converged once it
+ * has run n sweeps without converging (n = 0 restores the defaults, 60 for
+ * the one-workgroup kernels and 40 for block Jacobi).  The kernels stop
+ * normally at the cap. */
+int tn_svd_set_max_sweeps(int n);
 
 /* Warm start for a nearby matrix (M >= N): V0T is an N x N matrix with
  * orthonormal rows, e.g. the VT of a previous SVD of a matrix close to A;
@@ -235,8 +252,9 @@ int tn_svd_f64_warm(const double* A, int64_t M, int64_t N, const double* V0T, do
  * -1 = cold; u_off[i] = -1: U of problem i is not formed, and U may be NULL
  * at execution when no problem forms it).  tn_svd_plan_exec only launches kernels: it neither allocates
  * nor synchronises, so it can be captured into a graph (tn_graph_begin).
- * tn_svd_plan_check waits for the stream and reports the problems of the
- * last execution that did not converge and the most sweeps any needed.
+ * tn_svd_plan_check waits for the stream and reports the problems that did
+ * not converge and the most sweeps any needed, counted over every execution
+ * since the previous check (or the plan's creation), then clears the counts.
  * The batched matrixSVD of many chains (uni10_lapack_cpu.cpp:289-313 per
  * chain in the reference). */
 typedef struct tn_svd_plan_s* tn_svd_plan;

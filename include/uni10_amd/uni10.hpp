@@ -187,6 +187,12 @@ Matrix RDotR(const Matrix& a, const Matrix& b);
 // not converge.
 size_t lanczosEigh(Matrix& ori_mat, double& E0, Matrix& psi, size_t max_iter = 1000, double err_tol = 5e-15);
 
+// A swap gate between bonds b1 and b2 of one tensor (uni10_tools.h: _Swap).
+struct _Swap {
+  int b1;
+  int b2;
+};
+
 // ---------------------------------------------------------------------------
 // UniTensor (UniTensor.h:58-971), real FP64.
 struct BlockInfo {
@@ -253,6 +259,11 @@ class UniTensor {
   void set_zero();
   void save(const std::string& fname) const;
   std::string printDiagram() const;
+  // Fermionic swap gates (UniTensor.h:664-665): the bond pairs (shared with
+  // Tb, not shared) that cross when this tensor passes Tb, and the in-place
+  // sign (-1)^(sum over gates of prtF(b1) * prtF(b2)) per Qidx sub-block.
+  std::vector<_Swap> exSwap(const UniTensor& Tb) const;
+  void addGate(const std::vector<_Swap>& swaps);
 
   friend UniTensor contract(UniTensor& Ta, UniTensor& Tb, bool fast);
   friend UniTensor contract_impl(UniTensor& Ta, UniTensor& Tb, bool fast, bool allow_gather);
@@ -318,6 +329,10 @@ class Network {
   void fromfile(const std::string& fname);
   void construct();
   void destruct();
+  int rollcall();
+  void preprint(std::ostream& os, const NetNode* nd, int layer) const;
+  void findConOrd(NetNode* nd);
+  void addSwap();
   void matching(NetNode* sbj, NetNode* tar);
   void branch(NetNode* sbj, NetNode* tar);
   UniTensor merge(NetNode* nd);
@@ -330,6 +345,9 @@ class Network {
   std::vector<UniTensor*> tensors;
   std::vector<NetNode*> leafs;
   std::vector<NetNode*> owned;   // internal nodes
+  std::vector<int> conOrder;                  // leaves in pre-order of the tree
+  std::vector<bool> swapflags;                // gates applied to tensors[t]
+  std::vector<std::vector<_Swap>> swaps_arr;  // gates of tensors[t]
   NetNode* root = nullptr;
   bool load = false;
 };

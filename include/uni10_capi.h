@@ -34,6 +34,14 @@ int u10_set_autotune(int on);             /* tile autotuning of new GEMM plans  
  * run_qnums[3*r..3*r+2] and degeneracy run_degs[r]. */
 int u10_tensor_create(int nbonds, const int* types, const int* nruns, const int* run_qnums,
                       const int* run_degs, const char* name, u10_tensor* out);
+/* Bond::combine (Bond.cpp:163-196; use_free_fn = 0: bonds[0].combine(bonds[1])
+ * .combine(bonds[2])...) or the free combine(tp, bonds) (Bond.cpp:198-252;
+ * use_free_fn = 1).  Host-only (no device needed).  Outputs: type, dim, the
+ * run-length charges (up to cap runs, (U1, prt, prtF) triples) and
+ * degeneracy() in Qnum order. */
+int u10_bond_combine(int nbonds, const int* types, const int* nruns, const int* run_qnums, const int* run_degs,
+                     int use_free_fn, int tp, int* out_type, int* out_dim, int* out_nruns, int* out_qnums,
+                     int* out_degs, int* out_ndeg, int* deg_qnums, int* deg_counts, int cap);
 int u10_tensor_scalar(double value, u10_tensor* out);
 int u10_tensor_load(const char* fname, u10_tensor* out);        /* UniTensor(fname) */
 int u10_tensor_save(u10_tensor t, const char* fname);
@@ -60,12 +68,29 @@ int u10_tensor_norm(u10_tensor t, double* out);
 int u10_tensor_device_ptr(u10_tensor t, void** dptr);          /* elements in HBM */
 int u10_contract(u10_tensor a, u10_tensor b, int fast, u10_tensor* out);
 int u10_otimes(u10_tensor a, u10_tensor b, u10_tensor* out);
+/* operator*(UniTensor, UniTensor) (UniTensor.cpp:224-234): contract of copies. */
+int u10_tensor_mul(u10_tensor a, u10_tensor b, u10_tensor* out);
+/* exSwap (UniTensor.cpp:1237-1268): *n gate pairs (b1, b2), the first `cap`
+ * written to pairs[2*i], pairs[2*i+1]. */
+int u10_tensor_ex_swap(u10_tensor a, u10_tensor b, int* pairs, int cap, int* n);
+/* addGate (UniTensorReal.cpp:868-921): n gates as (b1, b2) pairs. */
+int u10_tensor_add_gate(u10_tensor t, int n, const int* pairs);
 
 int u10_network_create(const char* fname, u10_network* out);
 int u10_network_put(u10_network net, const char* name, u10_tensor t, int transposed);
 int u10_network_launch(u10_network net, const char* name, u10_tensor* out);
 int u10_network_tree(u10_network net, char* buf, int len);
+/* operator<<(ostream, Network) (Network.cpp:856-907): tensor list, then the
+ * contraction tree with element counts, in the reference's text format. */
+int u10_network_print(u10_network net, char* buf, int len);
 int u10_network_free(u10_network net);
+
+/* RDotR (BlockTools.cpp:38-76), the Matrix product of the C++ API, on host
+ * arrays: a diagonal operand is given by its min(rows, cols) diagonal
+ * entries; the result (rows x cols, diagonal when both operands are) is
+ * written to c (at most cap doubles). */
+int u10_rdotr(int64_t ar, int64_t ac, int adiag, const double* a, int64_t br, int64_t bc, int bdiag,
+              const double* b, int64_t* cr, int64_t* cc, int* cdiag, double* c, int64_t cap);
 
 /* Matrix::svd / eigh / exph on host arrays (computed on the device). */
 int u10_svd(const double* A, int64_t M, int64_t N, double* U, double* S, double* VT);

@@ -32,14 +32,24 @@ Pinning (what each part is checked against; see tests/test_oracle.py):
                             four-site Heisenberg ring; the eigenvector by its
                             residual.
   * Network               - Simple.net and egN1_network known answers; ORDER
-                            brackets vs greedy plan on the same tensors.
+                            brackets vs greedy plan on the same tensors; the
+                            greedy tree, node element counts and label order
+                            printed in Network.h:145-193 (print_tree).
   * iTEBD energy          - Bethe-ansatz bound 1/4 - ln 2 and the two-site
                             exact ground energy -3/4 at chi=1.
   * rec_swap              - inversion-set identity and permutation sign.
-  * swap_signs_setup      - PARITY UNPINNED: the convention (column bonds
-                            taken in reverse order) follows
-                            UniTensorReal.cpp:657-681; the reference holds no
-                            fermionic permute fixture or test to pin it.
+  * swap_signs_setup      - the convention (row bonds, then column bonds in
+                            reverse, reordered as Grassmann generators;
+                            UniTensorReal.cpp:657-681) against an independent
+                            per-element reordering sign (cycle decomposition),
+                            round trips and composition across row-bond
+                            counts (bit-exact), and hand-derived rank-2 cases.
+  * add_gate / add_swap   - addGate against the per-element parity product;
+                            addSwap's gate list on a three-tensor chain (hand
+                            derived) and order independence of the launched
+                            fermionic result (the Network.h network under
+                            three trees with different gates; a chain against
+                            the plain matrix product).
 """
 from __future__ import annotations
 
@@ -626,6 +636,35 @@ class UniTensor:
     def __getitem__(self, i):
         return float(self.elem[i])
 
+    # fermionic swap gates -----------------------------------------------------
+    def ex_swap(self, Tb: "UniTensor"):
+        """UniTensor::exSwap (UniTensor.cpp:1237-1268): every (shared bond,
+        unshared bond) pair of this tensor with respect to Tb."""
+        if not (self.bonds and Tb.bonds):
+            return []
+        shared = [a for a in range(len(self.labels)) if self.labels[a] in Tb.labels]
+        left = [a for a in range(len(self.labels)) if self.labels[a] not in Tb.labels]
+        return [(i, j) for i in shared for j in left]
+
+    def add_gate(self, swaps) -> "UniTensor":
+        """UniTensor::addGate (UniTensorReal.cpp:868-921): each Qidx sub-block
+        is multiplied by (-1)^(XOR over gates of prtF(b1) & prtF(b2))."""
+        if not self.bonds:
+            raise ValueError("Adding swap gates to a tensor without bonds(scalar).")
+        if not self.have_elem:
+            raise ValueError("Cannot add swap gates to a tensor before setting its elements.")
+        L = self.layout
+        for qidx in L.QidxEnc:
+            q = L.qidx_decode(qidx)
+            bit = 0
+            for b1, b2 in swaps:
+                bit ^= self.bonds[b1].Qnums[q[b1]].prtF & self.bonds[b2].Qnums[q[b2]].prtF
+            if bit:
+                eoff, bcols, sr, sc = L.subblock(qidx)
+                for r in range(sr):
+                    self.elem[eoff + r * bcols:eoff + r * bcols + sc] *= -1.0
+        return self
+
 
 def swap_signs_setup(T: UniTensor, out: UniTensor, rsp, new_labels):
     """Fermionic swap list (UniTensorReal.cpp:657-681 -> recSwap, uni10_tools.cpp:52-69)."""
@@ -779,6 +818,47 @@ def contract(Ta: UniTensor, Tb: UniTensor, fast: bool = False) -> UniTensor:
 # ---------------------------------------------------------------------------
 # Dense linear algebra with the reference's conventions
 # ---------------------------------------------------------------------------
+def otimes(Ta: UniTensor, Tb: UniTensor) -> UniTensor:
+    """otimes (UniTensorTools.cpp:246-273): relabel so that nothing is shared
+    (A's rows, B's rows, A's columns, B's columns) and contract."""
+    T1, T2 = Ta.copy(), Tb.copy()
+    l1 = [i if i < T1.RBondNum else T2.RBondNum + i for i in range(len(T1.bonds))]
+    l2 = [i + T1.RBondNum if i < T2.RBondNum else i + len(T1.bonds) for i in range(len(T2.bonds))]
+    T1.set_label(l1)
+    T2.set_label(l2)
+    return contract(T1, T2, True)
+
+
+def rdotr(a: np.ndarray, a_shape, a_diag: bool, b: np.ndarray, b_shape, b_diag: bool):
+    """RDotR (BlockTools.cpp:38-76): dense x dense -> matrixMul; diagonal x
+    dense -> copy of b resized to (a rows, b cols), row i times a_i
+    (diagRowMul); dense x diagonal -> copy of a resized, column j times b_j
+    (diagColMul); diagonal x diagonal -> diagonal, entrywise product of the
+    first min(|a|, |b|) entries.  A diagonal operand is its entry vector."""
+    (ar, ac), (br, bc) = a_shape, b_shape
+    if ac != br:
+        raise ValueError("The dimensions of the two matrices do not match for matrix multiplication.")
+
+    def resized(m, r, c):                       # Matrix::resize of a dense matrix (MatrixReal.cpp:289-359)
+        out = np.zeros((r, c))
+        rr, cc = min(r, m.shape[0]), min(c, m.shape[1])
+        out[:rr, :cc] = m[:rr, :cc]
+        return out
+    if not a_diag and not b_diag:
+        return np.asarray(a).reshape(ar, ac) @ np.asarray(b).reshape(br, bc), (ar, bc), False
+    if a_diag and not b_diag:
+        c = resized(np.asarray(b).reshape(br, bc), ar, bc)
+        return c * np.asarray(a)[:ar, None], (ar, bc), False
+    if not a_diag and b_diag:
+        c = resized(np.asarray(a).reshape(ar, ac), ar, bc)
+        return c * np.asarray(b)[None, :bc], (ar, bc), False
+    n = min(ar, bc)
+    out = np.zeros(n)
+    k = min(len(a), len(b), n)
+    out[:k] = np.asarray(a)[:k] * np.asarray(b)[:k]
+    return out, (ar, bc), True
+
+
 def svd(M: np.ndarray):
     """Block::svd -> matrixSVD (uni10_lapack_cpu.cpp:289-313).
 
@@ -981,6 +1061,9 @@ class Network:
         self.leafs: List[Optional[Node]] = [None] * n
         self.root: Optional[Node] = None
         self.load = False
+        self.swapflags = [False] * n
+        self.swaps_arr: List[list] = [[] for _ in range(n)]
+        self.con_order: List[int] = []
 
     def _fromfile(self, fname):
         """Network::fromfile (Network.cpp:271-426)."""
@@ -1063,6 +1146,7 @@ class Network:
             self.leafs[idx] = Node(t)
         else:
             self.leafs[idx].T = t
+            self.swapflags[idx] = False
 
     def put_tensor_T(self, key, T: UniTensor, force: bool = True):
         """Network::putTensorT (Network.cpp:576-591)."""
@@ -1070,12 +1154,48 @@ class Network:
         self.put_tensor(key, t, force)
 
     def destruct(self):
+        """Network::destruct (Network.cpp:679-695): applied gates are applied
+        again (undone), the gate lists cleared."""
         self.root = None
         for lf in self.leafs:
             if lf is not None:
                 lf.parent = lf.left = lf.right = None
                 lf.point = 0.0
+        self.con_order = []
+        for t in range(len(self.tensors)):
+            if Qnum.fermionic and self.swapflags[t]:
+                self.tensors[t].add_gate(self.swaps_arr[t])
+                self.swapflags[t] = False
+            self.swaps_arr[t] = []
         self.load = False
+
+    def _find_con_ord(self, nd):
+        """Network::findConOrd (Network.cpp:923-942): leaves in pre-order."""
+        if nd is None or len(self.con_order) == len(self.tensors):
+            return
+        if nd.T is not None:
+            pos = [i for i, t in enumerate(self.tensors) if t is nd.T]
+            if not pos:
+                raise RuntimeError("Fatal error(code = N2).")
+            self.con_order.append(pos[0])
+        self._find_con_ord(nd.left)
+        self._find_con_ord(nd.right)
+
+    def add_swap(self):
+        """Network::addSwap (Network.cpp:944-993)."""
+        n = len(self.leafs)
+        self._find_con_ord(self.root)
+        if len(self.con_order) != n:
+            raise RuntimeError("Fatal error(code = N3).")
+        for b1, b2 in rec_swap(self.con_order, list(range(n))):
+            self.swaps_arr[b1] += self.tensors[b1].ex_swap(self.tensors[b2])
+        for t in range(n):
+            nb = len(self.tensors[t].bonds)
+            recs: Dict[int, bool] = {}
+            for a, b in self.swaps_arr[t]:
+                h = min(a, b) * nb + max(a, b)
+                recs[h] = not recs[h] if h in recs else True
+            self.swaps_arr[t] = [(h // nb, h % nb) for h in sorted(recs) if recs[h]]
 
     def construct(self):
         """Network::construct (Network.cpp:428-506)."""
@@ -1111,6 +1231,7 @@ class Network:
         tout = self.label_arr[-1]
         if len(self.root.labels) != len(tout) or any(l not in tout for l in self.root.labels):
             raise ValueError("The labels of the resulting tensor do not match with the labels of 'TOUT'")
+        self.add_swap()
         self.load = True
 
     def _branch(self, sbj: Node, tar: Node):
@@ -1180,12 +1301,45 @@ class Network:
         """Network::launch (Network.cpp:697-717)."""
         if not self.load:
             self.construct()
+        for t in range(len(self.tensors)):
+            if Qnum.fermionic and not self.swapflags[t]:
+                self.tensors[t].add_gate(self.swaps_arr[t])
+                self.swapflags[t] = True
         T = self._merge(self.root)
         tout = self.label_arr[-1]
         if len(tout) > 1:
             T.permute(tout, self.Rnums[-1])
         T.name = name
         return T
+
+    def print_tree(self) -> str:
+        """operator<< (Network.cpp:856-907): the tensor list, then the tree
+        pre-order as "name(elemNum): labels," with "|   " per level."""
+        out = ["\n"]
+        for nm, labs, r in zip(self.names, self.label_arr, self.Rnums):
+            line = nm + ": "
+            if r:
+                line += "i[" + ", ".join(map(str, labs[:r])) + "] "
+            if len(labs) - r:
+                line += "o[" + ", ".join(map(str, labs[r:])) + "]"
+            out.append(line + "\n")
+        out.append("\n")
+        if all(lf is not None for lf in self.leafs):
+            if not self.load:
+                self.construct()
+
+            def rec(n, layer):
+                if n is None:
+                    return
+                head = "|   " * layer + ("%s(%d): " % (n.T.name, n.elemNum) if n.T is not None
+                                         else "*(%d): " % n.elemNum)
+                out.append(head + "".join("%d, " % l for l in n.labels) + "\n")
+                rec(n.left, layer + 1)
+                rec(n.right, layer + 1)
+            rec(self.root, 0)
+        else:
+            out.append("\nSome tensors have not yet been given!\n\n")
+        return "".join(out)
 
     def tree(self) -> list:
         """Nested (left, right) tuples of leaf names; for checking the planner."""

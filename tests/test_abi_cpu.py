@@ -129,3 +129,28 @@ def test_cpp_api_compiles_and_links_as_the_reference_call_sites(built, tmp_path)
                         "-L", pkg, "-luni10_amd", "-ltn_hip", "-Wl,-rpath," + pkg],
                        stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     assert r.returncode == 0, r.stdout
+
+
+def test_product_bond_combine_known_answer_and_oracle(built, ora):
+    """The product's Bond::combine (host code, no device) against
+    testBond.cpp:101-160 (dimension 24, degeneracies -2:1, -1:5, 0:9, 1:7,
+    2:2) and, for the free combine(tp, bonds) over mixed in/out bonds,
+    against the oracle's run-length result and degeneracy map."""
+    from trim_amd import uni10 as u
+    u.load()
+    bd = u.Bond(u.BD_IN, [(1, 0, 0), (1, 0, 0), (0, 0, 0), (0, 0, 0), (0, 0, 0), (-1, 0, 0)])
+    bd2 = u.Bond(u.BD_IN, [(1, 0, 0), (0, 0, 0), (0, 0, 0), (-1, 0, 0)])
+    tp, dim, runs, deg = u.combine_bonds([bd2, bd])
+    assert tp == u.BD_IN and dim == 24
+    assert [(q.U1, n) for q, n in deg] == [(-2, 1), (-1, 5), (0, 9), (1, 7), (2, 2)]
+    Q = ora.Qnum
+    spec = [(ora.BD_IN, [Q(1), Q(-1)]), (ora.BD_OUT, [Q(2), Q(0), Q(0), Q(1, 0, 1)]),
+            (ora.BD_IN, [Q(1), Q(0), Q(-1), Q(0, 1, 0)])]
+    obonds = [ora.Bond(t, qs) for t, qs in spec]
+    pbonds = [u.Bond(t, [q.triple() for q in qs]) for t, qs in spec]
+    for ttp in (ora.BD_IN, ora.BD_OUT):
+        oc = ora.combine_bonds(ttp, obonds)
+        tp, dim, runs, deg = u.combine_bonds(pbonds, tp=ttp)
+        assert tp == oc.type and dim == oc.dim
+        assert [(q.triple(), n) for q, n in runs] == [(q.triple(), n) for q, n in zip(oc.Qnums, oc.Qdegs)]
+        assert [(q.triple(), n) for q, n in deg] == [(q.triple(), n) for q, n in oc.degeneracy().items()]

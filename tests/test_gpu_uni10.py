@@ -527,3 +527,119 @@ def test_lanczos_errors_and_edge_cases(u10):
     X = rng.uniform(-1, 1, (200, 200))
     with pytest.raises(RuntimeError, match="fails in converging"):
         u10.lanczosEigh(X + X.T, rng.uniform(-1, 1, 200), max_iter=3)
+
+
+def test_network_print_matches_network_h_known_answer(u10, ora, tmp_path):
+    """The product's greedy planner and its operator<< reproduce the tree,
+    node element counts and label order printed in Network.h:145-193."""
+    from netcases import NETWORK_H_NET, NETWORK_H_PRINT, network_h_bonds
+    p = tmp_path / "network_h.net"
+    p.write_text(NETWORK_H_NET)
+    names = [l.split(":")[0] for l in NETWORK_H_NET.splitlines()][:8]
+    net = u10.Network(str(p))
+    spec = ora.Network(str(p))
+    for nm, labs, r in zip(spec.names[:-1], spec.label_arr[:-1], spec.Rnums[:-1]):
+        T = ora.UniTensor(network_h_bonds(ora, labs, r), nm)
+        net.putTensor(nm, to_product(u10, T))
+    assert names == spec.names[:-1]
+    assert str(net) == NETWORK_H_PRINT
+
+
+def test_batch_tebd_reports_a_nonconverged_svd(u10, tn, golden):
+    """BatchTebd.wait checks the SVD plans of both updates over every step
+    launched: with the sweep cap forced to one sweep from the host, the first
+    (cold) update cannot converge and the step raises instead of returning."""
+    _say("batch tebd with sweep cap 1")
+    tn.call("tn_svd_set_max_sweeps", 1)
+    try:
+        B = u10.BatchTebd([16, 24], [1.0, 0.5], [3, 4], 0.01, golden)
+        with pytest.raises(u10.Uni10Error, match="did not converge"):
+            B.step(1)
+    finally:
+        tn.call("tn_svd_set_max_sweeps", 0)
+    B2 = u10.BatchTebd([16, 24], [1.0, 0.5], [3, 4], 0.01, golden)
+    B2.step(2)
+
+
+def test_otimes_and_operator_mul_values(u10, ora):
+    """otimes and operator*(UniTensor, UniTensor) against the oracle: the
+    tensor product is bit-exact (one product per element), operator* is
+    contract of copies within 1e-12 and leaves its operands untouched."""
+    A, B = _pair_u1(ora, 3)
+    PA, PB = to_product(u10, A), to_product(u10, B)
+    O = ora.otimes(A, B)
+    PO = u10.otimes(PA, PB)
+    assert PO.inBondNum() == O.RBondNum and PO.label() == O.labels
+    assert PO.getElem().tobytes() == O.elem.tobytes()
+    C = ora.contract(A.copy(), B.copy(), True)
+    PC = PA * PB
+    assert PC.label() == C.labels
+    assert np.linalg.norm(PC.getElem() - C.elem) <= 1e-12 * np.linalg.norm(C.elem)
+    assert PA.getElem().tobytes() == A.elem.tobytes() and PA.label() == A.labels
+
+
+@pytest.mark.parametrize("case", [((5, 7), False, (7, 4), False), ((6, 6), True, (6, 3), False),
+                                  ((4, 6), True, (6, 5), False), ((5, 6), False, (6, 6), True),
+                                  ((5, 6), False, (6, 4), True), ((6, 6), True, (6, 6), True),
+                                  ((3, 5), True, (5, 4), True)])
+def test_rdotr_every_branch_matches_oracle(u10, ora, case):
+    """RDotR: matrixMul, diagRowMul, diagColMul and the diagonal product, on
+    square and rectangular diagonal operands (BlockTools.cpp:38-76)."""
+    sa, da, sb, db = case
+    _say("RDotR %s" % (case,))
+    rng = np.random.default_rng(sum(sa) + 10 * sum(sb))
+    a = rng.uniform(-1, 1, min(sa) if da else sa[0] * sa[1])
+    b = rng.uniform(-1, 1, min(sb) if db else sb[0] * sb[1])
+    c, shape, diag = u10.rdotr(a, sa, da, b, sb, db)
+    w, wshape, wdiag = ora.rdotr(a, sa, da, b, sb, db)
+    assert shape == wshape and diag == wdiag
+    if da or db:
+        assert c.tobytes() == np.ascontiguousarray(w).tobytes()    # one product per element
+    else:
+        assert np.linalg.norm(c - w) <= 1e-13 * np.linalg.norm(w)
+    with pytest.raises(u10.Uni10Error, match="do not match"):
+        u10.rdotr(a, sa, da, b, (sb[0] + 1, sb[1]), db)
+
+
+def test_network_file_errors_match_oracle(u10, ora, tmp_path):
+    """The .net errors the oracle raises (missing TOUT, imbalanced ORDER
+    brackets, an unterminated last line dropped, unknown or repeated names
+    in ORDER) are errors of the product's parser too."""
+    cases = ["A: 1; 2\nB: 2; 3\n",
+             "A: 1; 2\nB: 2; 3\nTOUT: 1; 3\nORDER: ((A B)\n",
+             "A: 1; 2\nB: 2; 3\nTOUT: 1; 3",
+             "A: 1; 2\nB: 2; 3\nTOUT: 1; 3\nORDER: A C\n",
+             "A: 1; 2\nB: 2; 3\nTOUT: 1; 3\nORDER: A A\n",
+             "A: 1; 2\nTOUT: 1; 2\n"]
+    for k, text in enumerate(cases):
+        p = tmp_path / ("bad%d.net" % k)
+        p.write_text(text)
+        with pytest.raises(ValueError):
+            ora.Network(str(p))
+        with pytest.raises(u10.Uni10Error):
+            u10.Network(str(p))
+    good = tmp_path / "good.net"
+    good.write_text("A: 1; 2\nB: 2; 3\nTOUT: 1; 3\nORDER: B A\n")
+    u10.Network(str(good))
+    ora.Network(str(good))
+
+
+def test_contract_dimension_mismatch_is_an_error(u10, ora):
+    """Contracting bonds of different dimensions raises in both (the
+    reference's "Cannot contract two bonds having different dimensions")."""
+    A = u10.UniTensor([u10.Bond(u10.BD_IN, 4), u10.Bond(u10.BD_OUT, 5)])
+    B = u10.UniTensor([u10.Bond(u10.BD_IN, 6), u10.Bond(u10.BD_OUT, 3)])
+    A.setElem(np.ones(20))
+    B.setElem(np.ones(18))
+    A.setLabel([1, 2])
+    B.setLabel([2, 3])
+    with pytest.raises(u10.Uni10Error, match="different dimensions"):
+        u10.contract(A, B)
+    oA = ora.UniTensor([ora.Bond(ora.BD_IN, 4), ora.Bond(ora.BD_OUT, 5)])
+    oB = ora.UniTensor([ora.Bond(ora.BD_IN, 6), ora.Bond(ora.BD_OUT, 3)])
+    oA.set_elem(np.ones(20))
+    oB.set_elem(np.ones(18))
+    oA.set_label([1, 2])
+    oB.set_label([2, 3])
+    with pytest.raises(ValueError):
+        ora.contract(oA, oB)

@@ -12,6 +12,9 @@ import os
 import numpy as np
 import pytest
 
+from netcases import (CHAIN3_NET, FERMI_ORDERS, fermi_tensor, NETWORK_H_NET, NETWORK_H_PRINT, network_h_bonds,
+                      network_h_fermionic_bonds)
+
 
 # ---------------------------------------------------------------------------
 # Qnum known answers (uni10/test/testQnum.cpp)
@@ -301,8 +304,10 @@ def test_contract_dense_c2_shape(ora):
     assert C.labels == [0, 1, 4, 5]
     ref = np.tensordot(a.reshape(chi, 2, 2, chi), b.reshape(chi, 2, 2, chi), axes=([2, 3], [1, 0]))
     assert np.allclose(C.elem.reshape(ref.shape), ref, rtol=0, atol=1e-13)
+    before = C.elem.reshape(chi, 2, 2, chi).copy()
     C.permute([5, 1, 0, 4], 2)
-    assert np.array_equal(C.elem.reshape(chi, 2, chi, 2), np.transpose(C.elem.reshape(chi, 2, chi, 2), (0, 1, 2, 3)))
+    # labels (0,1,4,5) -> (5,1,0,4): output axis j is input axis (3,1,0,2)[j]
+    assert C.elem.reshape(chi, 2, chi, 2).tobytes() == np.transpose(before, (3, 1, 0, 2)).tobytes()
 
 
 # ---------------------------------------------------------------------------
@@ -585,3 +590,233 @@ def test_lanczos_complex_hermitian_matches_eigh(ora, n):
     ov = np.vdot(start, psi)
     assert ov.real > 0 and abs(ov.imag) <= 1e-6 * abs(ov)
     assert np.linalg.norm(S @ psi - e0 * psi) <= 1e-4 * abs(w[-1]) * np.linalg.norm(psi)
+
+
+# ---------------------------------------------------------------------------
+# Greedy contraction tree: the known answer printed in Network.h:145-193
+def test_greedy_tree_matches_network_h_known_answer(ora, tmp_path):
+    p = tmp_path / "network_h.net"
+    p.write_text(NETWORK_H_NET)
+    net = ora.Network(str(p))
+    for nm, labs, r in zip(net.names[:-1], net.label_arr[:-1], net.Rnums[:-1]):
+        T = ora.UniTensor(network_h_bonds(ora, labs, r), nm)
+        net.put_tensor(nm, T)
+    assert net.print_tree() == NETWORK_H_PRINT
+
+
+# ---------------------------------------------------------------------------
+# Fermionic signs.  The convention (UniTensorReal.cpp:657-681): a tensor's
+# bonds are ordered row bonds first, then column bonds in reverse, and a
+# permute reorders them as Grassmann generators, one per fermion-odd index.
+# The checks below compute that sign independently (cycle decomposition of
+# the reordering of the odd generators), and check the algebra it must obey.
+def _grassmann_sign_mask(T, rsp, rb_out):
+    """Per element of the dense tensor: the sign of reordering the odd
+    generators from the input order (rows, then reversed columns) to the
+    output order, as the parity of the cycle decomposition."""
+    nb = len(T.bonds)
+    rb = T.RBondNum
+    fin = list(range(rb)) + list(range(nb - 1, rb - 1, -1))
+    fout = [rsp[j] for j in range(rb_out)] + [rsp[j] for j in range(nb - 1, rb_out - 1, -1)]
+    par = [np.array([q.prtF for q in b.Qlist()]) for b in T.bonds]
+    dims = [b.dim for b in T.bonds]
+    mask = np.ones(dims)
+    for idx in np.ndindex(*dims):
+        odd = {b for b in range(nb) if par[b][idx[b]]}
+        a = [b for b in fin if b in odd]
+        t = [b for b in fout if b in odd]
+        pos = {b: k for k, b in enumerate(a)}
+        order = [pos[b] for b in t]
+        mask[idx] = _perm_sign_by_cycles(order)
+    return mask
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_fermionic_permute_is_the_grassmann_reordering(ora, monkeypatch, seed):
+    monkeypatch.setattr(ora.Qnum, "fermionic", True)
+    rng = np.random.default_rng(300 + seed)
+    nb = int(rng.integers(2, 5))
+    T = fermi_tensor(ora, rng, nb, int(rng.integers(0, nb + 1)))
+    if T is None:
+        pytest.skip("no charge-conserving layout drawn")
+    perm = [int(v) for v in rng.permutation(nb)]
+    rb_out = int(rng.integers(0, nb + 1))
+    dense = T.get_raw_elem().reshape([b.dim for b in T.bonds])
+    try:
+        P = T.copy().permute([T.labels[p] for p in perm], rb_out)
+    except ValueError:
+        pytest.skip("permuted bonds carry no common charge")
+    want = np.transpose(dense * _grassmann_sign_mask(T, perm, rb_out), perm)
+    got = P.get_raw_elem().reshape([b.dim for b in P.bonds])
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_fermionic_permute_round_trip_and_composition(ora, monkeypatch, seed):
+    """A permute followed by its inverse is the identity bit for bit, and two
+    permutes (each with its own row-bond count) equal one permute to the
+    final order: the signs compose as the reorderings do."""
+    monkeypatch.setattr(ora.Qnum, "fermionic", True)
+    rng = np.random.default_rng(400 + seed)
+    nb = int(rng.integers(2, 5))
+    T = fermi_tensor(ora, rng, nb, int(rng.integers(0, nb + 1)))
+    if T is None:
+        pytest.skip("no charge-conserving layout drawn")
+    l1 = [T.labels[p] for p in rng.permutation(nb)]
+    l2 = [T.labels[p] for p in rng.permutation(nb)]
+    r1, r2 = int(rng.integers(0, nb + 1)), int(rng.integers(0, nb + 1))
+    try:
+        two = T.copy().permute(l1, r1).permute(l2, r2)
+        one = T.copy().permute(l2, r2)
+        back = T.copy().permute(l1, r1).permute(T.labels, T.RBondNum)
+    except ValueError:
+        pytest.skip("an intermediate layout carries no common charge")
+    assert two.elem.tobytes() == one.elem.tobytes()
+    assert back.elem.tobytes() == T.elem.tobytes()
+
+
+def test_fermionic_sign_known_answers(ora, monkeypatch):
+    """Rank 2, both bonds carrying one even and one odd state: exchanging
+    the two row bonds flips exactly the odd-odd element; moving the column
+    bond into the rows keeps every sign (rows then reversed columns is the
+    same generator order), and moving the row bond into the columns
+    reverses the two (odd-odd flips)."""
+    monkeypatch.setattr(ora.Qnum, "fermionic", True)
+    Q = ora.Qnum
+    qs = [Q(0, ora.PRT_EVEN, ora.PRTF_EVEN), Q(0, ora.PRT_EVEN, ora.PRTF_ODD)]
+
+    def make(rb):
+        T = ora.UniTensor([ora.Bond(ora.BD_IN if b < rb else ora.BD_OUT, qs) for b in range(2)])
+        T.set_label([1, 2])
+        return T
+    raw = np.array([[1.0, 0.0], [0.0, 4.0]])        # parity conserved: (e,e), (o,o)
+    T = make(2)
+    T.set_raw_elem(raw)
+    assert np.array_equal(T.copy().permute([2, 1], 2).get_raw_elem().reshape(2, 2), [[1.0, 0.0], [0.0, -4.0]])
+    T1 = make(1)
+    T1.set_raw_elem(raw)
+    assert np.array_equal(T1.copy().permute([1, 2], 2).get_raw_elem().reshape(2, 2), raw)
+    assert np.array_equal(T1.copy().permute([1, 2], 0).get_raw_elem().reshape(2, 2), [[1.0, 0.0], [0.0, -4.0]])
+
+
+def test_add_gate_is_the_elementwise_parity_sign(ora, monkeypatch):
+    """addGate multiplies element (i_0, .., i_n) by (-1)^(sum over gates of
+    p(i_b1) p(i_b2)); computed here per dense element from its indices."""
+    monkeypatch.setattr(ora.Qnum, "fermionic", True)
+    rng = np.random.default_rng(7)
+    done = 0
+    for _ in range(40):
+        nb = int(rng.integers(2, 5))
+        T = fermi_tensor(ora, rng, nb, int(rng.integers(0, nb + 1)))
+        if T is None:
+            continue
+        gates = [tuple(int(v) for v in rng.choice(nb, 2, replace=False)) for _ in range(int(rng.integers(1, 4)))]
+        dense = T.get_raw_elem().reshape([b.dim for b in T.bonds])
+        par = [np.array([q.prtF for q in b.Qlist()]) for b in T.bonds]
+        sign = np.ones(dense.shape)
+        for idx in np.ndindex(*dense.shape):
+            s = sum(int(par[a][idx[a]]) * int(par[b][idx[b]]) for a, b in gates)
+            sign[idx] = -1.0 if s % 2 else 1.0
+        G = T.copy().add_gate(gates)
+        assert np.array_equal(G.get_raw_elem().reshape(dense.shape), dense * sign)
+        done += 1
+    assert done >= 20
+
+
+def test_add_swap_known_answer(ora, monkeypatch, tmp_path):
+    """Network ((A C) B) over the chain A-B-C: the tree lists the leaves as
+    A, C, B, so B passes C once; B's bond shared with C (label 3, position 1)
+    crosses its other bond (label 2, position 0): one gate (0, 1) on B and
+    none elsewhere.  That gate cancels the signs the permutes inside the
+    contractions pick up along this order, so the result is the one of the
+    order ((A B) C), which needs no gate and no sign: the plain product of
+    the three matrices (fermionic contraction does not depend on the order)."""
+    monkeypatch.setattr(ora.Qnum, "fermionic", True)
+    Q = ora.Qnum
+    qs = [Q(0, ora.PRT_EVEN, ora.PRTF_EVEN), Q(0, ora.PRT_EVEN, ora.PRTF_ODD), Q(0, ora.PRT_EVEN, ora.PRTF_ODD)]
+    rng = np.random.default_rng(5)
+    ts = {}
+    for nm in "ABC":
+        T = ora.UniTensor([ora.Bond(ora.BD_IN, qs), ora.Bond(ora.BD_OUT, qs)], nm)
+        T.set_elem(rng.uniform(-1, 1, T.elem_num()))
+        ts[nm] = T
+    d = {nm: ts[nm].get_raw_elem() for nm in "ABC"}
+    want = d["A"] @ d["B"] @ d["C"]
+    for order, gates in (("((A C) B)", [[], [(0, 1)], []]), ("((A B) C)", [[], [], []])):
+        p = tmp_path / "chain3.net"
+        p.write_text(CHAIN3_NET % order)
+        net = ora.Network(str(p))
+        for nm in "ABC":
+            net.put_tensor(nm, ts[nm])
+        net.construct()
+        assert net.swaps_arr == gates
+        assert np.allclose(net.launch().get_raw_elem(), want, rtol=0, atol=1e-14)
+        # a second launch does not apply the gates again
+        assert np.allclose(net.launch().get_raw_elem(), want, rtol=0, atol=1e-14)
+
+
+def test_fermionic_network_result_independent_of_order(ora, monkeypatch, tmp_path):
+    """Fermionic contraction of a closed network is one number whatever the
+    tree: the greedy tree and two bracketed trees (each with its own swap
+    gates) must give the same scalar."""
+    monkeypatch.setattr(ora.Qnum, "fermionic", True)
+    vals, gatesets = [], []
+    for k, order in enumerate(FERMI_ORDERS):
+        p = tmp_path / ("f%d.net" % k)
+        p.write_text(NETWORK_H_NET.replace("ORDER: " + FERMI_ORDERS[0], "ORDER: " + order))
+        net = ora.Network(str(p))
+        rng = np.random.default_rng(1)
+        for nm, labs, r in zip(net.names[:-1], net.label_arr[:-1], net.Rnums[:-1]):
+            T = ora.UniTensor(network_h_fermionic_bonds(ora, labs, r), nm)
+            T.set_elem(rng.uniform(-1, 1, T.elem_num()))
+            net.put_tensor(nm, T)
+        vals.append(float(net.launch().elem[0]))
+        gatesets.append(net.swaps_arr)
+    assert gatesets[0] != gatesets[2]                  # the trees need different gates
+    assert max(vals) - min(vals) <= 1e-13 * max(1.0, abs(vals[0]))
+
+
+def _dense_of(v, shape, diag):
+    if not diag:
+        return np.asarray(v).reshape(shape)
+    D = np.zeros(shape)
+    k = min(shape)
+    D[np.arange(k), np.arange(k)] = np.asarray(v)[:k]
+    return D
+
+
+RDOTR_CASES = [((5, 7), False, (7, 4), False), ((6, 6), True, (6, 3), False), ((4, 6), True, (6, 5), False),
+               ((5, 6), False, (6, 6), True), ((5, 6), False, (6, 4), True), ((6, 6), True, (6, 6), True),
+               ((3, 5), True, (5, 4), True)]
+
+
+@pytest.mark.parametrize("case", RDOTR_CASES)
+def test_rdotr_is_the_product_of_the_dense_matrices(ora, case):
+    """Every branch of RDotR (dense/diagonal operands, square and
+    rectangular diagonal matrices) equals numpy's product of the operands
+    written out densely."""
+    sa, da, sb, db = case
+    rng = np.random.default_rng(sum(sa) + 10 * sum(sb))
+    a = rng.uniform(-1, 1, min(sa) if da else sa[0] * sa[1])
+    b = rng.uniform(-1, 1, min(sb) if db else sb[0] * sb[1])
+    c, shape, diag = ora.rdotr(a, sa, da, b, sb, db)
+    assert shape == (sa[0], sb[1]) and diag == (da and db)
+    want = _dense_of(a, sa, da) @ _dense_of(b, sb, db)
+    assert np.allclose(_dense_of(c, shape, diag), want, rtol=0, atol=1e-15)
+
+
+def test_otimes_is_the_outer_product(ora):
+    """otimes of two U(1) tensors: the dense result is the outer product with
+    bonds ordered (A rows, B rows, A columns, B columns)."""
+    A = _random_u1(ora, 4)
+    B = _random_u1(ora, 5, n_in=1, n_out=2)
+    O = ora.otimes(A, B)
+    da = A.get_raw_elem().reshape([b.dim for b in A.bonds])
+    db = B.get_raw_elem().reshape([b.dim for b in B.bonds])
+    ra, rb = A.RBondNum, B.RBondNum
+    na, nb = len(A.bonds), len(B.bonds)
+    outer = np.multiply.outer(da, db)
+    order = list(range(ra)) + [na + i for i in range(rb)] + list(range(ra, na)) + [na + i for i in range(rb, nb)]
+    want = np.transpose(outer, order)
+    assert O.RBondNum == ra + rb
+    assert np.array_equal(O.get_raw_elem().reshape(want.shape), want)

@@ -36,6 +36,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <mutex>
 #include <climits>
 #include <cstring>
 #include <vector>
@@ -652,21 +654,34 @@ bool make_map(const tn_index_map& in, int64_t range, int64_t plain_stride, MapDe
 
 }  // namespace
 
-struct tn_gemm_plan_s {
-  int mode = 0;
-  int cfg = -1;
-  std::vector<GroupDev> groups;
-  GroupDev* d_groups = nullptr;
+// The tile list of one tile shape: built on first use and kept until the
+// plan is destroyed, so a launch still queued with one shape never reads a
+// list freed by a later switch (set_tile, autotune) to another.
+struct TileSet {
   TileDev* d_tiles = nullptr;
   int64_t ntiles = 0;     // real tiles
   int64_t nslots = 0;     // launched blocks (with XCD padding)
+  bool built = false;
+};
+
+struct tn_gemm_plan_s {
+  int mode = 0;
+  std::atomic<int> cfg{-1};   // current tile shape
+  std::mutex mu;              // guards tiles[]
+  TileSet tiles[kMaxCfg];
+  std::vector<GroupDev> groups;
+  GroupDev* d_groups = nullptr;
   double flops = 0;
   bool vec_ok = true;     // every group admits 16-byte pair loads
   int64_t max_k = 0;      // longest K over the groups
   bool koff32 = true;     // every K-side offset fits in int32 (LDS tables)
 };
 
-static int build_tiles(tn_gemm_plan_s* P, int cfg) {
+// Builds (once) the tile list of shape cfg; the plan is unchanged on
+// failure.
+static int ensure_tiles(tn_gemm_plan_s* P, int cfg) {
+  std::lock_guard<std::mutex> g(P->mu);
+  if (P->tiles[cfg].built) return TN_OK;
   const TileCfg sh = cfg_shape(P->mode, cfg);
   const int bm = sh.bm, bn = sh.bn;
   std::vector<TileDev> nat;
@@ -705,18 +720,32 @@ static int build_tiles(tn_gemm_plan_s* P, int cfg) {
   std::vector<TileDev> lay(slots, TileDev{-1, 0, 0, 0});
   for (int64_t i = 0; i < n; ++i) lay[(i % per) * kX + i / per] = nat[i];
   TN_REQUIRE(slots < (int64_t(1) << 31), TN_EUNSUPP, "too many GEMM tiles");
-  if (P->d_tiles) tn::pool_free(P->d_tiles);
-  P->d_tiles = nullptr;
+  TileSet t;
   if (slots > 0) {
-    int rc = tn::pool_alloc((void**)&P->d_tiles, sizeof(TileDev) * slots);
-    if (rc != TN_OK) return rc;
-    rc = tn_memcpy_h2d(P->d_tiles, lay.data(), sizeof(TileDev) * slots, nullptr);
-    if (rc != TN_OK) return rc;
+    int rc = tn::pool_alloc((void**)&t.d_tiles, sizeof(TileDev) * slots);
+    if (rc == TN_OK) rc = tn_memcpy_h2d(t.d_tiles, lay.data(), sizeof(TileDev) * slots, nullptr);
+    if (rc != TN_OK) {
+      tn::pool_free(t.d_tiles);
+      return rc;
+    }
   }
-  P->ntiles = n;
-  P->nslots = slots;
-  P->cfg = cfg;
+  t.ntiles = n;
+  t.nslots = slots;
+  t.built = true;
+  P->tiles[cfg] = t;
   return TN_OK;
+}
+
+static TileSet tiles_of(tn_gemm_plan_s* P, int cfg) {
+  std::lock_guard<std::mutex> g(P->mu);
+  return P->tiles[cfg];
+}
+
+// Makes cfg the plan's tile shape (building its list first).
+static int select_cfg(tn_gemm_plan_s* P, int cfg) {
+  int rc = ensure_tiles(P, cfg);
+  if (rc == TN_OK) P->cfg.store(cfg);
+  return rc;
 }
 
 // Default tile: the largest tile that still gives every CU at least one
@@ -743,9 +772,11 @@ static int default_cfg(const tn_gemm_plan_s* P) {
 
 static void destroy_plan(tn_gemm_plan_s* P) {
   tn::pool_free(P->d_groups);
-  tn::pool_free(P->d_tiles);
+  for (TileSet& t : P->tiles) tn::pool_free(t.d_tiles);
   delete P;
 }
+
+static int exec_cfg(tn_gemm_plan_s* P, int cfg, const double* A, const double* B, double* C, void* stream);
 
 extern "C" {
 
@@ -797,7 +828,7 @@ int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups, const 
     if (rc == TN_OK)
       rc = tn_memcpy_h2d(P->d_groups, P->groups.data(), sizeof(GroupDev) * P->groups.size(), nullptr);
   }
-  if (rc == TN_OK) rc = build_tiles(P, default_cfg(P));
+  if (rc == TN_OK) rc = select_cfg(P, default_cfg(P));
   if (rc != TN_OK) {
     destroy_plan(P);
     return rc;
@@ -829,8 +860,7 @@ int tn_gemm_plan_set_tile(tn_gemm_plan P, int tile_id) {
   TN_REQUIRE(P != nullptr, TN_EINVAL, "tn_gemm_plan_set_tile: NULL plan");
   TN_REQUIRE(tile_id >= 0 && tile_id < num_cfg(P->mode), TN_EINVAL, "tile id %d outside [0,%d)", tile_id,
              num_cfg(P->mode));
-  if (tile_id == P->cfg) return TN_OK;
-  return build_tiles(P, tile_id);
+  return select_cfg(P, tile_id);
 }
 
 int tn_gemm_plan_autotune(tn_gemm_plan P, const double* A, const double* B, double* C, int reps,
@@ -843,16 +873,18 @@ int tn_gemm_plan_autotune(tn_gemm_plan P, const double* A, const double* B, doub
   hipEvent_t e0, e1;
   TN_HIP_CHECK(hipEventCreate(&e0));
   TN_HIP_CHECK(hipEventCreate(&e1));
-  int best = P->cfg;
+  // Each shape is timed through its own tile list; callers on other threads
+  // keep launching the current shape until the best one is selected.
+  int best = P->cfg.load();
   float best_t = 1e30f;
   int rc = TN_OK;
   for (int c = 0; c < num_cfg(P->mode) && rc == TN_OK; ++c) {
-    rc = build_tiles(P, c);
+    rc = ensure_tiles(P, c);
     if (rc != TN_OK) break;
-    rc = tn_gemm_exec(P, A, B, C, stream);   // warm-up
+    rc = exec_cfg(P, c, A, B, C, stream);   // warm-up
     if (rc != TN_OK) break;
     (void)hipEventRecord(e0, s);
-    for (int r = 0; r < reps && rc == TN_OK; ++r) rc = tn_gemm_exec(P, A, B, C, stream);
+    for (int r = 0; r < reps && rc == TN_OK; ++r) rc = exec_cfg(P, c, A, B, C, stream);
     (void)hipEventRecord(e1, s);
     if (rc != TN_OK) break;
     (void)hipEventSynchronize(e1);
@@ -868,14 +900,15 @@ int tn_gemm_plan_autotune(tn_gemm_plan P, const double* A, const double* B, doub
   (void)hipEventDestroy(e1);
   if (rc != TN_OK) return rc;
   if (best_ms) *best_ms = best_t;
-  return build_tiles(P, best);
+  return select_cfg(P, best);
 }
 
 int tn_gemm_plan_info(tn_gemm_plan P, int64_t* tiles, double* flops, int* tile_id) {
   TN_REQUIRE(P != nullptr, TN_EINVAL, "tn_gemm_plan_info: NULL plan");
-  if (tiles) *tiles = P->ntiles;
+  const int c = P->cfg.load();
+  if (tiles) *tiles = tiles_of(P, c).ntiles;
   if (flops) *flops = P->flops;
-  if (tile_id) *tile_id = P->cfg;
+  if (tile_id) *tile_id = c;
   return TN_OK;
 }
 
@@ -892,7 +925,14 @@ __global__ void zero_c_kernel(double* __restrict__ C, const GroupDev* __restrict
 int tn_gemm_exec(tn_gemm_plan P, const double* A, const double* B, double* C, void* stream) {
   TN_INIT_OR_RETURN();
   TN_REQUIRE(P != nullptr, TN_EINVAL, "tn_gemm_exec: NULL plan");
+  return exec_cfg(P, P->cfg.load(), A, B, C, stream);
+}
+
+}  // extern "C"
+
+static int exec_cfg(tn_gemm_plan_s* P, int cfg, const double* A, const double* B, double* C, void* stream) {
   hipStream_t s = tn::stream_of(stream);
+  const TileSet t = tiles_of(P, cfg);
   for (size_t gi = 0; gi < P->groups.size(); ++gi) {
     const GroupDev& g = P->groups[gi];
     if (g.K == 0 && g.M > 0 && g.N > 0) {
@@ -902,16 +942,18 @@ int tn_gemm_exec(tn_gemm_plan P, const double* A, const double* B, double* C, vo
       TN_LAUNCH_CHECK();
     }
   }
-  if (P->nslots == 0) return TN_OK;
+  if (t.nslots == 0) return TN_OK;
   TN_REQUIRE(A && B && C, TN_EINVAL, "tn_gemm_exec: NULL operand");
   const bool vec = P->vec_ok && ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) & 15) == 0;
-  KernFn k = pick_kernel(P->mode, P->cfg, vec, P->koff32 && P->max_k <= kKTab);
-  TN_REQUIRE(k != nullptr, TN_EINVAL, "no kernel for mode %d tile %d", P->mode, P->cfg);
-  hipLaunchKernelGGL(k, dim3((unsigned)P->nslots), dim3(cfg_threads(P->mode, P->cfg)), 0, s, A, B, C, P->d_groups,
-                     P->d_tiles);
+  KernFn k = pick_kernel(P->mode, cfg, vec, P->koff32 && P->max_k <= kKTab);
+  TN_REQUIRE(k != nullptr, TN_EINVAL, "no kernel for mode %d tile %d", P->mode, cfg);
+  hipLaunchKernelGGL(k, dim3((unsigned)t.nslots), dim3(cfg_threads(P->mode, cfg)), 0, s, A, B, C, P->d_groups,
+                     t.d_tiles);
   TN_LAUNCH_CHECK();
   return TN_OK;
 }
+
+extern "C" {
 
 int tn_gemm_plan_destroy(tn_gemm_plan P) {
   if (!P) return TN_OK;

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <mutex>
@@ -35,6 +36,7 @@ struct SvdProb {
   int64_t m, n;              // A is m x n row-major
   int64_t a_off, u_off, s_off, vt_off;
   int64_t v0_off;            // warm start: offset of an n x n V0^T (m >= n), -1 = none
+  int64_t max_sweeps;        // sweeps before the problem is reported as not converged
 };
 
 // Sum of a, b, c over each aligned group of TP lanes (TP a power of two),
@@ -148,7 +150,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_kernel(const SvdProb* __r
   const double abs_tol = (double)m * DBL_EPSILON * DBL_EPSILON * normF2;
 
   int sweep = 0;
-  const int max_sweeps = 60;
+  const int max_sweeps = (int)pr.max_sweeps;
   for (; sweep < max_sweeps; ++sweep) {
     int rotated = 0;
     for (int r = 0; r < np - 1; ++r) {
@@ -818,7 +820,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
     const int mr = qr ? n : m;
     const double tol = 10.0 * DBL_EPSILON * (double)mr;
     const double abs_tol = qr ? 0.0 : (double)m * DBL_EPSILON * DBL_EPSILON * normF2;
-    const int max_sweeps = 60;
+    const int max_sweeps = (int)pr.max_sweeps;
     int sweep = 0;
     for (; sweep < max_sweeps; ++sweep) {
       if (sweep > 0) exact_norms();
@@ -925,7 +927,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
     exact_norms();
     const int sw = sweeps(true);
     total = sw + 1;
-    nonconv = sw >= 60;
+    nonconv = sw >= (int)pr.max_sweeps;
     exact_norms();
     // a column at the underflow level cannot be normalised: plain sweeps
     int tiny = 0;
@@ -939,7 +941,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
   if (!qr) {
     const int sw = sweeps(false);
     total += sw + 1;
-    nonconv = sw >= 60;
+    nonconv = sw >= (int)pr.max_sweeps;
     exact_norms();
   }
   if (tid == 0) {
@@ -1085,6 +1087,10 @@ void raise_lds_limit(int tp, size_t bytes);
 // path; smaller ones that fit one workgroup's LDS to the LDS kernel.
 constexpr int64_t kBlockMinDefault = kSmallN + 1;
 int64_t g_block_min = kBlockMinDefault;
+// Sweep cap of the one-workgroup kernels (60) and of the block path (40);
+// tn_svd_set_max_sweeps lowers it for plans created afterwards.
+constexpr int kOneWgMaxSweeps = 60;
+std::atomic<int> g_max_sweeps{0};
 
 int svd_plan_build(tn_svd_plan_s** out, int64_t batch, const int64_t* m, const int64_t* n,
                    const int64_t* a_off, const int64_t* v0_off, const int64_t* u_off,
@@ -1099,8 +1105,9 @@ int svd_plan_build(tn_svd_plan_s** out, int64_t batch, const int64_t* m, const i
       return TN_EINVAL;
     }
     const int64_t mm = std::max(m[i], n[i]), nn = std::min(m[i], n[i]);
+    const int cap = g_max_sweeps.load();
     SvdProb p{m[i], n[i], a_off ? a_off[i] : 0, u_off ? u_off[i] : 0, s_off ? s_off[i] : 0,
-              vt_off ? vt_off[i] : 0, v0_off ? v0_off[i] : -1};
+              vt_off ? vt_off[i] : 0, v0_off ? v0_off[i] : -1, cap > 0 ? std::min(cap, kOneWgMaxSweeps) : kOneWgMaxSweeps};
     P->v0_any |= p.v0_off >= 0;
     P->u_any |= p.u_off >= 0;
     const size_t bytes = sizeof(double) * (size_t)((mm + 1) * nn + (nn + 1) * nn);
@@ -1128,8 +1135,12 @@ int svd_plan_build(tn_svd_plan_s** out, int64_t batch, const int64_t* m, const i
   P->n_lds = lds_probs.size();
   P->probs = small_probs;
   P->probs.insert(P->probs.end(), lds_probs.begin(), lds_probs.end());
-  int rc = tn::bj::build(&P->blk, blk);
+  int rc = tn::bj::build(&P->blk, blk, g_max_sweeps.load());
   if (rc == TN_OK) rc = tn::pool_alloc((void**)&P->d_status, 2 * sizeof(int));
+  if (rc == TN_OK) {
+    const int zero[2] = {0, 0};   // counters start clear (synchronous copy)
+    rc = tn_memcpy_h2d(P->d_status, zero, sizeof(zero), nullptr);
+  }
   if (rc == TN_OK && !P->probs.empty()) rc = tn::pool_alloc((void**)&P->d_p, sizeof(SvdProb) * P->probs.size());
   if (rc == TN_OK && !P->probs.empty())
     rc = tn_memcpy_h2d(P->d_p, P->probs.data(), sizeof(SvdProb) * P->probs.size(), nullptr);
@@ -1158,10 +1169,11 @@ void raise_lds_limit(int tp, size_t bytes) {
 
 bool svd_plan_empty(const tn_svd_plan_s* P) { return P->probs.empty() && tn::bj::empty(P->blk); }
 
+// The status counters (problems not converged, most sweeps) accumulate over
+// executions; svd_plan_status reads and clears them, so a non-converged
+// problem in any execution since the last check is reported.
 int svd_plan_launch(const tn_svd_plan_s* P, const double* A, const double* V0T, double* U, double* S,
                     double* VT, hipStream_t st) {
-  int rc = tn_zero(P->d_status, 2 * sizeof(int), st);
-  if (rc != TN_OK) return rc;
   if (P->n_small > 0) {
     SvdSmallKern k = (P->small_mm <= 32) ? svd_jacobi_small<4>
                      : (P->small_mm <= 64) ? svd_jacobi_small<8> : svd_jacobi_small<16>;
@@ -1179,7 +1191,9 @@ int svd_plan_launch(const tn_svd_plan_s* P, const double* A, const double* V0T, 
 
 int svd_plan_status(const tn_svd_plan_s* P, int* nonconv, int* sweeps, hipStream_t st) {
   int h[2] = {0, 0};
-  const int rc = tn_memcpy_d2h(h, P->d_status, 2 * sizeof(int), st);   // synchronises
+  int rc = tn_memcpy_d2h(h, P->d_status, 2 * sizeof(int), st);   // synchronises
+  if (rc != TN_OK) return rc;
+  rc = tn_zero(P->d_status, 2 * sizeof(int), st);                // fill kernel, ordered before later runs
   if (rc != TN_OK) return rc;
   g_last_sweeps = h[1];
   if (nonconv) *nonconv = h[0];
@@ -1208,6 +1222,12 @@ int svd_batched(int64_t batch, const int64_t* m, const int64_t* n, const double*
 
 
 extern "C" {
+
+int tn_svd_set_max_sweeps(int n) {
+  TN_REQUIRE(n >= 0, TN_EINVAL, "tn_svd_set_max_sweeps: negative count");
+  g_max_sweeps.store(n);
+  return TN_OK;
+}
 
 int tn_svd_set_block_min(int64_t n) {
   TN_REQUIRE(n >= 0, TN_EINVAL, "tn_svd_set_block_min: negative size");

@@ -501,6 +501,26 @@ __global__ void __launch_bounds__(256) block_permute_kernel(const double* __rest
   }
 }
 
+// In-place scaling of rectangular sub-blocks (the swap-gate signs of
+// addGate): chunk c covers elements [start, start + count) of rectangle
+// `rect`, row-major over its cols.
+struct RectDev {
+  int64_t off, rows, cols, pitch;
+};
+
+__global__ void __launch_bounds__(256) rect_scale_kernel(double* __restrict__ base,
+                                                         const RectDev* __restrict__ rects,
+                                                         const ChunkDev* __restrict__ chunks, double a) {
+  const ChunkDev c = chunks[blockIdx.x];
+  const RectDev r = rects[c.move];
+  for (int64_t t = threadIdx.x; t < c.count; t += blockDim.x) {
+    const int64_t f = c.start + t;
+    const int64_t row = f / r.cols, col = f - row * r.cols;
+    double* p = base + r.off + row * r.pitch + col;
+    *p = a * *p;
+  }
+}
+
 }  // namespace
 
 struct tn_blockperm_plan_s {
@@ -658,6 +678,69 @@ int tn_blockperm_exec(tn_blockperm_plan P, const double* src, double* dst, void*
 int tn_blockperm_plan_destroy(tn_blockperm_plan P) {
   if (!P) return TN_OK;
   tn::pool_free(P->d_moves);
+  tn::pool_free(P->d_chunks);
+  delete P;
+  return TN_OK;
+}
+
+
+struct tn_rectscale_plan_s {
+  int64_t nrects = 0, nchunks = 0;
+  RectDev* d_rects = nullptr;
+  ChunkDev* d_chunks = nullptr;
+};
+
+int tn_rectscale_plan_create(tn_rectscale_plan* plan, int64_t nrects, const int64_t* rects) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(plan != nullptr && nrects >= 0 && (nrects == 0 || rects != nullptr), TN_EINVAL,
+             "tn_rectscale_plan_create: bad arguments");
+  const int64_t kChunk = 4096;
+  std::vector<RectDev> rv;
+  std::vector<ChunkDev> ch;
+  for (int64_t i = 0; i < nrects; ++i) {
+    RectDev r{rects[4 * i], rects[4 * i + 1], rects[4 * i + 2], rects[4 * i + 3]};
+    TN_REQUIRE(r.off >= 0 && r.rows >= 0 && r.cols >= 0 && (r.rows <= 1 || r.pitch >= r.cols), TN_EINVAL,
+               "tn_rectscale_plan_create: rectangle %lld is malformed", (long long)i);
+    const int64_t total = r.rows * r.cols;
+    if (total == 0) continue;
+    const int64_t idx = (int64_t)rv.size();
+    rv.push_back(r);
+    for (int64_t s0 = 0; s0 < total; s0 += kChunk) ch.push_back(ChunkDev{idx, s0, std::min(kChunk, total - s0)});
+  }
+  TN_REQUIRE((int64_t)ch.size() < (int64_t(1) << 31), TN_EUNSUPP, "too many chunks");
+  auto* P = new tn_rectscale_plan_s();
+  P->nrects = (int64_t)rv.size();
+  P->nchunks = (int64_t)ch.size();
+  if (!rv.empty()) {
+    int rc = tn::pool_alloc((void**)&P->d_rects, sizeof(RectDev) * rv.size());
+    if (rc == TN_OK) rc = tn::pool_alloc((void**)&P->d_chunks, sizeof(ChunkDev) * ch.size());
+    if (rc == TN_OK) rc = tn_memcpy_h2d(P->d_rects, rv.data(), sizeof(RectDev) * rv.size(), nullptr);
+    if (rc == TN_OK) rc = tn_memcpy_h2d(P->d_chunks, ch.data(), sizeof(ChunkDev) * ch.size(), nullptr);
+    if (rc != TN_OK) {
+      tn::pool_free(P->d_rects);
+      tn::pool_free(P->d_chunks);
+      delete P;
+      return rc;
+    }
+  }
+  *plan = P;
+  return TN_OK;
+}
+
+int tn_rectscale_exec(tn_rectscale_plan P, double* base, double a, void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(P != nullptr, TN_EINVAL, "tn_rectscale_exec: NULL plan");
+  if (P->nchunks == 0) return TN_OK;
+  TN_REQUIRE(base != nullptr, TN_EINVAL, "tn_rectscale_exec: NULL buffer");
+  hipLaunchKernelGGL(rect_scale_kernel, dim3((unsigned)P->nchunks), dim3(256), 0, tn::stream_of(stream), base,
+                     P->d_rects, P->d_chunks, a);
+  TN_LAUNCH_CHECK();
+  return TN_OK;
+}
+
+int tn_rectscale_plan_destroy(tn_rectscale_plan P) {
+  if (!P) return TN_OK;
+  tn::pool_free(P->d_rects);
   tn::pool_free(P->d_chunks);
   delete P;
   return TN_OK;

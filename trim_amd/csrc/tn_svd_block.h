@@ -19,8 +19,9 @@ struct Spec {
 struct Plan;
 
 // Builds the device descriptors, scratch and (for warm starts) the W = A V0
-// product plan.  Returns a TN_* code.
-int build(Plan** out, const std::vector<Spec>& specs);
+// product plan.  max_sweeps > 0 lowers the sweep cap (40) past which a
+// problem is reported as not converged.  Returns a TN_* code.
+int build(Plan** out, const std::vector<Spec>& specs, int max_sweeps = 0);
 // Launches the whole solve on `st`; status[0] += problems not converged,
 // status[1] = max(status[1], sweeps).  Under stream capture the sweep count
 // is fixed (max_sweeps_captured); otherwise the host stops once a sweep

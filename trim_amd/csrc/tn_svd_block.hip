@@ -937,6 +937,7 @@ struct Plan {
   double* d_scr = nullptr;
   int* d_iscr = nullptr;
   tn_gemm_plan warm = nullptr;    // W = A V0 of the warm-started problems
+  int max_sw = kMaxSweeps;        // sweep cap (tn_svd_set_max_sweeps)
   int* d_qlist = nullptr;         // QR-preconditioned problems: 32-wide panels first, then 16-wide
   int nqr = 0, nqr32 = 0, max_n32 = 0, max_n16 = 0;
   tn_gemm_plan qr_vt = nullptr;   // their VT = U_X^T V0^T
@@ -962,8 +963,9 @@ void destroy(Plan* P) {
 
 bool empty(const Plan* P) { return P == nullptr || P->probs.empty(); }
 
-int build(Plan** out, const std::vector<Spec>& specs) {
+int build(Plan** out, const std::vector<Spec>& specs, int max_sweeps) {
   auto* P = new Plan();
+  P->max_sw = max_sweeps > 0 ? std::min(max_sweeps, kMaxSweeps) : kMaxSweeps;
   int64_t scr = 0, iscr = kMaxSweeps;
   std::vector<int2> map;
   std::vector<tn_gemm_desc_ex> gd, qd, qg, qh;
@@ -1166,7 +1168,7 @@ int launch(Plan* P, const double* A, const double* V0T, double* U, double* S, do
   // Under capture every sweep is launched (a converged problem's workgroups
   // return at once); otherwise the host stops after a sweep that rotated
   // nothing in any problem.
-  const int max_sw = capturing ? kMaxSweepsCaptured : kMaxSweeps;
+  const int max_sw = std::min(capturing ? kMaxSweepsCaptured : kMaxSweeps, P->max_sw);
   int ran = 0;
   for (int s = 0; s < max_sw; ++s) {
     hipLaunchKernelGGL(bj_sort, dim3(np), dim3(kThreads), P->lds, st, P->d_probs, P->d_scr, P->d_iscr, s);

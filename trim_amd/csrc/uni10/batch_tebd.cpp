@@ -715,10 +715,18 @@ void BatchTebd::launch(int n) {
 
 void BatchTebd::wait() {
   if (!last_svd_) return;
-  int nonconv = 0;
-  // status of the last SVD launch of the step (the second update's)
-  detail::check(tn_svd_plan_check(last_svd_, &nonconv, &sweeps_, stream_), "BatchTebd svd");
-  if (nonconv) detail::fail("BatchTebd: an SVD did not converge");
+  // Both updates' SVD plans: their counters cover every execution since the
+  // last check (all steps of launch(n), replayed or issued directly).
+  int bad = 0, most = 0;
+  for (int p = 0; p < 2; ++p) {
+    if (!svd_[p]) continue;
+    int nonconv = 0, sw = 0;
+    detail::check(tn_svd_plan_check(svd_[p], &nonconv, &sw, stream_), "BatchTebd svd");
+    bad += nonconv;
+    most = std::max(most, sw);
+  }
+  sweeps_ = most;
+  if (bad) detail::fail("BatchTebd: " + std::to_string(bad) + " SVD problem(s) did not converge");
 }
 
 bool BatchTebd::ready() const {

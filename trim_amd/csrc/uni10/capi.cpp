@@ -2,7 +2,9 @@
 #include "uni10_capi.h"
 
 #include <cstring>
+#include <sstream>
 #include <string>
+#include <vector>
 
 #include "batch_tebd.h"
 #include "common.h"
@@ -84,6 +86,51 @@ int u10_tensor_create(int nbonds, const int* types, const int* nruns, const int*
       bonds.emplace_back(types[b] == 1 ? BD_IN : BD_OUT, full);
     }
     *out = new u10_tensor_s{UniTensor(bonds, name ? name : "")};
+  });
+}
+
+int u10_bond_combine(int nbonds, const int* types, const int* nruns, const int* run_qnums, const int* run_degs,
+                     int use_free_fn, int tp, int* out_type, int* out_dim, int* out_nruns, int* out_qnums,
+                     int* out_degs, int* out_ndeg, int* deg_qnums, int* deg_counts, int cap) {
+  return guard([&] {
+    need(types, "u10_bond_combine");
+    if (nbonds < 1) detail::fail("There should be at least one bond in the input vector to be combined.");
+    std::vector<Bond> bonds;
+    int r = 0;
+    for (int b = 0; b < nbonds; ++b) {
+      std::vector<Qnum> full;
+      for (int k = 0; k < nruns[b]; ++k, ++r) full.insert(full.end(), run_degs[r], qn(run_qnums + 3 * r));
+      bonds.emplace_back(types[b] == 1 ? BD_IN : BD_OUT, full);
+    }
+    Bond out;
+    if (use_free_fn) {
+      out = combine(tp == 1 ? BD_IN : BD_OUT, bonds);
+    } else {
+      out = bonds[0];
+      for (int b = 1; b < nbonds; ++b) out.combine(bonds[b]);
+    }
+    *out_type = (int)out.type();
+    *out_dim = out.dim();
+    *out_nruns = (int)out.qnums().size();
+    for (int i = 0; i < *out_nruns && i < cap; ++i) {
+      const Qnum& q = out.qnums()[i];
+      out_qnums[3 * i] = q.U1();
+      out_qnums[3 * i + 1] = (int)q.prt();
+      out_qnums[3 * i + 2] = (int)q.prtF();
+      out_degs[i] = out.qdegs()[i];
+    }
+    std::map<Qnum, int> dg = out.degeneracy();
+    *out_ndeg = (int)dg.size();
+    int i = 0;
+    for (auto& kv : dg) {
+      if (i < cap) {
+        deg_qnums[3 * i] = kv.first.U1();
+        deg_qnums[3 * i + 1] = (int)kv.first.prt();
+        deg_qnums[3 * i + 2] = (int)kv.first.prtF();
+        deg_counts[i] = kv.second;
+      }
+      ++i;
+    }
   });
 }
 
@@ -241,6 +288,30 @@ int u10_otimes(u10_tensor a, u10_tensor b, u10_tensor* out) {
   return guard([&] { *out = new u10_tensor_s{otimes(a->T, b->T)}; });
 }
 
+int u10_tensor_mul(u10_tensor a, u10_tensor b, u10_tensor* out) {
+  return guard([&] { *out = new u10_tensor_s{a->T * b->T}; });
+}
+
+int u10_tensor_ex_swap(u10_tensor a, u10_tensor b, int* pairs, int cap, int* n) {
+  return guard([&] {
+    std::vector<_Swap> sw = a->T.exSwap(b->T);
+    *n = (int)sw.size();
+    for (int i = 0; i < (int)sw.size() && i < cap; ++i) {
+      pairs[2 * i] = sw[i].b1;
+      pairs[2 * i + 1] = sw[i].b2;
+    }
+  });
+}
+
+int u10_tensor_add_gate(u10_tensor t, int n, const int* pairs) {
+  return guard([&] {
+    if (n > 0) need(pairs, "u10_tensor_add_gate");
+    std::vector<_Swap> sw(n);
+    for (int i = 0; i < n; ++i) sw[i] = _Swap{pairs[2 * i], pairs[2 * i + 1]};
+    t->T.addGate(sw);
+  });
+}
+
 int u10_network_create(const char* fname, u10_network* out) {
   return guard([&] { *out = new u10_network_s(fname); });
 }
@@ -265,8 +336,30 @@ int u10_network_tree(u10_network net, char* buf, int len) {
   });
 }
 
+int u10_network_print(u10_network net, char* buf, int len) {
+  return guard([&] {
+    std::ostringstream os;
+    os << net->N;
+    std::snprintf(buf, (size_t)len, "%s", os.str().c_str());
+  });
+}
+
 int u10_network_free(u10_network net) {
   return guard([&] { delete net; });
+}
+
+int u10_rdotr(int64_t ar, int64_t ac, int adiag, const double* a, int64_t br, int64_t bc, int bdiag,
+              const double* b, int64_t* cr, int64_t* cc, int* cdiag, double* c, int64_t cap) {
+  return guard([&] {
+    Matrix A((size_t)ar, (size_t)ac, a, adiag != 0, false), B((size_t)br, (size_t)bc, b, bdiag != 0, false);
+    Matrix C = RDotR(A, B);
+    *cr = (int64_t)C.row();
+    *cc = (int64_t)C.col();
+    *cdiag = C.isDiag() ? 1 : 0;
+    std::vector<double> h = C.getHostElem();
+    if ((int64_t)h.size() > cap) detail::fail("u10_rdotr: output larger than the given buffer");
+    std::copy(h.begin(), h.end(), c);
+  });
 }
 
 int u10_svd(const double* A, int64_t M, int64_t N, double* U, double* S, double* VT) {

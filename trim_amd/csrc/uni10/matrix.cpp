@@ -320,11 +320,16 @@ Matrix RDotR(const Matrix& a, const Matrix& b) {
                   "diagColMul");
     return c;
   }
+  // diagonal x diagonal (BlockTools.cpp:62-69): the first min(|a|, |b|)
+  // entries of a times those of b (vectorMul), the rest zero.
   Matrix c(a.row(), b.col(), true);
-  auto ha = a.getHostElem(), hb = b.getHostElem();
-  std::vector<double> hc(c.elemNum(), 0.0);
-  for (size_t i = 0; i < hc.size() && i < ha.size() && i < hb.size(); ++i) hc[i] = ha[i] * hb[i];
-  c.setElem(hc);
+  c.set_zero();
+  const size_t n = std::min(std::min(a.elemNum(), b.elemNum()), c.elemNum());
+  if (n) {
+    detail::check(tn_memcpy_d2d(c.getElem(), a.getElem(), n * sizeof(double), detail::stream()), "RDotR");
+    // row i of the n x 1 matrix c times b_i: the elementwise product
+    detail::check(tn_diag_row_mul_f64(c.getElem(), b.getElem(), (int64_t)n, 1, detail::stream()), "vectorMul");
+  }
   return c;
 }
 

@@ -191,6 +191,8 @@ Network::Network(const std::string& fname) {
   const int n = (int)label_arr.size() - 1;
   leafs.assign(n, nullptr);
   tensors.assign(n, nullptr);
+  swapflags.assign(n, false);
+  swaps_arr.assign(n, std::vector<_Swap>());
 }
 
 Network::Network(const std::string& fname, const std::vector<UniTensor*>& tens) {
@@ -201,6 +203,8 @@ Network::Network(const std::string& fname, const std::vector<UniTensor*>& tens) 
                  fname + "'.");
   leafs.assign(n, nullptr);
   tensors.assign(n, nullptr);
+  swapflags.assign(n, false);
+  swaps_arr.assign(n, std::vector<_Swap>());
   for (int i = 0; i < n; ++i) putTensor((size_t)i, tens[i]);
   construct();
 }
@@ -222,6 +226,7 @@ void Network::putTensor(size_t idx, const UniTensor* T, bool force) {
     *tensors[idx] = *T;
     tensors[idx]->setLabel(label_arr[idx]);
     tensors[idx]->setName(names[idx]);
+    swapflags[idx] = false;
   } else {
     if (!T->hasBond()) detail::fail("Cannot create node of a network from tensor without bond.");
     UniTensor* t = new UniTensor(*T);
@@ -258,6 +263,8 @@ void Network::putTensorT(const std::string& nameT, const UniTensor* T, bool forc
 
 void Network::putTensorT(const std::string& nameT, const UniTensor& T, bool force) { putTensorT(nameT, &T, force); }
 
+// Network.cpp:679-695: the swap gates already applied are applied again,
+// which undoes them (each sign squares to one).
 void Network::destruct() {
   for (NetNode* p : owned) delete p;
   owned.clear();
@@ -267,7 +274,62 @@ void Network::destruct() {
       l->parent = l->left = l->right = nullptr;
       l->point = 0;
     }
+  conOrder.clear();
+  for (size_t t = 0; t < tensors.size(); ++t) {
+    if (Qnum::isFermionic() && swapflags[t]) {
+      tensors[t]->addGate(swaps_arr[t]);
+      swapflags[t] = false;
+    }
+    swaps_arr[t].clear();
+  }
   load = false;
+}
+
+// findConOrd (Network.cpp:923-942): leaf tensor indices in pre-order.
+void Network::findConOrd(NetNode* nd) {
+  if (nd == nullptr || conOrder.size() == tensors.size()) return;
+  if (nd->T) {
+    auto it = std::find(tensors.begin(), tensors.end(), nd->T);
+    if (it == tensors.end()) detail::fail("Fatal error(code = N2). Please contact the developer of the uni10 library.");
+    conOrder.push_back((int)(it - tensors.begin()));
+  }
+  findConOrd(nd->left);
+  findConOrd(nd->right);
+}
+
+// addSwap (Network.cpp:944-993): sorting the tensors from their order in the
+// tree back to the file order, each time tensor b1 passes tensor b2 the
+// bonds b1 shares with b2 cross the bonds it does not share; a bond pair
+// crossed an even number of times cancels.
+void Network::addSwap() {
+  const int Tnum = (int)leafs.size();
+  findConOrd(root);
+  if ((int)conOrder.size() != Tnum)
+    detail::fail("Fatal error(code = N3). Please contact the developer of the uni10 library.");
+  std::vector<int> ord = conOrder;
+  for (int i = 0; i < Tnum - 1; ++i)
+    for (int j = 0; j < Tnum - i - 1; ++j)
+      if (ord[j] > ord[j + 1]) {
+        const int b1 = ord[j + 1], b2 = ord[j];
+        std::vector<_Swap> sw = tensors[b1]->exSwap(*tensors[b2]);
+        swaps_arr[b1].insert(swaps_arr[b1].end(), sw.begin(), sw.end());
+        std::swap(ord[j], ord[j + 1]);
+      }
+  for (int t = 0; t < Tnum; ++t) {
+    const int nb = (int)tensors[t]->bondNum();
+    std::map<int, bool> recs;
+    for (const _Swap& sp : swaps_arr[t]) {
+      const int is = std::min(sp.b1, sp.b2), ib = std::max(sp.b1, sp.b2);
+      auto it = recs.find(is * nb + ib);
+      if (it != recs.end())
+        it->second = !it->second;
+      else
+        recs[is * nb + ib] = true;
+    }
+    swaps_arr[t].clear();
+    for (auto& kv : recs)
+      if (kv.second) swaps_arr[t].push_back(_Swap{kv.first / nb, kv.first % nb});
+  }
 }
 
 void Network::branch(NetNode* sbj, NetNode* tar) {
@@ -382,6 +444,7 @@ void Network::construct() {
   for (int l : root->labels)
     if (std::find(tout.begin(), tout.end(), l) == tout.end())
       detail::fail("Error when constructing the network. The labels of the resulting tensor do not match with the labels of 'TOUT' in the network file");
+  addSwap();
   load = true;
 }
 
@@ -403,6 +466,11 @@ UniTensor Network::merge(NetNode* nd) {
 
 UniTensor Network::launch(const std::string& name) {
   if (!load) construct();
+  for (size_t t = 0; t < tensors.size(); ++t)
+    if (Qnum::isFermionic() && !swapflags[t]) {
+      tensors[t]->addGate(swaps_arr[t]);
+      swapflags[t] = true;
+    }
   UniTensor T = merge(root);
   const std::vector<int>& tout = label_arr.back();
   if (tout.size() > 1) T.permute(tout, Rnums.back());
@@ -439,19 +507,51 @@ std::string Network::profile(bool print) {
   return os.str();
 }
 
-std::ostream& operator<<(std::ostream& os, Network& net) {
-  for (size_t i = 0; i + 1 < net.names.size(); ++i) {
-    os << net.names[i] << ": i[";
-    for (int j = 0; j < net.Rnums[i]; ++j) os << (j ? ", " : "") << net.label_arr[i][j];
-    os << "] o[";
-    for (size_t j = net.Rnums[i]; j < net.label_arr[i].size(); ++j)
-      os << (j > (size_t)net.Rnums[i] ? ", " : "") << net.label_arr[i][j];
-    os << "]\n";
+int Network::rollcall() {
+  if (!load) {
+    for (size_t i = 0; i < leafs.size(); ++i)
+      if (leafs[i] == nullptr) return (int)i;
+    construct();
   }
-  os << "TOUT: ";
-  for (int l : net.label_arr.back()) os << l << " ";
-  os << "\n";
-  if (net.load) os << net.tree() << "\n";
+  return -1;
+}
+
+// Network::preprint (Network.cpp:856-870): one line per node in pre-order,
+// "name(elemNum): labels," indented by "|   " per level, "*" for inner nodes.
+void Network::preprint(std::ostream& os, const NetNode* nd, int layer) const {
+  if (nd == nullptr) return;
+  for (int i = 0; i < layer; ++i) os << "|   ";
+  if (nd->T)
+    os << nd->name << "(" << nd->elemNum << "): ";
+  else
+    os << "*(" << nd->elemNum << "): ";
+  for (int l : nd->labels) os << l << ", ";
+  os << std::endl;
+  preprint(os, nd->left, layer + 1);
+  preprint(os, nd->right, layer + 1);
+}
+
+// operator<< (Network.cpp:872-907): the tensor list with in/out labels, then
+// the contraction tree once every tensor has been given.
+std::ostream& operator<<(std::ostream& os, Network& net) {
+  os << std::endl;
+  for (size_t i = 0; i < net.names.size(); ++i) {
+    const int R = net.Rnums[i];
+    const size_t nl = net.label_arr[i].size();
+    os << net.names[i] << ": ";
+    if (R) os << "i[";
+    for (int l = 0; l < R; ++l) os << net.label_arr[i][l] << (l < R - 1 ? ", " : "");
+    if (R) os << "] ";
+    if (nl - R) os << "o[";
+    for (size_t l = R; l < nl; ++l) os << net.label_arr[i][l] << (l < nl - 1 ? ", " : "");
+    if (nl - R) os << "]";
+    os << std::endl;
+  }
+  os << std::endl;
+  if (net.rollcall() < 0)
+    net.preprint(os, net.root, 0);
+  else
+    os << "\nSome tensors have not yet been given!\n\n";
   return os;
 }
 

@@ -738,6 +738,77 @@ UniTensor& UniTensor::transpose() {
   return *this;
 }
 
+// exSwap (UniTensor.cpp:1237-1268): every pair (bond shared with Tb, bond
+// not shared), in bond order.
+std::vector<_Swap> UniTensor::exSwap(const UniTensor& Tb) const {
+  std::vector<_Swap> swaps;
+  if (status & Tb.status & HAVEBOND) {
+    std::vector<int> shared, left;
+    for (int a = 0; a < (int)labels.size(); ++a) {
+      bool found = false;
+      for (int l : Tb.labels)
+        if (labels[a] == l) found = true;
+      (found ? shared : left).push_back(a);
+    }
+    for (int i : shared)
+      for (int j : left) swaps.push_back(_Swap{i, j});
+  }
+  return swaps;
+}
+
+namespace {
+struct GatePlanEntry {
+  tn_rectscale_plan plan = nullptr;
+};
+std::mutex g_gate_mu;
+std::unordered_map<std::string, std::shared_ptr<GatePlanEntry>> g_gate_cache;
+}  // namespace
+
+// addGate (UniTensorReal.cpp:868-921): the sub-block of Qidx q is multiplied
+// by (-1)^(XOR over gates of prtF(b1, q) & prtF(b2, q)).  Only the negated
+// sub-blocks are listed in the device plan, which scales them in place.
+void UniTensor::addGate(const std::vector<_Swap>& swaps) {
+  if (!(status & HAVEBOND)) detail::fail("Adding swap gates to a tensor without bonds(scalar).");
+  if (!(status & HAVEELEM)) detail::fail("Cannot add swap gates to a tensor before setting its elements.");
+  const int nb = (int)bonds.size();
+  for (const _Swap& sw : swaps)
+    if (sw.b1 < 0 || sw.b1 >= nb || sw.b2 < 0 || sw.b2 >= nb)
+      detail::fail("addGate: swap gate bond index out of range.");
+  if (swaps.empty()) return;
+  std::string key = m_layout->key + "#G";
+  for (const _Swap& sw : swaps) key += std::to_string(sw.b1) + "," + std::to_string(sw.b2) + ";";
+  std::shared_ptr<GatePlanEntry> ent;
+  {
+    std::lock_guard<std::mutex> g(g_gate_mu);
+    auto it = g_gate_cache.find(key);
+    if (it != g_gate_cache.end()) ent = it->second;
+  }
+  if (!ent) {
+    std::vector<int64_t> rects;
+    std::vector<int> q;
+    for (auto& e : m_layout->QidxEnc) {
+      m_layout->decode(e.first, q);
+      int bit = 0;
+      for (const _Swap& sw : swaps)
+        bit ^= (int)bonds[sw.b1].qnums()[q[sw.b1]].prtF() & (int)bonds[sw.b2].qnums()[q[sw.b2]].prtF();
+      if (!bit) continue;
+      size_t eoff, bcols, sr, sc;
+      m_layout->subblock(e.first, eoff, bcols, sr, sc);
+      rects.insert(rects.end(), {(int64_t)eoff, (int64_t)sr, (int64_t)sc, (int64_t)bcols});
+    }
+    ent = std::make_shared<GatePlanEntry>();
+    detail::check(tn_rectscale_plan_create(&ent->plan, (int64_t)(rects.size() / 4), rects.data()),
+                  "tn_rectscale_plan_create");
+    std::lock_guard<std::mutex> g(g_gate_mu);
+    auto ins = g_gate_cache.emplace(key, ent);
+    if (!ins.second) {
+      tn_rectscale_plan_destroy(ent->plan);
+      ent = ins.first->second;
+    }
+  }
+  detail::check(tn_rectscale_exec(ent->plan, buf.get(), -1.0, detail::stream()), "addGate");
+}
+
 UniTensor& UniTensor::operator*=(double a) {
   detail::check(tn_scale_f64(buf.get(), (int64_t)m_elemNum, a, detail::stream()), "operator*=");
   return *this;

@@ -54,6 +54,9 @@ SIGNATURES = {
     "tn_blockperm_plan_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, I64, P, P, P, P, I64],
     "tn_blockperm_exec": [P, P, P, P],
     "tn_blockperm_plan_destroy": [P],
+    "tn_rectscale_plan_create": [ctypes.POINTER(ctypes.c_void_p), I64, P],
+    "tn_rectscale_exec": [P, P, ctypes.c_double, P],
+    "tn_rectscale_plan_destroy": [P],
     "tn_transpose_f64": [P, I64, I64, P, P],
     "tn_gemm_plan_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, I64, P],
     "tn_gemm_plan_create_ex": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, I64, P],
@@ -74,6 +77,7 @@ SIGNATURES = {
     "tn_svd_f64_batched": [I64, P, P, P, P, P, P, P, P, P, P, P],
     "tn_svd_last_sweeps": [ctypes.POINTER(ctypes.c_int)],
     "tn_svd_set_block_min": [ctypes.c_int64],
+    "tn_svd_set_max_sweeps": [ctypes.c_int],
     "tn_svd_f64_warm": [P, I64, I64, P, P, P, P, P],
     "tn_svd_plan_create": [ctypes.POINTER(ctypes.c_void_p), I64, P, P, P, P, P, P, P],
     "tn_svd_plan_exec": [P, P, P, P, P, P, P],

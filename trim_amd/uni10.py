@@ -39,6 +39,7 @@ SIGNATURES = {
     "u10_set_stream": [P],
     "u10_set_autotune": [I],
     "u10_tensor_create": [I, P, P, P, P, S, PP],
+    "u10_bond_combine": [I, P, P, P, P, I, I, PI, PI, PI, P, P, PI, P, P, I],
     "u10_tensor_scalar": [D, PP],
     "u10_tensor_load": [S, PP],
     "u10_tensor_save": [P, S],
@@ -65,11 +66,16 @@ SIGNATURES = {
     "u10_tensor_device_ptr": [P, PP],
     "u10_contract": [P, P, I, PP],
     "u10_otimes": [P, P, PP],
+    "u10_tensor_mul": [P, P, PP],
+    "u10_tensor_ex_swap": [P, P, P, I, PI],
+    "u10_tensor_add_gate": [P, I, P],
+    "u10_network_print": [P, ctypes.c_char_p, I],
     "u10_network_create": [S, PP],
     "u10_network_put": [P, S, P, I],
     "u10_network_launch": [P, S, PP],
     "u10_network_tree": [P, ctypes.c_char_p, I],
     "u10_network_free": [P],
+    "u10_rdotr": [I64, I64, I, P, I64, I64, I, P, PI64, PI64, PI, P, I64],
     "u10_svd": [P, I64, I64, P, P, P],
     "u10_eigh": [P, I64, P, P],
     "u10_lanczos_eigh": [P, I64, P, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
@@ -181,6 +187,28 @@ class Bond:
             else:
                 out.append([q, 1])
         return out
+
+
+def combine_bonds(bonds: Sequence[Bond], tp: Optional[int] = None):
+    """Bond::combine of the C++ API: bonds[0].combine(bonds[1])... (tp None),
+    or the free combine(tp, bonds).  Returns (type, dim, runs, degeneracy)
+    with runs and degeneracy as [(Qnum, count)]."""
+    types = np.array([b.type for b in bonds], np.int32)
+    runs = [b.runs() for b in bonds]
+    nruns = np.array([len(r) for r in runs], np.int32)
+    q = np.array([x[0].triple() for r in runs for x in r], np.int32).reshape(-1)
+    d = np.array([x[1] for r in runs for x in r], np.int32)
+    cap = 4096
+    oq, od = np.zeros(3 * cap, np.int32), np.zeros(cap, np.int32)
+    gq, gd = np.zeros(3 * cap, np.int32), np.zeros(cap, np.int32)
+    ot, odim, onr, ong = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _call("u10_bond_combine", len(bonds), types.ctypes.data_as(P), nruns.ctypes.data_as(P), q.ctypes.data_as(P),
+          d.ctypes.data_as(P), 0 if tp is None else 1, int(tp or 1), ctypes.byref(ot), ctypes.byref(odim),
+          ctypes.byref(onr), oq.ctypes.data_as(P), od.ctypes.data_as(P), ctypes.byref(ong), gq.ctypes.data_as(P),
+          gd.ctypes.data_as(P), cap)
+    rl = [(Qnum(*oq[3 * i:3 * i + 3]), int(od[i])) for i in range(onr.value)]
+    dg = [(Qnum(*gq[3 * i:3 * i + 3]), int(gd[i])) for i in range(ong.value)]
+    return ot.value, odim.value, rl, dg
 
 
 class UniTensor:
@@ -352,11 +380,27 @@ class UniTensor:
         return p.value
 
     def __mul__(self, other):
-        if isinstance(other, UniTensor):
-            return contract(self.copy(), other.copy(), True)
+        if isinstance(other, UniTensor):      # operator*(UniTensor, UniTensor) of the C++ API
+            h = ctypes.c_void_p()
+            _call("u10_tensor_mul", self.h, other.h, ctypes.byref(h))
+            return UniTensor(_handle=h.value)
         t = self.copy()
         t *= other
         return t
+
+    def exSwap(self, other) -> List[tuple]:
+        n = ctypes.c_int()
+        buf = np.zeros(2 * 256, np.int32)
+        _call("u10_tensor_ex_swap", self.h, other.h, buf.ctypes.data_as(P), 256, ctypes.byref(n))
+        if n.value > 256:
+            buf = np.zeros(2 * n.value, np.int32)
+            _call("u10_tensor_ex_swap", self.h, other.h, buf.ctypes.data_as(P), n.value, ctypes.byref(n))
+        return [(int(buf[2 * i]), int(buf[2 * i + 1])) for i in range(n.value)]
+
+    def addGate(self, swaps):
+        a = np.array([v for sw in swaps for v in sw] or [0], np.int32)
+        _call("u10_tensor_add_gate", self.h, len(swaps), a.ctypes.data_as(P))
+        return self
 
 
 def contract(a: UniTensor, b: UniTensor, fast: bool = False) -> UniTensor:
@@ -393,6 +437,12 @@ class Network:
         _call("u10_network_tree", self.h, buf, 4096)
         return buf.value.decode()
 
+    def __str__(self) -> str:
+        """operator<< of the C++ API: the reference's network print-out."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        _call("u10_network_print", self.h, buf, 1 << 16)
+        return buf.value.decode()
+
     def __del__(self):
         try:
             if getattr(self, "h", None):
@@ -400,6 +450,21 @@ class Network:
                 self.h = None
         except Exception:
             pass
+
+
+def rdotr(a, a_shape, a_diag, b, b_shape, b_diag):
+    """RDotR of the C++ API on host arrays; a diagonal operand is its vector
+    of min(rows, cols) entries.  Returns (array, shape, diag)."""
+    a, pa = _arr(a, np.float64)
+    b, pb = _arr(b, np.float64)
+    cap = max(1, int(a_shape[0]) * int(b_shape[1]))
+    out = np.zeros(cap)
+    cr, cc, cd = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+    _call("u10_rdotr", a_shape[0], a_shape[1], int(a_diag), pa, b_shape[0], b_shape[1], int(b_diag), pb,
+          ctypes.byref(cr), ctypes.byref(cc), ctypes.byref(cd), out.ctypes.data_as(P), cap)
+    n = min(cr.value, cc.value) if cd.value else cr.value * cc.value
+    res = out[:n].copy()
+    return (res if cd.value else res.reshape(cr.value, cc.value)), (cr.value, cc.value), bool(cd.value)
 
 
 def svd(A):
