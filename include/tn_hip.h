@@ -212,6 +212,17 @@ int tn_diag_col_mul_f64(double* mat, const double* diag, int64_t M, int64_t N, v
  * for M >= N, M <= 2048). */
 int tn_svd_f64(const double* A, int64_t M, int64_t N, double* U, double* S,
                double* VT, void* stream);
+/* tn_svd_f64 without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * kernel and results (bit for bit) as tn_svd_f64, limited to the
+ * one-workgroup size min(M, N) <= 64, max(M, N) <= 128 (larger shapes fail
+ * with TN_EUNSUPP).  The problem descriptor and convergence counters live in
+ * a 256-byte static device array per device, held until the process ends and
+ * shared by every call on that device: calls that overlap on one device race.
+ * Synchronises the stream (it reads the convergence counters). */
+int tn_svd_f64_noalloc(const double* A, int64_t M, int64_t N, double* U, double* S, double* VT,
+                       void* stream);
 /* Batched: problem i reads A + a_off[i] (m[i] x n[i]) and writes
  * U + u_off[i], S + s_off[i], VT + vt_off[i]. */
 int tn_svd_f64_batched(int64_t batch, const int64_t* m, const int64_t* n,

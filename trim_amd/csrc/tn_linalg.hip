@@ -1218,10 +1218,85 @@ int svd_batched(int64_t batch, const int64_t* m, const int64_t* n, const double*
   return TN_OK;
 }
 
+// ---------------------------------------------------------------------------
+// The allocation-free SVD (tn_svd_f64_noalloc): the one-workgroup kernel's
+// problem descriptor and convergence counters live in this file-scope device
+// array instead of a pool-allocated plan.  256 bytes per device, held until
+// the process ends; every call on a device shares them, so calls that
+// overlap on one device race.
+struct NoAllocScratch {
+  SvdProb prob;
+  int status[2];
+};
+static_assert(sizeof(NoAllocScratch) <= 256, "no-alloc scratch too small");
+
+static __device__ __align__(256) unsigned char scratch[256];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+NoAllocScratch* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this device's copy of `scratch`, looked up once per
+// device with that device current (the array's name is not a device address
+// in host code).
+int scratch_ptr(NoAllocScratch** out) {
+  int dev = 0;
+  TN_HIP_CHECK(hipGetDevice(&dev));
+  TN_REQUIRE(dev >= 0 && dev < kMaxDevices, TN_EUNSUPP, "device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    TN_REQUIRE(e == hipSuccess && ptr != nullptr, TN_EHIP, "hipGetSymbolAddress(scratch): %s",
+               hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<NoAllocScratch*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return TN_OK;
+}
+
+// Writes the problem descriptor and clears the counters (a kernel, so no
+// memset node or host staging is involved).
+__global__ void noalloc_setup_kernel(NoAllocScratch* __restrict__ d, SvdProb p) {
+  if (threadIdx.x == 0) {
+    d->prob = p;
+    d->status[0] = 0;
+    d->status[1] = 0;
+  }
+}
+
 }  // namespace
 
 
 extern "C" {
+
+int tn_svd_f64_noalloc(const double* A, int64_t M, int64_t N, double* U, double* S, double* VT, void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(A && S && VT, TN_EINVAL, "tn_svd_f64_noalloc: NULL buffer");
+  TN_REQUIRE(M >= 0 && N >= 0, TN_EINVAL, "tn_svd_f64_noalloc: negative shape");
+  const int64_t mm = std::max(M, N), nn = std::min(M, N);
+  TN_REQUIRE(nn <= kSmallN && mm <= kSmallTP * 16, TN_EUNSUPP,
+             "tn_svd_f64_noalloc: %lld x %lld exceeds the maximum min(M, N) <= %d, max(M, N) <= %d",
+             (long long)M, (long long)N, kSmallN, kSmallTP * 16);
+  NoAllocScratch* d = nullptr;
+  int rc = scratch_ptr(&d);
+  if (rc != TN_OK) return rc;
+  hipStream_t st = tn::stream_of(stream);
+  const int cap = g_max_sweeps.load();
+  const SvdProb p{M, N, 0, U ? 0 : -1, 0, 0, -1, cap > 0 ? std::min(cap, kOneWgMaxSweeps) : kOneWgMaxSweeps};
+  hipLaunchKernelGGL(noalloc_setup_kernel, dim3(1), dim3(64), 0, st, d, p);
+  TN_LAUNCH_CHECK();
+  // the same kernel and template choice as a plan holding this one problem
+  SvdSmallKern k = (mm <= 32) ? svd_jacobi_small<4> : (mm <= 64) ? svd_jacobi_small<8> : svd_jacobi_small<16>;
+  hipLaunchKernelGGL(k, dim3(1), dim3(kThreads), 0, st, &d->prob, A, nullptr, U, S, VT, d->status);
+  TN_LAUNCH_CHECK();
+  int h[2] = {0, 0};
+  rc = tn_memcpy_d2h(h, d->status, sizeof(h), stream);   // synchronises the stream
+  if (rc != TN_OK) return rc;
+  g_last_sweeps = h[1];
+  TN_REQUIRE(h[0] == 0, TN_ENOCONV, "svd: %d problem(s) did not converge in %d sweeps", h[0], (int)p.max_sweeps);
+  return TN_OK;
+}
 
 int tn_svd_set_max_sweeps(int n) {
   TN_REQUIRE(n >= 0, TN_EINVAL, "tn_svd_set_max_sweeps: negative count");

@@ -74,6 +74,7 @@ SIGNATURES = {
     "tn_diag_row_mul_f64": [P, P, I64, I64, P],
     "tn_diag_col_mul_f64": [P, P, I64, I64, P],
     "tn_svd_f64": [P, I64, I64, P, P, P, P],
+    "tn_svd_f64_noalloc": [P, I64, I64, P, P, P, P],
     "tn_svd_f64_batched": [I64, P, P, P, P, P, P, P, P, P, P, P],
     "tn_svd_last_sweeps": [ctypes.POINTER(ctypes.c_int)],
     "tn_svd_set_block_min": [ctypes.c_int64],
