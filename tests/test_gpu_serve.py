@@ -83,3 +83,47 @@ def test_mixed_classes_same_alone_and_in_mix(serve, ora, golden):
     res = mixed[t.rid]
     assert np.max(np.abs(res[:t.chi] - O.lam[0])) <= 1e-10
     assert np.max(np.abs(res[t.chi:] - O.lam[1])) <= 1e-10
+
+
+def test_configured_chi_range_same_alone_and_in_mix(serve, ora, golden):
+    """Serving inside the configured range chi in [32, 1024]: classes chi 32
+    (spin 1/2 dense), 64 (spin 1 U(1)), 256 (spin 1 U(1)) and 1024 (spin 1/2
+    U(1)), TEBD and contraction requests arriving while others run.  Each
+    picked result is the same alone and in the mix, and the chi = 256 spin-1
+    U(1) TEBD request follows the oracle's per-charge-block update to 1e-10."""
+    classes = [(32, 2, 0), (64, 3, 1), (256, 3, 1), (1024, 2, 1)]
+    reqs = []
+    rng = np.random.default_rng(12)
+    t = 0.0
+    for i in range(24):
+        c = classes[i % len(classes)]
+        t += float(rng.exponential(1.0 / 2000.0))
+        kind = "tebd" if i % 3 != 2 else "contract"
+        reqs.append(serve.Request(rid=i, t_arrival=t, kind=kind, chi=c[0], delta=float(rng.uniform(0, 2)),
+                                  seed=500 + i, steps=int(rng.integers(1, 4)), d=c[1], sym=c[2]))
+    budget = sum(2 * serve.slot_bytes(*c) for c in classes)
+    srv = serve.Server(classes, budget * len(classes), golden, max_slots=2)
+    stats = srv.run(reqs, realtime=False)
+    _say("configured range: %d requests, p50 %.2f ms p99 %.2f ms" % (stats["requests"], stats["p50_ms"],
+                                                                      stats["p99_ms"]))
+    assert stats["requests"] == len(reqs)
+    mixed = {r.rid: r.result for r in stats["done"]}
+    picks = [r for r in reqs if r.chi in (256, 1024) and r.kind == "tebd"][:2] + \
+            [next(r for r in reqs if r.kind == "contract" and r.sym == 1)]
+    for target in picks:
+        solo = dataclass_copy(target)
+        out = serve.Server(classes, budget * len(classes), golden, max_slots=2).run([solo], realtime=False)
+        _say("request %d (%s chi=%d d=%d sym=%d)" % (target.rid, target.kind, target.chi, target.d, target.sym))
+        assert out["done"][0].result.tobytes() == mixed[target.rid].tobytes()
+    t = next(r for r in reqs if r.chi == 256 and r.kind == "tebd")
+    O = ora.ITEBDU1(256, 3, t.delta, 0.01, t.seed, golden)
+    for _ in range(t.steps):
+        O.step()
+    res = mixed[t.rid]
+    assert np.max(np.abs(res[:256] - O.lam[0])) <= 1e-10
+    assert np.max(np.abs(res[256:] - O.lam[1])) <= 1e-10
+
+
+def dataclass_copy(r):
+    import dataclasses
+    return dataclasses.replace(r, t_admit=-1.0, t_done=-1.0, result=None)

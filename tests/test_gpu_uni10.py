@@ -271,15 +271,19 @@ def test_network_greedy_plan_matches_oracle(u10, ora, golden):
 MODE_TOL = {0: 1e-12, 1: 2e-6, 2: 5e-5, 3: 2e-6}
 
 
+@pytest.mark.parametrize("size", [(96, 4, 21), (1024, 8, 11)], ids=["chi96", "chi1024"])
 @pytest.mark.parametrize("mode", [0, 1, 2, 3])
-def test_u1_two_site_contraction_modes(u10, ora, mode):
-    """Theta = (A1 . A2) . G for U(1) tensors (chi=96 over Sz in {-4..4},
-    binomial sectors): FP64 within 1e-12 of the oracle; FP32-MFMA and
-    split-BF16 within their stated tolerances."""
+def test_u1_two_site_contraction_modes(u10, ora, mode, size):
+    """Theta = (A1 . A2) . G for U(1) tensors with binomial sectors, at chi=96
+    over Sz in {-4..4} and at the configured chi=1024 over Sz in {-8..8}:
+    FP64 within 1e-12 of the oracle; FP32-MFMA and split-BF16 within their
+    stated tolerances."""
     from trim_amd import workloads
-    A1, A2, G, dims = workloads.u1_two_site(u10, 96, 4, 21, 0.01)
+    chi, qmax, seed = size
+    _say("U(1) two-site contraction chi=%d qmax=%d mode=%d" % (chi, qmax, mode))
+    A1, A2, G, dims = workloads.u1_two_site(u10, chi, qmax, seed, 0.01)
     Q = ora.Qnum
-    charges = list(range(-4, 5))
+    charges = list(range(-qmax, qmax + 1))
     def ob(tp, ch, ds):
         qs = []
         for c, d in zip(ch, ds):
@@ -287,12 +291,12 @@ def test_u1_two_site_contraction_modes(u10, ora, mode):
         return ora.Bond(tp, qs)
     ci, co = ob(ora.BD_IN, charges, dims), ob(ora.BD_OUT, charges, dims)
     po, pi = ob(ora.BD_OUT, [1, -1], [1, 1]), ob(ora.BD_IN, [1, -1], [1, 1])
-    oA1 = ora.UniTensor([ci, po, co], elem=workloads.splitmix_uniform(21, A1.elemNum()))
-    oA2 = ora.UniTensor([ci, po, co], elem=workloads.splitmix_uniform(22, A2.elemNum()))
+    oA1 = ora.UniTensor([ci, po, co], elem=workloads.splitmix_uniform(seed, A1.elemNum()))
+    oA2 = ora.UniTensor([ci, po, co], elem=workloads.splitmix_uniform(seed + 1, A2.elemNum()))
     oA1.set_label([-1, 10, 0])
     oA2.set_label([0, 11, -2])
     oG = ora.UniTensor([pi, pi, po, po])
-    oG.set_raw_elem(ora.exph(-0.01, workloads.sz_hamiltonian(23)))
+    oG.set_raw_elem(ora.exph(-0.01, workloads.sz_hamiltonian(seed + 2)))
     oG.set_label([10, 11, 20, 21])
     assert same_layout(A1, oA1) and same_layout(G, oG)
     ref = ora.contract(ora.contract(oA1, oA2), oG)
@@ -643,3 +647,41 @@ def test_contract_dimension_mismatch_is_an_error(u10, ora):
     oB.set_label([2, 3])
     with pytest.raises(ValueError):
         ora.contract(oA, oB)
+
+
+def test_batch_tebd_512_chains_configured_size_matches_oracle(u10, ora, golden):
+    """The configured batch: 512 spin-1/2 XXZ chains, Delta ~ U[0, 2],
+    chi cycling over 16, 32, ..., 256, chains 0-255 U(1) and 256-511 dense,
+    stepped twice with graph replay as in the benchmark.  A stratified
+    sample of 16 chains (every chi band; U(1) chains with even bands, dense
+    chains with odd ones) follows the oracle's single-chain iTEBD (dense) or
+    per-charge-block update (U(1)) to 1e-10 in energy, both Schmidt spectra
+    and the discarded weight."""
+    n, tau = 512, 0.01
+    chis = list(range(16, 257, 16))
+    chi = [chis[i % 16] for i in range(n)]
+    sym = [1 if i < n // 2 else 0 for i in range(n)]
+    delta = np.random.default_rng(4).uniform(0.0, 2.0, n)
+    seeds = [1000 + 4 * i for i in range(n)]
+    _say("512-chain batch: build")
+    B = u10.BatchTebd(chi, delta, seeds, tau, golden, sym=sym)
+    B.set_graph(True)
+    B.step(1)
+    B.step(1)
+    _say("512-chain batch: 2 steps done")
+    sample = [2 * k for k in range(8)] + [n // 2 + 2 * k + 1 for k in range(8)]
+    assert sorted(chi[i] for i in sample) == chis
+    for i in sample:
+        if sym[i]:
+            O = ora.ITEBDU1(chi[i], 2, float(delta[i]), tau, seeds[i], golden)
+        else:
+            O = ora.ITEBD(chi[i], 2, ora.xxz_two_site_d(float(delta[i]), 2), tau, seeds[i], golden)
+        for _ in range(2):
+            O.step()
+        la, lb = (O.lam[0], O.lam[1]) if sym[i] else (O.La, O.Lb)
+        e, eo = B.energy(i), O.energy()
+        _say("chain %d chi=%d %s: %.15f oracle %.15f" % (i, chi[i], "U(1)" if sym[i] else "dense", e, eo))
+        assert abs(e - eo) <= 1e-10
+        assert np.max(np.abs(B.lam(i, 0) - la)) <= 1e-10
+        assert np.max(np.abs(B.lam(i, 1) - lb)) <= 1e-10
+        assert abs(B.trunc_err(i) - O.trunc_err) <= 1e-10
