@@ -36,7 +36,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-NETDIR = os.path.join(ROOT, "tests", "golden")
+NETDIR = os.path.join(ROOT, "trim_amd", "nets")     # the network files shipped with the package
 
 HBM_PEAK_GBPS = 8000.0
 F64_MFMA_PEAK_TFLOPS = 78.6
@@ -112,27 +112,40 @@ def kernel_parts(tn):
         {"gemm_tiles": info["tiles"], "gemm_tile_cfg": info["tile_id"]}
 
 
-def time_parts(tn, parts, steps):
-    """Average launch duration of each part: a HIP event is recorded just
-    before and just after every launch, on the launch stream.  A few large
-    fill kernels are queued first so the device is still busy while the host
-    enqueues the (event, kernel, event) triples: the intervals then hold the
-    kernel itself, not host enqueue gaps."""
+def time_parts(tn, parts, steps, pairs=None):
+    """Average launch duration of each part, with HIP events on the launch
+    stream.  A few large fill kernels are queued first so the device is
+    still busy while the host enqueues: the events then see device time, not
+    host enqueue gaps.  The figure returned is one event pair around `steps`
+    back-to-back launches, divided by `steps` (kernel time plus the device's
+    dispatch gap between dependent launches).  With `pairs` (a dict), the
+    mean over an event pair around every single launch is stored there too
+    (it adds the events' own cost to each launch)."""
     spacer = tn.DeviceArray((256 * 1024 * 1024,))   # 2 GiB
     out = {}
     for name, fn in parts:
         for _ in range(3):
             fn()
         tn.sync()
-        evs = [(tn.Event(), tn.Event()) for _ in range(steps)]
         for _ in range(max(4, steps // 25)):
             tn.call("tn_zero", tn._p(spacer), spacer.nbytes, None)
-        for a, b in evs:
-            a.record()
+        e0, e1 = tn.Event(), tn.Event()
+        e0.record()
+        for _ in range(steps):
             fn()
-            b.record()
+        e1.record()
         tn.sync()
-        out[name] = float(np.mean([a.elapsed_ms(b) for a, b in evs]))
+        out[name] = e0.elapsed_ms(e1) / steps
+        if pairs is not None:
+            evs = [(tn.Event(), tn.Event()) for _ in range(steps)]
+            for _ in range(max(4, steps // 25)):
+                tn.call("tn_zero", tn._p(spacer), spacer.nbytes, None)
+            for a, b in evs:
+                a.record()
+                fn()
+                b.record()
+            tn.sync()
+            pairs[name] = float(np.mean([a.elapsed_ms(b) for a, b in evs]))
     spacer.free()
     return out
 
@@ -204,6 +217,42 @@ def recorded_traffic(kernel):
     return t.get("kernels", {}).get(kernel)
 
 
+def c3_sector_operands(tn, u10, A1, A2):
+    """The sector GEMMs of A1 . A2 (one per middle-bond charge): A1 permuted
+    to (left, phys | middle), A2 to (middle | phys, right); descriptors for
+    one grouped launch."""
+    A1p = A1.copy().permute([-1, 10, 0], 2)
+    A2p = A2.copy().permute([0, 11, -2], 1)
+    b1 = {q.triple(): (r, c, o) for q, r, c, o in A1p.blocks()}
+    b2 = {q.triple(): (r, c, o) for q, r, c, o in A2p.blocks()}
+    descs, oc = [], 0
+    for q, (r1, c1, o1) in b1.items():
+        if q in b2:
+            r2, c2, o2 = b2[q]
+            descs.append((r1, c2, c1, o1, o2, oc, max(c1, 1), max(c2, 1), max(c2, 1)))
+            oc += r1 * c2
+    C = tn.DeviceArray((oc,))
+    return descs, A1p.device_ptr(), A2p.device_ptr(), C, (A1p, A2p)
+
+
+def c3_sector_only(tn, u10, steps, mode=0):
+    """Only the grouped A1 . A2 sector GEMM of the U(1) chi=1024 workload:
+    autotune, then `steps` timed launches (for kernel-trace and counter
+    runs)."""
+    from trim_amd import workloads
+    A1, A2, G, dims = workloads.u1_two_site(u10, 1024, 8, 11, 0.01)
+    descs, pa, pb, C, keep = c3_sector_operands(tn, u10, A1, A2)
+    plan = tn.GemmPlan(mode, np.array(descs, dtype=tn.GEMM_DESC))
+    plan.autotune(pa, pb, C, reps=3)
+    info = plan.info()
+    pairs = {}
+    t = time_parts(tn, [("sector", lambda: plan(pa, pb, C))], steps, pairs)["sector"]
+    return {"sectors": len(descs), "mode": mode, "tiles": info["tiles"], "tile_cfg": info["tile_id"],
+            "flop": info["flops"], "kernel_ms": t, "kernel_ms_event_pairs": pairs["sector"],
+            "tflops": info["flops"] / (t * 1e-3) / 1e12,
+            "shapes_MNK": [(d[0], d[1], d[2]) for d in descs]}
+
+
 def c3_u1(tn, u10, steps=10):
     """U(1) two-site contraction, chi=1024 over Sz in {-8..8} (binomial sector
     dims), d=2: Theta = (A1 . A2) . G through the drop-in API in every GEMM
@@ -230,18 +279,7 @@ def c3_u1(tn, u10, steps=10):
         res[name] = {"ms_per_contraction": ms, "rel_frobenius_err_vs_f64": err}
     u10.set_gemm_mode(0)
     # the sector GEMM of A1 . A2 as one grouped launch
-    A1p = A1.copy().permute([-1, 10, 0], 2)
-    A2p = A2.copy().permute([0, 11, -2], 1)
-    b1 = {q.triple(): (r, c, o) for q, r, c, o in A1p.blocks()}
-    b2 = {q.triple(): (r, c, o) for q, r, c, o in A2p.blocks()}
-    descs, oc = [], 0
-    for q, (r1, c1, o1) in b1.items():
-        if q in b2:
-            r2, c2, o2 = b2[q]
-            descs.append((r1, c2, c1, o1, o2, oc, max(c1, 1), max(c2, 1), max(c2, 1)))
-            oc += r1 * c2
-    C = tn.DeviceArray((oc,))
-    pa, pb = A1p.device_ptr(), A2p.device_ptr()
+    descs, pa, pb, C, keep = c3_sector_operands(tn, u10, A1, A2)
     sect = {"sectors": len(descs)}
     for name, mode in modes:
         plan = tn.GemmPlan(mode, np.array(descs, dtype=tn.GEMM_DESC))
@@ -415,15 +453,25 @@ def lanczos_leg(tn):
             "note": "matrix bytes per step / step time (whole step: matvec, vector work, host bisection)"}
 
 
-def cpu_baseline(budget_s=10.0, max_steps=400):
+def cpu_threads():
+    """Threads the host leg uses for 'all cores': the CPUs this process may
+    run on, capped by OMP_NUM_THREADS when the environment sets it (the
+    machine's share for this job)."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return (min(avail, omp) if omp > 0 else avail), avail
+
+
+def cpu_baseline(budget_s=8.0, max_steps=400):
     """The oracle's contraction step on the host: C odometer permutes (one
-    thread) and numpy/OpenBLAS dgemm (BLAS threads = cores)."""
+    thread) and numpy/OpenBLAS dgemm, timed with one BLAS thread and with
+    every thread available to the process."""
     from oracle import uni10_oracle as ora
     try:
         from threadpoolctl import threadpool_limits
     except Exception:  # pragma: no cover
         threadpool_limits = None
-    cores = min(os.cpu_count() or 1, 16)
+    cores, avail = cpu_threads()
     dims = np.array(DIMS, np.int64)
     n = int(np.prod(dims))
     A = splitmix_uniform(1, n)
@@ -440,16 +488,21 @@ def cpu_baseline(budget_s=10.0, max_steps=400):
         L.ora_permute_dense(ora._ptr(Bp), ora._ptr(Bb), 4, ora._ptr(dims), ora._ptr(pb))
         L.ora_permute_dense(ora._ptr(C), ora._ptr(D), 4, ora._ptr(dims), ora._ptr(pc))
 
-    lim = threadpool_limits(limits=cores) if threadpool_limits else None
-    step()
-    t0 = time.perf_counter()
-    k = 0
-    while k < max_steps and time.perf_counter() - t0 < budget_s:
-        step()
-        k += 1
-    dt = (time.perf_counter() - t0) / k
-    if lim is not None and hasattr(lim, "unregister"):
-        lim.unregister()
+    def timed(threads):
+        lim = threadpool_limits(limits=threads) if threadpool_limits else None
+        try:
+            step()
+            t0 = time.perf_counter()
+            k = 0
+            while k < max_steps and time.perf_counter() - t0 < budget_s:
+                step()
+                k += 1
+            return (time.perf_counter() - t0) / k, k
+        finally:
+            if lim is not None and hasattr(lim, "unregister"):
+                lim.unregister()
+    dt1, k1 = timed(1)
+    dt, k = timed(cores)
     # iTEBD on the oracle (chi=32), a short sample
     t = ora.ITEBD(32, 2, ora.heisenberg_two_site(1.0), 0.01, 7, NETDIR)
     t.step()
@@ -472,9 +525,12 @@ def cpu_baseline(budget_s=10.0, max_steps=400):
         lim.unregister()
     return {"value": 2.0 * 512 ** 3 / dt / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
             "sample": "%d contraction-and-permute steps (256,2,2,256) FP64 in %.1f s: oracle C odometer "
-                      "permutes (1 thread) + numpy/OpenBLAS dgemm (%d threads), %.3f ms/step; oracle iTEBD "
-                      "chi=32: %d steps, %.1f steps/s" % (k, k * dt, cores, dt * 1e3, ns, tebd),
-            "ms_per_step": dt * 1e3, "tebd_steps_per_s": tebd, "lanczos_ms_per_step_n8192": lz * 1e3}
+                      "permutes (1 thread) + numpy/OpenBLAS dgemm (%d threads), %.3f ms/step (1 thread: %d steps, "
+                      "%.3f ms/step); oracle iTEBD chi=32: %d steps, %.1f steps/s"
+                      % (k, k * dt, cores, dt * 1e3, k1, dt1 * 1e3, ns, tebd),
+            "value_1_thread": 2.0 * 512 ** 3 / dt1 / 1e12, "ms_per_step": dt * 1e3, "ms_per_step_1_thread": dt1 * 1e3,
+            "nproc": os.cpu_count(), "cpus_available": avail,
+            "tebd_steps_per_s": tebd, "lanczos_ms_per_step_n8192": lz * 1e3}
 
 
 def progress(msg):
@@ -495,6 +551,10 @@ def main():
     ap.add_argument("--batch-only", type=int, default=0, metavar="STEPS",
                     help="only the 512-chain chi 16..256 half-U(1) batch for STEPS steps, one JSON line "
                          "(for kernel-trace runs of the batched update)")
+    ap.add_argument("--c3-only", type=int, default=0, metavar="STEPS",
+                    help="only the grouped U(1) chi=1024 A1.A2 sector GEMM, STEPS timed launches, one JSON line "
+                         "(for kernel-trace and counter runs)")
+    ap.add_argument("--c3-mode", type=int, default=0, help="GEMM mode of --c3-only (0 FP64)")
     ap.add_argument("--lanczos-only", action="store_true",
                     help="only the Lanczos leg (8192 x 8192), one JSON line (for kernel-trace runs)")
     args = ap.parse_args()
@@ -509,6 +569,9 @@ def main():
     info = tn.device_info()
     if args.lanczos_only:
         print(json.dumps(lanczos_leg(tn)), flush=True)
+        return
+    if args.c3_only:
+        print(json.dumps(c3_sector_only(tn, uni10, args.c3_only, args.c3_mode)), flush=True)
         return
     if args.batch_only:
         out = c4_batch(uni10, list(range(16, 257, 16)), args.batch_only, "16..256 step 16, 32 chains each",
@@ -536,7 +599,8 @@ def main():
 
     progress("headline step done: %.4f ms/step" % ms)
     parts, kflops, n, keep, ginfo = kernel_parts(tn)
-    per = time_parts(tn, parts, max(20, min(args.steps, 200)))
+    pairs = {}
+    per = time_parts(tn, parts, max(20, min(args.steps, 200)), pairs)
     gemm_tf = kflops / (per["gemm_f64"] * 1e-3) / 1e12
     perm_gbps = 16.0 * n / (per["permute"] * 1e-3) / 1e9
     hbm_gbps, hbm_ms = permute_hbm(tn, 10)
@@ -556,7 +620,7 @@ def main():
     progress("kernel parts and HBM permute done")
     if args.headline_only:
         print(json.dumps({"metric": "headline step (counter run)", "ms_per_step": ms, "roofline": roof,
-                          "kernels_ms": per, "copy_calibration_bytes": copy_bytes,
+                          "kernels_ms": per, "kernels_ms_event_pairs": pairs, "copy_calibration_bytes": copy_bytes,
                           "permute_GBps_hbm_4096x2x2x4096": hbm_gbps, "copy_GBps_same_bytes": copy_gbps}),
               flush=True)
         return
@@ -595,6 +659,7 @@ def main():
                    "parallelism": "1 GPU"},
         "roofline": roof,
         "kernels_ms": per,
+        "kernels_ms_event_pairs": pairs,
         "gemm_plan": ginfo,
         "permute_GBps_config": perm_gbps,
         "permute_GBps_hbm_4096x2x2x4096": hbm_gbps,

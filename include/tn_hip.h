@@ -198,6 +198,14 @@ int tn_matmul_c128(const double* A, const double* B, int64_t M, int64_t N, int64
  * diagRowMul / diagColMul (:46-57).  tn_norm2_f64 writes ||x||_2 to a device
  * double with a fixed reduction order (deterministic, run to run). */
 int tn_norm2_f64(const double* x, int64_t n, double* out_dev, void* stream);
+/* vectorSum (uni10_lapack_cpu.cpp:387-395): x[0] + x[inc] + ... (n terms)
+ * to a device double, in a fixed order; vectorAdd (:59-74, daxpy):
+ * y += a x; the strided copy dst[i*dst_inc] = src[i*src_inc] (i < n) of
+ * setDiag / getDiag (uni10_tools_cpu.cpp:170-181). */
+int tn_sum_f64(const double* x, int64_t n, int64_t inc, double* out_dev, void* stream);
+int tn_axpy_f64(double a, const double* x, double* y, int64_t n, void* stream);
+int tn_copy_strided_f64(double* dst, int64_t dst_inc, const double* src, int64_t src_inc, int64_t n,
+                        void* stream);
 int tn_scale_f64(double* x, int64_t n, double a, void* stream);
 int tn_scale_inv_dev_f64(double* x, int64_t n, const double* denom_dev, void* stream);
 int tn_diag_row_mul_f64(double* mat, const double* diag, int64_t M, int64_t N, void* stream);

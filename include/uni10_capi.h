@@ -101,7 +101,10 @@ int u10_exph(double a, const double* H, int64_t N, double* out);
 int u10_lanczos_eigh(const double* A, int64_t N, double* psi, uint64_t max_iter, double err_tol, double* E0,
                      uint64_t* iters);
 
-/* iTEBD of the spin-1/2 XXZ chain through Networks (the example driver). */
+/* iTEBD of the spin-1/2 XXZ chain through Networks (the example driver).
+ * netdir (here and in u10_batch_tebd_create*): the directory holding
+ * itebd.net, updateA.net, MPS.net and measure.net; NULL or "" = the nets/
+ * directory installed beside libuni10_amd.so. */
 int u10_itebd_create(int chi, int d, double delta, double tau, uint64_t seed, const char* netdir,
                      u10_itebd* out);
 int u10_itebd_step(u10_itebd t, int nsteps);

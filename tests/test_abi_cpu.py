@@ -154,3 +154,29 @@ def test_product_bond_combine_known_answer_and_oracle(built, ora):
         assert tp == oc.type and dim == oc.dim
         assert [(q.triple(), n) for q, n in runs] == [(q.triple(), n) for q, n in zip(oc.Qnums, oc.Qdegs)]
         assert [(q.triple(), n) for q, n in deg] == [(q.triple(), n) for q, n in oc.degeneracy().items()]
+
+
+def test_seam_library_exports_the_reference_signatures(built):
+    """libuni10_seam_amd.so defines every function of include/uni10_seam_amd.h
+    with the reference's C++ signature (matched on the demangled names), and
+    links only the C ABI library."""
+    hdr = open(os.path.join(ROOT, "include", "uni10_seam_amd.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    hdr = re.sub(r"//[^\n]*", "", hdr)
+    decls = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\(([^)]*)\);", hdr)
+    assert len(decls) >= 24
+    lib = os.path.join(os.path.dirname(built.TN_LIB), "libuni10_seam_amd.so")
+    out = subprocess.run(["nm", "-D", "-C", "--defined-only", lib], stdout=subprocess.PIPE, text=True).stdout
+    exported = set(l.split(" T ", 1)[1] for l in out.splitlines() if " T " in l)
+    for name, args in decls:
+        types = []
+        for a in args.split(","):
+            a = a.strip()
+            t = re.sub(r"\s*\b[A-Za-z_][A-Za-z0-9_]*$", "", a)       # drop the parameter name
+            t = t.replace(" *", "*").replace(" &", "&")
+            types.append(re.sub(r"^const (\w+)\*$", r"\1 const*", t))    # the demangler's spelling
+        sig = "uni10::%s(%s)" % (name, ", ".join(types))
+        sig = sig.replace("size_t", "unsigned long")
+        assert sig in exported, sig
+    deps = subprocess.run(["ldd", lib], stdout=subprocess.PIPE, text=True).stdout
+    assert "libtn_hip.so" in deps and "libuni10_amd.so" not in deps

@@ -7,6 +7,10 @@ Outputs (git-ignored, shipped with the tree):
     trim_amd/libuni10_amd.so   host C++ Uni10 API (include/uni10_amd/*.h) and
                                its C facade (include/uni10_capi.h), linked
                                against libtn_hip.so
+    trim_amd/libuni10_seam_amd.so  the reference's backend seam functions
+                               (include/uni10_seam_amd.h) over libtn_hip.so
+    trim_amd/bin/itebd_dropin  an iTEBD program written against the C++ API
+    trim_amd/bin/seam_check    a driver of the backend seam functions
 """
 from __future__ import annotations
 
@@ -35,6 +39,9 @@ CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-I" + INC, "-I" + os.path.join(CSRC,
 
 TN_LIB = os.path.join(PKG, "libtn_hip.so")
 U10_LIB = os.path.join(PKG, "libuni10_amd.so")
+SEAM_LIB = os.path.join(PKG, "libuni10_seam_amd.so")
+BIN = os.path.join(PKG, "bin")
+EXAMPLES = {"itebd_dropin": "-luni10_amd", "seam_check": "-luni10_seam_amd"}
 
 
 def _run(cmd):
@@ -56,6 +63,22 @@ def _compile_cxx(src):
     return out
 
 
+def _link_so(target, objs, libs):
+    tmp = target + ".tmp%d" % os.getpid()
+    _run([CXX, "-shared", "-fPIC", "-o", tmp] + objs + ["-L" + PKG] + libs + ["-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, target)
+
+
+def _build_example(name):
+    src = os.path.join(PKG, "examples", name + ".cpp")
+    out = os.path.join(BIN, name)
+    tmp = out + ".tmp%d" % os.getpid()
+    _run([CXX, "-O2", "-std=c++17", "-Wall", "-I" + INC, src, "-o", tmp, "-L" + PKG, EXAMPLES[name], "-ltn_hip",
+          "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(tmp, out)
+    return out
+
+
 def build(verbose: bool = True) -> None:
     os.makedirs(OBJ, exist_ok=True)
     hip_srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
@@ -66,13 +89,18 @@ def build(verbose: bool = True) -> None:
     tmp = TN_LIB + ".tmp%d" % os.getpid()
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + hip_objs)
     os.replace(tmp, TN_LIB)
-    if cxx_objs:
-        tmp = U10_LIB + ".tmp%d" % os.getpid()
-        _run([CXX, "-shared", "-fPIC", "-o", tmp] + cxx_objs +
-             ["-L" + PKG, "-ltn_hip", "-Wl,-rpath,$ORIGIN"])
-        os.replace(tmp, U10_LIB)
+    _link_so(U10_LIB, cxx_objs, ["-ltn_hip", "-ldl"])
+    seam_objs = []
+    for src in sorted(glob.glob(os.path.join(CSRC, "seam", "*.cpp"))):
+        out = os.path.join(OBJ, "seam_" + os.path.basename(src) + ".o")
+        _run([CXX, "-O2", "-fPIC", "-std=c++17", "-Wall", "-I" + INC, "-c", src, "-o", out])
+        seam_objs.append(out)
+    _link_so(SEAM_LIB, seam_objs, ["-ltn_hip"])
+    os.makedirs(BIN, exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=2) as ex:
+        list(ex.map(_build_example, sorted(EXAMPLES)))
     if verbose:
-        print("built", TN_LIB, "and", U10_LIB if cxx_objs else "(no host C++ sources)")
+        print("built", TN_LIB, U10_LIB, SEAM_LIB, "and", ", ".join(os.path.join(BIN, n) for n in sorted(EXAMPLES)))
 
 
 if __name__ == "__main__":

@@ -384,6 +384,28 @@ __global__ void __launch_bounds__(1024) norm2_kernel(const double* __restrict__ 
   if (threadIdx.x == 0) *out = sqrt(t);
 }
 
+// x[0], x[inc], ... summed in a fixed order (per-thread strided partials,
+// then the block tree), like the norm.
+__global__ void __launch_bounds__(1024) sum_kernel(const double* __restrict__ x, int64_t n, int64_t inc,
+                                                   double* __restrict__ out) {
+  __shared__ double red[1024];
+  double s = 0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) s += x[i * inc];
+  double t = block_sum<1024>(s, red);
+  if (threadIdx.x == 0) *out = t;
+}
+
+__global__ void axpy_kernel(double a, const double* __restrict__ x, double* __restrict__ y, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
+
+__global__ void copy_strided_kernel(double* __restrict__ dst, int64_t dinc, const double* __restrict__ src,
+                                    int64_t sinc, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i * dinc] = src[i * sinc];
+}
+
 __global__ void scale_kernel(double* __restrict__ x, int64_t n, double a) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -1411,6 +1433,35 @@ int tn_eigh_f64(const double* A, int64_t N, double* eig, double* vec_rows, void*
   tn::pool_free(d_status);
   if (rc != TN_OK) return rc;
   TN_REQUIRE(h == 0, TN_ENOCONV, "eigh: Jacobi did not converge");
+  return TN_OK;
+}
+
+int tn_sum_f64(const double* x, int64_t n, int64_t inc, double* out_dev, void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(out_dev && (n == 0 || x) && n >= 0 && inc >= 1, TN_EINVAL, "tn_sum_f64: bad argument");
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(1024), 0, tn::stream_of(stream), x, n, inc, out_dev);
+  TN_LAUNCH_CHECK();
+  return TN_OK;
+}
+
+int tn_axpy_f64(double a, const double* x, double* y, int64_t n, void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(n >= 0 && (n == 0 || (x && y)), TN_EINVAL, "tn_axpy_f64: bad argument");
+  if (n == 0) return TN_OK;
+  hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(n)), dim3(256), 0, tn::stream_of(stream), a, x, y, n);
+  TN_LAUNCH_CHECK();
+  return TN_OK;
+}
+
+int tn_copy_strided_f64(double* dst, int64_t dst_inc, const double* src, int64_t src_inc, int64_t n,
+                        void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(n >= 0 && dst_inc >= 1 && src_inc >= 1 && (n == 0 || (dst && src)), TN_EINVAL,
+             "tn_copy_strided_f64: bad argument");
+  if (n == 0) return TN_OK;
+  hipLaunchKernelGGL(copy_strided_kernel, dim3(grid_for(n)), dim3(256), 0, tn::stream_of(stream), dst, dst_inc,
+                     src, src_inc, n);
+  TN_LAUNCH_CHECK();
   return TN_OK;
 }
 

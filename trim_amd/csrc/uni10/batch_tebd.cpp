@@ -287,7 +287,7 @@ void BatchTebd::layout_u1(int i) {
 BatchTebd::BatchTebd(const std::vector<int>& chi, const std::vector<double>& delta,
                      const std::vector<uint64_t>& seed, double tau, int gemm_mode, const std::string& netdir,
                      const std::vector<int>& d, const std::vector<int>& sym)
-    : chi_(chi), delta_(delta), netdir_(netdir) {
+    : chi_(chi), delta_(delta), netdir_(detail::netdir_or_default(netdir)) {
   const size_t nc = chi.size();
   if (delta.size() != nc || seed.size() != nc) detail::fail("BatchTebd: chi, delta and seed differ in length");
   if (!d.empty() && d.size() != nc) detail::fail("BatchTebd: d and chi differ in length");

@@ -407,7 +407,7 @@ int u10_exph(double a, const double* H, int64_t N, double* out) {
 }
 
 int u10_itebd_create(int chi, int d, double delta, double tau, uint64_t seed, const char* netdir, u10_itebd* out) {
-  return guard([&] { *out = new u10_itebd_s(chi, d, delta, tau, seed, std::string(netdir)); });
+  return guard([&] { *out = new u10_itebd_s(chi, d, delta, tau, seed, std::string(netdir ? netdir : "")); });
 }
 
 int u10_itebd_step(u10_itebd t, int n) {
@@ -440,18 +440,18 @@ int u10_itebd_free(u10_itebd t) {
 int u10_batch_tebd_create(int n, const int* chi, const double* delta, const uint64_t* seed, double tau,
                           int gemm_mode, const char* netdir, u10_batch_tebd* out) {
   return guard([&] {
-    if (n < 1 || !chi || !delta || !seed || !netdir || !out) detail::fail("u10_batch_tebd_create: bad argument");
+    if (n < 1 || !chi || !delta || !seed || !out) detail::fail("u10_batch_tebd_create: bad argument");
     *out = new u10_batch_tebd_s(std::vector<int>(chi, chi + n), std::vector<double>(delta, delta + n),
-                                std::vector<uint64_t>(seed, seed + n), tau, gemm_mode, std::string(netdir));
+                                std::vector<uint64_t>(seed, seed + n), tau, gemm_mode, std::string(netdir ? netdir : ""));
   });
 }
 
 int u10_batch_tebd_create_ex(int n, const int* chi, const double* delta, const uint64_t* seed, const int* d,
                              const int* sym, double tau, int gemm_mode, const char* netdir, u10_batch_tebd* out) {
   return guard([&] {
-    if (n < 1 || !chi || !delta || !seed || !netdir || !out) detail::fail("u10_batch_tebd_create_ex: bad argument");
+    if (n < 1 || !chi || !delta || !seed || !out) detail::fail("u10_batch_tebd_create_ex: bad argument");
     *out = new u10_batch_tebd_s(std::vector<int>(chi, chi + n), std::vector<double>(delta, delta + n),
-                                std::vector<uint64_t>(seed, seed + n), tau, gemm_mode, std::string(netdir),
+                                std::vector<uint64_t>(seed, seed + n), tau, gemm_mode, std::string(netdir ? netdir : ""),
                                 d ? std::vector<int>(d, d + n) : std::vector<int>(),
                                 sym ? std::vector<int>(sym, sym + n) : std::vector<int>());
   });

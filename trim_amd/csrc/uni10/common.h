@@ -35,5 +35,9 @@ void* stream();
 int gemm_mode();
 void sync();
 
+// The directory of the network files the iTEBD drivers read: `netdir` when
+// given, else the `nets` directory installed beside libuni10_amd.so.
+std::string netdir_or_default(const std::string& netdir);
+
 }  // namespace detail
 }  // namespace uni10

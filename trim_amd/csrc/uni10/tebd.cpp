@@ -1,6 +1,8 @@
 // iTEBD driver written against the Uni10 API, as uni10/examples/iTEBD.cpp
 // (update :148-173, measure :189-203, expectation :205-216) with the
 // .net files shipped beside it; spin-1/2 XXZ two-site gate exp(-tau H).
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cmath>
 
@@ -8,6 +10,16 @@
 #include "tebd.h"
 
 namespace uni10 {
+
+std::string detail::netdir_or_default(const std::string& netdir) {
+  if (!netdir.empty()) return netdir;
+  Dl_info info;
+  if (dladdr(reinterpret_cast<void*>(&detail::netdir_or_default), &info) == 0 || !info.dli_fname)
+    detail::fail("cannot locate libuni10_amd.so to find its network files; pass a directory");
+  std::string lib(info.dli_fname);
+  const size_t slash = lib.rfind('/');
+  return (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) + "/nets";
+}
 
 Matrix xxz_two_site(double delta, double J) {
   // J (Sx Sx + Sy Sy + delta Sz Sz), basis |uu>,|ud>,|du>,|dd>
@@ -37,8 +49,9 @@ Matrix xxz_two_site_spin1(double delta, double J) {
 
 ITebd::ITebd(int chi, int d, double delta, double tau, uint64_t seed, const std::string& netdir)
     : chi_(chi), d_(d), tau_(tau),
-      iTEBD_(netdir + "/itebd.net"), updateA_(netdir + "/updateA.net"), MPS_(netdir + "/MPS.net"),
-      meas_(netdir + "/measure.net") {
+      iTEBD_(detail::netdir_or_default(netdir) + "/itebd.net"),
+      updateA_(detail::netdir_or_default(netdir) + "/updateA.net"),
+      MPS_(detail::netdir_or_default(netdir) + "/MPS.net"), meas_(detail::netdir_or_default(netdir) + "/measure.net") {
   if (d != 2) detail::fail("ITebd: only the spin-1/2 chain (d = 2) is provided");
   Matrix Hm = xxz_two_site(delta, 1.0);
   std::vector<Bond> bH = {Bond(BD_IN, d), Bond(BD_IN, d), Bond(BD_OUT, d), Bond(BD_OUT, d)};

@@ -94,7 +94,7 @@ def _cls(c):
 
 
 class Server:
-    def __init__(self, classes, hbm_budget: float, netdir: str, gemm_mode: int = 0,
+    def __init__(self, classes, hbm_budget: float, netdir: Optional[str] = None, gemm_mode: int = 0,
                  max_slots: int = 256, tau: float = 0.01):
         """classes: (chi, d, sym) tuples (or bare chi for (chi, 2, dense))."""
         self.netdir = netdir

@@ -69,6 +69,9 @@ SIGNATURES = {
     "tn_permute_dense_c128": [P, P, ctypes.c_int, P, P, P],
     "tn_matmul_c128": [P, P, I64, I64, I64, P, P],
     "tn_norm2_f64": [P, I64, P, P],
+    "tn_sum_f64": [P, I64, I64, P, P],
+    "tn_axpy_f64": [ctypes.c_double, P, P, I64, P],
+    "tn_copy_strided_f64": [P, I64, P, I64, I64, P],
     "tn_scale_f64": [P, I64, ctypes.c_double, P],
     "tn_scale_inv_dev_f64": [P, I64, P, P],
     "tn_diag_row_mul_f64": [P, P, I64, I64, P],

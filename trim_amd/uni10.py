@@ -17,6 +17,7 @@ from . import tn as _tn
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libuni10_amd.so")
+NETDIR = os.path.join(_PKG, "nets")     # the iTEBD drivers' network files
 
 BD_IN, BD_OUT = 1, -1
 PRT_EVEN, PRT_ODD = 0, 1
@@ -529,7 +530,8 @@ class BatchTebd:
     d[i] = 2 (spin 1/2) or 3 (spin 1), sym[i] = 0 (dense) or 1 (U(1));
     None = all 2 / all dense."""
 
-    def __init__(self, chi, delta, seed, tau: float, netdir: str, gemm_mode: int = 0, d=None, sym=None):
+    def __init__(self, chi, delta, seed, tau: float, netdir: Optional[str] = None, gemm_mode: int = 0, d=None,
+                 sym=None):
         chi = np.ascontiguousarray(chi, dtype=np.int32)
         delta = np.ascontiguousarray(delta, dtype=np.float64)
         seed = np.ascontiguousarray(seed, dtype=np.uint64)
@@ -538,7 +540,7 @@ class BatchTebd:
         h = ctypes.c_void_p()
         _call("u10_batch_tebd_create_ex", len(chi), chi.ctypes.data_as(P), delta.ctypes.data_as(P),
               seed.ctypes.data_as(P), None if dd is None else dd.ctypes.data_as(P),
-              None if sy is None else sy.ctypes.data_as(P), float(tau), int(gemm_mode), netdir.encode(),
+              None if sy is None else sy.ctypes.data_as(P), float(tau), int(gemm_mode), (netdir or "").encode(),
               ctypes.byref(h))
         self.h = h.value
         self.chi = chi
@@ -612,10 +614,10 @@ class BatchTebd:
 class ITebd:
     """The reference example's iTEBD (Networks + SVD truncation) in C++."""
 
-    def __init__(self, chi: int, d: int, delta: float, tau: float, seed: int, netdir: str):
+    def __init__(self, chi: int, d: int, delta: float, tau: float, seed: int, netdir: Optional[str] = None):
         h = ctypes.c_void_p()
         _call("u10_itebd_create", int(chi), int(d), float(delta), float(tau), int(seed),
-              netdir.encode(), ctypes.byref(h))
+              (netdir or "").encode(), ctypes.byref(h))
         self.h = h.value
         self.chi = chi
 
