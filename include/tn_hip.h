@@ -69,7 +69,11 @@ int tn_event_elapsed_ms(void* ev_start, void* ev_stop, float* ms);
  * an executable graph (no host round trips inside: calls that copy from or
  * to host memory synchronise and are not capturable).  Replaying it issues
  * the same kernels with no host gaps.  No memset nodes are recorded: the
- * library clears memory with fill kernels. */
+ * library clears memory with fill kernels.  Device memory allocated from the
+ * pool during a capture belongs to the graph (its replays write it) and is
+ * returned to the pool only when the graph is destroyed, whenever its holder
+ * frees it; memory freed during a capture is likewise held until then.  One
+ * capture at a time per process; tn_graph_destroy synchronises the device. */
 int tn_graph_begin(void* stream);
 int tn_graph_end(void* stream, void** graph_exec);
 int tn_graph_launch(void* graph_exec, void* stream);

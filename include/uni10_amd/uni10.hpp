@@ -321,6 +321,12 @@ class Network {
   void putTensorT(const std::string& nameT, const UniTensor* T, bool force = true);
   void putTensorT(const std::string& nameT, const UniTensor& T, bool force = true);
   UniTensor launch(const std::string& name = "");
+  // Graph replay (no counterpart in the reference): with on, launch()
+  // captures the whole contraction (copies of the leaf tensors, every
+  // permute and sector GEMM, the final permute) into a HIP graph once per
+  // leaf layout and buffer, then replays it; the leaves keep their put
+  // state and the result is a copy of the graph's output.
+  void setGraph(bool on);
   std::string profile(bool print = true);
   std::string tree() const;     // contraction tree as nested (a b) of tensor names
   friend std::ostream& operator<<(std::ostream& os, Network& net);
@@ -333,6 +339,10 @@ class Network {
   void preprint(std::ostream& os, const NetNode* nd, int layer) const;
   void findConOrd(NetNode* nd);
   void addSwap();
+  UniTensor merge_on(NetNode* nd, std::vector<UniTensor>& W);
+  UniTensor run_on_copies();
+  UniTensor launch_graph(const std::string& name);
+  void drop_graph();
   void matching(NetNode* sbj, NetNode* tar);
   void branch(NetNode* sbj, NetNode* tar);
   UniTensor merge(NetNode* nd);
@@ -348,6 +358,10 @@ class Network {
   std::vector<int> conOrder;                  // leaves in pre-order of the tree
   std::vector<bool> swapflags;                // gates applied to tensors[t]
   std::vector<std::vector<_Swap>> swaps_arr;  // gates of tensors[t]
+  bool use_graph = false;
+  void* graph = nullptr;                      // captured launch
+  std::string graph_sig, graph_failed_sig;    // leaf layouts/buffers it was captured for
+  UniTensor* graph_out = nullptr;             // its output (graph-owned memory)
   NetNode* root = nullptr;
   bool load = false;
 };

@@ -83,6 +83,9 @@ int u10_network_tree(u10_network net, char* buf, int len);
 /* operator<<(ostream, Network) (Network.cpp:856-907): tensor list, then the
  * contraction tree with element counts, in the reference's text format. */
 int u10_network_print(u10_network net, char* buf, int len);
+/* Network::setGraph: launches replay one captured HIP graph per leaf state
+ * (no counterpart in the reference). */
+int u10_network_set_graph(u10_network net, int on);
 int u10_network_free(u10_network net);
 
 /* RDotR (BlockTools.cpp:38-76), the Matrix product of the C++ API, on host

@@ -685,3 +685,53 @@ def test_batch_tebd_512_chains_configured_size_matches_oracle(u10, ora, golden):
         assert np.max(np.abs(B.lam(i, 0) - la)) <= 1e-10
         assert np.max(np.abs(B.lam(i, 1) - lb)) <= 1e-10
         assert abs(B.trunc_err(i) - O.trunc_err) <= 1e-10
+
+
+def test_network_graph_replay_matches_eager(u10, ora, golden):
+    """Network.setGraph: launches replay a captured graph.  On fresh inputs
+    put before every launch (with other device work queued in between), on a
+    change of bond dimension (a new capture) and back, the replayed result
+    equals the eager launch's bit for bit."""
+    E = u10.Network(os.path.join(golden, "itebd.net"))
+    G = u10.Network(os.path.join(golden, "itebd.net"))
+    G.setGraph(True)
+    d = 2
+    U = u10.UniTensor([u10.Bond(u10.BD_IN, d)] * 2 + [u10.Bond(u10.BD_OUT, d)] * 2)
+    U.putBlock(ora.exph(-0.01, ora.heisenberg_two_site(0.8)))
+    for it, chi in enumerate((12, 12, 12, 20, 12)):
+        b3 = [u10.Bond(u10.BD_IN, chi), u10.Bond(u10.BD_IN, chi), u10.Bond(u10.BD_OUT, d)]
+        A, B = u10.UniTensor(b3), u10.UniTensor(b3)
+        A.setElem(ora.splitmix64_uniform(100 + it, A.elemNum()))
+        B.setElem(ora.splitmix64_uniform(200 + it, B.elemNum()))
+        for net in (E, G):
+            net.putTensor("ALa", A)
+            net.putTensor("BLb", B)
+            net.putTensor("expH", U)
+        re_ = E.launch()
+        junk = u10.contract(A.copy().setLabel([1, 2, 3]), B.copy().setLabel([5, 4, 3]))   # other work between
+        rg = G.launch()
+        _say("graph replay iteration %d chi=%d" % (it, chi))
+        assert rg.label() == re_.label() and rg.inBondNum() == re_.inBondNum()
+        assert rg.getElem().tobytes() == re_.getElem().tobytes()
+        del junk
+
+
+def test_network_graph_replay_u1_network(u10, ora, tmp_path):
+    """The Network.h network with U(1) bonds (a scalar result) in graph mode:
+    two replays on different inputs equal the eager launches."""
+    from netcases import NETWORK_H_NET, network_h_bonds
+    p = tmp_path / "network_h.net"
+    p.write_text(NETWORK_H_NET)
+    spec = ora.Network(str(p))
+    E, G = u10.Network(str(p)), u10.Network(str(p))
+    G.setGraph(True)
+    for rep in range(2):
+        rng = np.random.default_rng(60 + rep)
+        for nm, labs, r in zip(spec.names[:-1], spec.label_arr[:-1], spec.Rnums[:-1]):
+            T = ora.UniTensor(network_h_bonds(ora, labs, r), nm)
+            T.set_elem(rng.uniform(-1, 1, T.elem_num()))
+            P = to_product(u10, T)
+            E.putTensor(nm, P)
+            G.putTensor(nm, P)
+        a, b = E.launch(), G.launch()
+        assert a.getElem().tobytes() == b.getElem().tobytes()

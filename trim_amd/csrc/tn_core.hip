@@ -13,6 +13,7 @@
 #include <map>
 #include <mutex>
 #include <unordered_map>
+#include <vector>
 
 #include "tn_internal.h"
 
@@ -40,6 +41,19 @@ struct DeviceState {
   std::multimap<size_t, void*> free_blocks;       // size -> block
   std::unordered_map<void*, size_t> live;         // block -> size
   size_t in_use = 0, cached = 0, peak = 0, budget = 0;
+  // Memory of captured graphs.  A block allocated or freed while a capture
+  // is in progress belongs to that graph: its replays read and write it, so
+  // freeing it only marks it released, and it returns to the pool when the
+  // graph is destroyed.  Within the capture, a block released earlier is
+  // handed to later allocations of the same capture: the graph runs in
+  // stream order, as the captured calls did.
+  struct GraphMem {
+    std::vector<void*> owned;
+    std::multimap<size_t, void*> local_free;      // released in this capture
+  };
+  GraphMem* capturing = nullptr;                  // the capture in progress
+  std::unordered_map<void*, GraphMem*> graph_mem; // graph exec -> its memory
+  std::unordered_map<void*, bool> owned;          // graph-owned block -> released by its holder
 };
 
 static DeviceState& st() {
@@ -111,6 +125,17 @@ int pool_alloc(void** p, size_t bytes) {
   DeviceState& s = st();
   std::lock_guard<std::mutex> g(s.mu);
   size_t sz = bin_size(bytes);
+  if (s.capturing) {
+    auto& lf = s.capturing->local_free;
+    auto it = lf.lower_bound(sz);
+    if (it != lf.end() && it->first <= sz + sz / 4) {
+      void* q = it->second;
+      lf.erase(it);
+      s.owned[q] = false;    // held again (already counted in use)
+      *p = q;
+      return TN_OK;
+    }
+  }
   TN_REQUIRE(s.budget == 0 || s.in_use + sz <= s.budget, TN_ENOMEM,
              "device memory budget exceeded: in use %zu + request %zu > budget %zu",
              s.in_use, sz, s.budget);
@@ -142,8 +167,36 @@ int pool_alloc(void** p, size_t bytes) {
   s.live[q] = sz;
   s.in_use += sz;
   s.peak = std::max(s.peak, s.in_use);
+  if (s.capturing) {
+    s.capturing->owned.push_back(q);
+    s.owned[q] = false;
+  }
   *p = q;
   return TN_OK;
+}
+
+// Returns a live block to its size bin.
+static void free_locked(DeviceState& s, void* p) {
+  auto it = s.live.find(p);
+  if (it == s.live.end()) return;
+  const size_t sz = it->second;
+  s.live.erase(it);
+  s.in_use -= sz;
+  s.free_blocks.emplace(sz, p);
+  s.cached += sz;
+}
+
+// The graph's blocks go back to the pool (those its holders released) or
+// to their holders (the rest, freed normally later).
+static void release_graph_mem_locked(DeviceState& s, DeviceState::GraphMem* m) {
+  for (void* p : m->owned) {
+    auto it = s.owned.find(p);
+    if (it == s.owned.end()) continue;
+    const bool released = it->second;
+    s.owned.erase(it);
+    if (released) free_locked(s, p);
+  }
+  delete m;
 }
 
 int pool_free(void* p) {
@@ -152,11 +205,25 @@ int pool_free(void* p) {
   std::lock_guard<std::mutex> g(s.mu);
   auto it = s.live.find(p);
   TN_REQUIRE(it != s.live.end(), TN_EINVAL, "tn_free: %p was not allocated by tn_malloc", p);
-  size_t sz = it->second;
-  s.live.erase(it);
-  s.in_use -= sz;
-  s.free_blocks.emplace(sz, p);
-  s.cached += sz;
+  auto ow = s.owned.find(p);
+  if (s.capturing) {
+    DeviceState::GraphMem* m = s.capturing;
+    const bool mine = ow != s.owned.end() &&
+                      std::find(m->owned.begin(), m->owned.end(), p) != m->owned.end();
+    if (ow == s.owned.end() || mine) {
+      // the captured calls may still read it: it joins this graph and is
+      // reusable by the capture's later allocations only
+      if (ow == s.owned.end()) m->owned.push_back(p);
+      s.owned[p] = true;
+      m->local_free.emplace(it->second, p);
+      return TN_OK;
+    }
+  }
+  if (ow != s.owned.end()) {
+    ow->second = true;       // a graph still uses it: returned when that graph is destroyed
+    return TN_OK;
+  }
+  free_locked(s, p);
   return TN_OK;
 }
 
@@ -327,22 +394,45 @@ int tn_event_record(void* ev, void* stream) {
 
 int tn_graph_begin(void* stream) {
   TN_INIT_OR_RETURN();
-  TN_HIP_CHECK(hipStreamBeginCapture(stream_of(stream), hipStreamCaptureModeThreadLocal));
+  DeviceState& s = st();
+  {
+    std::lock_guard<std::mutex> g(s.mu);
+    TN_REQUIRE(s.capturing == nullptr, TN_EINVAL, "tn_graph_begin: another capture is in progress");
+    s.capturing = new DeviceState::GraphMem();
+  }
+  // relaxed: a pool miss during the capture may call hipMalloc
+  const hipError_t e = hipStreamBeginCapture(stream_of(stream), hipStreamCaptureModeRelaxed);
+  if (e != hipSuccess) {
+    std::lock_guard<std::mutex> g(s.mu);
+    release_graph_mem_locked(s, s.capturing);
+    s.capturing = nullptr;
+    set_error("hipStreamBeginCapture: %s", hipGetErrorString(e));
+    return TN_EHIP;
+  }
   return TN_OK;
 }
 
 int tn_graph_end(void* stream, void** graph_exec) {
   TN_INIT_OR_RETURN();
-  TN_REQUIRE(graph_exec != nullptr, TN_EINVAL, "tn_graph_end: NULL out pointer");
+  DeviceState& s = st();
   hipGraph_t g = nullptr;
-  TN_HIP_CHECK(hipStreamEndCapture(stream_of(stream), &g));
+  hipError_t e = hipStreamEndCapture(stream_of(stream), &g);
   hipGraphExec_t ge = nullptr;
-  hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(g);
-  if (e != hipSuccess) {
-    set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
+  if (e == hipSuccess && graph_exec) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  std::lock_guard<std::mutex> lk(s.mu);
+  DeviceState::GraphMem* m = s.capturing;
+  s.capturing = nullptr;
+  if (e != hipSuccess || !graph_exec) {
+    if (m) release_graph_mem_locked(s, m);   // nothing captured will ever run
+    if (!graph_exec) {
+      set_error("tn_graph_end: NULL out pointer");
+      return TN_EINVAL;
+    }
+    set_error("graph capture: %s", hipGetErrorString(e));
     return TN_EHIP;
   }
+  if (m) s.graph_mem[(void*)ge] = m;
   *graph_exec = (void*)ge;
   return TN_OK;
 }
@@ -356,7 +446,19 @@ int tn_graph_launch(void* graph_exec, void* stream) {
 
 int tn_graph_destroy(void* graph_exec) {
   if (!graph_exec) return TN_OK;
+  // a replay may still be running: its memory is reused only after it ends
+  const hipError_t e = hipDeviceSynchronize();
   TN_HIP_CHECK(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+  DeviceState& s = st();
+  {
+    std::lock_guard<std::mutex> g(s.mu);
+    auto it = s.graph_mem.find(graph_exec);
+    if (it != s.graph_mem.end()) {
+      release_graph_mem_locked(s, it->second);
+      s.graph_mem.erase(it);
+    }
+  }
+  TN_REQUIRE(e == hipSuccess, TN_EHIP, "hipDeviceSynchronize: %s", hipGetErrorString(e));
   return TN_OK;
 }
 

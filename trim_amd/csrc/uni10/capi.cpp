@@ -336,6 +336,10 @@ int u10_network_tree(u10_network net, char* buf, int len) {
   });
 }
 
+int u10_network_set_graph(u10_network net, int on) {
+  return guard([&] { net->N.setGraph(on != 0); });
+}
+
 int u10_network_print(u10_network net, char* buf, int len) {
   return guard([&] {
     std::ostringstream os;

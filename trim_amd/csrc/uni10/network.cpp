@@ -8,6 +8,7 @@
 #include <fstream>
 #include <functional>
 #include <iostream>
+#include <sstream>
 
 #include "common.h"
 #include "layout.h"
@@ -210,6 +211,7 @@ Network::Network(const std::string& fname, const std::vector<UniTensor*>& tens) 
 }
 
 Network::~Network() {
+  drop_graph();
   for (NetNode* p : owned) delete p;
   for (NetNode* p : leafs) delete p;
   for (UniTensor* t : tensors) delete t;
@@ -266,6 +268,7 @@ void Network::putTensorT(const std::string& nameT, const UniTensor& T, bool forc
 // Network.cpp:679-695: the swap gates already applied are applied again,
 // which undoes them (each sign squares to one).
 void Network::destruct() {
+  drop_graph();
   for (NetNode* p : owned) delete p;
   owned.clear();
   root = nullptr;
@@ -464,8 +467,104 @@ UniTensor Network::merge(NetNode* nd) {
   return contract_nopermute(*nd->left->T, *nd->right->T);
 }
 
+void Network::setGraph(bool on) {
+  use_graph = on;
+  if (!on) drop_graph();
+}
+
+void Network::drop_graph() {
+  delete graph_out;
+  graph_out = nullptr;
+  if (graph) tn_graph_destroy(graph);
+  graph = nullptr;
+  graph_sig.clear();
+}
+
+// merge() over working copies W of the leaves (W[t] for tensors[t]).
+UniTensor Network::merge_on(NetNode* nd, std::vector<UniTensor>& W) {
+  auto operand = [&](NetNode* n, UniTensor& tmp) -> UniTensor& {
+    if (n->T == nullptr) {
+      tmp = merge_on(n, W);
+      return tmp;
+    }
+    const size_t t = std::find(leafs.begin(), leafs.end(), n) - leafs.begin();
+    return W.at(t);
+  };
+  UniTensor lt, rt;
+  UniTensor& a = operand(nd->left, lt);
+  UniTensor& b = operand(nd->right, rt);
+  return contract_nopermute(a, b);
+}
+
+UniTensor Network::run_on_copies() {
+  std::vector<UniTensor> W;
+  W.reserve(tensors.size());
+  for (size_t t = 0; t < tensors.size(); ++t) {
+    W.emplace_back(*tensors[t]);
+    if (Qnum::isFermionic() && !swapflags[t]) W.back().addGate(swaps_arr[t]);
+  }
+  UniTensor T = merge_on(root, W);
+  const std::vector<int>& tout = label_arr.back();
+  if (tout.size() > 1) T.permute(tout, Rnums.back());
+  return T;
+}
+
+UniTensor Network::launch_graph(const std::string& name) {
+  std::ostringstream sig;
+  for (size_t t = 0; t < tensors.size(); ++t) {
+    const UniTensor& T = *tensors[t];
+    sig << (T.layout() ? T.layout()->key : std::string()) << "@" << (const void*)T.getElem() << ":" << T.inBondNum();
+    for (int l : T.label()) sig << "," << l;
+    if (Qnum::isFermionic()) {
+      sig << (swapflags[t] ? "g" : "G");
+      for (const _Swap& w : swaps_arr[t]) sig << w.b1 << "-" << w.b2 << ";";
+    }
+    sig << "|";
+  }
+  const std::string key = sig.str();
+  if (key == graph_failed_sig) {   // this leaf state could not be captured: run it directly
+    UniTensor T = run_on_copies();
+    T.setName(name);
+    return T;
+  }
+  if (!graph || key != graph_sig) {
+    drop_graph();
+    {
+      // every plan is built (and tuned) here: host uploads cannot be captured
+      UniTensor warm = run_on_copies();
+    }
+    detail::check(tn_stream_sync(detail::stream()), "Network graph");
+    detail::check(tn_graph_begin(detail::stream()), "Network graph");
+    UniTensor* out = nullptr;
+    try {
+      out = new UniTensor(run_on_copies());
+    } catch (const std::exception&) {
+      void* g = nullptr;
+      tn_graph_end(detail::stream(), &g);   // ends (and discards) the capture
+      if (g) tn_graph_destroy(g);
+      graph_failed_sig = key;
+      UniTensor T = run_on_copies();
+      T.setName(name);
+      return T;
+    }
+    const int rc = tn_graph_end(detail::stream(), &graph);
+    if (rc != TN_OK) {
+      delete out;
+      graph = nullptr;
+      detail::check(rc, "Network graph");
+    }
+    graph_out = out;
+    graph_sig = key;
+  }
+  detail::check(tn_graph_launch(graph, detail::stream()), "Network graph replay");
+  UniTensor T(*graph_out);   // the next replay rewrites the graph's output
+  T.setName(name);
+  return T;
+}
+
 UniTensor Network::launch(const std::string& name) {
   if (!load) construct();
+  if (use_graph) return launch_graph(name);
   for (size_t t = 0; t < tensors.size(); ++t)
     if (Qnum::isFermionic() && !swapflags[t]) {
       tensors[t]->addGate(swaps_arr[t]);

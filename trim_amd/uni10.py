@@ -71,6 +71,7 @@ SIGNATURES = {
     "u10_tensor_ex_swap": [P, P, P, I, PI],
     "u10_tensor_add_gate": [P, I, P],
     "u10_network_print": [P, ctypes.c_char_p, I],
+    "u10_network_set_graph": [P, I],
     "u10_network_create": [S, PP],
     "u10_network_put": [P, S, P, I],
     "u10_network_launch": [P, S, PP],
@@ -432,6 +433,9 @@ class Network:
         h = ctypes.c_void_p()
         _call("u10_network_launch", self.h, name.encode(), ctypes.byref(h))
         return UniTensor(_handle=h.value)
+
+    def setGraph(self, on: bool = True):
+        _call("u10_network_set_graph", self.h, 1 if on else 0)
 
     def tree(self) -> str:
         buf = ctypes.create_string_buffer(4096)
