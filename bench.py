@@ -292,8 +292,9 @@ def c3_u1(tn, u10, steps=10):
     return res
 
 
-def tebd_rate(u10, steps, chi=32):
+def tebd_rate(u10, steps, chi=32, graph=False):
     t = u10.ITebd(chi, 2, 1.0, 0.01, 7, NETDIR)
+    t.set_graph(graph)
     t.step(5)
     t0 = time.perf_counter()
     t.step(steps)
@@ -626,6 +627,8 @@ def main():
         return
     tebd_sps, tebd_e, tebd_te = tebd_rate(uni10, args.tebd_steps)
     progress("iTEBD chi=32: %.1f steps/s" % tebd_sps)
+    tebd_g_sps, tebd_g_e, _ = tebd_rate(uni10, args.tebd_steps, graph=True)
+    progress("iTEBD chi=32, networks replayed as graphs: %.1f steps/s" % tebd_g_sps)
     c3 = c3_u1(tn, uni10)
     progress("U(1) chi=1024 contraction modes done")
     lz = lanczos_leg(tn)
@@ -667,7 +670,8 @@ def main():
         "copy_GBps_same_bytes": copy_gbps,
         "permute_frac_of_copy": hbm_gbps / copy_gbps,
         "tebd_chi32": {"steps_per_s": tebd_sps, "steps": args.tebd_steps, "energy_per_site": tebd_e,
-                       "trunc_err_last": tebd_te},
+                       "trunc_err_last": tebd_te, "steps_per_s_network_graphs": tebd_g_sps,
+                       "energy_per_site_network_graphs": tebd_g_e},
         "c3_u1_chi1024": c3,
         "lanczos": lz,
         "c4_batched_itebd": c4,

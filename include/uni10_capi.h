@@ -111,6 +111,8 @@ int u10_lanczos_eigh(const double* A, int64_t N, double* psi, uint64_t max_iter,
 int u10_itebd_create(int chi, int d, double delta, double tau, uint64_t seed, const char* netdir,
                      u10_itebd* out);
 int u10_itebd_step(u10_itebd t, int nsteps);
+/* replay the driver's update networks as captured graphs (Network::setGraph) */
+int u10_itebd_set_graph(u10_itebd t, int on);
 int u10_itebd_energy(u10_itebd t, double* out);
 int u10_itebd_lambda(u10_itebd t, int which, double* out, int* n);
 int u10_itebd_trunc_err(u10_itebd t, double* out);

@@ -414,6 +414,10 @@ int u10_itebd_create(int chi, int d, double delta, double tau, uint64_t seed, co
   return guard([&] { *out = new u10_itebd_s(chi, d, delta, tau, seed, std::string(netdir ? netdir : "")); });
 }
 
+int u10_itebd_set_graph(u10_itebd t, int on) {
+  return guard([&] { t->E.set_graph(on != 0); });
+}
+
 int u10_itebd_step(u10_itebd t, int n) {
   return guard([&] {
     for (int i = 0; i < n; ++i) t->E.step();

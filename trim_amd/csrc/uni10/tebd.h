@@ -22,6 +22,11 @@ class ITebd {
   void step();                 // update(A,B) then update(B,A)
   double energy();             // mean of the two bond energies
   double truncation_error() const;   // discarded weight of the last update
+  // replay the update networks as captured graphs (Network::setGraph)
+  void set_graph(bool on) {
+    iTEBD_.setGraph(on);
+    updateA_.setGraph(on);
+  }
   const Matrix& La() const { return La_; }
   const Matrix& Lb() const { return Lb_; }
   UniTensor& ALa() { return ALa_; }

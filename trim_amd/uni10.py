@@ -85,6 +85,7 @@ SIGNATURES = {
     "u10_exph": [D, P, I64, P],
     "u10_itebd_create": [I, I, D, D, ctypes.c_uint64, S, PP],
     "u10_itebd_step": [P, I],
+    "u10_itebd_set_graph": [P, I],
     "u10_itebd_energy": [P, PD],
     "u10_itebd_lambda": [P, I, P, PI],
     "u10_itebd_trunc_err": [P, PD],
@@ -627,6 +628,9 @@ class ITebd:
 
     def step(self, n: int = 1):
         _call("u10_itebd_step", self.h, int(n))
+
+    def set_graph(self, on: bool = True):
+        _call("u10_itebd_set_graph", self.h, 1 if on else 0)
 
     def energy(self) -> float:
         v = ctypes.c_double()
