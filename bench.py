@@ -261,7 +261,7 @@ def c3_u1(tn, u10, steps=10):
     A1, A2, G, dims = workloads.u1_two_site(u10, 1024, 8, 11, 0.01)
     res = {"sector_dims": dims}
     ref = None
-    modes = (("f64", 0), ("f32", 1), ("bf16x3", 2), ("bf16x6", 3))
+    modes = (("f64", 0), ("f32", 1), ("bf16x3", 2), ("bf16x6", 3), ("bf16_exact", 4))
     for name, mode in modes:
         u10.set_gemm_mode(mode)
         R = u10.contract(u10.contract(A1, A2), G)      # plans + autotune
@@ -356,7 +356,7 @@ def mixed_precision_energy(u10, checkpoints=(20, 2000)):
     chi, delta, seed = [8, 16, 24, 32], [0.5, 1.0, 1.5, 1.0], [21, 22, 23, 24]
     e = {c: {} for c in checkpoints}
     last_move = {}
-    for mode, name in ((0, "f64"), (1, "f32"), (2, "bf16x3"), (3, "bf16x6")):
+    for mode, name in ((0, "f64"), (1, "f32"), (2, "bf16x3"), (3, "bf16x6"), (4, "bf16_exact")):
         B = u10.BatchTebd(chi, delta, seed, 0.01, NETDIR, mode)
         B.set_graph(True)
         done = 0
@@ -393,7 +393,7 @@ def c5_serve(u10, n_req, rate, chis=(32, 64, 128), mixed=True):
         classes = [(c, 2, 0) for c in chis]
         gen = {}
     out, res = {}, {}
-    for mode, name in ((0, "f64"), (1, "f32")):
+    for mode, name in ((0, "f64"), (1, "f32"), (4, "bf16_exact")):
         srv = serve.Server(classes, 16e9, NETDIR, gemm_mode=mode, max_slots=64 if mixed else 256)
         srv.warmup()
         reqs = serve.poisson_requests(n_req, rate, seed=5, chi_classes=list(chis), **gen)
@@ -406,6 +406,8 @@ def c5_serve(u10, n_req, rate, chis=(32, 64, 128), mixed=True):
         progress("serving %s: p99 %.1f ms" % (name, out[name]["p99_ms"]))
     diff = max(float(np.max(np.abs(res["f32"][k] - res["f64"][k]))) for k in res["f64"])
     out["max_result_diff_f32_vs_f64"] = diff
+    out["max_result_diff_bf16_exact_vs_f64"] = max(float(np.max(np.abs(res["bf16_exact"][k] - res["f64"][k])))
+                                                   for k in res["f64"])
     out["workload"] = ("Poisson %.0f req/s, %d requests, chi log-uniform over %s, %s, 1..50 steps, half TEBD "
                        "half contraction" % (rate, n_req, list(chis),
                                              "d in {2,3}, 60% U(1) (charges -12..12)" if mixed else "dense spin 1/2"))
@@ -650,6 +652,8 @@ def main():
         progress("spin-1/2 dense serving done")
         c5["large_chi_single_chain_steps"] = large_chi_steps(uni10)
         c5["speedup_f32_serving_vs_f64_dropin"] = c5["f32"]["tebd_steps_per_s"] / tebd_sps
+        c5["speedup_bf16_exact_serving_vs_f64_dropin"] = c5["bf16_exact"]["tebd_steps_per_s"] / tebd_sps
+        c5["speedup_f64_serving_vs_f64_dropin"] = c5["f64"]["tebd_steps_per_s"] / tebd_sps
         c5["spin_half_dense_chi_16_32"]["speedup_f32_serving_vs_f64_dropin"] = \
             c5["spin_half_dense_chi_16_32"]["f32"]["tebd_steps_per_s"] / tebd_sps
     out = {

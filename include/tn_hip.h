@@ -139,6 +139,12 @@ int tn_transpose_f64(const double* A, int64_t M, int64_t N, double* AT, void* st
 #define TN_GEMM_F32 1      /* FP32 MFMA on FP64->FP32 rounded inputs, FP32 acc. */
 #define TN_GEMM_BF16X3 2   /* split-BF16 (hi+lo), 3 products, FP32 accumulate    */
 #define TN_GEMM_BF16X6 3   /* split-BF16 (hi+mid+lo), 6 products, FP32 accumulate*/
+/* FP64-grade split-BF16: each A row / B column scaled by a power of two and
+ * cut into six 7-bit integer slices (exact in BF16); the 21 slice products of
+ * total weight >= 2^-49 on BF16 MFMA, whose FP32 sums stay exact integers
+ * and are moved to FP64 every 128 k; error about K 2^-49 relative to
+ * max|A row| max|B column|.  Needs finite inputs. */
+#define TN_GEMM_BF16_EXACT 4
 typedef struct {
   int64_t M, N, K;
   int64_t a_off, b_off, c_off;

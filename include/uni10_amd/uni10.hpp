@@ -32,7 +32,7 @@ enum parityType { PRT_EVEN = 0, PRT_ODD = 1 };
 enum parityFType { PRTF_EVEN = 0, PRTF_ODD = 1 };
 
 /* GEMM arithmetic for contractions (storage stays FP64). */
-enum GemmMode { GEMM_F64 = 0, GEMM_F32 = 1, GEMM_BF16X3 = 2, GEMM_BF16X6 = 3 };
+enum GemmMode { GEMM_F64 = 0, GEMM_F32 = 1, GEMM_BF16X3 = 2, GEMM_BF16X6 = 3, GEMM_BF16_EXACT = 4 };
 void setGemmMode(GemmMode m);          // thread-local; default GEMM_F64
 GemmMode gemmMode();
 void setAutotune(bool on);             // thread-local; time tile shapes on first use (default on)

@@ -183,7 +183,7 @@ def test_block_permute_bitexact(tn, ora, perm, rb):
 
 # ---------------------------------------------------------------------------
 # grouped GEMM
-GEMM_TOL = {0: 1e-12, 1: 2e-6, 2: 5e-5, 3: 2e-6}
+GEMM_TOL = {0: 1e-12, 1: 2e-6, 2: 5e-5, 3: 2e-6, 4: 1e-11}   # relative Frobenius vs dgemm
 
 
 def _group_case(seed, shapes):
@@ -221,13 +221,16 @@ def _check_gemm(tn, mode, shapes, seed=0, tile=None):
         assert err <= GEMM_TOL[mode], (mode, M, N, K, err)
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
 def test_gemm_single_512(tn, mode):
     _say("gemm 512^3 mode=%d" % mode)
     _check_gemm(tn, mode, [(512, 512, 512)])
 
 
-@pytest.mark.parametrize("mode,tile", [(m, t) for m in (0, 1, 2, 3) for t in range(8 if m < 2 else 5)])
+NTILES = {0: 8, 1: 8, 2: 5, 3: 5, 4: 4}
+
+
+@pytest.mark.parametrize("mode,tile", [(m, t) for m in (0, 1, 2, 3, 4) for t in range(NTILES[m])])
 def test_gemm_grouped_uneven(tn, mode, tile):
     shapes = [(1, 1, 1), (17, 33, 5), (64, 64, 64), (130, 70, 200), (3, 200, 1), (200, 3, 300),
               (0, 5, 5), (5, 0, 5), (5, 5, 0), (96, 128, 17)]
@@ -235,7 +238,7 @@ def test_gemm_grouped_uneven(tn, mode, tile):
     _check_gemm(tn, mode, shapes, seed=mode * 10 + tile, tile=tile)
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
 def test_gemm_tile_choice_is_bitwise_neutral(tn, mode):
     """Every tile shape (and so the autotuner's choice) gives identical bits:
     each output element accumulates K in the same order."""
@@ -244,7 +247,7 @@ def test_gemm_tile_choice_is_bitwise_neutral(tn, mode):
     plan = tn.GemmPlan(mode, np.array(descs, dtype=tn.GEMM_DESC))
     dA, dB = tn.DeviceArray.from_host(A), tn.DeviceArray.from_host(B)
     outs = []
-    for tile in range(8 if mode < 2 else 5):   # FP64/FP32 add three 512-thread tiles
+    for tile in range(NTILES[mode]):   # FP64/FP32 add three 512-thread tiles
         plan.set_tile(tile)
         dC = tn.DeviceArray((nc,))
         plan(dA, dB, dC)
@@ -255,7 +258,7 @@ def test_gemm_tile_choice_is_bitwise_neutral(tn, mode):
     assert ms > 0 and dC.to_host().tobytes() == outs[0]
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
 def test_gemm_index_maps(tn, mode):
     """Multi-stride operands equal the same GEMM on explicitly permuted
     copies: A read as the (0,2)x(1,3) matricisation of a rank-4 tensor, B
@@ -980,3 +983,37 @@ def test_budget_exhaustion_fails_cleanly_and_plans_stay_usable(tn):
     plan.autotune(dA, dB, dC, reps=1)
     plan(dA, dB, dC)
     assert np.linalg.norm(dC.to_host() - a @ b) <= 1e-13 * np.linalg.norm(a @ b)
+
+
+def test_gemm_exact_mode_is_exact_on_fixed_point_inputs(tn):
+    """The exact split-BF16 mode on inputs that its first slices hold exactly
+    (integers up to 1000 times 2^-10: 11 significant bits, two 7-bit slices
+    after the row / column scaling): every slice product and sum is exact,
+    so the result equals numpy's FP64 product (itself exact here: integer
+    sums far below 2^53) bit for bit, for every tile shape, including uneven
+    and K > 128 (several FP64 flushes) groups."""
+    rng = np.random.default_rng(21)
+    shapes = [(37, 29, 300), (64, 64, 64), (5, 130, 7), (100, 3, 513)]
+    descs, A, B, co = [], [], [], 0
+    ao = bo = 0
+    for (M, N, K) in shapes:
+        a = rng.integers(-1000, 1001, (M, K)) * 2.0 ** -10
+        b = rng.integers(-1000, 1001, (K, N)) * 2.0 ** -10
+        descs.append((M, N, K, ao, bo, co, K, N, N))
+        A.append(a.reshape(-1))
+        B.append(b.reshape(-1))
+        ao += M * K
+        bo += K * N
+        co += M * N
+    A, B = np.concatenate(A), np.concatenate(B)
+    plan = tn.GemmPlan(tn.TN_GEMM_BF16_EXACT, np.array(descs, dtype=tn.GEMM_DESC))
+    dA, dB = tn.DeviceArray.from_host(A), tn.DeviceArray.from_host(B)
+    for tile in range(NTILES[4]):
+        _say("exact mode fixed point, tile %d" % tile)
+        plan.set_tile(tile)
+        dC = tn.DeviceArray((co,))
+        plan(dA, dB, dC)
+        C = dC.to_host()
+        for (M, N, K, a0, b0, c0, *_r) in descs:
+            ref = A[a0:a0 + M * K].reshape(M, K) @ B[b0:b0 + K * N].reshape(K, N)
+            assert C[c0:c0 + M * N].tobytes() == ref.reshape(-1).tobytes()

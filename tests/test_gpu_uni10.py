@@ -268,11 +268,11 @@ def test_network_greedy_plan_matches_oracle(u10, ora, golden):
     assert np.max(np.abs(Cp2.getElem() - Co.elem)) <= 1e-14
 
 
-MODE_TOL = {0: 1e-12, 1: 2e-6, 2: 5e-5, 3: 2e-6}
+MODE_TOL = {0: 1e-12, 1: 2e-6, 2: 5e-5, 3: 2e-6, 4: 1e-11}
 
 
 @pytest.mark.parametrize("size", [(96, 4, 21), (1024, 8, 11)], ids=["chi96", "chi1024"])
-@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
 def test_u1_two_site_contraction_modes(u10, ora, mode, size):
     """Theta = (A1 . A2) . G for U(1) tensors with binomial sectors, at chi=96
     over Sz in {-4..4} and at the configured chi=1024 over Sz in {-8..8}:
@@ -735,3 +735,19 @@ def test_network_graph_replay_u1_network(u10, ora, tmp_path):
             G.putTensor(nm, P)
         a, b = E.launch(), G.launch()
         assert a.getElem().tobytes() == b.getElem().tobytes()
+
+
+def test_batch_tebd_exact_bf16_mode_is_fp64_grade(u10, golden):
+    """Batched iTEBD with the exact split-BF16 sector GEMMs against the same
+    batch in FP64: energies per site within 1e-10 after 200 steps (the
+    FP32 and split-BF16 forms with FP32 sums differ by ~1e-7 here)."""
+    chi, delta, seed = [8, 16, 24, 32], [0.5, 1.0, 1.5, 1.0], [21, 22, 23, 24]
+    e = {}
+    for mode in (0, 4):
+        _say("batched TEBD mode %d" % mode)
+        B = u10.BatchTebd(chi, delta, seed, 0.01, golden, mode)
+        B.set_graph(True)
+        B.step(200)
+        e[mode] = np.array([B.energy(i) for i in range(len(chi))])
+    _say("exact vs f64 energies: %s" % np.abs(e[4] - e[0]))
+    assert np.max(np.abs(e[4] - e[0])) <= 1e-10

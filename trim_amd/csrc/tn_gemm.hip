@@ -73,6 +73,7 @@ struct GroupDev {
   int64_t a_off, b_off, c_off;
   int64_t lda, ldb, ldc;
   MapDev a_row, a_col, b_row, b_col, c_row, c_col;
+  int64_t ea_off, eb_off;     // exact mode: offsets of the A-row / B-column scale exponents
 };
 
 __device__ __forceinline__ int64_t map_off(const MapDev& m, int64_t i64) {
@@ -515,6 +516,244 @@ __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict_
 }
 
 // ----------------------------------------------------------------------------
+// Exact split-BF16 mode (TN_GEMM_BF16_EXACT): FP64-grade products from BF16
+// MFMAs with no rounding in the slice products or their sums (fixed-point
+// slicing after K. Ozaki, T. Ogita, S. Oishi, S. M. Rump, Numer. Algorithms
+// 59 (2012); integer slices on low-precision matrix units as in H. Ootomo,
+// K. Ozaki, R. Yokota, IJHPCA (2024)).
+//   * Row m of A is scaled by 2^-ea[m] and column n of B by 2^-eb[n], with
+//     ea, eb the exponents that bring the row / column maximum into
+//     [1/2, 1) (exact: powers of two).
+//   * A scaled value x in (-1, 1) is cut into kExactSlices integer slices of
+//     7 bits: y = 128 x, q = rint(y), x = y - q, repeated; q_1 is in
+//     [-128, 128], later slices in [-64, 64], all exact in BF16 (8
+//     significant bits), and x = sum_s q_s 2^-7s + r with |r| <= 2^-7S / 2.
+//   * The slice products q_s(A) q_t(B) with s + t <= S + 1 are accumulated
+//     per diagonal d = s + t on v_mfma_f32_16x16x32_bf16: every product is
+//     an integer of magnitude at most 2^14 and a diagonal gains at most
+//     6 * 32 * 2^14 per 32 k, so after 128 k its FP32 sum is still an integer
+//     below 2^24, i.e. exact; it is then added to an FP64 accumulator (exact
+//     integers below 2^53) and cleared.
+//   * C(m, n) = 2^(ea[m] + eb[n]) sum_d 2^-7d acc_d, smallest terms first.
+// The error is the dropped slice products (s + t > S + 1) and remainders:
+// relative to max_k |A(m, k)| max_k |B(k, n)|, at most about K 2^-7(S+1).
+constexpr int kExactSlices = 6;
+
+// Scale exponents (blockIdx.y = group): the first ceil(M/4) blocks take one
+// wave per A row (lanes stride over k, then a butterfly max), the rest 64 B
+// columns each (lanes over adjacent columns, coalesced; the four waves split
+// k).  A row m at ea_off + m, B column n at eb_off + n.
+__global__ void __launch_bounds__(256) exact_scales_kernel(const double* __restrict__ A, const double* __restrict__ B,
+                                                          const GroupDev* __restrict__ groups,
+                                                          int32_t* __restrict__ ex) {
+  const GroupDev& g = groups[blockIdx.y];
+  if (g.K == 0) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t rblocks = (g.M + 3) / 4;
+  if ((int64_t)blockIdx.x < rblocks) {
+    const int64_t m = (int64_t)blockIdx.x * 4 + wave;
+    if (m >= g.M) return;                       // the whole wave
+    const int64_t ro = map_off(g.a_row, m);
+    double mx = 0.0;
+    for (int64_t k = lane; k < g.K; k += 64) mx = fmax(mx, fabs(A[g.a_off + ro + map_off(g.a_col, k)]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    if (lane == 0) ex[g.ea_off + m] = (mx > 0.0) ? ilogb(mx) + 1 : 0;
+  } else {
+    // 64 columns per block; wave w takes k = w, w + 4, ... four loads at a time
+    __shared__ double part[4][64];
+    const int64_t n = ((int64_t)blockIdx.x - rblocks) * 64 + lane;
+    const bool live = n < g.N;
+    const int64_t co = live ? map_off(g.b_col, n) : 0;
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+    int64_t k = wave;
+    for (; k + 12 < g.K; k += 16) {
+      if (live) {
+        m0 = fmax(m0, fabs(B[g.b_off + map_off(g.b_row, k) + co]));
+        m1 = fmax(m1, fabs(B[g.b_off + map_off(g.b_row, k + 4) + co]));
+        m2 = fmax(m2, fabs(B[g.b_off + map_off(g.b_row, k + 8) + co]));
+        m3 = fmax(m3, fabs(B[g.b_off + map_off(g.b_row, k + 12) + co]));
+      }
+    }
+    for (; k < g.K; k += 4)
+      if (live) m0 = fmax(m0, fabs(B[g.b_off + map_off(g.b_row, k) + co]));
+    part[wave][lane] = fmax(fmax(m0, m1), fmax(m2, m3));
+    __syncthreads();
+    if (wave == 0 && live) {
+      const double mx = fmax(fmax(part[0][lane], part[1][lane]), fmax(part[2][lane], part[3][lane]));
+      ex[g.eb_off + n] = (mx > 0.0) ? ilogb(mx) + 1 : 0;
+    }
+  }
+}
+
+template <int BM, int BN, int BK>
+__global__ void __launch_bounds__(256) gemm_exact_bf16(const double* __restrict__ A, const double* __restrict__ B,
+                                                       double* __restrict__ C, const GroupDev* __restrict__ groups,
+                                                       const TileDev* __restrict__ tiles,
+                                                       const int32_t* __restrict__ ex) {
+  constexpr int S = kExactSlices;
+  static_assert(BK % 32 == 0 && 256 % BK == 0 && 256 % BN == 0, "tile shape");
+  constexpr int KP = BK + 8;                       // padded row (bf16 elements)
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  constexpr int A_PER = BM * BK / 256;
+  constexpr int B_PER = BN * BK / 256;
+  constexpr int A_RSTEP = 256 / BK;
+  constexpr int B_KSTEP = 256 / BN;
+  constexpr int kFlush = 4;                        // 32-k steps between FP64 flushes (128 k)
+  __shared__ __attribute__((aligned(16))) __bf16 As[S][BM][KP];    // [slice][m][k]
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[S][BN][KP];    // [slice][n][k]
+
+  const TileDev tile = tiles[blockIdx.x];
+  if (tile.group < 0) return;
+  const GroupDev& g = groups[tile.group];
+  const int64_t M = g.M, N = g.N, K = g.K;
+  const int64_t m0 = (int64_t)tile.tm * BM, n0 = (int64_t)tile.tn * BN;
+  const double* __restrict__ Ab = A + g.a_off;
+  const double* __restrict__ Bb = B + g.b_off;
+  double* __restrict__ Cb = C + g.c_off;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
+
+  const int a_kk = tid % BK;
+  const int a_mm0 = tid / BK;
+  int64_t a_roff[A_PER];
+  int a_e[A_PER];
+#pragma unroll
+  for (int i = 0; i < A_PER; ++i) {
+    const int64_t m = m0 + a_mm0 + i * A_RSTEP;
+    a_roff[i] = (m < M) ? map_off(g.a_row, m) : -1;
+    a_e[i] = (m < M) ? ex[g.ea_off + m] : 0;
+  }
+  const int b_nn = tid % BN;
+  const int b_kk0 = tid / BN;
+  const int64_t b_coff = (n0 + b_nn < N) ? map_off(g.b_col, n0 + b_nn) : -1;
+  const int b_e = (n0 + b_nn < N) ? ex[g.eb_off + n0 + b_nn] : 0;
+
+  f4 acc[S][TM][TN];
+  double dacc[S][TM][TN][4];
+#pragma unroll
+  for (int d = 0; d < S; ++d)
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        acc[d][i][j] = f4{0, 0, 0, 0};
+#pragma unroll
+        for (int v = 0; v < 4; ++v) dacc[d][i][j][v] = 0.0;
+      }
+  auto flush = [&]() {
+#pragma unroll
+    for (int d = 0; d < S; ++d)
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) dacc[d][i][j][v] += (double)acc[d][i][j][v];
+          acc[d][i][j] = f4{0, 0, 0, 0};
+        }
+  };
+
+  double ra[A_PER], rb[B_PER];
+  auto load_tiles = [&](int64_t k0) {
+    const int64_t ka = k0 + a_kk;
+    const int64_t a_coff = (ka < K) ? map_off(g.a_col, ka) : -1;
+#pragma unroll
+    for (int i = 0; i < A_PER; ++i) ra[i] = (a_roff[i] >= 0 && a_coff >= 0) ? Ab[a_roff[i] + a_coff] : 0.0;
+#pragma unroll
+    for (int i = 0; i < B_PER; ++i) {
+      const int64_t kb = k0 + b_kk0 + i * B_KSTEP;
+      const int64_t b_roff = (kb < K) ? map_off(g.b_row, kb) : -1;
+      rb[i] = (b_roff >= 0 && b_coff >= 0) ? Bb[b_roff + b_coff] : 0.0;
+    }
+  };
+  auto store_tiles = [&]() {
+#pragma unroll
+    for (int i = 0; i < A_PER; ++i) {
+      double x = ldexp(ra[i], -a_e[i]);
+      const int mm = a_mm0 + i * A_RSTEP;
+#pragma unroll
+      for (int sl = 0; sl < S; ++sl) {
+        const double y = 128.0 * x, q = rint(y);
+        x = y - q;
+        As[sl][mm][a_kk] = (__bf16)(float)q;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < B_PER; ++i) {
+      double x = ldexp(rb[i], -b_e);
+      const int kk = b_kk0 + i * B_KSTEP;
+#pragma unroll
+      for (int sl = 0; sl < S; ++sl) {
+        const double y = 128.0 * x, q = rint(y);
+        x = y - q;
+        Bs[sl][b_nn][kk] = (__bf16)(float)q;
+      }
+    }
+  };
+
+  const int64_t nk = (K + BK - 1) / BK;
+  int steps = 0;
+  load_tiles(0);
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    __syncthreads();
+    store_tiles();
+    __syncthreads();
+    if (kt + 1 < nk) load_tiles((kt + 1) * BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 32) {
+      const int kq = ks + 8 * (lane >> 4);
+      bf16x8 a[S][TM], b[S][TN];
+#pragma unroll
+      for (int sl = 0; sl < S; ++sl) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+          a[sl][i] = *reinterpret_cast<const bf16x8*>(&As[sl][wm0 + i * 16 + (lane & 15)][kq]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          b[sl][j] = *reinterpret_cast<const bf16x8*>(&Bs[sl][wn0 + j * 16 + (lane & 15)][kq]);
+      }
+#pragma unroll
+      for (int d = 0; d < S; ++d)
+#pragma unroll
+        for (int sa = 0; sa <= d; ++sa)
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+              acc[d][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[sa][i], b[d - sa][j], acc[d][i][j], 0, 0, 0);
+      if (++steps == kFlush) {
+        flush();
+        steps = 0;
+      }
+    }
+  }
+  flush();
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int64_t gn = n0 + wn0 + j * 16 + (lane & 15);
+    if (gn >= N) continue;
+    const int64_t coff = map_off(g.c_col, gn);
+    const int en = ex[g.eb_off + gn];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t gm = m0 + wm0 + i * 16 + 4 * (lane >> 4) + v;
+        if (gm >= M) continue;
+        double val = 0.0;
+#pragma unroll
+        for (int d = S - 1; d >= 0; --d) val += ldexp(dacc[d][i][j][v], -7 * (d + 2));
+        Cb[map_off(g.c_row, gm) + coff] = ldexp(val, ex[g.ea_off + gm] + en);
+      }
+  }
+}
+
+// ----------------------------------------------------------------------------
 // Tile configurations (autotuner ids), per arithmetic mode.
 struct TileCfg {
   int bm, bn;
@@ -528,11 +767,27 @@ constexpr TileCfg kCfgMfma[kMaxCfg] = {{32, 32}, {64, 64}, {128, 64}, {64, 128},
 constexpr TileCfg kCfgBf16[kMaxCfg] = {{32, 32}, {64, 64}, {128, 64}, {64, 128}, {128, 128}, {128, 128}, {128, 128},
                                        {128, 128}};
 
+// exact split-BF16: 32x32 at 64 and 32 k per stage, 64x32, 32x64 (the
+// slices of six operands and the FP64 accumulators bound the tile size)
+constexpr TileCfg kCfgExact[kMaxCfg] = {{32, 32}, {32, 32}, {64, 32}, {32, 64}, {32, 32}, {32, 32}, {32, 32}, {32, 32}};
 bool mfma_mode(int mode) { return mode == TN_GEMM_F64 || mode == TN_GEMM_F32; }
-int num_cfg(int mode) { return mfma_mode(mode) ? 8 : 5; }
+int num_cfg(int mode) { return mfma_mode(mode) ? 8 : (mode == TN_GEMM_BF16_EXACT ? 4 : 5); }
 int cfg_threads(int mode, int cfg) { return (mfma_mode(mode) && cfg >= 5) ? 512 : 256; }
 
-TileCfg cfg_shape(int mode, int cfg) { return mfma_mode(mode) ? kCfgMfma[cfg] : kCfgBf16[cfg]; }
+TileCfg cfg_shape(int mode, int cfg) {
+  return mfma_mode(mode) ? kCfgMfma[cfg] : (mode == TN_GEMM_BF16_EXACT ? kCfgExact[cfg] : kCfgBf16[cfg]);
+}
+
+typedef void (*ExactFn)(const double*, const double*, double*, const GroupDev*, const TileDev*, const int32_t*);
+ExactFn pick_exact(int cfg) {
+  switch (cfg) {
+    case 0: return gemm_exact_bf16<32, 32, 64>;
+    case 1: return gemm_exact_bf16<32, 32, 32>;
+    case 2: return gemm_exact_bf16<64, 32, 32>;
+    case 3: return gemm_exact_bf16<32, 64, 32>;
+  }
+  return nullptr;
+}
 
 typedef void (*KernFn)(const double*, const double*, double*, const GroupDev*, const TileDev*);
 
@@ -671,6 +926,8 @@ struct tn_gemm_plan_s {
   TileSet tiles[kMaxCfg];
   std::vector<GroupDev> groups;
   GroupDev* d_groups = nullptr;
+  int32_t* d_exp = nullptr;   // exact mode: scale exponents (A rows, then B columns, per group)
+  int64_t max_rows = 0;       // exact mode: blocks of the scale kernel for the largest group
   double flops = 0;
   bool vec_ok = true;     // every group admits 16-byte pair loads
   int64_t max_k = 0;      // longest K over the groups
@@ -772,6 +1029,7 @@ static int default_cfg(const tn_gemm_plan_s* P) {
 
 static void destroy_plan(tn_gemm_plan_s* P) {
   tn::pool_free(P->d_groups);
+  tn::pool_free(P->d_exp);
   for (TileSet& t : P->tiles) tn::pool_free(t.d_tiles);
   delete P;
 }
@@ -783,7 +1041,7 @@ extern "C" {
 int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups, const tn_gemm_desc_ex* d) {
   TN_INIT_OR_RETURN();
   TN_REQUIRE(plan != nullptr, TN_EINVAL, "tn_gemm_plan_create: NULL out pointer");
-  TN_REQUIRE(mode >= TN_GEMM_F64 && mode <= TN_GEMM_BF16X6, TN_EINVAL, "unknown GEMM mode %d", mode);
+  TN_REQUIRE(mode >= TN_GEMM_F64 && mode <= TN_GEMM_BF16_EXACT, TN_EINVAL, "unknown GEMM mode %d", mode);
   TN_REQUIRE(ngroups >= 0 && (ngroups == 0 || d), TN_EINVAL, "tn_gemm_plan_create: NULL descs");
   auto* P = new tn_gemm_plan_s();
   P->mode = mode;
@@ -813,6 +1071,11 @@ int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups, const 
       tn::set_error("GEMM group %lld has an invalid shape, leading dimension or index map", (long long)i);
       return TN_EINVAL;
     }
+    int64_t nexp = 0;
+    for (const GroupDev& h : P->groups) nexp += h.M + h.N;
+    G.ea_off = nexp;
+    G.eb_off = nexp + x.M;
+    P->max_rows = std::max(P->max_rows, (x.M + 3) / 4 + (x.N + 63) / 64);   // exact mode: scale blocks
     P->groups.push_back(G);
     P->flops += 2.0 * (double)x.M * (double)x.N * (double)x.K;
     if (x.M > 0 && x.N > 0 && x.K > 0 && !desc_vec_ok(x)) P->vec_ok = false;
@@ -827,6 +1090,10 @@ int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups, const 
     rc = tn::pool_alloc((void**)&P->d_groups, sizeof(GroupDev) * P->groups.size());
     if (rc == TN_OK)
       rc = tn_memcpy_h2d(P->d_groups, P->groups.data(), sizeof(GroupDev) * P->groups.size(), nullptr);
+    if (rc == TN_OK && mode == TN_GEMM_BF16_EXACT) {
+      const GroupDev& last = P->groups.back();
+      rc = tn::pool_alloc((void**)&P->d_exp, sizeof(int32_t) * std::max<int64_t>(1, last.eb_off + last.N));
+    }
   }
   if (rc == TN_OK) rc = select_cfg(P, default_cfg(P));
   if (rc != TN_OK) {
@@ -944,6 +1211,17 @@ static int exec_cfg(tn_gemm_plan_s* P, int cfg, const double* A, const double* B
   }
   if (t.nslots == 0) return TN_OK;
   TN_REQUIRE(A && B && C, TN_EINVAL, "tn_gemm_exec: NULL operand");
+  if (P->mode == TN_GEMM_BF16_EXACT) {
+    ExactFn k = pick_exact(cfg);
+    TN_REQUIRE(k != nullptr, TN_EINVAL, "no exact kernel for tile %d", cfg);
+    const unsigned gx = (unsigned)P->max_rows;
+    hipLaunchKernelGGL(exact_scales_kernel, dim3(std::max(1u, gx), (unsigned)P->groups.size()), dim3(256), 0, s, A,
+                       B, P->d_groups, P->d_exp);
+    TN_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k, dim3((unsigned)t.nslots), dim3(256), 0, s, A, B, C, P->d_groups, t.d_tiles, P->d_exp);
+    TN_LAUNCH_CHECK();
+    return TN_OK;
+  }
   const bool vec = P->vec_ok && ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) & 15) == 0;
   KernFn k = pick_kernel(P->mode, cfg, vec, P->koff32 && P->max_k <= kKTab);
   TN_REQUIRE(k != nullptr, TN_EINVAL, "no kernel for mode %d tile %d", P->mode, cfg);

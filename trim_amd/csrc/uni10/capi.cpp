@@ -61,7 +61,7 @@ const char* u10_last_error(void) { return g_err.c_str(); }
 
 int u10_set_gemm_mode(int mode) {
   return guard([&] {
-    if (mode < 0 || mode > 3) detail::fail("unknown GEMM mode");
+    if (mode < GEMM_F64 || mode > GEMM_BF16_EXACT) detail::fail("unknown GEMM mode");
     setGemmMode((GemmMode)mode);
   });
 }

@@ -14,8 +14,8 @@ import numpy as np
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libtn_hip.so")
 
-TN_GEMM_F64, TN_GEMM_F32, TN_GEMM_BF16X3, TN_GEMM_BF16X6 = 0, 1, 2, 3
-GEMM_MODES = {"f64": 0, "f32": 1, "bf16x3": 2, "bf16x6": 3}
+TN_GEMM_F64, TN_GEMM_F32, TN_GEMM_BF16X3, TN_GEMM_BF16X6, TN_GEMM_BF16_EXACT = 0, 1, 2, 3, 4
+GEMM_MODES = {"f64": 0, "f32": 1, "bf16x3": 2, "bf16x6": 3, "bf16_exact": 4}
 
 P = ctypes.c_void_p
 I64 = ctypes.c_int64

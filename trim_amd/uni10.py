@@ -22,7 +22,7 @@ NETDIR = os.path.join(_PKG, "nets")     # the iTEBD drivers' network files
 BD_IN, BD_OUT = 1, -1
 PRT_EVEN, PRT_ODD = 0, 1
 PRTF_EVEN, PRTF_ODD = 0, 1
-GEMM_F64, GEMM_F32, GEMM_BF16X3, GEMM_BF16X6 = 0, 1, 2, 3
+GEMM_F64, GEMM_F32, GEMM_BF16X3, GEMM_BF16X6, GEMM_BF16_EXACT = 0, 1, 2, 3, 4
 
 P = ctypes.c_void_p
 PP = ctypes.POINTER(ctypes.c_void_p)
