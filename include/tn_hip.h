@@ -186,6 +186,14 @@ int tn_gemm_plan_set_tile(tn_gemm_plan plan, int tile_id); /* autotuner hook  */
 int tn_gemm_plan_autotune(tn_gemm_plan plan, const double* A, const double* B, double* C,
                           int reps, void* stream, float* best_ms);
 int tn_gemm_plan_info(tn_gemm_plan plan, int64_t* tiles, double* flops, int* tile_id);
+/* Device-side activity mask (no counterpart in the reference; for batched
+ * engines whose slots are not all busy): with mask != NULL, group g is
+ * skipped -- its output keeps its contents -- while mask[group_idx[g]] == 0
+ * (group_idx[g] < 0: never).  mask is a device int array the caller updates
+ * between launches (captured graphs read it on every replay); NULL removes
+ * the mask.  The same for SVD plans (per problem: outputs, warm-start data
+ * and convergence counts untouched) and vector-op plans (per item). */
+int tn_gemm_plan_set_mask(tn_gemm_plan plan, const int* mask, const int32_t* group_idx);
 int tn_gemm_exec(tn_gemm_plan plan, const double* A, const double* B, double* C,
                  void* stream);
 int tn_gemm_plan_destroy(tn_gemm_plan plan);
@@ -293,6 +301,7 @@ int tn_svd_plan_create(tn_svd_plan* plan, int64_t batch, const int64_t* m, const
 int tn_svd_plan_exec(tn_svd_plan plan, const double* A, const double* V0T, double* U, double* S,
                      double* VT, void* stream);
 int tn_svd_plan_check(tn_svd_plan plan, int* nonconverged, int* sweeps, void* stream);
+int tn_svd_plan_set_mask(tn_svd_plan plan, const int* mask, const int32_t* prob_idx);
 int tn_svd_plan_destroy(tn_svd_plan plan);
 
 /* eigSyDecompose (uni10_lapack_cpu.cpp:131-152): symmetric N x N;
@@ -353,6 +362,7 @@ typedef struct {
 typedef struct tn_vecops_plan_s* tn_vecops_plan;
 int tn_vecops_plan_create(tn_vecops_plan* plan, int64_t nitems, const tn_vecop* items);
 int tn_vecops_plan_exec(tn_vecops_plan plan, double* base, void* stream);
+int tn_vecops_plan_set_mask(tn_vecops_plan plan, const int* mask, const int32_t* item_idx);
 int tn_vecops_plan_destroy(tn_vecops_plan plan);
 
 #ifdef __cplusplus

@@ -152,6 +152,9 @@ int u10_batch_tebd_wait(u10_batch_tebd b);
  * poll that never blocks; wait still checks the SVDs) */
 int u10_batch_tebd_ready(u10_batch_tebd b, int* ready);
 int u10_batch_tebd_set_graph(u10_batch_tebd b, int on);
+/* chain on/off: an inactive chain is skipped on the device by every launch
+ * (its tensors, spectra and warm-start data are kept) */
+int u10_batch_tebd_set_active(u10_batch_tebd b, int chain, int on);
 /* restart one chain (slot reuse): the seeded start of (delta, seed) */
 int u10_batch_tebd_load(u10_batch_tebd b, int chain, double delta, uint64_t seed);
 int u10_batch_tebd_energy(u10_batch_tebd b, int chain, double* out);

@@ -27,7 +27,7 @@ constexpr int64_t kChunk = 4096;
 
 struct ChunkDev {
   int32_t item;
-  int32_t pad;
+  int32_t mask1;        // 1 + the mask entry that gates the item (0: never skipped)
   int64_t begin, end;   // element range of the item
 };
 
@@ -47,9 +47,11 @@ __device__ inline double block_sum(double v, double* red) {
 
 __global__ void __launch_bounds__(kThreads) vecops_kernel(double* __restrict__ base,
                                                           const tn_vecop* __restrict__ items,
-                                                          const ChunkDev* __restrict__ chunks) {
+                                                          const ChunkDev* __restrict__ chunks,
+                                                          const int* __restrict__ mask) {
   __shared__ double red[kThreads];
   const ChunkDev ck = chunks[blockIdx.x];
+  if (mask != nullptr && ck.mask1 > 0 && mask[ck.mask1 - 1] == 0) return;
   const tn_vecop it = items[ck.item];
   double* __restrict__ dst = base + it.dst;
   const double* __restrict__ src = base + it.src;
@@ -97,6 +99,8 @@ struct tn_vecops_plan_s {
   tn_vecop* d_items = nullptr;
   ChunkDev* d_chunks = nullptr;
   int64_t nchunks = 0;
+  std::vector<ChunkDev> chunks;   // host copy (for set_mask)
+  const int* mask = nullptr;
 };
 
 extern "C" {
@@ -125,6 +129,7 @@ int tn_vecops_plan_create(tn_vecops_plan* plan, int64_t nitems, const tn_vecop* 
   }
   auto* P = new tn_vecops_plan_s();
   P->nchunks = (int64_t)chunks.size();
+  P->chunks = chunks;
   int rc = TN_OK;
   if (nitems > 0) rc = tn::pool_alloc((void**)&P->d_items, sizeof(tn_vecop) * nitems);
   if (rc == TN_OK && nitems > 0) rc = tn_memcpy_h2d(P->d_items, items, sizeof(tn_vecop) * nitems, nullptr);
@@ -148,9 +153,22 @@ int tn_vecops_plan_exec(tn_vecops_plan P, double* base, void* stream) {
   TN_REQUIRE(base != nullptr, TN_EINVAL, "tn_vecops_plan_exec: NULL base");
   TN_REQUIRE(P->nchunks < (int64_t(1) << 31), TN_EUNSUPP, "too many vecop chunks");
   hipLaunchKernelGGL(vecops_kernel, dim3((unsigned)P->nchunks), dim3(kThreads), 0, tn::stream_of(stream), base,
-                     P->d_items, P->d_chunks);
+                     P->d_items, P->d_chunks, P->mask);
   TN_LAUNCH_CHECK();
   return TN_OK;
+}
+
+int tn_vecops_plan_set_mask(tn_vecops_plan P, const int* mask, const int32_t* item_idx) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(P != nullptr, TN_EINVAL, "tn_vecops_plan_set_mask: NULL plan");
+  TN_REQUIRE(mask == nullptr || item_idx != nullptr || P->chunks.empty(), TN_EINVAL,
+             "tn_vecops_plan_set_mask: NULL item indices");
+  for (ChunkDev& c : P->chunks) c.mask1 = (mask && item_idx[c.item] >= 0) ? item_idx[c.item] + 1 : 0;
+  int rc = tn_device_sync();   // launches queued with the old chunks finish first
+  if (rc == TN_OK && P->nchunks > 0)
+    rc = tn_memcpy_h2d(P->d_chunks, P->chunks.data(), sizeof(ChunkDev) * P->nchunks, nullptr);
+  if (rc == TN_OK) P->mask = mask;
+  return rc;
 }
 
 int tn_vecops_plan_destroy(tn_vecops_plan P) {

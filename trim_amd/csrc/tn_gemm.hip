@@ -74,7 +74,13 @@ struct GroupDev {
   int64_t lda, ldb, ldc;
   MapDev a_row, a_col, b_row, b_col, c_row, c_col;
   int64_t ea_off, eb_off;     // exact mode: offsets of the A-row / B-column scale exponents
+  const int* mask;            // with mask_idx >= 0: the group is skipped while mask[mask_idx] == 0
+  int64_t mask_idx;
 };
+
+__device__ __forceinline__ bool group_off(const GroupDev& g) {
+  return g.mask != nullptr && g.mask_idx >= 0 && g.mask[g.mask_idx] == 0;
+}
 
 __device__ __forceinline__ int64_t map_off(const MapDev& m, int64_t i64) {
   uint32_t i = (uint32_t)i64;
@@ -138,6 +144,7 @@ __global__ void __launch_bounds__(NT) gemm_mfma_k32(const double* __restrict__ A
   const TileDev tile = tiles[blockIdx.x];
   if (tile.group < 0) return;
   const GroupDev& g = groups[tile.group];
+  if (group_off(g)) return;
   const int64_t M = g.M, N = g.N, K = g.K;
   const int64_t m0 = (int64_t)tile.tm * BM, n0 = (int64_t)tile.tn * BN;
   const double* __restrict__ Ab = A + g.a_off;
@@ -400,6 +407,7 @@ __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict_
   const TileDev tile = tiles[blockIdx.x];
   if (tile.group < 0) return;
   const GroupDev& g = groups[tile.group];
+  if (group_off(g)) return;
   const int64_t M = g.M, N = g.N, K = g.K;
   const int64_t m0 = (int64_t)tile.tm * BM, n0 = (int64_t)tile.tn * BN;
   const double* __restrict__ Ab = A + g.a_off;
@@ -547,7 +555,7 @@ __global__ void __launch_bounds__(256) exact_scales_kernel(const double* __restr
                                                           const GroupDev* __restrict__ groups,
                                                           int32_t* __restrict__ ex) {
   const GroupDev& g = groups[blockIdx.y];
-  if (g.K == 0) return;
+  if (g.K == 0 || group_off(g)) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t rblocks = (g.M + 3) / 4;
   if ((int64_t)blockIdx.x < rblocks) {
@@ -607,6 +615,7 @@ __global__ void __launch_bounds__(256) gemm_exact_bf16(const double* __restrict_
   const TileDev tile = tiles[blockIdx.x];
   if (tile.group < 0) return;
   const GroupDev& g = groups[tile.group];
+  if (group_off(g)) return;
   const int64_t M = g.M, N = g.N, K = g.K;
   const int64_t m0 = (int64_t)tile.tm * BM, n0 = (int64_t)tile.tn * BN;
   const double* __restrict__ Ab = A + g.a_off;
@@ -1170,6 +1179,22 @@ int tn_gemm_plan_autotune(tn_gemm_plan P, const double* A, const double* B, doub
   return select_cfg(P, best);
 }
 
+int tn_gemm_plan_set_mask(tn_gemm_plan P, const int* mask, const int32_t* group_idx) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(P != nullptr, TN_EINVAL, "tn_gemm_plan_set_mask: NULL plan");
+  TN_REQUIRE(mask == nullptr || group_idx != nullptr || P->groups.empty(), TN_EINVAL,
+             "tn_gemm_plan_set_mask: NULL group indices");
+  for (size_t g = 0; g < P->groups.size(); ++g) {
+    P->groups[g].mask = mask;
+    P->groups[g].mask_idx = mask ? (int64_t)group_idx[g] : -1;
+  }
+  if (P->groups.empty()) return TN_OK;
+  int rc = tn_device_sync();   // launches queued with the old descriptors finish first
+  if (rc == TN_OK)
+    rc = tn_memcpy_h2d(P->d_groups, P->groups.data(), sizeof(GroupDev) * P->groups.size(), nullptr);
+  return rc;
+}
+
 int tn_gemm_plan_info(tn_gemm_plan P, int64_t* tiles, double* flops, int* tile_id) {
   TN_REQUIRE(P != nullptr, TN_EINVAL, "tn_gemm_plan_info: NULL plan");
   const int c = P->cfg.load();
@@ -1182,6 +1207,7 @@ int tn_gemm_plan_info(tn_gemm_plan P, int64_t* tiles, double* flops, int* tile_i
 // Fill kernel for sectors with K == 0 (dgemm with beta = 0 writes zeros).
 __global__ void zero_c_kernel(double* __restrict__ C, const GroupDev* __restrict__ gp) {
   const GroupDev& g = *gp;
+  if (group_off(g)) return;
   const int64_t total = g.M * g.N;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     int64_t m = e / g.N, n = e % g.N;

@@ -37,6 +37,8 @@ struct SvdProb {
   int64_t a_off, u_off, s_off, vt_off;
   int64_t v0_off;            // warm start: offset of an n x n V0^T (m >= n), -1 = none
   int64_t max_sweeps;        // sweeps before the problem is reported as not converged
+  const int* mask;           // with mask_idx >= 0: skipped while mask[mask_idx] == 0
+  int64_t mask_idx;
 };
 
 // Sum of a, b, c over each aligned group of TP lanes (TP a power of two),
@@ -85,6 +87,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_kernel(const SvdProb* __r
   __shared__ int rank_of[512];
   __shared__ double red[kThreads];
   const SvdProb pr = probs[blockIdx.x];
+  if (pr.mask != nullptr && pr.mask_idx >= 0 && pr.mask[pr.mask_idx] == 0) return;
   const int64_t M = pr.m, N = pr.n;
   const bool tr = (M < N);           // work on A^T
   const int m = (int)(tr ? N : M);   // column length
@@ -569,6 +572,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
   __shared__ int qr_col[kSmallN];
   __shared__ double hh_xx[2];
   const SvdProb pr = probs[blockIdx.x];
+  if (pr.mask != nullptr && pr.mask_idx >= 0 && pr.mask[pr.mask_idx] == 0) return;
   const int64_t M = pr.m, N = pr.n;
   const bool tr = (M < N);
   const int m = (int)(tr ? N : M);
@@ -1085,6 +1089,8 @@ struct tn_svd_plan_s {
   int* d_status = nullptr;      // [0] problems not converged, [1] max sweeps
   bool v0_any = false;          // some problem has a warm start
   bool u_any = false;           // some problem writes U (u_off >= 0)
+  std::vector<int64_t> orig;    // caller's index of each entry of probs
+  std::vector<int64_t> blk_orig;   // caller's index of each block-Jacobi problem
 };
 
 namespace {
@@ -1118,6 +1124,7 @@ int svd_plan_build(tn_svd_plan_s** out, int64_t batch, const int64_t* m, const i
                    const int64_t* a_off, const int64_t* v0_off, const int64_t* u_off,
                    const int64_t* s_off, const int64_t* vt_off) {
   std::vector<SvdProb> small_probs, lds_probs;
+  std::vector<int64_t> small_orig, lds_orig;
   std::vector<tn::bj::Spec> blk;
   auto* P = new tn_svd_plan_s();
   for (int64_t i = 0; i < batch; ++i) {
@@ -1129,18 +1136,22 @@ int svd_plan_build(tn_svd_plan_s** out, int64_t batch, const int64_t* m, const i
     const int64_t mm = std::max(m[i], n[i]), nn = std::min(m[i], n[i]);
     const int cap = g_max_sweeps.load();
     SvdProb p{m[i], n[i], a_off ? a_off[i] : 0, u_off ? u_off[i] : 0, s_off ? s_off[i] : 0,
-              vt_off ? vt_off[i] : 0, v0_off ? v0_off[i] : -1, cap > 0 ? std::min(cap, kOneWgMaxSweeps) : kOneWgMaxSweeps};
+              vt_off ? vt_off[i] : 0, v0_off ? v0_off[i] : -1, cap > 0 ? std::min(cap, kOneWgMaxSweeps) : kOneWgMaxSweeps,
+              nullptr, -1};
     P->v0_any |= p.v0_off >= 0;
     P->u_any |= p.u_off >= 0;
     const size_t bytes = sizeof(double) * (size_t)((mm + 1) * nn + (nn + 1) * nn);
     if (nn <= kSmallN && mm <= kSmallTP * 16) {
       small_probs.push_back(p);
+      small_orig.push_back(i);
       P->small_mm = std::max(P->small_mm, mm);
     } else if (bytes <= kLdsBytes && nn < g_block_min) {
       P->lds_need = std::max(P->lds_need, bytes);
       lds_probs.push_back(p);
+      lds_orig.push_back(i);
     } else if (mm > 0 && nn > 0) {
       blk.push_back(tn::bj::Spec{p.m, p.n, p.a_off, p.u_off, p.s_off, p.vt_off, p.v0_off});
+      P->blk_orig.push_back(i);
     }
   }
   // lanes per column pair: as few as the widest problem allows with one
@@ -1157,6 +1168,8 @@ int svd_plan_build(tn_svd_plan_s** out, int64_t batch, const int64_t* m, const i
   P->n_lds = lds_probs.size();
   P->probs = small_probs;
   P->probs.insert(P->probs.end(), lds_probs.begin(), lds_probs.end());
+  P->orig = small_orig;
+  P->orig.insert(P->orig.end(), lds_orig.begin(), lds_orig.end());
   int rc = tn::bj::build(&P->blk, blk, g_max_sweeps.load());
   if (rc == TN_OK) rc = tn::pool_alloc((void**)&P->d_status, 2 * sizeof(int));
   if (rc == TN_OK) {
@@ -1305,7 +1318,8 @@ int tn_svd_f64_noalloc(const double* A, int64_t M, int64_t N, double* U, double*
   if (rc != TN_OK) return rc;
   hipStream_t st = tn::stream_of(stream);
   const int cap = g_max_sweeps.load();
-  const SvdProb p{M, N, 0, U ? 0 : -1, 0, 0, -1, cap > 0 ? std::min(cap, kOneWgMaxSweeps) : kOneWgMaxSweeps};
+  const SvdProb p{M, N, 0, U ? 0 : -1, 0, 0, -1, cap > 0 ? std::min(cap, kOneWgMaxSweeps) : kOneWgMaxSweeps,
+                  nullptr, -1};
   hipLaunchKernelGGL(noalloc_setup_kernel, dim3(1), dim3(64), 0, st, d, p);
   TN_LAUNCH_CHECK();
   // the same kernel and template choice as a plan holding this one problem
@@ -1386,6 +1400,24 @@ int tn_svd_plan_exec(tn_svd_plan P, const double* A, const double* V0T, double* 
   bool needs_v0 = P->v0_any;
   TN_REQUIRE(!needs_v0 || V0T, TN_EINVAL, "tn_svd_plan_exec: the plan has warm starts but V0T is NULL");
   return svd_plan_launch(P, A, V0T, U, S, VT, tn::stream_of(stream));
+}
+
+int tn_svd_plan_set_mask(tn_svd_plan P, const int* mask, const int32_t* prob_idx) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(P != nullptr, TN_EINVAL, "tn_svd_plan_set_mask: NULL plan");
+  TN_REQUIRE(mask == nullptr || prob_idx != nullptr || (P->probs.empty() && P->blk_orig.empty()), TN_EINVAL,
+             "tn_svd_plan_set_mask: NULL problem indices");
+  int rc = tn_device_sync();   // launches queued with the old descriptors finish first
+  if (rc != TN_OK) return rc;
+  for (size_t k = 0; k < P->probs.size(); ++k) {
+    P->probs[k].mask = mask;
+    P->probs[k].mask_idx = mask ? (int64_t)prob_idx[P->orig[k]] : -1;
+  }
+  if (!P->probs.empty()) rc = tn_memcpy_h2d(P->d_p, P->probs.data(), sizeof(SvdProb) * P->probs.size(), nullptr);
+  if (rc != TN_OK) return rc;
+  std::vector<int64_t> bidx(P->blk_orig.size(), -1);
+  for (size_t k = 0; k < bidx.size(); ++k) bidx[k] = mask ? (int64_t)prob_idx[P->blk_orig[k]] : -1;
+  return tn::bj::set_mask(P->blk, mask, bidx);
 }
 
 int tn_svd_plan_check(tn_svd_plan P, int* nonconverged, int* sweeps, void* stream) {

@@ -29,6 +29,10 @@ int build(Plan** out, const std::vector<Spec>& specs, int max_sweeps = 0);
 int launch(Plan* P, const double* A, const double* V0T, double* U, double* S, double* VT, int* status,
            hipStream_t st);
 void destroy(Plan* P);
+// Problem k (in the order of the specs) is skipped by every kernel while
+// mask[idx[k]] == 0 (idx[k] < 0 or mask == NULL: never skipped); a skipped
+// problem's outputs and scratch keep their contents.  Returns a TN_* code.
+int set_mask(Plan* P, const int* mask, const std::vector<int64_t>& idx);
 bool empty(const Plan* P);
 
 }  // namespace bj

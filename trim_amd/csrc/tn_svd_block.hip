@@ -83,7 +83,13 @@ struct Prob {
   int32_t qr;                             // QR-preconditioned: sweeps on R^T, V holds V0
   int32_t qpw;                            // its QR panel width (32, or 16 beyond 1024 rows)
   int32_t ms;                             // rows the sweeps touch (mp; n rounded up to 16 for QR)
+  const int* mask;                        // with mask_idx >= 0: skipped while mask[mask_idx] == 0
+  int64_t mask_idx;
 };
+
+__device__ inline bool prob_off(const Prob& p) {
+  return p.mask != nullptr && p.mask_idx >= 0 && p.mask[p.mask_idx] == 0;
+}
 
 __device__ inline void rr_pair(int r, int k, int np, int& p, int& q) {
   const int L = np - 1;
@@ -181,6 +187,11 @@ __global__ void __launch_bounds__(kThreads) bj_setup(const Prob* __restrict__ pr
                                                      const double* __restrict__ V0T, double* __restrict__ scr,
                                                      int* __restrict__ iscr) {
   const Prob pr = probs[blockIdx.y];
+  if (prob_off(pr)) {   // the shared per-sweep flags are still cleared
+    if (blockIdx.x == 0 && blockIdx.y == 0)
+      for (int s = threadIdx.x; s < kMaxSweeps; s += kThreads) iscr[s] = 0;
+    return;
+  }
   const bool warm = pr.v0_off >= 0;
   double* W = scr + pr.w_off;
   double* V = scr + pr.v_off;
@@ -218,6 +229,7 @@ __device__ inline double wave_sum(double v) {
 // strided over the lanes (coalesced) and summed by a fixed butterfly.
 __global__ void __launch_bounds__(kThreads) bj_colnorm(const Prob* __restrict__ probs, double* __restrict__ scr) {
   const Prob pr = probs[blockIdx.y];
+  if (prob_off(pr)) return;
   const double* W = scr + pr.w_off;
   double* nrm = scr + pr.sig_off;
   const int lane = threadIdx.x & 63;
@@ -257,6 +269,7 @@ __global__ void __launch_bounds__(kThreads) bj_sort(const Prob* __restrict__ pro
   extern __shared__ double sh[];
   __shared__ double red[kThreads];
   const Prob pr = probs[blockIdx.x];
+  if (prob_off(pr)) return;
   if (sweep > 0 && iscr[pr.flag_off + sweep - 1] == 0) return;
   const double* nrm = scr + pr.sig_off;
   int* pos = iscr + pr.pos_off;
@@ -291,6 +304,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 
   const int2 wm = wmap[blockIdx.x];
   const Prob pr = probs[wm.x];
+  if (prob_off(pr)) return;
   if (round >= pr.nb - 1) return;
   int* flags = iscr + pr.flag_off;
   if (sweep > 0 && flags[sweep - 1] == 0) return;   // converged in an earlier sweep
@@ -590,6 +604,7 @@ __global__ void __launch_bounds__(kThreads) bj_qr_panel(const Prob* __restrict__
   __shared__ double res[PW], row[PW], rdiag[PW];
   __shared__ double Ts[PW][PW + 1];
   const Prob pr = probs[qlist[blockIdx.x]];
+  if (prob_off(pr)) return;
   const int kb = panel * PW;
   if (kb >= pr.n) return;
   const int nb = min(PW, pr.n - kb);
@@ -702,6 +717,7 @@ __global__ void __launch_bounds__(kThreads) bj_qr_update(const Prob* __restrict_
   __shared__ double Ys[64][PW + 1], Cs[64][33];
   __shared__ double Zs[PW][33], Z2[PW][33], Tt[PW][PW + 1];
   const Prob pr = probs[qlist[blockIdx.y]];
+  if (prob_off(pr)) return;
   const int kb = panel * PW;
   const int c0 = kb + PW + 32 * (int)blockIdx.x;
   if (c0 >= pr.n) return;
@@ -767,6 +783,7 @@ __global__ void __launch_bounds__(kThreads) bj_qr_update(const Prob* __restrict_
 __global__ void __launch_bounds__(kThreads) bj_qr_transpose(const Prob* __restrict__ probs,
                                                             const int* __restrict__ qlist, double* __restrict__ scr) {
   const Prob pr = probs[qlist[blockIdx.y]];
+  if (prob_off(pr)) return;
   double* W = scr + pr.w_off;
   const int64_t n = pr.n, mp = pr.mp;
   const int64_t stride = (int64_t)gridDim.x * kThreads;
@@ -784,6 +801,7 @@ __global__ void __launch_bounds__(kThreads) bj_qr_transpose(const Prob* __restri
 __global__ void __launch_bounds__(kThreads) bj_qr_ut(const Prob* __restrict__ probs, const int* __restrict__ qlist,
                                                      double* __restrict__ scr, const int* __restrict__ iscr) {
   const Prob pr = probs[qlist[blockIdx.y]];
+  if (prob_off(pr)) return;
   const double* W = scr + pr.w_off;
   const double* sig = scr + pr.sig_off;
   const int* rank = iscr + pr.rank_off;
@@ -807,6 +825,7 @@ __global__ void __launch_bounds__(kThreads) bj_qr_complete(const Prob* __restric
                                                            const double* __restrict__ S) {
   __shared__ double u[kQrMaxM], d[kQrMaxM], red[kThreads];
   const Prob pr = probs[qlist[blockIdx.x]];
+  if (prob_off(pr)) return;
   const int n = pr.n, tid = threadIdx.x;
   double* Ut = scr + pr.ut_off;
   int good = n;   // rows [0, good) are orthonormal (S is descending)
@@ -849,6 +868,7 @@ __global__ void __launch_bounds__(kThreads) bj_qr_complete(const Prob* __restric
 __global__ void __launch_bounds__(kThreads) bj_qr_polish(const Prob* __restrict__ probs, const int* __restrict__ qlist,
                                                          const double* __restrict__ scr, double* __restrict__ VT) {
   const Prob pr = probs[qlist[blockIdx.y]];
+  if (prob_off(pr)) return;
   const double* H = scr + pr.w_off;
   const int64_t nn = (int64_t)pr.n * pr.n;
   const int64_t stride = (int64_t)gridDim.x * kThreads;
@@ -863,6 +883,7 @@ __global__ void __launch_bounds__(kThreads) bj_sigma(const Prob* __restrict__ pr
                                                      int* __restrict__ status) {
   extern __shared__ double sh[];
   const Prob pr = probs[blockIdx.x];
+  if (prob_off(pr)) return;
   const double* nrm = scr + pr.sig_off;
   int* rank = iscr + pr.rank_off;
   const int tid = threadIdx.x;
@@ -893,6 +914,7 @@ __global__ void __launch_bounds__(kThreads) bj_vectors(const Prob* __restrict__ 
                                                        const int* __restrict__ iscr, double* __restrict__ U,
                                                        double* __restrict__ VT) {
   const Prob pr = probs[blockIdx.y];
+  if (prob_off(pr)) return;
   if (pr.qr) return;   // VT by the U_X^T V0^T product
   const double* W = scr + pr.w_off;
   const double* V = scr + pr.v_off;
@@ -937,6 +959,7 @@ struct Plan {
   double* d_scr = nullptr;
   int* d_iscr = nullptr;
   tn_gemm_plan warm = nullptr;    // W = A V0 of the warm-started problems
+  std::vector<int> gd_prob, qd_prob, qg_prob, qh_prob;   // problem of each group of warm / qr_vt / qr_g / qr_h
   int max_sw = kMaxSweeps;        // sweep cap (tn_svd_set_max_sweeps)
   int* d_qlist = nullptr;         // QR-preconditioned problems: 32-wide panels first, then 16-wide
   int nqr = 0, nqr32 = 0, max_n32 = 0, max_n16 = 0;
@@ -962,6 +985,28 @@ void destroy(Plan* P) {
 }
 
 bool empty(const Plan* P) { return P == nullptr || P->probs.empty(); }
+
+int set_mask(Plan* P, const int* mask, const std::vector<int64_t>& idx) {
+  if (P == nullptr || P->probs.empty()) return TN_OK;
+  TN_REQUIRE(idx.size() == P->probs.size(), TN_EINVAL, "block SVD mask: %zu indices for %zu problems", idx.size(),
+             P->probs.size());
+  for (size_t k = 0; k < P->probs.size(); ++k) {
+    P->probs[k].mask = mask;
+    P->probs[k].mask_idx = mask ? idx[k] : -1;
+  }
+  int rc = tn_memcpy_h2d(P->d_probs, P->probs.data(), sizeof(Prob) * P->probs.size(), nullptr);
+  auto sub = [&](tn_gemm_plan g, const std::vector<int>& prob) {
+    if (rc != TN_OK || g == nullptr) return;
+    std::vector<int32_t> gi(prob.size());
+    for (size_t k = 0; k < prob.size(); ++k) gi[k] = (int32_t)idx[prob[k]];
+    rc = tn_gemm_plan_set_mask(g, mask, gi.data());
+  };
+  sub(P->warm, P->gd_prob);
+  sub(P->qr_vt, P->qd_prob);
+  sub(P->qr_g, P->qg_prob);
+  sub(P->qr_h, P->qh_prob);
+  return rc;
+}
 
 int build(Plan** out, const std::vector<Spec>& specs, int max_sweeps) {
   auto* P = new Plan();
@@ -1041,6 +1086,7 @@ int build(Plan** out, const std::vector<Spec>& specs, int max_sweeps) {
       g.c_off = s.vt_off;
       g.ldc = n;
       qd.push_back(g);
+      P->qd_prob.push_back((int)i);
       // Newton-Schulz step on VT: G = VT VT^T into Ut's storage, H = G VT
       // into W's (both free by then)
       tn_gemm_desc_ex h{};
@@ -1055,6 +1101,7 @@ int build(Plan** out, const std::vector<Spec>& specs, int max_sweeps) {
       h.c_off = p.ut_off;
       h.ldc = n;
       qg.push_back(h);
+      P->qg_prob.push_back((int)i);
       tn_gemm_desc_ex f{};
       f.M = n;
       f.N = n;
@@ -1066,6 +1113,7 @@ int build(Plan** out, const std::vector<Spec>& specs, int max_sweeps) {
       f.c_off = p.w_off;
       f.ldc = n;
       qh.push_back(f);
+      P->qh_prob.push_back((int)i);
     }
     for (int k = 0; k < p.nb / 2; ++k) map.push_back(make_int2((int)i, k));
     P->max_nb = std::max(P->max_nb, p.nb);
@@ -1087,6 +1135,7 @@ int build(Plan** out, const std::vector<Spec>& specs, int max_sweeps) {
       g.ldb = n;
       g.ldc = mp;
       gd.push_back(g);
+      P->gd_prob.push_back((int)i);
     }
     P->probs.push_back(p);
   }

@@ -361,16 +361,31 @@ BatchTebd::BatchTebd(const std::vector<int>& chi, const std::vector<double>& del
     init_chain((int)i, delta[i], seed[i], host.data() + off_[i].x[0]);
   }
 
+  // every chain starts active (device flags read by every launch)
+  active_host_.assign(nc, 1);
+  detail::check(tn_malloc(reinterpret_cast<void**>(&active_), sizeof(int) * nc), "BatchTebd active");
+  detail::check(tn_memcpy_h2d(active_, active_host_.data(), sizeof(int) * nc, stream_), "BatchTebd active");
+
   // plans, one set per update parity p (p = 0: X = ALa, Y = BLb; p = 1: swapped)
   for (int p = 0; p < 2; ++p) {
     std::vector<tn_gemm_desc_ex> d1, d2, d3;
     std::vector<int64_t> svd[7];   // m, n, a, u, s, vt, v0
     std::vector<tn_vecop> rows, gather, trunc, scale;
+    // the chain of every GEMM group, SVD problem and vector item (for the mask)
+    std::vector<int32_t> c1, c2, c3, cs, crows, cgather, ctrunc, cscale;
     for (size_t i = 0; i < nc; ++i) {
       if (sym_[i])
         plans_u1((int)i, p, d1, d2, d3, rows, gather, trunc, scale, svd);
       else
         plans_dense((int)i, p, d1, d2, d3, rows, gather, trunc, scale, svd);
+      c1.resize(d1.size(), (int32_t)i);
+      c2.resize(d2.size(), (int32_t)i);
+      c3.resize(d3.size(), (int32_t)i);
+      cs.resize(svd[0].size(), (int32_t)i);
+      crows.resize(rows.size(), (int32_t)i);
+      cgather.resize(gather.size(), (int32_t)i);
+      ctrunc.resize(trunc.size(), (int32_t)i);
+      cscale.resize(scale.size(), (int32_t)i);
     }
     auto gemm = [&](tn_gemm_plan* g, const std::vector<tn_gemm_desc_ex>& ds, const char* what) {
       detail::check(tn_gemm_plan_create_ex(g, gemm_mode, (int64_t)ds.size(), ds.data()), what);
@@ -399,6 +414,15 @@ BatchTebd::BatchTebd(const std::vector<int>& chi, const std::vector<double>& del
     vec(&gather_[p], gather, "BatchTebd gather");
     vec(&trunc_[p], trunc, "BatchTebd trunc");
     vec(&scale_[p], scale, "BatchTebd scale");
+    // an inactive chain's work is skipped on the device, its state untouched
+    detail::check(tn_gemm_plan_set_mask(g1_[p], active_, c1.data()), "BatchTebd mask");
+    detail::check(tn_gemm_plan_set_mask(g2_[p], active_, c2.data()), "BatchTebd mask");
+    detail::check(tn_gemm_plan_set_mask(g3_[p], active_, c3.data()), "BatchTebd mask");
+    detail::check(tn_svd_plan_set_mask(svd_[p], active_, cs.data()), "BatchTebd mask");
+    detail::check(tn_vecops_plan_set_mask(rows_[p], active_, crows.data()), "BatchTebd mask");
+    detail::check(tn_vecops_plan_set_mask(gather_[p], active_, cgather.data()), "BatchTebd mask");
+    detail::check(tn_vecops_plan_set_mask(trunc_[p], active_, ctrunc.data()), "BatchTebd mask");
+    detail::check(tn_vecops_plan_set_mask(scale_[p], active_, cscale.data()), "BatchTebd mask");
   }
   detail::check(tn_memcpy_h2d(arena_.get(), host.data(), sizeof(double) * (size_t)o, stream_), "BatchTebd upload");
 }
@@ -662,6 +686,7 @@ BatchTebd::~BatchTebd() {
     tn_vecops_plan_destroy(trunc_[p]);
     tn_vecops_plan_destroy(scale_[p]);
   }
+  tn_free(active_);
   tn_stream_destroy(stream_);
 }
 
@@ -697,7 +722,21 @@ void BatchTebd::step(int n) {
   wait();
 }
 
+void BatchTebd::set_active(int chain, bool on) {
+  if (chain < 0 || chain >= chains()) detail::fail("BatchTebd::set_active: bad index");
+  if (active_host_[chain] != (on ? 1 : 0)) {
+    active_host_[chain] = on ? 1 : 0;
+    active_dirty_ = true;
+  }
+}
+
 void BatchTebd::launch(int n) {
+  if (active_dirty_) {
+    // in stream order before the next step (graph replays read the flags)
+    detail::check(tn_memcpy_h2d(active_, active_host_.data(), sizeof(int) * active_host_.size(), stream_),
+                  "BatchTebd active");
+    active_dirty_ = false;
+  }
   for (int k = 0; k < n; ++k) {
     if (use_graph_ && warm_[0] && warm_[1]) {
       // every later step issues the same launches: capture once, replay

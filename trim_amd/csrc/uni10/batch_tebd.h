@@ -55,6 +55,9 @@ class BatchTebd {
   // reuse of continuous batching; other chains are untouched
   void load(int chain, double delta, uint64_t seed);
   void set_graph(bool on);         // replay steps through a captured HIP graph
+  // An inactive chain is skipped by every launch (its state is kept); takes
+  // effect from the next launch.  All chains start active.
+  void set_active(int chain, bool on);
   double energy(int chain);        // energy per site (networks, one chain)
   // 0: lambda_A, 1: lambda_B (a U(1) chain: sector by sector, charges ascending)
   std::vector<double> lambda(int chain, int which) const;
@@ -105,6 +108,9 @@ class BatchTebd {
   void* stream_ = nullptr;
   tn_gemm_plan g1_[2] = {nullptr, nullptr}, g2_[2] = {nullptr, nullptr}, g3_[2] = {nullptr, nullptr};
   tn_svd_plan svd_[2] = {nullptr, nullptr};
+  int* active_ = nullptr;               // device activity flags, one per chain
+  std::vector<int> active_host_;
+  bool active_dirty_ = false;
   tn_vecops_plan rows_[2] = {nullptr, nullptr}, gather_[2] = {nullptr, nullptr}, trunc_[2] = {nullptr, nullptr},
                  scale_[2] = {nullptr, nullptr};
   tn_svd_plan last_svd_ = nullptr;

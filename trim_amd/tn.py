@@ -63,6 +63,7 @@ SIGNATURES = {
     "tn_gemm_plan_set_tile": [P, ctypes.c_int],
     "tn_gemm_plan_autotune": [P, P, P, P, ctypes.c_int, P, ctypes.POINTER(ctypes.c_float)],
     "tn_gemm_plan_info": [P, PI64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)],
+    "tn_gemm_plan_set_mask": [P, P, P],
     "tn_gemm_exec": [P, P, P, P, P],
     "tn_gemm_plan_destroy": [P],
     "tn_matmul_f64": [P, P, I64, I64, I64, P, ctypes.c_int, P],
@@ -86,9 +87,11 @@ SIGNATURES = {
     "tn_svd_plan_create": [ctypes.POINTER(ctypes.c_void_p), I64, P, P, P, P, P, P, P],
     "tn_svd_plan_exec": [P, P, P, P, P, P, P],
     "tn_svd_plan_check": [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), P],
+    "tn_svd_plan_set_mask": [P, P, P],
     "tn_svd_plan_destroy": [P],
     "tn_vecops_plan_create": [ctypes.POINTER(ctypes.c_void_p), I64, P],
     "tn_vecops_plan_exec": [P, P, P],
+    "tn_vecops_plan_set_mask": [P, P, P],
     "tn_vecops_plan_destroy": [P],
     "tn_eigh_f64": [P, I64, P, P, P],
     "tn_lanczos_f64": [P, I64, P, PI64, ctypes.c_double, ctypes.POINTER(ctypes.c_double),

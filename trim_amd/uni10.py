@@ -99,6 +99,7 @@ SIGNATURES = {
     "u10_batch_tebd_wait": [P],
     "u10_batch_tebd_ready": [P, PI],
     "u10_batch_tebd_set_graph": [P, I],
+    "u10_batch_tebd_set_active": [P, I, I],
     "u10_batch_tebd_load": [P, I, D, ctypes.c_uint64],
     "u10_batch_tebd_energy": [P, I, PD],
     "u10_batch_tebd_lambda": [P, I, I, P, PI],
@@ -581,6 +582,10 @@ class BatchTebd:
 
     def set_graph(self, on: bool = True):
         _call("u10_batch_tebd_set_graph", self.h, int(bool(on)))
+
+    def set_active(self, chain: int, on: bool = True):
+        """An inactive chain is skipped on the device (its state is kept)."""
+        _call("u10_batch_tebd_set_active", self.h, int(chain), int(bool(on)))
 
     def load(self, chain: int, delta: float, seed: int):
         _call("u10_batch_tebd_load", self.h, int(chain), float(delta), int(seed))
