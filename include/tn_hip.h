@@ -221,6 +221,9 @@ int tn_norm2_f64(const double* x, int64_t n, double* out_dev, void* stream);
  * y += a x; the strided copy dst[i*dst_inc] = src[i*src_inc] (i < n) of
  * setDiag / getDiag (uni10_tools_cpu.cpp:170-181). */
 int tn_sum_f64(const double* x, int64_t n, int64_t inc, double* out_dev, void* stream);
+/* Seeded fill of elemRand / randomize (uni10_tools.h:67,101): dst[i] =
+ * SplitMix64 uniform[-1, 1) of (seed, i + 1), generated on the device. */
+int tn_fill_uniform_f64(double* dst, int64_t n, uint64_t seed, void* stream);
 int tn_axpy_f64(double a, const double* x, double* y, int64_t n, void* stream);
 int tn_copy_strided_f64(double* dst, int64_t dst_inc, const double* src, int64_t src_inc, int64_t n,
                         void* stream);

@@ -64,6 +64,9 @@ int u10_tensor_get_block(u10_tensor t, const int* qnum, double* host);
 int u10_tensor_permute(u10_tensor t, int n, const int* labels, int in_bond_num);
 int u10_tensor_transpose(u10_tensor t);
 int u10_tensor_scale(u10_tensor t, double a);
+/* randomize: elements (block layout order) = SplitMix64 uniform[-1, 1) of
+ * (seed, 1, 2, ...), generated on the device (elemRand, uni10_tools.h:101) */
+int u10_tensor_randomize(u10_tensor t, uint64_t seed);
 int u10_tensor_norm(u10_tensor t, double* out);
 int u10_tensor_device_ptr(u10_tensor t, void** dptr);          /* elements in HBM */
 int u10_contract(u10_tensor a, u10_tensor b, int fast, u10_tensor* out);

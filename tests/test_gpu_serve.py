@@ -127,3 +127,104 @@ def test_configured_chi_range_same_alone_and_in_mix(serve, ora, golden):
 def dataclass_copy(r):
     import dataclasses
     return dataclasses.replace(r, t_admit=-1.0, t_done=-1.0, result=None)
+
+
+def _burst(serve, n, chi, steps=3, kind="tebd", seed0=700):
+    return [serve.Request(rid=i, t_arrival=0.0, kind=kind, chi=chi, delta=0.3 + 0.1 * i, seed=seed0 + i,
+                          steps=steps) for i in range(n)]
+
+
+def test_over_budget_requests_queue(serve, tn, golden):
+    """Eight requests at once against a budget of about two slots: they
+    wait for slots instead of failing, the class keeps one engine, the
+    library never holds more than the budget, and every result equals the
+    one from a server with room to grow (four engines of two slots)."""
+    import ctypes
+    reqs = _burst(serve, 8, 16)
+    tight = serve.Server([16], 3 * serve.slot_bytes(16), golden, slots_per_engine=2)
+    try:
+        assert tight.engine_count(16) == 1 and tight.slots(16) == 2
+        tn.call("tn_pool_trim")
+        stats = tight.run(reqs, realtime=False)
+        v = [ctypes.c_size_t() for _ in range(4)]
+        tn.call("tn_mem_stats", *[ctypes.byref(x) for x in v])
+        assert v[0].value <= tight.budget
+    finally:
+        tight.close()
+    _say("tight: %d requests, %d failed, engines %s" % (stats["requests"], stats["failed"], stats["engines"]))
+    assert stats["requests"] == 8 and stats["failed"] == 0
+    assert stats["engines"] == {"chi16_d2_dense": 1}
+    roomy = serve.Server([16], 64 * serve.slot_bytes(16), golden, slots_per_engine=2)
+    try:
+        out = roomy.run(_burst(serve, 8, 16), realtime=False)
+    finally:
+        roomy.close()
+    _say("roomy: engines %s" % out["engines"])
+    assert out["engines"]["chi16_d2_dense"] >= 2
+    a = {r.rid: r.result for r in stats["done"]}
+    for r in out["done"]:
+        assert r.result.tobytes() == a[r.rid].tobytes()
+
+
+def test_budget_applies_only_to_server_allocations(serve, tn, golden):
+    """The server's budget limits its own allocations only: outside its
+    calls the library keeps the budget it had."""
+    import ctypes
+    b = ctypes.c_size_t()
+    tn.call("tn_set_budget", 0)
+    srv = serve.Server([8], 4 * serve.slot_bytes(8), golden)
+    srv.run(_burst(serve, 2, 8, steps=2) + _burst(serve, 1, 8, kind="contract", seed0=950), realtime=False)
+    tn.call("tn_mem_stats", None, None, None, ctypes.byref(b))
+    assert b.value == 0 and srv.budget > 0
+    srv.close()
+
+
+def test_failed_engine_is_isolated(serve, tn, golden):
+    """A class whose SVDs cannot converge (its plans capped at one sweep)
+    fails its own requests with an error; the other class's requests and
+    the contractions in the same stream complete, each equal to its result
+    from a server without the failing class."""
+    # max_slots=2: the failing class cannot grow an engine with default plans
+    srv = serve.Server([8], 8 * serve.slot_bytes(16), golden, slots_per_engine=2, max_slots=2)
+    tn.call("tn_svd_set_max_sweeps", 1)
+    try:
+        srv.add_class(16)
+    finally:
+        tn.call("tn_svd_set_max_sweeps", 0)
+    reqs = _burst(serve, 3, 8) + [dataclass_copy(r) for r in _burst(serve, 3, 16, seed0=800)] \
+        + _burst(serve, 2, 8, kind="contract", seed0=900)
+    for i, r in enumerate(reqs):
+        r.rid = i
+    try:
+        stats = srv.run(reqs, realtime=False)
+    finally:
+        srv.close()
+    _say("isolation: %d requests, %d failed" % (stats["requests"], stats["failed"]))
+    assert stats["requests"] == len(reqs)
+    by = {r.rid: r for r in stats["done"]}
+    for r in reqs:
+        if r.chi == 16:
+            assert by[r.rid].error is not None and "did not converge" in by[r.rid].error
+        else:
+            assert by[r.rid].error is None
+    good = serve.Server([8], 8 * serve.slot_bytes(16), golden, slots_per_engine=2, max_slots=2)
+    try:
+        ref = good.run([dataclass_copy(r) for r in reqs if r.chi == 8], realtime=False)
+    finally:
+        good.close()
+    for r in ref["done"]:
+        assert r.result.tobytes() == by[r.rid].result.tobytes()
+
+
+def test_idle_slots_are_switched_off(serve, golden):
+    """Between requests every slot is inactive: a step of an engine holding
+    one request computes one chain, and an idle server launches nothing."""
+    srv = serve.Server([8], 8 * serve.slot_bytes(8), golden, slots_per_engine=4)
+    try:
+        e = srv.engines[(8, 2, 0)][0]
+        assert not e.busy and not e.inflight
+        stats = srv.run(_burst(serve, 1, 8, steps=4), realtime=False)
+        assert stats["chain_steps"] == 4
+        assert not e.busy and sorted(e.free) == [0, 1, 2, 3]
+    finally:
+        srv.close()

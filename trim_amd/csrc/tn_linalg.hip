@@ -398,6 +398,18 @@ __global__ void __launch_bounds__(1024) sum_kernel(const double* __restrict__ x,
   if (threadIdx.x == 0) *out = t;
 }
 
+// SplitMix64 -> uniform[-1, 1): dst[i] = 2 (mix(seed + (i+1) golden) >> 11) 2^-53 - 1
+// (every step exact, so the values equal a host evaluation bit for bit)
+__global__ void fill_uniform_kernel(double* __restrict__ dst, int64_t n, uint64_t seed) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t z = seed + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z = z ^ (z >> 31);
+    dst[i] = 2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+  }
+}
+
 __global__ void axpy_kernel(double a, const double* __restrict__ x, double* __restrict__ y, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     y[i] += a * x[i];
@@ -1472,6 +1484,15 @@ int tn_sum_f64(const double* x, int64_t n, int64_t inc, double* out_dev, void* s
   TN_INIT_OR_RETURN();
   TN_REQUIRE(out_dev && (n == 0 || x) && n >= 0 && inc >= 1, TN_EINVAL, "tn_sum_f64: bad argument");
   hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(1024), 0, tn::stream_of(stream), x, n, inc, out_dev);
+  TN_LAUNCH_CHECK();
+  return TN_OK;
+}
+
+int tn_fill_uniform_f64(double* dst, int64_t n, uint64_t seed, void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(n >= 0 && (n == 0 || dst), TN_EINVAL, "tn_fill_uniform_f64: bad argument");
+  if (n == 0) return TN_OK;
+  hipLaunchKernelGGL(fill_uniform_kernel, dim3(grid_for(n)), dim3(256), 0, tn::stream_of(stream), dst, n, seed);
   TN_LAUNCH_CHECK();
   return TN_OK;
 }

@@ -300,6 +300,20 @@ BatchTebd::BatchTebd(const std::vector<int>& chi, const std::vector<double>& del
     if (d_[i] != 2 && d_[i] != 3) detail::fail("BatchTebd: d must be 2 (spin 1/2) or 3 (spin 1)");
     if (sym_[i] != 0 && sym_[i] != 1) detail::fail("BatchTebd: sym must be 0 (dense) or 1 (U(1))");
   }
+  // a failure part-way (the device memory budget, say) returns everything
+  // taken so far before the error reaches the caller
+  try {
+    build(seed, tau, gemm_mode);
+  } catch (...) {
+    release();
+    throw;
+  }
+}
+
+void BatchTebd::build(const std::vector<uint64_t>& seed, double tau, int gemm_mode) {
+  const size_t nc = chi_.size();
+  const std::vector<int>& chi = chi_;
+  const std::vector<double>& delta = delta_;
   // the engine's own stream (default priority): independent engines overlap
   detail::check(tn_stream_create(&stream_), "BatchTebd stream");
   u1_.resize(nc);
@@ -673,9 +687,12 @@ void BatchTebd::load(int chain, double delta, uint64_t seed) {
                 "BatchTebd load");
 }
 
-BatchTebd::~BatchTebd() {
-  tn_stream_sync(stream_);
+BatchTebd::~BatchTebd() { release(); }
+
+void BatchTebd::release() noexcept {
+  if (stream_) tn_stream_sync(stream_);
   if (graph_) tn_graph_destroy(graph_);
+  graph_ = nullptr;
   for (int p = 0; p < 2; ++p) {
     tn_gemm_plan_destroy(g1_[p]);
     tn_gemm_plan_destroy(g2_[p]);
@@ -685,9 +702,14 @@ BatchTebd::~BatchTebd() {
     tn_vecops_plan_destroy(gather_[p]);
     tn_vecops_plan_destroy(trunc_[p]);
     tn_vecops_plan_destroy(scale_[p]);
+    g1_[p] = g2_[p] = g3_[p] = nullptr;
+    svd_[p] = nullptr;
+    rows_[p] = gather_[p] = trunc_[p] = scale_[p] = nullptr;
   }
   tn_free(active_);
+  active_ = nullptr;
   tn_stream_destroy(stream_);
+  stream_ = nullptr;
 }
 
 void BatchTebd::update(int p) {

@@ -95,6 +95,8 @@ class BatchTebd {
                 std::vector<tn_gemm_desc_ex>& d3, std::vector<tn_vecop>& rows, std::vector<tn_vecop>& gather,
                 std::vector<tn_vecop>& trunc, std::vector<tn_vecop>& scale, std::vector<int64_t>* svd);
   void update(int p);
+  void release() noexcept;   // streams, plans, graph and flags (the arena frees itself)
+  void build(const std::vector<uint64_t>& seed, double tau, int gemm_mode);
   void init_chain(int i, double delta, uint64_t seed, double* base);
   std::vector<double> gate(int i, double delta) const;
   std::vector<int64_t> region_;   // arena doubles of each chain (from its x[0])

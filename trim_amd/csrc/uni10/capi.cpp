@@ -268,6 +268,10 @@ int u10_tensor_transpose(u10_tensor t) {
   return guard([&] { t->T.transpose(); });
 }
 
+int u10_tensor_randomize(u10_tensor t, uint64_t seed) {
+  return guard([&] { t->T.randomize(seed); });
+}
+
 int u10_tensor_scale(u10_tensor t, double a) {
   return guard([&] { t->T *= a; });
 }

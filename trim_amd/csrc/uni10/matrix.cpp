@@ -154,9 +154,7 @@ void Matrix::identity() {
 }
 
 void Matrix::randomize(uint64_t seed) {
-  std::vector<double> h(elemNum());
-  for (size_t i = 0; i < h.size(); ++i) h[i] = detail::splitmix_uniform(seed, i + 1);
-  setElem(h);
+  detail::check(tn_fill_uniform_f64(buf.get(), (int64_t)elemNum(), seed, detail::stream()), "randomize");
 }
 
 Matrix& Matrix::resize(size_t row, size_t col) {

@@ -829,9 +829,8 @@ UniTensor& UniTensor::normalize() {
 }
 
 void UniTensor::randomize(uint64_t seed) {
-  std::vector<double> h(m_elemNum);
-  for (size_t i = 0; i < h.size(); ++i) h[i] = detail::splitmix_uniform(seed, i + 1);
-  setElem(h);
+  detail::check(tn_fill_uniform_f64(buf.get(), (int64_t)m_elemNum, seed, detail::stream()), "randomize");
+  status |= HAVEELEM;
 }
 
 void UniTensor::set_zero() {

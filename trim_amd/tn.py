@@ -71,6 +71,7 @@ SIGNATURES = {
     "tn_matmul_c128": [P, P, I64, I64, I64, P, P],
     "tn_norm2_f64": [P, I64, P, P],
     "tn_sum_f64": [P, I64, I64, P, P],
+    "tn_fill_uniform_f64": [P, I64, ctypes.c_uint64, P],
     "tn_axpy_f64": [ctypes.c_double, P, P, I64, P],
     "tn_copy_strided_f64": [P, I64, P, I64, I64, P],
     "tn_scale_f64": [P, I64, ctypes.c_double, P],

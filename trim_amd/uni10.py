@@ -63,6 +63,7 @@ SIGNATURES = {
     "u10_tensor_permute": [P, I, P, I],
     "u10_tensor_transpose": [P],
     "u10_tensor_scale": [P, D],
+    "u10_tensor_randomize": [P, ctypes.c_uint64],
     "u10_tensor_norm": [P, PD],
     "u10_tensor_device_ptr": [P, PP],
     "u10_contract": [P, P, I, PP],
@@ -143,6 +144,11 @@ def _arr(a, dtype):
 
 def set_gemm_mode(mode: int):
     _call("u10_set_gemm_mode", int(mode))
+
+
+def set_stream(stream):
+    """The stream later calls of this thread enqueue on (None: the library's)."""
+    _call("u10_set_stream", stream)
 
 
 def set_autotune(on: bool):
@@ -371,6 +377,10 @@ class UniTensor:
 
     def __imul__(self, a):
         _call("u10_tensor_scale", self.h, float(a))
+        return self
+
+    def randomize(self, seed: int):
+        _call("u10_tensor_randomize", self.h, int(seed))
         return self
 
     def norm(self) -> float:
