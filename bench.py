@@ -380,11 +380,19 @@ def mixed_precision_energy(u10, checkpoints=(20, 2000)):
     return out
 
 
+def serve_budget():
+    """The serving memory budget: 80 % of the device's HBM (288 GB on one
+    MI355X) beyond what the process holds already."""
+    from trim_amd import tn
+    return 0.8 * tn.device_info()["hbm_bytes"]
+
+
 def c5_serve(u10, n_req, rate, chis=(32, 64, 128), mixed=True):
     """Poisson stream of TEBD-step and contraction requests (bond dimension
     log-uniform over chis, 1..50 steps; mixed: d in {2, 3} and 60 % U(1),
     else dense spin 1/2) served by continuous batching over resident slots,
-    FP64 and FP32-MFMA GEMMs."""
+    FP64, FP32-MFMA and exact split-BF16 GEMMs, every request served to the
+    end."""
     from trim_amd import serve
     if mixed:
         classes = [(c, d, s) for c in chis for d in (2, 3) for s in (0, 1)]
@@ -393,24 +401,84 @@ def c5_serve(u10, n_req, rate, chis=(32, 64, 128), mixed=True):
         classes = [(c, 2, 0) for c in chis]
         gen = {}
     out, res = {}, {}
+    probe = None
     for mode, name in ((0, "f64"), (1, "f32"), (4, "bf16_exact")):
-        srv = serve.Server(classes, 16e9, NETDIR, gemm_mode=mode, max_slots=64 if mixed else 256)
-        srv.warmup()
+        srv = serve.Server(classes, serve_budget(), NETDIR, gemm_mode=mode, max_slots=256,
+                           slots_per_engine=64 if mixed else 256, step_ms=probe)
+        probe = srv.step_ms
         reqs = serve.poisson_requests(n_req, rate, seed=5, chi_classes=list(chis), **gen)
         st = srv.run(reqs, realtime=True)
         res[name] = {r.rid: r.result for r in st["done"]}
-        out[name] = {k: st[k] for k in ("requests", "wall_s", "p50_ms", "p99_ms", "tebd_steps_per_s")}
+        out[name] = {k: st[k] for k in ("requests", "failed", "wall_s", "requests_per_s", "p50_ms", "p99_ms",
+                                        "tebd_steps_per_s", "chain_steps_per_s", "engines", "lanes", "host_s")}
         out[name]["slots"] = {"chi%d_d%d_%s" % (c, d, "u1" if s else "dense"): srv.slots((c, d, s))
                               for c, d, s in classes}
+        srv.close()
         del srv
-        progress("serving %s: p99 %.1f ms" % (name, out[name]["p99_ms"]))
+        progress("serving %s: %.0f req/s, p99 %.1f ms" % (name, out[name]["requests_per_s"], out[name]["p99_ms"]))
     diff = max(float(np.max(np.abs(res["f32"][k] - res["f64"][k]))) for k in res["f64"])
     out["max_result_diff_f32_vs_f64"] = diff
     out["max_result_diff_bf16_exact_vs_f64"] = max(float(np.max(np.abs(res["bf16_exact"][k] - res["f64"][k])))
                                                    for k in res["f64"])
     out["workload"] = ("Poisson %.0f req/s, %d requests, chi log-uniform over %s, %s, 1..50 steps, half TEBD "
-                       "half contraction" % (rate, n_req, list(chis),
-                                             "d in {2,3}, 60% U(1) (charges -12..12)" if mixed else "dense spin 1/2"))
+                       "half contraction, served to the end" %
+                       (rate, n_req, list(chis), "d in {2,3}, 60% U(1) (charges -12..12)" if mixed
+                        else "dense spin 1/2"))
+    return out
+
+
+SERVE_RANGE = (32, 64, 128, 256, 512, 1024)
+
+
+def c5_serve_window(u10, rate, window_s, modes=((0, "f64"), (4, "bf16_exact"))):
+    """The serving workload over the whole bond-dimension range 32..1024
+    (classes 32, 64, ..., 1024; d in {2, 3}; 60 % U(1); 1..50 steps; half
+    TEBD, half contraction), Poisson at `rate`, for a fixed serving window:
+    requests that arrive in the window are admitted as slots and the budget
+    allow; at its end the work on the device finishes and the rest counts
+    as unfinished.  Engines start at 8 slots and grow on demand."""
+    from trim_amd import serve
+    classes = [(c, d, s) for c in SERVE_RANGE for d in (2, 3) for s in (0, 1)]
+    n = int(rate * window_s * 1.05) + 16
+    out = {}
+    probe = None
+    for mode, name in modes:
+        t = time.perf_counter()
+        srv = serve.Server(classes, serve_budget(), NETDIR, gemm_mode=mode, max_slots=256, slots_per_engine=64,
+                           step_ms=probe)
+        probe = srv.step_ms
+        t_setup = time.perf_counter() - t
+        reqs = serve.poisson_requests(n, rate, seed=21, chi_classes=list(SERVE_RANGE), d_choices=(2, 3),
+                                      u1_fraction=0.6)
+        st = srv.run(reqs, realtime=True, deadline_s=window_s)
+        # the TEBD work offered in the window, in seconds of one-chain steps
+        # (each class's measured step time): how far beyond one device the
+        # offered load is
+        offered_work = sum(r.steps * srv.step_ms[(r.chi, r.d, r.sym)] * 1e-3 for r in reqs
+                           if r.kind == "tebd" and r.t_arrival < window_s)
+        done = [r for r in st["done"] if r.error is None]
+        by_chi = {}
+        for c in SERVE_RANGE:
+            for kind in ("tebd", "contract"):
+                lat = [r.t_done - r.t_arrival for r in done if r.chi == c and r.kind == kind]
+                by_chi["chi%d_%s" % (c, kind)] = {
+                    "completed": len(lat), "p50_ms": float(np.percentile(lat, 50) * 1e3) if lat else None}
+        out[name] = {"offered": st["offered"], "completed": st["requests"] - st["failed"], "failed": st["failed"],
+                     "unfinished": st["unfinished"], "window_s": window_s, "wall_s": st["wall_s"],
+                     "completed_per_s": (st["requests"] - st["failed"]) / window_s,
+                     "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+                     "chain_steps_per_s": st["chain_steps_per_s"], "engines": st["engines"],
+                     "setup_s": t_setup, "host_s": st["host_s"], "lanes": st["lanes"],
+                     "offered_tebd_one_chain_step_s_per_s": offered_work / window_s,
+                     "step_ms_one_chain": {"chi%d_d%d_%s" % (c[0], c[1], "u1" if c[2] else "dense"): v
+                                           for c, v in srv.step_ms.items()},
+                     "by_chi": by_chi}
+        srv.close()
+        del srv
+        progress("serving window %s: %d of %d offered completed, p99 %s ms" %
+                 (name, out[name]["completed"], st["offered"], out[name]["p99_ms"]))
+    out["workload"] = ("Poisson %.0f req/s offered for a %.0f s window, chi log-uniform over %s, d in {2,3}, "
+                       "60%% U(1), 1..50 steps, half TEBD half contraction" % (rate, window_s, list(SERVE_RANGE)))
     return out
 
 
@@ -549,6 +617,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-chain and serving legs")
     ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-window", type=float, default=15.0,
+                    help="seconds of the serving window over chi 32..1024")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the serving legs, one JSON line")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the headline step, its kernels and the copy calibration (for counter runs)")
     ap.add_argument("--batch-only", type=int, default=0, metavar="STEPS",
@@ -575,6 +647,13 @@ def main():
         return
     if args.c3_only:
         print(json.dumps(c3_sector_only(tn, uni10, args.c3_only, args.c3_mode)), flush=True)
+        return
+    if args.serve_only:
+        out = c5_serve(uni10, args.serve_requests, 2000.0)
+        progress("serving done")
+        out["chi_32_to_1024_window"] = c5_serve_window(uni10, 2000.0, args.serve_window)
+        out["large_chi_single_chain_steps"] = large_chi_steps(uni10)
+        print(json.dumps({"metric": "serving (Poisson 2000 req/s)", "detail": out}), flush=True)
         return
     if args.batch_only:
         out = c4_batch(uni10, list(range(16, 257, 16)), args.batch_only, "16..256 step 16, 32 chains each",
@@ -648,6 +727,8 @@ def main():
         progress("mixed-precision energies done")
         c5 = c5_serve(uni10, args.serve_requests, 2000.0)
         progress("serving done")
+        c5["chi_32_to_1024_window"] = c5_serve_window(uni10, 2000.0, args.serve_window)
+        progress("serving window over chi 32..1024 done")
         c5["spin_half_dense_chi_16_32"] = c5_serve(uni10, args.serve_requests, 2000.0, chis=(16, 32), mixed=False)
         progress("spin-1/2 dense serving done")
         c5["large_chi_single_chain_steps"] = large_chi_steps(uni10)

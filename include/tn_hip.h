@@ -50,6 +50,12 @@ int tn_free(void* dptr);                             /* elemFree              */
 int tn_pool_trim(void);                              /* release cached blocks */
 int tn_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
 int tn_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+/* Pinned host memory and an asynchronous device-to-host copy into it: the
+ * copy is ordered on `stream` and complete once an event recorded after it
+ * is (tn_event_query), so a caller polls results without blocking. */
+int tn_host_alloc(void** p, size_t bytes);
+int tn_host_free(void* p);
+int tn_memcpy_d2h_async(void* dst_pinned, const void* src, size_t bytes, void* stream);
 int tn_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
 int tn_zero(void* dst, size_t bytes, void* stream);  /* elemBzero (fill kernel) */
 int tn_stream_create(void** stream);
@@ -64,6 +70,7 @@ int tn_event_create(void** ev);
 int tn_event_destroy(void* ev);
 int tn_event_record(void* ev, void* stream);
 int tn_event_elapsed_ms(void* ev_start, void* ev_stop, float* ms);
+int tn_event_query(void* ev, int* done);   /* 1 once the recorded work is done; never blocks */
 
 /* HIP graphs: capture every call made on `stream` between begin and end into
  * an executable graph (no host round trips inside: calls that copy from or

@@ -155,6 +155,9 @@ int u10_batch_tebd_wait(u10_batch_tebd b);
  * poll that never blocks; wait still checks the SVDs) */
 int u10_batch_tebd_ready(u10_batch_tebd b, int* ready);
 int u10_batch_tebd_set_graph(u10_batch_tebd b, int on);
+/* The engine's work runs on `stream` (NULL: its own) from now on; waits for
+ * the engine to be idle first. */
+int u10_batch_tebd_set_stream(u10_batch_tebd b, void* stream);
 /* chain on/off: an inactive chain is skipped on the device by every launch
  * (its tensors, spectra and warm-start data are kept) */
 int u10_batch_tebd_set_active(u10_batch_tebd b, int chain, int on);

@@ -307,6 +307,28 @@ int tn_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
   return TN_OK;
 }
 
+int tn_host_alloc(void** p, size_t bytes) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(p != nullptr, TN_EINVAL, "tn_host_alloc: NULL out pointer");
+  *p = nullptr;
+  TN_HIP_CHECK(hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault));
+  return TN_OK;
+}
+
+int tn_host_free(void* p) {
+  if (!p) return TN_OK;
+  TN_HIP_CHECK(hipHostFree(p));
+  return TN_OK;
+}
+
+int tn_memcpy_d2h_async(void* dst_pinned, const void* src, size_t bytes, void* stream) {
+  TN_INIT_OR_RETURN();
+  if (bytes == 0) return TN_OK;
+  TN_REQUIRE(dst_pinned && src, TN_EINVAL, "tn_memcpy_d2h_async: NULL pointer");
+  TN_HIP_CHECK(hipMemcpyAsync(dst_pinned, src, bytes, hipMemcpyDeviceToHost, stream_of(stream)));
+  return TN_OK;
+}
+
 int tn_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
   TN_INIT_OR_RETURN();
   if (bytes == 0) return TN_OK;
@@ -389,6 +411,20 @@ int tn_event_destroy(void* ev) {
 int tn_event_record(void* ev, void* stream) {
   TN_INIT_OR_RETURN();
   TN_HIP_CHECK(hipEventRecord((hipEvent_t)ev, stream_of(stream)));
+  return TN_OK;
+}
+
+int tn_event_query(void* ev, int* done) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(ev && done, TN_EINVAL, "tn_event_query: NULL pointer");
+  const hipError_t e = hipEventQuery((hipEvent_t)ev);
+  if (e == hipErrorNotReady) {
+    (void)hipGetLastError();
+    *done = 0;
+    return TN_OK;
+  }
+  TN_REQUIRE(e == hipSuccess, TN_EHIP, "hipEventQuery: %s", hipGetErrorString(e));
+  *done = 1;
   return TN_OK;
 }
 

@@ -315,7 +315,8 @@ void BatchTebd::build(const std::vector<uint64_t>& seed, double tau, int gemm_mo
   const std::vector<int>& chi = chi_;
   const std::vector<double>& delta = delta_;
   // the engine's own stream (default priority): independent engines overlap
-  detail::check(tn_stream_create(&stream_), "BatchTebd stream");
+  detail::check(tn_stream_create(&own_stream_), "BatchTebd stream");
+  stream_ = own_stream_;
   u1_.resize(nc);
   for (size_t i = 0; i < nc; ++i)
     if (sym_[i]) layout_u1((int)i);
@@ -683,7 +684,10 @@ void BatchTebd::load(int chain, double delta, uint64_t seed) {
   if (chain < 0 || chain >= chains()) detail::fail("BatchTebd::load: bad index");
   std::vector<double> host((size_t)region_[chain]);
   init_chain(chain, delta, seed, host.data());
-  detail::check(tn_memcpy_h2d(arena_.get() + off_[chain].x[0], host.data(), sizeof(double) * host.size(), stream_),
+  // on the engine's own stream: the copy is complete when this returns, and
+  // a stream shared with other engines' steps is not waited for
+  detail::check(tn_memcpy_h2d(arena_.get() + off_[chain].x[0], host.data(), sizeof(double) * host.size(),
+                              own_stream_),
                 "BatchTebd load");
 }
 
@@ -691,6 +695,7 @@ BatchTebd::~BatchTebd() { release(); }
 
 void BatchTebd::release() noexcept {
   if (stream_) tn_stream_sync(stream_);
+  if (own_stream_ && own_stream_ != stream_) tn_stream_sync(own_stream_);
   if (graph_) tn_graph_destroy(graph_);
   graph_ = nullptr;
   for (int p = 0; p < 2; ++p) {
@@ -708,8 +713,14 @@ void BatchTebd::release() noexcept {
   }
   tn_free(active_);
   active_ = nullptr;
-  tn_stream_destroy(stream_);
-  stream_ = nullptr;
+  tn_stream_destroy(own_stream_);
+  own_stream_ = stream_ = nullptr;
+}
+
+void BatchTebd::set_stream(void* stream) {
+  detail::check(tn_stream_sync(stream_), "BatchTebd set_stream");
+  stream_ = stream ? stream : own_stream_;
+  active_dirty_ = true;     // the flags are uploaded again, in the new stream's order
 }
 
 void BatchTebd::update(int p) {

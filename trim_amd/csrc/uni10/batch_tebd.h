@@ -58,6 +58,10 @@ class BatchTebd {
   // An inactive chain is skipped by every launch (its state is kept); takes
   // effect from the next launch.  All chains start active.
   void set_active(int chain, bool on);
+  // Run the engine's work on the caller's stream from now on (NULL: back to
+  // its own); the engine must be idle.  Engines that share a stream run one
+  // after another, in launch order.
+  void set_stream(void* stream);
   double energy(int chain);        // energy per site (networks, one chain)
   // 0: lambda_A, 1: lambda_B (a U(1) chain: sector by sector, charges ascending)
   std::vector<double> lambda(int chain, int which) const;
@@ -107,7 +111,8 @@ class BatchTebd {
   std::vector<Off> off_;
   std::vector<U1Chain> u1_;       // per chain (empty layout for dense chains)
   DevBuf arena_;
-  void* stream_ = nullptr;
+  void* stream_ = nullptr;              // where the engine's work runs
+  void* own_stream_ = nullptr;          // the engine's own stream
   tn_gemm_plan g1_[2] = {nullptr, nullptr}, g2_[2] = {nullptr, nullptr}, g3_[2] = {nullptr, nullptr};
   tn_svd_plan svd_[2] = {nullptr, nullptr};
   int* active_ = nullptr;               // device activity flags, one per chain

@@ -523,6 +523,10 @@ int u10_batch_tebd_set_active(u10_batch_tebd b, int chain, int on) {
   return guard([&] { b->B.set_active(chain, on != 0); });
 }
 
+int u10_batch_tebd_set_stream(u10_batch_tebd b, void* stream) {
+  return guard([&] { b->B.set_stream(stream); });
+}
+
 int u10_batch_tebd_set_graph(u10_batch_tebd b, int on) {
   return guard([&] { b->B.set_graph(on != 0); });
 }

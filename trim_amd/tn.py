@@ -40,6 +40,10 @@ SIGNATURES = {
     "tn_stream_destroy": [P],
     "tn_stream_sync": [P],
     "tn_stream_query": [P, ctypes.POINTER(ctypes.c_int)],
+    "tn_event_query": [P, ctypes.POINTER(ctypes.c_int)],
+    "tn_host_alloc": [ctypes.POINTER(P), ctypes.c_size_t],
+    "tn_host_free": [P],
+    "tn_memcpy_d2h_async": [P, P, ctypes.c_size_t, P],
     "tn_device_sync": [],
     "tn_event_create": [ctypes.POINTER(ctypes.c_void_p)],
     "tn_event_destroy": [P],
@@ -370,6 +374,17 @@ class Event:
 
     def record(self, stream=None):
         call("tn_event_record", ctypes.c_void_p(self.h), stream)
+
+    def query(self) -> bool:
+        """True once the work recorded before the event is done (never blocks)."""
+        done = ctypes.c_int()
+        call("tn_event_query", ctypes.c_void_p(self.h), ctypes.byref(done))
+        return bool(done.value)
+
+    def destroy(self):
+        if self.h:
+            call("tn_event_destroy", ctypes.c_void_p(self.h))
+            self.h = None
 
     def elapsed_ms(self, later: "Event") -> float:
         ms = ctypes.c_float()

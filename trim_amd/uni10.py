@@ -100,6 +100,7 @@ SIGNATURES = {
     "u10_batch_tebd_wait": [P],
     "u10_batch_tebd_ready": [P, PI],
     "u10_batch_tebd_set_graph": [P, I],
+    "u10_batch_tebd_set_stream": [P, P],
     "u10_batch_tebd_set_active": [P, I, I],
     "u10_batch_tebd_load": [P, I, D, ctypes.c_uint64],
     "u10_batch_tebd_energy": [P, I, PD],
@@ -592,6 +593,10 @@ class BatchTebd:
 
     def set_graph(self, on: bool = True):
         _call("u10_batch_tebd_set_graph", self.h, int(bool(on)))
+
+    def set_stream(self, stream):
+        """Run the engine's work on `stream` (None: its own stream)."""
+        _call("u10_batch_tebd_set_stream", self.h, stream)
 
     def set_active(self, chain: int, on: bool = True):
         """An inactive chain is skipped on the device (its state is kept)."""
