@@ -247,7 +247,13 @@ def c3_sector_only(tn, u10, steps, mode=0):
     info = plan.info()
     pairs = {}
     t = time_parts(tn, [("sector", lambda: plan(pa, pb, C))], steps, pairs)["sector"]
+    per_cfg = {}
+    for cfg in range(8 if mode in (0, 1) else (4 if mode == 4 else 5)):
+        plan.set_tile(cfg)
+        per_cfg[cfg] = time_parts(tn, [("c", lambda: plan(pa, pb, C))], steps, {})["c"]
+    plan.set_tile(info["tile_id"])
     return {"sectors": len(descs), "mode": mode, "tiles": info["tiles"], "tile_cfg": info["tile_id"],
+            "kernel_ms_per_tile_cfg": per_cfg,
             "flop": info["flops"], "kernel_ms": t, "kernel_ms_event_pairs": pairs["sector"],
             "tflops": info["flops"] / (t * 1e-3) / 1e12,
             "shapes_MNK": [(d[0], d[1], d[2]) for d in descs]}

@@ -258,6 +258,55 @@ def test_gemm_tile_choice_is_bitwise_neutral(tn, mode):
     assert ms > 0 and dC.to_host().tobytes() == outs[0]
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_gemm_tile_choice_is_bitwise_neutral_paired_loads(tn, mode):
+    """The same with every K and N even, so the tiles read A and B pairs
+    with 16-byte loads; K = 640 runs twenty 32-k chunks, K = 6 one partial
+    chunk."""
+    shapes = [(200, 130, 300), (18, 34, 6), (64, 64, 640), (50, 96, 256)]
+    descs, A, B, nc = _group_case(78, shapes)
+    plan = tn.GemmPlan(mode, np.array(descs, dtype=tn.GEMM_DESC))
+    dA, dB = tn.DeviceArray.from_host(A), tn.DeviceArray.from_host(B)
+    outs = []
+    for tile in range(NTILES[mode]):
+        plan.set_tile(tile)
+        dC = tn.DeviceArray((nc,))
+        plan(dA, dB, dC)
+        outs.append(dC.to_host())
+    assert all(o.tobytes() == outs[0].tobytes() for o in outs)
+    ref = np.concatenate([(A[d[3]:d[3] + d[0] * d[2]].reshape(d[0], d[2]) @
+                           B[d[4]:d[4] + d[2] * d[1]].reshape(d[2], d[1])).ravel() for d in descs])
+    assert np.linalg.norm(outs[0] - ref) <= GEMM_TOL[mode] * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_gemm_index_maps_every_tile_same_bits(tn, mode):
+    """Multi-level index maps on every tile shape: identical bits."""
+    rng = np.random.default_rng(15 + mode)
+    a = rng.uniform(-1, 1, (6, 5, 7, 4))
+    M, K, N = 6 * 7, 5 * 4, 9
+    bt = rng.uniform(-1, 1, (N, K))
+    st = [5 * 7 * 4, 7 * 4, 4, 1]
+    desc = np.zeros(1, dtype=tn.GEMM_DESC_EX)
+    desc["M"], desc["N"], desc["K"] = M, N, K
+    desc["lda"], desc["ldb"], desc["ldc"] = K, N, N
+    desc["a_row"] = tn.index_map([(6, st[0]), (7, st[2])])
+    desc["a_col"] = tn.index_map([(5, st[1]), (4, st[3])])
+    desc["b_row"] = tn.index_map([(K, 1)])
+    desc["b_col"] = tn.index_map([(N, K)])
+    desc["c_row"] = tn.index_map([(M, 1)])
+    desc["c_col"] = tn.index_map([(N, M)])
+    plan = tn.GemmPlan(mode, desc)
+    dA, dB = tn.DeviceArray.from_host(a), tn.DeviceArray.from_host(bt)
+    outs = []
+    for tile in range(NTILES[mode]):
+        plan.set_tile(tile)
+        dC = tn.DeviceArray((N, M))
+        plan(dA, dB, dC)
+        outs.append(dC.to_host().tobytes())
+    assert all(o == outs[0] for o in outs)
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
 def test_gemm_index_maps(tn, mode):
     """Multi-stride operands equal the same GEMM on explicitly permuted
