@@ -636,6 +636,9 @@ def main():
                     help="only the grouped U(1) chi=1024 A1.A2 sector GEMM, STEPS timed launches, one JSON line "
                          "(for kernel-trace and counter runs)")
     ap.add_argument("--c3-mode", type=int, default=0, help="GEMM mode of --c3-only (0 FP64)")
+    ap.add_argument("--tebd-only", type=int, default=0, metavar="STEPS",
+                    help="only the single-chain chi=32 iTEBD through the drop-in API for STEPS steps, one JSON "
+                         "line (for kernel-trace runs)")
     ap.add_argument("--lanczos-only", action="store_true",
                     help="only the Lanczos leg (8192 x 8192), one JSON line (for kernel-trace runs)")
     args = ap.parse_args()
@@ -648,6 +651,11 @@ def main():
     tn.init(0)
     uni10.load()
     info = tn.device_info()
+    if args.tebd_only:
+        sps, e, te = tebd_rate(uni10, args.tebd_only)
+        print(json.dumps({"metric": "iTEBD steps/s (chi=32, d=2, drop-in API)", "value": sps, "unit": "steps/s",
+                          "energy": e, "trunc_err": te}), flush=True)
+        return
     if args.lanczos_only:
         print(json.dumps(lanczos_leg(tn)), flush=True)
         return

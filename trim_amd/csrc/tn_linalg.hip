@@ -866,13 +866,14 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
       if (qr) permute_sorted();   // no V to carry along
       int rotated = 0;
       // the pair of the next round is looked up before the barrier (the
-      // positions do not change within a sweep)
+      // positions do not change within a sweep); after permute_sorted (the
+      // QR path) position = column, so no table read is needed
       auto pair_of = [&](int r, int& p, int& q) {
         if (g >= npairs) return false;
         int pp, qq;
         rr_pair_fast(r, g, np, pp, qq);
-        p = col_at[pp];
-        q = col_at[qq];
+        p = qr ? pp : col_at[pp];
+        q = qr ? qq : col_at[qq];
         return p < n && q < n;
       };
       int pn = 0, qn = 0;
@@ -909,6 +910,10 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
             xq[e] = v.x;
             xq[e + 1] = v.y;
           }
+          // the two norms are read with the columns, so their LDS latency
+          // overlaps the dot product instead of following its reduction
+          const double a = nrm[p], b = nrm[q];
+          __builtin_amdgcn_sched_barrier(0);
           double ca = 0.0, cb = 0.0;
 #pragma unroll
           for (int e = 0; e < EMX; e += 2) {
@@ -916,7 +921,6 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
             cb = fma(xp[e + 1], xq[e + 1], cb);
           }
           const double c = group8_sum(ca + cb);
-          const double a = nrm[p], b = nrm[q];
           if (c * c > tol * tol * (a * b) && fabs(c) > abs_tol) {
             rotated = 1;
             // the tangent from the raw reciprocal and square-root estimates

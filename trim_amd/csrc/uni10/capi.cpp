@@ -425,6 +425,7 @@ int u10_itebd_set_graph(u10_itebd t, int on) {
 int u10_itebd_step(u10_itebd t, int n) {
   return guard([&] {
     for (int i = 0; i < n; ++i) t->E.step();
+    t->E.check();
   });
 }
 

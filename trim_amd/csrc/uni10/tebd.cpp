@@ -105,23 +105,55 @@ ITebd::ITebd(int chi, int d, double delta, double tau, uint64_t seed, const std:
 // theta on the same bond (the two updates of a step alternate between two
 // bonds): successive thetas differ by one small imaginary-time step, so the
 // Jacobi sweeps start close to convergence.  The factors of Matrix::svd that
-// the update uses: S (k x k diagonal) and VT (k x C); U is not formed.
-static std::vector<Matrix> svd_warm(const Matrix& T, Matrix& vt_prev) {
+// the update uses: S (k x k diagonal) and VT (k x C); U is not formed.  The
+// warm-started SVD of each bond runs through a plan built once for its shape:
+// no allocation, copy or wait per update; its convergence is checked by
+// check().
+std::vector<Matrix> ITebd::svd_warm(const Matrix& T) {
+  Matrix& vt_prev = vt_prev_[parity_];
   const size_t R = T.row(), C = T.col(), k = std::min(R, C);
   std::vector<Matrix> outs;
   outs.emplace_back(k, k, true);
   outs.emplace_back(k, C, false);
   if (k == 0) return outs;
-  if (R >= C && vt_prev.row() == C && vt_prev.col() == C)
-    detail::check(tn_svd_f64_warm(T.getElem(), (int64_t)R, (int64_t)C, vt_prev.getElem(), nullptr,
-                                  outs[0].getElem(), outs[1].getElem(), detail::stream()),
+  if (R >= C && vt_prev.row() == C && vt_prev.col() == C) {
+    tn_svd_plan& P = plan_[parity_];
+    if (!P || plan_shape_[parity_][0] != (int64_t)R || plan_shape_[parity_][1] != (int64_t)C) {
+      if (P) {
+        check();
+        tn_svd_plan_destroy(P);
+        P = nullptr;
+      }
+      const int64_t m = (int64_t)R, n = (int64_t)C, z = 0, none = -1;
+      detail::check(tn_svd_plan_create(&P, 1, &m, &n, &z, &z, &none, &z, &z), "ITebd svd plan");
+      plan_shape_[parity_][0] = m;
+      plan_shape_[parity_][1] = n;
+    }
+    detail::check(tn_svd_plan_exec(P, T.getElem(), vt_prev.getElem(), nullptr, outs[0].getElem(),
+                                   outs[1].getElem(), detail::stream()),
                   "ITebd svd");
-  else
+  } else {
     detail::check(tn_svd_f64(T.getElem(), (int64_t)R, (int64_t)C, nullptr, outs[0].getElem(),
                              outs[1].getElem(), detail::stream()),
                   "ITebd svd");
+  }
   vt_prev = outs[1];
   return outs;
+}
+
+ITebd::~ITebd() {
+  for (tn_svd_plan& P : plan_) tn_svd_plan_destroy(P);
+}
+
+void ITebd::check() {
+  int bad = 0;
+  for (tn_svd_plan P : plan_) {
+    if (!P) continue;
+    int nonconv = 0, sw = 0;
+    detail::check(tn_svd_plan_check(P, &nonconv, &sw, detail::stream()), "ITebd svd");
+    bad += nonconv;
+  }
+  if (bad) detail::fail("ITebd: " + std::to_string(bad) + " SVD(s) did not converge");
 }
 
 void ITebd::update(UniTensor& ALa, UniTensor& BLb, Matrix& La, Matrix& Lb) {
@@ -133,7 +165,7 @@ void ITebd::update(UniTensor& ALa, UniTensor& BLb, Matrix& La, Matrix& Lb) {
   UniTensor Theta(C.bond(), "Theta");
   Theta.putBlock(q0, Lb * C.getBlock(q0));
   Theta.permute(2);
-  std::vector<Matrix> rets = svd_warm(Theta.getBlock(q0), vt_prev_[parity_]);
+  std::vector<Matrix> rets = svd_warm(Theta.getBlock(q0));
   parity_ ^= 1;
   const int k = (int)rets[0].row();
   const int dim = chi_ < k ? chi_ : k;
@@ -200,6 +232,9 @@ double itebd_energy(UniTensor& ALa, UniTensor& BLb, const Matrix& La, const Matr
   return val / 2;
 }
 
-double ITebd::energy() { return itebd_energy(ALa_, BLb_, La_, Lb_, H_, MPS_, meas_); }
+double ITebd::energy() {
+  check();
+  return itebd_energy(ALa_, BLb_, La_, Lb_, H_, MPS_, meas_);
+}
 
 }  // namespace uni10

@@ -3,6 +3,7 @@
 #include <cstdint>
 #include <string>
 
+#include "tn_hip.h"
 #include "uni10_amd/uni10.hpp"
 
 namespace uni10 {
@@ -19,7 +20,14 @@ double itebd_energy(UniTensor& ALa, UniTensor& BLb, const Matrix& La, const Matr
 class ITebd {
  public:
   ITebd(int chi, int d, double delta, double tau, uint64_t seed, const std::string& netdir);
+  ~ITebd();
+  ITebd(const ITebd&) = delete;
+  ITebd& operator=(const ITebd&) = delete;
   void step();                 // update(A,B) then update(B,A)
+  // Waits for the device and throws if an SVD since the last check did not
+  // converge (the updates themselves never wait for the host); energy()
+  // checks first.
+  void check();
   double energy();             // mean of the two bond energies
   double truncation_error() const;   // discarded weight of the last update
   // replay the update networks as captured graphs (Network::setGraph)
@@ -34,6 +42,10 @@ class ITebd {
 
  private:
   void update(UniTensor& ALa, UniTensor& BLb, Matrix& La, Matrix& Lb);
+  std::vector<Matrix> svd_warm(const Matrix& T);
+  // warm-started SVD plan of each bond's theta (kept while the shape holds)
+  tn_svd_plan plan_[2] = {nullptr, nullptr};
+  int64_t plan_shape_[2][2] = {{-1, -1}, {-1, -1}};
   int chi_, d_;
   double tau_;
   DevBuf scal_{2, true};        // ||S[:dim]|| and the discarded weight of the last update
