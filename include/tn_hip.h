@@ -318,6 +318,13 @@ int tn_svd_plan_destroy(tn_svd_plan plan);
  * eigenvalues ascending; eigenvectors returned as ROWS of EigVec. */
 int tn_eigh_f64(const double* A, int64_t N, double* eig, double* eigvec_rows,
                 void* stream);
+/* Many eigSyDecompose calls (uni10_lapack_cpu.cpp:131-152) in one launch,
+ * one workgroup per matrix: problem i is the n[i] x n[i] symmetric matrix at
+ * A + a_off[i], eigenvalues ascending to eig + e_off[i], eigenvector rows to
+ * vec_rows + v_off[i] (n[i] <= 90; n[i] = 0 is skipped).  Waits for the
+ * stream; TN_ENOCONV names how many did not converge. */
+int tn_eigh_f64_batched(int64_t batch, const int64_t* n, const double* A, const int64_t* a_off, double* eig,
+                        const int64_t* e_off, double* vec_rows, const int64_t* v_off, void* stream);
 
 /* lanczosEV (uni10_lapack_cpu.cpp:416-509; the reference's GPU entry
  * uni10_lapack_gpu.cu:681-687 only throws): lowest eigenpair of the n x n

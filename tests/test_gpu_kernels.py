@@ -801,6 +801,51 @@ def test_eigh_random(tn, ora, n):
     assert np.max(np.abs(v @ a @ v.T - np.diag(w))) <= 1e-12 * np.abs(wr).max()
 
 
+def test_eigh_batched_matches_single_and_oracle(tn, ora):
+    """One launch over matrices of uneven size (n = 0 skipped) gives the
+    bits of one launch per matrix, and each equals dsyev's eigenpairs."""
+    ns = [4, 0, 9, 1, 16, 64, 90, 9]
+    mats = []
+    for i, n in enumerate(ns):
+        a = _rand(40 + i, (n, n)) if n else np.zeros((0, 0))
+        mats.append(a + a.T)
+    ao, eo, vo = [], [], []
+    flat, e_n, v_n = [], 0, 0
+    for a in mats:
+        ao.append(sum(x.size for x in flat))
+        flat.append(a.ravel())
+        eo.append(e_n)
+        vo.append(v_n)
+        e_n += a.shape[0]
+        v_n += a.size
+    dA = tn.DeviceArray.from_host(np.concatenate(flat))
+    de, dv = tn.DeviceArray((max(e_n, 1),)), tn.DeviceArray((max(v_n, 1),))
+    keep = [np.array(x, np.int64) for x in (ns, ao, eo, vo)]
+    p64 = [x.ctypes.data_as(tn.PI64) for x in keep]
+    tn.call("tn_eigh_f64_batched", len(ns), p64[0], tn._p(dA), p64[1], tn._p(de), p64[2], tn._p(dv), p64[3],
+            None)
+    e, v = de.to_host(), dv.to_host()
+    for i, (n, a) in enumerate(zip(ns, mats)):
+        if n == 0:
+            continue
+        w = e[eo[i]:eo[i] + n]
+        vec = v[vo[i]:vo[i] + n * n].reshape(n, n)
+        w1, v1 = (x.to_host() for x in tn.eigh(tn.DeviceArray.from_host(a), n))
+        assert w.tobytes() == w1.tobytes() and vec.tobytes() == v1.tobytes()
+        wr, _ = ora.eigh(a)
+        assert np.max(np.abs(w - wr)) <= 1e-12 * np.abs(wr).max()
+        assert np.max(np.abs(vec @ a @ vec.T - np.diag(w))) <= 1e-12 * np.abs(wr).max()
+
+
+def test_eigh_batched_rejects_large_n(tn):
+    a = tn.DeviceArray((91 * 91,))
+    e, v = tn.DeviceArray((91,)), tn.DeviceArray((91 * 91,))
+    z, n = np.zeros(1, np.int64), np.array([91], np.int64)
+    pz, pn = z.ctypes.data_as(tn.PI64), n.ctypes.data_as(tn.PI64)
+    with pytest.raises(tn.TNError):
+        tn.call("tn_eigh_f64_batched", 1, pn, tn._p(a), pz, tn._p(e), pz, tn._p(v), pz, None)
+
+
 # ---------------------------------------------------------------------------
 def test_vector_ops(tn):
     x = _rand(9, 12345)

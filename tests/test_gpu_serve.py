@@ -46,7 +46,7 @@ def test_result_same_alone_and_in_mix(serve, golden):
 
 def test_tebd_request_follows_oracle(serve, ora, golden):
     reqs = [serve.Request(rid=0, t_arrival=0.0, kind="tebd", chi=8, delta=0.8, seed=77, steps=5)]
-    srv = serve.Server([8], 4 * serve.slot_bytes(8), golden)
+    srv = serve.Server([8], 6 * serve.slot_bytes(8), golden)
     r = srv.run(reqs, realtime=False)["done"][0]
     O = ora.ITEBD(8, 2, ora.heisenberg_two_site(0.8), 0.01, 77, golden)
     for _ in range(5):
@@ -73,9 +73,10 @@ def test_mixed_classes_same_alone_and_in_mix(serve, ora, golden):
              for d in (2, 3)]
     for target in picks:
         solo = serve.poisson_requests(**gen)[target.rid]
-        out = serve.Server(classes, budget, golden).run([solo], realtime=False)["done"][0].result
+        r1 = serve.Server(classes, budget, golden).run([solo], realtime=False)["done"][0]
         _say("request %d (%s chi=%d d=%d sym=%d)" % (target.rid, target.kind, target.chi, target.d, target.sym))
-        assert out.tobytes() == mixed[target.rid].tobytes()
+        assert r1.error is None, r1.error
+        assert r1.result.tobytes() == mixed[target.rid].tobytes()
     t = next(r for r in picks if r.kind == "tebd" and r.d == 3)
     O = ora.ITEBDU1(t.chi, 3, t.delta, 0.01, t.seed, golden)
     for _ in range(t.steps):
@@ -141,7 +142,7 @@ def test_over_budget_requests_queue(serve, tn, golden):
     one from a server with room to grow (four engines of two slots)."""
     import ctypes
     reqs = _burst(serve, 8, 16)
-    tight = serve.Server([16], 3 * serve.slot_bytes(16), golden, slots_per_engine=2)
+    tight = serve.Server([16], 3.5 * serve.slot_bytes(16), golden, slots_per_engine=2)
     try:
         assert tight.engine_count(16) == 1 and tight.slots(16) == 2
         tn.call("tn_pool_trim")
@@ -172,7 +173,7 @@ def test_budget_applies_only_to_server_allocations(serve, tn, golden):
     import ctypes
     b = ctypes.c_size_t()
     tn.call("tn_set_budget", 0)
-    srv = serve.Server([8], 4 * serve.slot_bytes(8), golden)
+    srv = serve.Server([8], 6 * serve.slot_bytes(8), golden)
     srv.run(_burst(serve, 2, 8, steps=2) + _burst(serve, 1, 8, kind="contract", seed0=950), realtime=False)
     tn.call("tn_mem_stats", None, None, None, ctypes.byref(b))
     assert b.value == 0 and srv.budget > 0
@@ -219,7 +220,7 @@ def test_failed_engine_is_isolated(serve, tn, golden):
 def test_idle_slots_are_switched_off(serve, golden):
     """Between requests every slot is inactive: a step of an engine holding
     one request computes one chain, and an idle server launches nothing."""
-    srv = serve.Server([8], 8 * serve.slot_bytes(8), golden, slots_per_engine=4)
+    srv = serve.Server([8], 12 * serve.slot_bytes(8), golden, slots_per_engine=4)
     try:
         e = srv.engines[(8, 2, 0)][0]
         assert not e.busy and not e.inflight

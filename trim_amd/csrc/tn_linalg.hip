@@ -258,11 +258,23 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_kernel(const SvdProb* __r
 }
 
 // ---------------------------------------------------------------------------
-// Parallel two-sided Jacobi for a symmetric n x n matrix (n <= 256).
-__global__ void __launch_bounds__(kThreads) eigh_jacobi_kernel(const double* __restrict__ A, int n,
-                                                               double* __restrict__ eig,
-                                                               double* __restrict__ vec_rows,
+// Parallel two-sided Jacobi for a symmetric n x n matrix (n <= 90: both
+// matrices in LDS), one workgroup per matrix: problem blockIdx.x of probs,
+// or `one` when probs is NULL.
+struct EighProb {
+  int64_t n, a_off, e_off, v_off;
+};
+__global__ void __launch_bounds__(kThreads) eigh_jacobi_kernel(const EighProb* __restrict__ probs, EighProb one,
+                                                               const double* __restrict__ A0,
+                                                               double* __restrict__ eig0,
+                                                               double* __restrict__ vec0,
                                                                int* __restrict__ status) {
+  const EighProb pr = probs ? probs[blockIdx.x] : one;
+  const int n = (int)pr.n;
+  if (n <= 0) return;
+  const double* __restrict__ A = A0 + pr.a_off;
+  double* __restrict__ eig = eig0 + pr.e_off;
+  double* __restrict__ vec_rows = vec0 + pr.v_off;
   extern __shared__ double lds[];
   double* a = lds;              // n x n row-major (symmetric)
   double* v = lds + n * n;      // n x n, columns = eigenvectors
@@ -1450,25 +1462,23 @@ int tn_svd_plan_destroy(tn_svd_plan P) {
   return TN_OK;
 }
 
-int tn_eigh_f64(const double* A, int64_t N, double* eig, double* vec_rows, void* stream) {
-  TN_INIT_OR_RETURN();
-  TN_REQUIRE(A && eig && vec_rows, TN_EINVAL, "tn_eigh_f64: NULL buffer");
-  TN_REQUIRE(N >= 1 && N <= 90, TN_EUNSUPP, "tn_eigh_f64: N=%lld outside [1, 90]", (long long)N);
-  hipStream_t st = tn::stream_of(stream);
+// Launches the eigh kernel for `batch` problems (probs on the device, or
+// `one` for a single problem) and reports how many did not converge.
+static int eigh_launch(int64_t batch, const EighProb* d_probs, EighProb one, int64_t nmax, const double* A,
+                       double* eig, double* vec_rows, hipStream_t st) {
   int* d_status = nullptr;
   int rc = tn::pool_alloc((void**)&d_status, sizeof(int));
   if (rc != TN_OK) return rc;
   rc = tn_zero(d_status, sizeof(int), st);
   if (rc == TN_OK) {
-    size_t lds = sizeof(double) * 2 * N * N;
-    static bool attr_set = false;
-    if (!attr_set) {
-      (void)hipFuncSetAttribute((const void*)eigh_jacobi_kernel,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 140 * 1024);
+    size_t lds = sizeof(double) * 2 * nmax * nmax;
+    static std::once_flag attr;
+    std::call_once(attr, [] {
+      (void)hipFuncSetAttribute((const void*)eigh_jacobi_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                140 * 1024);
       (void)hipGetLastError();
-      attr_set = true;
-    }
-    hipLaunchKernelGGL(eigh_jacobi_kernel, dim3(1), dim3(kThreads), lds, st, A, (int)N, eig,
+    });
+    hipLaunchKernelGGL(eigh_jacobi_kernel, dim3((unsigned)batch), dim3(kThreads), lds, st, d_probs, one, A, eig,
                        vec_rows, d_status);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -1480,8 +1490,42 @@ int tn_eigh_f64(const double* A, int64_t N, double* eig, double* vec_rows, void*
   if (rc == TN_OK) rc = tn_memcpy_d2h(&h, d_status, sizeof(int), st);
   tn::pool_free(d_status);
   if (rc != TN_OK) return rc;
-  TN_REQUIRE(h == 0, TN_ENOCONV, "eigh: Jacobi did not converge");
+  TN_REQUIRE(h == 0, TN_ENOCONV, "eigh: Jacobi did not converge for %d matri%s", h, h == 1 ? "x" : "ces");
   return TN_OK;
+}
+
+int tn_eigh_f64(const double* A, int64_t N, double* eig, double* vec_rows, void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(A && eig && vec_rows, TN_EINVAL, "tn_eigh_f64: NULL buffer");
+  TN_REQUIRE(N >= 1 && N <= 90, TN_EUNSUPP, "tn_eigh_f64: N=%lld outside [1, 90]", (long long)N);
+  return eigh_launch(1, nullptr, EighProb{N, 0, 0, 0}, N, A, eig, vec_rows, tn::stream_of(stream));
+}
+
+int tn_eigh_f64_batched(int64_t batch, const int64_t* n, const double* A, const int64_t* a_off, double* eig,
+                        const int64_t* e_off, double* vec_rows, const int64_t* v_off, void* stream) {
+  TN_INIT_OR_RETURN();
+  TN_REQUIRE(batch >= 0, TN_EINVAL, "tn_eigh_f64_batched: negative batch");
+  if (batch == 0) return TN_OK;
+  TN_REQUIRE(n && A && a_off && eig && e_off && vec_rows && v_off, TN_EINVAL, "tn_eigh_f64_batched: NULL argument");
+  TN_REQUIRE(batch < (int64_t(1) << 31), TN_EUNSUPP, "tn_eigh_f64_batched: batch too large");
+  std::vector<EighProb> h((size_t)batch);
+  int64_t nmax = 0;
+  for (int64_t i = 0; i < batch; ++i) {
+    TN_REQUIRE(n[i] >= 0 && n[i] <= 90, TN_EUNSUPP, "tn_eigh_f64_batched: n[%lld]=%lld outside [0, 90]",
+               (long long)i, (long long)n[i]);
+    TN_REQUIRE(a_off[i] >= 0 && e_off[i] >= 0 && v_off[i] >= 0, TN_EINVAL, "tn_eigh_f64_batched: negative offset");
+    h[i] = EighProb{n[i], a_off[i], e_off[i], v_off[i]};
+    nmax = std::max(nmax, n[i]);
+  }
+  if (nmax == 0) return TN_OK;
+  hipStream_t st = tn::stream_of(stream);
+  EighProb* d = nullptr;
+  int rc = tn::pool_alloc((void**)&d, sizeof(EighProb) * (size_t)batch);
+  if (rc != TN_OK) return rc;
+  rc = tn_memcpy_h2d(d, h.data(), sizeof(EighProb) * (size_t)batch, st);
+  if (rc == TN_OK) rc = eigh_launch(batch, d, EighProb{0, 0, 0, 0}, nmax, A, eig, vec_rows, st);
+  tn::pool_free(d);
+  return rc;
 }
 
 int tn_sum_f64(const double* x, int64_t n, int64_t inc, double* out_dev, void* stream) {

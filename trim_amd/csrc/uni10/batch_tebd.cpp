@@ -370,10 +370,13 @@ void BatchTebd::build(const std::vector<uint64_t>& seed, double tau, int gemm_mo
   // initial state of every chain and its gate
   tau_ = tau;
   std::vector<double> host((size_t)o, 0.0);
+  std::vector<int> all(nc);
+  for (size_t i = 0; i < nc; ++i) all[i] = (int)i;
+  const std::vector<std::vector<double>> g = gates(all, delta);
   for (size_t i = 0; i < nc; ++i) {
     const int64_t end = (i + 1 < nc) ? off_[i + 1].x[0] : o;
     region_.push_back(end - off_[i].x[0]);
-    init_chain((int)i, delta[i], seed[i], host.data() + off_[i].x[0]);
+    init_chain((int)i, delta[i], seed[i], host.data() + off_[i].x[0], &g[i]);
   }
 
   // every chain starts active (device flags read by every launch)
@@ -592,9 +595,53 @@ void BatchTebd::plans_u1(int i, int p, std::vector<tn_gemm_desc_ex>& d1, std::ve
   flops_ += fl;
 }
 
-std::vector<double> BatchTebd::gate(int i, double delta) const {
-  const Matrix H = (d_[i] == 2) ? xxz_two_site(delta, 1.0) : xxz_two_site_spin1(delta, 1.0);
-  return takeExp(-tau_, H).getHostElem();
+// exp(-tau H(delta_j)) of chains[j], all in one batched eigendecomposition
+// (H = U^T diag(lambda) U with U the eigenvector rows, as exph,
+// MatrixTools.cpp:109-121); the products exp(-tau H) = U^T diag(e) U are
+// formed on the host in a fixed order, so a chain's gate is the same bits
+// whether it is built with the others or alone (a reloaded slot).
+std::vector<std::vector<double>> BatchTebd::gates(const std::vector<int>& chains,
+                                                  const std::vector<double>& delta) const {
+  const size_t nb = chains.size();
+  std::vector<int64_t> n(nb), ao(nb), eo(nb), vo(nb);
+  std::vector<double> hs;
+  int64_t ne = 0, nv = 0;
+  for (size_t j = 0; j < nb; ++j) {
+    const int kd = d_[chains[j]];
+    const std::vector<double> h = (kd == 2) ? xxz_two_site_host(delta[j], 1.0) : xxz_two_site_spin1_host(delta[j], 1.0);
+    n[j] = kd * kd;
+    ao[j] = (int64_t)hs.size();
+    eo[j] = ne;
+    vo[j] = nv;
+    hs.insert(hs.end(), h.begin(), h.end());
+    ne += n[j];
+    nv += n[j] * n[j];
+  }
+  std::vector<std::vector<double>> out(nb);
+  if (nb == 0) return out;
+  DevBuf dh((size_t)hs.size()), de((size_t)ne), dv((size_t)nv);
+  detail::check(tn_memcpy_h2d(dh.get(), hs.data(), sizeof(double) * hs.size(), own_stream_), "BatchTebd gates");
+  detail::check(tn_eigh_f64_batched((int64_t)nb, n.data(), dh.get(), ao.data(), de.get(), eo.data(), dv.get(),
+                                    vo.data(), own_stream_),
+                "BatchTebd gates");
+  std::vector<double> he((size_t)ne), hv((size_t)nv);
+  detail::check(tn_memcpy_d2h(he.data(), de.get(), sizeof(double) * he.size(), own_stream_), "BatchTebd gates");
+  detail::check(tn_memcpy_d2h(hv.data(), dv.get(), sizeof(double) * hv.size(), own_stream_), "BatchTebd gates");
+  for (size_t j = 0; j < nb; ++j) {
+    const int64_t m = n[j];
+    const double* lam = he.data() + eo[j];
+    const double* U = hv.data() + vo[j];   // row k: eigenvector k
+    std::vector<double> ex((size_t)m), g((size_t)(m * m), 0.0);
+    for (int64_t k = 0; k < m; ++k) ex[k] = std::exp(-tau_ * lam[k]);
+    for (int64_t r = 0; r < m; ++r)
+      for (int64_t c = 0; c < m; ++c) {
+        double acc = 0.0;
+        for (int64_t k = 0; k < m; ++k) acc += U[k * m + r] * (ex[k] * U[k * m + c]);
+        g[r * m + c] = acc;
+      }
+    out[j] = std::move(g);
+  }
+  return out;
 }
 
 // Host image of chain i's arena region (base = its x[0]): Gamma lambda
@@ -603,7 +650,7 @@ std::vector<double> BatchTebd::gate(int i, double delta) const {
 // an earlier occupant).  A dense chain starts as ITebd (splitmix seeds
 // seed..seed+3); a U(1) chain from the same dense Gammas restricted to its
 // charge structure, with each lambda sorted within its sectors.
-void BatchTebd::init_chain(int i, double delta, uint64_t seed, double* base) {
+void BatchTebd::init_chain(int i, double delta, uint64_t seed, double* base, const std::vector<double>* gate) {
   const int c = chi_[i], kd = d_[i];
   const Off& f = off_[i];
   auto at = [&](int64_t off) { return base + (off - f.x[0]); };
@@ -675,7 +722,7 @@ void BatchTebd::init_chain(int i, double delta, uint64_t seed, double* base) {
       }
     }
   }
-  const std::vector<double> g = gate(i, delta);
+  const std::vector<double> g = gate ? *gate : gates({i}, {delta})[0];
   std::copy(g.begin(), g.end(), at(f.g));
   delta_[i] = delta;
 }

@@ -101,8 +101,8 @@ class BatchTebd {
   void update(int p);
   void release() noexcept;   // streams, plans, graph and flags (the arena frees itself)
   void build(const std::vector<uint64_t>& seed, double tau, int gemm_mode);
-  void init_chain(int i, double delta, uint64_t seed, double* base);
-  std::vector<double> gate(int i, double delta) const;
+  void init_chain(int i, double delta, uint64_t seed, double* base, const std::vector<double>* gate = nullptr);
+  std::vector<std::vector<double>> gates(const std::vector<int>& chains, const std::vector<double>& delta) const;
   std::vector<int64_t> region_;   // arena doubles of each chain (from its x[0])
   double tau_ = 0;
   void run_step();

@@ -21,16 +21,26 @@ std::string detail::netdir_or_default(const std::string& netdir) {
   return (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) + "/nets";
 }
 
-Matrix xxz_two_site(double delta, double J) {
+std::vector<double> xxz_two_site_host(double delta, double J) {
   // J (Sx Sx + Sy Sy + delta Sz Sz), basis |uu>,|ud>,|du>,|dd>
   std::vector<double> h(16, 0.0);
   h[0] = h[15] = 0.25 * delta * J;
   h[5] = h[10] = -0.25 * delta * J;
   h[6] = h[9] = 0.5 * J;
+  return h;
+}
+
+Matrix xxz_two_site(double delta, double J) {
+  const std::vector<double> h = xxz_two_site_host(delta, J);
   return Matrix(4, 4, h.data(), false, false);
 }
 
 Matrix xxz_two_site_spin1(double delta, double J) {
+  const std::vector<double> h = xxz_two_site_spin1_host(delta, J);
+  return Matrix(9, 9, h.data(), false, false);
+}
+
+std::vector<double> xxz_two_site_spin1_host(double delta, double J) {
   // S+ = sqrt(2) (|+1><0| + |0><-1|); (S+S- + S-S+)/2 moves one unit of Sz
   // between the sites with amplitude 1 (sqrt(2) sqrt(2) / 2)
   std::vector<double> h(81, 0.0);
@@ -44,7 +54,7 @@ Matrix xxz_two_site_spin1(double delta, double J) {
       h[b * 9 + a] = J;
       h[a * 9 + b] = J;
     }
-  return Matrix(9, 9, h.data(), false, false);
+  return h;
 }
 
 ITebd::ITebd(int chi, int d, double delta, double tau, uint64_t seed, const std::string& netdir)

@@ -2,6 +2,7 @@
 #pragma once
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "tn_hip.h"
 #include "uni10_amd/uni10.hpp"
@@ -9,6 +10,8 @@
 namespace uni10 {
 
 Matrix xxz_two_site(double delta, double J);
+std::vector<double> xxz_two_site_host(double delta, double J);         // row-major 4 x 4
+std::vector<double> xxz_two_site_spin1_host(double delta, double J);   // row-major 9 x 9
 // Two-site spin-1 XXZ J ((S+S- + S-S+)/2 + delta Sz Sz), basis |s t> with
 // s, t in (+1, 0, -1), s the slower index.
 Matrix xxz_two_site_spin1(double delta, double J);

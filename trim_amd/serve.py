@@ -72,6 +72,8 @@ MAX_CONTRACTS = 4096
 # engines per class: a lane runs its engines in turn, so more engines add
 # slots but no speed
 MAX_ENGINES_PER_CLASS = 4
+# part of the memory budget that engines leave to contractions
+CONTRACT_RESERVE = 0.1
 # contractions on the device at once (later ones wait on the host)
 MAX_INFLIGHT_CONTRACTS = 32
 
@@ -198,8 +200,9 @@ class Server:
         """classes: (chi, d, sym) tuples (or bare chi for (chi, 2, dense)).
         hbm_budget: the device memory the server may take beyond what the
         library holds when it starts; it is the library's budget
-        (tn_set_budget) while the server allocates.  Every class gets an
-        equal share of it for its first slots (slot_bytes per slot, at most
+        (tn_set_budget) while the server allocates.  Engines take at most
+        90 % of it (the rest is left to contractions): every class gets an
+        equal share of that for its first slots (slot_bytes per slot, at most
         slots_per_engine, None: 64); the classes of a lane share one engine.
         A class whose slots are all busy gains an engine of its own while
         the budget admits one, up to max_slots slots.  step_ms: the classes'
@@ -224,7 +227,10 @@ class Server:
         self._pinned_view = (ctypes.c_double * MAX_CONTRACTS).from_address(self._pinned.value)
         self._free_pins = list(range(MAX_CONTRACTS))
         cl = sorted(set(_cls(c) for c in classes))
-        self._share = hbm_budget / max(1, len(cl))
+        # engines take at most (1 - CONTRACT_RESERVE) of the budget, so that
+        # contractions always find room
+        self._share = hbm_budget * (1.0 - CONTRACT_RESERVE) / max(1, len(cl))
+        self._reserve = int(hbm_budget * CONTRACT_RESERVE)
         try:
             for c in cl:
                 self._register(c)
@@ -341,7 +347,7 @@ class Server:
             return False
         have = sum(e.slots(c) for e in mine)
         k = min(self.k, self.max_slots - have)
-        return k >= 1 and self._build(self.lane_of[c], [(c, k)], warm=False)
+        return k >= 1 and self._build(self.lane_of[c], [(c, k)], limit=self.budget - self._reserve, warm=False)
 
     def warmup(self):
         """Engines are warmed when created; kept for callers of the old API."""
