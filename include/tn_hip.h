@@ -217,6 +217,23 @@ int tn_permute_dense_c128(const double* src, double* dst, int rank, const int64_
                           const int* perm, void* stream);
 int tn_matmul_c128(const double* A, const double* B, int64_t M, int64_t N, int64_t K, double* C,
                    void* stream);
+/* Complex SVD (matrixSVD on std::complex<double>, uni10_lapack_cpu.cpp:606-633,
+ * zgesvd): A (M x N, interleaved, row-major) = U diag(S) VT with U (M x k) and
+ * VT (k x N) complex row-major, k = min(M, N), S descending; U may be NULL.
+ * One-sided complex Jacobi (each column pair rotated by a real Jacobi angle
+ * after a phase), one workgroup; waits for the stream. */
+int tn_svd_c128(const double* A, int64_t M, int64_t N, double* U, double* S, double* VT, void* stream);
+/* Hermitian eigen solver (eigSyDecompose on std::complex<double>,
+ * uni10_lapack_cpu.cpp:865-890, zheev): eigenvalues ascending; row k of
+ * vec_rows (N x N complex) is the conjugate of eigenvector k, so that
+ * H = vec_rows^H diag(eig) vec_rows (zheev's vectors read back row-major).
+ * Computed as the SVD of H + 2 ||H||_F I (positive definite, so its right
+ * singular vectors are H's eigenvectors); waits for the stream. */
+int tn_eigh_c128(const double* A, int64_t N, double* eig, double* vec_rows, void* stream);
+/* X (n x 2 x 2 real, per complex element b: [re, im, -im, re]): the factor
+ * that turns a complex product into a real one (C = A X over the pair
+ * axis), for block-sparse complex contractions. */
+int tn_expand_c128(const double* B, int64_t n, double* X, void* stream);
 
 /* ---- vectors / diagonal scaling -----------------------------------------
  * vectorNorm (uni10_lapack_cpu.cpp:397-414), vectorScal (:75-89),

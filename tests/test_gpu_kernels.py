@@ -846,6 +846,67 @@ def test_eigh_batched_rejects_large_n(tn):
         tn.call("tn_eigh_f64_batched", 1, pn, tn._p(a), pz, tn._p(e), pz, tn._p(v), pz, None)
 
 
+def _crand(seed, shape):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)
+
+
+def _c2d(z):
+    """complex array -> interleaved float64 (device layout)"""
+    return np.ascontiguousarray(z, dtype=np.complex128).view(np.float64).ravel()
+
+
+def _d2c(x, shape):
+    return np.ascontiguousarray(x).view(np.complex128).reshape(shape)
+
+
+@pytest.mark.parametrize("shape", [(5, 3), (3, 5), (16, 16), (40, 24), (24, 40), (64, 64), (1, 7)])
+def test_svd_c128_matches_lapack(tn, shape):
+    """Complex SVD (zgesvd's A = U diag(S) VT): singular values against
+    numpy's LAPACK to 1e-12, A rebuilt to 1e-12, U and VT orthonormal."""
+    M, N = shape
+    k = min(M, N)
+    a = _crand(60 + M * N, (M, N))
+    dA = tn.DeviceArray.from_host(_c2d(a))
+    dU, dS, dV = tn.DeviceArray((2 * M * k,)), tn.DeviceArray((k,)), tn.DeviceArray((2 * k * N,))
+    tn.call("tn_svd_c128", tn._p(dA), M, N, tn._p(dU), tn._p(dS), tn._p(dV), None)
+    U, S, VT = _d2c(dU.to_host(), (M, k)), dS.to_host(), _d2c(dV.to_host(), (k, N))
+    sr = np.linalg.svd(a, compute_uv=False)
+    assert np.max(np.abs(S - sr)) <= 1e-12 * sr[0]
+    assert np.all(np.diff(S) <= 0)
+    assert np.linalg.norm(U @ np.diag(S) @ VT - a) <= 1e-12 * np.linalg.norm(a)
+    assert np.max(np.abs(U.conj().T @ U - np.eye(k))) <= 1e-12
+    assert np.max(np.abs(VT @ VT.conj().T - np.eye(k))) <= 1e-12
+
+
+@pytest.mark.parametrize("n", [1, 4, 9, 33, 64])
+def test_eigh_c128_matches_lapack(tn, n):
+    """Hermitian eigen solver: eigenvalues ascending against numpy's LAPACK
+    (zheev's absolute accuracy, 1e-12 of the largest), H = E^H diag(w) E to
+    1e-11 (the positive shift 2 ||H||_F costs a few digits of absolute
+    accuracy against zheev) and E E^H = I with the eigenvector rows E."""
+    a = _crand(70 + n, (n, n))
+    h = a + a.conj().T
+    dH = tn.DeviceArray.from_host(_c2d(h))
+    dw, dE = tn.DeviceArray((n,)), tn.DeviceArray((2 * n * n,))
+    tn.call("tn_eigh_c128", tn._p(dH), n, tn._p(dw), tn._p(dE), None)
+    w, E = dw.to_host(), _d2c(dE.to_host(), (n, n))
+    wr = np.linalg.eigvalsh(h)
+    scale = np.abs(wr).max()
+    assert np.max(np.abs(w - wr)) <= 1e-12 * scale
+    assert np.linalg.norm(E.conj().T @ np.diag(w) @ E - h) <= 1e-11 * np.linalg.norm(h)
+    assert np.max(np.abs(E @ E.conj().T - np.eye(n))) <= 1e-12
+
+
+def test_expand_c128(tn):
+    z = _crand(80, 37)
+    dz, dx = tn.DeviceArray.from_host(_c2d(z)), tn.DeviceArray((4 * 37,))
+    tn.call("tn_expand_c128", tn._p(dz), 37, tn._p(dx), None)
+    x = dx.to_host().reshape(37, 2, 2)
+    assert np.array_equal(x[:, 0, 0], z.real) and np.array_equal(x[:, 0, 1], z.imag)
+    assert np.array_equal(x[:, 1, 0], -z.imag) and np.array_equal(x[:, 1, 1], z.real)
+
+
 # ---------------------------------------------------------------------------
 def test_vector_ops(tn):
     x = _rand(9, 12345)

@@ -100,6 +100,9 @@ SIGNATURES = {
     "tn_vecops_plan_destroy": [P],
     "tn_eigh_f64": [P, I64, P, P, P],
     "tn_eigh_f64_batched": [I64, PI64, P, PI64, P, PI64, P, PI64, P],
+    "tn_svd_c128": [P, I64, I64, P, P, P, P],
+    "tn_eigh_c128": [P, I64, P, P, P],
+    "tn_expand_c128": [P, I64, P, P],
     "tn_lanczos_f64": [P, I64, P, PI64, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                        ctypes.POINTER(ctypes.c_int), P],
     "tn_lanczos_c128": [P, I64, P, PI64, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
