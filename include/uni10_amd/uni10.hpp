@@ -14,6 +14,7 @@
 #ifndef UNI10_AMD_UNI10_HPP
 #define UNI10_AMD_UNI10_HPP
 
+#include <complex>
 #include <cstddef>
 #include <cstdint>
 #include <iosfwd>
@@ -28,6 +29,9 @@ namespace uni10 {
 typedef double Real;
 
 enum bondType { BD_IN = 1, BD_OUT = -1 };
+// datatype flags (datatype.hpp:90-99): RTYPE real FP64, CTYPE complex FP64
+enum rflag { RNULL = 0, RTYPE = 1 };
+enum cflag { CNULL = 0, CTYPE = 2 };
 enum parityType { PRT_EVEN = 0, PRT_ODD = 1 };
 enum parityFType { PRTF_EVEN = 0, PRTF_ODD = 1 };
 
@@ -133,6 +137,9 @@ class Matrix {
   Matrix();
   Matrix(size_t Rnum, size_t Cnum, bool diag = false, bool ongpu = true);
   Matrix(size_t Rnum, size_t Cnum, const double* elem, bool diag = false, bool src_ongpu = false);
+  // complex matrices (MatrixComplex.cpp): elements interleaved (re, im) in HBM
+  Matrix(cflag tp, size_t Rnum, size_t Cnum, bool diag = false);
+  Matrix(cflag tp, size_t Rnum, size_t Cnum, const std::complex<double>* elem, bool diag = false);
   Matrix(const Matrix& m);
   Matrix(Matrix&& m) noexcept;
   Matrix& operator=(const Matrix& m);
@@ -144,8 +151,10 @@ class Matrix {
   bool isDiag() const { return diag; }
   bool isOngpu() const { return true; }
   size_t elemNum() const { return diag ? (Rnum < Cnum ? Rnum : Cnum) : Rnum * Cnum; }
-  double* getElem() const { return buf.get(); }     // device pointer
-  std::vector<double> getHostElem() const;
+  int typeID() const { return ctype ? 2 : 1; }
+  double* getElem() const { return buf.get(); }     // device pointer (interleaved for CTYPE)
+  std::vector<double> getHostElem() const;          // CTYPE: interleaved (re, im)
+  std::vector<std::complex<double>> getHostElemC() const;
   void setElem(const double* elem, bool src_ongpu = false);
   void setElem(const std::vector<double>& elem, bool src_ongpu = false);
   double operator[](size_t idx) const;
@@ -161,6 +170,8 @@ class Matrix {
   double trace() const;
   Matrix& operator*=(double a);
   Matrix& operator+=(const Matrix& m);
+  // CTYPE: U, S (real diagonal), VT with A = U S VT; eigh of a Hermitian
+  // matrix: real eigenvalues, eigenvector rows E with A = E^H diag E
   std::vector<Matrix> svd() const;
   std::vector<Matrix> eigh() const;
   Matrix getDiag() const;
@@ -174,7 +185,10 @@ class Matrix {
  private:
   size_t Rnum = 0, Cnum = 0;
   bool diag = false;
+  bool ctype = false;
   DevBuf buf;
+  size_t nstore() const { return elemNum() * (ctype ? 2 : 1); }   // doubles held
+  void real_only(const char* what) const;
   friend class UniTensor;
 };
 typedef Matrix Block;  // read-only views are returned as copies
@@ -207,6 +221,9 @@ class UniTensor {
   explicit UniTensor(double val);
   UniTensor(const std::vector<Bond>& bonds, const std::string& name = "");
   UniTensor(const std::vector<Bond>& bonds, std::vector<int>& labels, const std::string& name = "");
+  // complex tensors (UniTensor.h:798-800, UniTensorComplex.cpp)
+  UniTensor(cflag tp, const std::vector<Bond>& bonds, const std::string& name = "");
+  UniTensor(cflag tp, const std::vector<Bond>& bonds, std::vector<int>& labels, const std::string& name = "");
   explicit UniTensor(const std::string& fname);   // binary dump (current or older format)
   UniTensor(const UniTensor& T);
   UniTensor(UniTensor&& T) noexcept;
@@ -219,21 +236,31 @@ class UniTensor {
   std::string getName() const { return m_name; }
   void setLabel(const std::vector<int>& newLabels);
   void setLabel(int* newLabels);
-  std::vector<int> label() const { return labels; }
-  int label(size_t idx) const { return labels.at(idx); }
-  std::vector<Bond> bond() const { return bonds; }
-  Bond bond(size_t idx) const { return bonds.at(idx); }
-  size_t bondNum() const { return bonds.size(); }
+  // (a complex tensor keeps its (re, im) pair as a hidden last bond; these
+  // accessors show the tensor's own bonds)
+  std::vector<int> label() const {
+    return m_ctype ? std::vector<int>(labels.begin(), labels.end() - 1) : labels;
+  }
+  int label(size_t idx) const { return label().at(idx); }
+  std::vector<Bond> bond() const {
+    return m_ctype ? std::vector<Bond>(bonds.begin(), bonds.end() - 1) : bonds;
+  }
+  Bond bond(size_t idx) const { return bond().at(idx); }
+  size_t bondNum() const { return bonds.size() - (m_ctype ? 1 : 0); }
   size_t inBondNum() const { return RBondNum; }
-  size_t elemNum() const { return m_elemNum; }
+  size_t elemNum() const { return m_ctype ? m_elemNum / 2 : m_elemNum; }
   size_t blockNum() const;
   std::vector<Qnum> blockQnum() const;
   Qnum blockQnum(size_t idx) const;
-  int typeID() const { return 1; }
+  int typeID() const { return m_ctype ? 2 : 1; }
 
   void setRawElem(const std::vector<double>& raw);
   void setRawElem(const double* raw);
-  Matrix getRawElem() const;
+  Matrix getRawElem() const;                         // CTYPE: a complex matrix
+  void setRawElem(const std::vector<std::complex<double>>& raw);
+  void setRawElem(const std::complex<double>* raw);
+  void setElem(const std::complex<double>* elem, bool src_ongpu = false);
+  std::complex<double> at(cflag tp, size_t idx) const;
   void setElem(const double* elem, bool src_ongpu = false);
   void setElem(const std::vector<double>& elem, bool src_ongpu = false);
   double* getElem() const { return buf.get(); }     // device pointer, block layout
@@ -289,6 +316,11 @@ class UniTensor {
 
  private:
   void initUniT(bool zero = true);
+  void make_complex(const std::vector<Bond>& b, bool zero);
+  void real_only(const char* what) const;
+  UniTensor& permute_full(const std::vector<int>& newLabels, int inBondNum);
+  friend UniTensor contract_complex(UniTensor& Ta, UniTensor& Tb, bool fast, bool allow_gather);
+  bool m_ctype = false;
   std::string m_name;
   std::vector<Bond> bonds;
   std::vector<int> labels;

@@ -34,6 +34,12 @@ int u10_set_autotune(int on);             /* tile autotuning of new GEMM plans  
  * run_qnums[3*r..3*r+2] and degeneracy run_degs[r]. */
 int u10_tensor_create(int nbonds, const int* types, const int* nruns, const int* run_qnums,
                       const int* run_degs, const char* name, u10_tensor* out);
+/* A complex tensor (UniTensor(CTYPE, bonds), UniTensor.h:798): elements
+ * exchanged as interleaved (re, im) doubles by set/get_elem, set/get_raw and
+ * put/get_block; u10_tensor_type gives 1 (real) or 2 (complex). */
+int u10_tensor_create_c(int nbonds, const int* types, const int* nruns, const int* run_qnums, const int* run_degs,
+                        const char* name, u10_tensor* out);
+int u10_tensor_type(u10_tensor t, int* type);
 /* Bond::combine (Bond.cpp:163-196; use_free_fn = 0: bonds[0].combine(bonds[1])
  * .combine(bonds[2])...) or the free combine(tp, bonds) (Bond.cpp:198-252;
  * use_free_fn = 1).  Host-only (no device needed).  Outputs: type, dim, the
@@ -100,6 +106,10 @@ int u10_rdotr(int64_t ar, int64_t ac, int adiag, const double* a, int64_t br, in
 
 /* Matrix::svd / eigh / exph on host arrays (computed on the device). */
 int u10_svd(const double* A, int64_t M, int64_t N, double* U, double* S, double* VT);
+/* Complex SVD and Hermitian eigen solver through Matrix(CTYPE): A, U, VT,
+ * vec interleaved (re, im); S and eig real. */
+int u10_svd_c128(const double* A, int64_t M, int64_t N, double* U, double* S, double* VT);
+int u10_eigh_c128(const double* A, int64_t N, double* eig, double* vec);
 int u10_eigh(const double* A, int64_t N, double* eig, double* vec_rows);
 int u10_exph(double a, const double* H, int64_t N, double* out);
 /* lanczosEigh (Block.cpp:474-506) on host arrays: psi (N) is the start

@@ -788,3 +788,160 @@ def test_batch_tebd_inactive_chains_are_skipped(u10, golden):
         assert B.lam(i, 0).tobytes() == R.lam(i, 0).tobytes()
         assert B.lam(i, 1).tobytes() == R.lam(i, 1).tobytes()
         assert B.energy(i) == R.energy(i)
+
+
+# ---------------------------------------------------------------------------
+# Complex tensors (UniTensorComplex.cpp): checked against the real path,
+# which the oracle pins: a permute moves real and imaginary parts as the
+# real permute moves each, a contraction is the four real contractions of
+# the parts (UniTensorTools.cpp:58-82 for the real-complex promotion).
+def _twin(ora, T, seed):
+    rng = np.random.default_rng(seed)
+    U = ora.UniTensor([b.copy() for b in T.bonds])
+    U.set_raw_elem(rng.uniform(-1, 1, [b.dim for b in T.bonds]))
+    U.set_label(list(T.labels))
+    return U
+
+
+def to_product_c(u10, Tr, Ti):
+    bonds = [u10.Bond(b.type, [q.triple() for q in b.Qlist()]) for b in Tr.bonds]
+    P = u10.UniTensor(bonds, Tr.name, dtype=complex)
+    P.setLabel(Tr.labels)
+    P.setElem(Tr.elem + 1j * Ti.elem)
+    return P
+
+
+def same_layout_c(P, T):
+    pb = [(q.triple(), r, c // 2, o // 2) for q, r, c, o in P.blocks()]
+    tb = [(q.triple(), r, c, o) for q, (r, c, o) in T.layout.blocks.items()]
+    return pb == tb
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_complex_permute_moves_both_parts_bitexact(u10, ora, seed):
+    rng = np.random.default_rng(900 + seed)
+    Tr = _random_u1(ora, 40 + seed, n_in=int(rng.integers(1, 3)), n_out=int(rng.integers(1, 3)))
+    Ti = _twin(ora, Tr, 60 + seed)
+    P = to_product_c(u10, Tr, Ti)
+    assert P.is_complex and P.elemNum() == Tr.elem.size and same_layout_c(P, Tr)
+    nb = len(Tr.bonds)
+    perm = [int(v) for v in rng.permutation(nb)]
+    rb = int(rng.integers(0, nb + 1))
+    labels = [Tr.labels[p] for p in perm]
+    _say("complex permute seed=%d perm=%s rb=%d" % (seed, perm, rb))
+    try:
+        Ro = Tr.copy().permute(labels, rb)
+        Io = Ti.copy().permute(labels, rb)
+    except ValueError:
+        with pytest.raises(u10.Uni10Error):
+            P.permute(labels, rb)
+        return
+    P.permute(labels, rb)
+    assert P.label() == labels and P.inBondNum() == rb and same_layout_c(P, Ro)
+    e = P.getElem()
+    assert e.real.tobytes() == Ro.elem.tobytes() and e.imag.tobytes() == Io.elem.tobytes()
+    raw = P.getRawElem()
+    assert np.array_equal(raw.real.ravel(), Ro.get_raw_elem().ravel())
+    assert np.array_equal(raw.imag.ravel(), Io.get_raw_elem().ravel())
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("fast", [False, True])
+def test_complex_contract_is_the_real_composition(u10, ora, seed, fast):
+    A, B = _pair_u1(ora, 70 + seed)
+    Ai, Bi = _twin(ora, A, 80 + seed), _twin(ora, B, 90 + seed)
+    PA, PB = to_product_c(u10, A, Ai), to_product_c(u10, B, Bi)
+    rr = ora.contract(A.copy(), B.copy())
+    ii = ora.contract(Ai.copy(), Bi.copy())
+    ri = ora.contract(A.copy(), Bi.copy())
+    ir = ora.contract(Ai.copy(), B.copy())
+    ref = (rr.elem - ii.elem) + 1j * (ri.elem + ir.elem)
+    C = ora.contract(A, B, fast)          # labels, layout and the inputs' state
+    PC = u10.contract(PA, PB, fast)
+    assert PC.is_complex and PC.label() == C.labels and same_layout_c(PC, C)
+    err = np.linalg.norm(PC.getElem() - ref) / np.linalg.norm(ref)
+    _say("complex contract seed=%d fast=%s err=%.2e" % (seed, fast, err))
+    assert err <= 1e-12
+    assert PA.label() == A.labels and PB.label() == B.labels
+    assert PA.getElem().real.tobytes() == A.elem.tobytes()
+
+
+def test_complex_contract_promotes_a_real_operand(u10, ora):
+    """real x complex: the real operand is promoted on a copy (RtoC) and
+    stays as it was."""
+    A, B = _pair_u1(ora, 101)
+    Bi = _twin(ora, B, 102)
+    PA, PB = to_product(u10, A), to_product_c(u10, B, Bi)
+    before = PA.getElem().tobytes()
+    ref = ora.contract(A.copy(), B.copy()).elem + 1j * ora.contract(A.copy(), Bi.copy()).elem
+    PC = u10.contract(PA, PB)
+    assert PC.is_complex and not PA.is_complex and PA.getElem().tobytes() == before
+    assert np.linalg.norm(PC.getElem() - ref) <= 1e-12 * np.linalg.norm(ref)
+    PC2 = u10.contract(PB, PA)            # complex x real
+    ref2 = ora.contract(B.copy(), A.copy()).elem + 1j * ora.contract(Bi.copy(), A.copy()).elem
+    assert np.linalg.norm(PC2.getElem() - ref2) <= 1e-12 * np.linalg.norm(ref2)
+
+
+def _dense_pair(u10, a, labels, dtype=float):
+    T = u10.UniTensor([u10.Bond(u10.BD_IN, a.shape[0]), u10.Bond(u10.BD_OUT, a.shape[1])], dtype=dtype)
+    T.setRawElem(a)
+    T.setLabel(labels)
+    return T
+
+
+def test_complex_outer_product_and_dense_contract(u10):
+    """Dense bonds: against numpy's complex matrix product; no shared
+    bonds: the real path's bond order, values the real composition."""
+    rng = np.random.default_rng(7)
+    a = rng.uniform(-1, 1, (3, 4)) + 1j * rng.uniform(-1, 1, (3, 4))
+    b = rng.uniform(-1, 1, (4, 5)) + 1j * rng.uniform(-1, 1, (4, 5))
+    C = u10.contract(_dense_pair(u10, a, [1, 2], complex), _dense_pair(u10, b, [2, 3], complex))
+    assert C.label() == [1, 3]
+    assert np.max(np.abs(C.getRawElem() - a @ b)) <= 1e-13
+    O = u10.contract(_dense_pair(u10, a, [1, 2], complex), _dense_pair(u10, b, [5, 6], complex))
+
+    def real_outer(x, y):
+        return u10.contract(_dense_pair(u10, x, [1, 2]), _dense_pair(u10, y, [5, 6]))
+
+    rr, ii = real_outer(a.real, b.real), real_outer(a.imag, b.imag)
+    ri, ir = real_outer(a.real, b.imag), real_outer(a.imag, b.real)
+    assert O.label() == rr.label() and O.inBondNum() == rr.inBondNum()
+    ref = (rr.getRawElem() - ii.getRawElem()) + 1j * (ri.getRawElem() + ir.getRawElem())
+    assert np.max(np.abs(O.getRawElem() - ref)) <= 1e-13
+
+
+def test_complex_blocks_and_elements(u10, ora):
+    Tr = _random_u1(ora, 111)
+    Ti = _twin(ora, Tr, 112)
+    P = to_product_c(u10, Tr, Ti)
+    for q, (r, c, o) in Tr.layout.blocks.items():
+        blk = P.getBlock(u10.Qnum(*q.triple()))
+        exp = (Tr.elem[o:o + r * c] + 1j * Ti.elem[o:o + r * c]).reshape(r, c)
+        assert np.array_equal(blk, exp)
+        P.putBlock(2.0 * blk, u10.Qnum(*q.triple()))
+        assert np.array_equal(P.getBlock(u10.Qnum(*q.triple())), 2.0 * exp)
+    assert abs(P.norm() - 2.0 * np.sqrt(np.sum(Tr.elem ** 2) + np.sum(Ti.elem ** 2))) <= 1e-12 * P.norm()
+
+
+def test_complex_svd_eigh_through_matrix(u10):
+    rng = np.random.default_rng(5)
+    a = rng.uniform(-1, 1, (7, 5)) + 1j * rng.uniform(-1, 1, (7, 5))
+    U, S, VT = u10.svd(a)
+    assert np.max(np.abs(S - np.linalg.svd(a, compute_uv=False))) <= 1e-13 * S[0]
+    assert np.linalg.norm(U @ np.diag(S) @ VT - a) <= 1e-13 * np.linalg.norm(a)
+    h = a.conj().T @ a + np.diag(rng.uniform(-3, 3, 5))
+    w, E = u10.eigh(h)
+    assert np.max(np.abs(w - np.linalg.eigvalsh(h))) <= 1e-12 * np.abs(w).max()
+    assert np.linalg.norm(E.conj().T @ np.diag(w) @ E - h) <= 1e-11 * np.linalg.norm(h)
+
+
+def test_complex_tensor_unsupported_paths_fail(u10, golden, tmp_path):
+    P = u10.UniTensor([u10.Bond(u10.BD_IN, 2), u10.Bond(u10.BD_OUT, 2)], dtype=complex)
+    P.setRawElem(np.arange(4) + 1j)
+    with pytest.raises(u10.Uni10Error):
+        P.at([0, 0])
+    with pytest.raises(u10.Uni10Error):
+        P.save(str(tmp_path / "c.ut"))
+    net = u10.Network(os.path.join(golden, "Simple.net"))
+    with pytest.raises(u10.Uni10Error):
+        net.putTensor("A", P)

@@ -89,6 +89,28 @@ int u10_tensor_create(int nbonds, const int* types, const int* nruns, const int*
   });
 }
 
+int u10_tensor_create_c(int nbonds, const int* types, const int* nruns, const int* run_qnums, const int* run_degs,
+                        const char* name, u10_tensor* out) {
+  return guard([&] {
+    need(out, "u10_tensor_create_c");
+    std::vector<Bond> bonds;
+    int r = 0;
+    for (int b = 0; b < nbonds; ++b) {
+      std::vector<Qnum> full;
+      for (int k = 0; k < nruns[b]; ++k, ++r) full.insert(full.end(), run_degs[r], qn(run_qnums + 3 * r));
+      bonds.emplace_back(types[b] == 1 ? BD_IN : BD_OUT, full);
+    }
+    *out = new u10_tensor_s{UniTensor(CTYPE, bonds, name ? name : "")};
+  });
+}
+
+int u10_tensor_type(u10_tensor t, int* type) {
+  return guard([&] {
+    need(type, "u10_tensor_type");
+    *type = t->T.typeID();
+  });
+}
+
 int u10_bond_combine(int nbonds, const int* types, const int* nruns, const int* run_qnums, const int* run_degs,
                      int use_free_fn, int tp, int* out_type, int* out_dim, int* out_nruns, int* out_qnums,
                      int* out_degs, int* out_ndeg, int* deg_qnums, int* deg_counts, int cap) {
@@ -250,7 +272,13 @@ int u10_tensor_at(u10_tensor t, const int64_t* idxs, double* out) {
 }
 
 int u10_tensor_put_block(u10_tensor t, const int* q, int64_t rows, int64_t cols, const double* host) {
-  return guard([&] { t->T.putBlock(qn(q), Matrix((size_t)rows, (size_t)cols, host, false, false)); });
+  return guard([&] {
+    if (t->T.typeID() == 2)
+      t->T.putBlock(qn(q), Matrix(CTYPE, (size_t)rows, (size_t)cols,
+                                  reinterpret_cast<const std::complex<double>*>(host), false));
+    else
+      t->T.putBlock(qn(q), Matrix((size_t)rows, (size_t)cols, host, false, false));
+  });
 }
 
 int u10_tensor_get_block(u10_tensor t, const int* q, double* host) {
@@ -378,6 +406,27 @@ int u10_svd(const double* A, int64_t M, int64_t N, double* U, double* S, double*
     std::memcpy(U, u.data(), u.size() * 8);
     std::memcpy(S, s.data(), s.size() * 8);
     std::memcpy(VT, vt.data(), vt.size() * 8);
+  });
+}
+
+int u10_svd_c128(const double* A, int64_t M, int64_t N, double* U, double* S, double* VT) {
+  return guard([&] {
+    Matrix a(CTYPE, (size_t)M, (size_t)N, reinterpret_cast<const std::complex<double>*>(A), false);
+    auto r = a.svd();
+    auto u = r[0].getHostElem(), s = r[1].getHostElem(), vt = r[2].getHostElem();
+    std::memcpy(U, u.data(), u.size() * 8);
+    std::memcpy(S, s.data(), s.size() * 8);
+    std::memcpy(VT, vt.data(), vt.size() * 8);
+  });
+}
+
+int u10_eigh_c128(const double* A, int64_t N, double* eig, double* vec) {
+  return guard([&] {
+    Matrix a(CTYPE, (size_t)N, (size_t)N, reinterpret_cast<const std::complex<double>*>(A), false);
+    auto r = a.eigh();
+    auto e = r[0].getHostElem(), v = r[1].getHostElem();
+    std::memcpy(eig, e.data(), e.size() * 8);
+    std::memcpy(vec, v.data(), v.size() * 8);
   });
 }
 

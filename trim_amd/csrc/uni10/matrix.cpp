@@ -76,14 +76,25 @@ Matrix::Matrix(size_t R, size_t C, const double* elem, bool d, bool src_ongpu)
   setElem(elem, src_ongpu);
 }
 
-Matrix::Matrix(const Matrix& m) : Rnum(m.Rnum), Cnum(m.Cnum), diag(m.diag) {
-  buf.reset(elemNum(), false);
-  if (elemNum())
-    detail::check(tn_memcpy_d2d(buf.get(), m.buf.get(), elemNum() * sizeof(double), detail::stream()),
+Matrix::Matrix(cflag, size_t R, size_t C, bool d) : Rnum(R), Cnum(C), diag(d), ctype(true) {
+  buf.reset(nstore(), true);
+}
+
+Matrix::Matrix(cflag, size_t R, size_t C, const std::complex<double>* elem, bool d)
+    : Rnum(R), Cnum(C), diag(d), ctype(true) {
+  buf.reset(nstore(), false);
+  setElem(reinterpret_cast<const double*>(elem), false);
+}
+
+Matrix::Matrix(const Matrix& m) : Rnum(m.Rnum), Cnum(m.Cnum), diag(m.diag), ctype(m.ctype) {
+  buf.reset(nstore(), false);
+  if (nstore())
+    detail::check(tn_memcpy_d2d(buf.get(), m.buf.get(), nstore() * sizeof(double), detail::stream()),
                   "Matrix copy");
 }
 
-Matrix::Matrix(Matrix&& m) noexcept : Rnum(m.Rnum), Cnum(m.Cnum), diag(m.diag), buf(std::move(m.buf)) {
+Matrix::Matrix(Matrix&& m) noexcept
+    : Rnum(m.Rnum), Cnum(m.Cnum), diag(m.diag), ctype(m.ctype), buf(std::move(m.buf)) {
   m.Rnum = m.Cnum = 0;
 }
 
@@ -92,9 +103,10 @@ Matrix& Matrix::operator=(const Matrix& m) {
   Rnum = m.Rnum;
   Cnum = m.Cnum;
   diag = m.diag;
-  buf.reset(elemNum(), false);
-  if (elemNum())
-    detail::check(tn_memcpy_d2d(buf.get(), m.buf.get(), elemNum() * sizeof(double), detail::stream()),
+  ctype = m.ctype;
+  buf.reset(nstore(), false);
+  if (nstore())
+    detail::check(tn_memcpy_d2d(buf.get(), m.buf.get(), nstore() * sizeof(double), detail::stream()),
                   "Matrix assign");
   return *this;
 }
@@ -103,12 +115,26 @@ Matrix& Matrix::operator=(Matrix&& m) noexcept {
   Rnum = m.Rnum;
   Cnum = m.Cnum;
   diag = m.diag;
+  ctype = m.ctype;
   buf = std::move(m.buf);
   return *this;
 }
 
+void Matrix::real_only(const char* what) const {
+  if (ctype) detail::fail(std::string(what) + ": not provided for complex matrices");
+}
+
+std::vector<std::complex<double>> Matrix::getHostElemC() const {
+  if (!ctype) detail::fail("Matrix::getHostElemC: the matrix is real");
+  std::vector<std::complex<double>> h(elemNum());
+  if (!h.empty())
+    detail::check(tn_memcpy_d2h(h.data(), buf.get(), h.size() * sizeof(std::complex<double>), detail::stream()),
+                  "Matrix::getHostElemC");
+  return h;
+}
+
 std::vector<double> Matrix::getHostElem() const {
-  std::vector<double> h(elemNum());
+  std::vector<double> h(nstore());
   if (!h.empty())
     detail::check(tn_memcpy_d2h(h.data(), buf.get(), h.size() * sizeof(double), detail::stream()),
                   "Matrix::getHostElem");
@@ -117,17 +143,18 @@ std::vector<double> Matrix::getHostElem() const {
 
 void Matrix::setElem(const double* elem, bool src_ongpu) {
   if (!elemNum()) return;
-  int rc = src_ongpu ? tn_memcpy_d2d(buf.get(), elem, elemNum() * sizeof(double), detail::stream())
-                     : tn_memcpy_h2d(buf.get(), elem, elemNum() * sizeof(double), detail::stream());
+  int rc = src_ongpu ? tn_memcpy_d2d(buf.get(), elem, nstore() * sizeof(double), detail::stream())
+                     : tn_memcpy_h2d(buf.get(), elem, nstore() * sizeof(double), detail::stream());
   detail::check(rc, "Matrix::setElem");
 }
 
 void Matrix::setElem(const std::vector<double>& elem, bool src_ongpu) {
-  if (elem.size() < elemNum()) detail::fail("Matrix::setElem: not enough elements");
+  if (elem.size() < nstore()) detail::fail("Matrix::setElem: not enough elements");
   setElem(elem.data(), src_ongpu);
 }
 
 double Matrix::operator[](size_t idx) const {
+  real_only("Matrix::operator[]");
   if (idx >= elemNum()) detail::fail("Matrix::operator[]: index out of range");
   double v = 0;
   detail::check(tn_memcpy_d2h(&v, buf.get() + idx, sizeof(double), detail::stream()), "Matrix::[]");
@@ -141,10 +168,11 @@ double Matrix::at(size_t r, size_t c) const {
 }
 
 void Matrix::set_zero() {
-  if (elemNum()) detail::check(tn_zero(buf.get(), elemNum() * sizeof(double), detail::stream()), "set_zero");
+  if (elemNum()) detail::check(tn_zero(buf.get(), nstore() * sizeof(double), detail::stream()), "set_zero");
 }
 
 void Matrix::identity() {
+  real_only("Matrix::identity");
   std::vector<double> h(elemNum(), 0.0);
   if (diag)
     std::fill(h.begin(), h.end(), 1.0);
@@ -154,11 +182,12 @@ void Matrix::identity() {
 }
 
 void Matrix::randomize(uint64_t seed) {
-  detail::check(tn_fill_uniform_f64(buf.get(), (int64_t)elemNum(), seed, detail::stream()), "randomize");
+  detail::check(tn_fill_uniform_f64(buf.get(), (int64_t)nstore(), seed, detail::stream()), "randomize");
 }
 
 Matrix& Matrix::resize(size_t row, size_t col) {
   // MatrixReal.cpp:289-359
+  real_only("Matrix::resize");
   if (diag) {
     size_t n_new = std::min(row, col), n_old = elemNum();
     if (n_new != n_old) {
@@ -190,6 +219,7 @@ Matrix& Matrix::resize(size_t row, size_t col) {
 }
 
 Matrix& Matrix::transpose() {
+  real_only("Matrix::transpose");
   if (!diag && Rnum * Cnum) {
     DevBuf nb(Rnum * Cnum);
     detail::check(tn_transpose_f64(buf.get(), (int64_t)Rnum, (int64_t)Cnum, nb.get(), detail::stream()),
@@ -203,13 +233,14 @@ Matrix& Matrix::transpose() {
 double Matrix::norm() const {
   if (!elemNum()) return 0.0;
   DevBuf out(1);
-  detail::check(tn_norm2_f64(buf.get(), (int64_t)elemNum(), out.get(), detail::stream()), "Matrix::norm");
+  detail::check(tn_norm2_f64(buf.get(), (int64_t)nstore(), out.get(), detail::stream()), "Matrix::norm");
   double v = 0;
   detail::check(tn_memcpy_d2h(&v, out.get(), sizeof(double), detail::stream()), "Matrix::norm");
   return v;
 }
 
 double Matrix::sum() const {
+  real_only("Matrix::sum");
   auto h = getHostElem();
   double s = 0;
   for (double v : h) s += v;
@@ -217,6 +248,7 @@ double Matrix::sum() const {
 }
 
 double Matrix::trace() const {
+  real_only("Matrix::trace");
   auto h = getHostElem();
   double s = 0;
   if (diag)
@@ -228,12 +260,12 @@ double Matrix::trace() const {
 
 Matrix& Matrix::operator*=(double a) {
   if (elemNum())
-    detail::check(tn_scale_f64(buf.get(), (int64_t)elemNum(), a, detail::stream()), "Matrix::operator*=");
+    detail::check(tn_scale_f64(buf.get(), (int64_t)nstore(), a, detail::stream()), "Matrix::operator*=");
   return *this;
 }
 
 Matrix& Matrix::operator+=(const Matrix& m) {
-  if (m.Rnum != Rnum || m.Cnum != Cnum || m.diag != diag)
+  if (m.Rnum != Rnum || m.Cnum != Cnum || m.diag != diag || m.ctype != ctype)
     detail::fail("The dimensions of the two matrices do not match for matrix addition.");
   auto a = getHostElem();
   auto b = m.getHostElem();
@@ -243,6 +275,7 @@ Matrix& Matrix::operator+=(const Matrix& m) {
 }
 
 Matrix Matrix::getDiag() const {
+  real_only("Matrix::getDiag");
   if (diag) return *this;
   Matrix d(Rnum, Cnum, true);
   auto h = getHostElem();
@@ -263,6 +296,20 @@ static Matrix dense_of_diag(const Matrix& m) {
 
 std::vector<Matrix> Matrix::svd() const {
   // BlockReal.cpp:226-250 -> matrixSVD: U (R x k), S (k x k diagonal), VT (k x C)
+  if (ctype) {
+    // BlockComplex.cpp svd -> matrixSVD on std::complex<double> (zgesvd)
+    if (diag) detail::fail("Matrix::svd: a complex diagonal matrix is its own SVD");
+    size_t k = std::min(Rnum, Cnum);
+    std::vector<Matrix> outs;
+    outs.emplace_back(CTYPE, Rnum, k, false);
+    outs.emplace_back(k, k, true);
+    outs.emplace_back(CTYPE, k, Cnum, false);
+    if (k)
+      detail::check(tn_svd_c128(buf.get(), (int64_t)Rnum, (int64_t)Cnum, outs[0].buf.get(), outs[1].buf.get(),
+                                outs[2].buf.get(), detail::stream()),
+                    "Matrix::svd");
+    return outs;
+  }
   if (diag) return dense_of_diag(*this).svd();
   size_t k = std::min(Rnum, Cnum);
   std::vector<Matrix> outs;
@@ -281,6 +328,14 @@ std::vector<Matrix> Matrix::eigh() const {
   if (Rnum != Cnum) detail::fail("Cannot perform eigenvalue decomposition on a non-square matrix.");
   if (diag) detail::fail("Cannot perform eigenvalue decomposition on a diagonal matrix. Need not to do so.");
   std::vector<Matrix> outs;
+  if (ctype) {
+    // eigSyDecompose on std::complex<double> (zheev): real eigenvalues
+    outs.emplace_back(Rnum, Cnum, true);
+    outs.emplace_back(CTYPE, Rnum, Cnum, false);
+    detail::check(tn_eigh_c128(buf.get(), (int64_t)Rnum, outs[0].buf.get(), outs[1].buf.get(), detail::stream()),
+                  "Matrix::eigh");
+    return outs;
+  }
   outs.emplace_back(Rnum, Cnum, true);
   outs.emplace_back(Rnum, Cnum, false);
   detail::check(tn_eigh_f64(buf.get(), (int64_t)Rnum, outs[0].buf.get(), outs[1].buf.get(), detail::stream()),
@@ -289,6 +344,7 @@ std::vector<Matrix> Matrix::eigh() const {
 }
 
 Matrix RDotR(const Matrix& a, const Matrix& b) {
+  if (a.typeID() != 1 || b.typeID() != 1) detail::fail("matrix product: complex matrices are not provided");
   if (a.col() != b.row())
     detail::fail("The dimensions of the two matrices do not match for matrix multiplication.");
   if (!a.isDiag() && !b.isDiag()) {
@@ -349,7 +405,7 @@ Matrix operator+(const Matrix& a, const Matrix& b) {
 
 bool operator==(const Matrix& a, const Matrix& b) {
   // Block.cpp:110-131: same shape/diag flag, elementwise |a-b| < 1e-12
-  if (a.Rnum != b.Rnum || a.Cnum != b.Cnum || a.diag != b.diag) return false;
+  if (a.Rnum != b.Rnum || a.Cnum != b.Cnum || a.diag != b.diag || a.ctype != b.ctype) return false;
   auto ha = a.getHostElem(), hb = b.getHostElem();
   for (size_t i = 0; i < ha.size(); ++i)
     if (std::fabs(ha[i] - hb[i]) > 1e-12) return false;
@@ -357,8 +413,21 @@ bool operator==(const Matrix& a, const Matrix& b) {
 }
 
 std::ostream& operator<<(std::ostream& os, const Matrix& m) {
-  os << m.Rnum << " x " << m.Cnum << " = " << m.elemNum() << (m.diag ? ", Diagonal" : "") << "\n\n";
+  os << m.Rnum << " x " << m.Cnum << " = " << m.elemNum() << (m.diag ? ", Diagonal" : "")
+     << (m.ctype ? ", Complex" : "") << "\n\n";
   auto h = m.getHostElem();
+  if (m.ctype) {
+    for (size_t r = 0; r < m.Rnum; ++r) {
+      for (size_t c = 0; c < m.Cnum; ++c) {
+        const size_t e = m.diag ? r : r * m.Cnum + c;
+        const bool z = m.diag && r != c;
+        os << " (" << std::fixed << std::setprecision(3) << (z ? 0.0 : h[2 * e]) << "," << (z ? 0.0 : h[2 * e + 1])
+           << ")";
+      }
+      os << "\n\n";
+    }
+    return os;
+  }
   for (size_t r = 0; r < m.Rnum; ++r) {
     for (size_t c = 0; c < m.Cnum; ++c) {
       double v = m.diag ? (r == c ? h[r] : 0.0) : h[r * m.Cnum + c];
@@ -384,6 +453,7 @@ Matrix takeExp(double a, const Matrix& mat) { return exph(a, mat); }
 
 size_t lanczosEigh(Matrix& ori_mat, double& E0, Matrix& psi, size_t max_iter, double err_tol) {
   // Block.cpp:474-506 -> lanczosEV (uni10_lapack_cpu.cpp:416-509), on the device
+  if (ori_mat.typeID() != 1) detail::fail("lanczosEigh: the complex form is tn_lanczos_c128");
   if (ori_mat.row() != ori_mat.col())
     detail::fail("Cannot perform Lanczos algorithm to find the lowest eigen value and eigen vector on a "
                  "non-square matrix.");

@@ -218,6 +218,7 @@ Network::~Network() {
 }
 
 void Network::putTensor(size_t idx, const UniTensor* T, bool force) {
+  if (T && T->typeID() != 1) detail::fail("Network: complex tensors are not provided (contract them with contract())");
   if (idx >= label_arr.size() - 1)
     detail::fail("Index exceeds the number of the tensors in the list of network file.");
   if (!force && load) destruct();

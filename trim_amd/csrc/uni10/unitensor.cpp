@@ -229,6 +229,39 @@ UniTensor::UniTensor(const std::vector<Bond>& b, std::vector<int>& lab, const st
   setLabel(lab);
 }
 
+// A complex tensor is held as the real tensor with one more bond, an
+// outgoing bond of dimension 2 and charge 0 last: it doubles every block's
+// column count and keeps (re, im) of each element adjacent, so the block
+// layout is the complex layout with interleaved pairs, and every layout
+// operation (permute with the pair bond kept last, setRawElem, getBlock,
+// block contraction) is the real one on that tensor.
+namespace {
+constexpr int kPairLabel = -1073741823;        // the pair bond's label (no caller uses it)
+constexpr int kPairLabel2 = -1073741822;       // the second pair axis of a contraction's expansion
+}  // namespace
+
+void UniTensor::make_complex(const std::vector<Bond>& b, bool zero) {
+  bonds = b;
+  bonds.push_back(Bond(BD_OUT, 2));
+  initUniT(zero);
+  labels.back() = kPairLabel;
+  m_ctype = true;
+}
+
+UniTensor::UniTensor(cflag, const std::vector<Bond>& b, const std::string& name) : m_name(name) {
+  make_complex(b, true);
+}
+
+UniTensor::UniTensor(cflag, const std::vector<Bond>& b, std::vector<int>& lab, const std::string& name)
+    : m_name(name) {
+  make_complex(b, true);
+  setLabel(lab);
+}
+
+void UniTensor::real_only(const char* what) const {
+  if (m_ctype) detail::fail(std::string(what) + ": not provided for complex tensors");
+}
+
 void UniTensor::initUniT(bool zero) {
   m_layout = get_layout(bonds);
   status = 0;
@@ -252,8 +285,8 @@ void UniTensor::initUniT(bool zero) {
 }
 
 UniTensor::UniTensor(const UniTensor& T)
-    : m_name(T.m_name), bonds(T.bonds), labels(T.labels), m_layout(T.m_layout), m_elemNum(T.m_elemNum),
-      RBondNum(T.RBondNum), status(T.status) {
+    : m_ctype(T.m_ctype), m_name(T.m_name), bonds(T.bonds), labels(T.labels), m_layout(T.m_layout),
+      m_elemNum(T.m_elemNum), RBondNum(T.RBondNum), status(T.status) {
   buf.reset(m_elemNum, false);
   detail::check(tn_memcpy_d2d(buf.get(), T.buf.get(), m_elemNum * sizeof(double), detail::stream()),
                 "UniTensor copy");
@@ -263,7 +296,7 @@ UniTensor::UniTensor(const UniTensor& T)
 }
 
 UniTensor::UniTensor(UniTensor&& T) noexcept
-    : m_name(std::move(T.m_name)), bonds(std::move(T.bonds)), labels(std::move(T.labels)),
+    : m_ctype(T.m_ctype), m_name(std::move(T.m_name)), bonds(std::move(T.bonds)), labels(std::move(T.labels)),
       m_layout(std::move(T.m_layout)), buf(std::move(T.buf)), m_elemNum(T.m_elemNum), RBondNum(T.RBondNum),
       status(T.status) {
   T.m_elemNum = 0;
@@ -273,6 +306,7 @@ UniTensor::UniTensor(UniTensor&& T) noexcept
 UniTensor& UniTensor::operator=(const UniTensor& T) {
   if (this == &T) return *this;
   ELEMNUM -= m_elemNum;
+  m_ctype = T.m_ctype;
   m_name = T.m_name;
   bonds = T.bonds;
   labels = T.labels;
@@ -291,6 +325,7 @@ UniTensor& UniTensor::operator=(const UniTensor& T) {
 UniTensor& UniTensor::operator=(UniTensor&& T) noexcept {
   if (this == &T) return *this;
   ELEMNUM -= m_elemNum;
+  m_ctype = T.m_ctype;
   m_name = std::move(T.m_name);
   bonds = std::move(T.bonds);
   labels = std::move(T.labels);
@@ -310,20 +345,27 @@ UniTensor::~UniTensor() {
 }
 
 UniTensor& UniTensor::assign(const std::vector<Bond>& b) {
+  if (m_ctype) {
+    UniTensor T(CTYPE, b, m_name);
+    *this = std::move(T);
+    return *this;
+  }
   UniTensor T(b, m_name);
   *this = std::move(T);
   return *this;
 }
 
 void UniTensor::setLabel(const std::vector<int>& newLabels) {
-  if (newLabels.size() != bonds.size())
+  if (newLabels.size() != bondNum())
     detail::fail("The size of input vector(labels) does not match for the number of bonds.");
   std::set<int> s(newLabels.begin(), newLabels.end());
   if (s.size() != newLabels.size()) detail::fail("Labels must be distinct.");
+  if (m_ctype && s.count(kPairLabel)) detail::fail("Label -1073741823 is reserved for complex tensors.");
   labels = newLabels;
+  if (m_ctype) labels.push_back(kPairLabel);
 }
 
-void UniTensor::setLabel(int* newLabels) { setLabel(std::vector<int>(newLabels, newLabels + bonds.size())); }
+void UniTensor::setLabel(int* newLabels) { setLabel(std::vector<int>(newLabels, newLabels + bondNum())); }
 
 size_t UniTensor::blockNum() const { return m_layout->blocks.size(); }
 
@@ -386,6 +428,32 @@ void UniTensor::setRawElem(const std::vector<double>& raw) {
   setRawElem(raw.data());
 }
 
+// (for a complex tensor the real forms read and write (re, im) pairs)
+void UniTensor::setRawElem(const std::complex<double>* raw) {
+  if (!m_ctype) detail::fail("setRawElem: complex elements for a real tensor");
+  setRawElem(reinterpret_cast<const double*>(raw));
+}
+
+void UniTensor::setRawElem(const std::vector<std::complex<double>>& raw) {
+  size_t n = 1;
+  for (auto& b : bond()) n *= b.dim();
+  if (raw.size() < n) detail::fail("setRawElem: not enough elements");
+  setRawElem(raw.data());
+}
+
+void UniTensor::setElem(const std::complex<double>* elem, bool src_ongpu) {
+  if (!m_ctype) detail::fail("setElem: complex elements for a real tensor");
+  setElem(reinterpret_cast<const double*>(elem), src_ongpu);
+}
+
+std::complex<double> UniTensor::at(cflag, size_t idx) const {
+  if (!m_ctype) detail::fail("at(CTYPE, idx): the tensor is real");
+  if (idx >= m_elemNum / 2) detail::fail("Index exceeds the number of elements.");
+  double v[2] = {0, 0};
+  detail::check(tn_memcpy_d2h(v, buf.get() + 2 * idx, sizeof(v), detail::stream()), "at");
+  return {v[0], v[1]};
+}
+
 std::vector<double> UniTensor::getHostElem() const {
   std::vector<double> h(m_elemNum);
   if (m_elemNum)
@@ -430,6 +498,10 @@ Matrix UniTensor::getRawElem() const {
         }
       }
   }
+  if (m_ctype) {
+    // the pair bond is the last outgoing one: (re, im) adjacent in each row
+    return Matrix(CTYPE, rows, cols / 2, reinterpret_cast<const std::complex<double>*>(raw.data()), false);
+  }
   return Matrix(rows, cols, raw.data(), false, false);
 }
 
@@ -446,6 +518,7 @@ void UniTensor::setElem(const std::vector<double>& elem, bool src_ongpu) {
 }
 
 double UniTensor::at(size_t idx) const {
+  real_only("at (use at(CTYPE, idx))");
   if (idx >= m_elemNum) detail::fail("Index exceeds the number of elements.");
   double v = 0;
   detail::check(tn_memcpy_d2h(&v, buf.get() + idx, sizeof(double), detail::stream()), "at");
@@ -453,6 +526,7 @@ double UniTensor::at(size_t idx) const {
 }
 
 double UniTensor::at(const std::vector<size_t>& idxs) const {
+  real_only("at(idxs)");
   if (!(status & HAVEBOND)) detail::fail("The tensor is a scalar. Use UniTensor::operator[] instead.");
   if (idxs.size() != bonds.size())
     detail::fail("The size of input indices array does not match with the number of the bonds.");
@@ -490,6 +564,18 @@ void UniTensor::putBlock(const Qnum& qnum, const Matrix& mat) {
     detail::fail(err.str());
   }
   const BlockInfo& bi = it->second;
+  if (mat.typeID() != typeID()) detail::fail("putBlock: the block and the tensor differ in type (real/complex)");
+  if (m_ctype) {
+    if (mat.isDiag()) detail::fail("putBlock: a complex diagonal block is not provided");
+    if (!(mat.row() == bi.Rnum && 2 * mat.col() == bi.Cnum))
+      detail::fail("The dimension of input matrix does not match for the dimension of the block");
+    if (bi.Rnum * bi.Cnum)
+      detail::check(tn_memcpy_d2d(buf.get() + bi.offset, mat.getElem(), bi.Rnum * bi.Cnum * sizeof(double),
+                                  detail::stream()),
+                    "putBlock");
+    status |= HAVEELEM;
+    return;
+  }
   if (!(mat.row() == bi.Rnum && mat.col() == bi.Cnum)) {
     std::ostringstream err;
     err << "The dimension of input matrix does not match for the dimension of the block with quantum number "
@@ -518,13 +604,18 @@ Matrix UniTensor::getBlock(const Qnum& qnum, bool diag) const {
     detail::fail(err.str());
   }
   Matrix m(it->second.Rnum, it->second.Cnum, buf.get() + it->second.offset, false, true);
+  if (m_ctype) {
+    if (diag) detail::fail("getBlock: a complex diagonal block is not provided");
+    m.Cnum /= 2;     // the pair bond doubles the columns: the same doubles, read as complex
+    m.ctype = true;
+    return m;
+  }
   return diag ? m.getDiag() : m;
 }
 
 std::map<Qnum, Matrix> UniTensor::getBlocks() const {
   std::map<Qnum, Matrix> out;
-  for (auto& kv : m_layout->blocks)
-    out.emplace(kv.first, Matrix(kv.second.Rnum, kv.second.Cnum, buf.get() + kv.second.offset, false, true));
+  for (auto& kv : m_layout->blocks) out.emplace(kv.first, getBlock(kv.first, false));
   return out;
 }
 
@@ -646,6 +737,26 @@ static tn_blockperm_plan block_perm_plan(const Layout& in, const std::vector<int
 }
 
 UniTensor& UniTensor::permute(const std::vector<int>& newLabels, int rowBondNum) {
+  if (!m_ctype) return permute_full(newLabels, rowBondNum);
+  // complex (UniTensorComplex.cpp:592-825): the real permute with the pair
+  // bond kept last (an outgoing bond), a bit-exact move of (re, im) pairs
+  if (newLabels.size() != bondNum())
+    detail::fail("The size of the input new labels does not match for the number of bonds.");
+  if (rowBondNum < 0 || rowBondNum > (int)bondNum()) detail::fail("permute: inBondNum out of range");
+  std::vector<int> full(newLabels);
+  full.push_back(kPairLabel);
+  m_ctype = false;
+  try {
+    permute_full(full, rowBondNum);
+  } catch (...) {
+    m_ctype = true;
+    throw;
+  }
+  m_ctype = true;
+  return *this;
+}
+
+UniTensor& UniTensor::permute_full(const std::vector<int>& newLabels, int rowBondNum) {
   if (!(status & HAVEBOND)) detail::fail("There is no bond in the tensor(scalar) to permute.");
   if (labels.size() != newLabels.size())
     detail::fail("The size of the input new labels does not match for the number of bonds.");
@@ -697,12 +808,13 @@ UniTensor& UniTensor::permute(const std::vector<int>& newLabels, int rowBondNum)
 }
 
 UniTensor& UniTensor::permute(int* newLabels, int rowBondNum) {
-  return permute(std::vector<int>(newLabels, newLabels + bonds.size()), rowBondNum);
+  return permute(std::vector<int>(newLabels, newLabels + bondNum()), rowBondNum);
 }
 
-UniTensor& UniTensor::permute(int rowBondNum) { return permute(std::vector<int>(labels), rowBondNum); }
+UniTensor& UniTensor::permute(int rowBondNum) { return permute(label(), rowBondNum); }
 
 UniTensor& UniTensor::transpose() {
+  real_only("transpose");
   if (!(status & HAVEBOND)) detail::fail("There is no bond in the tensor(scalar) to perform transposition.");
   const int nb = (int)bonds.size();
   int rb = 0;
@@ -741,6 +853,7 @@ UniTensor& UniTensor::transpose() {
 // exSwap (UniTensor.cpp:1237-1268): every pair (bond shared with Tb, bond
 // not shared), in bond order.
 std::vector<_Swap> UniTensor::exSwap(const UniTensor& Tb) const {
+  real_only("exSwap");
   std::vector<_Swap> swaps;
   if (status & Tb.status & HAVEBOND) {
     std::vector<int> shared, left;
@@ -768,6 +881,7 @@ std::unordered_map<std::string, std::shared_ptr<GatePlanEntry>> g_gate_cache;
 // by (-1)^(XOR over gates of prtF(b1, q) & prtF(b2, q)).  Only the negated
 // sub-blocks are listed in the device plan, which scales them in place.
 void UniTensor::addGate(const std::vector<_Swap>& swaps) {
+  real_only("addGate");
   if (!(status & HAVEBOND)) detail::fail("Adding swap gates to a tensor without bonds(scalar).");
   if (!(status & HAVEELEM)) detail::fail("Cannot add swap gates to a tensor before setting its elements.");
   const int nb = (int)bonds.size();
@@ -992,6 +1106,7 @@ static tn_gemm_plan gather_plan(const UniTensor& Ta, const std::vector<int>& a_f
 UniTensor contract_impl(UniTensor& Ta, UniTensor& Tb, bool fast, bool allow_gather) {
   if (!(Ta.status & Tb.status & UniTensor::HAVEELEM))
     detail::fail("Cannot perform contraction of two tensors before setting their elements.");
+  if (Ta.m_ctype || Tb.m_ctype) return contract_complex(Ta, Tb, fast, allow_gather);
   if (&Ta == &Tb) {
     UniTensor tmp = Tb;
     return contract_impl(Ta, tmp, fast, allow_gather);
@@ -1088,6 +1203,80 @@ UniTensor contract_impl(UniTensor& Ta, UniTensor& Tb, bool fast, bool allow_gath
   return UniTensor(Ta.at((size_t)0) * Tb.at((size_t)0));
 }
 
+// Complex contraction (UniTensorTools.cpp:58-82, contract(CTYPE, ...)): a
+// real operand is promoted to a complex copy (RtoC) first.  With both
+// complex, B's elements b = br + i bi are expanded to the real 2 x 2 factor
+// X = [[br, bi], [-bi, br]] (one more outgoing pair bond); contracting A's
+// pair bond with X's first pair axis is the complex product, and X's second
+// pair axis is the result's pair bond, last.  All block GEMMs are the real
+// ones of the augmented tensors.
+UniTensor contract_complex(UniTensor& Ta, UniTensor& Tb, bool fast, bool allow_gather) {
+  auto promote = [](const UniTensor& R) {
+    UniTensor C(CTYPE, R.bond(), R.getName());
+    if (R.hasBond()) {
+      std::vector<int> l = R.label();
+      C.setLabel(l);
+    }
+    if (R.m_elemNum)
+      detail::check(tn_copy_strided_f64(C.buf.get(), 2, R.buf.get(), 1, (int64_t)R.m_elemNum, detail::stream()),
+                    "contract: promote to complex");
+    C.status |= UniTensor::HAVEELEM;
+    return C;
+  };
+  if (!Ta.m_ctype) {
+    UniTensor A = promote(Ta);
+    return contract_complex(A, Tb, fast, allow_gather);
+  }
+  if (!Tb.m_ctype) {
+    UniTensor B = promote(Tb);
+    return contract_complex(Ta, B, fast, allow_gather);
+  }
+  const std::vector<int> visA = Ta.label(), visB = Tb.label();
+  const int oldRA = Ta.RBondNum, oldRB = Tb.RBondNum;
+  std::vector<Bond> xb = Tb.bonds;
+  xb.push_back(Bond(BD_OUT, 2));
+  UniTensor X(xb, "", UniTensor::NoZero);
+  std::vector<int> xl = Tb.labels;   // ends with the pair label, matching A's
+  xl.push_back(kPairLabel2);
+  X.setLabel(xl);
+  detail::check(tn_expand_c128(Tb.buf.get(), (int64_t)(Tb.m_elemNum / 2), X.buf.get(), detail::stream()),
+                "contract: complex expansion");
+  X.status |= UniTensor::HAVEELEM;
+  Ta.m_ctype = false;
+  UniTensor Tc;
+  try {
+    Tc = contract_impl(Ta, X, fast, allow_gather);
+  } catch (...) {
+    Ta.m_ctype = true;
+    throw;
+  }
+  Ta.m_ctype = true;
+  Tc.labels.back() = kPairLabel;
+  Tc.m_ctype = true;
+  // the visible bonds shared by A and B (B's side of the reference's permute)
+  std::vector<int> inter, freeB;
+  for (int l : visA)
+    if (std::find(visB.begin(), visB.end(), l) != visB.end()) inter.push_back(l);
+  for (int l : visB)
+    if (std::find(inter.begin(), inter.end(), l) == inter.end()) freeB.push_back(l);
+  if (inter.empty() && Tc.bondNum() > 0) {
+    // outer product: the real path's order (rows of A, rows of B, columns of A, columns of B)
+    std::vector<int> lab;
+    for (int i = 0; i < oldRA; ++i) lab.push_back(visA[i]);
+    for (int i = 0; i < oldRB; ++i) lab.push_back(visB[i]);
+    for (int i = oldRA; i < (int)visA.size(); ++i) lab.push_back(visA[i]);
+    for (int i = oldRB; i < (int)visB.size(); ++i) lab.push_back(visB[i]);
+    Tc.permute(lab, oldRA + oldRB);
+  }
+  if (fast) {
+    // fast leaves B permuted to (shared bonds, free bonds), as the real path
+    std::vector<int> newB(inter);
+    newB.insert(newB.end(), freeB.begin(), freeB.end());
+    Tb.permute(newB, (int)inter.size());
+  }
+  return Tc;
+}
+
 UniTensor contract(UniTensor& Ta, UniTensor& Tb, bool fast) {
   // fast=true leaves the inputs permuted (observable, UniTensorTools.cpp:178-179),
   // so only fast=false may skip the input permutes.
@@ -1110,6 +1299,7 @@ UniTensor operator*(const UniTensor& T, double a) {
 UniTensor operator*(double a, const UniTensor& T) { return T * a; }
 
 UniTensor otimes(const UniTensor& Ta, const UniTensor& Tb) {
+  if (Ta.typeID() != 1 || Tb.typeID() != 1) detail::fail("otimes: not provided for complex tensors");
   UniTensor T1 = Ta, T2 = Tb;
   std::vector<int> l1(T1.bondNum()), l2(T2.bondNum());
   for (size_t i = 0; i < T1.bondNum(); ++i)
@@ -1124,6 +1314,7 @@ UniTensor otimes(const UniTensor& Ta, const UniTensor& Tb) {
 // ---------------------------------------------------------------------------
 // binary file format (UniTensor.cpp:922-986 writes; :486-582 reads)
 void UniTensor::save(const std::string& fname) const {
+  real_only("save");
   if (!(status & HAVEBOND)) detail::fail("Saving a tensor without bonds(scalar) is not supported.");
   FILE* fp = std::fopen(fname.c_str(), "wb");
   if (!fp) detail::fail("Error in writing to file '" + fname + "'.");

@@ -40,6 +40,10 @@ SIGNATURES = {
     "u10_set_stream": [P],
     "u10_set_autotune": [I],
     "u10_tensor_create": [I, P, P, P, P, S, PP],
+    "u10_tensor_create_c": [I, P, P, P, P, S, PP],
+    "u10_tensor_type": [P, ctypes.POINTER(ctypes.c_int)],
+    "u10_svd_c128": [P, I64, I64, P, P, P],
+    "u10_eigh_c128": [P, I64, P, P],
     "u10_bond_combine": [I, P, P, P, P, I, I, PI, PI, PI, P, P, PI, P, P, I],
     "u10_tensor_scalar": [D, PP],
     "u10_tensor_load": [S, PP],
@@ -222,8 +226,17 @@ def combine_bonds(bonds: Sequence[Bond], tp: Optional[int] = None):
     return ot.value, odim.value, rl, dg
 
 
+def _cplx(a):
+    """complex array -> interleaved (re, im) float64, the device layout"""
+    return np.ascontiguousarray(a, dtype=np.complex128).view(np.float64)
+
+
 class UniTensor:
-    def __init__(self, bonds: Sequence[Bond] = None, name: str = "", _handle=None):
+    """UniTensor (UniTensor.h).  dtype=complex makes a complex tensor
+    (UniTensor(CTYPE, bonds)); its elements, raw elements and blocks come
+    back as complex arrays."""
+
+    def __init__(self, bonds: Sequence[Bond] = None, name: str = "", _handle=None, dtype=float):
         if _handle is not None:
             self.h = _handle
             return
@@ -234,10 +247,21 @@ class UniTensor:
         q = np.array([x[0].triple() for r in runs for x in r], np.int32).reshape(-1)
         d = np.array([x[1] for r in runs for x in r], np.int32)
         h = ctypes.c_void_p()
-        _call("u10_tensor_create", len(bonds), types.ctypes.data_as(P), nruns.ctypes.data_as(P),
+        fn = "u10_tensor_create_c" if np.dtype(dtype).kind == "c" else "u10_tensor_create"
+        _call(fn, len(bonds), types.ctypes.data_as(P), nruns.ctypes.data_as(P),
               (q if q.size else np.zeros(1, np.int32)).ctypes.data_as(P),
               (d if d.size else np.zeros(1, np.int32)).ctypes.data_as(P), name.encode(), ctypes.byref(h))
         self.h = h.value
+
+    def typeID(self) -> int:
+        """1 real, 2 complex"""
+        t = ctypes.c_int()
+        _call("u10_tensor_type", self.h, ctypes.byref(t))
+        return t.value
+
+    @property
+    def is_complex(self) -> bool:
+        return self.typeID() == 2
 
     @classmethod
     def scalar(cls, v: float):
@@ -322,26 +346,38 @@ class UniTensor:
         return out
 
     def setElem(self, e):
+        if self.is_complex:
+            a = _cplx(np.ravel(e))
+            if a.size < 2 * self.elemNum():
+                raise ValueError("not enough elements")
+            _call("u10_tensor_set_elem", self.h, a.ctypes.data_as(P))
+            return
         a, p = _arr(e, np.float64)
         if a.size < self.elemNum():
             raise ValueError("not enough elements")
         _call("u10_tensor_set_elem", self.h, p)
 
     def getElem(self) -> np.ndarray:
-        out = np.empty(self.elemNum())
+        cx = self.is_complex
+        out = np.empty(self.elemNum() * (2 if cx else 1))
         _call("u10_tensor_get_elem", self.h, out.ctypes.data_as(P))
-        return out
+        return out.view(np.complex128) if cx else out
 
     def setRawElem(self, raw):
+        if self.is_complex:
+            a = _cplx(np.ravel(raw))
+            _call("u10_tensor_set_raw", self.h, a.ctypes.data_as(P))
+            return
         a, p = _arr(raw, np.float64)
         _call("u10_tensor_set_raw", self.h, p)
 
     def getRawElem(self) -> np.ndarray:
         r, c = ctypes.c_int64(), ctypes.c_int64()
         _call("u10_tensor_raw_shape", self.h, ctypes.byref(r), ctypes.byref(c))
-        out = np.empty((r.value, c.value))
+        cx = self.is_complex
+        out = np.empty((r.value, c.value * (2 if cx else 1)))
         _call("u10_tensor_get_raw", self.h, out.ctypes.data_as(P))
-        return out
+        return out.view(np.complex128) if cx else out
 
     def at(self, idxs):
         a, p = _arr(idxs, np.int64)
@@ -351,18 +387,23 @@ class UniTensor:
 
     def putBlock(self, mat, q: Optional[Qnum] = None):
         q = q or Qnum(0)
-        m, p = _arr(mat, np.float64)
         qa, qp = _arr(q.triple(), np.int32)
+        if self.is_complex:
+            m = np.ascontiguousarray(mat, dtype=np.complex128)
+            _call("u10_tensor_put_block", self.h, qp, m.shape[0], m.shape[1], _cplx(m).ctypes.data_as(P))
+            return
+        m, p = _arr(mat, np.float64)
         _call("u10_tensor_put_block", self.h, qp, m.shape[0], m.shape[1], p)
 
     def getBlock(self, q: Optional[Qnum] = None) -> np.ndarray:
         q = q or Qnum(0)
-        for bq, r, c, o in self.blocks():
+        cx = self.is_complex
+        for bq, r, c, o in self.blocks():     # (a complex block holds 2 doubles per element)
             if bq == q:
                 out = np.empty((r, c))
                 qa, qp = _arr(q.triple(), np.int32)
                 _call("u10_tensor_get_block", self.h, qp, out.ctypes.data_as(P))
-                return out
+                return out.view(np.complex128) if cx else out
         raise KeyError("There is no block with the given quantum number %r" % (q,))
 
     def permute(self, labels, in_bond_num=None):
@@ -486,6 +527,15 @@ def rdotr(a, a_shape, a_diag, b, b_shape, b_diag):
 
 
 def svd(A):
+    """A = U diag(S) VT (Block::svd); complex A through the complex SVD."""
+    if np.iscomplexobj(A):
+        a = np.ascontiguousarray(A, dtype=np.complex128)
+        M, N = a.shape
+        k = min(M, N)
+        U, Sv, VT = np.empty((M, k), np.complex128), np.empty(k), np.empty((k, N), np.complex128)
+        _call("u10_svd_c128", _cplx(a).ctypes.data_as(P), M, N, _cplx(U).ctypes.data_as(P),
+              Sv.ctypes.data_as(P), _cplx(VT).ctypes.data_as(P))
+        return U, Sv, VT
     a, p = _arr(A, np.float64)
     M, N = a.shape
     k = min(M, N)
@@ -495,6 +545,14 @@ def svd(A):
 
 
 def eigh(A):
+    """eigenvalues ascending and eigenvector rows (eigSyDecompose); a complex
+    (Hermitian) A gives rows E with A = E^H diag(w) E."""
+    if np.iscomplexobj(A):
+        a = np.ascontiguousarray(A, dtype=np.complex128)
+        n = a.shape[0]
+        w, v = np.empty(n), np.empty((n, n), np.complex128)
+        _call("u10_eigh_c128", _cplx(a).ctypes.data_as(P), n, w.ctypes.data_as(P), _cplx(v).ctypes.data_as(P))
+        return w, v
     a, p = _arr(A, np.float64)
     n = a.shape[0]
     w, v = np.empty(n), np.empty((n, n))
