@@ -48,6 +48,16 @@ int tn_mem_stats(size_t* in_use, size_t* cached, size_t* peak, size_t* budget);
 int tn_malloc(void** dptr, size_t bytes);            /* elemAlloc             */
 int tn_free(void* dptr);                             /* elemFree              */
 int tn_pool_trim(void);                              /* release cached blocks */
+/* The stream the calling thread's tn_malloc/tn_free are ordered on (NULL =
+ * the library's default stream).  A freed block is reused at once by an
+ * allocation on the stream it was freed on; an allocation on another stream
+ * first makes that stream wait for the freeing stream's work.  So a caller
+ * frees a buffer after enqueuing its last use on its pool stream, or after
+ * synchronising the stream that used it.  A stream given here outlives the
+ * blocks freed on it, or is destroyed through tn_stream_destroy (which
+ * drains it first).  No counterpart in the reference, whose frees are
+ * synchronous (cudaFree). */
+int tn_pool_set_stream(void* stream);
 int tn_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
 int tn_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
 /* Pinned host memory and an asynchronous device-to-host copy into it: the

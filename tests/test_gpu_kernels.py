@@ -997,6 +997,53 @@ def test_lanczos_c_abi(tn, ora):
     assert abs(e0.value - e_o) <= 1e-12 * np.abs(np.linalg.eigvalsh(S)).max()
 
 
+def test_pool_reuse_across_streams_waits_for_the_freeing_stream(tn):
+    """A block freed on stream A while A still writes it, then allocated on
+    stream B and cleared there: B's clear lands after A's copy."""
+    import ctypes
+    n = 1 << 25                                   # 256 MiB of doubles
+    tn.call("tn_pool_trim")                       # no other cached block of this size
+    a, b = tn.stream_create(), tn.stream_create()
+    try:
+        src = tn.DeviceArray.from_host(np.ones(n))
+        tn.pool_set_stream(a)
+        x = tn.DeviceArray(n)
+        ptr = x.ptr
+        for _ in range(4):                        # ~0.3 ms of copies queued on A
+            tn.call("tn_memcpy_d2d", ctypes.c_void_p(x.ptr), ctypes.c_void_p(src.ptr), 8 * n, a)
+        x.free()
+        tn.pool_set_stream(b)
+        y = tn.DeviceArray(n)
+        assert y.ptr == ptr                       # the same block, reused on B
+        tn.call("tn_zero", ctypes.c_void_p(y.ptr), 8 * n, b)
+        tn.call("tn_stream_sync", b)
+        out = y.to_host(b)
+        assert not out.any()
+        tn.call("tn_stream_sync", a)
+    finally:
+        tn.pool_set_stream(None)
+        tn.call("tn_stream_destroy", a)
+        tn.call("tn_stream_destroy", b)
+
+
+def test_lanczos_non_finite_is_an_error(tn):
+    """A NaN or an overflow in the operator is reported as no convergence
+    instead of a Krylov closure with a meaningless E0."""
+    import ctypes
+    n = 64
+    rng = np.random.default_rng(12)
+    X = rng.uniform(-1, 1, (n, n))
+    for bad in (np.nan, np.inf):
+        S = X + X.T
+        S[3, 5] = S[5, 3] = bad
+        A = tn.DeviceArray.from_host(S)
+        psi = tn.DeviceArray.from_host(rng.uniform(-1, 1, n))
+        it, e0, conv = ctypes.c_int64(100), ctypes.c_double(), ctypes.c_int()
+        with pytest.raises(tn.TNError, match="non-finite"):
+            tn.call("tn_lanczos_f64", tn._p(A), n, tn._p(psi), ctypes.byref(it), 5e-15, ctypes.byref(e0),
+                    ctypes.byref(conv), None)
+
+
 @pytest.mark.parametrize("dims,perm", [((7,), (0,)), ((3, 5), (1, 0)), ((4, 3, 5), (2, 0, 1)),
                                        ((256, 2, 2, 256), (3, 1, 0, 2)), ((2, 3, 1, 4, 5), (4, 2, 0, 3, 1))])
 def test_complex_permute_bit_exact(tn, dims, perm):

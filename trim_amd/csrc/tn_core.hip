@@ -37,8 +37,17 @@ struct DeviceState {
   size_t hbm = 0;
   char arch[64] = {0};
   hipStream_t stream = nullptr;  // library default stream (default priority)
-  // pool
-  std::multimap<size_t, void*> free_blocks;       // size -> block
+  // pool.  A freed block keeps the stream it was freed on (the freeing
+  // thread's pool stream): an allocation on that stream takes it at once
+  // (stream order protects the earlier uses); one on another stream first
+  // makes its stream wait for an event recorded on the freeing stream.
+  // A null tag means no work can be pending on the block.
+  struct Freed {
+    void* p;
+    hipStream_t s;
+  };
+  std::multimap<size_t, Freed> free_blocks;       // size -> block
+  hipEvent_t reuse_ev = nullptr;                  // cross-stream reuse fence
   std::unordered_map<void*, size_t> live;         // block -> size
   size_t in_use = 0, cached = 0, peak = 0, budget = 0;
   // Memory of captured graphs.  A block allocated or freed while a capture
@@ -98,6 +107,9 @@ int cu_count() { return st().cus > 0 ? st().cus : 256; }
 
 hipStream_t stream_of(void* h) { return h ? (hipStream_t)h : st().stream; }
 
+// The stream this thread's allocations and frees are ordered on.
+static thread_local void* t_pool_stream = nullptr;
+
 static size_t bin_size(size_t bytes) {
   if (bytes == 0) bytes = 1;
   const size_t small = size_t(1) << 20;
@@ -108,7 +120,7 @@ static size_t bin_size(size_t bytes) {
 
 static int release_cached_locked(DeviceState& s) {
   for (auto& kv : s.free_blocks) {
-    hipError_t e = hipFree(kv.second);
+    hipError_t e = hipFree(kv.second.p);
     if (e != hipSuccess) {
       set_error("hipFree failed: %s", hipGetErrorString(e));
       return TN_EHIP;
@@ -139,16 +151,33 @@ int pool_alloc(void** p, size_t bytes) {
   TN_REQUIRE(s.budget == 0 || s.in_use + sz <= s.budget, TN_ENOMEM,
              "device memory budget exceeded: in use %zu + request %zu > budget %zu",
              s.in_use, sz, s.budget);
-  auto it = s.free_blocks.lower_bound(sz);
-  if (it != s.free_blocks.end() && it->first <= sz + sz / 4) {
-    void* q = it->second;
-    size_t qs = it->first;
-    s.free_blocks.erase(it);
+  // a block of this stream (or idle) first; else one freed on another stream,
+  // fenced by an event (not inside a capture: the graph could not wait on it)
+  const hipStream_t mine = stream_of(t_pool_stream);
+  auto pick = s.free_blocks.end();
+  int seen = 0;
+  for (auto it = s.free_blocks.lower_bound(sz); it != s.free_blocks.end() && it->first <= sz + sz / 4 && seen < 16;
+       ++it, ++seen) {
+    if (it->second.s == mine || it->second.s == nullptr) {
+      pick = it;
+      break;
+    }
+    if (pick == s.free_blocks.end() && !s.capturing) pick = it;
+  }
+  if (pick != s.free_blocks.end()) {
+    const DeviceState::Freed f = pick->second;
+    if (f.s != nullptr && f.s != mine) {
+      if (!s.reuse_ev) TN_HIP_CHECK(hipEventCreateWithFlags(&s.reuse_ev, hipEventDisableTiming));
+      TN_HIP_CHECK(hipEventRecord(s.reuse_ev, f.s));
+      TN_HIP_CHECK(hipStreamWaitEvent(mine, s.reuse_ev, 0));
+    }
+    const size_t qs = pick->first;
+    s.free_blocks.erase(pick);
     s.cached -= qs;
-    s.live[q] = qs;
+    s.live[f.p] = qs;
     s.in_use += qs;
     s.peak = std::max(s.peak, s.in_use);
-    *p = q;
+    *p = f.p;
     return TN_OK;
   }
   void* q = nullptr;
@@ -182,7 +211,7 @@ static void free_locked(DeviceState& s, void* p) {
   const size_t sz = it->second;
   s.live.erase(it);
   s.in_use -= sz;
-  s.free_blocks.emplace(sz, p);
+  s.free_blocks.emplace(sz, DeviceState::Freed{p, stream_of(t_pool_stream)});
   s.cached += sz;
 }
 
@@ -363,7 +392,22 @@ int tn_stream_create(void** stream) {
 
 int tn_stream_destroy(void* stream) {
   if (!stream) return TN_OK;
+  // blocks freed on it are idle once it drains: untag them before the
+  // handle goes away (a later fence would record on a dead stream)
+  TN_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  {
+    DeviceState& s = st();
+    std::lock_guard<std::mutex> g(s.mu);
+    for (auto& kv : s.free_blocks)
+      if (kv.second.s == (hipStream_t)stream) kv.second.s = nullptr;
+  }
+  if (t_pool_stream == stream) t_pool_stream = nullptr;
   TN_HIP_CHECK(hipStreamDestroy((hipStream_t)stream));
+  return TN_OK;
+}
+
+int tn_pool_set_stream(void* stream) {
+  t_pool_stream = stream;
   return TN_OK;
 }
 

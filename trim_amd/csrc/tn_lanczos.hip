@@ -373,6 +373,12 @@ static int lanczos_impl(const double* A, int64_t n, bool cplx, double* psi, int6
     }
     const double alpha = hab[2 * it];
     beta = hab[2 * it + 1];
+    // an overflow or NaN is no Krylov closure: a NaN beta fails the closure
+    // test below and the Sturm count skips NaN entries, so it is caught here
+    if (!std::isfinite(alpha) || !std::isfinite(beta)) {
+      tn::set_error("lanczos: step %lld gave a non-finite alpha (%g) or beta (%g)", (long long)it, alpha, beta);
+      return fail(TN_ENOCONV);
+    }
     As.push_back(alpha);
     if (beta > kBetaErr) {
       Bs.push_back(it < M - 1 ? beta : 0.0);

@@ -20,7 +20,10 @@ thread_local void* t_stream = nullptr;
 
 void setGemmMode(GemmMode m) { t_gemm_mode = (int)m; }
 GemmMode gemmMode() { return (GemmMode)t_gemm_mode; }
-void setStream(void* s) { t_stream = s; }
+void setStream(void* s) {
+  t_stream = s;
+  tn_pool_set_stream(s);   // temporaries freed by this thread are ordered on it
+}
 void* currentStream() { return t_stream; }
 void synchronize() { detail::check(tn_stream_sync(t_stream), "synchronize"); }
 

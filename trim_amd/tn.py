@@ -32,6 +32,7 @@ SIGNATURES = {
     "tn_malloc": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t],
     "tn_free": [P],
     "tn_pool_trim": [],
+    "tn_pool_set_stream": [P],
     "tn_memcpy_h2d": [P, P, ctypes.c_size_t, P],
     "tn_memcpy_d2h": [P, P, ctypes.c_size_t, P],
     "tn_memcpy_d2d": [P, P, ctypes.c_size_t, P],
@@ -368,6 +369,11 @@ def stream_create():
     h = ctypes.c_void_p()
     call("tn_stream_create", ctypes.byref(h))
     return h.value
+
+
+def pool_set_stream(stream):
+    """The stream this thread's allocations and frees are ordered on."""
+    call("tn_pool_set_stream", stream)
 
 
 class Event:
