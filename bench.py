@@ -43,7 +43,8 @@ F64_MFMA_PEAK_TFLOPS = 78.6
 DIMS = (256, 2, 2, 256)
 # FP64 GEMM tile shapes by autotuner id: BM x BN block tile, KC k per stage
 F64_TILES = {0: "32x32,kc32", 1: "64x64,kc64", 2: "128x64,kc32", 3: "64x128,kc32", 4: "64x32,kc64",
-             5: "32x32,kc32,512 threads", 6: "32x32,kc64,512 threads", 7: "32x32,kc128,512 threads"}
+             5: "32x32,kc32,512 threads", 6: "32x32,kc64,512 threads", 7: "32x32,kc128,512 threads",
+             8: "32x32,kc32,2 stages in flight"}
 
 
 def splitmix_uniform(seed, n):
@@ -248,7 +249,7 @@ def c3_sector_only(tn, u10, steps, mode=0):
     pairs = {}
     t = time_parts(tn, [("sector", lambda: plan(pa, pb, C))], steps, pairs)["sector"]
     per_cfg = {}
-    for cfg in range(8 if mode in (0, 1) else (4 if mode == 4 else 5)):
+    for cfg in range(9 if mode in (0, 1) else (4 if mode == 4 else 5)):
         plan.set_tile(cfg)
         per_cfg[cfg] = time_parts(tn, [("c", lambda: plan(pa, pb, C))], steps, {})["c"]
     plan.set_tile(info["tile_id"])

@@ -227,7 +227,7 @@ def test_gemm_single_512(tn, mode):
     _check_gemm(tn, mode, [(512, 512, 512)])
 
 
-NTILES = {0: 8, 1: 8, 2: 5, 3: 5, 4: 4}
+NTILES = {0: 9, 1: 9, 2: 5, 3: 5, 4: 4}
 
 
 @pytest.mark.parametrize("mode,tile", [(m, t) for m in (0, 1, 2, 3, 4) for t in range(NTILES[m])])
@@ -247,7 +247,7 @@ def test_gemm_tile_choice_is_bitwise_neutral(tn, mode):
     plan = tn.GemmPlan(mode, np.array(descs, dtype=tn.GEMM_DESC))
     dA, dB = tn.DeviceArray.from_host(A), tn.DeviceArray.from_host(B)
     outs = []
-    for tile in range(NTILES[mode]):   # FP64/FP32 add three 512-thread tiles
+    for tile in range(NTILES[mode]):   # FP64/FP32: three 512-thread tiles and a two-stage 32x32
         plan.set_tile(tile)
         dC = tn.DeviceArray((nc,))
         plan(dA, dB, dC)

@@ -116,7 +116,12 @@ constexpr int kKTab = 2048;   // longest K whose offsets are tabulated in LDS
 // NT = 512: two waves per 16x16 sub-tile, one per K chain (steps s even /
 // odd of every chunk), so each SIMD holds two waves; the chains are summed
 // in the same order as with one wave holding both.
-template <typename T, int BM, int BN, int KC, int D, bool VEC, bool KT, int NT = 256>
+//
+// PK: every group reads A's columns and B's rows in the plain layout
+// (offset k and k * ldb), so the K-side offsets are one multiply and the
+// block builds no LDS tables (less LDS, no table pass before the first
+// stage).  The loop leaves right after the multiply of the last stage.
+template <typename T, int BM, int BN, int KC, int D, bool VEC, bool KT, int NT = 256, bool PK = false>
 __global__ void __launch_bounds__(NT) gemm_mfma_k32(const double* __restrict__ A,
                                                      const double* __restrict__ B,
                                                      double* __restrict__ C,
@@ -139,7 +144,8 @@ __global__ void __launch_bounds__(NT) gemm_mfma_k32(const double* __restrict__ A
   typedef T t2 __attribute__((ext_vector_type(2)));
   __shared__ __attribute__((aligned(16))) T As[2][BM][KC + PAD];
   __shared__ __attribute__((aligned(16))) T Bs[2][KC][BN + PAD];
-  __shared__ int32_t tabA[KT ? kKTab : 1], tabB[KT ? kKTab : 1];
+  constexpr bool TAB = KT && !PK;
+  __shared__ int32_t tabA[TAB ? kKTab : 1], tabB[TAB ? kKTab : 1];
 
   const TileDev tile = tiles[blockIdx.x];
   if (tile.group < 0) return;
@@ -184,15 +190,16 @@ __global__ void __launch_bounds__(NT) gemm_mfma_k32(const double* __restrict__ A
   // tabulates a_col(k) and b_row(k) in LDS once, so a stage costs one LDS
   // read per offset instead of a multi-level decode.
   const MapDev a_col = g.a_col, b_row = g.b_row;
-  if constexpr (KT) {
+  const int64_t a_ks = a_col.stride[TN_MAX_MAP_LEVELS - 1], b_ks = b_row.stride[TN_MAX_MAP_LEVELS - 1];
+  if constexpr (TAB) {
     for (int k = tid; k < K; k += NT) {
       tabA[k] = (int32_t)map_off(a_col, k);
       tabB[k] = (int32_t)map_off(b_row, k);
     }
     __syncthreads();
   }
-  auto koff_a = [&](int64_t k) -> int64_t { return KT ? (int64_t)tabA[k] : map_off(a_col, k); };
-  auto koff_b = [&](int64_t k) -> int64_t { return KT ? (int64_t)tabB[k] : map_off(b_row, k); };
+  auto koff_a = [&](int64_t k) -> int64_t { return PK ? k * a_ks : (TAB ? (int64_t)tabA[k] : map_off(a_col, k)); };
+  auto koff_b = [&](int64_t k) -> int64_t { return PK ? k * b_ks : (TAB ? (int64_t)tabB[k] : map_off(b_row, k)); };
 
   acc_t acc[NCH][TM][TN];
 #pragma unroll
@@ -322,22 +329,28 @@ __global__ void __launch_bounds__(NT) gemm_mfma_k32(const double* __restrict__ A
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const int64_t t = tb + d;
-      // Only the multiply is skipped past the last stage: every iteration
-      // stores stage t+1 (zeros past the end, never read) to the other LDS
-      // buffer and loads stage t+1+D into the freed register slot, so all
-      // iterations issue the same memory operations.
-      if (t < ns) compute((int)(t & 1));
+      // Every iteration up to the last multiply stores stage t+1 to the
+      // other LDS buffer (k past K zeroed) and loads stage t+1+D (clamped
+      // addresses past the end) into the freed register slot, so all
+      // iterations issue the same memory operations and the waits count
+      // them exactly; after the last multiply the block leaves the loop
+      // (the loads still in flight are never waited for).
+      compute((int)(t & 1));
+      if (t + 1 >= ns) goto done;
       store((d + 1) % D, (int)((t + 1) & 1), t + 1);
       load((d + 1) % D, t + 1 + D);
       __syncthreads();
     }
   }
 
+done:
   // write back: C = chain0 + chain1.  With NCH == 1 the chain-1 waves hand
   // their accumulators to the chain-0 waves through LDS (the stage buffers
-  // are free after the last barrier of the loop).
+  // are free once every wave has passed the barrier below, i.e. finished
+  // its last multiply).
   acc_t other[TM][TN];
   if constexpr (NCH == 1) {
+    __syncthreads();
     acc_t* xch = reinterpret_cast<acc_t*>(&As[0][0][0]);
     static_assert(sizeof(acc_t) * 4 * 64 * TM * TN <= sizeof(As), "exchange fits the A stages");
     if (ch == 1) {
@@ -767,21 +780,31 @@ __global__ void __launch_bounds__(256) gemm_exact_bf16(const double* __restrict_
 struct TileCfg {
   int bm, bn;
 };
-// FP64/FP32: five 256-thread shapes and the 32x32 tile with 512 threads
-// (two waves per sub-tile) at stage depths 32 (id 5), 64 (id 6) and 128
-// (id 7): a deeper stage means fewer barriers per K and a longer prefetch
-// distance in time; split-BF16: five shapes.
-constexpr int kMaxCfg = 8;
-constexpr TileCfg kCfgMfma[kMaxCfg] = {{32, 32}, {64, 64}, {128, 64}, {64, 128}, {64, 32}, {32, 32}, {32, 32}, {32, 32}};
-constexpr TileCfg kCfgBf16[kMaxCfg] = {{32, 32}, {64, 64}, {128, 64}, {64, 128}, {128, 128}, {128, 128}, {128, 128},
-                                       {128, 128}};
+// FP64/FP32: five 256-thread shapes, the 32x32 tile with 512 threads (two
+// waves per sub-tile) at stage depths 32 (id 5), 64 (id 6) and 128 (id 7):
+// a deeper stage means fewer barriers per K and a longer prefetch distance
+// in time; and the 32x32 tile with two stages of loads in flight instead of
+// three (id 8: fewer registers, four blocks per CU, for short K).
+// Split-BF16: five shapes.
+constexpr int kMaxCfg = 9;
+constexpr TileCfg kCfgMfma[kMaxCfg] = {{32, 32}, {64, 64}, {128, 64}, {64, 128}, {64, 32},
+                                       {32, 32}, {32, 32}, {32, 32}, {32, 32}};
+constexpr TileCfg kCfgBf16[kMaxCfg] = {{32, 32},   {64, 64},   {128, 64},  {64, 128}, {128, 128},
+                                       {128, 128}, {128, 128}, {128, 128}, {128, 128}};
 
 // exact split-BF16: 32x32 at 64 and 32 k per stage, 64x32, 32x64 (the
 // slices of six operands and the FP64 accumulators bound the tile size)
-constexpr TileCfg kCfgExact[kMaxCfg] = {{32, 32}, {32, 32}, {64, 32}, {32, 64}, {32, 32}, {32, 32}, {32, 32}, {32, 32}};
+constexpr TileCfg kCfgExact[kMaxCfg] = {{32, 32}, {32, 32}, {64, 32}, {32, 64}, {32, 32},
+                                        {32, 32}, {32, 32}, {32, 32}, {32, 32}};
 bool mfma_mode(int mode) { return mode == TN_GEMM_F64 || mode == TN_GEMM_F32; }
-int num_cfg(int mode) { return mfma_mode(mode) ? 8 : (mode == TN_GEMM_BF16_EXACT ? 4 : 5); }
-int cfg_threads(int mode, int cfg) { return (mfma_mode(mode) && cfg >= 5) ? 512 : 256; }
+int num_cfg(int mode) { return mfma_mode(mode) ? 9 : (mode == TN_GEMM_BF16_EXACT ? 4 : 5); }
+int cfg_threads(int mode, int cfg) { return (mfma_mode(mode) && cfg >= 5 && cfg <= 7) ? 512 : 256; }
+// k values per pipeline stage of a tile shape (the placement weighs tiles
+// by their number of stages)
+int cfg_kc(int mode, int cfg) {
+  if (mfma_mode(mode)) return (cfg == 1 || cfg == 4 || cfg == 6) ? 64 : (cfg == 7 ? 128 : 32);
+  return 32;
+}
 
 TileCfg cfg_shape(int mode, int cfg) {
   return mfma_mode(mode) ? kCfgMfma[cfg] : (mode == TN_GEMM_BF16_EXACT ? kCfgExact[cfg] : kCfgBf16[cfg]);
@@ -800,31 +823,34 @@ ExactFn pick_exact(int cfg) {
 
 typedef void (*KernFn)(const double*, const double*, double*, const GroupDev*, const TileDev*);
 
-template <typename T, bool VEC, bool KT>
+template <typename T, bool VEC, bool KT, bool PK>
 KernFn pick_mfma(int cfg) {
   switch (cfg) {
-    case 0: return gemm_mfma_k32<T, 32, 32, 32, 3, VEC, KT>;
-    case 1: return gemm_mfma_k32<T, 64, 64, 64, 2, VEC, KT>;
-    case 2: return gemm_mfma_k32<T, 128, 64, 32, 2, VEC, KT>;
-    case 3: return gemm_mfma_k32<T, 64, 128, 32, 2, VEC, KT>;
-    case 4: return gemm_mfma_k32<T, 64, 32, 64, 2, VEC, KT>;
-    case 5: return gemm_mfma_k32<T, 32, 32, 32, 3, VEC, KT, 512>;
-    case 6: return gemm_mfma_k32<T, 32, 32, 64, 4, VEC, KT, 512>;
-    case 7: return gemm_mfma_k32<T, 32, 32, 128, 2, VEC, KT, 512>;
+    case 0: return gemm_mfma_k32<T, 32, 32, 32, 3, VEC, KT, 256, PK>;
+    case 1: return gemm_mfma_k32<T, 64, 64, 64, 2, VEC, KT, 256, PK>;
+    case 2: return gemm_mfma_k32<T, 128, 64, 32, 2, VEC, KT, 256, PK>;
+    case 3: return gemm_mfma_k32<T, 64, 128, 32, 2, VEC, KT, 256, PK>;
+    case 4: return gemm_mfma_k32<T, 64, 32, 64, 2, VEC, KT, 256, PK>;
+    case 5: return gemm_mfma_k32<T, 32, 32, 32, 3, VEC, KT, 512, PK>;
+    case 6: return gemm_mfma_k32<T, 32, 32, 64, 4, VEC, KT, 512, PK>;
+    case 7: return gemm_mfma_k32<T, 32, 32, 128, 2, VEC, KT, 512, PK>;
+    case 8: return gemm_mfma_k32<T, 32, 32, 32, 2, VEC, KT, 256, PK>;
   }
   return nullptr;
 }
 
+// pk: plain K-side layouts (the tables are not needed then)
 template <typename T>
-KernFn pick_mfma(int cfg, bool vec, bool kt) {
-  if (vec) return kt ? pick_mfma<T, true, true>(cfg) : pick_mfma<T, true, false>(cfg);
-  return kt ? pick_mfma<T, false, true>(cfg) : pick_mfma<T, false, false>(cfg);
+KernFn pick_mfma(int cfg, bool vec, bool kt, bool pk) {
+  if (pk) return vec ? pick_mfma<T, true, false, true>(cfg) : pick_mfma<T, false, false, true>(cfg);
+  if (vec) return kt ? pick_mfma<T, true, true, false>(cfg) : pick_mfma<T, true, false, false>(cfg);
+  return kt ? pick_mfma<T, false, true, false>(cfg) : pick_mfma<T, false, false, false>(cfg);
 }
 
-KernFn pick_kernel(int mode, int cfg, bool vec, bool kt) {
+KernFn pick_kernel(int mode, int cfg, bool vec, bool kt, bool pk) {
   switch (mode) {
-    case TN_GEMM_F64: return pick_mfma<double>(cfg, vec, kt);
-    case TN_GEMM_F32: return pick_mfma<float>(cfg, vec, kt);
+    case TN_GEMM_F64: return pick_mfma<double>(cfg, vec, kt, pk);
+    case TN_GEMM_F32: return pick_mfma<float>(cfg, vec, kt, pk);
     case TN_GEMM_BF16X3:
       switch (cfg) {
         case 0: return gemm_split_bf16<32, 32, 2, 128>;
@@ -941,6 +967,7 @@ struct tn_gemm_plan_s {
   bool vec_ok = true;     // every group admits 16-byte pair loads
   int64_t max_k = 0;      // longest K over the groups
   bool koff32 = true;     // every K-side offset fits in int32 (LDS tables)
+  bool plain_k = true;    // every group reads A columns / B rows in the plain layout
 };
 
 // Builds (once) the tile list of shape cfg; the plan is unchanged on
@@ -950,16 +977,21 @@ static int ensure_tiles(tn_gemm_plan_s* P, int cfg) {
   if (P->tiles[cfg].built) return TN_OK;
   const TileCfg sh = cfg_shape(P->mode, cfg);
   const int bm = sh.bm, bn = sh.bn;
-  std::vector<TileDev> nat;
+  const int64_t kc = cfg_kc(P->mode, cfg);
+  // Each group's tile grid is cut into px x py compact regions (px * py =
+  // 8, the split that minimises the rows of A plus columns of B one region
+  // reads); regions are the unit of XCD placement, so tiles that share A
+  // rows and B columns share one XCD's L2.
+  struct Region {
+    std::vector<TileDev> tiles;
+    int64_t work = 0;   // tiles x (K stages + 2): the descriptor, first-stage and write-back latency count
+  };
+  std::vector<Region> regions;
   for (size_t gi = 0; gi < P->groups.size(); ++gi) {
     const GroupDev& g = P->groups[gi];
     if (g.M == 0 || g.N == 0 || g.K == 0) continue;
     int64_t tm = (g.M + bm - 1) / bm, tn_ = (g.N + bn - 1) / bn;
     TN_REQUIRE(tm < (1 << 30) && tn_ < (1 << 30), TN_EUNSUPP, "GEMM too large");
-    // The group's tile grid is cut into px x py regions (px * py = 8, the
-    // split that minimises the rows of A plus columns of B one region
-    // reads), listed region by region: with the XCD dealing below each XCD
-    // then computes one compact block of C instead of a full-height strip.
     int64_t px = 1, py = 8, best = INT64_MAX;
     for (int64_t cx : {1, 2, 4, 8}) {
       const int64_t cy = 8 / cx;
@@ -971,20 +1003,49 @@ static int ensure_tiles(tn_gemm_plan_s* P, int cfg) {
       }
     }
     const int64_t rm = (tm + px - 1) / px, rn = (tn_ + py - 1) / py;
+    const int64_t w = (g.K + kc - 1) / kc + 2;
     for (int64_t rx = 0; rx < px; ++rx)
-      for (int64_t ry = 0; ry < py; ++ry)
+      for (int64_t ry = 0; ry < py; ++ry) {
+        Region r;
         for (int64_t b = ry * rn; b < std::min(tn_, (ry + 1) * rn); ++b)
-          for (int64_t a = rx * rm; a < std::min(tm, (rx + 1) * rm); ++a)
-            nat.push_back(TileDev{(int32_t)gi, (int32_t)a, (int32_t)b, 0});
+          for (int64_t a = rx * rm; a < std::min(tm, (rx + 1) * rm); ++a) r.tiles.push_back(TileDev{(int32_t)gi, (int32_t)a, (int32_t)b, 0});
+        r.work = w * (int64_t)r.tiles.size();
+        if (!r.tiles.empty()) regions.push_back(std::move(r));
+      }
   }
-  const int64_t n = (int64_t)nat.size();
-  // XCD-aware placement: natural tile i goes to XCD i / per, slot i % per,
-  // block id = slot * 8 + xcd (the hardware deals block ids round-robin).
+  // XCD-aware placement (the hardware deals block ids round-robin over the
+  // 8 XCDs: block id = slot * 8 + xcd).  Regions go to XCDs heaviest first,
+  // each to the least loaded XCD (longest-processing-time rule), so every
+  // XCD gets about the same number of k stages.  Within an XCD the tiles
+  // run heaviest first, dealt to its CUs in rounds of one tile per CU with
+  // every other round reversed, so no CU collects the heavy tiles of
+  // several rounds.
   const int64_t kX = 8;
-  const int64_t per = (n + kX - 1) / kX;
+  const int64_t cus = std::max<int64_t>(1, tn::cu_count() / kX);
+  std::stable_sort(regions.begin(), regions.end(), [](const Region& a, const Region& b) { return a.work > b.work; });
+  std::vector<std::vector<TileDev>> xcd(kX);
+  int64_t load[kX] = {0};
+  for (const Region& r : regions) {
+    int x = 0;
+    for (int i = 1; i < kX; ++i)
+      if (load[i] < load[x]) x = i;
+    load[x] += r.work;
+    xcd[x].insert(xcd[x].end(), r.tiles.begin(), r.tiles.end());
+  }
+  int64_t n = 0, per = 0;
+  for (auto& v : xcd) {
+    std::stable_sort(v.begin(), v.end(), [&](const TileDev& a, const TileDev& b) {
+      return P->groups[a.group].K > P->groups[b.group].K;
+    });
+    for (size_t r = 1; (int64_t)r * cus < (int64_t)v.size(); r += 2)
+      std::reverse(v.begin() + r * cus, v.begin() + std::min<int64_t>(v.size(), (r + 1) * cus));
+    n += (int64_t)v.size();
+    per = std::max<int64_t>(per, (int64_t)v.size());
+  }
   const int64_t slots = per * kX;
   std::vector<TileDev> lay(slots, TileDev{-1, 0, 0, 0});
-  for (int64_t i = 0; i < n; ++i) lay[(i % per) * kX + i / per] = nat[i];
+  for (int64_t x = 0; x < kX; ++x)
+    for (size_t i = 0; i < xcd[x].size(); ++i) lay[i * kX + x] = xcd[x][i];
   TN_REQUIRE(slots < (int64_t(1) << 31), TN_EUNSUPP, "too many GEMM tiles");
   TileSet t;
   if (slots > 0) {
@@ -1088,6 +1149,7 @@ int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups, const 
     P->groups.push_back(G);
     P->flops += 2.0 * (double)x.M * (double)x.N * (double)x.K;
     if (x.M > 0 && x.N > 0 && x.K > 0 && !desc_vec_ok(x)) P->vec_ok = false;
+    if (x.M > 0 && x.N > 0 && x.K > 0 && (x.a_col.n != 0 || x.b_row.n != 0)) P->plain_k = false;
     if (x.M > 0 && x.N > 0) {
       P->max_k = std::max(P->max_k, x.K);
       if (map_max_off(x.a_col, x.K, 1) >= (int64_t(1) << 31) || map_max_off(x.b_row, x.K, x.ldb) >= (int64_t(1) << 31))
@@ -1249,7 +1311,7 @@ static int exec_cfg(tn_gemm_plan_s* P, int cfg, const double* A, const double* B
     return TN_OK;
   }
   const bool vec = P->vec_ok && ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) & 15) == 0;
-  KernFn k = pick_kernel(P->mode, cfg, vec, P->koff32 && P->max_k <= kKTab);
+  KernFn k = pick_kernel(P->mode, cfg, vec, P->koff32 && P->max_k <= kKTab, P->plain_k);
   TN_REQUIRE(k != nullptr, TN_EINVAL, "no kernel for mode %d tile %d", P->mode, cfg);
   hipLaunchKernelGGL(k, dim3((unsigned)t.nslots), dim3(cfg_threads(P->mode, cfg)), 0, s, A, B, C, P->d_groups,
                      t.d_tiles);
