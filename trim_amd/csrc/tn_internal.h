@@ -28,6 +28,16 @@ int cu_count();
 int pool_alloc(void** p, size_t bytes);
 int pool_free(void* p);
 
+// Device-to-device copy on stream s: a copy kernel (16 bytes per thread,
+// streaming loads and stores when the copy is larger than the on-chip
+// caches) when both pointers and the size are 16-byte multiples, the
+// runtime's copy otherwise.
+int copy_d2d(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+// Copies at or above this size (bytes read) use non-temporal loads and
+// stores: neither side is worth keeping in L2 / the Infinity Cache.
+constexpr size_t kStreamingBytes = size_t(64) << 20;
+
 }  // namespace tn
 
 #define TN_HIP_CHECK(expr)                                                   \
