@@ -120,7 +120,7 @@ constexpr int kKTab = 2048;   // longest K whose offsets are tabulated in LDS
 // PK: every group reads A's columns and B's rows in the plain layout
 // (offset k and k * ldb), so the K-side offsets are one multiply and the
 // block builds no LDS tables (less LDS, no table pass before the first
-// stage).  The loop leaves right after the multiply of the last stage.
+// stage).
 template <typename T, int BM, int BN, int KC, int D, bool VEC, bool KT, int NT = 256, bool PK = false>
 __global__ void __launch_bounds__(NT) gemm_mfma_k32(const double* __restrict__ A,
                                                      const double* __restrict__ B,
@@ -329,21 +329,24 @@ __global__ void __launch_bounds__(NT) gemm_mfma_k32(const double* __restrict__ A
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const int64_t t = tb + d;
-      // Every iteration up to the last multiply stores stage t+1 to the
+      // Every iteration before the last multiply stores stage t+1 to the
       // other LDS buffer (k past K zeroed) and loads stage t+1+D (clamped
-      // addresses past the end) into the freed register slot, so all
+      // addresses past the end) into the freed register slot, so those
       // iterations issue the same memory operations and the waits count
-      // them exactly; after the last multiply the block leaves the loop
-      // (the loads still in flight are never waited for).
-      compute((int)(t & 1));
-      if (t + 1 >= ns) goto done;
-      store((d + 1) % D, (int)((t + 1) & 1), t + 1);
-      load((d + 1) % D, t + 1 + D);
-      __syncthreads();
+      // them exactly; from the last multiply on, the iterations of the
+      // D-group only skip (the loads still in flight are never waited for).
+      // One exit (the loop condition) keeps the accumulators in the same
+      // registers on every path (an exit after the multiply of each slot
+      // made the compiler copy them out of the accumulation registers).
+      if (t < ns) compute((int)(t & 1));
+      if (t + 1 < ns) {
+        store((d + 1) % D, (int)((t + 1) & 1), t + 1);
+        load((d + 1) % D, t + 1 + D);
+        __syncthreads();
+      }
     }
   }
 
-done:
   // write back: C = chain0 + chain1.  With NCH == 1 the chain-1 waves hand
   // their accumulators to the chain-0 waves through LDS (the stage buffers
   // are free once every wave has passed the barrier below, i.e. finished
