@@ -272,7 +272,7 @@ int tn_diag_col_mul_f64(double* mat, const double* diag, int64_t M, int64_t N, v
  * One-sided Jacobi; singular vectors are unique up to a sign per pair.
  * U may be NULL (only S and VT are formed; for M >= N and min(M, N) <= 64
  * this takes the faster QR-preconditioned sweeps, as does the block path
- * for M >= N, M <= 2048). */
+ * for M >= N, M <= 4096). */
 int tn_svd_f64(const double* A, int64_t M, int64_t N, double* U, double* S,
                double* VT, void* stream);
 /* tn_svd_f64 without any allocation: no counterpart in the reference; for

@@ -19,9 +19,9 @@
 // dgesvd (uni10_lapack_cpu.cpp:289-313) or cuSOLVER gesvd
 // (uni10_lapack_gpu.cu:418-500).
 //
-// Problems that need only S and VT (no U; M >= N, M <= 2048) are first
+// Problems that need only S and VT (no U; M >= N, M <= 4096) are first
 // factorised A V0 = Q R by a blocked Householder QR (32-column panels, 16
-// beyond 1024 rows: one
+// beyond 1024 rows, 8 beyond 2048: one
 // workgroup factors a panel and builds its compact-WY factor T, many
 // workgroups apply I - Y T^T Y^T to the trailing columns), and the sweeps
 // run on X = R^T, whose column Gram matrix is close to diagonal (Drmac and
@@ -65,9 +65,11 @@ constexpr int kSetupBlocks = 64;
 constexpr int kNormBlocks = 16;
 constexpr int kMaxN = 8192;       // the sort keeps n norms in LDS (64 KiB)
 // QR panels are held in registers, 128 doubles per thread: 32 columns x 4
-// rows (M <= 1024) or 16 columns x 8 rows (M <= 2048)
-constexpr int kQrMaxM = 2048;
+// rows (M <= 1024), 16 columns x 8 rows (M <= 2048) or 8 columns x 16 rows
+// (M <= 4096)
+constexpr int kQrMaxM = 4096;
 constexpr int kQrWideMaxM = 1024;
+constexpr int kQrMidMaxM = 2048;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -81,7 +83,7 @@ struct Prob {
   int32_t tr;                             // W = A^T
   int32_t flag_off, rank_off, pos_off;    // int scratch: rotated-in-sweep flags, ranks, sweep order
   int32_t qr;                             // QR-preconditioned: sweeps on R^T, V holds V0
-  int32_t qpw;                            // its QR panel width (32, or 16 beyond 1024 rows)
+  int32_t qpw;                            // its QR panel width (32, 16 beyond 1024 rows, 8 beyond 2048)
   int32_t ms;                             // rows the sweeps touch (mp; n rounded up to 16 for QR)
   const int* mask;                        // with mask_idx >= 0: skipped while mask[mask_idx] == 0
   int64_t mask_idx;
@@ -961,8 +963,8 @@ struct Plan {
   tn_gemm_plan warm = nullptr;    // W = A V0 of the warm-started problems
   std::vector<int> gd_prob, qd_prob, qg_prob, qh_prob;   // problem of each group of warm / qr_vt / qr_g / qr_h
   int max_sw = kMaxSweeps;        // sweep cap (tn_svd_set_max_sweeps)
-  int* d_qlist = nullptr;         // QR-preconditioned problems: 32-wide panels first, then 16-wide
-  int nqr = 0, nqr32 = 0, max_n32 = 0, max_n16 = 0;
+  int* d_qlist = nullptr;         // QR-preconditioned problems: 32-wide panels first, then 16-, then 8-wide
+  int nqr = 0, nqr32 = 0, nqr16 = 0, max_n32 = 0, max_n16 = 0, max_n8 = 0;
   tn_gemm_plan qr_vt = nullptr;   // their VT = U_X^T V0^T
   tn_gemm_plan qr_g = nullptr;    // G = VT VT^T
   tn_gemm_plan qr_h = nullptr;    // H = G VT (then VT = (3 VT - H) / 2)
@@ -1014,7 +1016,7 @@ int build(Plan** out, const std::vector<Spec>& specs, int max_sweeps) {
   int64_t scr = 0, iscr = kMaxSweeps;
   std::vector<int2> map;
   std::vector<tn_gemm_desc_ex> gd, qd, qg, qh;
-  std::vector<int> qlist, qlist16;
+  std::vector<int> qlist, qlist16, qlist8;
   for (size_t i = 0; i < specs.size(); ++i) {
     const Spec& s = specs[i];
     Prob p{};
@@ -1059,7 +1061,7 @@ int build(Plan** out, const std::vector<Spec>& specs, int max_sweeps) {
     // only S and VT wanted: QR-preconditioned sweeps (W held by a panel
     // workgroup's registers up to kQrMaxM rows)
     p.qr = (s.u_off < 0 && !p.tr && m <= kQrMaxM) ? 1 : 0;
-    p.qpw = (m <= kQrWideMaxM) ? kPW : kPW / 2;
+    p.qpw = (m <= kQrWideMaxM) ? kPW : (m <= kQrMidMaxM ? kPW / 2 : kPW / 4);
     p.ms = p.qr ? (int32_t)round_up(n, 16) : p.mp;
     p.y_off = p.t_off = p.ut_off = -1;
     if (p.qr) {
@@ -1069,11 +1071,13 @@ int build(Plan** out, const std::vector<Spec>& specs, int max_sweeps) {
       scr += kPW * kPW;
       p.ut_off = scr;
       scr += n * n;
-      (p.qpw == kPW ? qlist : qlist16).push_back((int)i);
+      (p.qpw == kPW ? qlist : (p.qpw == kPW / 2 ? qlist16 : qlist8)).push_back((int)i);
       if (p.qpw == kPW)
         P->max_n32 = std::max(P->max_n32, (int)n);
-      else
+      else if (p.qpw == kPW / 2)
         P->max_n16 = std::max(P->max_n16, (int)n);
+      else
+        P->max_n8 = std::max(P->max_n8, (int)n);
       // VT(k, r) = sum_i Ut(k, i) V0T(i, r), V0T kept in V (row stride nvp)
       tn_gemm_desc_ex g{};
       g.M = n;
@@ -1152,7 +1156,9 @@ int build(Plan** out, const std::vector<Spec>& specs, int max_sweeps) {
   if (rc == TN_OK) rc = tn_memcpy_h2d(P->d_map, map.data(), sizeof(int2) * map.size(), nullptr);
   if (rc == TN_OK && !gd.empty()) rc = tn_gemm_plan_create_ex(&P->warm, TN_GEMM_F64, (int64_t)gd.size(), gd.data());
   P->nqr32 = (int)qlist.size();
+  P->nqr16 = (int)qlist16.size();
   qlist.insert(qlist.end(), qlist16.begin(), qlist16.end());
+  qlist.insert(qlist.end(), qlist8.begin(), qlist8.end());
   P->nqr = (int)qlist.size();
   if (rc == TN_OK && P->nqr > 0) rc = tn::pool_alloc((void**)&P->d_qlist, sizeof(int) * qlist.size());
   if (rc == TN_OK && P->nqr > 0) rc = tn_memcpy_h2d(P->d_qlist, qlist.data(), sizeof(int) * qlist.size(), nullptr);
@@ -1180,8 +1186,8 @@ int launch(Plan* P, const double* A, const double* V0T, double* U, double* S, do
   }
   if (P->nqr > 0) {
     const unsigned nq = (unsigned)P->nqr;
-    // panels of the 32-wide problems (list head), then of the 16-wide ones
-    const int n32 = P->nqr32, n16 = P->nqr - P->nqr32;
+    // panels of the 32-wide problems (list head), then of the 16- and 8-wide ones
+    const int n32 = P->nqr32, n16 = P->nqr16, n8 = P->nqr - P->nqr32 - P->nqr16;
     for (int p = 0; n32 > 0 && p < (P->max_n32 + kPW - 1) / kPW; ++p) {
       hipLaunchKernelGGL((bj_qr_panel<kPW, 4>), dim3((unsigned)n32), dim3(kThreads), 0, st, P->d_probs, P->d_qlist,
                          P->d_scr, p);
@@ -1202,6 +1208,18 @@ int launch(Plan* P, const double* A, const double* V0T, double* U, double* S, do
       if (tiles > 0) {
         hipLaunchKernelGGL(bj_qr_update<kPN>, dim3((unsigned)tiles, (unsigned)n16), dim3(kThreads), 0, st, P->d_probs,
                            P->d_qlist + n32, P->d_scr, p);
+        TN_LAUNCH_CHECK();
+      }
+    }
+    constexpr int kP8 = kPW / 4;
+    for (int p = 0; n8 > 0 && p < (P->max_n8 + kP8 - 1) / kP8; ++p) {
+      hipLaunchKernelGGL((bj_qr_panel<kP8, 16>), dim3((unsigned)n8), dim3(kThreads), 0, st, P->d_probs,
+                         P->d_qlist + n32 + n16, P->d_scr, p);
+      TN_LAUNCH_CHECK();
+      const int tiles = (P->max_n8 - kP8 * (p + 1) + 31) / 32;
+      if (tiles > 0) {
+        hipLaunchKernelGGL(bj_qr_update<kP8>, dim3((unsigned)tiles, (unsigned)n8), dim3(kThreads), 0, st, P->d_probs,
+                           P->d_qlist + n32 + n16, P->d_scr, p);
         TN_LAUNCH_CHECK();
       }
     }
