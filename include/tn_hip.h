@@ -187,11 +187,19 @@ typedef struct {
   int64_t ext[TN_MAX_MAP_LEVELS];
   int64_t stride[TN_MAX_MAP_LEVELS];
 } tn_index_map;
+/* Optional row-scaled second output (all zero: none).  With rs_div > 0 the
+ * launch also writes D(m,n) = C(m,n) * S[rs_off + m / rs_div] to
+ * C[d_off + c_row(m) + c_col(n)]; S and D are addressed from C's base
+ * pointer.  The stored C value times the scale is rounded once, exactly as
+ * the separate diagRowMul pass it replaces (uni10_lapack_cpu.cpp:46-50,
+ * uni10_lapack_gpu.cu:312-339), which it saves: one read of C and one
+ * launch (the batched TEBD engine's theta = lambda C). */
 typedef struct {
   int64_t M, N, K;
   int64_t a_off, b_off, c_off;
   int64_t lda, ldb, ldc;
   tn_index_map a_row, a_col, b_row, b_col, c_row, c_col;
+  int64_t d_off, rs_off, rs_div;
 } tn_gemm_desc_ex;
 int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups,
                            const tn_gemm_desc_ex* descs);

@@ -180,3 +180,27 @@ def test_seam_library_exports_the_reference_signatures(built):
         assert sig in exported, sig
     deps = subprocess.run(["ldd", lib], stdout=subprocess.PIPE, text=True).stdout
     assert "libtn_hip.so" in deps and "libuni10_amd.so" not in deps
+
+
+def test_descriptor_layouts_match_the_c_header(tmp_path):
+    """The ctypes binding's numpy records for the GEMM descriptors and index
+    maps have the size and field offsets of the C structs in tn_hip.h (a C
+    program built with gcc against the header prints them)."""
+    from trim_amd import tn
+    fields = {"tn_gemm_desc": tn.GEMM_DESC, "tn_index_map": tn.INDEX_MAP, "tn_gemm_desc_ex": tn.GEMM_DESC_EX}
+    src = ["#include <stdio.h>", "#include <stddef.h>", '#include "tn_hip.h"', "int main(void) {"]
+    for st, dt in fields.items():
+        src.append('printf("%s %%zu\\n", sizeof(%s));' % (st, st))
+        for name in dt.names:
+            src.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (st, name, st, name))
+    src.append("return 0; }")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(c), "-o", str(exe)], check=True)
+    out = dict(l.split() for l in subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True,
+                                                 check=True).stdout.splitlines())
+    for st, dt in fields.items():
+        assert int(out[st]) == dt.itemsize, st
+        for name in dt.names:
+            assert int(out["%s.%s" % (st, name)]) == dt.fields[name][1], (st, name)

@@ -222,6 +222,61 @@ def _check_gemm(tn, mode, shapes, seed=0, tile=None):
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
+def test_gemm_row_scaled_second_output(tn, mode):
+    """The row-scaled second output (d_off, rs_off, rs_div) on every tile
+    shape: C is bit-identical to the same plan without it, and D(m, n) is
+    exactly C(m, n) * S[rs_off + m // rs_div] (the product numpy forms of
+    the stored C), also through C index maps and for a K = 0 group, whose C
+    and D are zeros; the bits of a separate row-scaling pass."""
+    _say("gemm row-scaled second output mode=%d" % mode)
+    shapes = [(40, 24, 17), (7, 5, 0), (66, 33, 64), (1, 1, 1), (96, 64, 48)]
+    descs, A, B, nc = _group_case(71 + mode, shapes)
+    rng = np.random.default_rng(9)
+    ex = np.zeros(len(descs), dtype=tn.GEMM_DESC_EX)
+    ns, nd, divs = 0, 0, []
+    for g, (M, N, K, ao, bo, co, lda, ldb, ldc) in enumerate(descs):
+        ex[g]["M"], ex[g]["N"], ex[g]["K"] = M, N, K
+        ex[g]["a_off"], ex[g]["b_off"], ex[g]["c_off"] = ao, bo, co
+        ex[g]["lda"], ex[g]["ldb"], ex[g]["ldc"] = lda, ldb, ldc
+        if g == 2:   # C written transposed through index maps
+            ex[g]["c_row"] = tn.index_map([(M, 1)])
+            ex[g]["c_col"] = tn.index_map([(N, M)])
+        div = [3, 1, 11, 1, 8][g]
+        divs.append(div)
+    # the buffer handed as C holds C, then D, then the scales S
+    S = rng.uniform(0.1, 2.0, sum((M + d - 1) // d for (M, *_r), d in zip(descs, divs)))
+    so = 0
+    for g, ((M, *_r), div) in enumerate(zip(descs, divs)):
+        ex[g]["d_off"] = nc + ex[g]["c_off"]
+        ex[g]["rs_off"] = 2 * nc + so
+        ex[g]["rs_div"] = div
+        so += (M + div - 1) // div
+    plain = ex.copy()
+    plain["rs_div"] = 0
+    dA, dB = tn.DeviceArray.from_host(A), tn.DeviceArray.from_host(B)
+    ref = tn.DeviceArray((nc,))
+    tn.GemmPlan(mode, plain)(dA, dB, ref)
+    Cref = ref.to_host()
+    for tile in range(NTILES[mode]):
+        plan = tn.GemmPlan(mode, ex)
+        plan.set_tile(tile)
+        dC = tn.DeviceArray.from_host(np.concatenate([np.full(2 * nc, np.nan), S]))
+        plan(dA, dB, dC)
+        out = dC.to_host()
+        C, D = out[:nc], out[nc:2 * nc]
+        assert C.tobytes() == Cref.tobytes(), (mode, tile)
+        for g, (M, N, K, ao, bo, co, *_r) in enumerate(descs):
+            if g == 2:
+                Cg, Dg = C[co:co + M * N].reshape(N, M).T, D[co:co + M * N].reshape(N, M).T
+            else:
+                Cg, Dg = C[co:co + M * N].reshape(M, N), D[co:co + M * N].reshape(M, N)
+            scale = S[int(ex[g]["rs_off"]) - 2 * nc + np.arange(M) // divs[g]]
+            assert (Dg * 1.0).tobytes() == (Cg * scale[:, None]).tobytes(), (mode, tile, g)
+            if K == 0:
+                assert not np.any(Cg) and not np.any(Dg)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
 def test_gemm_single_512(tn, mode):
     _say("gemm 512^3 mode=%d" % mode)
     _check_gemm(tn, mode, [(512, 512, 512)])

@@ -76,7 +76,16 @@ struct GroupDev {
   int64_t ea_off, eb_off;     // exact mode: offsets of the A-row / B-column scale exponents
   const int* mask;            // with mask_idx >= 0: the group is skipped while mask[mask_idx] == 0
   int64_t mask_idx;
+  int64_t d_off, rs_off;      // with rs_div > 0: D(m,n) = C(m,n) * S[rs_off + m / rs_div] at d_off
+  int64_t rs_div;
 };
+
+// Write-back of one output element: C, and the row-scaled copy when the
+// group asks for it (the stored value times the scale, one rounding).
+__device__ __forceinline__ void put_c(double* __restrict__ C, const GroupDev& g, int64_t off, int64_t m, double v) {
+  C[g.c_off + off] = v;
+  if (g.rs_div > 0) C[g.d_off + off] = v * C[g.rs_off + m / g.rs_div];
+}
 
 __device__ __forceinline__ bool group_off(const GroupDev& g) {
   return g.mask != nullptr && g.mask_idx >= 0 && g.mask[g.mask_idx] == 0;
@@ -155,7 +164,6 @@ __global__ void __launch_bounds__(NT) gemm_mfma_k32(const double* __restrict__ A
   const int64_t m0 = (int64_t)tile.tm * BM, n0 = (int64_t)tile.tn * BN;
   const double* __restrict__ Ab = A + g.a_off;
   const double* __restrict__ Bb = B + g.b_off;
-  double* __restrict__ Cb = C + g.c_off;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -386,8 +394,7 @@ __global__ void __launch_bounds__(NT) gemm_mfma_k32(const double* __restrict__ A
         // accumulator row: f64 16x16x4 -> (l/16) + 4v ; f32 16x16x4 -> 4(l/16) + v
         const int row = (sizeof(T) == 8) ? (lg + 4 * v) : (4 * lg + v);
         const int64_t gm = m0 + wm0 + i * 16 + row;
-        if (gm < M)
-          Cb[map_off(g.c_row, gm) + coff] = (double)acc[0][i][j][v] + (double)other[i][j][v];
+        if (gm < M) put_c(C, g, map_off(g.c_row, gm) + coff, gm, (double)acc[0][i][j][v] + (double)other[i][j][v]);
       }
   }
 }
@@ -428,7 +435,6 @@ __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict_
   const int64_t m0 = (int64_t)tile.tm * BM, n0 = (int64_t)tile.tn * BN;
   const double* __restrict__ Ab = A + g.a_off;
   const double* __restrict__ Bb = B + g.b_off;
-  double* __restrict__ Cb = C + g.c_off;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -534,7 +540,7 @@ __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict_
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int64_t gm = m0 + wm0 + i * 16 + 4 * (lane >> 4) + v;
-        if (gm < M) Cb[map_off(g.c_row, gm) + coff] = (double)acc[i][j][v];
+        if (gm < M) put_c(C, g, map_off(g.c_row, gm) + coff, gm, (double)acc[i][j][v]);
       }
   }
 }
@@ -636,7 +642,6 @@ __global__ void __launch_bounds__(256) gemm_exact_bf16(const double* __restrict_
   const int64_t m0 = (int64_t)tile.tm * BM, n0 = (int64_t)tile.tn * BN;
   const double* __restrict__ Ab = A + g.a_off;
   const double* __restrict__ Bb = B + g.b_off;
-  double* __restrict__ Cb = C + g.c_off;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -773,7 +778,7 @@ __global__ void __launch_bounds__(256) gemm_exact_bf16(const double* __restrict_
         double val = 0.0;
 #pragma unroll
         for (int d = S - 1; d >= 0; --d) val += ldexp(dacc[d][i][j][v], -7 * (d + 2));
-        Cb[map_off(g.c_row, gm) + coff] = ldexp(val, ex[g.ea_off + gm] + en);
+        put_c(C, g, map_off(g.c_row, gm) + coff, gm, ldexp(val, ex[g.ea_off + gm] + en));
       }
   }
 }
@@ -1131,6 +1136,9 @@ int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups, const 
     G.lda = x.lda;
     G.ldb = x.ldb;
     G.ldc = x.ldc;
+    G.d_off = x.d_off;
+    G.rs_off = x.rs_off;
+    G.rs_div = x.rs_div;
     const bool ld_ok = (x.a_row.n || x.M == 0 || x.K == 0 || x.lda >= x.K) &&
                        (x.b_row.n || x.K == 0 || x.N == 0 || x.ldb >= x.N) &&
                        (x.c_row.n || x.M == 0 || x.N == 0 || x.ldc >= x.N);
@@ -1138,8 +1146,9 @@ int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups, const 
     const bool maps_ok = make_map(x.a_row, x.M, x.lda, &G.a_row) && make_map(x.a_col, x.K, 1, &G.a_col) &&
                          make_map(x.b_row, x.K, x.ldb, &G.b_row) && make_map(x.b_col, x.N, 1, &G.b_col) &&
                          make_map(x.c_row, x.M, x.ldc, &G.c_row) && make_map(x.c_col, x.N, 1, &G.c_col);
+    const bool rs_ok = x.rs_div == 0 || (x.rs_div > 0 && x.d_off >= 0 && x.rs_off >= 0);
     if (x.M < 0 || x.N < 0 || x.K < 0 || x.a_off < 0 || x.b_off < 0 || x.c_off < 0 || !ld_ok || !range_ok ||
-        !maps_ok) {
+        !maps_ok || !rs_ok) {
       delete P;
       tn::set_error("GEMM group %lld has an invalid shape, leading dimension or index map", (long long)i);
       return TN_EINVAL;
@@ -1276,7 +1285,7 @@ __global__ void zero_c_kernel(double* __restrict__ C, const GroupDev* __restrict
   const int64_t total = g.M * g.N;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     int64_t m = e / g.N, n = e % g.N;
-    C[g.c_off + map_off(g.c_row, m) + map_off(g.c_col, n)] = 0.0;
+    put_c(C, g, map_off(g.c_row, m) + map_off(g.c_col, n), m, 0.0);
   }
 }
 

@@ -6,7 +6,7 @@
 // with itebd.net, updateA.net):
 //   T1[a,s,b,t]   = sum_m X[a,m,s] Y[m,b,t]                     GEMM  D x c x D
 //   C[a,s',b,t']  = sum_{s,t} T1[a,s,b,t] G[(s,t),(s',t')]      GEMM  c^2 x d^2 x d^2
-//   Theta         = LY[a] C[(a,s'),(b,t')]                      row scaling
+//   Theta         = LY[a] C[(a,s'),(b,t')]                      same launch (row-scaled second output)
 //   Theta         = U S VT                                      SVD (D x D)
 //   LX            = S[:c] / ||S[:c]||,  Y = VT[:c, :]           truncation
 //   X[a,m',s']    = sum_{b,t'} C[a,s',b,t'] VT[m',(b,t')] / ||S[:c]||
@@ -388,19 +388,18 @@ void BatchTebd::build(const std::vector<uint64_t>& seed, double tau, int gemm_mo
   for (int p = 0; p < 2; ++p) {
     std::vector<tn_gemm_desc_ex> d1, d2, d3;
     std::vector<int64_t> svd[7];   // m, n, a, u, s, vt, v0
-    std::vector<tn_vecop> rows, gather, trunc, scale;
+    std::vector<tn_vecop> gather, trunc, scale;
     // the chain of every GEMM group, SVD problem and vector item (for the mask)
-    std::vector<int32_t> c1, c2, c3, cs, crows, cgather, ctrunc, cscale;
+    std::vector<int32_t> c1, c2, c3, cs, cgather, ctrunc, cscale;
     for (size_t i = 0; i < nc; ++i) {
       if (sym_[i])
-        plans_u1((int)i, p, d1, d2, d3, rows, gather, trunc, scale, svd);
+        plans_u1((int)i, p, d1, d2, d3, gather, trunc, scale, svd);
       else
-        plans_dense((int)i, p, d1, d2, d3, rows, gather, trunc, scale, svd);
+        plans_dense((int)i, p, d1, d2, d3, gather, trunc, scale, svd);
       c1.resize(d1.size(), (int32_t)i);
       c2.resize(d2.size(), (int32_t)i);
       c3.resize(d3.size(), (int32_t)i);
       cs.resize(svd[0].size(), (int32_t)i);
-      crows.resize(rows.size(), (int32_t)i);
       cgather.resize(gather.size(), (int32_t)i);
       ctrunc.resize(trunc.size(), (int32_t)i);
       cscale.resize(scale.size(), (int32_t)i);
@@ -428,7 +427,6 @@ void BatchTebd::build(const std::vector<uint64_t>& seed, double tau, int gemm_mo
     auto vec = [&](tn_vecops_plan* v, const std::vector<tn_vecop>& items, const char* what) {
       detail::check(tn_vecops_plan_create(v, (int64_t)items.size(), items.data()), what);
     };
-    vec(&rows_[p], rows, "BatchTebd rows");
     vec(&gather_[p], gather, "BatchTebd gather");
     vec(&trunc_[p], trunc, "BatchTebd trunc");
     vec(&scale_[p], scale, "BatchTebd scale");
@@ -437,7 +435,6 @@ void BatchTebd::build(const std::vector<uint64_t>& seed, double tau, int gemm_mo
     detail::check(tn_gemm_plan_set_mask(g2_[p], active_, c2.data()), "BatchTebd mask");
     detail::check(tn_gemm_plan_set_mask(g3_[p], active_, c3.data()), "BatchTebd mask");
     detail::check(tn_svd_plan_set_mask(svd_[p], active_, cs.data()), "BatchTebd mask");
-    detail::check(tn_vecops_plan_set_mask(rows_[p], active_, crows.data()), "BatchTebd mask");
     detail::check(tn_vecops_plan_set_mask(gather_[p], active_, cgather.data()), "BatchTebd mask");
     detail::check(tn_vecops_plan_set_mask(trunc_[p], active_, ctrunc.data()), "BatchTebd mask");
     detail::check(tn_vecops_plan_set_mask(scale_[p], active_, cscale.data()), "BatchTebd mask");
@@ -446,7 +443,7 @@ void BatchTebd::build(const std::vector<uint64_t>& seed, double tau, int gemm_mo
 }
 
 void BatchTebd::plans_dense(int i, int p, std::vector<tn_gemm_desc_ex>& d1, std::vector<tn_gemm_desc_ex>& d2,
-                            std::vector<tn_gemm_desc_ex>& d3, std::vector<tn_vecop>& rows, std::vector<tn_vecop>&,
+                            std::vector<tn_gemm_desc_ex>& d3, std::vector<tn_vecop>&,
                             std::vector<tn_vecop>& trunc, std::vector<tn_vecop>& scale, std::vector<int64_t>* svd) {
   const int64_t c = chi_[i], kd = d_[i], D = c * kd, dd = kd * kd;
   const Off& f = off_[i];
@@ -462,9 +459,11 @@ void BatchTebd::plans_dense(int i, int p, std::vector<tn_gemm_desc_ex>& d1, std:
   g2.a_col = imap({{kd, c * kd}, {kd, 1}});
   g2.c_row = imap({{c, kd * c * kd}, {c, kd}});
   g2.c_col = imap({{kd, c * kd}, {kd, 1}});
+  // and, in the same launch, Theta = LY[a] C[(a,s'),(b,t')] (row m = (a, b))
+  g2.d_off = f.th[p];
+  g2.rs_off = LY;
+  g2.rs_div = c;
   d2.push_back(g2);
-  // Theta = LY[a] C[(a,s'),(b,t')]
-  rows.push_back(tn_vecop{TN_VECOP_ROWSCALE, 0, D * D, f.cb[p], f.th[p], LY, c * kd * kd});
   // SVD of Theta (D x D) into U, S, VT[p]
   const int64_t sv[7] = {D, D, f.th[p], -1, f.s[p], f.vt[p], f.vt[p]};   // U is not formed
   for (int k = 0; k < 7; ++k) svd[k].push_back(sv[k]);
@@ -484,7 +483,7 @@ void BatchTebd::plans_dense(int i, int p, std::vector<tn_gemm_desc_ex>& d1, std:
 }
 
 void BatchTebd::plans_u1(int i, int p, std::vector<tn_gemm_desc_ex>& d1, std::vector<tn_gemm_desc_ex>& d2,
-                         std::vector<tn_gemm_desc_ex>& d3, std::vector<tn_vecop>& rows, std::vector<tn_vecop>& gather,
+                         std::vector<tn_gemm_desc_ex>& d3, std::vector<tn_vecop>& gather,
                          std::vector<tn_vecop>& trunc, std::vector<tn_vecop>& scale, std::vector<int64_t>* svd) {
   const U1Chain& u = u1_[i];
   const Off& f = off_[i];
@@ -548,6 +547,11 @@ void BatchTebd::plans_u1(int i, int p, std::vector<tn_gemm_desc_ex>& d1, std::ve
         g.b_col = imap({{1, 1}});
         g.c_row = imap({{L.n[a], D}, {L.n[b], 1}});
         g.c_col = imap({{1, 1}});
+        // and theta = LY[a] C in the same launch: theta row part(so) + a_idx,
+        // a_idx = m / n_b
+        g.d_off = g.c_off - f.cb[p] + f.th[p];
+        g.rs_off = f.l[1 - p] + L.off[a];
+        g.rs_div = L.n[b];
         d2.push_back(g);
         fl += 2.0 * M * nc;
       }
@@ -556,14 +560,6 @@ void BatchTebd::plans_u1(int i, int p, std::vector<tn_gemm_desc_ex>& d1, std::ve
   for (size_t j = 0; j < R.q.size(); ++j) {
     const int q = R.q[j];
     const int64_t D = u.D[p][j], k = R.n[j], sq = u.sq[p][j];
-    // theta = LY[a] C, row part by row part
-    for (int s = 0; s < d; ++s) {
-      const int a = L.find(q - u.phys[s]);
-      if (a < 0) continue;
-      const int64_t r0 = u.part[p][j][s] * D;
-      rows.push_back(tn_vecop{TN_VECOP_ROWSCALE, 0, (int64_t)L.n[a] * D, f.cb[p] + sq + r0, f.th[p] + sq + r0,
-                              f.l[1 - p] + L.off[a], D});
-    }
     const int64_t sv[7] = {D, D, f.th[p] + sq, -1, f.s[p] + u.sv[p][j], f.vt[p] + sq, f.vt[p] + sq};
     for (int e = 0; e < 7; ++e) svd[e].push_back(sv[e]);
     // kept singular values to the sector's place, the rest after all kept
@@ -750,13 +746,12 @@ void BatchTebd::release() noexcept {
     tn_gemm_plan_destroy(g2_[p]);
     tn_gemm_plan_destroy(g3_[p]);
     tn_svd_plan_destroy(svd_[p]);
-    tn_vecops_plan_destroy(rows_[p]);
     tn_vecops_plan_destroy(gather_[p]);
     tn_vecops_plan_destroy(trunc_[p]);
     tn_vecops_plan_destroy(scale_[p]);
     g1_[p] = g2_[p] = g3_[p] = nullptr;
     svd_[p] = nullptr;
-    rows_[p] = gather_[p] = trunc_[p] = scale_[p] = nullptr;
+    gather_[p] = trunc_[p] = scale_[p] = nullptr;
   }
   tn_free(active_);
   active_ = nullptr;
@@ -774,7 +769,6 @@ void BatchTebd::update(int p) {
   double* A = arena_.get();
   detail::check(tn_gemm_exec(g1_[p], A, A, A, stream_), "BatchTebd gemm1");
   detail::check(tn_gemm_exec(g2_[p], A, A, A, stream_), "BatchTebd gemm2");
-  detail::check(tn_vecops_plan_exec(rows_[p], A, stream_), "BatchTebd rows");
   last_svd_ = svd_[p];
   detail::check(tn_svd_plan_exec(last_svd_, A, A, A, A, A, stream_), "BatchTebd svd");
   detail::check(tn_vecops_plan_exec(gather_[p], A, stream_), "BatchTebd gather");

@@ -93,10 +93,10 @@ class BatchTebd {
   };
   void layout_u1(int i);
   void plans_dense(int i, int p, std::vector<tn_gemm_desc_ex>& d1, std::vector<tn_gemm_desc_ex>& d2,
-                   std::vector<tn_gemm_desc_ex>& d3, std::vector<tn_vecop>& rows, std::vector<tn_vecop>& gather,
+                   std::vector<tn_gemm_desc_ex>& d3, std::vector<tn_vecop>& gather,
                    std::vector<tn_vecop>& trunc, std::vector<tn_vecop>& scale, std::vector<int64_t>* svd);
   void plans_u1(int i, int p, std::vector<tn_gemm_desc_ex>& d1, std::vector<tn_gemm_desc_ex>& d2,
-                std::vector<tn_gemm_desc_ex>& d3, std::vector<tn_vecop>& rows, std::vector<tn_vecop>& gather,
+                std::vector<tn_gemm_desc_ex>& d3, std::vector<tn_vecop>& gather,
                 std::vector<tn_vecop>& trunc, std::vector<tn_vecop>& scale, std::vector<int64_t>* svd);
   void update(int p);
   void release() noexcept;   // streams, plans, graph and flags (the arena frees itself)
@@ -118,8 +118,7 @@ class BatchTebd {
   int* active_ = nullptr;               // device activity flags, one per chain
   std::vector<int> active_host_;
   bool active_dirty_ = false;
-  tn_vecops_plan rows_[2] = {nullptr, nullptr}, gather_[2] = {nullptr, nullptr}, trunc_[2] = {nullptr, nullptr},
-                 scale_[2] = {nullptr, nullptr};
+  tn_vecops_plan gather_[2] = {nullptr, nullptr}, trunc_[2] = {nullptr, nullptr}, scale_[2] = {nullptr, nullptr};
   tn_svd_plan last_svd_ = nullptr;
   bool warm_[2] = {false, false};
   bool use_graph_ = false;

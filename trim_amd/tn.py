@@ -267,7 +267,8 @@ INDEX_MAP = np.dtype([("n", "<i4"), ("pad", "<i4"), ("ext", "<i8", (4,)), ("stri
 GEMM_DESC_EX = np.dtype([("M", "<i8"), ("N", "<i8"), ("K", "<i8"), ("a_off", "<i8"), ("b_off", "<i8"),
                          ("c_off", "<i8"), ("lda", "<i8"), ("ldb", "<i8"), ("ldc", "<i8"),
                          ("a_row", INDEX_MAP), ("a_col", INDEX_MAP), ("b_row", INDEX_MAP),
-                         ("b_col", INDEX_MAP), ("c_row", INDEX_MAP), ("c_col", INDEX_MAP)])
+                         ("b_col", INDEX_MAP), ("c_row", INDEX_MAP), ("c_col", INDEX_MAP),
+                         ("d_off", "<i8"), ("rs_off", "<i8"), ("rs_div", "<i8")])
 
 
 def index_map(levels):
