@@ -78,6 +78,7 @@ struct GroupDev {
   int64_t mask_idx;
   int64_t d_off, rs_off;      // with rs_div > 0: D(m,n) = C(m,n) * S[rs_off + m / rs_div] at d_off
   int64_t rs_div;
+  int64_t sa_off, sb_off, kp; // exact mode: the group's A / B slices in the plan's slice buffer, padded K
 };
 
 // Write-back of one output element: C, and the row-scaled copy when the
@@ -565,6 +566,10 @@ __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict_
 //     below 2^24, i.e. exact; it is then added to an FP64 accumulator (exact
 //     integers below 2^53) and cleared.
 //   * C(m, n) = 2^(ea[m] + eb[n]) sum_d 2^-7d acc_d, smallest terms first.
+//   * The slices are formed once per launch (exact_slice_kernel, 12 bytes
+//     per operand element in the plan's slice buffer) and the tiles copy
+//     them; slicing in every tile cost each A element one slicing per tile
+//     column (the kernel ran at 9 TFLOP/s FP64-equivalent at 4096^3).
 // The error is the dropped slice products (s + t > S + 1) and remainders:
 // relative to max_k |A(m, k)| max_k |B(k, n)|, at most about K 2^-7(S+1).
 constexpr int kExactSlices = 6;
@@ -616,9 +621,63 @@ __global__ void __launch_bounds__(256) exact_scales_kernel(const double* __restr
   }
 }
 
+// The slices of every operand element, once per launch (blockIdx.y =
+// group): A row m as [slice][m][kp] and B column n as [slice][n][kp] (k
+// fastest, zero past K), each element scaled by the power of two of its row /
+// column and cut into six 7-bit integers x = sum_s q_s 2^-7(s+1) + r, each
+// exact in BF16.  A thread converts eight consecutive k of one row (A) or
+// column (B): one 16-byte store per slice.  B's lanes run along n, so its
+// reads are coalesced rows of B.
+constexpr int kSliceK = 64;   // K padding of the slices: a multiple of every exact tile's k per stage
+__global__ void __launch_bounds__(256) exact_slice_kernel(const double* __restrict__ A, const double* __restrict__ B,
+                                                         const GroupDev* __restrict__ groups,
+                                                         const int32_t* __restrict__ ex, __bf16* __restrict__ sl) {
+  const GroupDev& g = groups[blockIdx.y];
+  if (g.K == 0 || group_off(g)) return;
+  const int64_t k8n = g.kp / 8;
+  const int64_t na = g.M * k8n, nb = g.N * k8n;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < na + nb; e += stride) {
+    const bool isa = e < na;
+    int64_t r, k0;
+    double x[8];
+    if (isa) {
+      r = e / k8n;
+      k0 = 8 * (e % k8n);
+      const int64_t ro = map_off(g.a_row, r);
+      const int sc = ex[g.ea_off + r];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        x[u] = (k0 + u < g.K) ? ldexp(A[g.a_off + ro + map_off(g.a_col, k0 + u)], -sc) : 0.0;
+    } else {
+      const int64_t eb = e - na;
+      r = eb % g.N;
+      k0 = 8 * (eb / g.N);
+      const int64_t co = map_off(g.b_col, r);
+      const int sc = ex[g.eb_off + r];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        x[u] = (k0 + u < g.K) ? ldexp(B[g.b_off + map_off(g.b_row, k0 + u) + co], -sc) : 0.0;
+    }
+    const int64_t rows = isa ? g.M : g.N;
+    __bf16* dst = sl + (isa ? g.sa_off : g.sb_off) + r * g.kp + k0;
+#pragma unroll
+    for (int q = 0; q < kExactSlices; ++q) {
+      bf16x8 v;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const double y = 128.0 * x[u], t = rint(y);
+        x[u] = y - t;
+        v[u] = (__bf16)(float)t;
+      }
+      *reinterpret_cast<bf16x8*>(dst + (int64_t)q * rows * g.kp) = v;
+    }
+  }
+}
+
 template <int BM, int BN, int BK>
-__global__ void __launch_bounds__(256) gemm_exact_bf16(const double* __restrict__ A, const double* __restrict__ B,
-                                                       double* __restrict__ C, const GroupDev* __restrict__ groups,
+__global__ void __launch_bounds__(256) gemm_exact_bf16(const __bf16* __restrict__ sl, double* __restrict__ C,
+                                                       const GroupDev* __restrict__ groups,
                                                        const TileDev* __restrict__ tiles,
                                                        const int32_t* __restrict__ ex) {
   constexpr int S = kExactSlices;
@@ -626,10 +685,6 @@ __global__ void __launch_bounds__(256) gemm_exact_bf16(const double* __restrict_
   constexpr int KP = BK + 8;                       // padded row (bf16 elements)
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
-  constexpr int A_PER = BM * BK / 256;
-  constexpr int B_PER = BN * BK / 256;
-  constexpr int A_RSTEP = 256 / BK;
-  constexpr int B_KSTEP = 256 / BN;
   constexpr int kFlush = 4;                        // 32-k steps between FP64 flushes (128 k)
   __shared__ __attribute__((aligned(16))) __bf16 As[S][BM][KP];    // [slice][m][k]
   __shared__ __attribute__((aligned(16))) __bf16 Bs[S][BN][KP];    // [slice][n][k]
@@ -640,28 +695,11 @@ __global__ void __launch_bounds__(256) gemm_exact_bf16(const double* __restrict_
   if (group_off(g)) return;
   const int64_t M = g.M, N = g.N, K = g.K;
   const int64_t m0 = (int64_t)tile.tm * BM, n0 = (int64_t)tile.tn * BN;
-  const double* __restrict__ Ab = A + g.a_off;
-  const double* __restrict__ Bb = B + g.b_off;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
-
-  const int a_kk = tid % BK;
-  const int a_mm0 = tid / BK;
-  int64_t a_roff[A_PER];
-  int a_e[A_PER];
-#pragma unroll
-  for (int i = 0; i < A_PER; ++i) {
-    const int64_t m = m0 + a_mm0 + i * A_RSTEP;
-    a_roff[i] = (m < M) ? map_off(g.a_row, m) : -1;
-    a_e[i] = (m < M) ? ex[g.ea_off + m] : 0;
-  }
-  const int b_nn = tid % BN;
-  const int b_kk0 = tid / BN;
-  const int64_t b_coff = (n0 + b_nn < N) ? map_off(g.b_col, n0 + b_nn) : -1;
-  const int b_e = (n0 + b_nn < N) ? ex[g.eb_off + n0 + b_nn] : 0;
 
   f4 acc[S][TM][TN];
   double dacc[S][TM][TN][4];
@@ -688,40 +726,49 @@ __global__ void __launch_bounds__(256) gemm_exact_bf16(const double* __restrict_
         }
   };
 
-  double ra[A_PER], rb[B_PER];
+  // A stage of the tile is S x (BM + BN) x BK slices, read as 16-byte
+  // vectors of eight k (rows past M or N read as zeros)
+  constexpr int KV = BK / 8;                       // vectors per row of a stage
+  constexpr int AV = (BM * KV + 255) / 256, BV = (BN * KV + 255) / 256;
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  const u4* __restrict__ sa = reinterpret_cast<const u4*>(sl + g.sa_off);
+  const u4* __restrict__ sb = reinterpret_cast<const u4*>(sl + g.sb_off);
+  const int64_t kpv = g.kp / 8;                    // vectors per slice row
+  u4 ra[S][AV], rb[S][BV];
   auto load_tiles = [&](int64_t k0) {
-    const int64_t ka = k0 + a_kk;
-    const int64_t a_coff = (ka < K) ? map_off(g.a_col, ka) : -1;
+    const int64_t kv0 = k0 / 8;
 #pragma unroll
-    for (int i = 0; i < A_PER; ++i) ra[i] = (a_roff[i] >= 0 && a_coff >= 0) ? Ab[a_roff[i] + a_coff] : 0.0;
+    for (int i = 0; i < AV; ++i) {
+      const int v = tid + 256 * i, row = v / KV, kv = v % KV;
+      const bool ok = v < BM * KV && m0 + row < M;
+      const int64_t o = (m0 + row) * kpv + kv0 + kv;
 #pragma unroll
-    for (int i = 0; i < B_PER; ++i) {
-      const int64_t kb = k0 + b_kk0 + i * B_KSTEP;
-      const int64_t b_roff = (kb < K) ? map_off(g.b_row, kb) : -1;
-      rb[i] = (b_roff >= 0 && b_coff >= 0) ? Bb[b_roff + b_coff] : 0.0;
+      for (int q = 0; q < S; ++q) ra[q][i] = ok ? sa[(int64_t)q * M * kpv + o] : u4{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int i = 0; i < BV; ++i) {
+      const int v = tid + 256 * i, row = v / KV, kv = v % KV;
+      const bool ok = v < BN * KV && n0 + row < N;
+      const int64_t o = (n0 + row) * kpv + kv0 + kv;
+#pragma unroll
+      for (int q = 0; q < S; ++q) rb[q][i] = ok ? sb[(int64_t)q * N * kpv + o] : u4{0, 0, 0, 0};
     }
   };
   auto store_tiles = [&]() {
 #pragma unroll
-    for (int i = 0; i < A_PER; ++i) {
-      double x = ldexp(ra[i], -a_e[i]);
-      const int mm = a_mm0 + i * A_RSTEP;
+    for (int i = 0; i < AV; ++i) {
+      const int v = tid + 256 * i;
+      if (v < BM * KV) {
 #pragma unroll
-      for (int sl = 0; sl < S; ++sl) {
-        const double y = 128.0 * x, q = rint(y);
-        x = y - q;
-        As[sl][mm][a_kk] = (__bf16)(float)q;
+        for (int q = 0; q < S; ++q) *reinterpret_cast<u4*>(&As[q][v / KV][8 * (v % KV)]) = ra[q][i];
       }
     }
 #pragma unroll
-    for (int i = 0; i < B_PER; ++i) {
-      double x = ldexp(rb[i], -b_e);
-      const int kk = b_kk0 + i * B_KSTEP;
+    for (int i = 0; i < BV; ++i) {
+      const int v = tid + 256 * i;
+      if (v < BN * KV) {
 #pragma unroll
-      for (int sl = 0; sl < S; ++sl) {
-        const double y = 128.0 * x, q = rint(y);
-        x = y - q;
-        Bs[sl][b_nn][kk] = (__bf16)(float)q;
+        for (int q = 0; q < S; ++q) *reinterpret_cast<u4*>(&Bs[q][v / KV][8 * (v % KV)]) = rb[q][i];
       }
     }
   };
@@ -818,7 +865,7 @@ TileCfg cfg_shape(int mode, int cfg) {
   return mfma_mode(mode) ? kCfgMfma[cfg] : (mode == TN_GEMM_BF16_EXACT ? kCfgExact[cfg] : kCfgBf16[cfg]);
 }
 
-typedef void (*ExactFn)(const double*, const double*, double*, const GroupDev*, const TileDev*, const int32_t*);
+typedef void (*ExactFn)(const __bf16*, double*, const GroupDev*, const TileDev*, const int32_t*);
 ExactFn pick_exact(int cfg) {
   switch (cfg) {
     case 0: return gemm_exact_bf16<32, 32, 64>;
@@ -971,6 +1018,9 @@ struct tn_gemm_plan_s {
   GroupDev* d_groups = nullptr;
   int32_t* d_exp = nullptr;   // exact mode: scale exponents (A rows, then B columns, per group)
   int64_t max_rows = 0;       // exact mode: blocks of the scale kernel for the largest group
+  __bf16* d_sl = nullptr;     // exact mode: the operands' BF16 slices (per group: A, then B)
+  int64_t sl_elems = 0;       // their number
+  int64_t max_slice_v = 0;    // exact mode: 8-k vectors of the largest group (slice kernel grid)
   double flops = 0;
   bool vec_ok = true;     // every group admits 16-byte pair loads
   int64_t max_k = 0;      // longest K over the groups
@@ -1108,6 +1158,7 @@ static int default_cfg(const tn_gemm_plan_s* P) {
 static void destroy_plan(tn_gemm_plan_s* P) {
   tn::pool_free(P->d_groups);
   tn::pool_free(P->d_exp);
+  tn::pool_free(P->d_sl);
   for (TileSet& t : P->tiles) tn::pool_free(t.d_tiles);
   delete P;
 }
@@ -1158,6 +1209,13 @@ int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups, const 
     G.ea_off = nexp;
     G.eb_off = nexp + x.M;
     P->max_rows = std::max(P->max_rows, (x.M + 3) / 4 + (x.N + 63) / 64);   // exact mode: scale blocks
+    if (mode == TN_GEMM_BF16_EXACT && x.M > 0 && x.N > 0 && x.K > 0) {
+      G.kp = (x.K + kSliceK - 1) / kSliceK * kSliceK;
+      G.sa_off = P->sl_elems;
+      G.sb_off = G.sa_off + kExactSlices * x.M * G.kp;
+      P->sl_elems = G.sb_off + kExactSlices * x.N * G.kp;
+      P->max_slice_v = std::max(P->max_slice_v, (x.M + x.N) * (G.kp / 8));
+    }
     P->groups.push_back(G);
     P->flops += 2.0 * (double)x.M * (double)x.N * (double)x.K;
     if (x.M > 0 && x.N > 0 && x.K > 0 && !desc_vec_ok(x)) P->vec_ok = false;
@@ -1176,6 +1234,7 @@ int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups, const 
     if (rc == TN_OK && mode == TN_GEMM_BF16_EXACT) {
       const GroupDev& last = P->groups.back();
       rc = tn::pool_alloc((void**)&P->d_exp, sizeof(int32_t) * std::max<int64_t>(1, last.eb_off + last.N));
+      if (rc == TN_OK) rc = tn::pool_alloc((void**)&P->d_sl, sizeof(__bf16) * std::max<int64_t>(8, P->sl_elems));
     }
   }
   if (rc == TN_OK) rc = select_cfg(P, default_cfg(P));
@@ -1318,7 +1377,13 @@ static int exec_cfg(tn_gemm_plan_s* P, int cfg, const double* A, const double* B
     hipLaunchKernelGGL(exact_scales_kernel, dim3(std::max(1u, gx), (unsigned)P->groups.size()), dim3(256), 0, s, A,
                        B, P->d_groups, P->d_exp);
     TN_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k, dim3((unsigned)t.nslots), dim3(256), 0, s, A, B, C, P->d_groups, t.d_tiles, P->d_exp);
+    // every operand element sliced once per launch (not once per tile that
+    // reads it)
+    const unsigned sx = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (P->max_slice_v + 255) / 256), 2048);
+    hipLaunchKernelGGL(exact_slice_kernel, dim3(sx, (unsigned)P->groups.size()), dim3(256), 0, s, A, B,
+                       P->d_groups, P->d_exp, P->d_sl);
+    TN_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k, dim3((unsigned)t.nslots), dim3(256), 0, s, P->d_sl, C, P->d_groups, t.d_tiles, P->d_exp);
     TN_LAUNCH_CHECK();
     return TN_OK;
   }
