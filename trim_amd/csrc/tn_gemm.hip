@@ -563,9 +563,12 @@ __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict_
 //     per diagonal d = s + t on v_mfma_f32_16x16x32_bf16: every product is
 //     an integer of magnitude at most 2^14 and a diagonal gains at most
 //     6 * 32 * 2^14 per 32 k, so after 128 k its FP32 sum is still an integer
-//     below 2^24, i.e. exact; it is then added to an FP64 accumulator (exact
-//     integers below 2^53) and cleared.
-//   * C(m, n) = 2^(ea[m] + eb[n]) sum_d 2^-7d acc_d, smallest terms first.
+//     below 2^24, i.e. exact; it is then folded into one of two FP64
+//     accumulators and cleared: G1 += 2^14 acc_0 + 2^7 acc_1 + acc_2 and
+//     G2 += 2^14 acc_3 + 2^7 acc_4 + acc_5, integers below 2^39 per 128 k,
+//     so both sums stay exact for K below 2^21.
+//   * C(m, n) = 2^(ea[m] + eb[n]) (2^-49 G2 + 2^-28 G1): the sum of all six
+//     diagonals, sum_d 2^-7(d+2) acc_d, rounded once.
 //   * The slices are formed once per launch (exact_slice_kernel, 12 bytes
 //     per operand element in the plan's slice buffer) and the tiles copy
 //     them; slicing in every tile cost each A element one slicing per tile
@@ -701,28 +704,37 @@ __global__ void __launch_bounds__(256) gemm_exact_bf16(const __bf16* __restrict_
   const int wave = tid >> 6;
   const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
 
+  static_assert(S == 6, "the FP64 folding takes the diagonals three at a time");
   f4 acc[S][TM][TN];
-  double dacc[S][TM][TN][4];
+  double gacc[2][TM][TN][4];
 #pragma unroll
   for (int d = 0; d < S; ++d)
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
-      for (int j = 0; j < TN; ++j) {
-        acc[d][i][j] = f4{0, 0, 0, 0};
+      for (int j = 0; j < TN; ++j) acc[d][i][j] = f4{0, 0, 0, 0};
 #pragma unroll
-        for (int v = 0; v < 4; ++v) dacc[d][i][j][v] = 0.0;
-      }
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) gacc[h][i][j][v] = 0.0;
+  // every term is an integer below 2^53: the additions are exact
   auto flush = [&]() {
 #pragma unroll
-    for (int d = 0; d < S; ++d)
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
 #pragma unroll
-          for (int v = 0; v < 4; ++v) dacc[d][i][j][v] += (double)acc[d][i][j][v];
-          acc[d][i][j] = f4{0, 0, 0, 0};
+          for (int v = 0; v < 4; ++v)
+            gacc[h][i][j][v] += 16384.0 * (double)acc[3 * h][i][j][v] + 128.0 * (double)acc[3 * h + 1][i][j][v] +
+                                (double)acc[3 * h + 2][i][j][v];
+#pragma unroll
+          for (int d = 3 * h; d < 3 * h + 3; ++d) acc[d][i][j] = f4{0, 0, 0, 0};
         }
   };
 
@@ -822,9 +834,7 @@ __global__ void __launch_bounds__(256) gemm_exact_bf16(const __bf16* __restrict_
       for (int v = 0; v < 4; ++v) {
         const int64_t gm = m0 + wm0 + i * 16 + 4 * (lane >> 4) + v;
         if (gm >= M) continue;
-        double val = 0.0;
-#pragma unroll
-        for (int d = S - 1; d >= 0; --d) val += ldexp(dacc[d][i][j][v], -7 * (d + 2));
+        const double val = ldexp(gacc[1][i][j][v], -49) + ldexp(gacc[0][i][j][v], -28);
         put_c(C, g, map_off(g.c_row, gm) + coff, gm, ldexp(val, ex[g.ea_off + gm] + en));
       }
   }
@@ -847,8 +857,9 @@ constexpr TileCfg kCfgMfma[kMaxCfg] = {{32, 32}, {64, 64}, {128, 64}, {64, 128},
 constexpr TileCfg kCfgBf16[kMaxCfg] = {{32, 32},   {64, 64},   {128, 64},  {64, 128}, {128, 128},
                                        {128, 128}, {128, 128}, {128, 128}, {128, 128}};
 
-// exact split-BF16: 32x32 at 64 and 32 k per stage, 64x32, 32x64 (the
-// slices of six operands and the FP64 accumulators bound the tile size)
+// exact split-BF16: 32x32 at 64 and 32 k per stage, 64x32, 32x64 (a 64x64
+// tile, one wave per SIMD, was measured slower at 4096^3: the stage is
+// bound by the LDS reads of the slice fragments, not by reloading slices)
 constexpr TileCfg kCfgExact[kMaxCfg] = {{32, 32}, {32, 32}, {64, 32}, {32, 64}, {32, 32},
                                         {32, 32}, {32, 32}, {32, 32}, {32, 32}};
 bool mfma_mode(int mode) { return mode == TN_GEMM_F64 || mode == TN_GEMM_F32; }
