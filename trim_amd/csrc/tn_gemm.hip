@@ -1294,28 +1294,37 @@ int tn_gemm_plan_autotune(tn_gemm_plan P, const double* A, const double* B, doub
   TN_HIP_CHECK(hipEventCreate(&e0));
   TN_HIP_CHECK(hipEventCreate(&e1));
   // Each shape is timed through its own tile list; callers on other threads
-  // keep launching the current shape until the best one is selected.
-  int best = P->cfg.load();
-  float best_t = 1e30f;
+  // keep launching the current shape until the best one is selected.  The
+  // shapes are timed in interleaved rounds and each keeps its fastest round,
+  // so a clock ramp or a neighbour's burst during one shape's turn does not
+  // decide the choice.
+  constexpr int kRounds = 3;
+  const int ncfg = num_cfg(P->mode);
+  std::vector<float> t_cfg(ncfg, 1e30f);
   int rc = TN_OK;
-  for (int c = 0; c < num_cfg(P->mode) && rc == TN_OK; ++c) {
+  for (int c = 0; c < ncfg && rc == TN_OK; ++c) {
     rc = ensure_tiles(P, c);
-    if (rc != TN_OK) break;
-    rc = exec_cfg(P, c, A, B, C, stream);   // warm-up
-    if (rc != TN_OK) break;
-    (void)hipEventRecord(e0, s);
-    for (int r = 0; r < reps && rc == TN_OK; ++r) rc = exec_cfg(P, c, A, B, C, stream);
-    (void)hipEventRecord(e1, s);
-    if (rc != TN_OK) break;
-    (void)hipEventSynchronize(e1);
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    ms /= reps;
-    if (ms < best_t) {
-      best_t = ms;
-      best = c;
+    if (rc == TN_OK) rc = exec_cfg(P, c, A, B, C, stream);   // warm-up
+  }
+  for (int round = 0; round < kRounds && rc == TN_OK; ++round) {
+    for (int c = 0; c < ncfg && rc == TN_OK; ++c) {
+      (void)hipEventRecord(e0, s);
+      for (int r = 0; r < reps && rc == TN_OK; ++r) rc = exec_cfg(P, c, A, B, C, stream);
+      (void)hipEventRecord(e1, s);
+      if (rc != TN_OK) break;
+      (void)hipEventSynchronize(e1);
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      t_cfg[c] = std::min(t_cfg[c], ms / reps);
     }
   }
+  int best = P->cfg.load();
+  float best_t = 1e30f;
+  for (int c = 0; c < ncfg; ++c)
+    if (t_cfg[c] < best_t) {
+      best_t = t_cfg[c];
+      best = c;
+    }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc != TN_OK) return rc;

@@ -21,6 +21,7 @@
 #include <cfloat>
 #include <cmath>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "tn_internal.h"
@@ -513,8 +514,8 @@ constexpr int kSmallEN = 8;      // V chunk per lane (n <= 64)
 template <int CTRL>
 __device__ inline double dpp_mov(double x) {
   const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffff), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, true);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 __device__ inline double group8_sum(double x) {
@@ -559,6 +560,72 @@ __device__ inline void rr_pair_fast(int r, int k, int np, int& p, int& q) {
   }
 }
 
+// Lane moves of the register-resident sweeps, where lane 8 t + j of a wave
+// holds row chunk t of group j's columns.  group_xor<D>: the value held by
+// group j ^ D (same t), by DPP inside rows of 16 lanes: quad_perm for xor 1,
+// 2 and 3, row_half_mirror (lane i <-> 7 - i) for xor 7, and the two
+// composed for 4, 5 and 6.
+template <int D>
+__device__ inline double group_xor(double x) {
+  static_assert(D >= 1 && D <= 7, "group offset");
+  if constexpr (D == 1) return dpp_mov<0xB1>(x);
+  else if constexpr (D == 2) return dpp_mov<0x4E>(x);
+  else if constexpr (D == 3) return dpp_mov<0x1B>(x);
+  else if constexpr (D == 7) return dpp_mov<0x141>(x);
+  else return group_xor<D ^ 7>(dpp_mov<0x141>(x));
+}
+template <int D, int E>
+__device__ inline void group_xor(double (&x)[E]) {
+#pragma unroll
+  for (int e = 0; e < E; ++e) x[e] = group_xor<D>(x[e]);
+}
+
+// x + (x of the lane 16 (S = 16) or 32 (S = 32) away): the permlane swap of
+// a value with itself leaves, in every lane, the lower row's (half's) copy
+// in the first result and the upper one's in the second, so the sum has one
+// operand order everywhere.
+template <int S>
+__device__ inline double swap_add(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)b, hi = (unsigned)(b >> 32);
+  unsigned l0, l1, h0, h1;
+  if constexpr (S == 16) {
+    const auto l = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    l0 = l[0], l1 = l[1], h0 = h[0], h1 = h[1];
+  } else {
+    const auto l = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    l0 = l[0], l1 = l[1], h0 = h[0], h1 = h[1];
+  }
+  const double u = __longlong_as_double((long long)(((unsigned long long)h0 << 32) | l0));
+  const double v = __longlong_as_double((long long)(((unsigned long long)h1 << 32) | l1));
+  return u + v;
+}
+
+// Sum over the eight lanes j, j + 8, ..., j + 56 of a group; every lane of
+// the group ends with the same bits (each step adds a commuted pair or one
+// operand order everywhere).
+__device__ inline double group_sum_strided(double x) {
+  x += dpp_mov<0x128>(x);   // row_ror:8, lane ^ 8 inside a row of 16
+  x = swap_add<16>(x);
+  return swap_add<32>(x);
+}
+
+// The Jacobi rotation of a pair with squared norms np, nq and dot product
+// c: the tangent from the raw reciprocal and square-root estimates (its
+// error only moves the angle), cs = 1 / sqrt(1 + t^2) refined, sn = cs t.
+struct JacobiRot {
+  double cs, sn, tt;
+};
+__device__ inline JacobiRot jacobi_rot(double np, double nq, double c) {
+  const double zeta = (nq - np) * (0.5 * __builtin_amdgcn_rcp(c));
+  const double z2 = fma(zeta, zeta, 1.0);
+  const double tt = copysign(1.0, zeta) * __builtin_amdgcn_rcp(fabs(zeta) + z2 * __builtin_amdgcn_rsq(z2));
+  const double cs = fast_rsqrt(fma(tt, tt, 1.0));
+  return {cs, cs * tt, tt};
+}
+
 // When only S and VT are wanted (no U) and M >= N, the kernel first
 // factorises W = Q R by Householder reflections in LDS and runs the sweeps on
 // X = R^T, whose column Gram matrix is close to diagonal (Drmac and Veselic,
@@ -590,8 +657,8 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
   constexpr int kRows = kSmallTP * EMX;          // W rows held (m <= kRows)
   __shared__ __attribute__((aligned(16))) double W[kSmallN * CW];
   __shared__ __attribute__((aligned(16))) double V[kSmallN * CV];
-  __shared__ double nrm[kSmallN];
-  __shared__ int rank_of[kSmallN];
+  __shared__ __attribute__((aligned(16))) double nrm[kSmallN];
+  __shared__ int rank_of[kThreads];
   __shared__ int col_at[kSmallN];
   __shared__ int qr_col[kSmallN];
   __shared__ double hh_xx[2];
@@ -973,13 +1040,224 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
     return sweep;
   };
 
+  // Sweeps of the QR path (X = R^T: n columns of n rows) with the columns in
+  // registers.  After each sweep's sort the columns form blocks of 8 in
+  // decreasing-norm order; the blocks are paired round-robin, one pair
+  // (P, Q) per wave per stage, with a block barrier only between stages.
+  // In a stage lane 8 t + j of the wave holds row chunk t of A_j = column
+  // 8 P + j and of one column of block Q, B_{j ^ g}, and its squared norm.
+  // Round g pairs A_j with B_{j ^ g}; the rounds run g = 0, 1, 3, 2, 6, 7,
+  // 5, 4, so that each step moves the B chunks by an xor of 1, 2 or 4
+  // groups (DPP moves, no LDS).  In the first stage of a sweep every block
+  // also meets itself: for d = 1..7 groups j and j ^ d trade one column, the
+  // lower group rotates (A_j, A_{j ^ d}) and the higher (B_{j ^ d}, B_j),
+  // and the columns trade back.  Every pair of a sweep is rotated once, by
+  // one group; its dot product is a sum over the group's eight lanes that
+  // each of them holds bit for bit.
+  auto sweeps_blocked = [&]() {
+    const double tol = 10.0 * DBL_EPSILON * (double)n;
+    const int max_sweeps = (int)pr.max_sweeps;
+    const int wv = tid >> 6, j = tid & 7, tc = (tid >> 3) & 7;
+    const int nbk = (((n + 7) >> 3) + 1) & ~1;   // blocks of 8, an even count
+    const bool active = wv < nbk / 2;
+    int sweep = 0;
+    for (; sweep < max_sweeps; ++sweep) {
+      // position k of the sweep holds column col_at[k] (decreasing norm,
+      // ties by index): wave w counts, for column tid % 64, the larger
+      // norms among columns 16 w .. 16 w + 15, and the four counts add up
+      {
+        const int jc = tid & 63, i0 = 16 * wv;
+        const double sj = nrm[jc];
+        int rk = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i += 2) {
+          const double2 s2 = *reinterpret_cast<const double2*>(nrm + i0 + i);
+          rk += (i0 + i < n) && ((s2.x > sj) || (s2.x == sj && i0 + i < jc));
+          rk += (i0 + i + 1 < n) && ((s2.y > sj) || (s2.y == sj && i0 + i + 1 < jc));
+        }
+        rank_of[tid] = rk;
+      }
+      __syncthreads();
+      if (tid < n) col_at[rank_of[tid] + rank_of[tid + 64] + rank_of[tid + 128] + rank_of[tid + 192]] = tid;
+      __syncthreads();
+      int rotated = 0;
+      for (int stage = 0; stage < nbk - 1; ++stage) {
+        int P = 0, Q = 0;
+        double a[EMX], b[EMX], na = 0.0, nb = 0.0;
+        if (active) {
+          rr_pair_fast(stage, wv, nbk, P, Q);
+          // the first stage gathers every column from where the previous
+          // sweep left it into its sorted position
+          const int pa = 8 * P + j, pb = 8 * Q + j;
+          const int sa = (pa < n) ? (stage == 0 ? col_at[pa] : pa) : -1;
+          const int sb = (pb < n) ? (stage == 0 ? col_at[pb] : pb) : -1;
+          const double* wa = W + (sa < 0 ? 0 : sa) * CW + tc * CHW;
+          const double* wb = W + (sb < 0 ? 0 : sb) * CW + tc * CHW;
+#pragma unroll
+          for (int e = 0; e < EMX; e += 2) {
+            const double2 u = *reinterpret_cast<const double2*>(wa + e);
+            const double2 v = *reinterpret_cast<const double2*>(wb + e);
+            a[e] = (sa < 0) ? 0.0 : u.x;
+            a[e + 1] = (sa < 0) ? 0.0 : u.y;
+            b[e] = (sb < 0) ? 0.0 : v.x;
+            b[e + 1] = (sb < 0) ? 0.0 : v.y;
+          }
+          na = (sa < 0) ? 0.0 : nrm[sa];
+          nb = (sb < 0) ? 0.0 : nrm[sb];
+        }
+        // every column is read before any is stored in its new place
+        if (stage == 0) __syncthreads();
+        if (active) {
+          const int ca = 8 * P + j;
+
+          // one intra-block round: groups j and j ^ D trade a column
+          auto intra = [&](auto dc) {
+            constexpr int D = decltype(dc)::value;
+            constexpr int HB = (D >= 4) ? 4 : (D >= 2) ? 2 : 1;
+            const bool low = (j & HB) == 0;
+            const int cmax = (low ? 8 * P : 8 * Q) + (j > (j ^ D) ? j : (j ^ D));
+            double own[EMX], r[EMX];
+#pragma unroll
+            for (int e = 0; e < EMX; ++e) {
+              own[e] = low ? a[e] : b[e];
+              r[e] = low ? b[e] : a[e];
+            }
+            double n_own = low ? na : nb;
+            double nr = group_xor<D>(low ? nb : na);
+            group_xor<D>(r);
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (int e = 0; e < EMX; e += 2) {
+              s0 = fma(own[e], r[e], s0);
+              s1 = fma(own[e + 1], r[e + 1], s1);
+            }
+            const double c = group_sum_strided(s0 + s1);
+            // the lower group's pair is (own, r), the higher group's (r, own)
+            const double np_ = low ? n_own : nr, nq_ = low ? nr : n_own;
+            const bool rot = cmax < n && c * c > tol * tol * (np_ * nq_);
+            if (__builtin_amdgcn_ballot_w64(rot)) {
+              rotated |= rot;
+              const JacobiRot q = jacobi_rot(np_, nq_, rot ? c : 1.0);
+              const double cs = rot ? q.cs : 1.0, sn = rot ? q.sn : 0.0, tc_ = rot ? q.tt * c : 0.0;
+              const double s2 = low ? -sn : sn;
+#pragma unroll
+              for (int e = 0; e < EMX; ++e) {
+                const double u = own[e], v = r[e];
+                own[e] = cs * u + s2 * v;
+                r[e] = cs * v - s2 * u;
+              }
+              n_own = fmax(low ? n_own - tc_ : n_own + tc_, 0.0);
+              nr = fmax(low ? nr + tc_ : nr - tc_, 0.0);
+            }
+            group_xor<D>(r);
+            nr = group_xor<D>(nr);
+#pragma unroll
+            for (int e = 0; e < EMX; ++e) {
+              a[e] = low ? own[e] : r[e];
+              b[e] = low ? r[e] : own[e];
+            }
+            na = low ? n_own : nr;
+            nb = low ? nr : n_own;
+          };
+          // one cross round: (A_j, B_{j ^ g}), A the first of the pair
+          auto cross = [&](int g) {
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (int e = 0; e < EMX; e += 2) {
+              s0 = fma(a[e], b[e], s0);
+              s1 = fma(a[e + 1], b[e + 1], s1);
+            }
+            const double c = group_sum_strided(s0 + s1);
+            const bool rot = ca < n && 8 * Q + (j ^ g) < n && c * c > tol * tol * (na * nb);
+            // a wave whose groups all skip their pairs skips the update; in
+            // one that rotates, a skipping group applies (1, 0), which leaves
+            // its columns and norms unchanged bit for bit
+            if (__builtin_amdgcn_ballot_w64(rot)) {
+              rotated |= rot;
+              const JacobiRot q = jacobi_rot(na, nb, rot ? c : 1.0);
+              const double cs = rot ? q.cs : 1.0, sn = rot ? q.sn : 0.0, tc_ = rot ? q.tt * c : 0.0;
+#pragma unroll
+              for (int e = 0; e < EMX; ++e) {
+                const double u = a[e], v = b[e];
+                a[e] = cs * u - sn * v;
+                b[e] = sn * u + cs * v;
+              }
+              na = fmax(na - tc_, 0.0);
+              nb = fmax(nb + tc_, 0.0);
+            }
+          };
+          auto move = [&](auto dc) {
+            constexpr int D = decltype(dc)::value;
+            group_xor<D>(b);
+            nb = group_xor<D>(nb);
+          };
+          using I1 = std::integral_constant<int, 1>;
+          using I2 = std::integral_constant<int, 2>;
+          using I4 = std::integral_constant<int, 4>;
+          if (stage == 0) {
+            intra(I1{});
+            intra(I2{});
+            intra(std::integral_constant<int, 3>{});
+            intra(I4{});
+            intra(std::integral_constant<int, 5>{});
+            intra(std::integral_constant<int, 6>{});
+            intra(std::integral_constant<int, 7>{});
+          }
+          cross(0);
+          move(I1{});
+          cross(1);
+          move(I2{});
+          cross(3);
+          move(I1{});
+          cross(2);
+          move(I4{});
+          cross(6);
+          move(I1{});
+          cross(7);
+          move(I2{});
+          cross(5);
+          move(I1{});
+          cross(4);
+          // B_{j ^ 4} is now in group j
+          const int cb = 8 * Q + (j ^ 4);
+          double* pa = W + ca * CW + tc * CHW;
+          double* pb = W + cb * CW + tc * CHW;
+#pragma unroll
+          for (int e = 0; e < EMX; e += 2) {
+            *reinterpret_cast<double2*>(pa + e) = double2{a[e], a[e + 1]};
+            *reinterpret_cast<double2*>(pb + e) = double2{b[e], b[e + 1]};
+          }
+          // the last stage holds every block: it leaves the exact squared
+          // norms for the next sweep's ranking
+          if (stage == nbk - 2) {
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (int e = 0; e < EMX; ++e) {
+              s0 = fma(a[e], a[e], s0);
+              s1 = fma(b[e], b[e], s1);
+            }
+            na = group_sum_strided(s0);
+            nb = group_sum_strided(s1);
+          }
+          if (tc == 0) {
+            if (ca < n) nrm[ca] = na;
+            if (cb < n) nrm[cb] = nb;
+          }
+        }
+        __syncthreads();
+      }
+      if (!__syncthreads_or(rotated)) break;
+    }
+    return sweep;
+  };
+
   bool qr = !tr && !want_u;
   int total = 0;
   bool nonconv = false;
   if (qr) {
     householder_rt();
     exact_norms();
-    const int sw = sweeps(true);
+    const int sw = sweeps_blocked();
     total = sw + 1;
     nonconv = sw >= (int)pr.max_sweeps;
     exact_norms();
