@@ -662,6 +662,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
   __shared__ int col_at[kSmallN];
   __shared__ int qr_col[kSmallN];
   __shared__ double hh_xx[2];
+  __shared__ __attribute__((aligned(16))) double hh_v[2 * kRows];
   const SvdProb pr = probs[blockIdx.x];
   if (pr.mask != nullptr && pr.mask_idx >= 0 && pr.mask[pr.mask_idx] == 0) return;
   const int64_t M = pr.m, N = pr.n;
@@ -810,10 +811,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
   // of W.  Cold start: the columns are first moved into decreasing-norm
   // order (a pivot-free stand-in for column pivoting); warm start: A V0 is
   // already ordered by the previous singular values.  qr_col[j] is the
-  // column of A V0 at position j.  Step j: every group reads the pivot
-  // column (rows >= j) and reflects up to two later columns; the group that
-  // owns column j + 1 also leaves the squared norm of its rows > j for the
-  // next step.
+  // column of A V0 at position j.
   auto householder_rt = [&]() {
     if (warm) {
       for (int j = tid; j < n; j += kThreads) qr_col[j] = j;
@@ -823,89 +821,101 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
       permute_sorted();
     }
     __syncthreads();
-    if (tid == 0) hh_xx[0] = nrm[0];
-    __syncthreads();
-    constexpr int kGroups = kThreads / kSmallTP;
-    for (int j = 0; j < n; ++j) {
-      // every LDS read of the step first, then the two dot products, and the
-      // reflector's scalars (branch-free) while their reductions run
-      const double xx = hh_xx[j & 1];
-      const double xj = W[widx(j, j)];
-      const int k0 = j + 1 + g, k1 = k0 + kGroups;
-      const bool h0 = k0 < n, h1 = k1 < n;
-      const double* xc = W + j * CW + t * CHW;
-      double* wc0 = W + (h0 ? k0 : j) * CW + t * CHW;
-      double* wc1 = W + (h1 ? k1 : j) * CW + t * CHW;
-      double xr[EMX], w0[EMX], w1[EMX];
+    // The columns in registers: group G = 8 w + (lane & 7) of wave w holds
+    // columns G and G + 32, its lane 8 t + (lane & 7) row chunk t of each.
+    // Step j: the owner of column j has published its reflector v = x - alpha
+    // e_j (x = rows >= j of the column) and 1 / (v^T v / 2) in LDS; after the
+    // barrier every group reflects its columns (a finished column sees a
+    // zero coefficient), and the owner of column j + 1 publishes the next.
+    {
+      const int wv = tid >> 6, jl = tid & 7, tc = (tid >> 3) & 7;
+      const int G = 8 * wv + jl;
+      double col[2][EMX];
 #pragma unroll
-      for (int e = 0; e < EMX; e += 2) {
-        const double2 u = *reinterpret_cast<const double2*>(xc + e);
-        const double2 v0 = *reinterpret_cast<const double2*>(wc0 + e);
-        const double2 v1 = *reinterpret_cast<const double2*>(wc1 + e);
-        xr[e] = u.x;
-        xr[e + 1] = u.y;
-        w0[e] = v0.x;
-        w0[e + 1] = v0.y;
-        w1[e] = v1.x;
-        w1[e + 1] = v1.y;
+      for (int sl = 0; sl < 2; ++sl) {
+        const int k = G + 32 * sl;
+        const double* wk = W + (k < n ? k : 0) * CW + tc * CHW;
+#pragma unroll
+        for (int e = 0; e < EMX; e += 2) {
+          const double2 u = *reinterpret_cast<const double2*>(wk + e);
+          col[sl][e] = (k < n) ? u.x : 0.0;
+          col[sl][e + 1] = (k < n) ? u.y : 0.0;
+        }
       }
-      const double wj0 = W[widx(h0 ? k0 : j, j)];
-      const double wj1 = W[widx(h1 ? k1 : j, j)];
-#pragma unroll
-      for (int e = 0; e < EMX; ++e) xr[e] = (t * EMX + e >= j) ? xr[e] : 0.0;
-      double d0a = 0.0, d0b = 0.0, d1a = 0.0, d1b = 0.0;
-#pragma unroll
-      for (int e = 0; e < EMX; e += 2) {
-        d0a = fma(xr[e], w0[e], d0a);
-        d0b = fma(xr[e + 1], w0[e + 1], d0b);
-        d1a = fma(xr[e], w1[e], d1a);
-        d1b = fma(xr[e + 1], w1[e + 1], d1b);
-      }
-      double d0 = d0a + d0b, d1 = d1a + d1b;
-      d0 = group8_sum(d0);
-      d1 = group8_sum(d1);
-      const bool refl = xx > 0.0;
-      const double nrm_x = refl ? xx * fast_rsqrt(refl ? xx : 1.0) : 0.0;
-      const double alpha = -copysign(nrm_x, xj);
-      const double den = alpha * (alpha - xj);   // v^T v / 2 for v = x - alpha e_j
-      const double rden = refl ? fast_rcp(refl ? den : 1.0) : 0.0;
-      const double vj = xj - alpha;
-      if (h0) {
-        const double coef = (d0 - alpha * wj0) * rden;
-        double nx = 0.0;
+      // the owner of column jj (slot sl) publishes its reflector and turns
+      // the column into column jj of R
+      auto publish = [&](auto slc, int jj) {
+        constexpr int sl = decltype(slc)::value;
+        double x[EMX];
+        double q0 = 0.0, q1 = 0.0, xjl = 0.0;
 #pragma unroll
         for (int e = 0; e < EMX; ++e) {
-          const int r = t * EMX + e;
-          if (r >= j) w0[e] -= coef * ((r == j) ? vj : xr[e]);
-          if (r > j) nx += w0[e] * w0[e];
+          const int r = tc * EMX + e;
+          x[e] = (r >= jj) ? col[sl][e] : 0.0;
+          xjl = (r == jj) ? x[e] : xjl;
         }
 #pragma unroll
-        for (int e = 0; e < EMX; e += 2) *reinterpret_cast<double2*>(wc0 + e) = double2{w0[e], w0[e + 1]};
-        if (g == 0) {
-          nx = group8_sum(nx);
-          if (t == 0) hh_xx[(j + 1) & 1] = nx;
+        for (int e = 0; e < EMX; e += 2) {
+          q0 = fma(x[e], x[e], q0);
+          q1 = fma(x[e + 1], x[e + 1], q1);
         }
-      }
-      if (h1) {
-        const double coef = (d1 - alpha * wj1) * rden;
+        const double xx = group_sum_strided(q0 + q1);
+        const double xj = group_sum_strided(xjl);   // one nonzero term
+        const bool refl = xx > 0.0;
+        const double nrm_x = refl ? xx * fast_rsqrt(refl ? xx : 1.0) : 0.0;
+        const double alpha = -copysign(nrm_x, xj);
+        const double den = alpha * (alpha - xj);   // v^T v / 2
+        const double rden = refl ? fast_rcp(refl ? den : 1.0) : 0.0;
+        double* vb = hh_v + (jj & 1) * kRows + tc * EMX;
 #pragma unroll
         for (int e = 0; e < EMX; ++e) {
-          const int r = t * EMX + e;
-          if (r >= j) w1[e] -= coef * ((r == j) ? vj : xr[e]);
+          const int r = tc * EMX + e;
+          vb[e] = (r == jj) ? xj - alpha : x[e];
+          col[sl][e] = (r == jj) ? (refl ? alpha : xj) : (r > jj) ? 0.0 : col[sl][e];
         }
+        if (tc == 0) hh_xx[jj & 1] = rden;
+      };
+      if (G == 0) publish(std::integral_constant<int, 0>{}, 0);
+      for (int j = 0; j < n; ++j) {
+        __syncthreads();
+        double v[EMX];
+        const double* vb = hh_v + (j & 1) * kRows + tc * EMX;
 #pragma unroll
-        for (int e = 0; e < EMX; e += 2) *reinterpret_cast<double2*>(wc1 + e) = double2{w1[e], w1[e + 1]};
+        for (int e = 0; e < EMX; e += 2) {
+          const double2 u = *reinterpret_cast<const double2*>(vb + e);
+          v[e] = u.x;
+          v[e + 1] = u.y;
+        }
+        const double rden = hh_xx[j & 1];
+#pragma unroll
+        for (int sl = 0; sl < 2; ++sl) {
+          const int k = G + 32 * sl;
+          double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+          for (int e = 0; e < EMX; e += 2) {
+            d0 = fma(v[e], col[sl][e], d0);
+            d1 = fma(v[e + 1], col[sl][e + 1], d1);
+          }
+          const double coef = (k > j && k < n) ? group_sum_strided(d0 + d1) * rden : 0.0;
+#pragma unroll
+          for (int e = 0; e < EMX; ++e) col[sl][e] = fma(-coef, v[e], col[sl][e]);
+        }
+        const int jn = j + 1;
+        if (jn < n && (jn & 31) == G) {
+          if (jn < 32)
+            publish(std::integral_constant<int, 0>{}, jn);
+          else
+            publish(std::integral_constant<int, 1>{}, jn);
+        }
       }
       __syncthreads();
-      // the pivot column becomes column j of R (read again only by the
-      // transpose below)
-      if (g == 0) {
-        double* pc = W + j * CW + t * CHW;
 #pragma unroll
-        for (int e = 0; e < EMX; ++e) {
-          const int r = t * EMX + e;
-          if (r == j) pc[e] = (xx > 0.0) ? alpha : xj;
-          else if (r > j) pc[e] = 0.0;
+      for (int sl = 0; sl < 2; ++sl) {
+        const int k = G + 32 * sl;
+        if (k < n) {
+          double* wk = W + k * CW + tc * CHW;
+#pragma unroll
+          for (int e = 0; e < EMX; e += 2) *reinterpret_cast<double2*>(wk + e) = double2{col[sl][e], col[sl][e + 1]};
         }
       }
     }
@@ -1055,7 +1065,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
   // one group; its dot product is a sum over the group's eight lanes that
   // each of them holds bit for bit.
   auto sweeps_blocked = [&]() {
-    const double tol = 10.0 * DBL_EPSILON * (double)n;
+    const double tol = 10.0 * DBL_EPSILON * (double)n, tol2 = tol * tol;
     const int max_sweeps = (int)pr.max_sweeps;
     const int wv = tid >> 6, j = tid & 7, tc = (tid >> 3) & 7;
     const int nbk = (((n + 7) >> 3) + 1) & ~1;   // blocks of 8, an even count
@@ -1134,21 +1144,24 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
             const double c = group_sum_strided(s0 + s1);
             // the lower group's pair is (own, r), the higher group's (r, own)
             const double np_ = low ? n_own : nr, nq_ = low ? nr : n_own;
-            const bool rot = cmax < n && c * c > tol * tol * (np_ * nq_);
+            const bool rot = (cmax < n) & (c * c > tol2 * (np_ * nq_));
+            double cs = 1.0, sn = 0.0, tc_ = 0.0;
             if (__builtin_amdgcn_ballot_w64(rot)) {
               rotated |= rot;
               const JacobiRot q = jacobi_rot(np_, nq_, rot ? c : 1.0);
-              const double cs = rot ? q.cs : 1.0, sn = rot ? q.sn : 0.0, tc_ = rot ? q.tt * c : 0.0;
-              const double s2 = low ? -sn : sn;
-#pragma unroll
-              for (int e = 0; e < EMX; ++e) {
-                const double u = own[e], v = r[e];
-                own[e] = cs * u + s2 * v;
-                r[e] = cs * v - s2 * u;
-              }
-              n_own = fmax(low ? n_own - tc_ : n_own + tc_, 0.0);
-              nr = fmax(low ? nr + tc_ : nr - tc_, 0.0);
+              cs = rot ? q.cs : 1.0;
+              sn = rot ? q.sn : 0.0;
+              tc_ = rot ? q.tt * c : 0.0;
             }
+            const double s2 = low ? -sn : sn;
+#pragma unroll
+            for (int e = 0; e < EMX; ++e) {
+              const double u = own[e], v = r[e];
+              own[e] = cs * u + s2 * v;
+              r[e] = cs * v - s2 * u;
+            }
+            n_own = fmax(low ? n_own - tc_ : n_own + tc_, 0.0);
+            nr = fmax(low ? nr + tc_ : nr - tc_, 0.0);
             group_xor<D>(r);
             nr = group_xor<D>(nr);
 #pragma unroll
@@ -1168,23 +1181,26 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
               s1 = fma(a[e + 1], b[e + 1], s1);
             }
             const double c = group_sum_strided(s0 + s1);
-            const bool rot = ca < n && 8 * Q + (j ^ g) < n && c * c > tol * tol * (na * nb);
-            // a wave whose groups all skip their pairs skips the update; in
-            // one that rotates, a skipping group applies (1, 0), which leaves
-            // its columns and norms unchanged bit for bit
+            const bool rot = (ca < n) & (8 * Q + (j ^ g) < n) & (c * c > tol2 * (na * nb));
+            // a skipped pair applies (1, 0), which leaves its columns and
+            // norms unchanged bit for bit; a wave with no pair to rotate
+            // skips the rotation's square roots and reciprocals
+            double cs = 1.0, sn = 0.0, tc_ = 0.0;
             if (__builtin_amdgcn_ballot_w64(rot)) {
               rotated |= rot;
               const JacobiRot q = jacobi_rot(na, nb, rot ? c : 1.0);
-              const double cs = rot ? q.cs : 1.0, sn = rot ? q.sn : 0.0, tc_ = rot ? q.tt * c : 0.0;
-#pragma unroll
-              for (int e = 0; e < EMX; ++e) {
-                const double u = a[e], v = b[e];
-                a[e] = cs * u - sn * v;
-                b[e] = sn * u + cs * v;
-              }
-              na = fmax(na - tc_, 0.0);
-              nb = fmax(nb + tc_, 0.0);
+              cs = rot ? q.cs : 1.0;
+              sn = rot ? q.sn : 0.0;
+              tc_ = rot ? q.tt * c : 0.0;
             }
+#pragma unroll
+            for (int e = 0; e < EMX; ++e) {
+              const double u = a[e], v = b[e];
+              a[e] = cs * u - sn * v;
+              b[e] = sn * u + cs * v;
+            }
+            na = fmax(na - tc_, 0.0);
+            nb = fmax(nb + tc_, 0.0);
           };
           auto move = [&](auto dc) {
             constexpr int D = decltype(dc)::value;
