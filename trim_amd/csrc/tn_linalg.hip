@@ -681,34 +681,35 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
   // rows m..8*EMX zero
   // (all global loads of a thread are issued before any is used)
   const bool warm = pr.v0_off >= 0 && !tr;
-  {
-    constexpr int kPerV = kSmallN * kSmallTP * kSmallEN / kThreads;
-    double vv[kPerV];
-#pragma unroll
-    for (int u = 0; u < kPerV; ++u) {
-      const int e = tid + u * kThreads, c = e / (kSmallTP * kSmallEN), i = e % (kSmallTP * kSmallEN);
-      vv[u] = (c == i && c < n) ? 1.0 : 0.0;
-      if (warm) vv[u] = (c < n && i < n) ? V0T[pr.v0_off + (int64_t)c * n + i] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < kPerV; ++u) {
-      const int e = tid + u * kThreads;
-      V[vidx(e / (kSmallTP * kSmallEN), e % (kSmallTP * kSmallEN))] = vv[u];
-    }
-  }
-  if (warm) __syncthreads();
   // W = A (A^T for M < N), read in the source's row order so that
   // neighbouring lanes read neighbouring addresses; for a warm start W = A V0
-  // is then formed from LDS, each thread a block of EMX / 2 rows by 4 columns
-  auto load_w = [&]() {
+  // is then formed from LDS, each thread a block of EMX / 2 rows by 4 columns.
+  // The first call also fills V, its loads issued with those of A.
+  auto load_w = [&](bool with_v) {
+    constexpr int kPerV = kSmallN * kSmallTP * kSmallEN / kThreads;
     constexpr int kPerA = kSmallN * kRows / kThreads;
-    double xa[kPerA];
+    double vv[kPerV], xa[kPerA];
+    if (with_v) {
+#pragma unroll
+      for (int u = 0; u < kPerV; ++u) {
+        const int e = tid + u * kThreads, c = e / (kSmallTP * kSmallEN), i = e % (kSmallTP * kSmallEN);
+        vv[u] = (c == i && c < n) ? 1.0 : 0.0;
+        if (warm) vv[u] = (c < n && i < n) ? V0T[pr.v0_off + (int64_t)c * n + i] : 0.0;
+      }
+    }
 #pragma unroll
     for (int u = 0; u < kPerA; ++u) {
       const int e = tid + u * kThreads;
       const int i = tr ? e % kRows : e / kSmallN;
       const int c = tr ? e / kRows : e % kSmallN;
       xa[u] = (c < n && i < m) ? (tr ? Ab[(int64_t)c * N + i] : Ab[(int64_t)i * N + c]) : 0.0;
+    }
+    if (with_v) {
+#pragma unroll
+      for (int u = 0; u < kPerV; ++u) {
+        const int e = tid + u * kThreads;
+        V[vidx(e / (kSmallTP * kSmallEN), e % (kSmallTP * kSmallEN))] = vv[u];
+      }
     }
 #pragma unroll
     for (int u = 0; u < kPerA; ++u) {
@@ -724,6 +725,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
       for (int u = 0; u < RB; ++u)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[u][j] = 0.0;
+#pragma unroll 8
       for (int k = 0; k < n; ++k) {
         double a[RB], v[4];
 #pragma unroll
@@ -743,7 +745,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
       __syncthreads();
     }
   };
-  load_w();
+  load_w(true);
 
   // ||A||_F^2 from the exact column norms (group sums, then a fixed-order
   // sum over the columns by one thread)
@@ -1282,7 +1284,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
     for (int c = tid; c < n; c += kThreads) tiny |= !(nrm[c] >= 1e-270);
     if (__syncthreads_or(tiny)) {
       qr = false;
-      load_w();
+      load_w(false);
       exact_norms();
     }
   }
