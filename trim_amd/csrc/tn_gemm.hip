@@ -1297,7 +1297,8 @@ int tn_gemm_plan_autotune(tn_gemm_plan P, const double* A, const double* B, doub
   // keep launching the current shape until the best one is selected.  The
   // shapes are timed in interleaved rounds and each keeps its fastest round,
   // so a clock ramp or a neighbour's burst during one shape's turn does not
-  // decide the choice.
+  // decide the choice; after the first round only the shapes within 25% of
+  // the best are timed again.
   constexpr int kRounds = 3;
   const int ncfg = num_cfg(P->mode);
   std::vector<float> t_cfg(ncfg, 1e30f);
@@ -1307,7 +1308,9 @@ int tn_gemm_plan_autotune(tn_gemm_plan P, const double* A, const double* B, doub
     if (rc == TN_OK) rc = exec_cfg(P, c, A, B, C, stream);   // warm-up
   }
   for (int round = 0; round < kRounds && rc == TN_OK; ++round) {
+    const float lead = *std::min_element(t_cfg.begin(), t_cfg.end());
     for (int c = 0; c < ncfg && rc == TN_OK; ++c) {
+      if (round > 0 && t_cfg[c] > 1.25f * lead) continue;
       (void)hipEventRecord(e0, s);
       for (int r = 0; r < reps && rc == TN_OK; ++r) rc = exec_cfg(P, c, A, B, C, stream);
       (void)hipEventRecord(e1, s);
