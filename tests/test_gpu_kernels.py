@@ -558,15 +558,20 @@ def _check_s_vt(ora, a, s, vt, rank=None):
 
 
 @pytest.mark.parametrize("M,N", [(64, 64), (5, 4), (100, 64), (128, 33), (7, 1), (1, 1), (64, 63),
-                                 (33, 5), (20, 50), (96, 96), (200, 150), (300, 300), (700, 513),
+                                 (33, 5), (64, 8), (64, 9), (40, 16), (64, 17), (50, 24), (64, 40),
+                                 (64, 48), (64, 56), (120, 57), (32, 32), (32, 31), (16, 13),
+                                 (20, 50), (96, 96), (200, 150), (300, 300), (700, 513),
                                  (1030, 1030), (1500, 1100), (2100, 300), (3072, 1024), (4096, 520),
                                  (4100, 70)])
 def test_svd_without_u(tn, ora, M, N):
     """U not wanted: for M >= N the solver factorises A = Q R by Householder
-    reflections (in LDS for n <= 64; blocked for the block path: 32-column
-    panels up to M = 1024, 16-column ones up to 2048, 8-column ones up to
-    4096) and sweeps R^T, reading VT off its normalised columns; M < N, and
-    M > 4096 on the block path, take the plain sweeps without writing U."""
+    reflections (with the columns in registers for n <= 64; blocked for the
+    block path: 32-column panels up to M = 1024, 16-column ones up to 2048,
+    8-column ones up to 4096) and sweeps R^T, reading VT off its normalised
+    columns; M < N, and M > 4096 on the block path, take the plain sweeps
+    without writing U.  The n <= 64 shapes cover every count of 8-column
+    blocks, partly and wholly empty blocks, and the three row chunk sizes
+    (M <= 32, 64, 128)."""
     _say("svd no U %dx%d" % (M, N))
     a = _rand(M * 1000 + N + 3, (M, N))
     s, vt = _svd_no_u(tn, a)
