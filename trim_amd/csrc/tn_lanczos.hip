@@ -31,7 +31,11 @@
 
 namespace {
 
-constexpr int kMvThreads = 256;            // 4 waves = 4 rows per block
+constexpr int kMvThreads = 128;            // 2 waves = 2 rows per block
+constexpr int kMvRows = kMvThreads / 64;
+// a matrix at least this large does not stay in the 256 MiB last-level
+// cache between steps: its rows are read with streaming (non-temporal) loads
+constexpr size_t kStreamBytes = size_t(256) << 20;
 constexpr int kStepThreads = 1024;
 constexpr double kBetaErr = 1e-15;         // uni10_lapack_cpu.cpp:419
 constexpr int kMinIter = 2;                // uni10_lapack_cpu.cpp:418
@@ -40,16 +44,18 @@ constexpr int kMinIter = 2;                // uni10_lapack_cpu.cpp:418
 // step's, 0 at the first step); one wave per row, eight partial
 // sums per lane (a fixed order), then a butterfly over the wave.  VEC: the
 // rows are 16-byte aligned (n even, A aligned), so every lane loads pairs,
-// 4 KiB per wave in flight per iteration.
-template <bool VEC>
+// 4 KiB per wave in flight per iteration; NT: the row loads are streaming
+// (8192 x 8192: 94 -> 82 us, 5.7 -> 6.6 TB/s, with two rows per block).
+template <bool VEC, bool NT>
 __global__ void __launch_bounds__(kMvThreads) lz_matvec(const double* __restrict__ A, int64_t n,
                                                         const double* __restrict__ v,
                                                         const double* __restrict__ vprev,
                                                         const double* __restrict__ beta_prev,
                                                         double* __restrict__ w) {
+  typedef double d2v __attribute__((ext_vector_type(2)));
   const int lane = threadIdx.x & 63;
   const double beta = beta_prev ? *beta_prev : 0.0;
-  const int64_t row = (int64_t)blockIdx.x * (kMvThreads / 64) + (threadIdx.x >> 6);
+  const int64_t row = (int64_t)blockIdx.x * kMvRows + (threadIdx.x >> 6);
   if (row >= n) return;
   const double* __restrict__ a = A + row * n;
   double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -57,12 +63,17 @@ __global__ void __launch_bounds__(kMvThreads) lz_matvec(const double* __restrict
   if constexpr (VEC) {
     // pairs j, j+1 with j = 2 lane + 128 u, four pairs per lane per step
     for (j = 2 * lane; j + 384 + 1 < n; j += 512) {
+      d2v x[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const double2 x = *reinterpret_cast<const double2*>(a + j + 128 * u);
+        const d2v* p = reinterpret_cast<const d2v*>(a + j + 128 * u);
+        x[u] = NT ? __builtin_nontemporal_load(p) : *p;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
         const double2 y = *reinterpret_cast<const double2*>(v + j + 128 * u);
-        s[2 * u] = fma(x.x, y.x, s[2 * u]);
-        s[2 * u + 1] = fma(x.y, y.y, s[2 * u + 1]);
+        s[2 * u] = fma(x[u].x, y.x, s[2 * u]);
+        s[2 * u + 1] = fma(x[u].y, y.y, s[2 * u + 1]);
       }
     }
     for (; j + 1 < n; j += 128) {
@@ -93,7 +104,7 @@ __global__ void __launch_bounds__(kMvThreads) lzc_matvec(const double* __restric
                                                          const double* __restrict__ beta_prev,
                                                          double* __restrict__ w) {
   const int lane = threadIdx.x & 63;
-  const int64_t row = (int64_t)blockIdx.x * (kMvThreads / 64) + (threadIdx.x >> 6);
+  const int64_t row = (int64_t)blockIdx.x * kMvRows + (threadIdx.x >> 6);
   if (row >= n) return;
   const double beta = beta_prev ? *beta_prev : 0.0;
   const double2* __restrict__ a = reinterpret_cast<const double2*>(A + row * 2 * n);
@@ -165,6 +176,43 @@ __global__ void __launch_bounds__(kStepThreads) lz_step(const double* __restrict
   if (beta > kBetaErr) {
     const double r = 1.0 / beta;
     for (int64_t i = threadIdx.x; i < n; i += kStepThreads) w[i] *= r;
+  }
+  if (threadIdx.x == 0) {
+    ab[0] = alpha;
+    ab[1] = beta;
+  }
+}
+
+// lz_step with the thread's R elements of v and w held in registers (n <=
+// R * kStepThreads): one load and one store pass instead of three of each,
+// the same operations in the same order, so the same bits.
+template <int R>
+__global__ void __launch_bounds__(kStepThreads) lz_step_reg(const double* __restrict__ v, double* __restrict__ w,
+                                                            int64_t n, double* __restrict__ ab) {
+  __shared__ double red[kStepThreads];
+  double vr[R], wr[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t i = threadIdx.x + (int64_t)r * kStepThreads;
+    vr[r] = (i < n) ? v[i] : 0.0;
+    wr[r] = (i < n) ? w[i] : 0.0;
+  }
+  double p = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) p = fma(vr[r], wr[r], p);
+  const double alpha = block_sum(p, red);
+  double q = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    wr[r] = fma(-alpha, vr[r], wr[r]);
+    q = fma(wr[r], wr[r], q);
+  }
+  const double beta = sqrt(block_sum(q, red));
+  const double rb = (beta > kBetaErr) ? 1.0 / beta : 1.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t i = threadIdx.x + (int64_t)r * kStepThreads;
+    if (i < n) w[i] = (beta > kBetaErr) ? wr[r] * rb : wr[r];
   }
   if (threadIdx.x == 0) {
     ab[0] = alpha;
@@ -333,6 +381,7 @@ static int lanczos_impl(const double* A, int64_t n, bool cplx, double* psi, int6
 
   // rows of A and the Lanczos vectors (row stride n) are 16-byte aligned
   const bool vec = (n % 2 == 0) && ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(Vm)) & 15) == 0;
+  const bool stream_a = (size_t)n * (size_t)n * sizeof(double) >= kStreamBytes;
   std::vector<double> As, Bs;
   As.reserve(M);
   Bs.reserve(M);
@@ -355,16 +404,23 @@ static int lanczos_impl(const double* A, int64_t n, bool cplx, double* psi, int6
         const double* vprev = (i == 0) ? v : Vm + (i - 1) * L;
         const double* bprev = (i == 0) ? nullptr : d_ab + 2 * (i - 1) + 1;
         double* w = Vm + (i + 1) * L;
+        const dim3 grid((unsigned)((n + kMvRows - 1) / kMvRows));
         if (cplx)
-          hipLaunchKernelGGL(lzc_matvec, dim3((unsigned)((n + 3) / 4)), dim3(kMvThreads), 0, st, A, n, v, vprev,
-                             bprev, w);
+          hipLaunchKernelGGL(lzc_matvec, grid, dim3(kMvThreads), 0, st, A, n, v, vprev, bprev, w);
+        else if (vec && stream_a)
+          hipLaunchKernelGGL((lz_matvec<true, true>), grid, dim3(kMvThreads), 0, st, A, n, v, vprev, bprev, w);
         else if (vec)
-          hipLaunchKernelGGL(lz_matvec<true>, dim3((unsigned)((n + 3) / 4)), dim3(kMvThreads), 0, st, A, n, v,
-                             vprev, bprev, w);
+          hipLaunchKernelGGL((lz_matvec<true, false>), grid, dim3(kMvThreads), 0, st, A, n, v, vprev, bprev, w);
         else
-          hipLaunchKernelGGL(lz_matvec<false>, dim3((unsigned)((n + 3) / 4)), dim3(kMvThreads), 0, st, A, n, v,
-                             vprev, bprev, w);
-        hipLaunchKernelGGL(lz_step, dim3(1), dim3(kStepThreads), 0, st, v, w, L, d_ab + 2 * i);
+          hipLaunchKernelGGL((lz_matvec<false, false>), grid, dim3(kMvThreads), 0, st, A, n, v, vprev, bprev, w);
+        if (L <= 4 * kStepThreads)
+          hipLaunchKernelGGL(lz_step_reg<4>, dim3(1), dim3(kStepThreads), 0, st, v, w, L, d_ab + 2 * i);
+        else if (L <= 8 * kStepThreads)
+          hipLaunchKernelGGL(lz_step_reg<8>, dim3(1), dim3(kStepThreads), 0, st, v, w, L, d_ab + 2 * i);
+        else if (L <= 16 * kStepThreads)
+          hipLaunchKernelGGL(lz_step_reg<16>, dim3(1), dim3(kStepThreads), 0, st, v, w, L, d_ab + 2 * i);
+        else
+          hipLaunchKernelGGL(lz_step, dim3(1), dim3(kStepThreads), 0, st, v, w, L, d_ab + 2 * i);
       }
       if (hipGetLastError() != hipSuccess) return fail((tn::set_error("lanczos: launch failed"), TN_EHIP));
       rc = tn_memcpy_d2h(hab.data() + 2 * done, d_ab + 2 * done, sizeof(double) * 2 * k, st);
