@@ -569,7 +569,7 @@ __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict_
 //     so both sums stay exact for K below 2^21.
 //   * C(m, n) = 2^(ea[m] + eb[n]) (2^-49 G2 + 2^-28 G1): the sum of all six
 //     diagonals, sum_d 2^-7(d+2) acc_d, rounded once.
-//   * The slices are formed once per launch (exact_slice_kernel, 12 bytes
+//   * The slices are formed once per launch (exact_prep_kernel, 12 bytes
 //     per operand element in the plan's slice buffer) and the tiles copy
 //     them; slicing in every tile cost each A element one slicing per tile
 //     column (the kernel ran at 9 TFLOP/s FP64-equivalent at 4096^3).
@@ -577,103 +577,174 @@ __global__ void __launch_bounds__(256) gemm_split_bf16(const double* __restrict_
 // relative to max_k |A(m, k)| max_k |B(k, n)|, at most about K 2^-7(S+1).
 constexpr int kExactSlices = 6;
 
-// Scale exponents (blockIdx.y = group): the first ceil(M/4) blocks take one
-// wave per A row (lanes stride over k, then a butterfly max), the rest 64 B
-// columns each (lanes over adjacent columns, coalesced; the four waves split
-// k).  A row m at ea_off + m, B column n at eb_off + n.
-__global__ void __launch_bounds__(256) exact_scales_kernel(const double* __restrict__ A, const double* __restrict__ B,
-                                                          const GroupDev* __restrict__ groups,
-                                                          int32_t* __restrict__ ex) {
-  const GroupDev& g = groups[blockIdx.y];
-  if (g.K == 0 || group_off(g)) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t rblocks = (g.M + 3) / 4;
-  if ((int64_t)blockIdx.x < rblocks) {
-    const int64_t m = (int64_t)blockIdx.x * 4 + wave;
-    if (m >= g.M) return;                       // the whole wave
-    const int64_t ro = map_off(g.a_row, m);
-    double mx = 0.0;
-    for (int64_t k = lane; k < g.K; k += 64) mx = fmax(mx, fabs(A[g.a_off + ro + map_off(g.a_col, k)]));
+constexpr int kSliceK = 64;   // K padding of the slices: a multiple of every exact tile's k per stage
+
+// The slices of eight consecutive k of one row (A) or column (B), already
+// scaled by its power of two: six 7-bit integers per element, x = sum_s q_s
+// 2^-7(s+1) + r, each exact in BF16; slice q goes to dst + q * stride.
+__device__ __forceinline__ void store_slices(double (&x)[8], __bf16* __restrict__ dst, int64_t stride) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
-    if (lane == 0) ex[g.ea_off + m] = (mx > 0.0) ? ilogb(mx) + 1 : 0;
-  } else {
-    // 64 columns per block; wave w takes k = w, w + 4, ... four loads at a time
-    __shared__ double part[4][64];
-    const int64_t n = ((int64_t)blockIdx.x - rblocks) * 64 + lane;
-    const bool live = n < g.N;
-    const int64_t co = live ? map_off(g.b_col, n) : 0;
-    double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
-    int64_t k = wave;
-    for (; k + 12 < g.K; k += 16) {
-      if (live) {
-        m0 = fmax(m0, fabs(B[g.b_off + map_off(g.b_row, k) + co]));
-        m1 = fmax(m1, fabs(B[g.b_off + map_off(g.b_row, k + 4) + co]));
-        m2 = fmax(m2, fabs(B[g.b_off + map_off(g.b_row, k + 8) + co]));
-        m3 = fmax(m3, fabs(B[g.b_off + map_off(g.b_row, k + 12) + co]));
-      }
+  for (int q = 0; q < kExactSlices; ++q) {
+    bf16x8 v;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const double y = 128.0 * x[u], t = rint(y);
+      x[u] = y - t;
+      // t is an integer of magnitude at most 128: through int it converts
+      // exactly in two instructions (a direct double -> BF16 conversion
+      // brings a round-to-odd sequence of ten)
+      v[u] = (__bf16)(float)(int)t;
     }
-    for (; k < g.K; k += 4)
-      if (live) m0 = fmax(m0, fabs(B[g.b_off + map_off(g.b_row, k) + co]));
-    part[wave][lane] = fmax(fmax(m0, m1), fmax(m2, m3));
-    __syncthreads();
-    if (wave == 0 && live) {
-      const double mx = fmax(fmax(part[0][lane], part[1][lane]), fmax(part[2][lane], part[3][lane]));
-      ex[g.eb_off + n] = (mx > 0.0) ? ilogb(mx) + 1 : 0;
-    }
+    *reinterpret_cast<bf16x8*>(dst + (int64_t)q * stride) = v;
   }
 }
 
-// The slices of every operand element, once per launch (blockIdx.y =
-// group): A row m as [slice][m][kp] and B column n as [slice][n][kp] (k
-// fastest, zero past K), each element scaled by the power of two of its row /
-// column and cut into six 7-bit integers x = sum_s q_s 2^-7(s+1) + r, each
-// exact in BF16.  A thread converts eight consecutive k of one row (A) or
-// column (B): one 16-byte store per slice.  B's lanes run along n, so its
-// reads are coalesced rows of B.
-constexpr int kSliceK = 64;   // K padding of the slices: a multiple of every exact tile's k per stage
-__global__ void __launch_bounds__(256) exact_slice_kernel(const double* __restrict__ A, const double* __restrict__ B,
-                                                         const GroupDev* __restrict__ groups,
-                                                         const int32_t* __restrict__ ex, __bf16* __restrict__ sl) {
-  const GroupDev& g = groups[blockIdx.y];
-  if (g.K == 0 || group_off(g)) return;
-  const int64_t k8n = g.kp / 8;
-  const int64_t na = g.M * k8n, nb = g.N * k8n;
-  const int64_t stride = (int64_t)gridDim.x * 256;
-  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < na + nb; e += stride) {
-    const bool isa = e < na;
-    int64_t r, k0;
-    double x[8];
-    if (isa) {
-      r = e / k8n;
-      k0 = 8 * (e % k8n);
-      const int64_t ro = map_off(g.a_row, r);
-      const int sc = ex[g.ea_off + r];
+// The operands' scale exponents and slices, once per launch, one unit per
+// block of 16 waves from a list built with the plan: 64 rows of A or 64
+// columns of B.
+// Every wave issues all its loads at once (one memory round trip) when the
+// padded K is at most kPrepKp, keeping the values in registers from the
+// largest magnitude to the slicing; longer rows and columns loop.
+//   A: 16 lanes per row, four rows per wave; a lane takes 8-k chunks
+//      c = j, j + 16, ... (j = lane % 16), a 16-lane max gives the row's
+//      exponent, the lane slices its chunks.
+//   B: one column per lane; wave w takes chunks w, w + 16, ...; the waves'
+//      maxima meet in LDS, then each thread slices its chunks (with four
+//      waves per 64 columns their 64 elements per thread made B the long
+//      pole).
+// Exponents at ea_off + m / eb_off + n, slices as [slice][row][kp] (A) and
+// [slice][column][kp] (B), k fastest, zero past K.  A grid of (largest
+// group) x (groups) blocks, most of them empty, and one chunk per thread
+// (two kernels, scales then slices) took twice as long: most of it went to
+// dispatching workgroups.
+// PK: every group is plain on the K side (offset k * stride, no index map).
+struct PrepUnit {
+  int32_t group, kind, first, count;   // kind 0: A rows [first, first + count); 1: B columns
+};
+constexpr int kPrepThreads = 512;
+constexpr int kPrepWaves = kPrepThreads / 64;
+constexpr int kPrepRows = 4 * kPrepWaves;
+constexpr int kPrepKp = 256;           // K padding held in registers (2 chunks of 8 per thread)
+template <bool PK>
+__global__ void __launch_bounds__(kPrepThreads) exact_prep_kernel(const double* __restrict__ A, const double* __restrict__ B,
+                                                        const GroupDev* __restrict__ groups,
+                                                        const PrepUnit* __restrict__ units, int32_t* __restrict__ ex,
+                                                        __bf16* __restrict__ sl) {
+  __shared__ double part[kPrepWaves][64];
+  __shared__ int scl[64];
+  const PrepUnit un = units[blockIdx.x];
+  const GroupDev& g = groups[un.group];
+  if (group_off(g)) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t K = g.K, kp = g.kp;
+  const bool fits = kp <= kPrepKp;
+  if (un.kind == 0) {
+    constexpr int CH = kPrepKp / (8 * 16);
+    const int64_t a_ks = g.a_col.stride[TN_MAX_MAP_LEVELS - 1];
+    const int j = lane & 15, rw = 4 * wave + (lane >> 4);
+    const bool live = rw < un.count;
+    const int64_t m = un.first + (live ? rw : 0);
+    const int64_t ro = g.a_off + map_off(g.a_row, m);
+    auto at = [&](int64_t k) { return A[ro + (PK ? k * a_ks : map_off(g.a_col, k))]; };
+    // the 16 lanes of a row: max over lanes j ^ 1 .. j ^ 8 (every lane ends
+    // with the row's maximum)
+    auto row_max = [&](double v) {
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
-        x[u] = (k0 + u < g.K) ? ldexp(A[g.a_off + ro + map_off(g.a_col, k0 + u)], -sc) : 0.0;
-    } else {
-      const int64_t eb = e - na;
-      r = eb % g.N;
-      k0 = 8 * (eb / g.N);
-      const int64_t co = map_off(g.b_col, r);
-      const int sc = ex[g.eb_off + r];
+      for (int o = 8; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+      return v;
+    };
+    if (fits) {
+      double x[CH][8];
+      double mx = 0.0;
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
-        x[u] = (k0 + u < g.K) ? ldexp(B[g.b_off + map_off(g.b_row, k0 + u) + co], -sc) : 0.0;
-    }
-    const int64_t rows = isa ? g.M : g.N;
-    __bf16* dst = sl + (isa ? g.sa_off : g.sb_off) + r * g.kp + k0;
+      for (int c = 0; c < CH; ++c)
 #pragma unroll
-    for (int q = 0; q < kExactSlices; ++q) {
-      bf16x8 v;
+        for (int u = 0; u < 8; ++u) {
+          const int64_t k = 8 * (j + 16 * c) + u;
+          x[c][u] = (live && k < K) ? at(k) : 0.0;
+        }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const double y = 128.0 * x[u], t = rint(y);
-        x[u] = y - t;
-        v[u] = (__bf16)(float)t;
+      for (int c = 0; c < CH; ++c)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) mx = fmax(mx, fabs(x[c][u]));
+      mx = row_max(mx);
+      const int sc = (mx > 0.0) ? ilogb(mx) + 1 : 0;
+      if (live && j == 0) ex[g.ea_off + m] = sc;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int64_t k0 = 8 * (j + 16 * c);
+        if (live && k0 < kp) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) x[c][u] = ldexp(x[c][u], -sc);
+          store_slices(x[c], sl + g.sa_off + m * kp + k0, g.M * kp);
+        }
       }
-      *reinterpret_cast<bf16x8*>(dst + (int64_t)q * rows * g.kp) = v;
+    } else {
+      double mx = 0.0;
+      for (int64_t k = j; live && k < K; k += 16) mx = fmax(mx, fabs(at(k)));
+      mx = row_max(mx);
+      const int sc = (mx > 0.0) ? ilogb(mx) + 1 : 0;
+      if (live && j == 0) ex[g.ea_off + m] = sc;
+      for (int64_t k0 = 8 * j; live && k0 < kp; k0 += 128) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = (k0 + u < K) ? ldexp(at(k0 + u), -sc) : 0.0;
+        store_slices(x, sl + g.sa_off + m * kp + k0, g.M * kp);
+      }
+    }
+  } else {
+    constexpr int CH = kPrepKp / (8 * kPrepWaves);
+    const int64_t b_ks = g.b_row.stride[TN_MAX_MAP_LEVELS - 1];
+    const int64_t n = un.first + lane;
+    const bool live = lane < un.count;
+    const int64_t co = g.b_off + (live ? map_off(g.b_col, n) : 0);
+    auto at = [&](int64_t k) { return B[co + (PK ? k * b_ks : map_off(g.b_row, k))]; };
+    double x[CH][8];
+    double mx = 0.0;
+    if (fits) {
+#pragma unroll
+      for (int c = 0; c < CH; ++c)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int64_t k = 8 * (wave + kPrepWaves * c) + u;
+          x[c][u] = (live && k < K) ? at(k) : 0.0;
+        }
+#pragma unroll
+      for (int c = 0; c < CH; ++c)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) mx = fmax(mx, fabs(x[c][u]));
+    } else {
+      for (int64_t k = wave; live && k < K; k += kPrepWaves) mx = fmax(mx, fabs(at(k)));
+    }
+    part[wave][lane] = mx;
+    __syncthreads();
+    if (wave == 0) {
+      double m4 = part[0][lane];
+#pragma unroll
+      for (int w = 1; w < kPrepWaves; ++w) m4 = fmax(m4, part[w][lane]);
+      const int sc = (m4 > 0.0) ? ilogb(m4) + 1 : 0;
+      scl[lane] = sc;
+      if (live) ex[g.eb_off + n] = sc;
+    }
+    __syncthreads();
+    const int sc = scl[lane];
+    if (fits) {
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int64_t k0 = 8 * (wave + kPrepWaves * c);
+        if (live && k0 < kp) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) x[c][u] = ldexp(x[c][u], -sc);
+          store_slices(x[c], sl + g.sb_off + n * kp + k0, g.N * kp);
+        }
+      }
+    } else {
+      for (int64_t k0 = 8 * wave; live && k0 < kp; k0 += 8 * kPrepWaves) {
+        double y[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) y[u] = (k0 + u < K) ? ldexp(at(k0 + u), -sc) : 0.0;
+        store_slices(y, sl + g.sb_off + n * kp + k0, g.N * kp);
+      }
     }
   }
 }
@@ -1028,10 +1099,10 @@ struct tn_gemm_plan_s {
   std::vector<GroupDev> groups;
   GroupDev* d_groups = nullptr;
   int32_t* d_exp = nullptr;   // exact mode: scale exponents (A rows, then B columns, per group)
-  int64_t max_rows = 0;       // exact mode: blocks of the scale kernel for the largest group
   __bf16* d_sl = nullptr;     // exact mode: the operands' BF16 slices (per group: A, then B)
   int64_t sl_elems = 0;       // their number
-  int64_t max_slice_v = 0;    // exact mode: 8-k vectors of the largest group (slice kernel grid)
+  PrepUnit* d_units = nullptr;   // exact mode: the preparation kernel's work list
+  int64_t nunits = 0;
   double flops = 0;
   bool vec_ok = true;     // every group admits 16-byte pair loads
   int64_t max_k = 0;      // longest K over the groups
@@ -1170,6 +1241,7 @@ static void destroy_plan(tn_gemm_plan_s* P) {
   tn::pool_free(P->d_groups);
   tn::pool_free(P->d_exp);
   tn::pool_free(P->d_sl);
+  tn::pool_free(P->d_units);
   for (TileSet& t : P->tiles) tn::pool_free(t.d_tiles);
   delete P;
 }
@@ -1185,6 +1257,7 @@ int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups, const 
   TN_REQUIRE(ngroups >= 0 && (ngroups == 0 || d), TN_EINVAL, "tn_gemm_plan_create: NULL descs");
   auto* P = new tn_gemm_plan_s();
   P->mode = mode;
+  std::vector<PrepUnit> units;
   for (int64_t i = 0; i < ngroups; ++i) {
     const tn_gemm_desc_ex& x = d[i];
     GroupDev G;
@@ -1219,13 +1292,15 @@ int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups, const 
     for (const GroupDev& h : P->groups) nexp += h.M + h.N;
     G.ea_off = nexp;
     G.eb_off = nexp + x.M;
-    P->max_rows = std::max(P->max_rows, (x.M + 3) / 4 + (x.N + 63) / 64);   // exact mode: scale blocks
     if (mode == TN_GEMM_BF16_EXACT && x.M > 0 && x.N > 0 && x.K > 0) {
       G.kp = (x.K + kSliceK - 1) / kSliceK * kSliceK;
       G.sa_off = P->sl_elems;
       G.sb_off = G.sa_off + kExactSlices * x.M * G.kp;
       P->sl_elems = G.sb_off + kExactSlices * x.N * G.kp;
-      P->max_slice_v = std::max(P->max_slice_v, (x.M + x.N) * (G.kp / 8));
+      const int32_t gi = (int32_t)P->groups.size();
+      for (int64_t f = 0; f < x.M; f += kPrepRows)
+        units.push_back({gi, 0, (int32_t)f, (int32_t)std::min<int64_t>(kPrepRows, x.M - f)});
+      for (int64_t f = 0; f < x.N; f += 64) units.push_back({gi, 1, (int32_t)f, (int32_t)std::min<int64_t>(64, x.N - f)});
     }
     P->groups.push_back(G);
     P->flops += 2.0 * (double)x.M * (double)x.N * (double)x.K;
@@ -1246,6 +1321,10 @@ int tn_gemm_plan_create_ex(tn_gemm_plan* plan, int mode, int64_t ngroups, const 
       const GroupDev& last = P->groups.back();
       rc = tn::pool_alloc((void**)&P->d_exp, sizeof(int32_t) * std::max<int64_t>(1, last.eb_off + last.N));
       if (rc == TN_OK) rc = tn::pool_alloc((void**)&P->d_sl, sizeof(__bf16) * std::max<int64_t>(8, P->sl_elems));
+      P->nunits = (int64_t)units.size();
+      if (rc == TN_OK && !units.empty()) rc = tn::pool_alloc((void**)&P->d_units, sizeof(PrepUnit) * units.size());
+      if (rc == TN_OK && !units.empty())
+        rc = tn_memcpy_h2d(P->d_units, units.data(), sizeof(PrepUnit) * units.size(), nullptr);
     }
   }
   if (rc == TN_OK) rc = select_cfg(P, default_cfg(P));
@@ -1396,16 +1475,13 @@ static int exec_cfg(tn_gemm_plan_s* P, int cfg, const double* A, const double* B
   if (P->mode == TN_GEMM_BF16_EXACT) {
     ExactFn k = pick_exact(cfg);
     TN_REQUIRE(k != nullptr, TN_EINVAL, "no exact kernel for tile %d", cfg);
-    const unsigned gx = (unsigned)P->max_rows;
-    hipLaunchKernelGGL(exact_scales_kernel, dim3(std::max(1u, gx), (unsigned)P->groups.size()), dim3(256), 0, s, A,
-                       B, P->d_groups, P->d_exp);
-    TN_LAUNCH_CHECK();
-    // every operand element sliced once per launch (not once per tile that
-    // reads it)
-    const unsigned sx = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (P->max_slice_v + 255) / 256), 2048);
-    hipLaunchKernelGGL(exact_slice_kernel, dim3(sx, (unsigned)P->groups.size()), dim3(256), 0, s, A, B,
-                       P->d_groups, P->d_exp, P->d_sl);
-    TN_LAUNCH_CHECK();
+    // every operand element scaled and sliced once per launch (not once per
+    // tile that reads it)
+    if (P->nunits > 0) {
+      hipLaunchKernelGGL(P->plain_k ? exact_prep_kernel<true> : exact_prep_kernel<false>, dim3((unsigned)P->nunits),
+                         dim3(kPrepThreads), 0, s, A, B, P->d_groups, P->d_units, P->d_exp, P->d_sl);
+      TN_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(k, dim3((unsigned)t.nslots), dim3(256), 0, s, P->d_sl, C, P->d_groups, t.d_tiles, P->d_exp);
     TN_LAUNCH_CHECK();
     return TN_OK;
