@@ -185,7 +185,7 @@ void ITebd::update(UniTensor& ALa, UniTensor& BLb, Matrix& La, Matrix& Lb) {
   detail::check(tn_truncate_f64(rets[0].getElem(), k, chi_, lam.getElem(), scal_.get(), scal_.get() + 1,
                                 detail::stream()),
                 "ITebd truncate");
-  La = lam;
+  La = std::move(lam);
   Bond bdi(BD_IN, dim);
   std::vector<Bond> bond3 = ALa.bond();
   bond3[0] = bdi;

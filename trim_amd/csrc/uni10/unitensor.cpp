@@ -793,8 +793,9 @@ UniTensor& UniTensor::permute_full(const std::vector<int>& newLabels, int rowBon
         detail::check(tn_permute_dense_f64(buf.get(), out.buf.get(), nb, dims.data(), rsp.data(), detail::stream()),
                       "permute");
       } else {
-        detail::check(tn_memcpy_d2d(out.buf.get(), buf.get(), m_elemNum * sizeof(double), detail::stream()),
-                      "permute");
+        // the same element order, only the row / column split changes: the
+        // permuted tensor takes this tensor's buffer (no copy)
+        std::swap(out.buf, buf);
       }
     } else {
       tn_blockperm_plan p = block_perm_plan(*m_layout, labels, *out.m_layout, rsp, newLabels);
