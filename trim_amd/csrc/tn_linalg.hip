@@ -22,6 +22,7 @@
 #include <cmath>
 #include <mutex>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "tn_internal.h"
@@ -626,6 +627,17 @@ __device__ inline JacobiRot jacobi_rot(double np, double nq, double c) {
   return {cs, cs * tt, tt};
 }
 
+// f(integral_constant<int, 0>) ... f(integral_constant<int, N - 1>) in turn:
+// a loop whose index is a constant in its body.
+template <typename F, int... Is>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, Is...>) {
+  (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
 // When only S and VT are wanted (no U) and M >= N, the kernel first
 // factorises W = Q R by Householder reflections in LDS and runs the sweeps on
 // X = R^T, whose column Gram matrix is close to diagonal (Drmac and Veselic,
@@ -823,15 +835,17 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
       permute_sorted();
     }
     __syncthreads();
-    // The columns in registers: group G = 8 w + (lane & 7) of wave w holds
-    // columns G and G + 32, its lane 8 t + (lane & 7) row chunk t of each.
-    // Step j: the owner of column j has published its reflector v = x - alpha
-    // e_j (x = rows >= j of the column) and 1 / (v^T v / 2) in LDS; after the
-    // barrier every group reflects its columns (a finished column sees a
-    // zero coefficient), and the owner of column j + 1 publishes the next.
+    // The columns in registers: group G = 8 w + (lane >> 3) of wave w (eight
+    // adjacent lanes, so a column sum takes three DPP steps) holds columns G
+    // and G + 32, its lane 8 G + t row chunk t of each.  Step j: the owner of
+    // column j has published its reflector v = x - alpha e_j (x = rows >= j
+    // of the column) and 1 / (v^T v / 2) in LDS; after the barrier every
+    // group reflects its columns (a finished column sees a zero coefficient;
+    // from step 32 on the first slot holds only finished columns and is
+    // skipped), and the owner of column j + 1 publishes the next.
     {
-      const int wv = tid >> 6, jl = tid & 7, tc = (tid >> 3) & 7;
-      const int G = 8 * wv + jl;
+      const int tc = tid & 7;
+      const int G = tid >> 3;
       double col[2][EMX];
 #pragma unroll
       for (int sl = 0; sl < 2; ++sl) {
@@ -845,70 +859,80 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
         }
       }
       // the owner of column jj (slot sl) publishes its reflector and turns
-      // the column into column jj of R
-      auto publish = [&](auto slc, int jj) {
-        constexpr int sl = decltype(slc)::value;
+      // the column into column jj of R.  Row jj is element EJ = jj % EMX of
+      // lane jj / EMX's chunk: the steps are unrolled by EMX so that EJ is a
+      // constant (a runtime element index had the compiler pass the chunk
+      // through LDS and set the diagonal through a tree of branches, both on
+      // the step's critical path); the column's rows >= jj are cleared by
+      // subtracting x (exact) before the diagonal is set.
+      auto publish = [&](auto slc, auto ejc, int jj) {
+        constexpr int sl = decltype(slc)::value, EJ = decltype(ejc)::value;
+        const int tj = jj / EMX;
         double x[EMX];
-        double q0 = 0.0, q1 = 0.0, xjl = 0.0;
+        double q0 = 0.0, q1 = 0.0;
 #pragma unroll
-        for (int e = 0; e < EMX; ++e) {
-          const int r = tc * EMX + e;
-          x[e] = (r >= jj) ? col[sl][e] : 0.0;
-          xjl = (r == jj) ? x[e] : xjl;
-        }
+        for (int e = 0; e < EMX; ++e) x[e] = (tc * EMX + e >= jj) ? col[sl][e] : 0.0;
+        const double xjl = (tc == tj) ? x[EJ] : 0.0;
 #pragma unroll
         for (int e = 0; e < EMX; e += 2) {
           q0 = fma(x[e], x[e], q0);
           q1 = fma(x[e + 1], x[e + 1], q1);
         }
-        const double xx = group_sum_strided(q0 + q1);
-        const double xj = group_sum_strided(xjl);   // one nonzero term
+        const double xx = group8_sum(q0 + q1);
+        const double xj = group8_sum(xjl);   // one nonzero term
         const bool refl = xx > 0.0;
         const double nrm_x = refl ? xx * fast_rsqrt(refl ? xx : 1.0) : 0.0;
         const double alpha = -copysign(nrm_x, xj);
         const double den = alpha * (alpha - xj);   // v^T v / 2
         const double rden = refl ? fast_rcp(refl ? den : 1.0) : 0.0;
+        const bool mine = tc == tj;
         double* vb = hh_v + (jj & 1) * kRows + tc * EMX;
 #pragma unroll
         for (int e = 0; e < EMX; ++e) {
-          const int r = tc * EMX + e;
-          vb[e] = (r == jj) ? xj - alpha : x[e];
-          col[sl][e] = (r == jj) ? (refl ? alpha : xj) : (r > jj) ? 0.0 : col[sl][e];
+          vb[e] = (e == EJ && mine) ? xj - alpha : x[e];
+          col[sl][e] = (e == EJ && mine) ? (refl ? alpha : xj) : col[sl][e] - x[e];   // rows >= jj to zero, exactly
         }
         if (tc == 0) hh_xx[jj & 1] = rden;
       };
-      if (G == 0) publish(std::integral_constant<int, 0>{}, 0);
-      for (int j = 0; j < n; ++j) {
-        __syncthreads();
-        double v[EMX];
-        const double* vb = hh_v + (j & 1) * kRows + tc * EMX;
-#pragma unroll
-        for (int e = 0; e < EMX; e += 2) {
-          const double2 u = *reinterpret_cast<const double2*>(vb + e);
-          v[e] = u.x;
-          v[e + 1] = u.y;
-        }
-        const double rden = hh_xx[j & 1];
-#pragma unroll
-        for (int sl = 0; sl < 2; ++sl) {
-          const int k = G + 32 * sl;
-          double d0 = 0.0, d1 = 0.0;
+      if (G == 0) publish(std::integral_constant<int, 0>{}, std::integral_constant<int, 0>{}, 0);
+      for (int j0 = 0; j0 < n; j0 += EMX) {
+        static_for<EMX>([&](auto uc) {
+          constexpr int u = decltype(uc)::value;
+          const int j = j0 + u;
+          if (j >= n) return;
+          __syncthreads();
+          double v[EMX];
+          const double* vb = hh_v + (j & 1) * kRows + tc * EMX;
 #pragma unroll
           for (int e = 0; e < EMX; e += 2) {
-            d0 = fma(v[e], col[sl][e], d0);
-            d1 = fma(v[e + 1], col[sl][e + 1], d1);
+            const double2 w2 = *reinterpret_cast<const double2*>(vb + e);
+            v[e] = w2.x;
+            v[e + 1] = w2.y;
           }
-          const double coef = (k > j && k < n) ? group_sum_strided(d0 + d1) * rden : 0.0;
+          const double rden = hh_xx[j & 1];
 #pragma unroll
-          for (int e = 0; e < EMX; ++e) col[sl][e] = fma(-coef, v[e], col[sl][e]);
-        }
-        const int jn = j + 1;
-        if (jn < n && (jn & 31) == G) {
-          if (jn < 32)
-            publish(std::integral_constant<int, 0>{}, jn);
-          else
-            publish(std::integral_constant<int, 1>{}, jn);
-        }
+          for (int sl = 0; sl < 2; ++sl) {
+            if (sl == 0 && j >= 31) continue;   // columns 0..31 are all finished
+            const int k = G + 32 * sl;
+            double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+            for (int e = 0; e < EMX; e += 2) {
+              d0 = fma(v[e], col[sl][e], d0);
+              d1 = fma(v[e + 1], col[sl][e + 1], d1);
+            }
+            const double coef = (k > j && k < n) ? group8_sum(d0 + d1) * rden : 0.0;
+#pragma unroll
+            for (int e = 0; e < EMX; ++e) col[sl][e] = fma(-coef, v[e], col[sl][e]);
+          }
+          const int jn = j + 1;
+          constexpr int EJN = (u + 1) % EMX;
+          if (jn < n && (jn & 31) == G) {
+            if (jn < 32)
+              publish(std::integral_constant<int, 0>{}, std::integral_constant<int, EJN>{}, jn);
+            else
+              publish(std::integral_constant<int, 1>{}, std::integral_constant<int, EJN>{}, jn);
+          }
+        });
       }
       __syncthreads();
 #pragma unroll
