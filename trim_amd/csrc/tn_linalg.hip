@@ -1339,6 +1339,10 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
   double* VTb = VT + pr.vt_off;
   for (int j = tid; j < n; j += kThreads) Sb[rank_of[j]] = sqrt(nrm[j]);
   if (qr) {
+    // 1 / sigma_k once per column (hh_v is free here), read by the scaling
+    // of the first product below instead of a division per element
+    for (int j = tid; j < n; j += kThreads) hh_v[j] = 1.0 / sqrt(nrm[j]);
+    __syncthreads();
     // P = U_X^T (V0 Pi)^T, row k = the sum over QR positions i of
     // X[i][k] / sigma_k times V0[r][qr_col[i]]; then one Newton-Schulz step
     // VT = (3 P - (P P^T) P) / 2, which makes the rows orthonormal to
@@ -1356,6 +1360,8 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
     auto mm = [&](auto a_at, auto b_at, d4s(&acc)[NT]) {
 #pragma unroll
       for (int tt = 0; tt < NT; ++tt) acc[tt] = d4s{0.0, 0.0, 0.0, 0.0};
+      // four k-steps' operand reads issued together
+#pragma unroll 4
       for (int kk = 0; kk < kmax; kk += 4) {
         const int k = kk + lk;
 #pragma unroll
@@ -1366,13 +1372,13 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
     };
     d4s pk[NT], hk[NT];
     mm([&](int k, int i) { return W[widx(k, i)]; },
-       [&](int i, int r) { return (i < n && r < n) ? V[vidx(qr_col[i], r)] : 0.0; }, pk);
+       [&](int i, int r) { return (i < n && r < n) ? V[vidx(warm ? i : qr_col[i], r)] : 0.0; }, pk);   // warm: qr_col = identity
 #pragma unroll
     for (int tt = 0; tt < NT; ++tt)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int k = tile_r(tt) + lk + 4 * v;
-        pk[tt][v] = (k < n) ? pk[tt][v] / sqrt(nrm[k]) : 0.0;
+        pk[tt][v] = (k < n) ? pk[tt][v] * hh_v[k] : 0.0;
       }
     __syncthreads();
     // P row k -> W column k
