@@ -6,6 +6,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <type_traits>
+#include <utility>
 
 #include "tn_hip.h"
 
@@ -37,6 +39,19 @@ int copy_d2d(void* dst, const void* src, size_t bytes, hipStream_t s);
 // Copies at or above this size (bytes read) use non-temporal loads and
 // stores: neither side is worth keeping in L2 / the Infinity Cache.
 constexpr size_t kStreamingBytes = size_t(64) << 20;
+
+// f(integral_constant<int, 0>) ... f(integral_constant<int, N - 1>) in turn:
+// a loop whose index is a constant in its body, so that it can index
+// register arrays (a runtime index into one becomes select chains, LDS round
+// trips or branch trees).
+template <typename F, int... Is>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, Is...>) {
+  (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 }  // namespace tn
 

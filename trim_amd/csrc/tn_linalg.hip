@@ -627,17 +627,6 @@ __device__ inline JacobiRot jacobi_rot(double np, double nq, double c) {
   return {cs, cs * tt, tt};
 }
 
-// f(integral_constant<int, 0>) ... f(integral_constant<int, N - 1>) in turn:
-// a loop whose index is a constant in its body.
-template <typename F, int... Is>
-__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, Is...>) {
-  (f(std::integral_constant<int, Is>{}), ...);
-}
-template <int N, typename F>
-__device__ __forceinline__ void static_for(F&& f) {
-  static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
-
 // When only S and VT are wanted (no U) and M >= N, the kernel first
 // factorises W = Q R by Householder reflections in LDS and runs the sweeps on
 // X = R^T, whose column Gram matrix is close to diagonal (Drmac and Veselic,
@@ -896,7 +885,7 @@ __global__ void __launch_bounds__(kThreads) svd_jacobi_small(const SvdProb* __re
       };
       if (G == 0) publish(std::integral_constant<int, 0>{}, std::integral_constant<int, 0>{}, 0);
       for (int j0 = 0; j0 < n; j0 += EMX) {
-        static_for<EMX>([&](auto uc) {
+        tn::static_for<EMX>([&](auto uc) {
           constexpr int u = decltype(uc)::value;
           const int j = j0 + u;
           if (j >= n) return;

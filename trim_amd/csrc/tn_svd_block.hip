@@ -621,17 +621,21 @@ __global__ void __launch_bounds__(kThreads) bj_qr_panel(const Prob* __restrict__
 #pragma unroll
     for (int c = 0; c < PW; ++c) a[u][c] = (r < m && c < nb) ? W[(int64_t)(kb + c) * mp + r] : 0.0;
   }
-  for (int j = 0; j < nb; ++j) {
+  // the steps are unrolled (j a constant in each), so the pivot column is a
+  // register, not a select over all PW columns, and finished columns have no
+  // code (a runtime j cost 239-258 us per 32-column panel)
+  tn::static_for<PW>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    if (j >= nb) return;
     const int rj = kb + j;
     double x[ROWS];
 #pragma unroll
     for (int u = 0; u < ROWS; ++u) {
       const int r = kb + tid + kThreads * u;
-      double v = 0.0;
-#pragma unroll
-      for (int c = 0; c < PW; ++c) v = (c == j) ? a[u][c] : v;
-      x[u] = (r >= rj) ? v : 0.0;
+      x[u] = (r >= rj) ? a[u][j] : 0.0;
     }
+    // every column's x . a_c: the later ones for their update, the earlier
+    // (the reflectors y_c) for the new column of T
     if (tid == j) {   // row rj is row 0 of thread j
 #pragma unroll
       for (int c = 0; c < PW; ++c) row[c] = a[0][c];
@@ -664,8 +668,8 @@ __global__ void __launch_bounds__(kThreads) bj_qr_panel(const Prob* __restrict__
     const double tau = refl ? 1.0 / (alpha * (alpha - xj)) : 0.0;
     const double vj = xj - alpha;
 #pragma unroll
-    for (int c = 0; c < PW; ++c) {
-      if (c <= j || c >= nb) continue;
+    for (int c = j + 1; c < PW; ++c) {
+      if (c >= nb) continue;
       const double coef = tau * (res[c] - alpha * row[c]);
 #pragma unroll
       for (int u = 0; u < ROWS; ++u) {
@@ -677,9 +681,7 @@ __global__ void __launch_bounds__(kThreads) bj_qr_panel(const Prob* __restrict__
 #pragma unroll
     for (int u = 0; u < ROWS; ++u) {
       const int r = kb + tid + kThreads * u;
-#pragma unroll
-      for (int c = 0; c < PW; ++c)
-        if (c == j && r == rj) a[u][c] = vj;
+      if (r == rj) a[u][j] = vj;
     }
     if (tid < j) {
       double sum = 0.0;
@@ -691,7 +693,7 @@ __global__ void __launch_bounds__(kThreads) bj_qr_panel(const Prob* __restrict__
       rdiag[j] = alpha;
     }
     __syncthreads();
-  }
+  });
   double* Y = scr + pr.y_off;
 #pragma unroll
   for (int u = 0; u < ROWS; ++u) {
