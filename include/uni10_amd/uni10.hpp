@@ -189,7 +189,11 @@ class Matrix {
   DevBuf buf;
   size_t nstore() const { return elemNum() * (ctype ? 2 : 1); }   // doubles held
   void real_only(const char* what) const;
+  // storage left unset, for results a kernel writes in full
+  struct uninit_t {};
+  Matrix(uninit_t, size_t Rnum, size_t Cnum, bool diag);
   friend class UniTensor;
+  friend class ITebd;
 };
 typedef Matrix Block;  // read-only views are returned as copies
 Matrix takeExp(double a, const Matrix& mat);

@@ -79,6 +79,10 @@ Matrix::Matrix(size_t R, size_t C, const double* elem, bool d, bool src_ongpu)
   setElem(elem, src_ongpu);
 }
 
+Matrix::Matrix(uninit_t, size_t R, size_t C, bool d) : Rnum(R), Cnum(C), diag(d) {
+  buf.reset(elemNum(), false);
+}
+
 Matrix::Matrix(cflag, size_t R, size_t C, bool d) : Rnum(R), Cnum(C), diag(d), ctype(true) {
   buf.reset(nstore(), true);
 }

@@ -122,9 +122,10 @@ ITebd::ITebd(int chi, int d, double delta, double tau, uint64_t seed, const std:
 std::vector<Matrix> ITebd::svd_warm(const Matrix& T) {
   Matrix& vt_prev = vt_prev_[parity_];
   const size_t R = T.row(), C = T.col(), k = std::min(R, C);
+  // S and VT are written in full by the SVD: no zero fill
   std::vector<Matrix> outs;
-  outs.emplace_back(k, k, true);
-  outs.emplace_back(k, C, false);
+  outs.push_back(Matrix(Matrix::uninit_t{}, k, k, true));
+  outs.push_back(Matrix(Matrix::uninit_t{}, k, C, false));
   if (k == 0) return outs;
   if (R >= C && vt_prev.row() == C && vt_prev.col() == C) {
     tn_svd_plan& P = plan_[parity_];
@@ -181,7 +182,7 @@ void ITebd::update(UniTensor& ALa, UniTensor& BLb, Matrix& La, Matrix& Lb) {
   const int dim = chi_ < k ? chi_ : k;
   // lambda = S[:dim] / ||S[:dim]|| (iTEBD.cpp:157-161), the norm and the
   // discarded weight stay on the device: the update never waits for the host
-  Matrix lam(dim, dim, true);
+  Matrix lam(Matrix::uninit_t{}, dim, dim, true);  // every entry written below
   detail::check(tn_truncate_f64(rets[0].getElem(), k, chi_, lam.getElem(), scal_.get(), scal_.get() + 1,
                                 detail::stream()),
                 "ITebd truncate");
