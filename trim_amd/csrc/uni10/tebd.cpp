@@ -148,7 +148,7 @@ std::vector<Matrix> ITebd::svd_warm(const Matrix& T) {
                              outs[1].getElem(), detail::stream()),
                   "ITebd svd");
   }
-  vt_prev = outs[1];
+  // the caller keeps outs[1] as this bond's next start (update())
   return outs;
 }
 
@@ -191,9 +191,10 @@ void ITebd::update(UniTensor& ALa, UniTensor& BLb, Matrix& La, Matrix& Lb) {
   std::vector<Bond> bond3 = ALa.bond();
   bond3[0] = bdi;
   BLb.assign(bond3);
-  Matrix blk = BLb.getBlock(q0);
-  blk.setElem(rets[1].getElem(), true);
-  BLb.putBlock(q0, blk);
+  // one block (no symmetry), its elements the first dim rows of VT
+  BLb.setElem(rets[1].getElem(), true);
+  // VT starts this bond's next SVD (svd_warm flipped parity_)
+  vt_prev_[parity_ ^ 1] = std::move(rets[1]);
   updateA_.putTensor("BLb", &BLb);
   updateA_.putTensor("C", &C);
   ALa = updateA_.launch();
